@@ -1,0 +1,138 @@
+/* rampack_amd.h -- C ABI of the MI355X-native rampack Monte Carlo sweep.
+ *
+ * The boundary sits at sweep granularity: the reference calls its physics once per pair through virtual
+ * functions (Interaction::overlapBetween, src/core/Interaction.h:81-90; Packing::countParticleOverlaps,
+ * src/core/Packing.cpp:373-429), which cannot cross a device boundary per call.  The plug-ins in front of the
+ * path (ShapeTraits / Interaction, MoveSampler, TriclinicBoxScaler) are exported as plain descriptors, and one
+ * call runs whole cycles on the GPU.
+ *
+ * Conventions (src/utils/Exceptions.h:19-80, src/core/Packing.h:305-339):
+ *   0 on success; RP_ERR_PRECONDITION where the reference throws PreconditionException; RP_ERR_VALIDATION where
+ *   it throws ValidationException (e.g. overlaps at the start of integration, src/core/Simulation.cpp:131-132).
+ *   Overlaps are not errors.  rp_last_error() describes the last failure of the calling thread.
+ *   Host arrays are owned by the caller and copied in and out.  One context is used by one host thread at a time.
+ *   Positions are n x 3 doubles, orientations n x 9 doubles (row-major 3x3, Shape.h:20-23), boxes 9 doubles
+ *   (row-major matrix whose columns are the box sides, TriclinicBox.h:49-58).
+ */
+#ifndef RAMPACK_AMD_H
+#define RAMPACK_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RP_OK 0
+#define RP_ERR_PRECONDITION (-1)
+#define RP_ERR_VALIDATION (-2)
+#define RP_ERR_DEVICE (-3)
+#define RP_ERR_UNSUPPORTED (-4)
+
+/* Shape kinds: SphereTraits (src/core/shapes/SphereTraits.h:17-70), SpherocylinderTraits
+ * (src/core/shapes/SpherocylinderTraits.h:16-86). */
+#define RP_SHAPE_SPHERE 0
+#define RP_SHAPE_SPHEROCYLINDER 1
+
+/* Precision of the overlap tests.  RP_PREC_FP64 evaluates every test in the reference's FP64 operation order. */
+#define RP_PREC_FP64 0
+
+/* Box moves: TriclinicDeltaScaler with scale_together (src/core/volume_scalers/TriclinicDeltaScaler.cpp:7-25),
+ * log isotropic through TriclinicAdapter + AnisotropicVolumeScaler + LogScalingFactorSampler
+ * (src/core/volume_scalers/TriclinicAdapter.h:27-33, AnisotropicVolumeScaler.cpp:18-33,
+ * LogScalingFactorSampler.cpp:8-19). */
+#define RP_BOX_NONE 0
+#define RP_BOX_DELTA_TRICLINIC 1
+#define RP_BOX_LOG_ISOTROPIC 2
+#define RP_BOX_DELTA_V 3        /* TriclinicAdapter + DeltaVolumeScaler (src/core/volume_scalers/DeltaVolumeScaler.cpp:10-19) */
+
+typedef struct rp_ctx rp_ctx;
+
+/* ShapeTraits / Interaction descriptor (replaces the virtual Interaction, src/core/Interaction.h:18-150). */
+typedef struct rp_shape_desc {
+    int kind;       /* RP_SHAPE_* */
+    double radius;  /* sphere / spherocylinder radius */
+    double length;  /* spherocylinder distance between cap centres */
+} rp_shape_desc;
+
+/* RototranslationSampler descriptor (src/core/move_samplers/RototranslationSampler.h:17-60). */
+typedef struct rp_move_desc {
+    double trans_step;     /* translation step (> 0) */
+    double rot_step;       /* rotation step; <= 0 selects the automatic rule of RototranslationSampler.cpp:95-114 */
+    double max_trans_step; /* 0: implicit, half the smallest box height (RototranslationSampler.cpp:78-85) */
+} rp_move_desc;
+
+/* TriclinicBoxScaler descriptor (src/core/TriclinicBoxScaler.h:15-55). */
+typedef struct rp_box_desc {
+    int kind;      /* RP_BOX_* */
+    double step;   /* initial step size (> 0) */
+} rp_box_desc;
+
+/* Counters and step sizes after a call of rp_run_cycles (cumulative over the context's lifetime). */
+typedef struct rp_counters {
+    uint64_t cycles;
+    uint64_t moves_attempted;
+    uint64_t moves_accepted;
+    uint64_t moves_left_cell;    /* rejected because the particle left its active cell */
+    uint64_t box_attempted;
+    uint64_t box_accepted;
+    uint64_t box_scans;          /* box moves that ran the device-wide overlap scan */
+    uint64_t box_too_small;      /* box moves rejected because a box height fell below two interaction ranges */
+    uint64_t skipped_blocks;     /* active cells whose stencil did not fit the LDS staging area */
+    uint64_t pair_tests;         /* candidate slots examined by the sweep */
+    double trans_step, rot_step, box_step;
+} rp_counters;
+
+/* Event-timed sweep kernel statistics (filled when timing is on, see rp_set_timing). */
+typedef struct rp_timing {
+    uint64_t sweep_launches;
+    double sweep_ms;             /* summed duration of the sweep kernel launches */
+    uint64_t rebuild_launches;
+    double rebuild_ms;           /* summed duration of the cell-list rebuilds */
+    uint64_t scan_launches;
+    double scan_ms;
+} rp_timing;
+
+/* Packing constructor + Simulation constructor (src/core/Packing.cpp:46-82, src/core/Simulation.cpp:54-76):
+ * uploads n shapes into a triclinic periodic box.  Every position must lie inside the box (Packing.cpp:1226-1232).
+ * seed keys the counter-based random streams. */
+int rp_create(rp_ctx **out, const rp_shape_desc *shape, const double box[9], size_t n, const double *pos,
+              const double *rot, uint64_t seed, int precision);
+
+/* Simulation::Environment (src/core/Simulation.h:113-165): temperature, pressure, the move sampler and the box
+ * scaler (NULL or RP_BOX_NONE: NVT). */
+int rp_set_environment(rp_ctx *ctx, double temperature, double pressure, const rp_move_desc *move,
+                       const rp_box_desc *box);
+
+/* Simulation::performCycle x n_cycles (src/core/Simulation.cpp:325-363): N trial rototranslations as checkerboard
+ * colour passes, then one box move; adapt_steps != 0 applies the thermalisation step adaptation of
+ * Simulation.cpp:533-607.  Fails with RP_ERR_VALIDATION if the packing has overlaps (Simulation.cpp:131-132). */
+int rp_run_cycles(rp_ctx *ctx, size_t n_cycles, int adapt_steps, rp_counters *out);
+
+/* Packing::countTotalOverlaps (src/core/Packing.cpp:519-559) on the reference's own neighbour grid
+ * (Packing.cpp:858-899); early_exit returns 0 or 1. */
+int rp_count_total_overlaps(rp_ctx *ctx, int early_exit, uint64_t *out);
+
+/* Current state, in the caller's particle order (Packing::operator[], Packing::getBox). */
+int rp_download(rp_ctx *ctx, double box[9], double *pos, double *rot);
+
+/* Turns event timing of the sweep kernels on or off and reads the accumulated statistics. */
+int rp_set_timing(rp_ctx *ctx, int enable);
+int rp_get_timing(rp_ctx *ctx, rp_timing *out);
+
+void rp_destroy(rp_ctx *ctx);
+
+/* Batched pair decisions on the device: SpherocylinderTraits::overlapBetween (SpherocylinderTraits.cpp:28-46) or
+ * SphereTraits::HardInteraction::overlapBetween (SphereTraits.cpp:56-65), called with a fixed cell translation as
+ * Packing does (Packing.cpp:21-44, 381-398).  Each record is 27 doubles: pos1[3] O1[9] pos2[3] O2[9] T[3].
+ * decisions[i] = 1 for an overlap. */
+int rp_pair_decisions(const rp_shape_desc *shape, size_t n, const double *records, uint8_t *decisions);
+
+const char *rp_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RAMPACK_AMD_H */

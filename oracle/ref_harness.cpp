@@ -1,0 +1,328 @@
+// ref_harness.cpp -- C entry points over the REFERENCE's own compiled hot path.
+//
+// TEST INFRASTRUCTURE ONLY.  oracle/Makefile compiles this file together with
+// the reference's source files (read in place under the reference tree) into
+// oracle/_ref/libref.so.  Nothing of the reference is copied: this file only
+// calls its public classes (Packing, SpherocylinderTraits, SphereTraits,
+// PolysphereTraits, RototranslationSampler, TriclinicDeltaScaler,
+// TriclinicAdapter, DomainDecomposition, NematicOrder) the way the reference's
+// own Simulation and tests do.
+//
+// The reference's Simulation.cpp cannot be compiled here (it includes the
+// absent ZipIterator submodule), so ref_mc_run restates its cycle loop
+// (Simulation.cpp:325-531, 533-607) on top of the reference's Packing /
+// samplers / scalers / DomainDecomposition.  Draw order from the per-domain
+// std::mt19937 streams follows Simulation.cpp:401-404, 456-458, 483, 499-525.
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <chrono>
+#include <memory>
+#include <numeric>
+#include <random>
+#include <vector>
+#include <omp.h>
+
+#include "core/Packing.h"
+#include "core/PeriodicBoundaryConditions.h"
+#include "core/DomainDecomposition.h"
+#include "core/shapes/SphereTraits.h"
+#include "core/shapes/SpherocylinderTraits.h"
+#include "core/shapes/PolysphereTraits.h"
+#include "core/interactions/RepulsiveLennardJonesInteraction.h"
+#include "core/move_samplers/RototranslationSampler.h"
+#include "core/volume_scalers/TriclinicDeltaScaler.h"
+#include "core/volume_scalers/TriclinicAdapter.h"
+#include "core/volume_scalers/AnisotropicVolumeScaler.h"
+#include "core/volume_scalers/LogScalingFactorSampler.h"
+#include "core/observables/NematicOrder.h"
+
+namespace {
+
+// The BoundaryConditions the reference's Packing uses for neighbour-grid pairs (a fixed cell translation); the
+// reference keeps its own copy in an anonymous namespace, so the harness supplies the same behaviour through the
+// public BoundaryConditions interface.
+class FixedTranslation : public BoundaryConditions {
+    Vector<3> t;
+public:
+    explicit FixedTranslation(const Vector<3> &t) : t{t} {}
+    void setBox(const TriclinicBox &) override {}
+    [[nodiscard]] Vector<3> getCorrection(const Vector<3> &) const override { return {}; }
+    [[nodiscard]] Vector<3> getTranslation(const Vector<3> &, const Vector<3> &) const override { return t; }
+    [[nodiscard]] double getDistance2(const Vector<3> &p1, const Vector<3> &p2) const override {
+        return (p2 + this->t - p1).norm2();
+    }
+};
+
+std::unique_ptr<ShapeTraits> makeTraits(int kind, double radius, double length, int nCentres, const double *centres,
+                                        const double *radii, int soft, double eps, double sigma) {
+    if (kind == 0) {
+        if (soft == 1)
+            return std::make_unique<SphereTraits>(radius, std::make_shared<RepulsiveLennardJonesInteraction>(eps, sigma));
+        return std::make_unique<SphereTraits>(radius);
+    }
+    if (kind == 1)
+        return std::make_unique<SpherocylinderTraits>(length, radius);
+    std::vector<PolysphereTraits::SphereData> data;
+    for (int i = 0; i < nCentres; i++)
+        data.emplace_back(Vector<3>{centres[3 * i], centres[3 * i + 1], centres[3 * i + 2]}, radii[i]);
+    return std::make_unique<PolysphereTraits>(PolysphereTraits::PolysphereGeometry(data));
+}
+
+Matrix<3, 3> mat(const double *m) {
+    return Matrix<3, 3>{m[0], m[1], m[2], m[3], m[4], m[5], m[6], m[7], m[8]};
+}
+
+std::vector<Shape> makeShapes(std::size_t n, const double *pos, const double *rot) {
+    std::vector<Shape> shapes;
+    shapes.reserve(n);
+    for (std::size_t i = 0; i < n; i++)
+        shapes.emplace_back(Vector<3>{pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]}, mat(rot + 9 * i));
+    return shapes;
+}
+
+// Simulation::Counter (Simulation.cpp:610-640)
+struct Counter {
+    std::size_t moves = 0, accepted = 0, sinceEval = 0, acceptedSinceEval = 0;
+    void increment(bool a) { moves++; sinceEval++; if (a) { accepted++; acceptedSinceEval++; } }
+    double rate() const { return sinceEval ? (double)acceptedSinceEval / (double)sinceEval : 0; }
+    void resetCurrent() { sinceEval = 0; acceptedSinceEval = 0; }
+    Counter &operator+=(const Counter &o) {
+        moves += o.moves; accepted += o.accepted; sinceEval += o.sinceEval; acceptedSinceEval += o.acceptedSinceEval;
+        return *this;
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+int ref_pair_decisions(int kind, double radius, double length, std::size_t n, const double *rec, std::uint8_t *out) {
+    auto traits = makeTraits(kind, radius, length, 0, nullptr, nullptr, 0, 1, 1);
+    const Interaction &inter = traits->getInteraction();
+    for (std::size_t i = 0; i < n; i++) {
+        const double *q = rec + 27 * i;
+        Vector<3> p1{q[0], q[1], q[2]}, p2{q[12], q[13], q[14]};
+        FixedTranslation bc(Vector<3>{q[24], q[25], q[26]});
+        out[i] = inter.overlapBetween(p1, mat(q + 3), 0, p2, mat(q + 15), 0, bc) ? 1 : 0;
+    }
+    return 0;
+}
+
+int ref_snapshot_stats(int kind, double radius, double length, int nCentres, const double *centres,
+                       const double *radii, int soft, double eps, double sigma, const double *box, std::size_t n,
+                       const double *pos, const double *rot, int earlyExit, std::uint64_t *overlaps,
+                       double *energy) {
+    try {
+        auto traits = makeTraits(kind, radius, length, nCentres, centres, radii, soft, eps, sigma);
+        const Interaction &inter = traits->getInteraction();
+        Packing packing(TriclinicBox(mat(box)), makeShapes(n, pos, rot), std::make_unique<PeriodicBoundaryConditions>(),
+                        inter, 1, 1);
+        *overlaps = inter.hasHardPart() ? packing.countTotalOverlaps(inter, earlyExit != 0) : 0;
+        if (energy) *energy = packing.getTotalEnergy(inter);
+        return 0;
+    } catch (const std::exception &) {
+        return -1;
+    }
+}
+
+double ref_nematic_p2(std::size_t n, const double *rot) {
+    SpherocylinderTraits traits(1, 1);
+    std::vector<double> pos(3 * n, 0.5);
+    Packing packing(TriclinicBox(std::array<double, 3>{1, 1, 1}), makeShapes(n, pos.data(), rot),
+                    std::make_unique<PeriodicBoundaryConditions>(), traits.getInteraction(), 1, 1);
+    NematicOrder q;
+    q.calculate(packing, 1, 1, traits);
+    return q.getIntervalValues().front();
+}
+
+// One NpT (or NVT) run of the reference's hot path with the restated cycle loop.  box_kind: 0 none (NVT), 1
+// delta_triclinic(scale_together=True), 2 log(spec="isotropic").  rot_step <= 0 selects the automatic rotation step
+// (RototranslationSampler.cpp:95-114).  Samples of (packing fraction, P2) are taken every avg_every cycles of the
+// averaging phase.  The return value is the number of samples, or a negative number on error.
+long ref_mc_run(int kind, double radius, double length, const double *box, std::size_t n, const double *pos,
+                const double *rot, double temperature, double pressure, double transStep, double rotStep,
+                int boxKind, double boxStep, unsigned long seed, int ddx, int ddy, int ddz, int threads,
+                std::size_t thermCycles, std::size_t avgCycles, std::size_t avgEvery, double *samples,
+                double *outBox, double *outPos, double *outRot, double *seconds, double *finalSteps) {
+    try {
+        std::array<std::size_t, 3> dd{(std::size_t)ddx, (std::size_t)ddy, (std::size_t)ddz};
+        std::size_t numDomains = dd[0] * dd[1] * dd[2];
+        auto traits = makeTraits(kind, radius, length, 0, nullptr, nullptr, 0, 1, 1);
+        const Interaction &inter = traits->getInteraction();
+        auto packing = std::make_unique<Packing>(TriclinicBox(mat(box)), makeShapes(n, pos, rot),
+                                                 std::make_unique<PeriodicBoundaryConditions>(), inter,
+                                                 (std::size_t)threads, (std::size_t)threads);
+        RototranslationSampler sampler(transStep, rotStep > 0 ? std::optional<double>(rotStep) : std::nullopt, 0);
+        sampler.setupForShapeTraits(*traits);
+        std::unique_ptr<TriclinicBoxScaler> scaler;
+        if (boxKind == 1) {
+            scaler = std::make_unique<TriclinicDeltaScaler>(boxStep, true);
+        } else if (boxKind == 2) {
+            auto dir = AnisotropicVolumeScaler::X & AnisotropicVolumeScaler::Y & AnisotropicVolumeScaler::Z;
+            scaler = std::make_unique<TriclinicAdapter>(
+                std::make_unique<AnisotropicVolumeScaler>(std::make_unique<LogScalingFactorSampler>(), dir, false),
+                boxStep);
+        }
+
+        std::vector<std::mt19937> mts;
+        for (std::size_t i = 0; i < numDomains; i++) mts.emplace_back(seed + i);
+        std::uniform_real_distribution<double> unit;
+        std::vector<std::size_t> all(n);
+        std::iota(all.begin(), all.end(), 0);
+        Counter moveCounter, scalingCounter;
+        double volume0 = traits->getGeometry().getVolume();
+
+        // Simulation::tryMove (Simulation.cpp:447-491), one sampler
+        auto tryMove = [&](const std::vector<std::size_t> &idxs, Counter &counter, std::size_t numMoves,
+                           std::optional<ActiveDomain> bounds) {
+            auto &mt = mts[omp_get_thread_num()];
+            std::uniform_int_distribution<std::size_t> moveDistribution(0, numMoves - 1);
+            (void)moveDistribution(mt);
+            auto move = sampler.sampleMove(*packing, idxs, mt);
+            double dE = packing->tryMove(move.particleIdx, move.translation, move.rotation, inter, bounds);
+            if (unit(mt) <= std::exp(-dE / temperature)) {
+                packing->acceptMove();
+                counter.increment(true);
+            } else {
+                counter.increment(false);
+            }
+        };
+
+        // Simulation::performMoves (Simulation.cpp:365-445)
+        auto performMoves = [&]() {
+            if (numDomains == 1) {
+                for (std::size_t i = 0; i < n; i++) tryMove(all, moveCounter, n, std::nullopt);
+                return;
+            }
+            while (true) {
+                try {
+                    const auto &pbox = packing->getBox();
+                    auto &mt = mts[0];
+                    Vector<3> origin{unit(mt), unit(mt), unit(mt)};
+                    origin = pbox.relativeToAbsolute(origin);
+                    auto ngDiv = packing->getNeighbourGridCellDivisions();
+                    DomainDecomposition dec(*packing, inter, dd, ngDiv, origin);
+                    std::vector<Counter> temp((std::size_t)omp_get_max_threads());
+                    std::size_t avgN = n / numDomains;
+                    #pragma omp parallel for collapse(3) num_threads(packing->getMoveThreads())
+                    for (std::size_t i = 0; i < dd[0]; i++)
+                        for (std::size_t j = 0; j < dd[1]; j++)
+                            for (std::size_t k = 0; k < dd[2]; k++) {
+                                std::array<std::size_t, 3> c{i, j, k};
+                                const auto &idxs = dec.getParticlesInRegion(c);
+                                auto active = dec.getActiveDomainBounds(c);
+                                if (idxs.empty()) continue;
+                                for (std::size_t x = 0; x < avgN; x++)
+                                    tryMove(idxs, temp[omp_get_thread_num()], avgN, active);
+                            }
+                    for (auto &c : temp) moveCounter += c;
+                    break;
+                } catch (const TooNarrowDomainException &ex) {
+                    dd[ex.getCoord()]--;
+                    numDomains = dd[0] * dd[1] * dd[2];
+                    if (numDomains == 1) {
+                        for (std::size_t i = 0; i < n; i++) tryMove(all, moveCounter, n, std::nullopt);
+                        break;
+                    }
+                }
+            }
+        };
+
+        // Simulation::tryScaling (Simulation.cpp:493-531), hard interaction
+        auto tryScaling = [&]() -> bool {
+            auto &mt = mts.front();
+            TriclinicBox oldBox = packing->getBox();
+            TriclinicBox newBox = scaler->updateBox(oldBox, mt);
+            double oldV = std::abs(oldBox.getVolume()), newV = std::abs(newBox.getVolume());
+            double deltaV = newV - oldV, factor = newV / oldV;
+            double N = (double)packing->size();
+            if (inter.hasSoftPart()) {
+                double dE = packing->tryScaling(newBox, inter);
+                double exponent = N * log(factor) - dE / temperature - pressure * deltaV / temperature;
+                if (unit(mt) <= std::exp(exponent)) return true;
+                packing->revertScaling();
+                return false;
+            }
+            double exponent = N * log(factor) - pressure * deltaV / temperature;
+            if (unit(mt) > std::exp(exponent)) return false;
+            if (packing->tryScaling(newBox, inter) > 0) {
+                packing->revertScaling();
+                return false;
+            }
+            return true;
+        };
+
+        // Simulation::evaluateCounters (Simulation.cpp:533-607)
+        auto evaluate = [&]() {
+            if (moveCounter.sinceEval >= 100 * n) {
+                double r = moveCounter.rate();
+                moveCounter.resetCurrent();
+                if (r > 0.2) sampler.increaseStepSize();
+                else if (r < 0.1) sampler.decreaseStepSize();
+            }
+            if (scaler && scalingCounter.sinceEval >= 100) {
+                double r = scalingCounter.rate();
+                scalingCounter.resetCurrent();
+                if (r > 0.2) scaler->increaseStepSize();
+                else if (r < 0.1) scaler->decreaseStepSize();
+            }
+        };
+
+        packing->toggleOverlapCounting(false, inter);
+        if (packing->countTotalOverlaps(inter) != 0) return -2;
+
+        long nSamples = 0;
+        std::size_t totalCycles = 0;
+        auto start = std::chrono::high_resolution_clock::now();
+        for (std::size_t cyc = 0; cyc < thermCycles + avgCycles; cyc++) {
+            bool therm = cyc < thermCycles;
+            performMoves();
+            if (scaler) scalingCounter.increment(tryScaling());
+            if (therm) evaluate();
+            totalCycles++;
+            if (totalCycles % 10000 == 0) packing->renormalizeOrientations(inter, false);
+            if (!therm && avgEvery && (totalCycles - thermCycles) % avgEvery == 0 && samples) {
+                samples[2 * nSamples] = packing->getPackingFraction(volume0);
+                std::vector<double> r(9 * n);
+                for (std::size_t i = 0; i < n; i++) {
+                    const auto &O = (*packing)[i].getOrientation();
+                    for (int a = 0; a < 3; a++)
+                        for (int b = 0; b < 3; b++) r[9 * i + 3 * a + b] = O(a, b);
+                }
+                NematicOrder q;
+                q.calculate(*packing, temperature, pressure, *traits);
+                samples[2 * nSamples + 1] = q.getIntervalValues().front();
+                nSamples++;
+            }
+        }
+        auto end = std::chrono::high_resolution_clock::now();
+        if (seconds) *seconds = std::chrono::duration<double>(end - start).count();
+        if (outBox) {
+            const auto &d = packing->getBox().getDimensions();
+            for (int a = 0; a < 3; a++)
+                for (int b = 0; b < 3; b++) outBox[3 * a + b] = d(a, b);
+        }
+        for (std::size_t i = 0; outPos && i < n; i++) {
+            const auto &p = (*packing)[i].getPosition();
+            for (int a = 0; a < 3; a++) outPos[3 * i + a] = p[a];
+        }
+        for (std::size_t i = 0; outRot && i < n; i++) {
+            const auto &O = (*packing)[i].getOrientation();
+            for (int a = 0; a < 3; a++)
+                for (int b = 0; b < 3; b++) outRot[9 * i + 3 * a + b] = O(a, b);
+        }
+        if (finalSteps) {
+            auto steps = sampler.getStepSizes();
+            finalSteps[0] = steps[0].second;
+            finalSteps[1] = steps[1].second;
+            finalSteps[2] = scaler ? scaler->getStepSize() : 0;
+        }
+        return nSamples;
+    } catch (const std::exception &) {
+        return -1;
+    }
+}
+
+}  // extern "C"

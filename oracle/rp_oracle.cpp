@@ -1,0 +1,710 @@
+// rp_oracle.cpp -- CPU restatement of rampack's hot-path arithmetic.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP product in
+// rampack_amd/; the product never links, loads or calls it.  Only tests/,
+// __graft_entry__.smoke() and bench.py's cpu_baseline leg may use it.
+//
+// Every routine restates one reference function in the reference's own FP64
+// operation order (each * and + rounded on its own; built with
+// -ffp-contract=off).  The citations are path:line into the reference tree.
+//
+// Pinning (what fixes each routine beyond retyping it):
+//   * pair decisions: bit-compared with oracle/_ref (the reference's own
+//     SpherocylinderTraits / SphereTraits / PolysphereTraits compiled from its
+//     sources) on the near-contact corpus, and against the reference unit-test
+//     known answers (SpherocylinderTraitsTest.cpp, SphereTraitsTest.cpp);
+//   * segment distance: brute-force minimisation over the two segment
+//     parameters in long double, and the textbook special cases;
+//   * box / PBC: TriclinicBoxTest.cpp and PeriodicBoundaryConditionsTest.cpp
+//     known answers;
+//   * neighbour-grid total overlap counts and energies: O(N^2) brute force over
+//     periodic images, and oracle/_ref's Packing::countTotalOverlaps /
+//     getTotalEnergy on the same snapshots.
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <cstddef>
+#include <vector>
+#include <array>
+#include <algorithm>
+#include <limits>
+
+namespace orc {
+
+struct Vec { double v[3]; };
+struct Mat { double m[9]; };  // row-major
+
+// Vector.tpp:11-23 (elementwise + and -)
+static inline Vec add(const Vec &a, const Vec &b) { return {{a.v[0] + b.v[0], a.v[1] + b.v[1], a.v[2] + b.v[2]}}; }
+static inline Vec sub(const Vec &a, const Vec &b) { return {{a.v[0] - b.v[0], a.v[1] - b.v[1], a.v[2] - b.v[2]}}; }
+
+// Matrix.tpp:164-175: scalar multiply zero-fills for x == 0, is the identity for x == 1
+static inline Vec scal(double x, const Vec &a) {
+    if (x == 0) return {{0, 0, 0}};
+    if (x == 1) return a;
+    return {{a.v[0] * x, a.v[1] * x, a.v[2] * x}};
+}
+
+// Matrix.tpp:178-186: division skips x == 1
+static inline Vec divs(const Vec &a, double x) {
+    if (x == 1) return a;
+    return {{a.v[0] / x, a.v[1] / x, a.v[2] / x}};
+}
+
+// Vector.tpp:53-60: prod = 0; prod += x_i * y_i
+static inline double dot(const Vec &a, const Vec &b) {
+    double p = 0;
+    p += a.v[0] * b.v[0];
+    p += a.v[1] * b.v[1];
+    p += a.v[2] * b.v[2];
+    return p;
+}
+
+// Vector.tpp:105-112
+static inline Vec cross(const Vec &a, const Vec &b) {
+    return {{a.v[1] * b.v[2] - a.v[2] * b.v[1],
+             a.v[2] * b.v[0] - a.v[0] * b.v[2],
+             a.v[0] * b.v[1] - a.v[1] * b.v[0]}};
+}
+
+// Matrix.tpp:105-113: ret(i,j) starts at 0 and accumulates m1(i,k)*m2(k,j) over k
+static inline Vec mulv(const Mat &M, const Vec &x) {
+    Vec r{{0, 0, 0}};
+    for (int i = 0; i < 3; i++)
+        for (int k = 0; k < 3; k++)
+            r.v[i] += M.m[3 * i + k] * x.v[k];
+    return r;
+}
+
+static inline Mat mulm(const Mat &A, const Mat &B) {
+    Mat R{};
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            for (int k = 0; k < 3; k++)
+                R.m[3 * i + j] += A.m[3 * i + k] * B.m[3 * k + j];
+    return R;
+}
+
+static inline Vec column(const Mat &M, int c) { return {{M.m[c], M.m[3 + c], M.m[6 + c]}}; }
+
+// Matrix.tpp:352-359 (Sarrus determinant, evaluated left to right)
+static inline double det3(const Mat &M) {
+    const double *a = M.m;
+    return a[0] * a[4] * a[8] + a[1] * a[5] * a[6] + a[2] * a[3] * a[7]
+           - a[2] * a[4] * a[6] - a[1] * a[3] * a[8] - a[0] * a[5] * a[7];
+}
+
+// Matrix.tpp:296-331 for 3x3: the 2x2 minor built from the remaining rows/columns in order, det = a0*a3 - a1*a2
+static inline double minor3(const Mat &M, int row, int col) {
+    double b[4];
+    int n = 0;
+    for (int i = 0; i < 3; i++) {
+        if (i == row) continue;
+        for (int j = 0; j < 3; j++) {
+            if (j == col) continue;
+            b[n++] = M.m[3 * i + j];
+        }
+    }
+    return b[0] * b[3] - b[1] * b[2];
+}
+
+// Matrix.tpp:258-271
+static inline Mat inverse3(const Mat &M) {
+    double det = det3(M);
+    Mat out{};
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            out.m[3 * i + j] = ((i + j) % 2 == 0) ? (minor3(M, j, i) / det) : (-minor3(M, j, i) / det);
+    return out;
+}
+
+// Matrix.tpp:438-450: Rodrigues, I + K*sin + (1 - cos)*K*K, with the matrix ops left to right
+static inline Mat rotation(const Vec &axis, double angle) {
+    double s = std::sin(angle), c = std::cos(angle);
+    Mat K{{0, -axis.v[2], axis.v[1], axis.v[2], 0, -axis.v[0], -axis.v[1], axis.v[0], 0}};
+    Mat Ks{}, K2 = mulm(K, K), R{};
+    // K*sina (Matrix *= x with the 0 / 1 special cases)
+    for (int i = 0; i < 9; i++) Ks.m[i] = (s == 0) ? 0.0 : ((s == 1) ? K.m[i] : K.m[i] * s);
+    // (1 - cosa)*K first, then *K (operator* is left associative)
+    double f = 1 - c;
+    Mat fK{};
+    for (int i = 0; i < 9; i++) fK.m[i] = (f == 0) ? 0.0 : ((f == 1) ? K.m[i] : K.m[i] * f);
+    Mat fKK = mulm(fK, K);
+    (void)K2;
+    Mat I{{1, 0, 0, 0, 1, 0, 0, 0, 1}};
+    for (int i = 0; i < 9; i++) R.m[i] = (I.m[i] + Ks.m[i]) + fKK.m[i];
+    return R;
+}
+
+// TriclinicBox.h:61-70, TriclinicBox.cpp:27-51
+struct Box {
+    Mat dims, inv;
+    explicit Box(const double *d) { std::memcpy(dims.m, d, sizeof dims.m); inv = inverse3(dims); }
+    Vec toRel(const Vec &p) const { return mulv(inv, p); }
+    Vec toAbs(const Vec &r) const { return mulv(dims, r); }
+    std::array<Vec, 3> sides() const {
+        return {toAbs({{1, 0, 0}}), toAbs({{0, 1, 0}}), toAbs({{0, 0, 1}})};
+    }
+    double volume() const {
+        auto s = sides();
+        return std::abs(dot(cross(s[0], s[1]), s[2]));
+    }
+    std::array<double, 3> heights() const {
+        auto s = sides();
+        double vol = volume();
+        return {vol / std::sqrt(dot(cross(s[1], s[2]), cross(s[1], s[2]))),
+                vol / std::sqrt(dot(cross(s[2], s[0]), cross(s[2], s[0]))),
+                vol / std::sqrt(dot(cross(s[0], s[1]), cross(s[0], s[1])))};
+    }
+};
+
+// PeriodicBoundaryConditions.cpp:11-22
+static inline Vec pbcCorrection(const Box &box, const Vec &pos) {
+    Vec rel = box.toRel(pos);
+    Vec corr{{0, 0, 0}};
+    for (int i = 0; i < 3; i++) {
+        while (rel.v[i] + corr.v[i] < 0) corr.v[i] += 1;
+        while (rel.v[i] + corr.v[i] >= 1) corr.v[i] -= 1;
+    }
+    return box.toAbs(corr);
+}
+
+// PeriodicBoundaryConditions.cpp:24-36
+static inline Vec pbcTranslation(const Box &box, const Vec &p1, const Vec &p2) {
+    Vec r1 = box.toRel(p1), r2 = box.toRel(p2), t{{0, 0, 0}};
+    for (int i = 0; i < 3; i++) {
+        while (r2.v[i] + t.v[i] - r1.v[i] > 0.5) t.v[i] -= 1;
+        while (r2.v[i] + t.v[i] - r1.v[i] < -0.5) t.v[i] += 1;
+    }
+    return box.toAbs(t);
+}
+
+// PeriodicBoundaryConditions.cpp:38-50
+static inline double pbcDistance2(const Box &box, const Vec &p1, const Vec &p2) {
+    Vec r1 = box.toRel(p1), r2 = box.toRel(p2), d{};
+    for (int i = 0; i < 3; i++) {
+        double c = std::abs(r2.v[i] - r1.v[i]);
+        while (c > 0.5) c -= 1;
+        d.v[i] = c;
+    }
+    Vec a = box.toAbs(d);
+    return dot(a, a);
+}
+
+// Shape.cpp:9-12
+static inline Vec translateWrapped(const Box &box, const Vec &pos, const Vec &t) {
+    Vec p = add(pos, t);
+    return add(p, pbcCorrection(box, p));
+}
+
+// SegmentDistanceCalculator.h:22-85
+static inline double segmentDistance2(const Vec &s11, const Vec &s12, const Vec &s21, const Vec &s22) {
+    constexpr double EPS = 0.0000000001;
+    Vec u = sub(s12, s11), v = sub(s22, s21), w = sub(s11, s21);
+    double a = dot(u, u), b = dot(u, v), c = dot(v, v), d = dot(u, w), e = dot(v, w);
+    double D = a * c - b * b;
+    double sc, sN, sD = D;
+    double tc, tN, tD = D;
+    if (D < EPS) {
+        sN = 0.0; sD = 1.0; tN = e; tD = c;
+    } else {
+        sN = (b * e - c * d);
+        tN = (a * e - b * d);
+        if (sN < 0.0) { sN = 0.0; tN = e; tD = c; }
+        else if (sN > sD) { sN = sD; tN = e + b; tD = c; }
+    }
+    if (tN < 0.0) {
+        tN = 0.0;
+        if (-d < 0.0) sN = 0.0;
+        else if (-d > a) sN = sD;
+        else { sN = -d; sD = a; }
+    } else if (tN > tD) {
+        tN = tD;
+        if ((-d + b) < 0.0) sN = 0;
+        else if ((-d + b) > a) sN = sD;
+        else { sN = (-d + b); sD = a; }
+    }
+    sc = (std::abs(sN) < EPS ? 0.0 : sN / sD);
+    tc = (std::abs(tN) < EPS ? 0.0 : tN / tD);
+    Vec dP = sub(add(w, scal(sc, u)), scal(tc, v));
+    return dot(dP, dP);
+}
+
+// Shape descriptor used by the oracle entry points.
+enum Kind { SPHERE = 0, SPHEROCYLINDER = 1, POLYSPHERE = 2 };
+struct ShapeSpec {
+    int kind = SPHERE;
+    double radius = 0.5, length = 0;
+    std::vector<Vec> centres;       // polysphere bead positions (body frame)
+    std::vector<double> radii;      // polysphere bead radii
+    int soft = 0;                   // 0 none, 1 WCA on centre distance, 2 WCA on segment distance (no reference)
+    double eps = 1, sigma = 1;
+
+    // SphereTraits.h:34, SpherocylinderTraits.h:81, PolysphereTraits.cpp:116-121
+    double range() const {
+        if (kind == SPHERE) return 2 * radius;
+        if (kind == SPHEROCYLINDER) return 2 * radius + length;
+        double m = 0;
+        for (double r : radii) m = std::max(m, r);
+        double rr = 2 * m;
+        if (soft == 1) rr = std::max(rr, sigma * std::pow(2., 1. / 6));
+        return rr;
+    }
+    // Interaction.cpp:55-70
+    double totalRange() const {
+        if (kind != POLYSPHERE) return range();
+        double maxR2 = 0;
+        for (auto &c : centres) maxR2 = std::max(maxR2, dot(c, c));
+        return 2 * std::sqrt(maxR2) + range();
+    }
+    int nCentres() const { return kind == POLYSPHERE ? (int)centres.size() : 0; }
+    bool hasHard() const { return true; }
+};
+
+// Cap centre: SpherocylinderTraits.cpp:24-26
+static inline Vec capCentre(short beginOrEnd, const Vec &pos, const Mat &O, double length) {
+    return add(pos, scal(0.5 * beginOrEnd * length, column(O, 2)));
+}
+
+// SpherocylinderTraits.cpp:28-46, with pos2bc = pos2 + T as HardcodedTranslation gives (Packing.cpp:21-44)
+static inline bool spherocylinderOverlap(double r, double l, const Vec &p1, const Mat &O1, const Vec &p2, const Mat &O2,
+                                         const Vec &T) {
+    Vec p2bc = add(p2, T);
+    Vec d = sub(p2bc, p1);
+    double d2 = dot(d, d);
+    if (d2 < 4 * r * r) return true;
+    else if (d2 >= std::pow(2 * r + l, 2)) return false;
+    return segmentDistance2(capCentre(-1, p1, O1, l), capCentre(1, p1, O1, l),
+                            capCentre(-1, p2bc, O2, l), capCentre(1, p2bc, O2, l)) < 4 * r * r;
+}
+
+// HardcodedTranslation::getDistance2 (Packing.cpp:36-42): (p2 + T - p1).norm2()
+static inline double hcDistance2(const Vec &p1, const Vec &p2, const Vec &T) {
+    Vec d = sub(add(p2, T), p1);
+    return dot(d, d);
+}
+
+// SphereTraits.cpp:56-65
+static inline bool sphereOverlap(double r, const Vec &p1, const Vec &p2, const Vec &T) {
+    return hcDistance2(p1, p2, T) < std::pow(2 * r, 2);
+}
+
+// PolysphereTraits.cpp:97-106 (per bead pair, centres already absolute)
+static inline bool beadOverlap(double r1, double r2, const Vec &c1, const Vec &c2, const Vec &T) {
+    double r = r1 + r2;
+    return hcDistance2(c1, c2, T) < r * r;
+}
+
+// RepulsiveLennardJonesInteraction.cpp:8-16 (WCA)
+static inline double wcaEnergy(double eps, double sigma, double d2) {
+    double s216 = sigma * std::pow(2., 1. / 6);
+    if (d2 >= s216 * s216) return 0;
+    double x2 = sigma * sigma / d2;
+    double x6 = x2 * x2 * x2;
+    double x12 = x6 * x6;
+    return 4 * eps * (x12 - x6) + eps;
+}
+
+// Pair overlap with an explicit translation (the Packing/NG path).
+static inline bool pairOverlap(const ShapeSpec &s, const Vec &p1, const Mat &O1, int c1, const Vec &p2, const Mat &O2,
+                               int c2, const Vec &T) {
+    switch (s.kind) {
+        case SPHERE: return sphereOverlap(s.radius, p1, p2, T);
+        case SPHEROCYLINDER: return spherocylinderOverlap(s.radius, s.length, p1, O1, p2, O2, T);
+        default: return beadOverlap(s.radii[c1], s.radii[c2], p1, p2, T);
+    }
+}
+
+// Soft energy with an explicit translation.  soft == 1 follows CentralInteraction.h:58-67 on the centre distance;
+// soft == 2 (WCA on the spherocylinder segment distance) has no reference counterpart and is defined as
+// wca(segmentDistance2(caps)) with the same cap construction as the overlap test.
+static inline double pairEnergy(const ShapeSpec &s, const Vec &p1, const Mat &O1, const Vec &p2, const Mat &O2,
+                                const Vec &T) {
+    if (s.soft == 1) return wcaEnergy(s.eps, s.sigma, hcDistance2(p1, p2, T));
+    if (s.soft == 2) {
+        Vec p2bc = add(p2, T);
+        double l = s.length;
+        double d2 = segmentDistance2(capCentre(-1, p1, O1, l), capCentre(1, p1, O1, l),
+                                     capCentre(-1, p2bc, O2, l), capCentre(1, p2bc, O2, l));
+        return wcaEnergy(s.eps, s.sigma, d2);
+    }
+    return 0;
+}
+
+// NeighbourGrid restatement (NeighbourGrid.cpp:13-34, 80-143, 178-217), real cells in [1, div-2].
+struct Grid {
+    Box box;
+    std::array<std::size_t, 3> div{};
+    std::array<double, 3> relCell{};
+    std::vector<long> head, next;
+    std::vector<std::size_t> reflected, transIdx;
+    std::array<Vec, 27> trans{};
+    std::vector<long> offsets, posOffsets;
+    std::size_t numCells = 0;
+
+    Grid(const Box &b, double cellSize, std::size_t nItems) : box(b) {
+        auto h = box.heights();
+        for (int i = 0; i < 3; i++) div[i] = (std::size_t)std::floor(h[i] / cellSize) + 2;
+        for (int i = 0; i < 3; i++) relCell[i] = 1 / (double)(div[i] - 2);
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++)
+                for (int k = 0; k < 3; k++)
+                    trans[i * 9 + j * 3 + k] = box.toAbs({{(double)i - 1, (double)j - 1., (double)k - 1.}});
+        numCells = div[0] * div[1] * div[2];
+        head.assign(numCells, -1);
+        next.assign(nItems, -1);
+        reflected.resize(numCells);
+        transIdx.resize(numCells);
+        for (std::size_t c = 0; c < numCells; c++) reflectedData(c, reflected[c], transIdx[c]);
+        fillOffsets();
+    }
+    std::array<std::size_t, 3> coords(std::size_t c) const {
+        std::array<std::size_t, 3> r{};
+        for (int i = 0; i < 3; i++) { r[i] = c % div[i]; c /= div[i]; }
+        return r;
+    }
+    std::size_t cellNo(const std::array<std::size_t, 3> &c) const {
+        std::size_t r = 0;
+        for (int i = 2; i >= 0; i--) r = div[i] * r + c[i];
+        return r;
+    }
+    void reflectedData(std::size_t c, std::size_t &real, std::size_t &tIdx) const {
+        auto cc = coords(c);
+        std::array<std::size_t, 3> t{1, 1, 1};
+        for (int i = 0; i < 3; i++) {
+            if (cc[i] == 0) { cc[i] = div[i] - 2; t[i] = 0; }
+            else if (cc[i] == div[i] - 1) { cc[i] = 1; t[i] = 2; }
+        }
+        real = cellNo(cc);
+        tIdx = t[0] * 9 + t[1] * 3 + t[2];
+    }
+    void fillOffsets() {
+        std::array<std::size_t, 3> mid{div[0] / 2, div[1] / 2, div[2] / 2};
+        long midNo = (long)cellNo(mid);
+        for (int a = 0; a < 3; a++)           // neighbour[2] is the slowest index (NeighbourGrid.cpp:102-111)
+            for (int b = 0; b < 3; b++)
+                for (int c = 0; c < 3; c++) {
+                    std::array<int, 3> nb{c, b, a};
+                    std::array<std::size_t, 3> cc{mid[0] + nb[0] - 1, mid[1] + nb[1] - 1, mid[2] + nb[2] - 1};
+                    long no = (long)cellNo(cc);
+                    offsets.push_back(no - midNo);
+                    if (9 * nb[0] + 3 * nb[1] + nb[2] > 13) posOffsets.push_back(no - midNo);
+                }
+        std::sort(offsets.begin(), offsets.end());
+        offsets.erase(std::unique(offsets.begin(), offsets.end()), offsets.end());
+        std::sort(posOffsets.begin(), posOffsets.end());
+        posOffsets.erase(std::unique(posOffsets.begin(), posOffsets.end()), posOffsets.end());
+    }
+    std::size_t positionToCell(const Vec &pos) const {
+        std::size_t r = 0;
+        Vec rel = box.toRel(pos);
+        for (int i = 2; i >= 0; i--) {
+            constexpr double EPS = std::numeric_limits<double>::epsilon() * 10;
+            double x = rel.v[i];
+            if (x < 0) x = 0;
+            else if (x >= 1) x = 1 - EPS;
+            std::size_t c = (std::size_t)((int)(x / relCell[i]) + 1);
+            r = div[i] * r + c;
+        }
+        return r;
+    }
+    void add(long idx, std::size_t cell) { next[idx] = head[cell]; head[cell] = idx; }
+};
+
+// Packing::rebuildNeighbourGrid decision (Packing.cpp:858-883).  Returns the cell size, or 0 when the reference
+// would run without a neighbour grid.
+static inline double ngCellSize(const Box &box, double range, std::size_t n) {
+    double cell = range;
+    double minCell = std::cbrt(std::abs(box.volume()) / n) / 5;
+    if (range < minCell) cell = minCell;
+    auto h = box.heights();
+    if (cell * 4 > *std::max_element(h.begin(), h.end())) return 0;
+    if (cell * (1 + 1e-12) > *std::min_element(h.begin(), h.end())) return 0;
+    return cell;
+}
+
+struct Snapshot {
+    const ShapeSpec &spec;
+    Box box;
+    std::size_t n;
+    std::vector<Vec> pos;
+    std::vector<Mat> rot;
+    std::vector<Vec> absCentres;    // n * nc, Packing.cpp:1036-1047 and 940-944
+    Snapshot(const ShapeSpec &s, const double *b, std::size_t n_, const double *p, const double *r)
+            : spec(s), box(b), n(n_), pos(n_), rot(n_) {
+        for (std::size_t i = 0; i < n; i++) {
+            std::memcpy(pos[i].v, p + 3 * i, 24);
+            std::memcpy(rot[i].m, r + 9 * i, 72);
+        }
+        int nc = spec.nCentres();
+        absCentres.resize(n * nc);
+        for (std::size_t i = 0; i < n; i++)
+            for (int c = 0; c < nc; c++) {
+                Vec rc = mulv(rot[i], spec.centres[c]);
+                Vec a = add(pos[i], rc);
+                absCentres[i * nc + c] = add(a, pbcCorrection(box, a));
+            }
+    }
+    const Vec &itemPos(std::size_t item) const {
+        return spec.nCentres() ? absCentres[item] : pos[item];
+    }
+};
+
+// Packing::countTotalOverlaps with a neighbour grid (Packing.cpp:431-559), or all pairs without one (:582-617).
+static std::uint64_t countTotal(const Snapshot &S, bool earlyExit, double *energyOut) {
+    const ShapeSpec &sp = S.spec;
+    int nc = sp.nCentres();
+    std::size_t nItems = nc ? S.n * nc : S.n;
+    double cell = ngCellSize(S.box, sp.range(), S.n);
+    std::uint64_t count = 0;
+    double energy = 0;
+    auto partOf = [&](std::size_t item) { return nc ? item / nc : item; };
+    auto centreOf = [&](std::size_t item) { return nc ? (int)(item % nc) : 0; };
+    Vec zero{{0, 0, 0}};
+    if (cell == 0) {
+        for (std::size_t i = 0; i < S.n; i++)
+            for (std::size_t j = i + 1; j < S.n; j++) {
+                if (!nc) {
+                    if (sp.kind == SPHERE) {
+                        if (pbcDistance2(S.box, S.pos[i], S.pos[j]) < std::pow(2 * sp.radius, 2)) count++;
+                    } else {
+                        Vec T = pbcTranslation(S.box, S.pos[i], S.pos[j]);
+                        if (pairOverlap(sp, S.pos[i], S.rot[i], 0, S.pos[j], S.rot[j], 0, T)) count++;
+                        if (energyOut) energy += pairEnergy(sp, S.pos[i], S.rot[i], S.pos[j], S.rot[j], T);
+                    }
+                } else {
+                    for (int a = 0; a < nc; a++)
+                        for (int b = 0; b < nc; b++) {
+                            const Vec &c1 = S.absCentres[i * nc + a], &c2 = S.absCentres[j * nc + b];
+                            double r = sp.radii[a] + sp.radii[b];
+                            if (pbcDistance2(S.box, c1, c2) < r * r) count++;
+                        }
+                }
+                if (earlyExit && count) return 1;
+            }
+        if (energyOut) *energyOut = energy;
+        return count;
+    }
+    Grid g(S.box, cell, nItems);
+    for (std::size_t it = 0; it < nItems; it++) g.add((long)it, g.positionToCell(S.itemPos(it)));
+    for (std::size_t ci = 0; ci < g.div[0] - 2; ci++)
+        for (std::size_t cj = 0; cj < g.div[1] - 2; cj++)
+            for (std::size_t ck = 0; ck < g.div[2] - 2; ck++) {
+                std::size_t cno = g.cellNo({ci + 1, cj + 1, ck + 1});
+                for (long a = g.head[cno]; a != -1; a = g.next[a]) {
+                    std::size_t pa = partOf(a);
+                    for (long b = g.next[a]; b != -1; b = g.next[b]) {
+                        std::size_t pb = partOf(b);
+                        if (nc && pa == pb) continue;
+                        if (pairOverlap(sp, S.itemPos(a), S.rot[pa], centreOf(a), S.itemPos(b), S.rot[pb],
+                                        centreOf(b), zero)) {
+                            if (earlyExit) return 1;
+                            count++;
+                        }
+                        if (energyOut) energy += pairEnergy(sp, S.itemPos(a), S.rot[pa], S.itemPos(b), S.rot[pb], zero);
+                    }
+                    for (long off : g.posOffsets) {
+                        std::size_t nb = (std::size_t)((long)cno + off);
+                        const Vec &T = g.trans[g.transIdx[nb]];
+                        for (long b = g.head[g.reflected[nb]]; b != -1; b = g.next[b]) {
+                            std::size_t pb = partOf(b);
+                            if (nc && pa == pb) continue;
+                            if (pairOverlap(sp, S.itemPos(a), S.rot[pa], centreOf(a), S.itemPos(b), S.rot[pb],
+                                            centreOf(b), T)) {
+                                if (earlyExit) return 1;
+                                count++;
+                            }
+                            if (energyOut) energy += pairEnergy(sp, S.itemPos(a), S.rot[pa], S.itemPos(b), S.rot[pb], T);
+                        }
+                    }
+                }
+            }
+    if (energyOut) *energyOut = energy;
+    return count;
+}
+
+// Brute force over the 27 images of every pair: counts (pair, image) combinations that overlap.  It equals the
+// neighbour-grid count whenever at most one image of any pair lies within range.
+static std::uint64_t countBrute(const Snapshot &S) {
+    const ShapeSpec &sp = S.spec;
+    std::uint64_t count = 0;
+    Vec tr[27];
+    for (int i = 0; i < 27; i++) tr[i] = S.box.toAbs({{(double)(i / 9) - 1, (double)(i / 3 % 3) - 1, (double)(i % 3) - 1}});
+    int nc = sp.nCentres();
+    for (std::size_t i = 0; i < S.n; i++)
+        for (std::size_t j = i + 1; j < S.n; j++)
+            for (int t = 0; t < 27; t++) {
+                if (!nc) {
+                    if (pairOverlap(sp, S.pos[i], S.rot[i], 0, S.pos[j], S.rot[j], 0, tr[t])) count++;
+                } else {
+                    for (int a = 0; a < nc; a++)
+                        for (int b = 0; b < nc; b++)
+                            if (pairOverlap(sp, S.absCentres[i * nc + a], S.rot[i], a, S.absCentres[j * nc + b],
+                                            S.rot[j], b, tr[t]))
+                                count++;
+                }
+            }
+    return count;
+}
+
+static ShapeSpec makeSpec(int kind, double radius, double length, int nCentres, const double *centres,
+                          const double *radii, int soft, double eps, double sigma) {
+    ShapeSpec s;
+    s.kind = kind; s.radius = radius; s.length = length; s.soft = soft; s.eps = eps; s.sigma = sigma;
+    for (int i = 0; i < nCentres; i++) {
+        s.centres.push_back({{centres[3 * i], centres[3 * i + 1], centres[3 * i + 2]}});
+        s.radii.push_back(radii[i]);
+    }
+    return s;
+}
+
+}  // namespace orc
+
+using namespace orc;
+
+extern "C" {
+
+// Pair records: pos1[3] O1[9] pos2[3] O2[9] T[3] (27 doubles), O row-major.  kind 0 sphere (radius), 1
+// spherocylinder (radius, length).  Writes 1 for overlap, 0 otherwise.
+int orc_pair_decisions(int kind, double radius, double length, std::size_t n, const double *rec, std::uint8_t *out) {
+    for (std::size_t i = 0; i < n; i++) {
+        const double *q = rec + 27 * i;
+        Vec p1, p2, T;
+        Mat O1, O2;
+        std::memcpy(p1.v, q, 24); std::memcpy(O1.m, q + 3, 72);
+        std::memcpy(p2.v, q + 12, 24); std::memcpy(O2.m, q + 15, 72);
+        std::memcpy(T.v, q + 24, 24);
+        bool o = (kind == SPHERE) ? sphereOverlap(radius, p1, p2, T)
+                                  : spherocylinderOverlap(radius, length, p1, O1, p2, O2, T);
+        out[i] = o ? 1 : 0;
+    }
+    return 0;
+}
+
+// Squared segment distance for 4 points per record (12 doubles).
+void orc_segment_distance2(std::size_t n, const double *pts, double *out) {
+    for (std::size_t i = 0; i < n; i++) {
+        const double *q = pts + 12 * i;
+        Vec a{{q[0], q[1], q[2]}}, b{{q[3], q[4], q[5]}}, c{{q[6], q[7], q[8]}}, d{{q[9], q[10], q[11]}};
+        out[i] = segmentDistance2(a, b, c, d);
+    }
+}
+
+// Total overlap count (and soft energy) of a snapshot, as Packing::countTotalOverlaps / getTotalEnergy compute it.
+int orc_snapshot_stats(int kind, double radius, double length, int nCentres, const double *centres,
+                       const double *radii, int soft, double eps, double sigma, const double *box, std::size_t n,
+                       const double *pos, const double *rot, int earlyExit, std::uint64_t *overlaps,
+                       double *energy) {
+    ShapeSpec s = makeSpec(kind, radius, length, nCentres, centres, radii, soft, eps, sigma);
+    Snapshot S(s, box, n, pos, rot);
+    *overlaps = countTotal(S, earlyExit != 0, nullptr);
+    if (energy) {
+        double e = 0;
+        if (soft) countTotal(S, false, &e);
+        *energy = e;
+    }
+    return 0;
+}
+
+std::uint64_t orc_count_brute(int kind, double radius, double length, int nCentres, const double *centres,
+                              const double *radii, const double *box, std::size_t n, const double *pos,
+                              const double *rot) {
+    ShapeSpec s = makeSpec(kind, radius, length, nCentres, centres, radii, 0, 1, 1);
+    Snapshot S(s, box, n, pos, rot);
+    return countBrute(S);
+}
+
+// Box helpers: inverse, heights, volume, relative <-> absolute, PBC correction / translation / distance2.
+void orc_box_info(const double *box, double *inv, double *heights, double *volume) {
+    Box b(box);
+    std::memcpy(inv, b.inv.m, 72);
+    auto h = b.heights();
+    heights[0] = h[0]; heights[1] = h[1]; heights[2] = h[2];
+    *volume = b.volume();
+}
+
+void orc_pbc(const double *box, const double *p1, const double *p2, double *corr1, double *trans, double *dist2) {
+    Box b(box);
+    Vec a{{p1[0], p1[1], p1[2]}}, c{{p2[0], p2[1], p2[2]}};
+    Vec k = pbcCorrection(b, a), t = pbcTranslation(b, a, c);
+    std::memcpy(corr1, k.v, 24);
+    std::memcpy(trans, t.v, 24);
+    *dist2 = pbcDistance2(b, a, c);
+}
+
+// Packing.cpp:1155-1176
+int orc_is_box_upscaled(const double *oldBox, const double *newBox) {
+    Box o(oldBox), n(newBox);
+    auto os = o.sides(), ns = n.sides();
+    constexpr double EPSILON = 100 * std::numeric_limits<double>::epsilon();
+    for (int i = 0; i < 3; i++) {
+        double d = dot(os[i], ns[i]);
+        double n2o = dot(os[i], os[i]);
+        double n2n = dot(ns[i], ns[i]);
+        if (std::abs(d * d / n2o / n2n - 1) > EPSILON) return 0;
+        if (d * d / n2o / n2o < 1 + EPSILON) return 0;
+    }
+    return 1;
+}
+
+// Shape::translate with PBC (Shape.cpp:9-12) for n positions.
+void orc_translate_wrapped(const double *box, std::size_t n, const double *pos, const double *t, double *out) {
+    Box b(box);
+    for (std::size_t i = 0; i < n; i++) {
+        Vec p{{pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]}}, d{{t[3 * i], t[3 * i + 1], t[3 * i + 2]}};
+        Vec r = translateWrapped(b, p, d);
+        std::memcpy(out + 3 * i, r.v, 24);
+    }
+}
+
+// Packing.cpp:183-184: newBox.relativeToAbsolute(oldBox.absoluteToRelative(pos))
+void orc_rescale(const double *oldBox, const double *newBox, std::size_t n, const double *pos, double *out) {
+    Box o(oldBox), nb(newBox);
+    for (std::size_t i = 0; i < n; i++) {
+        Vec p{{pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]}};
+        Vec r = nb.toAbs(o.toRel(p));
+        std::memcpy(out + 3 * i, r.v, 24);
+    }
+}
+
+// Matrix<3,3>::rotation(axis, angle) (Matrix.tpp:438-450)
+void orc_rotation(const double *axis, double angle, double *out) {
+    Mat R = rotation({{axis[0], axis[1], axis[2]}}, angle);
+    std::memcpy(out, R.m, 72);
+}
+
+// Nematic order P2 (NematicOrder.cpp:9-28): Q = 0.5*(3*<a a^T> - I), a = orientation column 2
+// (SpherocylinderTraits.cpp:52-54); P2 = the eigenvalue of largest magnitude, eigenvalues from the Cardano form of
+// NematicOrder.cpp:30-59.
+double orc_nematic_p2(std::size_t n, const double *rot) {
+    double Q[9] = {0};
+    for (std::size_t i = 0; i < n; i++) {
+        const double *O = rot + 9 * i;
+        double a[3] = {O[2], O[5], O[8]};
+        for (int r = 0; r < 3; r++)
+            for (int c = 0; c < 3; c++)
+                Q[3 * r + c] += a[r] * a[c];
+    }
+    for (double &q : Q) q /= (double)n;
+    for (int i = 0; i < 9; i++) Q[i] = 0.5 * (3. * Q[i] - (i % 4 == 0 ? 1.0 : 0.0));
+    double A11 = Q[0], A12 = Q[1], A13 = Q[2], A22 = Q[4], A23 = Q[5], A33 = Q[8];
+    double b = A11 + A22 + A33;
+    double c = A11*A22 + A11*A33 + A22*A33 - A12*A12 - A13*A13 - A23*A23;
+    double d = A11*A23*A23 + A22*A13*A13 + A33*A12*A12 - A11*A22*A33 - 2*A12*A13*A23;
+    double p = b*b - 3*c;
+    if (!(p > 0)) return 0;
+    double sp = std::sqrt(p);
+    double q = 2*b*b*b - 9*b*c - 27*d;
+    double ratio = q / 2 / (sp*sp*sp);
+    if (std::abs(ratio) > 1) ratio = ratio < 0 ? -1 : 1;
+    double Delta = std::acos(ratio);
+    double e[3] = {1./3*(b + 2*sp*std::cos(Delta/3)),
+                   1./3*(b + 2*sp*std::cos((Delta + 2*M_PI)/3)),
+                   1./3*(b + 2*sp*std::cos((Delta - 2*M_PI)/3))};
+    double best = e[0];
+    for (int i = 1; i < 3; i++) if (std::abs(best) < std::abs(e[i])) best = e[i];
+    return best;
+}
+
+}  // extern "C"

@@ -1,0 +1,249 @@
+"""MI355X-native rampack Monte Carlo sweep.
+
+Python access to the C ABI declared in include/rampack_amd.h (librampack_amd.so, built for gfx950).  The
+classes mirror the reference's Packing / Simulation surface for the sweep (src/core/Packing.h,
+src/core/Simulation.h): shapes are uploaded once, cycles run on the GPU, and the state comes back on request.
+
+There is no CPU fallback: if the HIP library is missing or no GPU is visible, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+from typing import Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "librampack_amd.so")
+
+SHAPE_SPHERE = 0
+SHAPE_SPHEROCYLINDER = 1
+PREC_FP64 = 0
+BOX_NONE = 0
+BOX_DELTA_TRICLINIC = 1
+BOX_LOG_ISOTROPIC = 2
+BOX_DELTA_V = 3
+
+RP_OK = 0
+RP_ERR_PRECONDITION = -1
+RP_ERR_VALIDATION = -2
+RP_ERR_DEVICE = -3
+RP_ERR_UNSUPPORTED = -4
+
+# Every symbol include/rampack_amd.h declares.
+EXPORTED_SYMBOLS = (
+    "rp_create", "rp_set_environment", "rp_run_cycles", "rp_count_total_overlaps", "rp_download",
+    "rp_set_timing", "rp_get_timing", "rp_destroy", "rp_pair_decisions", "rp_last_error",
+)
+
+
+class ShapeDesc(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("radius", ctypes.c_double), ("length", ctypes.c_double)]
+
+
+class MoveDesc(ctypes.Structure):
+    _fields_ = [("trans_step", ctypes.c_double), ("rot_step", ctypes.c_double),
+                ("max_trans_step", ctypes.c_double)]
+
+
+class BoxDesc(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("step", ctypes.c_double)]
+
+
+class Counters(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_uint64) for name in (
+        "cycles", "moves_attempted", "moves_accepted", "moves_left_cell", "box_attempted", "box_accepted",
+        "box_scans", "box_too_small", "skipped_blocks", "pair_tests")] + [
+        ("trans_step", ctypes.c_double), ("rot_step", ctypes.c_double), ("box_step", ctypes.c_double)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class Timing(ctypes.Structure):
+    _fields_ = [("sweep_launches", ctypes.c_uint64), ("sweep_ms", ctypes.c_double),
+                ("rebuild_launches", ctypes.c_uint64), ("rebuild_ms", ctypes.c_double),
+                ("scan_launches", ctypes.c_uint64), ("scan_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class RampackError(RuntimeError):
+    """Error returned through the C ABI (PreconditionException / ValidationException analogues)."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(f"[{code}] {message}")
+        self.code = code
+
+
+class PreconditionError(RampackError):
+    pass
+
+
+class ValidationError(RampackError):
+    pass
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Loads librampack_amd.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: run __graft_entry__.build() to compile the HIP library")
+    lib = ctypes.CDLL(path)
+    dp = ctypes.POINTER(ctypes.c_double)
+    vp = ctypes.c_void_p
+    lib.rp_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(ShapeDesc), dp, ctypes.c_size_t, dp, dp,
+                              ctypes.c_uint64, ctypes.c_int]
+    lib.rp_set_environment.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.POINTER(MoveDesc),
+                                       ctypes.POINTER(BoxDesc)]
+    lib.rp_run_cycles.argtypes = [vp, ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(Counters)]
+    lib.rp_count_total_overlaps.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64)]
+    lib.rp_download.argtypes = [vp, dp, dp, dp]
+    lib.rp_set_timing.argtypes = [vp, ctypes.c_int]
+    lib.rp_get_timing.argtypes = [vp, ctypes.POINTER(Timing)]
+    lib.rp_destroy.argtypes = [vp]
+    lib.rp_destroy.restype = None
+    lib.rp_pair_decisions.argtypes = [ctypes.POINTER(ShapeDesc), ctypes.c_size_t, dp,
+                                      ctypes.POINTER(ctypes.c_uint8)]
+    lib.rp_last_error.restype = ctypes.c_char_p
+    _lib = lib
+    return lib
+
+
+def _check(code: int):
+    if code == RP_OK:
+        return
+    msg = _lib.rp_last_error().decode()
+    if code == RP_ERR_PRECONDITION:
+        raise PreconditionError(code, msg)
+    if code == RP_ERR_VALIDATION:
+        raise ValidationError(code, msg)
+    raise RampackError(code, msg)
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+@dataclass
+class Shape:
+    """ShapeTraits descriptor: 'sphere' (radius) or 'spherocylinder' (radius, length between cap centres)."""
+    kind: str
+    radius: float = 0.5
+    length: float = 0.0
+
+    def desc(self) -> ShapeDesc:
+        kinds = {"sphere": SHAPE_SPHERE, "spherocylinder": SHAPE_SPHEROCYLINDER}
+        if self.kind not in kinds:
+            raise ValueError(f"unknown shape {self.kind}")
+        return ShapeDesc(kinds[self.kind], float(self.radius), float(self.length))
+
+    @property
+    def volume(self) -> float:
+        """SphereTraits::getVolume / SpherocylinderTraits::getVolume (SpherocylinderTraits.cpp:48-50)."""
+        r, l = self.radius, self.length
+        if self.kind == "sphere":
+            return 4.0 / 3 * np.pi * r ** 3
+        return np.pi * r * r * l + 4.0 / 3 * np.pi * r ** 3
+
+    @property
+    def range(self) -> float:
+        return 2 * self.radius + (self.length if self.kind == "spherocylinder" else 0.0)
+
+
+def pair_decisions(shape: Shape, records: np.ndarray) -> np.ndarray:
+    """Device pair decisions on records of 27 doubles: pos1, O1 (row-major), pos2, O2, T."""
+    lib = load_library()
+    rec = np.ascontiguousarray(records, dtype=np.float64).reshape(-1, 27)
+    out = np.zeros(len(rec), dtype=np.uint8)
+    d = shape.desc()
+    _check(lib.rp_pair_decisions(ctypes.byref(d), len(rec), _dptr(rec),
+                                 out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))))
+    return out
+
+
+class GpuPacking:
+    """A packing resident on the GPU with its Simulation environment (Packing + Simulation surface)."""
+
+    def __init__(self, shape: Shape, box: np.ndarray, pos: np.ndarray, rot: np.ndarray, seed: int = 1234,
+                 precision: int = PREC_FP64):
+        lib = load_library()
+        self.shape = shape
+        self.n = len(pos)
+        box = np.ascontiguousarray(box, dtype=np.float64).reshape(9)
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+        rot = np.ascontiguousarray(rot, dtype=np.float64).reshape(-1, 9)
+        if len(rot) != self.n:
+            raise ValueError("positions and orientations differ in length")
+        self._ctx = ctypes.c_void_p()
+        d = shape.desc()
+        _check(lib.rp_create(ctypes.byref(self._ctx), ctypes.byref(d), _dptr(box), self.n, _dptr(pos), _dptr(rot),
+                             int(seed), int(precision)))
+
+    def close(self):
+        if self._ctx:
+            _lib.rp_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_environment(self, temperature: float, pressure: float, trans_step: float,
+                        rot_step: Optional[float] = None, max_trans_step: float = 0.0,
+                        box_move: Optional[tuple] = None):
+        """box_move: None (NVT), ('delta_triclinic', step), ('log', step) (isotropic) or ('delta_v', step)."""
+        m = MoveDesc(float(trans_step), float(rot_step) if rot_step else 0.0, float(max_trans_step))
+        bd = None
+        if box_move is not None:
+            kinds = {"delta_triclinic": BOX_DELTA_TRICLINIC, "log": BOX_LOG_ISOTROPIC, "delta_v": BOX_DELTA_V}
+            bd = BoxDesc(kinds[box_move[0]], float(box_move[1]))
+        _check(_lib.rp_set_environment(self._ctx, float(temperature), float(pressure), ctypes.byref(m),
+                                       ctypes.byref(bd) if bd is not None else None))
+
+    def run_cycles(self, n_cycles: int, adapt_steps: bool = False) -> dict:
+        c = Counters()
+        _check(_lib.rp_run_cycles(self._ctx, int(n_cycles), 1 if adapt_steps else 0, ctypes.byref(c)))
+        return c.as_dict()
+
+    def count_total_overlaps(self, early_exit: bool = False) -> int:
+        v = ctypes.c_uint64()
+        _check(_lib.rp_count_total_overlaps(self._ctx, 1 if early_exit else 0, ctypes.byref(v)))
+        return int(v.value)
+
+    def download(self):
+        box = np.zeros(9)
+        pos = np.zeros((self.n, 3))
+        rot = np.zeros((self.n, 9))
+        _check(_lib.rp_download(self._ctx, _dptr(box), _dptr(pos), _dptr(rot)))
+        return box.reshape(3, 3), pos, rot
+
+    def set_timing(self, enable: bool):
+        _check(_lib.rp_set_timing(self._ctx, 1 if enable else 0))
+
+    def timing(self) -> dict:
+        t = Timing()
+        _check(_lib.rp_get_timing(self._ctx, ctypes.byref(t)))
+        return t.as_dict()
+
+    def packing_fraction(self) -> float:
+        box, _, _ = self.download()
+        return self.n * self.shape.volume / abs(np.linalg.det(box))
+
+
+def nematic_p2(rot: np.ndarray) -> float:
+    """NematicOrder (NematicOrder.cpp:9-60): largest-magnitude eigenvalue of Q = (3<a a^T> - I)/2, a = column 2."""
+    a = np.asarray(rot).reshape(-1, 3, 3)[:, :, 2]
+    q = 0.5 * (3 * (a.T @ a) / len(a) - np.eye(3))
+    w = np.linalg.eigvalsh(q)
+    return float(w[np.argmax(np.abs(w))])

@@ -1,0 +1,721 @@
+// rp_capi.cpp -- host side of the C ABI (include/rampack_amd.h): device buffers, the cycle driver, box moves and
+// step adaptation.  The reference's Simulation/Packing orchestration (src/core/Simulation.cpp:325-607,
+// src/core/Packing.cpp:172-208, 858-899) is restated here on top of the gfx950 kernels of rp_kernels.hip.
+#include "rampack_amd.h"
+#include "rp_kernels.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace rp;
+
+namespace {
+
+thread_local std::string g_lastError;
+
+int fail(int code, const std::string &msg) {
+    g_lastError = msg;
+    return code;
+}
+
+#define RP_HIP(call)                                                                                   \
+    do {                                                                                               \
+        hipError_t e_ = (call);                                                                        \
+        if (e_ != hipSuccess) return fail(RP_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+struct HostBox {
+    double dims[9], inv[9];
+    void set(const double *d) {
+        std::memcpy(dims, d, sizeof dims);
+        inverse3(dims, inv);
+    }
+    // TriclinicBox::getSides / getVolume / getHeights (TriclinicBox.cpp:27-51)
+    V3 side(int i) const { return mulv(dims, v3(i == 0, i == 1, i == 2)); }
+    double volume() const { return std::fabs(dot(cross(side(0), side(1)), side(2))); }
+    void heights(double h[3]) const {
+        V3 s0 = side(0), s1 = side(1), s2 = side(2);
+        double vol = volume();
+        V3 c12 = cross(s1, s2), c20 = cross(s2, s0), c01 = cross(s0, s1);
+        h[0] = vol / std::sqrt(dot(c12, c12));
+        h[1] = vol / std::sqrt(dot(c20, c20));
+        h[2] = vol / std::sqrt(dot(c01, c01));
+    }
+};
+
+// Packing::isBoxUpscaled (Packing.cpp:1155-1176)
+bool isBoxUpscaled(const HostBox &o, const HostBox &nb) {
+    const double EPS = 100 * 2.220446049250313e-16;
+    for (int i = 0; i < 3; i++) {
+        V3 os = o.side(i), ns = nb.side(i);
+        double d = dot(os, ns), n2o = dot(os, os), n2n = dot(ns, ns);
+        if (std::fabs(d * d / n2o / n2n - 1) > EPS) return false;
+        if (d * d / n2o / n2o < 1 + EPS) return false;
+    }
+    return true;
+}
+
+struct DeviceParticles {
+    Particles p{};
+    void *mem = nullptr;
+};
+
+}  // namespace
+
+struct rp_ctx {
+    hipStream_t stream = nullptr;
+    rp_shape_desc shape{};
+    PairConsts k{};
+    double range = 0;
+    size_t n = 0;
+    HostBox box;
+    DeviceParticles cur, alt;
+    int *cellStart = nullptr, *cellCount = nullptr;
+    int ncellCap = 0;
+    int *tmpSlot = nullptr, *tmpOrder = nullptr;
+    double *tmpPos = nullptr;
+    unsigned long long *dCounters = nullptr;
+    unsigned long long *hCounters = nullptr;  // pinned
+    uint64_t seed = 0;
+    uint64_t cycle = 0;
+    Philox hostRng{};
+
+    // environment
+    bool envSet = false;
+    double temperature = 1, pressure = 0;
+    double transStep = 0.1, rotStep = 0.1, maxTransStep = 0;
+    bool maxTransImplicit = true;
+    int boxKind = RP_BOX_NONE;
+    double boxStep = 0;
+
+    // grid the current arrays are sorted by, and whether it is ready for the next cycle
+    Grid grid{};
+    bool sortedForCycle = false;
+
+    // counters
+    uint64_t boxAttempted = 0, boxAccepted = 0, boxScans = 0, boxTooSmall = 0, cycles = 0;
+    uint64_t accSinceEval = 0, attSinceEval = 0;      // host view used by the step adaptation
+    uint64_t accBase = 0, attBase = 0;
+    uint64_t boxSinceEval = 0, boxAccSinceEval = 0;
+
+    // timing
+    bool timing = false;
+    std::vector<hipEvent_t> evStart, evStop;
+    std::vector<int> evKind;
+    size_t evUsed = 0;
+    rp_timing tstats{};
+};
+
+namespace {
+
+int allocParticles(DeviceParticles &d, size_t n) {
+    size_t bytes = n * (12 * sizeof(double) + 2 * sizeof(int) + 1) + 4096;
+    RP_HIP(hipMalloc(&d.mem, bytes));
+    char *q = static_cast<char *>(d.mem);
+    auto take = [&](size_t b) {
+        char *r = q;
+        q += (b + 255) / 256 * 256;
+        return r;
+    };
+    d.p.x = reinterpret_cast<double *>(take(n * 8));
+    d.p.y = reinterpret_cast<double *>(take(n * 8));
+    d.p.z = reinterpret_cast<double *>(take(n * 8));
+    for (int c = 0; c < 9; c++) d.p.o[c] = reinterpret_cast<double *>(take(n * 8));
+    d.p.tag = reinterpret_cast<int *>(take(n * 4));
+    d.p.cell = reinterpret_cast<int *>(take(n * 4));
+    d.p.wbits = reinterpret_cast<uint8_t *>(take(n));
+    return RP_OK;
+}
+
+PairConsts makeConsts(const rp_shape_desc &s) {
+    PairConsts k{};
+    k.kind = s.kind;
+    if (s.kind == RP_SHAPE_SPHERE) {
+        k.contact2 = std::pow(2 * s.radius, 2);              // SphereTraits.cpp:64
+    } else {
+        k.contact2 = 4 * s.radius * s.radius;                // SpherocylinderTraits.cpp:35, 44
+        k.range2 = std::pow(2 * s.radius + s.length, 2);     // SpherocylinderTraits.cpp:37
+        k.halfLen = 0.5 * s.length;                          // SpherocylinderTraits.cpp:24-26 (0.5 * 1 * L)
+    }
+    return k;
+}
+
+double shapeRange(const rp_shape_desc &s) {
+    return s.kind == RP_SHAPE_SPHERE ? 2 * s.radius : 2 * s.radius + s.length;  // SphereTraits.h:34, SpherocylinderTraits.h:81
+}
+
+void fillTranslations(Grid &g) {
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            for (int k = 0; k < 3; k++) {
+                V3 t = mulv(g.dims, v3((double)i - 1, (double)j - 1., (double)k - 1.));
+                int idx = 3 * (i * 9 + j * 3 + k);
+                g.T[idx] = t.x;
+                g.T[idx + 1] = t.y;
+                g.T[idx + 2] = t.z;
+            }
+}
+
+int ensureCells(rp_ctx *c, int ncell) {
+    if (ncell <= c->ncellCap) return RP_OK;
+    if (c->cellStart) hipFree(c->cellStart);
+    if (c->cellCount) hipFree(c->cellCount);
+    c->cellStart = c->cellCount = nullptr;
+    int cap = ncell + ncell / 4 + 64;
+    RP_HIP(hipMalloc(&c->cellStart, sizeof(int) * (cap + 1)));
+    RP_HIP(hipMalloc(&c->cellCount, sizeof(int) * cap));
+    c->ncellCap = cap;
+    return RP_OK;
+}
+
+// Checkerboard grid: an even number of cells per axis, each at least the interaction range high and, as in the
+// reference's grid (Packing.cpp:862-866), at least a fifth of the mean interparticle spacing.
+int makeSweepGrid(rp_ctx *c, const HostBox &b, const double shiftUnit[3], Grid &g) {
+    double h[3];
+    b.heights(h);
+    std::memcpy(g.dims, b.dims, sizeof g.dims);
+    std::memcpy(g.inv, b.inv, sizeof g.inv);
+    double cellSize = std::max(c->range, std::cbrt(b.volume() / (double)c->n) / 5);
+    for (int d = 0; d < 3; d++) {
+        int n = (int)std::floor(h[d] / cellSize);
+        n -= n % 2;
+        if (n < 2)
+            return fail(RP_ERR_UNSUPPORTED, "box height " + std::to_string(h[d]) +
+                                            " is below two interaction ranges; the checkerboard needs two cells per axis");
+        g.n[d] = n;
+        g.relCell[d] = 1 / (double)n;
+        g.shift[d] = shiftUnit[d] * g.relCell[d];
+    }
+    g.ncell = g.n[0] * g.n[1] * g.n[2];
+    g.exact = 0;
+    fillTranslations(g);
+    int r = ensureCells(c, g.ncell);
+    if (r) return r;
+    g.cellStart = c->cellStart;
+    g.cellCount = c->cellCount;
+    return RP_OK;
+}
+
+// The reference's neighbour grid (Packing.cpp:858-883, NeighbourGrid.cpp:178-201).  Returns 1 when the
+// reference would run without a grid.
+int makeExactGrid(rp_ctx *c, const HostBox &b, Grid &g, int &disabled) {
+    double h[3];
+    b.heights(h);
+    double cell = c->range;
+    double minCell = std::cbrt(b.volume() / (double)c->n) / 5;
+    if (c->range < minCell) cell = minCell;
+    double hmax = std::max(h[0], std::max(h[1], h[2])), hmin = std::min(h[0], std::min(h[1], h[2]));
+    disabled = (cell * 4 > hmax || cell * (1 + 1e-12) > hmin) ? 1 : 0;
+    if (disabled) return RP_OK;
+    std::memcpy(g.dims, b.dims, sizeof g.dims);
+    std::memcpy(g.inv, b.inv, sizeof g.inv);
+    for (int d = 0; d < 3; d++) {
+        g.n[d] = (int)std::floor(h[d] / cell);
+        g.relCell[d] = 1 / (double)g.n[d];
+        g.shift[d] = 0;
+    }
+    g.ncell = g.n[0] * g.n[1] * g.n[2];
+    g.exact = 1;
+    fillTranslations(g);
+    int r = ensureCells(c, g.ncell);
+    if (r) return r;
+    g.cellStart = c->cellStart;
+    g.cellCount = c->cellCount;
+    return RP_OK;
+}
+
+void timingBegin(rp_ctx *c, int kind) {
+    if (!c->timing) return;
+    if (c->evUsed == c->evStart.size()) {
+        hipEvent_t a, b;
+        hipEventCreate(&a);
+        hipEventCreate(&b);
+        c->evStart.push_back(a);
+        c->evStop.push_back(b);
+        c->evKind.push_back(0);
+    }
+    c->evKind[c->evUsed] = kind;
+    hipEventRecord(c->evStart[c->evUsed], c->stream);
+}
+
+void timingEnd(rp_ctx *c) {
+    if (!c->timing) return;
+    hipEventRecord(c->evStop[c->evUsed], c->stream);
+    c->evUsed++;
+}
+
+int timingHarvest(rp_ctx *c) {
+    if (!c->timing || c->evUsed == 0) return RP_OK;
+    RP_HIP(hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i < c->evUsed; i++) {
+        float ms = 0;
+        RP_HIP(hipEventElapsedTime(&ms, c->evStart[i], c->evStop[i]));
+        if (c->evKind[i] == 0) { c->tstats.sweep_launches++; c->tstats.sweep_ms += ms; }
+        else if (c->evKind[i] == 1) { c->tstats.rebuild_launches++; c->tstats.rebuild_ms += ms; }
+        else { c->tstats.scan_launches++; c->tstats.scan_ms += ms; }
+    }
+    c->evUsed = 0;
+    return RP_OK;
+}
+
+int rebuild(rp_ctx *c, const Grid &g, int rescale, const double *oldInv) {
+    timingBegin(c, 1);
+    RP_HIP(launch_rebuild(c->stream, g, (int)c->n, c->cur.p, c->alt.p, c->tmpSlot, c->tmpOrder, c->tmpPos, rescale,
+                          oldInv));
+    timingEnd(c);
+    return RP_OK;
+}
+
+int readCounters(rp_ctx *c) {
+    RP_HIP(hipMemcpyAsync(c->hCounters, c->dCounters, sizeof(unsigned long long) * CNT_NUM, hipMemcpyDeviceToHost,
+                          c->stream));
+    RP_HIP(hipStreamSynchronize(c->stream));
+    return RP_OK;
+}
+
+// Early-exit overlap scan of the `alt` arrays sorted by `g`; sets found.
+int scanOverlaps(rp_ctx *c, const Grid &g, int earlyExit, uint64_t &count) {
+    RP_HIP(launch_fill_u64(c->stream, c->dCounters + CNT_OVERLAPS, 1, 0));
+    timingBegin(c, 2);
+    RP_HIP(launch_count_overlaps(c->stream, g, (int)c->n, c->alt.p, c->k, earlyExit, c->dCounters));
+    timingEnd(c);
+    int r = readCounters(c);
+    if (r) return r;
+    count = c->hCounters[CNT_OVERLAPS];
+    return RP_OK;
+}
+
+void swapBuffers(rp_ctx *c) { std::swap(c->cur, c->alt); }
+
+double minHeight(const HostBox &b) {
+    double h[3];
+    b.heights(h);
+    return std::min(h[0], std::min(h[1], h[2]));
+}
+
+// Simulation::tryScaling for a hard interaction (Simulation.cpp:493-531) + Packing::tryScaling
+// (Packing.cpp:172-208).  On acceptance the rescaled arrays, already sorted by the next cycle's grid, become
+// current.
+int boxMove(rp_ctx *c, bool &accepted) {
+    accepted = false;
+    HostBox nb;
+    double nd[9];
+    std::memcpy(nd, c->box.dims, sizeof nd);
+    if (c->boxKind == RP_BOX_DELTA_TRICLINIC) {
+        for (int i = 0; i < 9; i++) nd[i] += philox_uniform(c->hostRng, -c->boxStep, c->boxStep);
+    } else if (c->boxKind == RP_BOX_LOG_ISOTROPIC) {
+        double f = std::exp(philox_uniform(c->hostRng, -c->boxStep, c->boxStep));
+        for (int i = 0; i < 9; i++) nd[i] = nd[i] * f;
+    } else {
+        // DeltaVolumeScaler: factor = cbrt((dV + V) / V) with V the product of the heights
+        double h[3];
+        c->box.heights(h);
+        double curV = 1.;
+        curV = curV * h[0];
+        curV = curV * h[1];
+        curV = curV * h[2];
+        double dV = (2 * philox_unit(c->hostRng) - 1) * c->boxStep;
+        double f = std::cbrt((dV + curV) / curV);
+        for (int i = 0; i < 9; i++) nd[i] = nd[i] * f;
+    }
+    nb.set(nd);
+    double oldV = c->box.volume(), newV = nb.volume();
+    if (newV == 0) return fail(RP_ERR_PRECONDITION, "box move produced a zero-volume box");
+    double dV = newV - oldV, factor = newV / oldV;
+    double N = (double)c->n;
+    double exponent = N * std::log(factor) - c->pressure * dV / c->temperature;
+    double u = philox_unit(c->hostRng);
+    double shiftUnit[3] = {philox_unit(c->hostRng), philox_unit(c->hostRng), philox_unit(c->hostRng)};
+    if (u > std::exp(exponent)) return RP_OK;
+
+    Grid g{};
+    int r = makeSweepGrid(c, nb, shiftUnit, g);
+    if (r == RP_ERR_UNSUPPORTED) {
+        // the checkerboard needs two cells per axis; the proposal is rejected and counted
+        c->boxTooSmall++;
+        return RP_OK;
+    }
+    if (r) return r;
+    r = rebuild(c, g, 1, c->box.inv);
+    if (r) return r;
+    bool cannotOverlap = isBoxUpscaled(c->box, nb);  // spheres and spherocylinders are convex
+    if (!cannotOverlap) {
+        uint64_t found = 0;
+        c->boxScans++;
+        r = scanOverlaps(c, g, 1, found);
+        if (r) return r;
+        if (found) return RP_OK;
+    }
+    c->box = nb;
+    swapBuffers(c);
+    c->grid = g;
+    c->sortedForCycle = true;
+    accepted = true;
+    return RP_OK;
+}
+
+int prepareCycleGrid(rp_ctx *c) {
+    if (c->sortedForCycle) return RP_OK;
+    double shiftUnit[3] = {philox_unit(c->hostRng), philox_unit(c->hostRng), philox_unit(c->hostRng)};
+    Grid g{};
+    int r = makeSweepGrid(c, c->box, shiftUnit, g);
+    if (r) return r;
+    r = rebuild(c, g, 0, nullptr);
+    if (r) return r;
+    swapBuffers(c);
+    c->grid = g;
+    c->sortedForCycle = true;
+    return RP_OK;
+}
+
+// One cycle: 8 colour passes in random order (N trial moves in total), then one box move.
+int runCycle(rp_ctx *c) {
+    int r = prepareCycleGrid(c);
+    if (r) return r;
+    const Grid &g = c->grid;
+    SweepParams sp{};
+    sp.g = g;
+    sp.p = c->cur.p;
+    sp.k = c->k;
+    double maxT = c->maxTransImplicit ? minHeight(c->box) / 2 : c->maxTransStep;
+    sp.transStep = std::min(maxT, c->transStep);
+    sp.rotStep = std::min(c->rotStep, M_PI);
+    sp.seed = c->seed;
+    sp.cycle = (uint32_t)c->cycle;
+    for (int d = 0; d < 3; d++) sp.na[d] = g.n[d] / 2;
+    double lambda = (double)c->n / (double)g.ncell;
+    double expect = 27 * lambda;
+    int cap = (int)std::ceil(expect + 8 * std::sqrt(expect) + 64);
+    cap = (cap + 63) / 64 * 64;
+    const int maxCap = (int)(60 * 1024 / sweep_lds_bytes(c->k.kind, 1)) / 64 * 64;
+    sp.cap = std::min(cap, maxCap);
+    sp.counters = c->dCounters;
+    int order[8] = {0, 1, 2, 3, 4, 5, 6, 7};
+    for (int i = 7; i > 0; i--) {
+        int j = (int)philox_below(c->hostRng, (uint32_t)(i + 1));
+        std::swap(order[i], order[j]);
+    }
+    for (int i = 0; i < 8; i++) {
+        sp.colour = order[i];
+        timingBegin(c, 0);
+        RP_HIP(launch_sweep(c->stream, sp));
+        timingEnd(c);
+    }
+    c->sortedForCycle = false;
+    if (c->boxKind != RP_BOX_NONE) {
+        bool acc = false;
+        r = boxMove(c, acc);
+        if (r) return r;
+        c->boxAttempted++;
+        c->boxSinceEval++;
+        if (acc) {
+            c->boxAccepted++;
+            c->boxAccSinceEval++;
+        }
+    }
+    c->cycle++;
+    c->cycles++;
+    return RP_OK;
+}
+
+// Simulation::evaluateCounters during thermalisation (Simulation.cpp:533-607) with
+// RototranslationSampler::increase/decreaseStepSize (RototranslationSampler.cpp:51-62, 87-93) and
+// TriclinicBoxScaler (TriclinicBoxScaler.cpp:13-21).
+int adaptSteps(rp_ctx *c) {
+    if (c->cycles % 100 == 0) {
+        int r = readCounters(c);
+        if (r) return r;
+        uint64_t acc = c->hCounters[CNT_ACCEPTED] - c->accBase;
+        uint64_t att = c->hCounters[CNT_ATTEMPTED] - c->attBase;
+        c->accBase = c->hCounters[CNT_ACCEPTED];
+        c->attBase = c->hCounters[CNT_ATTEMPTED];
+        if (att > 0) {
+            double rate = (double)acc / (double)att;
+            double maxT = c->maxTransImplicit ? minHeight(c->box) / 2 : c->maxTransStep;
+            if (rate > 0.2) {
+                double old = c->transStep;
+                if (!(maxT > 0 && c->transStep >= maxT)) c->transStep *= 1.1;
+                c->rotStep *= c->transStep / old;
+            } else if (rate < 0.1) {
+                c->transStep /= 1.1;
+                c->rotStep /= 1.1;
+            }
+        }
+    }
+    if (c->boxKind != RP_BOX_NONE && c->boxSinceEval >= 100) {
+        double rate = (double)c->boxAccSinceEval / (double)c->boxSinceEval;
+        c->boxSinceEval = c->boxAccSinceEval = 0;
+        if (rate > 0.2) c->boxStep *= 1.1;
+        else if (rate < 0.1) c->boxStep /= 1.1;
+    }
+    return RP_OK;
+}
+
+int countExact(rp_ctx *c, int earlyExit, uint64_t &out) {
+    c->sortedForCycle = false;  // the shared cell arrays are about to describe another grid
+    Grid g{};
+    int disabled = 0;
+    int r = makeExactGrid(c, c->box, g, disabled);
+    if (r) return r;
+    if (disabled)
+        return fail(RP_ERR_UNSUPPORTED, "the reference would count without a neighbour grid for this box; "
+                                        "the all-pairs path is not implemented on the device");
+    r = rebuild(c, g, 0, nullptr);
+    if (r) return r;
+    r = scanOverlaps(c, g, earlyExit, out);
+    if (r) return r;
+    if (earlyExit && out) out = 1;
+    return RP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *rp_last_error(void) { return g_lastError.c_str(); }
+
+int rp_create(rp_ctx **out, const rp_shape_desc *shape, const double box[9], size_t n, const double *pos,
+              const double *rot, uint64_t seed, int precision) {
+    if (!out || !shape || !box || !pos || !rot) return fail(RP_ERR_PRECONDITION, "null argument");
+    *out = nullptr;
+    if (n == 0) return fail(RP_ERR_PRECONDITION, "empty packing (Packing.cpp:69)");
+    if (n > (size_t)0x3fffffff) return fail(RP_ERR_PRECONDITION, "too many particles");
+    if (precision != RP_PREC_FP64) return fail(RP_ERR_UNSUPPORTED, "unknown precision");
+    if (shape->kind != RP_SHAPE_SPHERE && shape->kind != RP_SHAPE_SPHEROCYLINDER)
+        return fail(RP_ERR_UNSUPPORTED, "unsupported shape kind");
+    if (!(shape->radius > 0)) return fail(RP_ERR_PRECONDITION, "radius must be > 0");
+    if (shape->kind == RP_SHAPE_SPHEROCYLINDER && !(shape->length >= 0))
+        return fail(RP_ERR_PRECONDITION, "length must be >= 0");
+    HostBox hb;
+    hb.set(box);
+    if (hb.volume() == 0) return fail(RP_ERR_PRECONDITION, "box volume is 0 (Packing.cpp:68)");
+    for (size_t i = 0; i < n; i++) {
+        V3 rel = mulv(hb.inv, v3(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]));
+        if (rel.x < 0 || rel.x >= 1 || rel.y < 0 || rel.y >= 1 || rel.z < 0 || rel.z >= 1)
+            return fail(RP_ERR_PRECONDITION, "particle " + std::to_string(i) + " lies outside the box (Packing.cpp:71)");
+    }
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return fail(RP_ERR_DEVICE, "no HIP device");
+
+    rp_ctx *c = new rp_ctx();
+    c->shape = *shape;
+    c->k = makeConsts(*shape);
+    c->range = shapeRange(*shape);
+    c->n = n;
+    c->box = hb;
+    c->seed = seed;
+    c->hostRng = philox_init(seed, 0xA5A5A5A5u, 0x0B0Bu, 0x1u);
+    auto cleanup = [&](int code) {
+        rp_destroy(c);
+        return code;
+    };
+    if (hipStreamCreate(&c->stream) != hipSuccess) return cleanup(fail(RP_ERR_DEVICE, "hipStreamCreate failed"));
+    int r = allocParticles(c->cur, n);
+    if (r) return cleanup(r);
+    r = allocParticles(c->alt, n);
+    if (r) return cleanup(r);
+    if (hipMalloc(&c->tmpSlot, sizeof(int) * 2 * n) != hipSuccess ||
+        hipMalloc(&c->tmpOrder, sizeof(int) * n + n + 256) != hipSuccess ||
+        hipMalloc(&c->tmpPos, sizeof(double) * 3 * n) != hipSuccess ||
+        hipMalloc(&c->dCounters, sizeof(unsigned long long) * CNT_NUM) != hipSuccess ||
+        hipHostMalloc(&c->hCounters, sizeof(unsigned long long) * CNT_NUM, hipHostMallocDefault) != hipSuccess)
+        return cleanup(fail(RP_ERR_DEVICE, "device allocation failed"));
+
+    // upload as SoA in the caller's order
+    std::vector<double> col(n);
+    std::vector<int> tags(n);
+    auto up = [&](double *dst, const double *src, int stride, int off) -> int {
+        for (size_t i = 0; i < n; i++) col[i] = src[stride * i + off];
+        RP_HIP(hipMemcpy(dst, col.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+        return RP_OK;
+    };
+    if ((r = up(c->cur.p.x, pos, 3, 0)) || (r = up(c->cur.p.y, pos, 3, 1)) || (r = up(c->cur.p.z, pos, 3, 2)))
+        return cleanup(r);
+    for (int k = 0; k < 9; k++)
+        if ((r = up(c->cur.p.o[k], rot, 9, k))) return cleanup(r);
+    for (size_t i = 0; i < n; i++) tags[i] = (int)i;
+    if (hipMemcpy(c->cur.p.tag, tags.data(), sizeof(int) * n, hipMemcpyHostToDevice) != hipSuccess)
+        return cleanup(fail(RP_ERR_DEVICE, "upload failed"));
+    if (launch_fill_u64(c->stream, c->dCounters, CNT_NUM, 0) != hipSuccess)
+        return cleanup(fail(RP_ERR_DEVICE, "counter init failed"));
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return cleanup(fail(RP_ERR_DEVICE, "sync failed"));
+    *out = c;
+    return RP_OK;
+}
+
+int rp_set_environment(rp_ctx *c, double temperature, double pressure, const rp_move_desc *move,
+                       const rp_box_desc *box) {
+    if (!c || !move) return fail(RP_ERR_PRECONDITION, "null argument");
+    if (!(temperature > 0)) return fail(RP_ERR_PRECONDITION, "temperature must be > 0");
+    if (!(move->trans_step > 0)) return fail(RP_ERR_PRECONDITION, "translation step must be > 0");
+    if (move->max_trans_step < 0 || (move->max_trans_step > 0 && move->trans_step > move->max_trans_step))
+        return fail(RP_ERR_PRECONDITION, "invalid maximal translation step (RototranslationSampler.cpp:14-16)");
+    c->temperature = temperature;
+    c->pressure = pressure;
+    c->transStep = move->trans_step;
+    c->maxTransStep = move->max_trans_step;
+    c->maxTransImplicit = move->max_trans_step == 0;
+    if (move->rot_step > 0) {
+        c->rotStep = move->rot_step;
+    } else {
+        // RototranslationSampler::setupForShapeTraits (RototranslationSampler.cpp:95-114)
+        double ratio = c->transStep / c->range;
+        c->rotStep = 0.133183 - 1.18634 * ratio + 264.37 * ratio * ratio;
+    }
+    c->boxKind = box ? box->kind : RP_BOX_NONE;
+    if (c->boxKind != RP_BOX_NONE && c->boxKind != RP_BOX_DELTA_TRICLINIC && c->boxKind != RP_BOX_LOG_ISOTROPIC &&
+        c->boxKind != RP_BOX_DELTA_V)
+        return fail(RP_ERR_UNSUPPORTED, "unknown box move");
+    if (c->boxKind != RP_BOX_NONE) {
+        if (!(box->step > 0)) return fail(RP_ERR_PRECONDITION, "box step must be > 0 (TriclinicBoxScaler.cpp:9-11)");
+        c->boxStep = box->step;
+    }
+    c->envSet = true;
+    return RP_OK;
+}
+
+int rp_run_cycles(rp_ctx *c, size_t nCycles, int adapt, rp_counters *out) {
+    if (!c) return fail(RP_ERR_PRECONDITION, "null context");
+    if (!c->envSet) return fail(RP_ERR_PRECONDITION, "environment not set");
+    uint64_t overl = 0;
+    int r = countExact(c, 1, overl);
+    if (r == RP_ERR_UNSUPPORTED) {
+        // the reference's grid is off for this box; check on the checkerboard grid instead
+        double zero[3] = {0, 0, 0};
+        c->sortedForCycle = false;
+        Grid g{};
+        r = makeSweepGrid(c, c->box, zero, g);
+        if (r) return r;
+        r = rebuild(c, g, 0, nullptr);
+        if (r) return r;
+        r = scanOverlaps(c, g, 1, overl);
+    }
+    if (r) return r;
+    if (overl)
+        return fail(RP_ERR_VALIDATION,
+                    "Overlaps are present at the start of integration. Perform overlap reduction beforehand.");
+    for (size_t i = 0; i < nCycles; i++) {
+        r = runCycle(c);
+        if (r) return r;
+        if (adapt) {
+            r = adaptSteps(c);
+            if (r) return r;
+        }
+        if (c->timing) {
+            r = timingHarvest(c);
+            if (r) return r;
+        }
+    }
+    r = readCounters(c);
+    if (r) return r;
+    if (out) {
+        out->cycles = c->cycles;
+        out->moves_attempted = c->hCounters[CNT_ATTEMPTED];
+        out->moves_accepted = c->hCounters[CNT_ACCEPTED];
+        out->moves_left_cell = c->hCounters[CNT_LEFT_CELL];
+        out->box_attempted = c->boxAttempted;
+        out->box_accepted = c->boxAccepted;
+        out->box_scans = c->boxScans;
+        out->box_too_small = c->boxTooSmall;
+        out->skipped_blocks = c->hCounters[CNT_SKIPPED_BLOCKS];
+        out->pair_tests = c->hCounters[CNT_TESTS];
+        double maxT = c->maxTransImplicit ? minHeight(c->box) / 2 : c->maxTransStep;
+        out->trans_step = std::min(maxT, c->transStep);
+        out->rot_step = c->rotStep;
+        out->box_step = c->boxStep;
+    }
+    return RP_OK;
+}
+
+int rp_count_total_overlaps(rp_ctx *c, int earlyExit, uint64_t *out) {
+    if (!c || !out) return fail(RP_ERR_PRECONDITION, "null argument");
+    return countExact(c, earlyExit, *out);
+}
+
+int rp_download(rp_ctx *c, double box[9], double *pos, double *rot) {
+    if (!c) return fail(RP_ERR_PRECONDITION, "null context");
+    RP_HIP(hipStreamSynchronize(c->stream));
+    if (box) std::memcpy(box, c->box.dims, sizeof c->box.dims);
+    size_t n = c->n;
+    std::vector<int> tags(n);
+    std::vector<double> col(n);
+    RP_HIP(hipMemcpy(tags.data(), c->cur.p.tag, sizeof(int) * n, hipMemcpyDeviceToHost));
+    auto down = [&](const double *src, double *dst, int stride, int off) -> int {
+        RP_HIP(hipMemcpy(col.data(), src, sizeof(double) * n, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; i++) dst[stride * (size_t)tags[i] + off] = col[i];
+        return RP_OK;
+    };
+    int r;
+    if (pos && ((r = down(c->cur.p.x, pos, 3, 0)) || (r = down(c->cur.p.y, pos, 3, 1)) ||
+                (r = down(c->cur.p.z, pos, 3, 2))))
+        return r;
+    if (rot)
+        for (int k = 0; k < 9; k++)
+            if ((r = down(c->cur.p.o[k], rot, 9, k))) return r;
+    return RP_OK;
+}
+
+int rp_set_timing(rp_ctx *c, int enable) {
+    if (!c) return fail(RP_ERR_PRECONDITION, "null context");
+    int r = timingHarvest(c);
+    if (r) return r;
+    c->timing = enable != 0;
+    c->tstats = rp_timing{};
+    return RP_OK;
+}
+
+int rp_get_timing(rp_ctx *c, rp_timing *out) {
+    if (!c || !out) return fail(RP_ERR_PRECONDITION, "null argument");
+    int r = timingHarvest(c);
+    if (r) return r;
+    *out = c->tstats;
+    return RP_OK;
+}
+
+void rp_destroy(rp_ctx *c) {
+    if (!c) return;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (auto e : c->evStart) hipEventDestroy(e);
+    for (auto e : c->evStop) hipEventDestroy(e);
+    if (c->cur.mem) hipFree(c->cur.mem);
+    if (c->alt.mem) hipFree(c->alt.mem);
+    if (c->cellStart) hipFree(c->cellStart);
+    if (c->cellCount) hipFree(c->cellCount);
+    if (c->tmpSlot) hipFree(c->tmpSlot);
+    if (c->tmpOrder) hipFree(c->tmpOrder);
+    if (c->tmpPos) hipFree(c->tmpPos);
+    if (c->dCounters) hipFree(c->dCounters);
+    if (c->hCounters) hipHostFree(c->hCounters);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int rp_pair_decisions(const rp_shape_desc *shape, size_t n, const double *records, uint8_t *decisions) {
+    if (!shape || (n && (!records || !decisions))) return fail(RP_ERR_PRECONDITION, "null argument");
+    if (shape->kind != RP_SHAPE_SPHERE && shape->kind != RP_SHAPE_SPHEROCYLINDER)
+        return fail(RP_ERR_UNSUPPORTED, "unsupported shape kind");
+    if (n == 0) return RP_OK;
+    PairConsts k = makeConsts(*shape);
+    double *dRec = nullptr;
+    uint8_t *dOut = nullptr;
+    RP_HIP(hipMalloc(&dRec, sizeof(double) * 27 * n));
+    if (hipMalloc(&dOut, n) != hipSuccess) {
+        hipFree(dRec);
+        return fail(RP_ERR_DEVICE, "hipMalloc failed");
+    }
+    int r = RP_OK;
+    hipError_t e = hipMemcpy(dRec, records, sizeof(double) * 27 * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_pair_decisions(nullptr, k, n, dRec, dOut);
+    if (e == hipSuccess) e = hipMemcpy(decisions, dOut, n, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) r = fail(RP_ERR_DEVICE, std::string("pair decisions: ") + hipGetErrorString(e));
+    hipFree(dRec);
+    hipFree(dOut);
+    return r;
+}
+
+}  // extern "C"

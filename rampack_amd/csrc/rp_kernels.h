@@ -1,0 +1,75 @@
+// rp_kernels.h -- device data layout and kernel launch wrappers (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include "rp_math.h"
+
+namespace rp {
+
+// Particle state in HBM, structure of arrays, sorted by cell (one sort per rebuild).  x,y,z are the FP64
+// accumulated positions, o[0..8] the row-major FP64 orientation matrix, tag the caller's particle index,
+// wbits the three wrap bits of the position relative to the current grid origin, cell the cell index.
+struct Particles {
+    double *x, *y, *z;
+    double *o[9];
+    int *tag;
+    uint8_t *wbits;
+    int *cell;
+};
+
+// One cell grid over the (triclinic) box in relative coordinates.  exact = 1 reproduces the reference's
+// NeighbourGrid binning (NeighbourGrid.cpp:13-34, no origin shift); exact = 0 is the checkerboard grid, with
+// an origin shift in [0, 1/n) per axis and an even number of cells per axis.
+struct Grid {
+    double dims[9], inv[9];
+    double T[81];           // translations dims * (i-1, j-1, k-1), index 9i + 3j + k (NeighbourGrid.cpp:203-217)
+    double relCell[3];      // 1 / n
+    double shift[3];
+    int n[3];
+    int ncell;
+    int exact;
+    int *cellStart;         // ncell + 1
+    int *cellCount;         // ncell
+};
+
+struct SweepParams {
+    Grid g;
+    Particles p;
+    PairConsts k;
+    double transStep, rotStep;
+    uint64_t seed;
+    uint32_t cycle;
+    int colour;
+    int na[3];              // active cells per axis (n / 2)
+    int cap;                // stencil entries that fit in LDS
+    unsigned long long *counters;
+};
+
+enum Counter : int {
+    CNT_ACCEPTED = 0,
+    CNT_ATTEMPTED = 1,
+    CNT_LEFT_CELL = 2,
+    CNT_SKIPPED_BLOCKS = 3,
+    CNT_OVERLAPS = 4,
+    CNT_MAX_STENCIL = 5,
+    CNT_TESTS = 6,
+    CNT_NUM = 8
+};
+
+// Launch wrappers (rp_kernels.hip).  All enqueue on `s` and return the launch status.
+hipError_t launch_pair_decisions(hipStream_t s, PairConsts k, size_t n, const double *rec, uint8_t *out);
+hipError_t launch_fill_u64(hipStream_t s, unsigned long long *p, size_t n, unsigned long long v);
+hipError_t launch_fill_i32(hipStream_t s, int *p, size_t n, int v);
+// Cell-list rebuild: src -> dst sorted by cell of `g`.  When rescale != 0 the positions are first mapped
+// newDims * (oldInv * pos) (Packing.cpp:183-184) with newDims = g.dims.
+hipError_t launch_rebuild(hipStream_t s, const Grid &g, int n, Particles src, Particles dst, int *tmpSlot,
+                          int *tmpOrder, double *tmpPos, int rescale, const double *oldInv);
+// Half-shell overlap count over the grid (Packing.cpp:431-559).  Adds to counters[CNT_OVERLAPS]; with
+// earlyExit the threads stop once any overlap is found.
+hipError_t launch_count_overlaps(hipStream_t s, const Grid &g, int n, Particles p, PairConsts k, int earlyExit,
+                                 unsigned long long *counters);
+hipError_t launch_sweep(hipStream_t s, const SweepParams &sp);
+size_t sweep_lds_bytes(int kind, int cap);
+
+}  // namespace rp

@@ -1,0 +1,151 @@
+"""ctypes access to the test-only checkers: oracle/lib/liboracle.so (the CPU restatement) and, where it was built,
+oracle/_ref/libref.so (the reference's own hot path compiled from its sources)."""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_PATH = os.path.join(ROOT, "oracle", "lib", "liboracle.so")
+REF_PATH = os.path.join(ROOT, "oracle", "_ref", "libref.so")
+REF_PERF_PATH = os.path.join(ROOT, "oracle", "_ref", "libref_perf.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+
+
+def _d(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64).ctypes.data_as(_dp)
+
+
+def _setup_common(lib):
+    lib.orc_pair_decisions.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_size_t, _dp, _u8p]
+    lib.orc_snapshot_stats.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp, _dp,
+                                       ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, ctypes.c_size_t, _dp, _dp,
+                                       ctypes.c_int, _u64p, _dp]
+    lib.orc_count_brute.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp, _dp, _dp,
+                                    ctypes.c_size_t, _dp, _dp]
+    lib.orc_count_brute.restype = ctypes.c_uint64
+    lib.orc_segment_distance2.argtypes = [ctypes.c_size_t, _dp, _dp]
+    lib.orc_segment_distance2.restype = None
+    lib.orc_box_info.argtypes = [_dp, _dp, _dp, _dp]
+    lib.orc_box_info.restype = None
+    lib.orc_pbc.argtypes = [_dp, _dp, _dp, _dp, _dp, _dp]
+    lib.orc_pbc.restype = None
+    lib.orc_is_box_upscaled.argtypes = [_dp, _dp]
+    lib.orc_translate_wrapped.argtypes = [_dp, ctypes.c_size_t, _dp, _dp, _dp]
+    lib.orc_translate_wrapped.restype = None
+    lib.orc_rescale.argtypes = [_dp, _dp, ctypes.c_size_t, _dp, _dp]
+    lib.orc_rescale.restype = None
+    lib.orc_rotation.argtypes = [_dp, ctypes.c_double, _dp]
+    lib.orc_rotation.restype = None
+    lib.orc_nematic_p2.argtypes = [ctypes.c_size_t, _dp]
+    lib.orc_nematic_p2.restype = ctypes.c_double
+
+
+class Checker:
+    """Common interface of the restatement (prefix orc_) and the reference build (prefix ref_)."""
+
+    def __init__(self, lib, prefix):
+        self.lib = lib
+        self.prefix = prefix
+
+    def pair_decisions(self, kind: int, radius: float, length: float, rec: np.ndarray) -> np.ndarray:
+        rec = np.ascontiguousarray(rec, dtype=np.float64).reshape(-1, 27)
+        out = np.zeros(len(rec), dtype=np.uint8)
+        f = getattr(self.lib, self.prefix + "pair_decisions")
+        f(kind, radius, length, len(rec), rec.ctypes.data_as(_dp), out.ctypes.data_as(_u8p))
+        return out
+
+    def snapshot_stats(self, kind, radius, length, box, pos, rot, early_exit=False, soft=0, eps=1.0, sigma=1.0,
+                       centres=None, radii=None):
+        box = np.ascontiguousarray(box, dtype=np.float64).reshape(9)
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        rot = np.ascontiguousarray(rot, dtype=np.float64)
+        nc = 0 if centres is None else len(centres)
+        cen = np.zeros(3) if centres is None else np.ascontiguousarray(centres, dtype=np.float64)
+        rad = np.zeros(1) if radii is None else np.ascontiguousarray(radii, dtype=np.float64)
+        ov = ctypes.c_uint64()
+        en = ctypes.c_double()
+        f = getattr(self.lib, self.prefix + "snapshot_stats")
+        rc = f(kind, radius, length, nc, cen.ctypes.data_as(_dp), rad.ctypes.data_as(_dp), soft, eps, sigma,
+               box.ctypes.data_as(_dp), len(pos), pos.ctypes.data_as(_dp), rot.ctypes.data_as(_dp),
+               1 if early_exit else 0, ctypes.byref(ov), ctypes.byref(en))
+        if rc != 0:
+            raise RuntimeError(f"{self.prefix}snapshot_stats failed ({rc})")
+        return int(ov.value), float(en.value)
+
+
+_oracle = None
+_ref = None
+
+
+def oracle() -> Checker:
+    global _oracle
+    if _oracle is None:
+        if not os.path.exists(ORACLE_PATH):
+            raise RuntimeError(f"{ORACLE_PATH} missing: run __graft_entry__.build()")
+        lib = ctypes.CDLL(ORACLE_PATH)
+        _setup_common(lib)
+        _oracle = Checker(lib, "orc_")
+    return _oracle
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_PATH)
+
+
+def _setup_ref(lib):
+    lib.ref_pair_decisions.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_size_t, _dp, _u8p]
+    lib.ref_snapshot_stats.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp, _dp,
+                                       ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, ctypes.c_size_t, _dp, _dp,
+                                       ctypes.c_int, _u64p, _dp]
+    lib.ref_nematic_p2.argtypes = [ctypes.c_size_t, _dp]
+    lib.ref_nematic_p2.restype = ctypes.c_double
+    lib.ref_mc_run.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, ctypes.c_size_t, _dp, _dp,
+                               ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                               ctypes.c_double, ctypes.c_ulong, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                               ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t, _dp, _dp, _dp, _dp, _dp, _dp]
+    lib.ref_mc_run.restype = ctypes.c_long
+
+
+def ref(perf: bool = False) -> Checker:
+    global _ref
+    path = REF_PERF_PATH if perf else REF_PATH
+    if perf:
+        lib = ctypes.CDLL(path, mode=os.RTLD_LAZY)
+        _setup_ref(lib)
+        return Checker(lib, "ref_")
+    if _ref is None:
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} missing (built only where the reference sources are present)")
+        lib = ctypes.CDLL(path, mode=os.RTLD_LAZY)
+        _setup_ref(lib)
+        _ref = Checker(lib, "ref_")
+    return _ref
+
+
+def ref_mc_run(checker: Checker, kind, radius, length, box, pos, rot, temperature, pressure, trans_step, rot_step,
+               box_kind, box_step, seed, dd=(1, 1, 1), threads=1, therm=0, avg=0, avg_every=1):
+    """Runs the reference hot path with the restated cycle loop; returns dict(samples, box, pos, rot, seconds, steps)."""
+    n = len(pos)
+    nsamp = (avg // avg_every) if avg_every else 0
+    samples = np.zeros(2 * max(nsamp, 1))
+    obox = np.zeros(9); opos = np.zeros((n, 3)); orot = np.zeros((n, 9))
+    secs = ctypes.c_double()
+    steps = np.zeros(3)
+    box = np.ascontiguousarray(box, dtype=np.float64).reshape(9)
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    rot = np.ascontiguousarray(rot, dtype=np.float64)
+    r = checker.lib.ref_mc_run(kind, radius, length, box.ctypes.data_as(_dp), n, pos.ctypes.data_as(_dp),
+                               rot.ctypes.data_as(_dp), temperature, pressure, trans_step, rot_step, box_kind,
+                               box_step, seed, dd[0], dd[1], dd[2], threads, therm, avg, avg_every,
+                               samples.ctypes.data_as(_dp), obox.ctypes.data_as(_dp), opos.ctypes.data_as(_dp),
+                               orot.ctypes.data_as(_dp), ctypes.byref(secs), steps.ctypes.data_as(_dp))
+    if r < 0:
+        raise RuntimeError(f"ref_mc_run failed ({r})")
+    return dict(samples=samples[:2 * r].reshape(-1, 2), box=obox.reshape(3, 3), pos=opos, rot=orot,
+                seconds=secs.value, steps=steps)

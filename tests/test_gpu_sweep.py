@@ -1,0 +1,144 @@
+"""GPU checkerboard sweep: invariants (no overlaps, N trial moves per cycle, determinism), error paths, and the
+reference's statistical validation cases (SimulationTest.cpp:49-101) against Carnahan-Starling."""
+import numpy as np
+import pytest
+
+import corpus
+import oracle_lib
+import rampack_amd as ra
+from rampack_amd import lattice
+
+pytestmark = pytest.mark.gpu
+
+SC = ra.Shape("spherocylinder", radius=0.5, length=5.0)
+HS = ra.Shape("sphere", radius=0.5)
+
+
+def overlaps_by_oracle(shape, box, pos, rot):
+    kind = 0 if shape.kind == "sphere" else 1
+    count, _ = oracle_lib.oracle().snapshot_stats(kind, shape.radius, shape.length, box, pos, rot)
+    return count
+
+
+def rod_start(n_side=(12, 12, 3)):
+    return lattice.spherocylinder_lattice(n_side=n_side, spacing=(1.45, 1.45, 6.3))
+
+
+def test_sweep_keeps_packing_overlap_free_and_moves_every_particle():
+    box, pos, rot = rod_start()
+    pk = ra.GpuPacking(SC, box, pos, rot, seed=11)
+    pk.set_environment(1.0, 2.0, 0.3, None, box_move=("delta_triclinic", 0.02))
+    c = pk.run_cycles(30, adapt_steps=True)
+    print(c, flush=True)
+    assert c["moves_attempted"] == 30 * len(pos)
+    assert c["skipped_blocks"] == 0
+    assert 0 < c["moves_accepted"] < c["moves_attempted"]
+    b, p, r = pk.download()
+    assert overlaps_by_oracle(SC, b, p, r) == 0
+    assert pk.count_total_overlaps() == 0
+    # positions stay in the box, orientations stay orthonormal
+    rel = np.linalg.solve(b, p.T).T
+    assert np.all(rel >= -1e-12) and np.all(rel < 1 + 1e-12)
+    o = r.reshape(-1, 3, 3)
+    np.testing.assert_allclose(np.einsum("nij,nkj->nik", o, o), np.broadcast_to(np.eye(3), o.shape), atol=1e-10)
+
+
+def test_sweep_moves_particles_across_cells_and_box_changes():
+    box, pos, rot = rod_start()
+    pk = ra.GpuPacking(SC, box, pos, rot, seed=5)
+    pk.set_environment(1.0, 1.5, 0.4, None, box_move=("delta_triclinic", 0.05))
+    pk.run_cycles(60, adapt_steps=True)
+    b, p, r = pk.download()
+    assert not np.allclose(b, box)
+    disp = p - pos
+    assert np.max(np.abs(disp)) > 1.0
+
+
+def test_same_seed_gives_identical_trajectories_and_other_seed_differs():
+    box, pos, rot = rod_start()
+    out = []
+    for seed in (3, 3, 4):
+        pk = ra.GpuPacking(SC, box, pos, rot, seed=seed)
+        pk.set_environment(1.0, 2.0, 0.3, None, box_move=("delta_triclinic", 0.02))
+        pk.run_cycles(15)
+        out.append(pk.download())
+        pk.close()
+    assert all(np.array_equal(a, b) for a, b in zip(out[0], out[1]))
+    assert not np.array_equal(out[0][1], out[2][1])
+
+
+def test_hard_spheres_overlap_free_log_moves():
+    box, pos, rot = lattice.hard_sphere_lattice(10, 0.30)
+    pk = ra.GpuPacking(HS, box, pos, rot, seed=1234)
+    pk.set_environment(1.0, 5.0, 0.1, None, box_move=("log", 0.01))
+    c = pk.run_cycles(200, adapt_steps=True)
+    assert c["moves_attempted"] == 200 * 1000
+    b, p, r = pk.download()
+    assert overlaps_by_oracle(HS, b, p, r) == 0
+
+
+def test_overlapping_start_is_a_validation_error():
+    box = np.diag([20.0, 20.0, 20.0])
+    pos, rot = corpus.random_snapshot(300, 1, box)
+    pk = ra.GpuPacking(SC, box, pos, rot)
+    pk.set_environment(1.0, 1.0, 0.1)
+    with pytest.raises(ra.ValidationError):
+        pk.run_cycles(1)
+
+
+def test_particle_outside_box_is_a_precondition_error():
+    box, pos, rot = rod_start()
+    pos = pos.copy()
+    pos[3, 0] = -0.1
+    with pytest.raises(ra.PreconditionError):
+        ra.GpuPacking(SC, box, pos, rot)
+
+
+def test_environment_preconditions():
+    box, pos, rot = rod_start()
+    pk = ra.GpuPacking(SC, box, pos, rot)
+    with pytest.raises(ra.PreconditionError):
+        pk.run_cycles(1)
+    with pytest.raises(ra.PreconditionError):
+        pk.set_environment(1.0, 1.0, -0.1)
+    with pytest.raises(ra.PreconditionError):
+        pk.set_environment(1.0, 1.0, 0.1, box_move=("delta_triclinic", 0.0))
+
+
+def naive_mean_error(samples):
+    """Quantity::calculateFromSamples (Quantity.cpp:45-62): mean and standard error of the mean."""
+    s = np.asarray(samples, dtype=float)
+    return s.mean(), s.std(ddof=1) / np.sqrt(len(s))
+
+
+def run_density(shape, box, pos, rot, temperature, pressure, trans, box_move, therm, avg, every, seed):
+    pk = ra.GpuPacking(shape, box, pos, rot, seed=seed)
+    pk.set_environment(temperature, pressure, trans, None, box_move=box_move)
+    pk.run_cycles(therm, adapt_steps=True)
+    samples = []
+    for _ in range(avg // every):
+        pk.run_cycles(every)
+        b, _, _ = pk.download()
+        samples.append(len(pos) / abs(np.linalg.det(b)))
+    return naive_mean_error(samples)
+
+
+def test_dilute_hard_sphere_gas_carnahan_starling():
+    """SimulationTest.cpp:49-75: 50 spheres of radius 0.05, T = 10, p = 1, delta_v scaling; CS density 0.0999791."""
+    dims = (5000.0 ** (1 / 3),) * 3
+    box, pos, rot = lattice.orthorhombic_arrange(50, dims)
+    rho, err = run_density(ra.Shape("sphere", radius=0.05), box, pos, rot, 10.0, 1.0, 1.0, ("delta_v", 1.0),
+                           5000, 10000, 100, 1234)
+    print(f"dilute HS: {rho} +- {err} (CS 0.0999791)", flush=True)
+    assert abs(rho - 0.0999791) <= 3 * err
+    assert err / rho < 0.03
+
+
+def test_degenerate_hard_sphere_gas_carnahan_starling():
+    """SimulationTest.cpp:77-101: 50 spheres of radius 0.5, T = 1, p = 1; CS density 0.398574."""
+    dims = (200.0 ** (1 / 3),) * 3
+    box, pos, rot = lattice.orthorhombic_arrange(50, dims)
+    rho, err = run_density(HS, box, pos, rot, 1.0, 1.0, 1.0, ("delta_v", 1.0), 5000, 10000, 100, 1234)
+    print(f"degenerate HS: {rho} +- {err} (CS 0.398574)", flush=True)
+    assert abs(rho - 0.398574) <= 3 * err
+    assert err / rho < 0.03
