@@ -1,0 +1,184 @@
+"""Benchmark: MC trial moves/s of the GPU sweep on N = 1e5 hard spherocylinders (L/D = 5) at p* = 2.0.
+
+One step = one MC cycle of rampack's Simulation::performCycle (src/core/Simulation.cpp:325-363): N trial
+rototranslations (eight checkerboard colour passes) plus one delta_triclinic box move, the same unit the reference
+reports as "Cycles per second" (src/frontend/modes/CasinoMode.cpp:405-412).  Trial moves/s = N * cycles / time.
+
+Start state: AA-stacked layers of parallel rods (packing fraction 0.42) equilibrated by the untimed warm-up cycles at
+p* = 2 with step adaptation; the timed cycles run without adaptation.  Inputs are resident in HBM before timing.
+
+Roofline: the dominant kernel is the sweep (k_sweep).  Its algorithmic traffic per trial move is
+(27 + a) * 96 B (the reference's 27-cell stencil of FP64 particle records plus the write-back of accepted moves, a =
+acceptance rate); achieved GB/s = that figure times the moves of one launch divided by the launch's average duration,
+measured with HIP events on the launch stream.
+
+CPU baseline: the reference's own hot path (oracle/_ref/libref_perf.so: Packing, NeighbourGrid,
+SpherocylinderTraits, RototranslationSampler, TriclinicDeltaScaler and DomainDecomposition compiled from the
+reference sources with -O3) driven by the restated cycle loop with OpenMP domain decomposition, on the same state,
+timed for a bounded number of cycles on the host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "MC trial moves/s @N=1e5 spherocyl.; plain FP64 vs guarded-FP32+learned order"
+HBM_PEAK_GBS = 8000.0
+RECORD_BYTES = 96  # FP64 position + FP64 3x3 orientation, the reference's particle record
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def near_cubic_divisions(cores: int):
+    best = (cores, 1, 1)
+    for a in range(1, cores + 1):
+        if cores % a:
+            continue
+        for b in range(1, cores // a + 1):
+            if (cores // a) % b:
+                continue
+            c = cores // a // b
+            cand = tuple(sorted((a, b, c), reverse=True))
+            if max(cand) - min(cand) < max(best) - min(best):
+                best = cand
+    return best
+
+
+def cpu_baseline(box, pos, rot, trans_step, rot_step, box_step, cycles, threads, seed):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+    if not os.path.exists(oracle_lib.REF_PERF_PATH):
+        return None
+    ref = oracle_lib.ref(perf=True)
+    dd = near_cubic_divisions(threads)
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    res = oracle_lib.ref_mc_run(ref, 1, 0.5, 5.0, box, pos, rot, 1.0, 2.0, trans_step, rot_step, 1, box_step, seed,
+                                dd=dd, threads=threads, therm=0, avg=cycles, avg_every=0)
+    rate = len(pos) * cycles / res["seconds"]
+    return {"value": rate, "unit": "trial moves/s", "cores": threads, "kind": "reference",
+            "sample": f"{cycles} cycles of the same N={len(pos)} state (after GPU warm-up), NpT p*=2, "
+                      f"rototranslation + delta_triclinic, OpenMP domain divisions {list(dd)}, "
+                      f"{res['seconds']:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=300)
+    ap.add_argument("--n", type=int, default=100000)
+    ap.add_argument("--seed", type=int, default=20261202)
+    ap.add_argument("--timing-cycles", type=int, default=50)
+    ap.add_argument("--cpu-cycles", type=int, default=-1, help="-1: sized to ~20 s of CPU work; 0: skip")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the sweep shards by spatial domains across "
+              f"GPUs by design but no multi-GPU path is built", file=sys.stderr)
+        sys.exit(2)
+
+    import rampack_amd as ra
+    from rampack_amd import lattice
+
+    shape = ra.Shape("spherocylinder", radius=0.5, length=5.0)
+    box, pos, rot = lattice.spherocylinder_smectic(args.n, 0.42)
+    pk = ra.GpuPacking(shape, box, pos, rot, seed=args.seed)
+    pk.set_environment(1.0, 2.0, 0.3, None, box_move=("delta_triclinic", 0.05))
+    t0 = time.perf_counter()
+    warm = pk.run_cycles(args.warmup, adapt_steps=True)
+    log(f"warm-up {args.warmup} cycles in {time.perf_counter() - t0:.2f} s: phi={pk.packing_fraction():.4f} "
+        f"steps trans={warm['trans_step']:.4f} rot={warm['rot_step']:.4f} box={warm['box_step']:.4f}")
+    phi_start = pk.packing_fraction()
+    before = pk.run_cycles(0)
+
+    t0 = time.perf_counter()
+    after = pk.run_cycles(args.steps)  # returns after the device finished (counters are read back)
+    elapsed = time.perf_counter() - t0
+    moves = after["moves_attempted"] - before["moves_attempted"]
+    accepted = after["moves_accepted"] - before["moves_accepted"]
+    acc_rate = accepted / moves
+    rate = moves / elapsed
+    phi_end = pk.packing_fraction()
+    log(f"timed {args.steps} cycles in {elapsed:.3f} s: {rate:.4g} trial moves/s, acceptance {acc_rate:.3f}, "
+        f"phi {phi_end:.4f}, box moves {after['box_accepted'] - before['box_accepted']}/"
+        f"{after['box_attempted'] - before['box_attempted']}, unstaged blocks {after['unstaged_blocks']}")
+
+    # event-timed leg for the roofline (syncs once per cycle, so it is not the throughput measurement)
+    pk.set_timing(True)
+    tb = pk.run_cycles(args.timing_cycles)
+    tm = pk.timing()
+    pk.set_timing(False)
+    moves_per_launch = len(pos) / 8.0
+    avg_sweep_ms = tm["sweep_ms"] / max(tm["sweep_launches"], 1)
+    bytes_per_launch = (27 + acc_rate) * RECORD_BYTES * moves_per_launch
+    achieved = bytes_per_launch / (avg_sweep_ms * 1e-3) / 1e9
+    log(f"sweep kernel: {tm['sweep_launches']} launches, avg {avg_sweep_ms * 1e3:.1f} us; rebuild avg "
+        f"{tm['rebuild_ms'] / max(tm['rebuild_launches'], 1) * 1e3:.1f} us x {tm['rebuild_launches']}; scan avg "
+        f"{tm['scan_ms'] / max(tm['scan_launches'], 1) * 1e3:.1f} us x {tm['scan_launches']}")
+
+    cpu = None
+    if args.cpu_cycles != 0:
+        b, p, r = pk.download()
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        cycles = args.cpu_cycles
+        if cycles < 0:
+            probe = cpu_baseline(b, p, r, tb["trans_step"], tb["rot_step"], tb["box_step"], 2, threads, args.seed)
+            if probe is not None:
+                per_cycle = len(p) / probe["value"]
+                cycles = max(3, min(400, int(20.0 / per_cycle)))
+        if cycles > 0:
+            cpu = cpu_baseline(b, p, r, tb["trans_step"], tb["rot_step"], tb["box_step"], cycles, threads, args.seed)
+        if cpu:
+            log(f"cpu baseline: {cpu['value']:.4g} trial moves/s on {cpu['cores']} threads ({cpu['sample']})")
+
+    out = {
+        "metric": METRIC,
+        "value": rate,
+        "unit": "trial moves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"hard spherocylinders L/D=5 (r=0.5, L=5), N={len(pos)}, NpT p*=2.0 T=1, rototranslation "
+                        f"+ delta_triclinic box move per cycle, plain FP64 checkerboard sweep",
+            "n_particles": len(pos),
+            "pressure": 2.0,
+            "packing_fraction_timed_start": round(phi_start, 5),
+            "packing_fraction_timed_end": round(phi_end, 5),
+            "acceptance": round(acc_rate, 4),
+            "precision": "fp64",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": "k_sweep",
+            "avg_launch_us": avg_sweep_ms * 1e3,
+            "algorithmic_bytes_per_launch": bytes_per_launch,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -79,7 +79,7 @@ typedef struct rp_counters {
     uint64_t box_accepted;
     uint64_t box_scans;          /* box moves that ran the device-wide overlap scan */
     uint64_t box_too_small;      /* box moves rejected because a box height fell below two interaction ranges */
-    uint64_t skipped_blocks;     /* active cells whose stencil did not fit the LDS staging area */
+    uint64_t unstaged_blocks;    /* active cells whose stencil did not fit LDS and was read from HBM instead */
     uint64_t pair_tests;         /* candidate slots examined by the sweep */
     double trans_step, rot_step, box_step;
 } rp_counters;

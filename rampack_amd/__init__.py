@@ -55,7 +55,7 @@ class BoxDesc(ctypes.Structure):
 class Counters(ctypes.Structure):
     _fields_ = [(name, ctypes.c_uint64) for name in (
         "cycles", "moves_attempted", "moves_accepted", "moves_left_cell", "box_attempted", "box_accepted",
-        "box_scans", "box_too_small", "skipped_blocks", "pair_tests")] + [
+        "box_scans", "box_too_small", "unstaged_blocks", "pair_tests")] + [
         ("trans_step", ctypes.c_double), ("rot_step", ctypes.c_double), ("box_step", ctypes.c_double)]
 
     def as_dict(self):

@@ -45,5 +45,18 @@ def build_library(verbose: bool = False) -> str:
     return LIB
 
 
+def build_facade_test() -> str:
+    """Compiles the C++ facade check program (tests/cpp/facade_test) against the built library."""
+    src = os.path.join(ROOT, "tests", "cpp", "facade_test.cpp")
+    out = os.path.join(ROOT, "tests", "cpp", "facade_test")
+    cmd = ["g++", "-std=c++17", "-O2", "-I", os.path.join(ROOT, "include"), "-I", os.path.join(HERE, "host"),
+           "-o", out, src, "-L", LIB_DIR, "-lrampack_amd", "-Wl,-rpath,$ORIGIN/../../rampack_amd/lib"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("the facade test program did not compile")
+    return out
+
+
 if __name__ == "__main__":
     print(build_library(verbose=True))

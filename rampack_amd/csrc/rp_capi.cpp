@@ -622,7 +622,7 @@ int rp_run_cycles(rp_ctx *c, size_t nCycles, int adapt, rp_counters *out) {
         out->box_accepted = c->boxAccepted;
         out->box_scans = c->boxScans;
         out->box_too_small = c->boxTooSmall;
-        out->skipped_blocks = c->hCounters[CNT_SKIPPED_BLOCKS];
+        out->unstaged_blocks = c->hCounters[CNT_UNSTAGED_BLOCKS];
         out->pair_tests = c->hCounters[CNT_TESTS];
         double maxT = c->maxTransImplicit ? minHeight(c->box) / 2 : c->maxTransStep;
         out->trans_step = std::min(maxT, c->transStep);

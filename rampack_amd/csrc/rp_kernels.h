@@ -50,7 +50,7 @@ enum Counter : int {
     CNT_ACCEPTED = 0,
     CNT_ATTEMPTED = 1,
     CNT_LEFT_CELL = 2,
-    CNT_SKIPPED_BLOCKS = 3,
+    CNT_UNSTAGED_BLOCKS = 3,
     CNT_OVERLAPS = 4,
     CNT_MAX_STENCIL = 5,
     CNT_TESTS = 6,

@@ -353,6 +353,152 @@ size_t sweep_lds_bytes(int kind, int cap) {
     return perEntry * (size_t)cap;
 }
 
+// Stencil view of one block: entry j of the 27 stencil cells (own cell first) either staged in LDS or read from
+// the cell-sorted arrays in HBM when the stencil does not fit the LDS area.
+struct Stencil {
+    const int *start;   // 28 prefix offsets (LDS)
+    const int *base;    // 27 first record indices (LDS)
+    const int *ecode;   // 27 packed cell image shifts (LDS)
+    double *x, *y, *z, *ax, *ay, *az;
+    int *code;
+};
+
+__device__ __forceinline__ int stencil_cell(const int *start, int j) {
+    int lo = 0, hi = 27;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (start[mid] <= j) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// Per-axis image code e + w_j + 1 (two bits per axis) of a record with wrap bits w in a stencil cell with packed
+// image shift ec.
+__device__ __forceinline__ int image_code(int ec, int w) {
+    return (((ec & 3) - 1 + (w & 1)) + 1) | ((((ec >> 2) & 3) - 1 + ((w >> 1) & 1) + 1) << 2) |
+           ((((ec >> 4) & 3) - 1 + ((w >> 2) & 1) + 1) << 4);
+}
+
+template <int KIND, bool STAGED>
+__device__ __forceinline__ void load_candidate(const SweepParams &P, const Stencil &S, int j, V3 &p2, V3 &a2,
+                                               int &code) {
+    if (STAGED) {
+        p2 = v3(S.x[j], S.y[j], S.z[j]);
+        a2 = KIND == 1 ? v3(S.ax[j], S.ay[j], S.az[j]) : v3(0, 0, 0);
+        code = S.code[j];
+    } else {
+        int q = stencil_cell(S.start, j);
+        int gi = S.base[q] + (j - S.start[q]);
+        p2 = load3(P.p.x, P.p.y, P.p.z, gi);
+        a2 = KIND == 1 ? v3(P.p.o[2][gi], P.p.o[5][gi], P.p.o[8][gi]) : v3(0, 0, 0);
+        code = image_code(S.ecode[q], P.p.wbits[gi]);
+    }
+}
+
+// The trial moves of one active cell: n_cell moves, each a uniform pick among the cell's particles and a
+// rototranslation (RototranslationSampler.cpp:22-49), rejected if the particle leaves the cell, otherwise tested
+// against the stencil 64 candidates at a time with a wavefront vote after every step.
+template <int KIND, bool STAGED>
+__device__ void sweep_cell(const SweepParams &P, const Stencil &S, int cell, int own0, int nOwn, int total) {
+    const Grid &g = P.g;
+    const int lane = threadIdx.x;
+    Philox rng = philox_init(P.seed, (P.cycle << 3) | (uint32_t)P.colour, (uint32_t)cell, 0x5eed5u);
+    unsigned long long accepted = 0, leftCell = 0, tests = 0;
+    const double ts = P.transStep, rs = P.rotStep;
+
+    for (int m = 0; m < nOwn; m++) {
+        int kIdx = (int)philox_below(rng, (uint32_t)nOwn);
+        V3 t;
+        t.x = philox_uniform(rng, -ts, ts);
+        t.y = philox_uniform(rng, -ts, ts);
+        t.z = philox_uniform(rng, -ts, ts);
+        V3 axis;
+        do {
+            axis.x = philox_uniform(rng, -1, 1);
+            axis.y = philox_uniform(rng, -1, 1);
+            axis.z = philox_uniform(rng, -1, 1);
+        } while (dot(axis, axis) > 1 || dot(axis, axis) == 0);
+        double angle = philox_uniform(rng, -rs, rs);
+
+        const int gi = own0 + kIdx;
+        V3 pos = STAGED ? v3(S.x[kIdx], S.y[kIdx], S.z[kIdx]) : load3(P.p.x, P.p.y, P.p.z, gi);
+        V3 npos = translate_wrapped(g.dims, g.inv, pos, t);
+        int nc[3];
+        unsigned nw;
+        bin_position(g, npos, nc, nw);
+        if (cell_index(g, nc[0], nc[1], nc[2]) != cell) {
+            leftCell++;
+            continue;
+        }
+
+        double O[9], R[9], NO[9];
+        for (int c = 0; c < 9; c++) O[c] = P.p.o[c][gi];
+        // Matrix<3,3>::rotation(axis.normalized(), angle) (Matrix.tpp:438-450), then Shape::rotate: R * O
+        double nrm = sqrt(dot(axis, axis));
+        V3 ua = v3(axis.x / nrm, axis.y / nrm, axis.z / nrm);
+        double sa = sin(angle), ca = cos(angle);
+        double K[9] = {0, -ua.z, ua.y, ua.z, 0, -ua.x, -ua.y, ua.x, 0};
+        double f1 = 1 - ca;
+        double fK[9];
+        for (int c = 0; c < 9; c++) fK[c] = K[c] * f1;
+        double fKK[9];
+        mulm(fK, K, fKK);
+        for (int c = 0; c < 9; c++) R[c] = ((c % 4 == 0 ? 1.0 : 0.0) + K[c] * sa) + fKK[c];
+        mulm(R, O, NO);
+        V3 nax = v3(NO[2], NO[5], NO[8]);
+        SCFirst f = make_first<KIND>(P.k, npos, nax);
+        const int wx = nw & 1, wy = (nw >> 1) & 1, wz = (nw >> 2) & 1;
+
+        bool hit = false;
+        for (int base = 0; base < total; base += WAVE) {
+            int j = base + lane;
+            bool h = false;
+            if (j < total && j != kIdx) {
+                V3 p2, a2;
+                int code;
+                load_candidate<KIND, STAGED>(P, S, j, p2, a2, code);
+                V3 T = translation(g, (code & 3) - 1 - wx, ((code >> 2) & 3) - 1 - wy, ((code >> 4) & 3) - 1 - wz);
+                h = pair_overlap<KIND>(P.k, f, p2, a2, T);
+            }
+            tests += (unsigned long long)min(WAVE, total - base);
+            if (__any(h)) {
+                hit = true;
+                break;
+            }
+        }
+        if (hit) continue;
+
+        // Packing::acceptMove: the cell belongs to this wavefront for the whole colour pass
+        accepted++;
+        if (lane == 0) {
+            P.p.x[gi] = npos.x;
+            P.p.y[gi] = npos.y;
+            P.p.z[gi] = npos.z;
+            for (int c = 0; c < 9; c++) P.p.o[c][gi] = NO[c];
+            P.p.wbits[gi] = (uint8_t)nw;
+            if (STAGED) {
+                S.x[kIdx] = npos.x;
+                S.y[kIdx] = npos.y;
+                S.z[kIdx] = npos.z;
+                if (KIND == 1) {
+                    S.ax[kIdx] = nax.x;
+                    S.ay[kIdx] = nax.y;
+                    S.az[kIdx] = nax.z;
+                }
+                S.code[kIdx] = (wx + 1) | ((wy + 1) << 2) | ((wz + 1) << 4);
+            }
+        }
+        __syncthreads();
+    }
+    if (lane == 0) {
+        atomicAdd(&P.counters[CNT_ACCEPTED], accepted);
+        atomicAdd(&P.counters[CNT_ATTEMPTED], (unsigned long long)nOwn);
+        atomicAdd(&P.counters[CNT_LEFT_CELL], leftCell);
+        atomicAdd(&P.counters[CNT_TESTS], tests);
+        if (!STAGED) atomicAdd(&P.counters[CNT_UNSTAGED_BLOCKS], 1ull);
+    }
+}
+
 template <int KIND>
 __global__ __launch_bounds__(WAVE) void k_sweep(SweepParams P) {
     extern __shared__ double smem[];
@@ -389,133 +535,38 @@ __global__ __launch_bounds__(WAVE) void k_sweep(SweepParams P) {
     }
     __syncthreads();
     const int total = s_start[27];
+
+    Stencil S;
+    S.start = s_start;
+    S.base = s_base;
+    S.ecode = s_ecode;
+    S.x = smem;
+    S.y = S.x + P.cap;
+    S.z = S.y + P.cap;
+    S.ax = S.z + P.cap;
+    S.ay = S.ax + P.cap;
+    S.az = S.ay + P.cap;
+    S.code = reinterpret_cast<int *>(KIND == 0 ? S.ax : S.az + P.cap);
+
     if (total > P.cap) {
-        if (lane == 0) atomicAdd(&P.counters[CNT_SKIPPED_BLOCKS], 1ull);
+        sweep_cell<KIND, false>(P, S, cell, own0, nOwn, total);
         return;
     }
-
-    double *sx = smem, *sy = sx + P.cap, *sz = sy + P.cap;
-    double *sax = sz + P.cap, *say = sax + P.cap, *saz = say + P.cap;
-    int *scode = reinterpret_cast<int *>(KIND == 0 ? sax : saz + P.cap);
-
-    // Stage the stencil: entry j -> stencil cell q (binary search over the prefix), global record index.
     for (int j = lane; j < total; j += WAVE) {
-        int lo = 0, hi = 27;
-        while (hi - lo > 1) {
-            int mid = (lo + hi) >> 1;
-            if (s_start[mid] <= j) lo = mid; else hi = mid;
-        }
-        int gi = s_base[lo] + (j - s_start[lo]);
-        sx[j] = P.p.x[gi];
-        sy[j] = P.p.y[gi];
-        sz[j] = P.p.z[gi];
+        int q = stencil_cell(s_start, j);
+        int gi = s_base[q] + (j - s_start[q]);
+        S.x[j] = P.p.x[gi];
+        S.y[j] = P.p.y[gi];
+        S.z[j] = P.p.z[gi];
         if (KIND == 1) {
-            sax[j] = P.p.o[2][gi];
-            say[j] = P.p.o[5][gi];
-            saz[j] = P.p.o[8][gi];
+            S.ax[j] = P.p.o[2][gi];
+            S.ay[j] = P.p.o[5][gi];
+            S.az[j] = P.p.o[8][gi];
         }
-        // per-axis image code e + w_j + 1, two bits per axis
-        int ec = s_ecode[lo];
-        int w = P.p.wbits[gi];
-        int code = (((ec & 3) - 1 + (w & 1)) + 1) | ((((ec >> 2) & 3) - 1 + ((w >> 1) & 1) + 1) << 2) |
-                   ((((ec >> 4) & 3) - 1 + ((w >> 2) & 1) + 1) << 4);
-        scode[j] = code;
+        S.code[j] = image_code(s_ecode[q], P.p.wbits[gi]);
     }
     __syncthreads();
-
-    Philox rng = philox_init(P.seed, (P.cycle << 3) | (uint32_t)P.colour, (uint32_t)cell, 0x5eed5u);
-    unsigned long long accepted = 0, leftCell = 0, tests = 0;
-    const double ts = P.transStep, rs = P.rotStep;
-
-    for (int m = 0; m < nOwn; m++) {
-        // RototranslationSampler::sampleMove (RototranslationSampler.cpp:22-49)
-        int kIdx = (int)philox_below(rng, (uint32_t)nOwn);
-        V3 t;
-        t.x = philox_uniform(rng, -ts, ts);
-        t.y = philox_uniform(rng, -ts, ts);
-        t.z = philox_uniform(rng, -ts, ts);
-        V3 axis;
-        do {
-            axis.x = philox_uniform(rng, -1, 1);
-            axis.y = philox_uniform(rng, -1, 1);
-            axis.z = philox_uniform(rng, -1, 1);
-        } while (dot(axis, axis) > 1 || dot(axis, axis) == 0);
-        double angle = philox_uniform(rng, -rs, rs);
-
-        const int gi = own0 + kIdx;
-        V3 pos = v3(sx[kIdx], sy[kIdx], sz[kIdx]);
-        V3 npos = translate_wrapped(g.dims, g.inv, pos, t);
-        int nc[3];
-        unsigned nw;
-        bin_position(g, npos, nc, nw);
-        if (cell_index(g, nc[0], nc[1], nc[2]) != cell) {
-            leftCell++;
-            continue;
-        }
-
-        double O[9], R[9], NO[9];
-        for (int c = 0; c < 9; c++) O[c] = P.p.o[c][gi];
-        // Matrix<3,3>::rotation(axis.normalized(), angle) (Matrix.tpp:438-450), then Shape::rotate: R * O
-        double nrm = sqrt(dot(axis, axis));
-        V3 ua = v3(axis.x / nrm, axis.y / nrm, axis.z / nrm);
-        double sa = sin(angle), ca = cos(angle);
-        double K[9] = {0, -ua.z, ua.y, ua.z, 0, -ua.x, -ua.y, ua.x, 0};
-        double f1 = 1 - ca;
-        double fK[9];
-        for (int c = 0; c < 9; c++) fK[c] = K[c] * f1;
-        double fKK[9];
-        mulm(fK, K, fKK);
-        for (int c = 0; c < 9; c++) R[c] = ((c % 4 == 0 ? 1.0 : 0.0) + K[c] * sa) + fKK[c];
-        mulm(R, O, NO);
-        V3 nax = v3(NO[2], NO[5], NO[8]);
-        SCFirst f = make_first<KIND>(P.k, npos, nax);
-        const int wx = nw & 1, wy = (nw >> 1) & 1, wz = (nw >> 2) & 1;
-
-        bool hit = false;
-        for (int base = 0; base < total; base += WAVE) {
-            int j = base + lane;
-            bool h = false;
-            if (j < total && j != kIdx) {
-                int code = scode[j];
-                V3 T = translation(g, (code & 3) - 1 - wx, ((code >> 2) & 3) - 1 - wy, ((code >> 4) & 3) - 1 - wz);
-                V3 p2 = v3(sx[j], sy[j], sz[j]);
-                V3 a2 = KIND == 1 ? v3(sax[j], say[j], saz[j]) : v3(0, 0, 0);
-                h = pair_overlap<KIND>(P.k, f, p2, a2, T);
-            }
-            tests += (unsigned long long)min(WAVE, total - base);
-            if (__any(h)) {
-                hit = true;
-                break;
-            }
-        }
-        if (hit) continue;
-
-        // Packing::acceptMove: the cell is owned by this wavefront for the whole colour pass
-        accepted++;
-        if (lane == 0) {
-            P.p.x[gi] = npos.x;
-            P.p.y[gi] = npos.y;
-            P.p.z[gi] = npos.z;
-            for (int c = 0; c < 9; c++) P.p.o[c][gi] = NO[c];
-            P.p.wbits[gi] = (uint8_t)nw;
-            sx[kIdx] = npos.x;
-            sy[kIdx] = npos.y;
-            sz[kIdx] = npos.z;
-            if (KIND == 1) {
-                sax[kIdx] = nax.x;
-                say[kIdx] = nax.y;
-                saz[kIdx] = nax.z;
-            }
-            scode[kIdx] = (wx + 1) | ((wy + 1) << 2) | ((wz + 1) << 4);
-        }
-        __syncthreads();
-    }
-    if (lane == 0) {
-        atomicAdd(&P.counters[CNT_ACCEPTED], accepted);
-        atomicAdd(&P.counters[CNT_ATTEMPTED], (unsigned long long)nOwn);
-        atomicAdd(&P.counters[CNT_LEFT_CELL], leftCell);
-        atomicAdd(&P.counters[CNT_TESTS], tests);
-    }
+    sweep_cell<KIND, true>(P, S, cell, own0, nOwn, total);
 }
 
 hipError_t launch_sweep(hipStream_t s, const SweepParams &sp) {

@@ -1,0 +1,359 @@
+// rampack_gpu.hpp -- C++17 facade over the C ABI (include/rampack_amd.h) with the reference's plug-in seams.
+//
+// The classes keep the names, constructor arguments and error behaviour of the reference types they stand in
+// for, so a caller written against rampack's Simulation / Packing / ShapeTraits / MoveSampler /
+// TriclinicBoxScaler API reads the same here:
+//   ShapeTraits, Interaction, ShapeGeometry           src/core/ShapeTraits.h:20-42, src/core/Interaction.h:18-150
+//   SphereTraits(radius)                              src/core/shapes/SphereTraits.h:60
+//   SpherocylinderTraits(length, radius)              src/core/shapes/SpherocylinderTraits.h:53
+//   RototranslationSampler(trans, rot, maxTrans)      src/core/move_samplers/RototranslationSampler.h:34-35
+//   TriclinicDeltaScaler(step, scaleTogether)         src/core/volume_scalers/TriclinicDeltaScaler.h:25
+//   TriclinicAdapter(volumeScaler, step)              src/core/volume_scalers/TriclinicAdapter.h:22-24
+//   Packing(box, shapes, interaction)                 src/core/Packing.h:154-160
+//   Simulation(packing, moveSamplers, seed, scaler)   src/core/Simulation.h:281-283
+//   Simulation::integrate(T, p, therm, avg, every)    src/core/Simulation.h:321-340
+// Physics plug-ins cannot run per pair across a device boundary, so every plug-in exports a plain descriptor and
+// the whole sweep runs behind the C ABI.  Errors map to the reference's exception types (src/utils/Exceptions.h).
+#pragma once
+
+#include "rampack_amd.h"
+
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <functional>
+#include <memory>
+#include <optional>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+namespace rampack_amd {
+
+struct PreconditionException : std::logic_error {
+    using std::logic_error::logic_error;
+};
+struct ValidationException : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct DeviceException : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+inline void check(int code) {
+    if (code == RP_OK) return;
+    std::string msg = rp_last_error();
+    if (code == RP_ERR_PRECONDITION) throw PreconditionException(msg);
+    if (code == RP_ERR_VALIDATION) throw ValidationException(msg);
+    throw DeviceException(msg);
+}
+
+// ---------------------------------------------------------------------------------------------- shapes / physics
+
+class Interaction {
+public:
+    virtual ~Interaction() = default;
+    [[nodiscard]] virtual bool hasHardPart() const = 0;
+    [[nodiscard]] virtual bool hasSoftPart() const = 0;
+    [[nodiscard]] virtual bool hasWallPart() const = 0;
+    [[nodiscard]] virtual bool isConvex() const = 0;
+    [[nodiscard]] virtual double getRangeRadius() const = 0;
+    [[nodiscard]] virtual double getTotalRangeRadius() const { return this->getRangeRadius(); }
+    // The device form of this interaction.
+    [[nodiscard]] virtual rp_shape_desc descriptor() const = 0;
+};
+
+class ShapeGeometry {
+public:
+    virtual ~ShapeGeometry() = default;
+    [[nodiscard]] virtual double getVolume() const = 0;
+};
+
+class ShapeTraits {
+public:
+    virtual ~ShapeTraits() = default;
+    [[nodiscard]] virtual const Interaction &getInteraction() const = 0;
+    [[nodiscard]] virtual const ShapeGeometry &getGeometry() const = 0;
+};
+
+class SphereTraits : public ShapeTraits, public Interaction, public ShapeGeometry {
+    double radius;
+
+public:
+    explicit SphereTraits(double radius = 1) : radius{radius} {
+        if (!(radius > 0)) throw PreconditionException("SphereTraits: radius > 0");
+    }
+    [[nodiscard]] const Interaction &getInteraction() const override { return *this; }
+    [[nodiscard]] const ShapeGeometry &getGeometry() const override { return *this; }
+    [[nodiscard]] bool hasHardPart() const override { return true; }
+    [[nodiscard]] bool hasSoftPart() const override { return false; }
+    [[nodiscard]] bool hasWallPart() const override { return true; }
+    [[nodiscard]] bool isConvex() const override { return true; }
+    [[nodiscard]] double getRangeRadius() const override { return 2 * this->radius; }
+    [[nodiscard]] double getVolume() const override { return 4. / 3 * M_PI * std::pow(this->radius, 3); }
+    [[nodiscard]] rp_shape_desc descriptor() const override { return {RP_SHAPE_SPHERE, this->radius, 0}; }
+};
+
+class SpherocylinderTraits : public ShapeTraits, public Interaction, public ShapeGeometry {
+    double length, radius;
+
+public:
+    SpherocylinderTraits() : SpherocylinderTraits(1, 1) {}
+    SpherocylinderTraits(double length, double radius) : length{length}, radius{radius} {
+        if (!(length >= 0)) throw PreconditionException("SpherocylinderTraits: length >= 0");
+        if (!(radius > 0)) throw PreconditionException("SpherocylinderTraits: radius > 0");
+    }
+    [[nodiscard]] const Interaction &getInteraction() const override { return *this; }
+    [[nodiscard]] const ShapeGeometry &getGeometry() const override { return *this; }
+    [[nodiscard]] bool hasHardPart() const override { return true; }
+    [[nodiscard]] bool hasSoftPart() const override { return false; }
+    [[nodiscard]] bool hasWallPart() const override { return true; }
+    [[nodiscard]] bool isConvex() const override { return true; }
+    [[nodiscard]] double getRangeRadius() const override { return 2 * this->radius + this->length; }
+    [[nodiscard]] double getVolume() const override {
+        return M_PI * this->radius * this->radius * this->length + 4. / 3 * M_PI * std::pow(this->radius, 3);
+    }
+    [[nodiscard]] rp_shape_desc descriptor() const override {
+        return {RP_SHAPE_SPHEROCYLINDER, this->radius, this->length};
+    }
+};
+
+// ---------------------------------------------------------------------------------------------- moves / box moves
+
+class MoveSampler {
+public:
+    virtual ~MoveSampler() = default;
+    [[nodiscard]] virtual std::string getName() const = 0;
+    [[nodiscard]] virtual std::size_t getNumOfRequestedMoves(std::size_t numParticles) const = 0;
+    [[nodiscard]] virtual std::vector<std::pair<std::string, double>> getStepSizes() const = 0;
+    [[nodiscard]] virtual rp_move_desc descriptor() const = 0;
+    virtual void updateStepSizes(double translation, double rotation) = 0;
+};
+
+class RototranslationSampler : public MoveSampler {
+    double translationStepSize;
+    std::optional<double> rotationStepSize;
+    double maxTranslationStepSize;
+
+public:
+    RototranslationSampler(double translationStepSize, std::optional<double> rotationStepSize,
+                           double maxTranslationStepSize = 0)
+            : translationStepSize{translationStepSize}, rotationStepSize{rotationStepSize},
+              maxTranslationStepSize{maxTranslationStepSize} {
+        if (!(maxTranslationStepSize >= 0)) throw PreconditionException("maxTranslationStepSize >= 0");
+        if (maxTranslationStepSize > 0 && translationStepSize > maxTranslationStepSize)
+            throw PreconditionException("translationStepSize <= maxTranslationStepSize");
+        if (!(translationStepSize > 0)) throw PreconditionException("translationStepSize > 0");
+        if (rotationStepSize.has_value() && !(*rotationStepSize > 0))
+            throw PreconditionException("rotationStepSize > 0");
+    }
+    [[nodiscard]] std::string getName() const override { return "rototranslation"; }
+    [[nodiscard]] std::size_t getNumOfRequestedMoves(std::size_t numParticles) const override { return numParticles; }
+    [[nodiscard]] std::vector<std::pair<std::string, double>> getStepSizes() const override {
+        return {{"translation", this->translationStepSize}, {"rotation", this->rotationStepSize.value_or(0)}};
+    }
+    [[nodiscard]] rp_move_desc descriptor() const override {
+        return {this->translationStepSize, this->rotationStepSize.value_or(0), this->maxTranslationStepSize};
+    }
+    void updateStepSizes(double translation, double rotation) override {
+        this->translationStepSize = translation;
+        this->rotationStepSize = rotation;
+    }
+};
+
+class TriclinicBoxScaler {
+protected:
+    double stepSize{};
+
+public:
+    explicit TriclinicBoxScaler(double stepSize) : stepSize{stepSize} {
+        if (!(stepSize > 0)) throw PreconditionException("TriclinicBoxScaler: stepSize > 0");
+    }
+    virtual ~TriclinicBoxScaler() = default;
+    [[nodiscard]] double getStepSize() const { return this->stepSize; }
+    void setStepSize(double s) {
+        if (!(s > 0)) throw PreconditionException("TriclinicBoxScaler: stepSize > 0");
+        this->stepSize = s;
+    }
+    [[nodiscard]] virtual rp_box_desc descriptor() const = 0;
+};
+
+class TriclinicDeltaScaler : public TriclinicBoxScaler {
+public:
+    TriclinicDeltaScaler(double stepSize, bool scaleTogether) : TriclinicBoxScaler(stepSize) {
+        if (!scaleTogether)
+            throw PreconditionException("TriclinicDeltaScaler: only scale_together = True runs on the device");
+    }
+    [[nodiscard]] rp_box_desc descriptor() const override { return {RP_BOX_DELTA_TRICLINIC, this->stepSize}; }
+};
+
+// VolumeScaler kinds accepted by TriclinicAdapter.
+class VolumeScaler {
+public:
+    virtual ~VolumeScaler() = default;
+    [[nodiscard]] virtual int kind() const = 0;
+};
+class LogIsotropicVolumeScaler : public VolumeScaler {  // log(spec="isotropic"), AnisotropicVolumeScaler + LogScaling
+public:
+    [[nodiscard]] int kind() const override { return RP_BOX_LOG_ISOTROPIC; }
+};
+class DeltaVolumeScaler : public VolumeScaler {
+public:
+    [[nodiscard]] int kind() const override { return RP_BOX_DELTA_V; }
+};
+
+class TriclinicAdapter : public TriclinicBoxScaler {
+    std::unique_ptr<VolumeScaler> volumeScaler;
+
+public:
+    TriclinicAdapter(std::unique_ptr<VolumeScaler> volumeScaler, double stepSize)
+            : TriclinicBoxScaler(stepSize), volumeScaler{std::move(volumeScaler)} {}
+    [[nodiscard]] rp_box_desc descriptor() const override { return {this->volumeScaler->kind(), this->stepSize}; }
+};
+
+// ---------------------------------------------------------------------------------------------- packing
+
+struct Shape {
+    std::array<double, 3> position{};
+    std::array<double, 9> orientation{1, 0, 0, 0, 1, 0, 0, 0, 1};  // row-major
+};
+
+// Box as a row-major matrix whose columns are the box sides (TriclinicBox.h:49-58).
+using Box = std::array<double, 9>;
+
+class Packing {
+    Box box;
+    std::vector<Shape> shapes;
+    rp_shape_desc shape;
+    rp_ctx *ctx = nullptr;
+    bool stale = false;
+
+    void upload(uint64_t seed) {
+        std::vector<double> pos(3 * shapes.size()), rot(9 * shapes.size());
+        for (std::size_t i = 0; i < shapes.size(); i++) {
+            for (int k = 0; k < 3; k++) pos[3 * i + k] = shapes[i].position[k];
+            for (int k = 0; k < 9; k++) rot[9 * i + k] = shapes[i].orientation[k];
+        }
+        rp_ctx *c = nullptr;
+        check(rp_create(&c, &shape, box.data(), shapes.size(), pos.data(), rot.data(), seed, RP_PREC_FP64));
+        if (ctx) rp_destroy(ctx);
+        ctx = c;
+    }
+
+public:
+    Packing(const Box &box, std::vector<Shape> shapes, const Interaction &interaction)
+            : box{box}, shapes{std::move(shapes)}, shape{interaction.descriptor()} {
+        if (this->shapes.empty()) throw PreconditionException("Packing: no shapes");
+        this->upload(0);
+    }
+    Packing(const Packing &) = delete;
+    Packing &operator=(const Packing &) = delete;
+    ~Packing() {
+        if (ctx) rp_destroy(ctx);
+    }
+
+    // Re-keys the device random streams (Simulation's seed, Simulation.cpp:66-68).
+    void reseed(uint64_t seed) {
+        this->refresh();
+        this->upload(seed);
+    }
+    void refresh() const {
+        if (!stale) return;
+        auto *self = const_cast<Packing *>(this);
+        std::vector<double> pos(3 * shapes.size()), rot(9 * shapes.size());
+        check(rp_download(ctx, self->box.data(), pos.data(), rot.data()));
+        for (std::size_t i = 0; i < shapes.size(); i++) {
+            for (int k = 0; k < 3; k++) self->shapes[i].position[k] = pos[3 * i + k];
+            for (int k = 0; k < 9; k++) self->shapes[i].orientation[k] = rot[9 * i + k];
+        }
+        self->stale = false;
+    }
+    void markStale() { stale = true; }
+    [[nodiscard]] rp_ctx *context() const { return ctx; }
+    [[nodiscard]] std::size_t size() const { return shapes.size(); }
+    [[nodiscard]] const Shape &operator[](std::size_t i) const {
+        if (i >= shapes.size()) throw PreconditionException("Packing: index out of range");
+        refresh();
+        return shapes[i];
+    }
+    [[nodiscard]] const Box &getBox() const {
+        refresh();
+        return box;
+    }
+    [[nodiscard]] double getVolume() const {
+        const Box &b = getBox();
+        double det = b[0] * b[4] * b[8] + b[1] * b[5] * b[6] + b[2] * b[3] * b[7] - b[2] * b[4] * b[6] -
+                     b[1] * b[3] * b[8] - b[0] * b[5] * b[7];
+        return std::abs(det);
+    }
+    [[nodiscard]] double getPackingFraction(double shapeVolume) const {
+        return (double)this->size() / this->getVolume() * shapeVolume;
+    }
+    // Packing::countTotalOverlaps (Packing.cpp:519-559)
+    [[nodiscard]] std::size_t countTotalOverlaps(const Interaction &, bool earlyExit = true) const {
+        uint64_t n = 0;
+        check(rp_count_total_overlaps(ctx, earlyExit ? 1 : 0, &n));
+        return (std::size_t)n;
+    }
+};
+
+// ---------------------------------------------------------------------------------------------- simulation
+
+class Simulation {
+    std::unique_ptr<Packing> packing;
+    std::vector<std::unique_ptr<MoveSampler>> moveSamplers;
+    std::unique_ptr<TriclinicBoxScaler> boxScaler;
+    rp_counters counters{};
+    double temperature = 1, pressure = 0;
+
+public:
+    Simulation(std::unique_ptr<Packing> packing, std::vector<std::unique_ptr<MoveSampler>> moveSamplers,
+               unsigned long seed, std::unique_ptr<TriclinicBoxScaler> boxScaler)
+            : packing{std::move(packing)}, moveSamplers{std::move(moveSamplers)}, boxScaler{std::move(boxScaler)} {
+        if (!this->packing) throw PreconditionException("Simulation: no packing");
+        if (this->moveSamplers.size() != 1 || this->moveSamplers.front()->getName() != "rototranslation")
+            throw PreconditionException("Simulation: the device sweep runs one rototranslation sampler");
+        this->packing->reseed(seed);
+    }
+
+    // Simulation::integrate (Simulation.cpp:98-199): thermalisation with step adaptation, then averaging; the
+    // observer is called every `averagingEvery` averaging cycles with the current packing.
+    void integrate(double temperature_, double pressure_, std::size_t thermalisationCycles,
+                   std::size_t averagingCycles, std::size_t averagingEvery,
+                   const std::function<void(const Packing &, std::size_t)> &observer = {}) {
+        if (thermalisationCycles == 0 && averagingCycles == 0) throw PreconditionException("no cycles");
+        if (averagingCycles > 0 && (averagingEvery == 0 || averagingEvery > averagingCycles))
+            throw PreconditionException("averagingEvery");
+        this->temperature = temperature_;
+        this->pressure = pressure_;
+        rp_move_desc md = this->moveSamplers.front()->descriptor();
+        rp_box_desc bd{RP_BOX_NONE, 0};
+        if (this->boxScaler) bd = this->boxScaler->descriptor();
+        check(rp_set_environment(this->packing->context(), temperature_, pressure_, &md, &bd));
+        if (thermalisationCycles > 0)
+            this->run(thermalisationCycles, true);
+        for (std::size_t done = 0; done < averagingCycles; done += averagingEvery) {
+            std::size_t chunk = std::min(averagingEvery, averagingCycles - done);
+            this->run(chunk, false);
+            if (observer && chunk == averagingEvery) observer(*this->packing, this->counters.cycles);
+        }
+    }
+
+    [[nodiscard]] const Packing &getPacking() const { return *this->packing; }
+    [[nodiscard]] const rp_counters &getCounters() const { return this->counters; }
+    [[nodiscard]] double getMoveAcceptanceRate() const {
+        return this->counters.moves_attempted ? (double)this->counters.moves_accepted / this->counters.moves_attempted
+                                              : 0;
+    }
+
+private:
+    void run(std::size_t cycles, bool adapt) {
+        check(rp_run_cycles(this->packing->context(), cycles, adapt ? 1 : 0, &this->counters));
+        this->packing->markStale();
+        this->moveSamplers.front()->updateStepSizes(this->counters.trans_step, this->counters.rot_step);
+        if (this->boxScaler) this->boxScaler->setStepSize(this->counters.box_step);
+    }
+};
+
+}  // namespace rampack_amd

@@ -1,0 +1,78 @@
+// Exercises the C++ facade (rampack_amd/host/rampack_gpu.hpp) the way the reference's own validation tests drive
+// Simulation (test/validation_tests/SimulationTest.cpp): build traits, samplers and a box scaler, integrate, read
+// the packing back.  Needs a GPU; prints one line per check and exits non-zero on the first failure.
+#include "rampack_gpu.hpp"
+
+#include <cstdio>
+#include <cstdlib>
+
+using namespace rampack_amd;
+
+#define CHECK(cond)                                                        \
+    do {                                                                   \
+        if (!(cond)) {                                                     \
+            std::printf("FAIL %s:%d %s\n", __FILE__, __LINE__, #cond);     \
+            std::exit(1);                                                  \
+        }                                                                  \
+        std::printf("ok   %s\n", #cond);                                   \
+    } while (0)
+
+template <typename E, typename F>
+bool throws(F f) {
+    try {
+        f();
+    } catch (const E &) {
+        return true;
+    }
+    return false;
+}
+
+int main() {
+    CHECK(throws<PreconditionException>([] { SpherocylinderTraits(-1, 1); }));
+    CHECK(throws<PreconditionException>([] { RototranslationSampler(0.5, std::nullopt, 0.1); }));
+    CHECK(throws<PreconditionException>([] { TriclinicDeltaScaler(0.1, false); }));
+
+    // 1000 hard spheres of radius 0.5 on a 10^3 sc lattice at packing fraction 0.3, NpT p = 5 with log moves
+    SphereTraits traits(0.5);
+    double side = std::cbrt(1000 * traits.getVolume() / 0.3);
+    std::vector<Shape> shapes;
+    for (int i = 0; i < 10; i++)
+        for (int j = 0; j < 10; j++)
+            for (int k = 0; k < 10; k++) {
+                Shape s;
+                s.position = {(i + 0.5) * side / 10, (j + 0.5) * side / 10, (k + 0.5) * side / 10};
+                shapes.push_back(s);
+            }
+    Box box{side, 0, 0, 0, side, 0, 0, 0, side};
+    auto packing = std::make_unique<Packing>(box, shapes, traits.getInteraction());
+    std::vector<std::unique_ptr<MoveSampler>> samplers;
+    samplers.push_back(std::make_unique<RototranslationSampler>(0.1, std::nullopt));
+    auto scaler = std::make_unique<TriclinicAdapter>(std::make_unique<LogIsotropicVolumeScaler>(), 0.01);
+    Simulation sim(std::move(packing), std::move(samplers), 1234, std::move(scaler));
+    std::vector<double> phis;
+    sim.integrate(1.0, 5.0, 300, 300, 50, [&](const Packing &p, std::size_t) {
+        phis.push_back(p.getPackingFraction(traits.getVolume()));
+    });
+    for (double phi : phis) std::printf("phi %.5f\n", phi);
+    std::printf("moves %llu/%llu box %llu/%llu steps %g %g %g\n", (unsigned long long)sim.getCounters().moves_accepted,
+                (unsigned long long)sim.getCounters().moves_attempted,
+                (unsigned long long)sim.getCounters().box_accepted, (unsigned long long)sim.getCounters().box_attempted,
+                sim.getCounters().trans_step, sim.getCounters().rot_step, sim.getCounters().box_step);
+    CHECK(phis.size() == 6);
+    CHECK(sim.getPacking().countTotalOverlaps(traits.getInteraction(), false) == 0);
+    CHECK(sim.getCounters().moves_attempted == 600 * 1000);
+    CHECK(phis.back() > 0.33 && phis.back() < 0.45);
+    CHECK(sim.getMoveAcceptanceRate() > 0.05 && sim.getMoveAcceptanceRate() < 0.6);
+
+    // overlapping start: Simulation::integrate refuses it (Simulation.cpp:131-132)
+    std::vector<Shape> bad(2);
+    bad[0].position = {1, 1, 1};
+    bad[1].position = {1.5, 1, 1};
+    auto badPacking = std::make_unique<Packing>(Box{10, 0, 0, 0, 10, 0, 0, 0, 10}, bad, traits.getInteraction());
+    std::vector<std::unique_ptr<MoveSampler>> s2;
+    s2.push_back(std::make_unique<RototranslationSampler>(0.1, std::nullopt));
+    Simulation badSim(std::move(badPacking), std::move(s2), 1, nullptr);
+    CHECK(throws<ValidationException>([&] { badSim.integrate(1.0, 1.0, 10, 0, 0); }));
+    std::printf("facade ok\n");
+    return 0;
+}
