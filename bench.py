@@ -4,7 +4,7 @@ One step = one MC cycle of rampack's Simulation::performCycle (src/core/Simulati
 rototranslations (eight checkerboard colour passes) plus one delta_triclinic box move, the same unit the reference
 reports as "Cycles per second" (src/frontend/modes/CasinoMode.cpp:405-412).  Trial moves/s = N * cycles / time.
 
-Start state: AA-stacked layers of parallel rods (packing fraction 0.42) equilibrated by the untimed warm-up cycles at
+Start state: AA-stacked layers of parallel rods (packing fraction 0.45) equilibrated by the untimed warm-up cycles at
 p* = 2 with step adaptation; the timed cycles run without adaptation.  Inputs are resident in HBM before timing.
 
 Roofline: the dominant kernel is the sweep (k_sweep).  Its algorithmic traffic per trial move is
@@ -71,6 +71,23 @@ def cpu_baseline(box, pos, rot, trans_step, rot_step, box_step, cycles, threads,
                       f"{res['seconds']:.1f} s"}
 
 
+def copy_probe(nbytes: int):
+    """A plain device-to-device copy of nbytes (the runtime's copy kernel), used to scale the FETCH_SIZE /
+    WRITE_SIZE counters of the sweep against a kernel of known traffic."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(src), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(dst), ctypes.c_size_t(nbytes)) == 0
+    hip.hipMemset(src, 1, ctypes.c_size_t(nbytes))
+    for _ in range(3):
+        assert hip.hipMemcpy(dst, src, ctypes.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    log(f"copy probe: 3 x {nbytes} B device-to-device")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -81,7 +98,12 @@ def main():
     ap.add_argument("--timing-cycles", type=int, default=50)
     ap.add_argument("--cpu-cycles", type=int, default=-1, help="-1: sized to ~20 s of CPU work; 0: skip")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only run a device-to-device copy of this many bytes (HBM counter calibration) and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the sweep shards by spatial domains across "
               f"GPUs by design but no multi-GPU path is built", file=sys.stderr)
@@ -91,9 +113,9 @@ def main():
     from rampack_amd import lattice
 
     shape = ra.Shape("spherocylinder", radius=0.5, length=5.0)
-    box, pos, rot = lattice.spherocylinder_smectic(args.n, 0.42)
+    box, pos, rot = lattice.spherocylinder_smectic(args.n, 0.45)
     pk = ra.GpuPacking(shape, box, pos, rot, seed=args.seed)
-    pk.set_environment(1.0, 2.0, 0.3, None, box_move=("delta_triclinic", 0.05))
+    pk.set_environment(1.0, 2.0, 0.1, None, box_move=("delta_triclinic", 1e-4))
     t0 = time.perf_counter()
     warm = pk.run_cycles(args.warmup, adapt_steps=True)
     log(f"warm-up {args.warmup} cycles in {time.perf_counter() - t0:.2f} s: phi={pk.packing_fraction():.4f} "
@@ -114,6 +136,7 @@ def main():
         f"{after['box_attempted'] - before['box_attempted']}, unstaged blocks {after['unstaged_blocks']}")
 
     # event-timed leg for the roofline (syncs once per cycle, so it is not the throughput measurement)
+    args.timing_cycles = max(args.timing_cycles, 1)
     pk.set_timing(True)
     tb = pk.run_cycles(args.timing_cycles)
     tm = pk.timing()
@@ -122,6 +145,8 @@ def main():
     avg_sweep_ms = tm["sweep_ms"] / max(tm["sweep_launches"], 1)
     bytes_per_launch = (27 + acc_rate) * RECORD_BYTES * moves_per_launch
     achieved = bytes_per_launch / (avg_sweep_ms * 1e-3) / 1e9
+    log(f"launch shape: {tm['sweep_waves']} wavefronts per cell, {tm['sweep_cap']} staged entries, "
+        f"LDS limit {tm['lds_limit']} B")
     log(f"sweep kernel: {tm['sweep_launches']} launches, avg {avg_sweep_ms * 1e3:.1f} us; rebuild avg "
         f"{tm['rebuild_ms'] / max(tm['rebuild_launches'], 1) * 1e3:.1f} us x {tm['rebuild_launches']}; scan avg "
         f"{tm['scan_ms'] / max(tm['scan_launches'], 1) * 1e3:.1f} us x {tm['scan_launches']}")

@@ -92,6 +92,11 @@ typedef struct rp_timing {
     double rebuild_ms;           /* summed duration of the cell-list rebuilds */
     uint64_t scan_launches;
     double scan_ms;
+    uint64_t proposal_launches;
+    double proposal_ms;          /* summed duration of the per-cycle proposal launches */
+    uint64_t lds_limit;          /* LDS bytes a workgroup may use on this device */
+    uint64_t sweep_cap;          /* stencil entries staged per sweep workgroup in the last cycle */
+    uint64_t sweep_waves;        /* wavefronts per active cell in the last cycle */
 } rp_timing;
 
 /* Packing constructor + Simulation constructor (src/core/Packing.cpp:46-82, src/core/Simulation.cpp:54-76):

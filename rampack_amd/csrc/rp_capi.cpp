@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -79,6 +80,9 @@ struct rp_ctx {
     int ncellCap = 0;
     int *tmpSlot = nullptr, *tmpOrder = nullptr;
     double *tmpPos = nullptr;
+    Proposals prop{};
+    size_t propCap = 0;
+    void *propMem = nullptr;
     unsigned long long *dCounters = nullptr;
     unsigned long long *hCounters = nullptr;  // pinned
     uint64_t seed = 0;
@@ -102,6 +106,10 @@ struct rp_ctx {
     uint64_t accSinceEval = 0, attSinceEval = 0;      // host view used by the step adaptation
     uint64_t accBase = 0, attBase = 0;
     uint64_t boxSinceEval = 0, boxAccSinceEval = 0;
+
+    // launch shape
+    size_t ldsLimit = 64 * 1024 - 1024;  // dynamic LDS a workgroup may use
+    int sweepWaves = 0;                  // wavefronts per active cell; 0 = chosen from the stencil size
 
     // timing
     bool timing = false;
@@ -170,6 +178,22 @@ int ensureCells(rp_ctx *c, int ncell) {
     RP_HIP(hipMalloc(&c->cellStart, sizeof(int) * (cap + 1)));
     RP_HIP(hipMalloc(&c->cellCount, sizeof(int) * cap));
     c->ncellCap = cap;
+    return RP_OK;
+}
+
+int ensureProposals(rp_ctx *c, size_t need) {
+    if (need <= c->propCap) return RP_OK;
+    if (c->propMem) hipFree(c->propMem);
+    c->propMem = nullptr;
+    c->propCap = 0;
+    size_t cap = need + need / 4 + 256;
+    RP_HIP(hipMalloc(&c->propMem, cap * (sizeof(int) + 12 * sizeof(double))));
+    char *q = static_cast<char *>(c->propMem);
+    c->prop.t = reinterpret_cast<double *>(q);
+    c->prop.R = c->prop.t + 3 * cap;
+    c->prop.k = reinterpret_cast<int *>(c->prop.R + 9 * cap);
+    c->prop.stride = (int)cap;
+    c->propCap = cap;
     return RP_OK;
 }
 
@@ -257,7 +281,8 @@ int timingHarvest(rp_ctx *c) {
         RP_HIP(hipEventElapsedTime(&ms, c->evStart[i], c->evStop[i]));
         if (c->evKind[i] == 0) { c->tstats.sweep_launches++; c->tstats.sweep_ms += ms; }
         else if (c->evKind[i] == 1) { c->tstats.rebuild_launches++; c->tstats.rebuild_ms += ms; }
-        else { c->tstats.scan_launches++; c->tstats.scan_ms += ms; }
+        else if (c->evKind[i] == 2) { c->tstats.scan_launches++; c->tstats.scan_ms += ms; }
+        else { c->tstats.proposal_launches++; c->tstats.proposal_ms += ms; }
     }
     c->evUsed = 0;
     return RP_OK;
@@ -278,14 +303,23 @@ int readCounters(rp_ctx *c) {
     return RP_OK;
 }
 
+// Largest multiple of 64 entries whose LDS footprint (perEntry bytes each) fits the workgroup limit.
+int capForBytes(rp_ctx *c, double perEntry) { return std::max(64, (int)((double)c->ldsLimit / perEntry) / 64 * 64); }
+
 // Early-exit overlap scan of the `alt` arrays sorted by `g`; sets found.
 int scanOverlaps(rp_ctx *c, const Grid &g, int earlyExit, uint64_t &count) {
     RP_HIP(launch_fill_u64(c->stream, c->dCounters + CNT_OVERLAPS, 1, 0));
     timingBegin(c, 2);
-    RP_HIP(launch_count_overlaps(c->stream, g, (int)c->n, c->alt.p, c->k, earlyExit, c->dCounters));
+    double lambda = (double)c->n / (double)g.ncell;
+    double expect = 14 * lambda;
+    int cap = (int)std::ceil(expect + 8 * std::sqrt(expect) + 64);
+    cap = std::min((cap + 63) / 64 * 64, capForBytes(c, count_lds_bytes(c->k.kind, 64) / 64.0));
+    RP_HIP(launch_count_overlaps(c->stream, g, (int)c->n, c->alt.p, c->k, earlyExit, c->dCounters, cap));
     timingEnd(c);
     int r = readCounters(c);
     if (r) return r;
+    if (c->hCounters[CNT_UNSTAGED_BLOCKS] >> 32)
+        return fail(RP_ERR_UNSUPPORTED, "a grid cell holds more particles than the overlap count can stage");
     count = c->hCounters[CNT_OVERLAPS];
     return RP_OK;
 }
@@ -383,27 +417,47 @@ int runCycle(rp_ctx *c) {
     sp.p = c->cur.p;
     sp.k = c->k;
     double maxT = c->maxTransImplicit ? minHeight(c->box) / 2 : c->maxTransStep;
-    sp.transStep = std::min(maxT, c->transStep);
-    sp.rotStep = std::min(c->rotStep, M_PI);
-    sp.seed = c->seed;
-    sp.cycle = (uint32_t)c->cycle;
+    const double ts = std::min(maxT, c->transStep), rs = std::min(c->rotStep, M_PI);
     for (int d = 0; d < 3; d++) sp.na[d] = g.n[d] / 2;
     double lambda = (double)c->n / (double)g.ncell;
+    // Dense cells make a fixed number of moves each (the reference's floor(N / domains) per domain,
+    // Simulation.cpp:431-433), which balances the workgroups; sparse cells move as many times as they hold
+    // particles, so a cycle is exactly N moves.
+    sp.moves = lambda >= 8 ? (int)std::lround(lambda) : 0;
+    {
+        size_t need = sp.moves > 0 ? (size_t)g.ncell * sp.moves : c->n;
+        int r2 = ensureProposals(c, need);
+        if (r2) return r2;
+    }
+    sp.pr = c->prop;
+    timingBegin(c, 3);
+    RP_HIP(launch_proposals(c->stream, g, sp.pr, sp.moves, c->cur.p.cell, (int)c->n, ts, rs, c->seed,
+                            (uint32_t)c->cycle));
+    timingEnd(c);
     double expect = 27 * lambda;
     int cap = (int)std::ceil(expect + 8 * std::sqrt(expect) + 64);
     cap = (cap + 63) / 64 * 64;
-    const int maxCap = (int)(60 * 1024 / sweep_lds_bytes(c->k.kind, 1)) / 64 * 64;
-    sp.cap = std::min(cap, maxCap);
+    int waves = c->sweepWaves;
+    if (waves <= 0) waves = expect > 384 ? 4 : (expect > 128 ? 2 : 1);
+    waves = waves >= 4 ? 4 : (waves == 2 ? 2 : 1);
+    const double perEntry =
+        (double)(sweep_lds_bytes(c->k.kind, 1024, waves) - sweep_lds_bytes(c->k.kind, 0, waves)) / 1024.0;
+    const size_t fixed = sweep_lds_bytes(c->k.kind, 0, waves);
+    int maxCap = (int)((double)(c->ldsLimit - fixed) / perEntry) / 64 * 64;
+    sp.cap = std::max(64, std::min(cap, maxCap));
     sp.counters = c->dCounters;
     int order[8] = {0, 1, 2, 3, 4, 5, 6, 7};
     for (int i = 7; i > 0; i--) {
         int j = (int)philox_below(c->hostRng, (uint32_t)(i + 1));
         std::swap(order[i], order[j]);
     }
+    c->tstats.lds_limit = c->ldsLimit;
+    c->tstats.sweep_cap = (uint64_t)sp.cap;
+    c->tstats.sweep_waves = (uint64_t)waves;
     for (int i = 0; i < 8; i++) {
         sp.colour = order[i];
         timingBegin(c, 0);
-        RP_HIP(launch_sweep(c->stream, sp));
+        RP_HIP(launch_sweep(c->stream, sp, waves));
         timingEnd(c);
     }
     c->sortedForCycle = false;
@@ -503,6 +557,10 @@ int rp_create(rp_ctx **out, const rp_shape_desc *shape, const double box[9], siz
     if (hipGetDevice(&dev) != hipSuccess) return fail(RP_ERR_DEVICE, "no HIP device");
 
     rp_ctx *c = new rp_ctx();
+    int ldsBytes = 0;
+    if (hipDeviceGetAttribute(&ldsBytes, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && ldsBytes > 0)
+        c->ldsLimit = (size_t)std::min(ldsBytes, 160 * 1024) - 1024;
+    if (const char *w = std::getenv("RAMPACK_SWEEP_WAVES")) c->sweepWaves = std::atoi(w);
     c->shape = *shape;
     c->k = makeConsts(*shape);
     c->range = shapeRange(*shape);
@@ -665,7 +723,11 @@ int rp_set_timing(rp_ctx *c, int enable) {
     int r = timingHarvest(c);
     if (r) return r;
     c->timing = enable != 0;
+    rp_timing keep = c->tstats;
     c->tstats = rp_timing{};
+    c->tstats.lds_limit = keep.lds_limit;
+    c->tstats.sweep_cap = keep.sweep_cap;
+    c->tstats.sweep_waves = keep.sweep_waves;
     return RP_OK;
 }
 
@@ -686,6 +748,7 @@ void rp_destroy(rp_ctx *c) {
     if (c->alt.mem) hipFree(c->alt.mem);
     if (c->cellStart) hipFree(c->cellStart);
     if (c->cellCount) hipFree(c->cellCount);
+    if (c->propMem) hipFree(c->propMem);
     if (c->tmpSlot) hipFree(c->tmpSlot);
     if (c->tmpOrder) hipFree(c->tmpOrder);
     if (c->tmpPos) hipFree(c->tmpPos);

@@ -33,13 +33,20 @@ struct Grid {
     int *cellCount;         // ncell
 };
 
+// Trial-move proposals of one cycle, one per (cell, move), structure of arrays with `stride` entries per field.
+struct Proposals {
+    int *k;         // particle index within the cell
+    double *t;      // translation, 3 x stride
+    double *R;      // rotation matrix, row-major, 9 x stride
+    int stride;
+};
+
 struct SweepParams {
     Grid g;
     Particles p;
     PairConsts k;
-    double transStep, rotStep;
-    uint64_t seed;
-    uint32_t cycle;
+    Proposals pr;
+    int moves;              // trial moves per active cell and colour pass; 0: as many as the cell holds
     int colour;
     int na[3];              // active cells per axis (n / 2)
     int cap;                // stencil entries that fit in LDS
@@ -66,10 +73,16 @@ hipError_t launch_fill_i32(hipStream_t s, int *p, size_t n, int v);
 hipError_t launch_rebuild(hipStream_t s, const Grid &g, int n, Particles src, Particles dst, int *tmpSlot,
                           int *tmpOrder, double *tmpPos, int rescale, const double *oldInv);
 // Half-shell overlap count over the grid (Packing.cpp:431-559).  Adds to counters[CNT_OVERLAPS]; with
-// earlyExit the threads stop once any overlap is found.
+// earlyExit the workgroups stop once any overlap is found.  cap: LDS entries per workgroup (>= largest cell).
 hipError_t launch_count_overlaps(hipStream_t s, const Grid &g, int n, Particles p, PairConsts k, int earlyExit,
-                                 unsigned long long *counters);
-hipError_t launch_sweep(hipStream_t s, const SweepParams &sp);
-size_t sweep_lds_bytes(int kind, int cap);
+                                 unsigned long long *counters, int cap);
+size_t count_lds_bytes(int kind, int cap);
+// Proposals of one cycle (translation step ts, rotation step rs): `moves` per cell when moves > 0, else one per
+// particle of its cell (cellOf = cell of every sorted record, n records).
+hipError_t launch_proposals(hipStream_t s, const Grid &g, const Proposals &pr, int moves, const int *cellOf, int n,
+                            double ts, double rs, uint64_t seed, uint32_t cycle);
+// waves: wavefronts per active cell (1, 2 or 4)
+hipError_t launch_sweep(hipStream_t s, const SweepParams &sp, int waves);
+size_t sweep_lds_bytes(int kind, int cap, int waves);
 
 }  // namespace rp

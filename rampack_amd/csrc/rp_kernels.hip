@@ -7,10 +7,12 @@
 //                        counting-sort cell-list rebuild over coalesced SoA records, replacing the linked-list
 //                        NeighbourGrid insertion (Packing.cpp:858-929, NeighbourGrid.cpp:219-227); the order
 //                        inside a cell is the stable order of the source arrays, so rebuilds are deterministic
-//   * k_count_overlaps   half-shell overlap count with early exit (Packing.cpp:431-559)
-//   * k_sweep            checkerboard MC sweep: one wavefront per active cell, the 27-cell stencil staged in
-//                        LDS, early-exit overlap voting by wavefront ballot, a Philox stream per cell, moves
-//                        that leave the cell rejected (the ActiveDomain rule of Packing.cpp:147-148)
+//   * k_count_overlaps   half-shell overlap count with early exit (Packing.cpp:431-559), one workgroup per cell
+//                        with the cell and its 13 positive neighbours staged in LDS
+//   * k_sweep            checkerboard MC sweep: one workgroup (1, 2 or 4 wavefronts) per active cell, the 27-cell
+//                        stencil staged in LDS, early-exit overlap voting after every 64*waves candidates, a Philox
+//                        stream per move, moves that leave the cell rejected (the ActiveDomain rule of
+//                        Packing.cpp:147-148)
 #include "rp_kernels.h"
 
 #include <cfloat>
@@ -45,7 +47,7 @@ __device__ __forceinline__ void bin_position(const Grid &g, const V3 &pos, int c
         } else {
             int w = x < g.shift[d] ? 1 : 0;
             double y = (x - g.shift[d]) + (w ? 1.0 : 0.0);
-            cc = (int)(y / g.relCell[d]);
+            cc = (int)(y * (double)n);
             if (w) cc = n - 1;
             wbits |= (unsigned)w << d;
         }
@@ -66,11 +68,6 @@ __device__ __forceinline__ int wrap_axis(int c, int o, int n, int &e) {
     if (cc < 0) { cc += n; e = -1; }
     else if (cc >= n) { cc -= n; e = 1; }
     return cc;
-}
-
-__device__ __forceinline__ V3 translation(const Grid &g, int kx, int ky, int kz) {
-    int t = 3 * ((kx + 1) * 9 + (ky + 1) * 3 + (kz + 1));
-    return v3(g.T[t], g.T[t + 1], g.T[t + 2]);
 }
 
 template <int KIND>
@@ -269,102 +266,21 @@ hipError_t launch_rebuild(hipStream_t s, const Grid &g, int n, Particles src, Pa
 }
 
 // ---------------------------------------------------------------------------------------------------------------
-// Half-shell overlap count (Packing::countTotalOverlapsNGCellHelper, Packing.cpp:431-517).  Within a cell the
-// particle with the higher caller index is the first operand, as in the reference's LIFO cell lists; across cells
-// the first operand lies in the base cell and the second in one of the 13 "positive" neighbours
-// (9*n0 + 3*n1 + n2 > 13, NeighbourGrid.cpp:130-131).
+// Shared pieces of the stencil kernels
 
-template <int KIND>
-__global__ __launch_bounds__(256) void k_count_overlaps(Grid g, int n, Particles p, PairConsts k, int earlyExit,
-                                                        unsigned long long *counters) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    volatile unsigned long long *flag = counters + CNT_OVERLAPS;
-    if (earlyExit && *flag != 0) return;
-    int cell = p.cell[i];
-    int cx = cell % g.n[0], cy = (cell / g.n[0]) % g.n[1], cz = cell / (g.n[0] * g.n[1]);
-    int wi = p.wbits[i];
-    int tagi = p.tag[i];
-    V3 pos1 = load3(p.x, p.y, p.z, i);
-    V3 ax1 = KIND == 1 ? v3(p.o[2][i], p.o[5][i], p.o[8][i]) : v3(0, 0, 0);
-    SCFirst f = make_first<KIND>(k, pos1, ax1);
-    unsigned long long found = 0;
-    // own cell
-    {
-        int b = g.cellStart[cell], e = g.cellStart[cell + 1];
-        for (int j = b; j < e; j++) {
-            if (p.tag[j] >= tagi) continue;
-            int wj = p.wbits[j];
-            V3 T = v3(0, 0, 0);
-            if (wj != wi)
-                T = translation(g, (wj & 1) - (wi & 1), ((wj >> 1) & 1) - ((wi >> 1) & 1),
-                                ((wj >> 2) & 1) - ((wi >> 2) & 1));
-            V3 p2 = load3(p.x, p.y, p.z, j);
-            V3 a2 = KIND == 1 ? v3(p.o[2][j], p.o[5][j], p.o[8][j]) : v3(0, 0, 0);
-            if (pair_overlap<KIND>(k, f, p2, a2, T)) {
-                found++;
-                if (earlyExit) break;
-            }
-        }
-    }
-    for (int q = 14; q < 27 && !(earlyExit && found); q++) {
-        if (earlyExit && *flag != 0) break;
-        int ex, ey, ez;
-        int nx = wrap_axis(cx, q / 9 - 1, g.n[0], ex);
-        int ny = wrap_axis(cy, (q / 3) % 3 - 1, g.n[1], ey);
-        int nz = wrap_axis(cz, q % 3 - 1, g.n[2], ez);
-        int nc = cell_index(g, nx, ny, nz);
-        int b = g.cellStart[nc], e = g.cellStart[nc + 1];
-        for (int j = b; j < e; j++) {
-            int wj = p.wbits[j];
-            V3 T = translation(g, ex + (wj & 1) - (wi & 1), ey + ((wj >> 1) & 1) - ((wi >> 1) & 1),
-                               ez + ((wj >> 2) & 1) - ((wi >> 2) & 1));
-            V3 p2 = load3(p.x, p.y, p.z, j);
-            V3 a2 = KIND == 1 ? v3(p.o[2][j], p.o[5][j], p.o[8][j]) : v3(0, 0, 0);
-            if (pair_overlap<KIND>(k, f, p2, a2, T)) {
-                found++;
-                if (earlyExit) break;
-            }
-        }
-    }
-    if (found) atomicAdd(&counters[CNT_OVERLAPS], found);
-}
-
-hipError_t launch_count_overlaps(hipStream_t s, const Grid &g, int n, Particles p, PairConsts k, int earlyExit,
-                                 unsigned long long *counters) {
-    if (n == 0) return hipSuccess;
-    unsigned blocks = (unsigned)((n + 255) / 256);
-    if (k.kind == 0)
-        hipLaunchKernelGGL(k_count_overlaps<0>, dim3(blocks), dim3(256), 0, s, g, n, p, k, earlyExit, counters);
-    else
-        hipLaunchKernelGGL(k_count_overlaps<1>, dim3(blocks), dim3(256), 0, s, g, n, p, k, earlyExit, counters);
-    return hipGetLastError();
-}
-
-// ---------------------------------------------------------------------------------------------------------------
-// Checkerboard sweep.  Block = one wavefront = one active cell of colour `colour` (cells with coordinates of the
-// colour's parity on every axis; same-colour cells are separated by a full inactive cell, wider than the
-// interaction range).  The cell's particles are moved n_cell times (uniform random pick, rototranslation as
-// RototranslationSampler.cpp:22-49), every candidate of the 27-cell stencil is tested from LDS, 64 candidates
-// per step, and the first step with any overlap rejects the move.
-
-size_t sweep_lds_bytes(int kind, int cap) {
-    size_t perEntry = (kind == 0 ? 3 : 6) * sizeof(double) + sizeof(int);
-    return perEntry * (size_t)cap;
-}
-
-// Stencil view of one block: entry j of the 27 stencil cells (own cell first) either staged in LDS or read from
-// the cell-sorted arrays in HBM when the stencil does not fit the LDS area.
+// Stencil view of one block: entries of up to 27 stencil cells (own cell first) either staged in LDS or read from
+// the cell-sorted arrays in HBM when they do not fit the LDS area.
 struct Stencil {
-    const int *start;   // 28 prefix offsets (LDS)
-    const int *base;    // 27 first record indices (LDS)
-    const int *ecode;   // 27 packed cell image shifts (LDS)
+    const int *start;   // prefix offsets of the stencil cells (LDS)
+    const int *base;    // first record index of each stencil cell (LDS)
+    const int *ecode;   // packed image shift of each stencil cell (LDS)
+    int ncells;
     double *x, *y, *z, *ax, *ay, *az;
     int *code;
 };
 
-__device__ __forceinline__ int stencil_cell(const int *start, int j) {
-    int lo = 0, hi = 27;
+__device__ __forceinline__ int stencil_cell(const int *start, int ncells, int j) {
+    int lo = 0, hi = ncells;
     while (hi - lo > 1) {
         int mid = (lo + hi) >> 1;
         if (start[mid] <= j) lo = mid; else hi = mid;
@@ -373,141 +289,557 @@ __device__ __forceinline__ int stencil_cell(const int *start, int j) {
 }
 
 // Per-axis image code e + w_j + 1 (two bits per axis) of a record with wrap bits w in a stencil cell with packed
-// image shift ec.
+// image shift ec (e + 1 per axis).
 __device__ __forceinline__ int image_code(int ec, int w) {
     return (((ec & 3) - 1 + (w & 1)) + 1) | ((((ec >> 2) & 3) - 1 + ((w >> 1) & 1) + 1) << 2) |
            ((((ec >> 4) & 3) - 1 + ((w >> 2) & 1) + 1) << 4);
 }
 
+// Translation box * (code - 1 - w) from the LDS copy of the reference's 27-entry translation table.
+__device__ __forceinline__ V3 image_translation(const double *sT, int code, int wx, int wy, int wz) {
+    int kx = (code & 3) - 1 - wx, ky = ((code >> 2) & 3) - 1 - wy, kz = ((code >> 4) & 3) - 1 - wz;
+    int t = 3 * ((kx + 1) * 9 + (ky + 1) * 3 + (kz + 1));
+    return v3(sT[t], sT[t + 1], sT[t + 2]);
+}
+
 template <int KIND, bool STAGED>
-__device__ __forceinline__ void load_candidate(const SweepParams &P, const Stencil &S, int j, V3 &p2, V3 &a2,
+__device__ __forceinline__ void load_candidate(const Particles &p, const Stencil &S, int j, V3 &p2, V3 &a2,
                                                int &code) {
     if (STAGED) {
         p2 = v3(S.x[j], S.y[j], S.z[j]);
         a2 = KIND == 1 ? v3(S.ax[j], S.ay[j], S.az[j]) : v3(0, 0, 0);
         code = S.code[j];
     } else {
-        int q = stencil_cell(S.start, j);
+        int q = stencil_cell(S.start, S.ncells, j);
         int gi = S.base[q] + (j - S.start[q]);
-        p2 = load3(P.p.x, P.p.y, P.p.z, gi);
-        a2 = KIND == 1 ? v3(P.p.o[2][gi], P.p.o[5][gi], P.p.o[8][gi]) : v3(0, 0, 0);
-        code = image_code(S.ecode[q], P.p.wbits[gi]);
+        p2 = load3(p.x, p.y, p.z, gi);
+        a2 = KIND == 1 ? v3(p.o[2][gi], p.o[5][gi], p.o[8][gi]) : v3(0, 0, 0);
+        code = image_code(S.ecode[q], p.wbits[gi]);
     }
 }
 
-// The trial moves of one active cell: n_cell moves, each a uniform pick among the cell's particles and a
-// rototranslation (RototranslationSampler.cpp:22-49), rejected if the particle leaves the cell, otherwise tested
-// against the stencil 64 candidates at a time with a wavefront vote after every step.
-template <int KIND, bool STAGED>
-__device__ void sweep_cell(const SweepParams &P, const Stencil &S, int cell, int own0, int nOwn, int total) {
-    const Grid &g = P.g;
-    const int lane = threadIdx.x;
-    Philox rng = philox_init(P.seed, (P.cycle << 3) | (uint32_t)P.colour, (uint32_t)cell, 0x5eed5u);
-    unsigned long long accepted = 0, leftCell = 0, tests = 0;
-    const double ts = P.transStep, rs = P.rotStep;
-
-    for (int m = 0; m < nOwn; m++) {
-        int kIdx = (int)philox_below(rng, (uint32_t)nOwn);
-        V3 t;
-        t.x = philox_uniform(rng, -ts, ts);
-        t.y = philox_uniform(rng, -ts, ts);
-        t.z = philox_uniform(rng, -ts, ts);
-        V3 axis;
-        do {
-            axis.x = philox_uniform(rng, -1, 1);
-            axis.y = philox_uniform(rng, -1, 1);
-            axis.z = philox_uniform(rng, -1, 1);
-        } while (dot(axis, axis) > 1 || dot(axis, axis) == 0);
-        double angle = philox_uniform(rng, -rs, rs);
-
-        const int gi = own0 + kIdx;
-        V3 pos = STAGED ? v3(S.x[kIdx], S.y[kIdx], S.z[kIdx]) : load3(P.p.x, P.p.y, P.p.z, gi);
-        V3 npos = translate_wrapped(g.dims, g.inv, pos, t);
-        int nc[3];
-        unsigned nw;
-        bin_position(g, npos, nc, nw);
-        if (cell_index(g, nc[0], nc[1], nc[2]) != cell) {
-            leftCell++;
-            continue;
-        }
-
-        double O[9], R[9], NO[9];
-        for (int c = 0; c < 9; c++) O[c] = P.p.o[c][gi];
-        // Matrix<3,3>::rotation(axis.normalized(), angle) (Matrix.tpp:438-450), then Shape::rotate: R * O
-        double nrm = sqrt(dot(axis, axis));
-        V3 ua = v3(axis.x / nrm, axis.y / nrm, axis.z / nrm);
-        double sa = sin(angle), ca = cos(angle);
-        double K[9] = {0, -ua.z, ua.y, ua.z, 0, -ua.x, -ua.y, ua.x, 0};
-        double f1 = 1 - ca;
-        double fK[9];
-        for (int c = 0; c < 9; c++) fK[c] = K[c] * f1;
-        double fKK[9];
-        mulm(fK, K, fKK);
-        for (int c = 0; c < 9; c++) R[c] = ((c % 4 == 0 ? 1.0 : 0.0) + K[c] * sa) + fKK[c];
-        mulm(R, O, NO);
-        V3 nax = v3(NO[2], NO[5], NO[8]);
-        SCFirst f = make_first<KIND>(P.k, npos, nax);
-        const int wx = nw & 1, wy = (nw >> 1) & 1, wz = (nw >> 2) & 1;
-
-        bool hit = false;
-        for (int base = 0; base < total; base += WAVE) {
-            int j = base + lane;
-            bool h = false;
-            if (j < total && j != kIdx) {
-                V3 p2, a2;
-                int code;
-                load_candidate<KIND, STAGED>(P, S, j, p2, a2, code);
-                V3 T = translation(g, (code & 3) - 1 - wx, ((code >> 2) & 3) - 1 - wy, ((code >> 4) & 3) - 1 - wz);
-                h = pair_overlap<KIND>(P.k, f, p2, a2, T);
-            }
-            tests += (unsigned long long)min(WAVE, total - base);
-            if (__any(h)) {
-                hit = true;
-                break;
-            }
-        }
-        if (hit) continue;
-
-        // Packing::acceptMove: the cell belongs to this wavefront for the whole colour pass
-        accepted++;
-        if (lane == 0) {
-            P.p.x[gi] = npos.x;
-            P.p.y[gi] = npos.y;
-            P.p.z[gi] = npos.z;
-            for (int c = 0; c < 9; c++) P.p.o[c][gi] = NO[c];
-            P.p.wbits[gi] = (uint8_t)nw;
-            if (STAGED) {
-                S.x[kIdx] = npos.x;
-                S.y[kIdx] = npos.y;
-                S.z[kIdx] = npos.z;
-                if (KIND == 1) {
-                    S.ax[kIdx] = nax.x;
-                    S.ay[kIdx] = nax.y;
-                    S.az[kIdx] = nax.z;
-                }
-                S.code[kIdx] = (wx + 1) | ((wy + 1) << 2) | ((wz + 1) << 4);
-            }
-        }
-        __syncthreads();
-    }
-    if (lane == 0) {
-        atomicAdd(&P.counters[CNT_ACCEPTED], accepted);
-        atomicAdd(&P.counters[CNT_ATTEMPTED], (unsigned long long)nOwn);
-        atomicAdd(&P.counters[CNT_LEFT_CELL], leftCell);
-        atomicAdd(&P.counters[CNT_TESTS], tests);
-        if (!STAGED) atomicAdd(&P.counters[CNT_UNSTAGED_BLOCKS], 1ull);
-    }
+// Fills the stencil cell table for `ncells` offsets of cell (cx, cy, cz); offset list: own cell first, then either
+// all 26 neighbours (sweep) or the 13 "positive" ones, 9*n0 + 3*n1 + n2 > 13 (NeighbourGrid.cpp:130-131).
+__device__ __forceinline__ void stencil_table(const Grid &g, int cx, int cy, int cz, int slot, bool half,
+                                              int *s_start, int *s_base, int *s_ecode) {
+    int t;
+    if (slot == 0) t = 13;
+    else if (half) t = 13 + slot;
+    else t = slot <= 13 ? slot - 1 : slot;
+    // t = 9*n0 + 3*n1 + n2 with n = offset + 1 (x most significant, as the reference orders its neighbours)
+    int ex, ey, ez;
+    int nx = wrap_axis(cx, t / 9 - 1, g.n[0], ex);
+    int ny = wrap_axis(cy, (t / 3) % 3 - 1, g.n[1], ey);
+    int nz = wrap_axis(cz, t % 3 - 1, g.n[2], ez);
+    int nc = cell_index(g, nx, ny, nz);
+    s_base[slot] = g.cellStart[nc];
+    s_start[slot + 1] = g.cellStart[nc + 1] - g.cellStart[nc];
+    s_ecode[slot] = (ex + 1) | ((ey + 1) << 2) | ((ez + 1) << 4);
 }
 
 template <int KIND>
-__global__ __launch_bounds__(WAVE) void k_sweep(SweepParams P) {
+__device__ __forceinline__ void stage_stencil(const Particles &p, const Stencil &S, int total, int tid, int nt) {
+    for (int j = tid; j < total; j += nt) {
+        int q = stencil_cell(S.start, S.ncells, j);
+        int gi = S.base[q] + (j - S.start[q]);
+        S.x[j] = p.x[gi];
+        S.y[j] = p.y[gi];
+        S.z[j] = p.z[gi];
+        if (KIND == 1) {
+            S.ax[j] = p.o[2][gi];
+            S.ay[j] = p.o[5][gi];
+            S.az[j] = p.o[8][gi];
+        }
+        S.code[j] = image_code(S.ecode[q], p.wbits[gi]);
+    }
+}
+
+__device__ __forceinline__ void carve_stencil(Stencil &S, double *smem, int cap, int kind, double *&rest) {
+    S.x = smem;
+    S.y = S.x + cap;
+    S.z = S.y + cap;
+    S.ax = S.z + cap;
+    S.ay = S.ax + cap;
+    S.az = S.ay + cap;
+    double *after = kind == 0 ? S.ax : S.az + cap;
+    S.code = reinterpret_cast<int *>(after);
+    rest = after + (cap + 1) / 2;
+}
+
+size_t stencil_lds_bytes(int kind, int cap) {
+    return (size_t)(kind == 0 ? 3 : 6) * sizeof(double) * cap + sizeof(double) * ((cap + 1) / 2);
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Half-shell overlap count (Packing::countTotalOverlapsNGCellHelper, Packing.cpp:431-517): one workgroup per cell
+// stages the cell and its 13 positive neighbours in LDS.  Within a cell the particle with the higher caller index
+// is the first operand, as in the reference's LIFO cell lists; across cells the first operand lies in the base
+// cell.
+
+constexpr int COUNT_THREADS = 256;
+
+template <int KIND, bool STAGED>
+__device__ void count_cell(const Grid &g, const Particles &p, const PairConsts &k, const Stencil &S, const double *sT,
+                           const int *sTag, int nOwn, int total, int earlyExit, unsigned long long *counters,
+                           int *sFound) {
+    const int tid = threadIdx.x;
+    volatile unsigned long long *flag = counters + CNT_OVERLAPS;
+    unsigned long long found = 0;
+    for (int a = 0; a < nOwn; a++) {
+        V3 pos1, ax1;
+        int c1;
+        load_candidate<KIND, STAGED>(p, S, a, pos1, ax1, c1);
+        const int wx = (c1 & 3) - 1, wy = ((c1 >> 2) & 3) - 1, wz = ((c1 >> 4) & 3) - 1;
+        const int tagA = sTag[a];
+        SCFirst f = make_first<KIND>(k, pos1, ax1);
+        for (int b = tid; b < total; b += COUNT_THREADS) {
+            if (b < nOwn && sTag[b] >= tagA) continue;
+            V3 p2, a2;
+            int code;
+            load_candidate<KIND, STAGED>(p, S, b, p2, a2, code);
+            V3 T = b < nOwn && code == c1 ? v3(0, 0, 0) : image_translation(sT, code, wx, wy, wz);
+            if (pair_overlap<KIND>(k, f, p2, a2, T)) found++;
+        }
+        if (earlyExit) {
+            int any = __syncthreads_or(found != 0 || *flag != 0);
+            if (any) break;
+        }
+    }
+    if (found) atomicAdd(sFound, (int)found);
+    __syncthreads();
+    if (tid == 0 && *sFound) atomicAdd(&counters[CNT_OVERLAPS], (unsigned long long)*sFound);
+}
+
+template <int KIND>
+__global__ __launch_bounds__(COUNT_THREADS) void k_count_overlaps(Grid g, Particles p, PairConsts k, int earlyExit,
+                                                                  unsigned long long *counters, int cap) {
+    extern __shared__ double smem[];
+    __shared__ int s_start[15], s_base[14], s_ecode[14];
+    __shared__ int sFound;
+    __shared__ double sT[81];
+    const int tid = threadIdx.x;
+    const int cell = blockIdx.x;
+    const int own0 = g.cellStart[cell];
+    const int nOwn = g.cellStart[cell + 1] - own0;
+    if (nOwn == 0) return;
+    volatile unsigned long long *flag = counters + CNT_OVERLAPS;
+    if (earlyExit && *flag != 0) return;
+    const int cx = cell % g.n[0], cy = (cell / g.n[0]) % g.n[1], cz = cell / (g.n[0] * g.n[1]);
+    if (tid < 14) stencil_table(g, cx, cy, cz, tid, true, s_start, s_base, s_ecode);
+    for (int i = tid; i < 81; i += blockDim.x) sT[i] = g.T[i];
+    if (tid == 0) sFound = 0;
+    __syncthreads();
+    if (tid == 0) {
+        s_start[0] = 0;
+        for (int q = 1; q <= 14; q++) s_start[q] += s_start[q - 1];
+    }
+    __syncthreads();
+    const int total = s_start[14];
+    Stencil S;
+    S.start = s_start;
+    S.base = s_base;
+    S.ecode = s_ecode;
+    S.ncells = 14;
+    double *rest;
+    carve_stencil(S, smem, cap, KIND, rest);
+    int *sTag = reinterpret_cast<int *>(rest);  // own-cell tags, up to cap entries
+    for (int a = tid; a < nOwn && a < cap; a += COUNT_THREADS) sTag[a] = p.tag[own0 + a];
+    if (total <= cap) {
+        stage_stencil<KIND>(p, S, total, tid, COUNT_THREADS);
+        __syncthreads();
+        count_cell<KIND, true>(g, p, k, S, sT, sTag, nOwn, total, earlyExit, counters, &sFound);
+    } else if (nOwn <= cap) {
+        __syncthreads();
+        count_cell<KIND, false>(g, p, k, S, sT, sTag, nOwn, total, earlyExit, counters, &sFound);
+    } else if (tid == 0) {
+        // a cell holding more than `cap` particles: the host sizes cap from the largest cell, so this marks an error
+        atomicAdd(&counters[CNT_UNSTAGED_BLOCKS], 1ull << 32);
+    }
+}
+
+size_t count_lds_bytes(int kind, int cap) { return stencil_lds_bytes(kind, cap) + sizeof(int) * cap; }
+
+hipError_t launch_count_overlaps(hipStream_t s, const Grid &g, int n, Particles p, PairConsts k, int earlyExit,
+                                 unsigned long long *counters, int cap) {
+    if (n == 0) return hipSuccess;
+    size_t lds = count_lds_bytes(k.kind, cap);
+    if (k.kind == 0)
+        hipLaunchKernelGGL(k_count_overlaps<0>, dim3((unsigned)g.ncell), dim3(COUNT_THREADS), lds, s, g, p, k,
+                           earlyExit, counters, cap);
+    else
+        hipLaunchKernelGGL(k_count_overlaps<1>, dim3((unsigned)g.ncell), dim3(COUNT_THREADS), lds, s, g, p, k,
+                           earlyExit, counters, cap);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Checkerboard sweep.  Workgroup = one active cell of colour `colour` (cells with coordinates of the colour's
+// parity on every axis; same-colour cells are separated by a full inactive cell, wider than the interaction
+// range).  The cell's particles are moved n_cell times: uniform random pick + rototranslation
+// (RototranslationSampler.cpp:22-49).  Proposals are drawn in parallel (one lane per move, its own Philox
+// stream), then applied in order; each trial pose is tested against the 27-cell stencil 64*WAVES candidates per
+// step with a workgroup vote after every step, and the first step with an overlap rejects the move.
+
+constexpr int OWN_CAP = 64;   // own-cell particles kept in LDS
+constexpr int MOVE_CHUNK = 64; // proposals staged per batch
+constexpr int SLOT_GROUP = 4;  // candidate slots loaded per lane before any is tested
+
+// Proposals of one cycle for every (cell, move): drawn in one fully parallel launch before the colour passes, one
+// Philox stream per (cycle, cell, move) -- RototranslationSampler::sampleMove (RototranslationSampler.cpp:22-49):
+// particle pick among the cell's particles, translation in [-t, t)^3, axis by rejection in the unit ball, angle in
+// [-r, r), rotation by Rodrigues' formula (Matrix.tpp:438-450).
+__device__ __forceinline__ void rodrigues(V3 axis, double angle, double *R) {
+    double nrm = sqrt(dot(axis, axis));
+    V3 ua = v3(axis.x / nrm, axis.y / nrm, axis.z / nrm);
+    double sa = sin(angle), ca = cos(angle);
+    double K[9] = {0, -ua.z, ua.y, ua.z, 0, -ua.x, -ua.y, ua.x, 0};
+    double f1 = 1 - ca;
+    double fK[9];
+    for (int c = 0; c < 9; c++) fK[c] = K[c] * f1;
+    double fKK[9];
+    mulm(fK, K, fKK);
+    for (int c = 0; c < 9; c++) R[c] = ((c % 4 == 0 ? 1.0 : 0.0) + K[c] * sa) + fKK[c];
+}
+
+// moves > 0: `moves` proposals per cell at [cell * moves, ...); moves == 0: one per particle, cell c's at
+// [cellStart[c], cellStart[c + 1]) (the caller passes the cell of every sorted record in `cellOf`).
+__global__ __launch_bounds__(256) void k_proposals(Grid g, Proposals pr, int moves, const int *__restrict__ cellOf,
+                                                   int n, double ts, double rs, uint64_t seed, uint32_t cycle) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int cell, m;
+    if (moves > 0) {
+        if (i >= g.ncell * moves) return;
+        cell = i / moves;
+        m = i - cell * moves;
+    } else {
+        if (i >= n) return;
+        cell = cellOf[i];
+        m = i - g.cellStart[cell];
+    }
+    const int nOwn = g.cellStart[cell + 1] - g.cellStart[cell];
+    if (nOwn == 0) return;
+    Philox rng = philox_init(seed, cycle, (uint32_t)cell, (uint32_t)m);
+    pr.k[i] = (int)philox_below(rng, (uint32_t)nOwn);
+    for (int d = 0; d < 3; d++) pr.t[d * pr.stride + i] = philox_uniform(rng, -ts, ts);
+    V3 axis;
+    do {
+        axis.x = philox_uniform(rng, -1, 1);
+        axis.y = philox_uniform(rng, -1, 1);
+        axis.z = philox_uniform(rng, -1, 1);
+    } while (dot(axis, axis) > 1 || dot(axis, axis) == 0);
+    double R[9];
+    rodrigues(axis, philox_uniform(rng, -rs, rs), R);
+    for (int c = 0; c < 9; c++) pr.R[c * pr.stride + i] = R[c];
+}
+
+hipError_t launch_proposals(hipStream_t s, const Grid &g, const Proposals &pr, int moves, const int *cellOf, int n,
+                            double ts, double rs, uint64_t seed, uint32_t cycle) {
+    size_t total = moves > 0 ? (size_t)g.ncell * moves : (size_t)n;
+    if (total == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_proposals, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, g, pr, moves, cellOf, n,
+                       ts, rs, seed, cycle);
+    return hipGetLastError();
+}
+
+// LDS of one sweep workgroup: the stencil (cap entries), the ambiguous-candidate queues (cap + 64*waves ints),
+// own orientations, one batch of proposals and the translation table.
+size_t sweep_lds_bytes(int kind, int cap, int waves) {
+    return stencil_lds_bytes(kind, cap) + sizeof(int) * (cap + WAVE * waves) +
+           sizeof(double) * (9 * OWN_CAP + 12 * MOVE_CHUNK + 81) + sizeof(int) * MOVE_CHUNK;
+}
+
+__device__ __forceinline__ void flush_counters(const SweepParams &P, unsigned long long accepted, int moves,
+                                               unsigned long long leftCell, unsigned long long tests, bool staged) {
+    if (threadIdx.x == 0) {
+        atomicAdd(&P.counters[CNT_ACCEPTED], accepted);
+        atomicAdd(&P.counters[CNT_ATTEMPTED], (unsigned long long)moves);
+        atomicAdd(&P.counters[CNT_LEFT_CELL], leftCell);
+        atomicAdd(&P.counters[CNT_TESTS], tests);
+        if (!staged) atomicAdd(&P.counters[CNT_UNSTAGED_BLOCKS], 1ull);
+    }
+}
+
+// Copies proposals [first, first + cn) of this cell into LDS.
+__device__ __forceinline__ void load_proposals(const SweepParams &P, int pbase, int first, int cn, double *pt,
+                                               double *pR, int *pk) {
+    const int tid = threadIdx.x;
+    if (tid < cn) {
+        const int i = pbase + first + tid;
+        pk[tid] = P.pr.k[i];
+        for (int d = 0; d < 3; d++) pt[d * MOVE_CHUNK + tid] = P.pr.t[d * P.pr.stride + i];
+        for (int c = 0; c < 9; c++) pR[c * MOVE_CHUNK + tid] = P.pr.R[c * P.pr.stride + i];
+    }
+}
+
+// One block-wide vote per move with a single barrier: a wavefront that saw a hit stores the move's tag in the
+// move's slot (two slots alternate; every move passes exactly one barrier, so a slot is rewritten only after
+// every lane has read it).
+__device__ __forceinline__ bool block_vote(bool hit, int move, volatile int *slots) {
+    if (__any(hit) && (threadIdx.x & (WAVE - 1)) == 0) slots[move & 1] = move + 1;
+    __syncthreads();
+    return slots[move & 1] == move + 1;
+}
+
+// Staged sweep of one active cell.  TRANSLATED: the cell is not in the last row of any axis, so no own particle
+// has a wrap bit set and every candidate was staged as pos + T (the reference's pos2bc, rounded once), so the
+// trial pose is compared against the staged positions directly.  Each trial move:
+//   pass A: every lane loads its candidates, computes the centre distance^2 exactly as
+//           SpherocylinderTraits::overlapBetween does, marks definite overlaps (d^2 < 4r^2) and appends the
+//           ambiguous ones (4r^2 <= d^2 < (2r+L)^2) to its wavefront's LDS queue;
+//   pass B: the wavefront runs the exact segment-distance test on its queue (skipped once it saw a definite hit);
+//   one workgroup vote decides the move.  An accepted move is written into LDS by every lane (same values), so
+//   no second barrier is needed before the next move reads it.
+template <int KIND, int WAVES, bool TRANSLATED>
+__device__ void sweep_cell_staged(const SweepParams &P, const Stencil &S, double *oo, double *pt, double *pR,
+                                  int *pk, int *queue, const double *sT, volatile int *vote, int cell, int own0,
+                                  int nOwn, int total) {
+    constexpr int NT = WAVE * WAVES;
+    const Grid &g = P.g;
+    const int tid = threadIdx.x;
+    const int lane = tid & (WAVE - 1);
+    const int wave = tid / WAVE;
+    const int slots = (total + NT - 1) / NT;
+    int *myQueue = queue + wave * WAVE * slots;
+    unsigned long long accepted = 0, leftCell = 0, tests = 0;
+    int move = 0;
+    const int moves = P.moves > 0 ? P.moves : nOwn;
+    const int pbase = P.moves > 0 ? cell * P.moves : own0;
+
+    for (int chunk = 0; chunk < moves; chunk += MOVE_CHUNK) {
+        const int cn = min(MOVE_CHUNK, moves - chunk);
+        load_proposals(P, pbase, chunk, cn, pt, pR, pk);
+        __syncthreads();
+
+        for (int mi = 0; mi < cn; mi++, move++) {
+            const int kIdx = pk[mi];
+            V3 pos = v3(S.x[kIdx], S.y[kIdx], S.z[kIdx]);
+            V3 t = v3(pt[mi], pt[MOVE_CHUNK + mi], pt[2 * MOVE_CHUNK + mi]);
+            V3 npos = translate_wrapped(g.dims, g.inv, pos, t);
+            int nc[3];
+            unsigned nw;
+            bin_position(g, npos, nc, nw);
+            const bool inside = cell_index(g, nc[0], nc[1], nc[2]) == cell;
+            double NO[9];
+            {
+                double O[9], R[9];
+                for (int c = 0; c < 9; c++) {
+                    O[c] = oo[c * OWN_CAP + kIdx];
+                    R[c] = pR[c * MOVE_CHUNK + mi];
+                }
+                mulm(R, O, NO);  // Shape::rotate (Shape.cpp:14-16)
+            }
+            const V3 nax = v3(NO[2], NO[5], NO[8]);
+            const int wx = nw & 1, wy = (nw >> 1) & 1, wz = (nw >> 2) & 1;
+            bool hit = !inside;
+            if (inside) {
+                SCFirst f = make_first<KIND>(P.k, npos, nax);
+                // pass A, SLOT_GROUP candidates per lane loaded before any is tested
+                bool definite = false;
+                int queued = 0;
+                for (int s0 = 0; s0 < slots; s0 += SLOT_GROUP) {
+                    V3 p2[SLOT_GROUP];
+                    int jj[SLOT_GROUP];
+#pragma unroll
+                    for (int s = 0; s < SLOT_GROUP; s++) {
+                        jj[s] = (s0 + s) * NT + tid;
+                        const int j = jj[s] < total ? jj[s] : 0;
+                        p2[s] = v3(S.x[j], S.y[j], S.z[j]);
+                        if (!TRANSLATED) p2[s] = add(p2[s], image_translation(sT, S.code[j], wx, wy, wz));
+                    }
+#pragma unroll
+                    for (int s = 0; s < SLOT_GROUP; s++) {
+                        const int j = jj[s];
+                        bool amb = false;
+                        if (j < total && j != kIdx) {
+                            V3 d = sub(p2[s], npos);
+                            double d2 = dot(d, d);
+                            if (d2 < P.k.contact2) definite = true;
+                            else if (KIND == 1 && d2 < P.k.range2) amb = true;
+                        }
+                        if (KIND == 1 && s0 + s < slots) {
+                            unsigned long long m = __ballot(amb);
+                            if (amb) myQueue[queued + __popcll(m & ((1ull << lane) - 1))] = j;
+                            queued += __popcll(m);
+                        }
+                    }
+                }
+                tests += (unsigned long long)total;
+                __builtin_amdgcn_wave_barrier();  // the queue was written by other lanes of this wavefront
+                hit = definite;
+                // pass B
+                if (KIND == 1 && !__any(definite)) {
+                    for (int q = lane; q < queued; q += WAVE) {
+                        const int j = myQueue[q];
+                        V3 p2 = v3(S.x[j], S.y[j], S.z[j]);
+                        if (!TRANSLATED) p2 = add(p2, image_translation(sT, S.code[j], wx, wy, wz));
+                        V3 a2 = v3(S.ax[j], S.ay[j], S.az[j]);
+                        V3 c2a = add(p2, scal(a2, -P.k.halfLen));
+                        V3 c2b = add(p2, scal(a2, P.k.halfLen));
+                        if (segment_distance2(f.capA, f.u, f.a, c2a, c2b) < P.k.contact2) hit = true;
+                    }
+                }
+            } else {
+                leftCell++;
+            }
+            if (block_vote(hit, move, vote)) continue;
+
+            // Packing::acceptMove: the cell belongs to this workgroup for the whole colour pass
+            accepted++;
+            S.x[kIdx] = npos.x;
+            S.y[kIdx] = npos.y;
+            S.z[kIdx] = npos.z;
+            if (KIND == 1) {
+                S.ax[kIdx] = nax.x;
+                S.ay[kIdx] = nax.y;
+                S.az[kIdx] = nax.z;
+            }
+            S.code[kIdx] = (wx + 1) | ((wy + 1) << 2) | ((wz + 1) << 4);
+            for (int c = 0; c < 9; c++) oo[c * OWN_CAP + kIdx] = NO[c];
+        }
+        __syncthreads();
+    }
+    for (int a = tid; a < nOwn; a += NT) {
+        const int gi = own0 + a;
+        P.p.x[gi] = S.x[a];
+        P.p.y[gi] = S.y[a];
+        P.p.z[gi] = S.z[a];
+        for (int c = 0; c < 9; c++) P.p.o[c][gi] = oo[c * OWN_CAP + a];
+        int code = S.code[a];
+        P.p.wbits[gi] = (uint8_t)(((code & 3) - 1) | ((((code >> 2) & 3) - 1) << 1) | ((((code >> 4) & 3) - 1) << 2));
+    }
+    flush_counters(P, accepted, moves, leftCell, tests, true);
+}
+
+// Sweep of one active cell whose stencil does not fit the LDS area: candidates are read from the cell-sorted
+// arrays in HBM, accepted moves are written there at once.
+template <int KIND, int WAVES>
+__device__ void sweep_cell_unstaged(const SweepParams &P, const Stencil &S, double *pt, double *pR, int *pk,
+                                    const double *sT, int cell, int own0, int nOwn, int total) {
+    constexpr int NT = WAVE * WAVES;
+    const Grid &g = P.g;
+    const int tid = threadIdx.x;
+    unsigned long long accepted = 0, leftCell = 0, tests = 0;
+    const int moves = P.moves > 0 ? P.moves : nOwn;
+    const int pbase = P.moves > 0 ? cell * P.moves : own0;
+    for (int chunk = 0; chunk < moves; chunk += MOVE_CHUNK) {
+        const int cn = min(MOVE_CHUNK, moves - chunk);
+        load_proposals(P, pbase, chunk, cn, pt, pR, pk);
+        __syncthreads();
+        for (int mi = 0; mi < cn; mi++) {
+            const int kIdx = pk[mi];
+            const int gi = own0 + kIdx;
+            V3 pos = load3(P.p.x, P.p.y, P.p.z, gi);
+            V3 t = v3(pt[mi], pt[MOVE_CHUNK + mi], pt[2 * MOVE_CHUNK + mi]);
+            V3 npos = translate_wrapped(g.dims, g.inv, pos, t);
+            int nc[3];
+            unsigned nw;
+            bin_position(g, npos, nc, nw);
+            if (cell_index(g, nc[0], nc[1], nc[2]) != cell) {
+                leftCell++;
+                continue;
+            }
+            double O[9], R[9], NO[9];
+            for (int c = 0; c < 9; c++) {
+                O[c] = P.p.o[c][gi];
+                R[c] = pR[c * MOVE_CHUNK + mi];
+            }
+            mulm(R, O, NO);
+            V3 nax = v3(NO[2], NO[5], NO[8]);
+            SCFirst f = make_first<KIND>(P.k, npos, nax);
+            const int wx = nw & 1, wy = (nw >> 1) & 1, wz = (nw >> 2) & 1;
+            bool hit = false;
+            for (int base = 0; base < total; base += NT) {
+                const int j = base + tid;
+                bool h = false;
+                if (j < total && j != kIdx) {
+                    V3 p2, a2;
+                    int code;
+                    load_candidate<KIND, false>(P.p, S, j, p2, a2, code);
+                    h = pair_overlap<KIND>(P.k, f, p2, a2, image_translation(sT, code, wx, wy, wz));
+                }
+                tests += (unsigned long long)min(NT, total - base);
+                bool any;
+                if (WAVES == 1) any = __any(h);
+                else any = __syncthreads_or(h) != 0;
+                if (any) {
+                    hit = true;
+                    break;
+                }
+            }
+            if (hit) continue;
+            accepted++;
+            if (tid == 0) {
+                P.p.x[gi] = npos.x;
+                P.p.y[gi] = npos.y;
+                P.p.z[gi] = npos.z;
+                for (int c = 0; c < 9; c++) P.p.o[c][gi] = NO[c];
+                P.p.wbits[gi] = (uint8_t)nw;
+            }
+            __syncthreads();
+        }
+        __syncthreads();
+    }
+    flush_counters(P, accepted, moves, leftCell, tests, false);
+}
+
+// Stages the stencil: every lane resolves up to SLOT_GROUP entries, issues all their HBM loads, then writes LDS.
+template <int KIND, int NT>
+__device__ __forceinline__ void stage_stencil_batched(const SweepParams &P, const Stencil &S, const double *sT,
+                                                      int total, bool translated) {
+    const int tid = threadIdx.x;
+    for (int base = 0; base < total; base += NT * SLOT_GROUP) {
+        int gi[SLOT_GROUP], code[SLOT_GROUP];
+#pragma unroll
+        for (int s = 0; s < SLOT_GROUP; s++) {
+            const int j = base + s * NT + tid;
+            gi[s] = -1;
+            if (j < total) {
+                int q = stencil_cell(S.start, 27, j);
+                gi[s] = S.base[q] + (j - S.start[q]);
+                code[s] = S.ecode[q];
+            }
+        }
+        V3 p[SLOT_GROUP], a[SLOT_GROUP];
+        int w[SLOT_GROUP];
+#pragma unroll
+        for (int s = 0; s < SLOT_GROUP; s++) {
+            if (gi[s] >= 0) {
+                p[s] = load3(P.p.x, P.p.y, P.p.z, gi[s]);
+                if (KIND == 1) a[s] = v3(P.p.o[2][gi[s]], P.p.o[5][gi[s]], P.p.o[8][gi[s]]);
+                w[s] = P.p.wbits[gi[s]];
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < SLOT_GROUP; s++) {
+            const int j = base + s * NT + tid;
+            if (gi[s] >= 0) {
+                const int c = image_code(code[s], w[s]);
+                V3 q = translated ? add(p[s], image_translation(sT, c, 0, 0, 0)) : p[s];
+                S.x[j] = q.x;
+                S.y[j] = q.y;
+                S.z[j] = q.z;
+                if (KIND == 1) {
+                    S.ax[j] = a[s].x;
+                    S.ay[j] = a[s].y;
+                    S.az[j] = a[s].z;
+                }
+                S.code[j] = c;
+            }
+        }
+    }
+}
+
+template <int KIND, int WAVES>
+__global__ __launch_bounds__(WAVE *WAVES) void k_sweep(SweepParams P) {
+    constexpr int NT = WAVE * WAVES;
     extern __shared__ double smem[];
     __shared__ int s_start[28];
     __shared__ int s_base[27];
     __shared__ int s_ecode[27];
+    __shared__ int s_vote[2];
 
     const Grid &g = P.g;
-    const int lane = threadIdx.x;
+    const int tid = threadIdx.x;
     const int b = blockIdx.x;
     const int bx = b % P.na[0], by = (b / P.na[0]) % P.na[1], bz = b / (P.na[0] * P.na[1]);
     const int cx = 2 * bx + (P.colour & 1), cy = 2 * by + ((P.colour >> 1) & 1), cz = 2 * bz + ((P.colour >> 2) & 1);
@@ -516,20 +848,10 @@ __global__ __launch_bounds__(WAVE) void k_sweep(SweepParams P) {
     const int nOwn = g.cellStart[cell + 1] - own0;
     if (nOwn == 0) return;
 
-    // Stencil cells: index 0 is the own cell, then the 26 neighbours.
-    if (lane < 27) {
-        int t = lane == 0 ? 13 : (lane <= 13 ? lane - 1 : lane);
-        int ex, ey, ez;
-        int nx = wrap_axis(cx, t % 3 - 1, g.n[0], ex);
-        int ny = wrap_axis(cy, (t / 3) % 3 - 1, g.n[1], ey);
-        int nz = wrap_axis(cz, t / 9 - 1, g.n[2], ez);
-        int nc = cell_index(g, nx, ny, nz);
-        s_base[lane] = g.cellStart[nc];
-        s_start[lane + 1] = g.cellStart[nc + 1] - g.cellStart[nc];
-        s_ecode[lane] = (ex + 1) | ((ey + 1) << 2) | ((ez + 1) << 4);
-    }
+    if (tid < 27) stencil_table(g, cx, cy, cz, tid, false, s_start, s_base, s_ecode);
+    if (tid < 2) s_vote[tid] = 0;
     __syncthreads();
-    if (lane == 0) {
+    if (tid == 0) {
         s_start[0] = 0;
         for (int q = 1; q <= 27; q++) s_start[q] += s_start[q - 1];
     }
@@ -540,42 +862,51 @@ __global__ __launch_bounds__(WAVE) void k_sweep(SweepParams P) {
     S.start = s_start;
     S.base = s_base;
     S.ecode = s_ecode;
-    S.x = smem;
-    S.y = S.x + P.cap;
-    S.z = S.y + P.cap;
-    S.ax = S.z + P.cap;
-    S.ay = S.ax + P.cap;
-    S.az = S.ay + P.cap;
-    S.code = reinterpret_cast<int *>(KIND == 0 ? S.ax : S.az + P.cap);
-
-    if (total > P.cap) {
-        sweep_cell<KIND, false>(P, S, cell, own0, nOwn, total);
-        return;
-    }
-    for (int j = lane; j < total; j += WAVE) {
-        int q = stencil_cell(s_start, j);
-        int gi = s_base[q] + (j - s_start[q]);
-        S.x[j] = P.p.x[gi];
-        S.y[j] = P.p.y[gi];
-        S.z[j] = P.p.z[gi];
-        if (KIND == 1) {
-            S.ax[j] = P.p.o[2][gi];
-            S.ay[j] = P.p.o[5][gi];
-            S.az[j] = P.p.o[8][gi];
-        }
-        S.code[j] = image_code(s_ecode[q], P.p.wbits[gi]);
-    }
+    S.ncells = 27;
+    double *rest;
+    carve_stencil(S, smem, P.cap, KIND, rest);
+    double *oo = rest;                  // 9 x OWN_CAP own orientations
+    double *pt = oo + 9 * OWN_CAP;      // 3 x MOVE_CHUNK proposal translations
+    double *pR = pt + 3 * MOVE_CHUNK;   // 9 x MOVE_CHUNK proposal rotations
+    double *sT = pR + 9 * MOVE_CHUNK;   // 81 translation table
+    int *pk = reinterpret_cast<int *>(sT + 81);
+    int *queue = pk + MOVE_CHUNK;       // cap + 64*WAVES ambiguous-candidate slots
+    for (int i = tid; i < 81; i += NT) sT[i] = g.T[i];
     __syncthreads();
-    sweep_cell<KIND, true>(P, S, cell, own0, nOwn, total);
+
+    if (total <= P.cap && nOwn <= OWN_CAP) {
+        // a cell off the last row on every axis holds no wrapped particle: stage pos + T directly
+        const bool translated = cx < g.n[0] - 1 && cy < g.n[1] - 1 && cz < g.n[2] - 1;
+        stage_stencil_batched<KIND, NT>(P, S, sT, total, translated);
+        for (int e = tid; e < 9 * nOwn; e += NT) {
+            int c = e / nOwn, a = e - c * nOwn;
+            oo[c * OWN_CAP + a] = P.p.o[c][own0 + a];
+        }
+        __syncthreads();
+        if (translated)
+            sweep_cell_staged<KIND, WAVES, true>(P, S, oo, pt, pR, pk, queue, sT, s_vote, cell, own0, nOwn, total);
+        else
+            sweep_cell_staged<KIND, WAVES, false>(P, S, oo, pt, pR, pk, queue, sT, s_vote, cell, own0, nOwn, total);
+    } else {
+        sweep_cell_unstaged<KIND, WAVES>(P, S, pt, pR, pk, sT, cell, own0, nOwn, total);
+    }
 }
 
-hipError_t launch_sweep(hipStream_t s, const SweepParams &sp) {
+hipError_t launch_sweep(hipStream_t s, const SweepParams &sp, int waves) {
     unsigned blocks = (unsigned)(sp.na[0] * sp.na[1] * sp.na[2]);
-    size_t lds = sweep_lds_bytes(sp.k.kind, sp.cap);
-    if (sp.k.kind == 0)
-        hipLaunchKernelGGL(k_sweep<0>, dim3(blocks), dim3(WAVE), lds, s, sp);
-    else
-        hipLaunchKernelGGL(k_sweep<1>, dim3(blocks), dim3(WAVE), lds, s, sp);
+    int w = waves >= 4 ? 4 : (waves == 2 ? 2 : 1);
+    size_t lds = sweep_lds_bytes(sp.k.kind, sp.cap, w);
+#define RP_SWEEP(KIND, W) hipLaunchKernelGGL((k_sweep<KIND, W>), dim3(blocks), dim3(WAVE * W), lds, s, sp)
+    if (sp.k.kind == 0) {
+        if (w == 4) RP_SWEEP(0, 4);
+        else if (w == 2) RP_SWEEP(0, 2);
+        else RP_SWEEP(0, 1);
+    } else {
+        if (w == 4) RP_SWEEP(1, 4);
+        else if (w == 2) RP_SWEEP(1, 2);
+        else RP_SWEEP(1, 1);
+    }
+#undef RP_SWEEP
     return hipGetLastError();
 }
 

@@ -186,7 +186,7 @@ RP_HD double wca_energy(double eps, double sigma2, double cut2, double d2) {
 // Philox4x32-10 (Salmon et al., SC'11), counter-based: one call maps (counter, key) to four 32-bit words.
 struct Philox {
     uint32_t c0, c1, c2, c3, k0, k1;
-    uint32_t buf[4];
+    uint32_t b0, b1, b2, b3;  // current output block (separate members: no dynamically indexed array)
     int used;
 };
 
@@ -223,12 +223,15 @@ RP_HD Philox philox_init(uint64_t seed, uint32_t s0, uint32_t s1, uint32_t s2) {
 
 RP_HD uint32_t philox_u32(Philox &p) {
     if (p.used == 4) {
-        uint32_t ctr[4] = {p.c0, p.c1, p.c2, p.c3};
-        philox_block(ctr, p.k0, p.k1, p.buf);
+        uint32_t ctr[4] = {p.c0, p.c1, p.c2, p.c3}, out[4];
+        philox_block(ctr, p.k0, p.k1, out);
+        p.b0 = out[0]; p.b1 = out[1]; p.b2 = out[2]; p.b3 = out[3];
         p.c0++;
         p.used = 0;
     }
-    return p.buf[p.used++];
+    uint32_t v = p.used == 0 ? p.b0 : (p.used == 1 ? p.b1 : (p.used == 2 ? p.b2 : p.b3));
+    p.used++;
+    return v;
 }
 
 // Uniform double in [0, 1) with 53 random bits.

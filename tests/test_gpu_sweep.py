@@ -30,7 +30,8 @@ def test_sweep_keeps_packing_overlap_free_and_moves_every_particle():
     pk.set_environment(1.0, 2.0, 0.3, None, box_move=("delta_triclinic", 0.02))
     c = pk.run_cycles(30, adapt_steps=True)
     print(c, flush=True)
-    assert c["moves_attempted"] == 30 * len(pos)
+    # dense cells make round(N / cells) moves each (the reference's floor(N / domains) per domain)
+    assert abs(c["moves_attempted"] - 30 * len(pos)) <= 0.1 * 30 * len(pos)
     assert c["unstaged_blocks"] == 0
     assert 0 < c["moves_accepted"] < c["moves_attempted"]
     b, p, r = pk.download()
@@ -59,8 +60,9 @@ def test_two_cell_grid_reads_stencil_from_hbm():
 def test_sweep_moves_particles_across_cells_and_box_changes():
     box, pos, rot = rod_start()
     pk = ra.GpuPacking(SC, box, pos, rot, seed=5)
-    pk.set_environment(1.0, 1.5, 0.4, None, box_move=("delta_triclinic", 0.05))
-    pk.run_cycles(60, adapt_steps=True)
+    pk.set_environment(1.0, 1.5, 0.4, None, box_move=("delta_triclinic", 0.002))
+    c = pk.run_cycles(150, adapt_steps=True)
+    assert c["box_accepted"] > 0
     b, p, r = pk.download()
     assert not np.allclose(b, box)
     disp = p - pos
@@ -155,3 +157,24 @@ def test_degenerate_hard_sphere_gas_carnahan_starling():
     print(f"degenerate HS: {rho} +- {err} (CS 0.398574)", flush=True)
     assert abs(rho - 0.398574) <= 3 * err
     assert err / rho < 0.03
+
+
+@pytest.mark.parametrize("waves", [1, 2, 4])
+@pytest.mark.parametrize("kind", ["sphere", "spherocylinder"])
+def test_every_launch_shape_is_overlap_free(kind, waves, monkeypatch):
+    """Each workgroup size (1, 2 or 4 wavefronts per active cell) for both shapes keeps the packing overlap-free."""
+    monkeypatch.setenv("RAMPACK_SWEEP_WAVES", str(waves))
+    if kind == "sphere":
+        shape = HS
+        box, pos, rot = lattice.hard_sphere_lattice(10, 0.45)
+        move = ("log", 0.001)
+    else:
+        shape = SC
+        box, pos, rot = rod_start()
+        move = ("delta_triclinic", 0.002)
+    pk = ra.GpuPacking(shape, box, pos, rot, seed=100 + waves)
+    pk.set_environment(1.0, 5.0 if kind == "sphere" else 2.0, 0.15, None, box_move=move)
+    c = pk.run_cycles(100, adapt_steps=True)
+    assert abs(c["moves_attempted"] - 100 * len(pos)) <= 0.1 * 100 * len(pos)
+    b, p, r = pk.download()
+    assert overlaps_by_oracle(shape, b, p, r) == 0
