@@ -141,12 +141,14 @@ def main():
     tb = pk.run_cycles(args.timing_cycles)
     tm = pk.timing()
     pk.set_timing(False)
-    moves_per_launch = len(pos) / 8.0
+    moves_per_launch = (tb["moves_attempted"] - after["moves_attempted"]) / max(tm["sweep_launches"], 1)
     avg_sweep_ms = tm["sweep_ms"] / max(tm["sweep_launches"], 1)
     bytes_per_launch = (27 + acc_rate) * RECORD_BYTES * moves_per_launch
     achieved = bytes_per_launch / (avg_sweep_ms * 1e-3) / 1e9
-    log(f"launch shape: {tm['sweep_waves']} wavefronts per cell, {tm['sweep_cap']} staged entries, "
+    log(f"launch shape: colour period {tm['sweep_period']}, {tm['sweep_waves']} wavefronts per cell, "
+        f"{tm['sweep_cap']} staged entries, "
         f"LDS limit {tm['lds_limit']} B")
+    log(f"proposals avg {tm['proposal_ms'] / max(tm['proposal_launches'], 1) * 1e3:.1f} us")
     log(f"sweep kernel: {tm['sweep_launches']} launches, avg {avg_sweep_ms * 1e3:.1f} us; rebuild avg "
         f"{tm['rebuild_ms'] / max(tm['rebuild_launches'], 1) * 1e3:.1f} us x {tm['rebuild_launches']}; scan avg "
         f"{tm['scan_ms'] / max(tm['scan_launches'], 1) * 1e3:.1f} us x {tm['scan_launches']}")
@@ -202,6 +204,7 @@ def main():
         },
         "cpu_baseline": cpu,
     }
+    pk.close()
     print(json.dumps(out), flush=True)
 
 

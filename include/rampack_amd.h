@@ -97,6 +97,7 @@ typedef struct rp_timing {
     uint64_t lds_limit;          /* LDS bytes a workgroup may use on this device */
     uint64_t sweep_cap;          /* stencil entries staged per sweep workgroup in the last cycle */
     uint64_t sweep_waves;        /* wavefronts per active cell in the last cycle */
+    uint64_t sweep_period;       /* checkerboard colour period (colours = period^3) in the last cycle */
 } rp_timing;
 
 /* Packing constructor + Simulation constructor (src/core/Packing.cpp:46-82, src/core/Simulation.cpp:54-76):

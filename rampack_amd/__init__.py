@@ -67,7 +67,8 @@ class Timing(ctypes.Structure):
                 ("rebuild_launches", ctypes.c_uint64), ("rebuild_ms", ctypes.c_double),
                 ("scan_launches", ctypes.c_uint64), ("scan_ms", ctypes.c_double),
                 ("proposal_launches", ctypes.c_uint64), ("proposal_ms", ctypes.c_double),
-                ("lds_limit", ctypes.c_uint64), ("sweep_cap", ctypes.c_uint64), ("sweep_waves", ctypes.c_uint64)]
+                ("lds_limit", ctypes.c_uint64), ("sweep_cap", ctypes.c_uint64), ("sweep_waves", ctypes.c_uint64),
+                ("sweep_period", ctypes.c_uint64)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}

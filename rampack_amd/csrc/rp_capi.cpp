@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
 #include <cstdlib>
 #include <cmath>
 #include <cstring>
@@ -84,6 +85,7 @@ struct rp_ctx {
     size_t propCap = 0;
     void *propMem = nullptr;
     unsigned long long *dCounters = nullptr;
+    unsigned long long *dPhases = nullptr;  // sweep phase timers (RAMPACK_SWEEP_PHASES=1)
     unsigned long long *hCounters = nullptr;  // pinned
     uint64_t seed = 0;
     uint64_t cycle = 0;
@@ -110,6 +112,8 @@ struct rp_ctx {
     // launch shape
     size_t ldsLimit = 64 * 1024 - 1024;  // dynamic LDS a workgroup may use
     int sweepWaves = 0;                  // wavefronts per active cell; 0 = chosen from the stencil size
+    int colourPeriod = 0;                // checkerboard colour period (2 or 3); 0 = period 2
+    int pipelined = 1;                   // planner/tester split of the staged sweep
 
     // timing
     bool timing = false;
@@ -197,32 +201,46 @@ int ensureProposals(rp_ctx *c, size_t need) {
     return RP_OK;
 }
 
-// Checkerboard grid: an even number of cells per axis, each at least the interaction range high and, as in the
-// reference's grid (Packing.cpp:862-866), at least a fifth of the mean interparticle spacing.
-int makeSweepGrid(rp_ctx *c, const HostBox &b, const double shiftUnit[3], Grid &g) {
+// Checkerboard grid with colour period P: cells at least range / (P - 1) high and, as in the reference's grid
+// (Packing.cpp:862-866), at least a fifth of the mean interparticle spacing; a multiple of P cells per axis, at
+// least 2 * reach + 1 of them, so that a stencil offset wraps around the box at most once.
+int makeSweepGrid(rp_ctx *c, const HostBox &b, const double shiftUnit[3], Grid &g, int period) {
     double h[3];
     b.heights(h);
     std::memcpy(g.dims, b.dims, sizeof g.dims);
     std::memcpy(g.inv, b.inv, sizeof g.inv);
-    double cellSize = std::max(c->range, std::cbrt(b.volume() / (double)c->n) / 5);
+    const int reach = period - 1;
+    double cellSize = std::max(c->range / reach, std::cbrt(b.volume() / (double)c->n) / 5);
     for (int d = 0; d < 3; d++) {
-        int n = (int)std::floor(h[d] / cellSize);
-        n -= n % 2;
-        if (n < 2)
-            return fail(RP_ERR_UNSUPPORTED, "box height " + std::to_string(h[d]) +
-                                            " is below two interaction ranges; the checkerboard needs two cells per axis");
+        int n = period * (int)std::floor(h[d] / (period * cellSize));
+        if (n < period || (reach > 1 && n < 2 * reach + 1))
+            return fail(RP_ERR_UNSUPPORTED, "box height " + std::to_string(h[d]) + " is too small for a checkerboard "
+                                            "of period " + std::to_string(period));
         g.n[d] = n;
         g.relCell[d] = 1 / (double)n;
         g.shift[d] = shiftUnit[d] * g.relCell[d];
     }
     g.ncell = g.n[0] * g.n[1] * g.n[2];
     g.exact = 0;
+    g.reach = reach;
+    g.period = period;
     fillTranslations(g);
     int r = ensureCells(c, g.ncell);
     if (r) return r;
     g.cellStart = c->cellStart;
     g.cellCount = c->cellCount;
     return RP_OK;
+}
+
+// Colour period of the checkerboard: RAMPACK_COLOURING (2 or 3) if set, otherwise 3 -- finer cells, 27 colours,
+// more active cells per pass and a tighter 125-cell stencil -- when those cells still hold two particles on
+// average, else 2.
+// Period 2 (8 colours, cells >= range) unless period 3 is asked for: at the same density the period-3 grid has
+// 27 launches per cycle of workgroups with a few particles each and 125-cell stencils, which measured about four
+// times slower per trial move on 1e5 rods.
+int chooseSweepGrid(rp_ctx *c, const HostBox &b, const double shiftUnit[3], Grid &g) {
+    if (c->colourPeriod == 3 && makeSweepGrid(c, b, shiftUnit, g, 3) == RP_OK) return RP_OK;
+    return makeSweepGrid(c, b, shiftUnit, g, 2);
 }
 
 // The reference's neighbour grid (Packing.cpp:858-883, NeighbourGrid.cpp:178-201).  Returns 1 when the
@@ -245,6 +263,8 @@ int makeExactGrid(rp_ctx *c, const HostBox &b, Grid &g, int &disabled) {
     }
     g.ncell = g.n[0] * g.n[1] * g.n[2];
     g.exact = 1;
+    g.reach = 1;
+    g.period = 2;
     fillTranslations(g);
     int r = ensureCells(c, g.ncell);
     if (r) return r;
@@ -311,7 +331,8 @@ int scanOverlaps(rp_ctx *c, const Grid &g, int earlyExit, uint64_t &count) {
     RP_HIP(launch_fill_u64(c->stream, c->dCounters + CNT_OVERLAPS, 1, 0));
     timingBegin(c, 2);
     double lambda = (double)c->n / (double)g.ncell;
-    double expect = 14 * lambda;
+    const int side = 2 * g.reach + 1;
+    double expect = ((side * side * side - 1) / 2 + 1) * lambda;
     int cap = (int)std::ceil(expect + 8 * std::sqrt(expect) + 64);
     cap = std::min((cap + 63) / 64 * 64, capForBytes(c, count_lds_bytes(c->k.kind, 64) / 64.0));
     RP_HIP(launch_count_overlaps(c->stream, g, (int)c->n, c->alt.p, c->k, earlyExit, c->dCounters, cap));
@@ -368,7 +389,7 @@ int boxMove(rp_ctx *c, bool &accepted) {
     if (u > std::exp(exponent)) return RP_OK;
 
     Grid g{};
-    int r = makeSweepGrid(c, nb, shiftUnit, g);
+    int r = chooseSweepGrid(c, nb, shiftUnit, g);
     if (r == RP_ERR_UNSUPPORTED) {
         // the checkerboard needs two cells per axis; the proposal is rejected and counted
         c->boxTooSmall++;
@@ -397,7 +418,7 @@ int prepareCycleGrid(rp_ctx *c) {
     if (c->sortedForCycle) return RP_OK;
     double shiftUnit[3] = {philox_unit(c->hostRng), philox_unit(c->hostRng), philox_unit(c->hostRng)};
     Grid g{};
-    int r = makeSweepGrid(c, c->box, shiftUnit, g);
+    int r = chooseSweepGrid(c, c->box, shiftUnit, g);
     if (r) return r;
     r = rebuild(c, g, 0, nullptr);
     if (r) return r;
@@ -418,12 +439,12 @@ int runCycle(rp_ctx *c) {
     sp.k = c->k;
     double maxT = c->maxTransImplicit ? minHeight(c->box) / 2 : c->maxTransStep;
     const double ts = std::min(maxT, c->transStep), rs = std::min(c->rotStep, M_PI);
-    for (int d = 0; d < 3; d++) sp.na[d] = g.n[d] / 2;
+    for (int d = 0; d < 3; d++) sp.na[d] = g.n[d] / g.period;
     double lambda = (double)c->n / (double)g.ncell;
     // Dense cells make a fixed number of moves each (the reference's floor(N / domains) per domain,
     // Simulation.cpp:431-433), which balances the workgroups; sparse cells move as many times as they hold
     // particles, so a cycle is exactly N moves.
-    sp.moves = lambda >= 8 ? (int)std::lround(lambda) : 0;
+    sp.moves = lambda >= 3 ? (int)std::lround(lambda) : 0;
     {
         size_t need = sp.moves > 0 ? (size_t)g.ncell * sp.moves : c->n;
         int r2 = ensureProposals(c, need);
@@ -434,7 +455,8 @@ int runCycle(rp_ctx *c) {
     RP_HIP(launch_proposals(c->stream, g, sp.pr, sp.moves, c->cur.p.cell, (int)c->n, ts, rs, c->seed,
                             (uint32_t)c->cycle));
     timingEnd(c);
-    double expect = 27 * lambda;
+    const int side = 2 * g.reach + 1;
+    double expect = side * side * side * lambda;
     int cap = (int)std::ceil(expect + 8 * std::sqrt(expect) + 64);
     cap = (cap + 63) / 64 * 64;
     int waves = c->sweepWaves;
@@ -446,15 +468,20 @@ int runCycle(rp_ctx *c) {
     int maxCap = (int)((double)(c->ldsLimit - fixed) / perEntry) / 64 * 64;
     sp.cap = std::max(64, std::min(cap, maxCap));
     sp.counters = c->dCounters;
-    int order[8] = {0, 1, 2, 3, 4, 5, 6, 7};
-    for (int i = 7; i > 0; i--) {
+    sp.phases = c->dPhases;
+    sp.pipelined = c->pipelined;
+    const int colours = g.period * g.period * g.period;
+    int order[27];
+    for (int i = 0; i < colours; i++) order[i] = i;
+    for (int i = colours - 1; i > 0; i--) {
         int j = (int)philox_below(c->hostRng, (uint32_t)(i + 1));
         std::swap(order[i], order[j]);
     }
     c->tstats.lds_limit = c->ldsLimit;
     c->tstats.sweep_cap = (uint64_t)sp.cap;
     c->tstats.sweep_waves = (uint64_t)waves;
-    for (int i = 0; i < 8; i++) {
+    c->tstats.sweep_period = (uint64_t)g.period;
+    for (int i = 0; i < colours; i++) {
         sp.colour = order[i];
         timingBegin(c, 0);
         RP_HIP(launch_sweep(c->stream, sp, waves));
@@ -561,6 +588,15 @@ int rp_create(rp_ctx **out, const rp_shape_desc *shape, const double box[9], siz
     if (hipDeviceGetAttribute(&ldsBytes, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && ldsBytes > 0)
         c->ldsLimit = (size_t)std::min(ldsBytes, 160 * 1024) - 1024;
     if (const char *w = std::getenv("RAMPACK_SWEEP_WAVES")) c->sweepWaves = std::atoi(w);
+    if (const char *p = std::getenv("RAMPACK_COLOURING")) c->colourPeriod = std::atoi(p);
+    if (const char *p = std::getenv("RAMPACK_SWEEP_PIPELINE")) c->pipelined = std::atoi(p) != 0;
+    if (const char *ph = std::getenv("RAMPACK_SWEEP_PHASES"); ph && std::atoi(ph) != 0) {
+        if (hipMalloc(&c->dPhases, sizeof(unsigned long long) * RP_PHASE_NUM) != hipSuccess)
+            return fail(RP_ERR_DEVICE, "phase timer allocation failed");
+        unsigned long long init[RP_PHASE_NUM] = {};
+        if (hipMemcpy(c->dPhases, init, sizeof init, hipMemcpyHostToDevice) != hipSuccess)
+            return fail(RP_ERR_DEVICE, "phase timer init failed");
+    }
     c->shape = *shape;
     c->k = makeConsts(*shape);
     c->range = shapeRange(*shape);
@@ -647,7 +683,7 @@ int rp_run_cycles(rp_ctx *c, size_t nCycles, int adapt, rp_counters *out) {
         double zero[3] = {0, 0, 0};
         c->sortedForCycle = false;
         Grid g{};
-        r = makeSweepGrid(c, c->box, zero, g);
+        r = chooseSweepGrid(c, c->box, zero, g);
         if (r) return r;
         r = rebuild(c, g, 0, nullptr);
         if (r) return r;
@@ -728,6 +764,7 @@ int rp_set_timing(rp_ctx *c, int enable) {
     c->tstats.lds_limit = keep.lds_limit;
     c->tstats.sweep_cap = keep.sweep_cap;
     c->tstats.sweep_waves = keep.sweep_waves;
+    c->tstats.sweep_period = keep.sweep_period;
     return RP_OK;
 }
 
@@ -742,6 +779,28 @@ int rp_get_timing(rp_ctx *c, rp_timing *out) {
 void rp_destroy(rp_ctx *c) {
     if (!c) return;
     if (c->stream) hipStreamSynchronize(c->stream);
+    if (c->dPhases) {
+        // wall_clock64 ticks at the device's wall-clock rate (kHz attribute)
+        int khz = 100000;
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0)
+            khz = 100000;
+        const double perUs = khz / 1000.0;
+        unsigned long long ph[RP_PHASE_NUM] = {};
+        if (hipMemcpy(ph, c->dPhases, sizeof ph, hipMemcpyDeviceToHost) == hipSuccess && ph[RP_PHASE_BLOCKS]) {
+            double b = (double)ph[RP_PHASE_BLOCKS];
+            std::fprintf(stderr,
+                         "sweep phases per workgroup (us): table %.2f stage %.2f moves %.2f write %.2f; "
+                         "%llu workgroups, longest %.1f us; wall clock %d kHz, %d workgroups per CU\n",
+                         ph[RP_PHASE_TABLE] / b / perUs, ph[RP_PHASE_STAGE] / b / perUs,
+                         ph[RP_PHASE_MOVES] / b / perUs, ph[RP_PHASE_WRITE] / b / perUs,
+                         (unsigned long long)ph[RP_PHASE_BLOCKS], ph[RP_PHASE_LAST] / perUs, khz,
+                         sweep_occupancy(c->shape.kind, c->tstats.sweep_cap ? (int)c->tstats.sweep_cap : 64,
+                                         c->tstats.sweep_waves ? (int)c->tstats.sweep_waves : 4));
+        }
+        hipFree(c->dPhases);
+    }
     for (auto e : c->evStart) hipEventDestroy(e);
     for (auto e : c->evStop) hipEventDestroy(e);
     if (c->cur.mem) hipFree(c->cur.mem);

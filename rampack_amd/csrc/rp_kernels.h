@@ -19,8 +19,9 @@ struct Particles {
 };
 
 // One cell grid over the (triclinic) box in relative coordinates.  exact = 1 reproduces the reference's
-// NeighbourGrid binning (NeighbourGrid.cpp:13-34, no origin shift); exact = 0 is the checkerboard grid, with
-// an origin shift in [0, 1/n) per axis and an even number of cells per axis.
+// NeighbourGrid binning (NeighbourGrid.cpp:13-34, no origin shift, reach 1); exact = 0 is the checkerboard grid,
+// with an origin shift in [0, 1/n) per axis, cells at least range / reach high and a multiple of period = reach + 1
+// cells per axis, so that same-colour cells are separated by reach cells.
 struct Grid {
     double dims[9], inv[9];
     double T[81];           // translations dims * (i-1, j-1, k-1), index 9i + 3j + k (NeighbourGrid.cpp:203-217)
@@ -29,6 +30,8 @@ struct Grid {
     int n[3];
     int ncell;
     int exact;
+    int reach;              // stencil half-width in cells: cells are at least range / reach high
+    int period;             // checkerboard colour period per axis (reach + 1); colours = period^3
     int *cellStart;         // ncell + 1
     int *cellCount;         // ncell
 };
@@ -48,9 +51,22 @@ struct SweepParams {
     Proposals pr;
     int moves;              // trial moves per active cell and colour pass; 0: as many as the cell holds
     int colour;
-    int na[3];              // active cells per axis (n / 2)
+    int na[3];              // active cells per axis (n / period)
     int cap;                // stencil entries that fit in LDS
+    int pipelined;          // 1: wavefront 0 plans the next move while the others test (needs >= 2 wavefronts)
     unsigned long long *counters;
+    unsigned long long *phases;  // optional per-phase wall-clock sums (RP_PHASE_*), nullptr when not profiling
+};
+
+// Phases of a sweep workgroup timed with the device wall clock when SweepParams::phases is set.
+enum SweepPhase : int {
+    RP_PHASE_TABLE = 0,    // stencil cell table and prefix sums
+    RP_PHASE_STAGE = 1,    // stencil and own orientations into LDS
+    RP_PHASE_MOVES = 2,    // the trial moves
+    RP_PHASE_WRITE = 3,    // write-back and counters
+    RP_PHASE_BLOCKS = 4,   // workgroups timed
+    RP_PHASE_LAST = 6,     // longest workgroup (max)
+    RP_PHASE_NUM = 8
 };
 
 enum Counter : int {
@@ -82,6 +98,8 @@ size_t count_lds_bytes(int kind, int cap);
 hipError_t launch_proposals(hipStream_t s, const Grid &g, const Proposals &pr, int moves, const int *cellOf, int n,
                             double ts, double rs, uint64_t seed, uint32_t cycle);
 // waves: wavefronts per active cell (1, 2 or 4)
+// Workgroups of the sweep kernel one compute unit can hold at this LDS size (occupancy API).
+int sweep_occupancy(int kind, int cap, int waves);
 hipError_t launch_sweep(hipStream_t s, const SweepParams &sp, int waves);
 size_t sweep_lds_bytes(int kind, int cap, int waves);
 

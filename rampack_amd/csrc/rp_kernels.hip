@@ -318,23 +318,46 @@ __device__ __forceinline__ void load_candidate(const Particles &p, const Stencil
     }
 }
 
-// Fills the stencil cell table for `ncells` offsets of cell (cx, cy, cz); offset list: own cell first, then either
-// all 26 neighbours (sweep) or the 13 "positive" ones, 9*n0 + 3*n1 + n2 > 13 (NeighbourGrid.cpp:130-131).
+constexpr int MAX_STENCIL = 125;  // (2 * reach + 1)^3 cells for reach <= 2
+constexpr int MAX_HALF = 63;      // own cell + the positive half of the other 124
+
+// Number of stencil cells: the full (2k+1)^3 block, or the own cell plus the lexicographically positive half.
+__device__ __forceinline__ int stencil_cells(int reach, bool half) {
+    const int side = 2 * reach + 1, all = side * side * side;
+    return half ? (all - 1) / 2 + 1 : all;
+}
+
+// Fills one slot of the stencil cell table of cell (cx, cy, cz): slot 0 is the own cell, then either all other
+// offsets in [-k, k]^3 (sweep) or only the positive ones, (o0, o1, o2) > 0 lexicographically with o0 the most
+// significant -- for k = 1 the reference's 9*n0 + 3*n1 + n2 > 13 (NeighbourGrid.cpp:130-131).
 __device__ __forceinline__ void stencil_table(const Grid &g, int cx, int cy, int cz, int slot, bool half,
                                               int *s_start, int *s_base, int *s_ecode) {
+    const int k = g.reach, side = 2 * k + 1, centre = (side * side * side - 1) / 2;
     int t;
-    if (slot == 0) t = 13;
-    else if (half) t = 13 + slot;
-    else t = slot <= 13 ? slot - 1 : slot;
-    // t = 9*n0 + 3*n1 + n2 with n = offset + 1 (x most significant, as the reference orders its neighbours)
+    if (slot == 0) t = centre;
+    else if (half) t = centre + slot;
+    else t = slot <= centre ? slot - 1 : slot;
     int ex, ey, ez;
-    int nx = wrap_axis(cx, t / 9 - 1, g.n[0], ex);
-    int ny = wrap_axis(cy, (t / 3) % 3 - 1, g.n[1], ey);
-    int nz = wrap_axis(cz, t % 3 - 1, g.n[2], ez);
+    int nx = wrap_axis(cx, t / (side * side) - k, g.n[0], ex);
+    int ny = wrap_axis(cy, (t / side) % side - k, g.n[1], ey);
+    int nz = wrap_axis(cz, t % side - k, g.n[2], ez);
     int nc = cell_index(g, nx, ny, nz);
     s_base[slot] = g.cellStart[nc];
     s_start[slot + 1] = g.cellStart[nc + 1] - g.cellStart[nc];
     s_ecode[slot] = (ex + 1) | ((ey + 1) << 2) | ((ez + 1) << 4);
+}
+
+// Builds the table of `ncells` stencil cells and its prefix sums (s_start[ncells] = total entries).
+__device__ __forceinline__ void build_stencil(const Grid &g, int cx, int cy, int cz, int ncells, bool half,
+                                              int *s_start, int *s_base, int *s_ecode) {
+    for (int slot = threadIdx.x; slot < ncells; slot += blockDim.x)
+        stencil_table(g, cx, cy, cz, slot, half, s_start, s_base, s_ecode);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        s_start[0] = 0;
+        for (int q = 1; q <= ncells; q++) s_start[q] += s_start[q - 1];
+    }
+    __syncthreads();
 }
 
 template <int KIND>
@@ -414,7 +437,7 @@ template <int KIND>
 __global__ __launch_bounds__(COUNT_THREADS) void k_count_overlaps(Grid g, Particles p, PairConsts k, int earlyExit,
                                                                   unsigned long long *counters, int cap) {
     extern __shared__ double smem[];
-    __shared__ int s_start[15], s_base[14], s_ecode[14];
+    __shared__ int s_start[MAX_HALF + 1], s_base[MAX_HALF], s_ecode[MAX_HALF];
     __shared__ int sFound;
     __shared__ double sT[81];
     const int tid = threadIdx.x;
@@ -425,21 +448,16 @@ __global__ __launch_bounds__(COUNT_THREADS) void k_count_overlaps(Grid g, Partic
     volatile unsigned long long *flag = counters + CNT_OVERLAPS;
     if (earlyExit && *flag != 0) return;
     const int cx = cell % g.n[0], cy = (cell / g.n[0]) % g.n[1], cz = cell / (g.n[0] * g.n[1]);
-    if (tid < 14) stencil_table(g, cx, cy, cz, tid, true, s_start, s_base, s_ecode);
-    for (int i = tid; i < 81; i += blockDim.x) sT[i] = g.T[i];
+    const int ncells = stencil_cells(g.reach, true);
+    for (int i = tid; i < 81; i += COUNT_THREADS) sT[i] = g.T[i];
     if (tid == 0) sFound = 0;
-    __syncthreads();
-    if (tid == 0) {
-        s_start[0] = 0;
-        for (int q = 1; q <= 14; q++) s_start[q] += s_start[q - 1];
-    }
-    __syncthreads();
-    const int total = s_start[14];
+    build_stencil(g, cx, cy, cz, ncells, true, s_start, s_base, s_ecode);
+    const int total = s_start[ncells];
     Stencil S;
     S.start = s_start;
     S.base = s_base;
     S.ecode = s_ecode;
-    S.ncells = 14;
+    S.ncells = ncells;
     double *rest;
     carve_stencil(S, smem, cap, KIND, rest);
     int *sTag = reinterpret_cast<int *>(rest);  // own-cell tags, up to cap entries
@@ -545,7 +563,7 @@ hipError_t launch_proposals(hipStream_t s, const Grid &g, const Proposals &pr, i
 // own orientations, one batch of proposals and the translation table.
 size_t sweep_lds_bytes(int kind, int cap, int waves) {
     return stencil_lds_bytes(kind, cap) + sizeof(int) * (cap + WAVE * waves) +
-           sizeof(double) * (9 * OWN_CAP + 12 * MOVE_CHUNK + 81) + sizeof(int) * MOVE_CHUNK;
+           sizeof(double) * (9 * OWN_CAP + 12 * MOVE_CHUNK + 81 + 2 * 24) + sizeof(int) * (MOVE_CHUNK + 8);
 }
 
 __device__ __forceinline__ void flush_counters(const SweepParams &P, unsigned long long accepted, int moves,
@@ -592,7 +610,7 @@ __device__ __forceinline__ bool block_vote(bool hit, int move, volatile int *slo
 template <int KIND, int WAVES, bool TRANSLATED>
 __device__ void sweep_cell_staged(const SweepParams &P, const Stencil &S, double *oo, double *pt, double *pR,
                                   int *pk, int *queue, const double *sT, volatile int *vote, int cell, int own0,
-                                  int nOwn, int total) {
+                                  int nOwn, int total, unsigned long long *clk) {
     constexpr int NT = WAVE * WAVES;
     const Grid &g = P.g;
     const int tid = threadIdx.x;
@@ -698,7 +716,185 @@ __device__ void sweep_cell_staged(const SweepParams &P, const Stencil &S, double
         }
         __syncthreads();
     }
+    if (clk) clk[3] = wall_clock64();
     for (int a = tid; a < nOwn; a += NT) {
+        const int gi = own0 + a;
+        P.p.x[gi] = S.x[a];
+        P.p.y[gi] = S.y[a];
+        P.p.z[gi] = S.z[a];
+        for (int c = 0; c < 9; c++) P.p.o[c][gi] = oo[c * OWN_CAP + a];
+        int code = S.code[a];
+        P.p.wbits[gi] = (uint8_t)(((code & 3) - 1) | ((((code >> 2) & 3) - 1) << 1) | ((((code >> 4) & 3) - 1) << 2));
+    }
+    flush_counters(P, accepted, moves, leftCell, tests, true);
+}
+
+// Pipelined sweep of one staged cell (two or more wavefronts).  Wavefront 0 is the planner: while the other
+// wavefronts (testers) test trial move m against the stencil, it applies proposal m+1 to the current pose --
+// PBC wrap, cell check, R * O, cap geometry -- and publishes the plan in LDS; the move vote is the only
+// workgroup barrier per move.  If move m is accepted and move m+1 picks the same particle, the plan was made
+// from the old pose and is recomputed after the vote.
+constexpr int PLAN_D = 24;  // doubles per plan: npos 3, nax 3, capA 3, u 3, a 1, NO 9, spare 2
+
+template <int KIND>
+__device__ __forceinline__ void plan_move(const SweepParams &P, const Stencil &S, const double *oo, const double *pt,
+                                          const double *pR, const int *pk, int mi, int cell, double *plan,
+                                          int *planI) {
+    const Grid &g = P.g;
+    const int kIdx = pk[mi];
+    V3 pos = v3(S.x[kIdx], S.y[kIdx], S.z[kIdx]);
+    V3 t = v3(pt[mi], pt[MOVE_CHUNK + mi], pt[2 * MOVE_CHUNK + mi]);
+    V3 npos = translate_wrapped(g.dims, g.inv, pos, t);
+    int nc[3];
+    unsigned nw;
+    bin_position(g, npos, nc, nw);
+    const int inside = cell_index(g, nc[0], nc[1], nc[2]) == cell;
+    double O[9], R[9], NO[9];
+    for (int c = 0; c < 9; c++) {
+        O[c] = oo[c * OWN_CAP + kIdx];
+        R[c] = pR[c * MOVE_CHUNK + mi];
+    }
+    mulm(R, O, NO);  // Shape::rotate (Shape.cpp:14-16)
+    const V3 nax = v3(NO[2], NO[5], NO[8]);
+    SCFirst f = make_first<KIND>(P.k, npos, nax);
+    if ((threadIdx.x & (WAVE - 1)) == 0) {
+        plan[0] = npos.x; plan[1] = npos.y; plan[2] = npos.z;
+        plan[3] = nax.x; plan[4] = nax.y; plan[5] = nax.z;
+        if (KIND == 1) {
+            plan[6] = f.capA.x; plan[7] = f.capA.y; plan[8] = f.capA.z;
+            plan[9] = f.u.x; plan[10] = f.u.y; plan[11] = f.u.z;
+            plan[12] = f.a;
+        }
+        for (int c = 0; c < 9; c++) plan[13 + c] = NO[c];
+        planI[0] = kIdx;
+        planI[1] = (int)nw;
+        planI[2] = inside;
+    }
+}
+
+template <int KIND, int WAVES, bool TRANSLATED>
+__device__ void sweep_cell_pipelined(const SweepParams &P, const Stencil &S, double *oo, double *pt, double *pR,
+                                     int *pk, int *queue, const double *sT, volatile int *vote, double *plans,
+                                     int *plansI, int own0, int nOwn, int total, int cell,
+                                     unsigned long long *clk) {
+    constexpr int NTT = WAVE * (WAVES - 1);  // tester lanes
+    const int tid = threadIdx.x;
+    const int lane = tid & (WAVE - 1);
+    const int wave = tid / WAVE;
+    const bool planner = wave == 0;
+    const int ttid = tid - WAVE;  // tester index
+    const int slots = (total + NTT - 1) / NTT;
+    int *myQueue = queue + (wave > 0 ? (wave - 1) : 0) * WAVE * slots;
+    unsigned long long accepted = 0, leftCell = 0, tests = 0;
+    int move = 0;
+    const int moves = P.moves > 0 ? P.moves : nOwn;
+    const int pbase = P.moves > 0 ? cell * P.moves : own0;
+
+    for (int chunk = 0; chunk < moves; chunk += MOVE_CHUNK) {
+        const int cn = min(MOVE_CHUNK, moves - chunk);
+        load_proposals(P, pbase, chunk, cn, pt, pR, pk);
+        __syncthreads();
+        if (planner) plan_move<KIND>(P, S, oo, pt, pR, pk, 0, cell, plans, plansI);
+        __syncthreads();
+
+        for (int mi = 0; mi < cn; mi++, move++) {
+            double *plan = plans + (mi & 1) * PLAN_D;
+            int *planI = plansI + (mi & 1) * 4;
+            // the pose is read before the vote: once a wavefront has passed it, the planner may overwrite this
+            // plan buffer with the plan of move mi + 2
+            const int kIdx = planI[0];
+            const int nw = planI[1];
+            const bool inside = planI[2] != 0;
+            const V3 npos = v3(plan[0], plan[1], plan[2]);
+            const V3 nax = v3(plan[3], plan[4], plan[5]);
+            bool hit = !inside;
+            if (planner) {
+                if (mi + 1 < cn)
+                    plan_move<KIND>(P, S, oo, pt, pR, pk, mi + 1, cell, plans + ((mi + 1) & 1) * PLAN_D,
+                                    plansI + ((mi + 1) & 1) * 4);
+            } else if (inside) {
+                const int wx = nw & 1, wy = (nw >> 1) & 1, wz = (nw >> 2) & 1;
+                bool definite = false;
+                int queued = 0;
+                for (int s0 = 0; s0 < slots; s0 += SLOT_GROUP) {
+                    V3 p2[SLOT_GROUP];
+                    int jj[SLOT_GROUP];
+#pragma unroll
+                    for (int s = 0; s < SLOT_GROUP; s++) {
+                        jj[s] = (s0 + s) * NTT + ttid;
+                        const int j = jj[s] < total ? jj[s] : 0;
+                        if (s0 + s < slots) {
+                            p2[s] = v3(S.x[j], S.y[j], S.z[j]);
+                            if (!TRANSLATED) p2[s] = add(p2[s], image_translation(sT, S.code[j], wx, wy, wz));
+                        }
+                    }
+#pragma unroll
+                    for (int s = 0; s < SLOT_GROUP; s++) {
+                        if (s0 + s >= slots) continue;
+                        const int j = jj[s];
+                        bool amb = false;
+                        if (j < total && j != kIdx) {
+                            V3 d = sub(p2[s], npos);
+                            double d2 = dot(d, d);
+                            if (d2 < P.k.contact2) definite = true;
+                            else if (KIND == 1 && d2 < P.k.range2) amb = true;
+                        }
+                        if (KIND == 1) {
+                            unsigned long long m = __ballot(amb);
+                            if (amb) myQueue[queued + __popcll(m & ((1ull << lane) - 1))] = j;
+                            queued += __popcll(m);
+                        }
+                    }
+                }
+                __builtin_amdgcn_wave_barrier();  // the queue was written by other lanes of this wavefront
+                hit = definite;
+                if (KIND == 1 && !__any(definite)) {
+                    SCFirst f;
+                    f.capA = v3(plan[6], plan[7], plan[8]);
+                    f.u = v3(plan[9], plan[10], plan[11]);
+                    f.a = plan[12];
+                    for (int q = lane; q < queued; q += WAVE) {
+                        const int j = myQueue[q];
+                        V3 p2 = v3(S.x[j], S.y[j], S.z[j]);
+                        if (!TRANSLATED) p2 = add(p2, image_translation(sT, S.code[j], wx, wy, wz));
+                        V3 a2 = v3(S.ax[j], S.ay[j], S.az[j]);
+                        V3 c2a = add(p2, scal(a2, -P.k.halfLen));
+                        V3 c2b = add(p2, scal(a2, P.k.halfLen));
+                        if (segment_distance2(f.capA, f.u, f.a, c2a, c2b) < P.k.contact2) hit = true;
+                    }
+                }
+            }
+            if (!inside) leftCell++;
+            else tests += (unsigned long long)total;
+            if (block_vote(hit, move, vote)) continue;
+
+            // Packing::acceptMove: every lane writes the same values, so the next move sees them without a barrier
+            accepted++;
+            S.x[kIdx] = npos.x;
+            S.y[kIdx] = npos.y;
+            S.z[kIdx] = npos.z;
+            if (KIND == 1) {
+                S.ax[kIdx] = nax.x;
+                S.ay[kIdx] = nax.y;
+                S.az[kIdx] = nax.z;
+            }
+            S.code[kIdx] = ((nw & 1) + 1) | ((((nw >> 1) & 1) + 1) << 2) | ((((nw >> 2) & 1) + 1) << 4);
+            if (planner) {
+                for (int c = 0; c < 9; c++) oo[c * OWN_CAP + kIdx] = plan[13 + c];
+            }
+            if (mi + 1 < cn && pk[mi + 1] == kIdx) {
+                // the next plan used the pose this move just replaced
+                __syncthreads();
+                if (planner)
+                    plan_move<KIND>(P, S, oo, pt, pR, pk, mi + 1, cell, plans + ((mi + 1) & 1) * PLAN_D,
+                                    plansI + ((mi + 1) & 1) * 4);
+                __syncthreads();
+            }
+        }
+        __syncthreads();
+    }
+    if (clk) clk[3] = wall_clock64();
+    for (int a = tid; a < nOwn; a += WAVE * WAVES) {
         const int gi = own0 + a;
         P.p.x[gi] = S.x[a];
         P.p.y[gi] = S.y[a];
@@ -794,7 +990,7 @@ __device__ __forceinline__ void stage_stencil_batched(const SweepParams &P, cons
             const int j = base + s * NT + tid;
             gi[s] = -1;
             if (j < total) {
-                int q = stencil_cell(S.start, 27, j);
+                int q = stencil_cell(S.start, S.ncells, j);
                 gi[s] = S.base[q] + (j - S.start[q]);
                 code[s] = S.ecode[q];
             }
@@ -833,44 +1029,46 @@ template <int KIND, int WAVES>
 __global__ __launch_bounds__(WAVE *WAVES) void k_sweep(SweepParams P) {
     constexpr int NT = WAVE * WAVES;
     extern __shared__ double smem[];
-    __shared__ int s_start[28];
-    __shared__ int s_base[27];
-    __shared__ int s_ecode[27];
+    __shared__ int s_start[MAX_STENCIL + 1];
+    __shared__ int s_base[MAX_STENCIL];
+    __shared__ int s_ecode[MAX_STENCIL];
     __shared__ int s_vote[2];
 
     const Grid &g = P.g;
     const int tid = threadIdx.x;
     const int b = blockIdx.x;
     const int bx = b % P.na[0], by = (b / P.na[0]) % P.na[1], bz = b / (P.na[0] * P.na[1]);
-    const int cx = 2 * bx + (P.colour & 1), cy = 2 * by + ((P.colour >> 1) & 1), cz = 2 * bz + ((P.colour >> 2) & 1);
+    const int per = g.period;
+    const int cx = per * bx + P.colour % per, cy = per * by + (P.colour / per) % per,
+              cz = per * bz + P.colour / (per * per);
     const int cell = cell_index(g, cx, cy, cz);
     const int own0 = g.cellStart[cell];
     const int nOwn = g.cellStart[cell + 1] - own0;
     if (nOwn == 0) return;
+    unsigned long long clk[5];
+    if (P.phases) clk[0] = wall_clock64();
 
-    if (tid < 27) stencil_table(g, cx, cy, cz, tid, false, s_start, s_base, s_ecode);
+    const int ncells = stencil_cells(g.reach, false);
     if (tid < 2) s_vote[tid] = 0;
-    __syncthreads();
-    if (tid == 0) {
-        s_start[0] = 0;
-        for (int q = 1; q <= 27; q++) s_start[q] += s_start[q - 1];
-    }
-    __syncthreads();
-    const int total = s_start[27];
+    build_stencil(g, cx, cy, cz, ncells, false, s_start, s_base, s_ecode);
+    const int total = s_start[ncells];
+    if (P.phases) clk[1] = wall_clock64();
 
     Stencil S;
     S.start = s_start;
     S.base = s_base;
     S.ecode = s_ecode;
-    S.ncells = 27;
+    S.ncells = ncells;
     double *rest;
     carve_stencil(S, smem, P.cap, KIND, rest);
     double *oo = rest;                  // 9 x OWN_CAP own orientations
     double *pt = oo + 9 * OWN_CAP;      // 3 x MOVE_CHUNK proposal translations
     double *pR = pt + 3 * MOVE_CHUNK;   // 9 x MOVE_CHUNK proposal rotations
     double *sT = pR + 9 * MOVE_CHUNK;   // 81 translation table
-    int *pk = reinterpret_cast<int *>(sT + 81);
-    int *queue = pk + MOVE_CHUNK;       // cap + 64*WAVES ambiguous-candidate slots
+    double *plans = sT + 81;            // 2 x PLAN_D planner output
+    int *pk = reinterpret_cast<int *>(plans + 2 * 24);
+    int *plansI = pk + MOVE_CHUNK;      // 2 x 4 planner output
+    int *queue = plansI + 8;            // cap + 64*WAVES ambiguous-candidate slots
     for (int i = tid; i < 81; i += NT) sT[i] = g.T[i];
     __syncthreads();
 
@@ -883,13 +1081,52 @@ __global__ __launch_bounds__(WAVE *WAVES) void k_sweep(SweepParams P) {
             oo[c * OWN_CAP + a] = P.p.o[c][own0 + a];
         }
         __syncthreads();
-        if (translated)
-            sweep_cell_staged<KIND, WAVES, true>(P, S, oo, pt, pR, pk, queue, sT, s_vote, cell, own0, nOwn, total);
-        else
-            sweep_cell_staged<KIND, WAVES, false>(P, S, oo, pt, pR, pk, queue, sT, s_vote, cell, own0, nOwn, total);
+        if (P.phases) clk[2] = wall_clock64();
+        unsigned long long *c3 = P.phases ? clk : nullptr;
+        if (WAVES >= 2 && P.pipelined) {
+            if (translated)
+                sweep_cell_pipelined<KIND, WAVES, true>(P, S, oo, pt, pR, pk, queue, sT, s_vote, plans, plansI, own0,
+                                                        nOwn, total, cell, c3);
+            else
+                sweep_cell_pipelined<KIND, WAVES, false>(P, S, oo, pt, pR, pk, queue, sT, s_vote, plans, plansI, own0,
+                                                         nOwn, total, cell, c3);
+        } else if (translated) {
+            sweep_cell_staged<KIND, WAVES, true>(P, S, oo, pt, pR, pk, queue, sT, s_vote, cell, own0, nOwn, total, c3);
+        } else {
+            sweep_cell_staged<KIND, WAVES, false>(P, S, oo, pt, pR, pk, queue, sT, s_vote, cell, own0, nOwn, total, c3);
+        }
     } else {
+        if (P.phases) clk[2] = wall_clock64();
         sweep_cell_unstaged<KIND, WAVES>(P, S, pt, pR, pk, sT, cell, own0, nOwn, total);
+        if (P.phases) clk[3] = wall_clock64();
     }
+    if (P.phases && tid == 0) {
+        clk[4] = wall_clock64();
+        atomicAdd(&P.phases[RP_PHASE_TABLE], clk[1] - clk[0]);
+        atomicAdd(&P.phases[RP_PHASE_STAGE], clk[2] - clk[1]);
+        atomicAdd(&P.phases[RP_PHASE_MOVES], clk[3] - clk[2]);
+        atomicAdd(&P.phases[RP_PHASE_WRITE], clk[4] - clk[3]);
+        atomicAdd(&P.phases[RP_PHASE_BLOCKS], 1ull);
+        atomicMax(&P.phases[RP_PHASE_LAST], clk[4] - clk[0]);
+    }
+}
+
+int sweep_occupancy(int kind, int cap, int waves) {
+    int w = waves >= 4 ? 4 : (waves == 2 ? 2 : 1);
+    size_t lds = sweep_lds_bytes(kind, cap, w);
+    int blocks = 0;
+#define RP_OCC(KIND, W) hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_sweep<KIND, W>, WAVE * W, lds)
+    if (kind == 0) {
+        if (w == 4) RP_OCC(0, 4);
+        else if (w == 2) RP_OCC(0, 2);
+        else RP_OCC(0, 1);
+    } else {
+        if (w == 4) RP_OCC(1, 4);
+        else if (w == 2) RP_OCC(1, 2);
+        else RP_OCC(1, 1);
+    }
+#undef RP_OCC
+    return blocks;
 }
 
 hipError_t launch_sweep(hipStream_t s, const SweepParams &sp, int waves) {

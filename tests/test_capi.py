@@ -48,4 +48,4 @@ def test_structure_layouts_match_header():
     assert ctypes.sizeof(ra.MoveDesc) == 24
     assert ctypes.sizeof(ra.BoxDesc) == 16
     assert ctypes.sizeof(ra.Counters) == 10 * 8 + 3 * 8
-    assert ctypes.sizeof(ra.Timing) == 11 * 8
+    assert ctypes.sizeof(ra.Timing) == 12 * 8

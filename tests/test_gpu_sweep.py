@@ -159,11 +159,13 @@ def test_degenerate_hard_sphere_gas_carnahan_starling():
     assert err / rho < 0.03
 
 
-@pytest.mark.parametrize("waves", [1, 2, 4])
+@pytest.mark.parametrize("waves,pipe", [(1, 0), (2, 0), (4, 0), (2, 1), (4, 1)])
 @pytest.mark.parametrize("kind", ["sphere", "spherocylinder"])
-def test_every_launch_shape_is_overlap_free(kind, waves, monkeypatch):
-    """Each workgroup size (1, 2 or 4 wavefronts per active cell) for both shapes keeps the packing overlap-free."""
+def test_every_launch_shape_is_overlap_free(kind, waves, pipe, monkeypatch):
+    """Each workgroup size (1, 2 or 4 wavefronts per active cell, with and without the planner/tester split) for
+    both shapes keeps the packing overlap-free."""
     monkeypatch.setenv("RAMPACK_SWEEP_WAVES", str(waves))
+    monkeypatch.setenv("RAMPACK_SWEEP_PIPELINE", str(pipe))
     if kind == "sphere":
         shape = HS
         box, pos, rot = lattice.hard_sphere_lattice(10, 0.45)
@@ -172,9 +174,55 @@ def test_every_launch_shape_is_overlap_free(kind, waves, monkeypatch):
         shape = SC
         box, pos, rot = rod_start()
         move = ("delta_triclinic", 0.002)
-    pk = ra.GpuPacking(shape, box, pos, rot, seed=100 + waves)
+    pk = ra.GpuPacking(shape, box, pos, rot, seed=100 + waves + 10 * pipe)
     pk.set_environment(1.0, 5.0 if kind == "sphere" else 2.0, 0.15, None, box_move=move)
     c = pk.run_cycles(100, adapt_steps=True)
     assert abs(c["moves_attempted"] - 100 * len(pos)) <= 0.1 * 100 * len(pos)
     b, p, r = pk.download()
     assert overlaps_by_oracle(shape, b, p, r) == 0
+
+
+@pytest.mark.parametrize("period", [2, 3])
+def test_both_colourings_are_overlap_free(period, monkeypatch):
+    """Period-2 (cells >= range, 8 colours, 27-cell stencil) and period-3 (cells >= range/2, 27 colours, 125-cell
+    stencil) checkerboards both keep the packing overlap-free, including the box-move scans on their grids."""
+    monkeypatch.setenv("RAMPACK_COLOURING", str(period))
+    box, pos, rot = rod_start()
+    pk = ra.GpuPacking(SC, box, pos, rot, seed=40 + period)
+    pk.set_environment(1.0, 2.0, 0.2, None, box_move=("delta_triclinic", 0.002))
+    pk.set_timing(True)
+    c = pk.run_cycles(120, adapt_steps=True)
+    t = pk.timing()
+    assert t["sweep_period"] == period
+    assert t["sweep_launches"] == 120 * period ** 3
+    assert c["box_accepted"] > 0
+    b, p, r = pk.download()
+    assert overlaps_by_oracle(SC, b, p, r) == 0
+
+
+@pytest.mark.parametrize("kind", ["sphere", "spherocylinder"])
+def test_launch_shapes_give_identical_trajectories(kind, monkeypatch):
+    """Proposals are drawn per (cycle, cell, move) and every decision is the same FP64 evaluation, so the
+    single-wavefront, multi-wavefront and pipelined (planner + testers) sweeps produce bit-identical packings."""
+    if kind == "sphere":
+        shape = HS
+        box, pos, rot = lattice.hard_sphere_lattice(10, 0.45)
+        move = ("log", 0.001)
+    else:
+        shape = SC
+        box, pos, rot = rod_start()
+        move = ("delta_triclinic", 0.002)
+    results = {}
+    for waves, pipe in ((1, 0), (2, 0), (4, 0), (2, 1), (4, 1)):
+        monkeypatch.setenv("RAMPACK_SWEEP_WAVES", str(waves))
+        monkeypatch.setenv("RAMPACK_SWEEP_PIPELINE", str(pipe))
+        pk = ra.GpuPacking(shape, box, pos, rot, seed=77)
+        pk.set_environment(1.0, 5.0 if kind == "sphere" else 2.0, 0.15, None, box_move=move)
+        c = pk.run_cycles(40, adapt_steps=True)
+        results[(waves, pipe)] = (pk.download(), c["moves_accepted"])
+        pk.close()
+    (b0, p0, r0), acc0 = results[(1, 0)]
+    assert acc0 > 0
+    for key, ((b, p, r), acc) in results.items():
+        assert acc == acc0, key
+        assert np.array_equal(b, b0) and np.array_equal(p, p0) and np.array_equal(r, r0), key
