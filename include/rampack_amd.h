@@ -100,6 +100,15 @@ typedef struct rp_timing {
     uint64_t sweep_period;       /* checkerboard colour period (colours = period^3) in the last cycle */
 } rp_timing;
 
+/* Observables of the current state (src/core/observables/PackingFraction.h:19-23, NematicOrder.cpp:9-60). */
+typedef struct rp_observables {
+    double packing_fraction;  /* N v0 / V (Packing::getPackingFraction, Packing.cpp:225-232) */
+    double number_density;    /* N / V */
+    double volume;            /* |(s1 x s2) . s3| (TriclinicBox::getVolume, TriclinicBox.cpp:35-38) */
+    double nematic_p2;        /* largest-magnitude eigenvalue of Q; NaN for spheres (no primary axis) */
+    double q_tensor[6];       /* Q_11 Q_12 Q_13 Q_22 Q_23 Q_33 (NematicOrder::getIntervalValues order) */
+} rp_observables;
+
 /* Packing constructor + Simulation constructor (src/core/Packing.cpp:46-82, src/core/Simulation.cpp:54-76):
  * uploads n shapes into a triclinic periodic box.  Every position must lie inside the box (Packing.cpp:1226-1232).
  * seed keys the counter-based random streams. */
@@ -122,6 +131,11 @@ int rp_count_total_overlaps(rp_ctx *ctx, int early_exit, uint64_t *out);
 
 /* Current state, in the caller's particle order (Packing::operator[], Packing::getBox). */
 int rp_download(rp_ctx *ctx, double box[9], double *pos, double *rot);
+
+/* PackingFraction::calculate and NematicOrder::calculate on the device-resident state: the Q-tensor sum runs on the
+ * GPU (a fixed reduction tree, so the same state always gives the same bits; the reference sums sequentially, so
+ * results agree to the rounding of an N-term sum), the 3x3 eigenvalue step on the host. */
+int rp_get_observables(rp_ctx *ctx, rp_observables *out);
 
 /* Turns event timing of the sweep kernels on or off and reads the accumulated statistics. */
 int rp_set_timing(rp_ctx *ctx, int enable);

@@ -127,6 +127,21 @@ int ref_snapshot_stats(int kind, double radius, double length, int nCentres, con
     }
 }
 
+// PackingFraction::calculate (PackingFraction.h:19-23) = Packing::getPackingFraction (Packing.cpp:225-232) with the
+// shape's ShapeGeometry::getVolume and TriclinicBox::getVolume (TriclinicBox.cpp:35-38).
+double ref_packing_fraction(int kind, double radius, double length, const double *box, std::size_t n) {
+    auto traits = makeTraits(kind, radius, length, 0, nullptr, nullptr, 0, 1, 1);
+    std::vector<double> pos(3 * n, 0.0), rot(9 * n, 0.0);
+    for (std::size_t i = 0; i < n; i++) rot[9 * i] = rot[9 * i + 4] = rot[9 * i + 8] = 1;
+    TriclinicBox b(mat(box));
+    auto side = b.getSides();
+    for (std::size_t i = 0; i < n; i++)  // any in-box position: only the count matters
+        for (int d = 0; d < 3; d++) pos[3 * i + d] = 0.5 * side[0][d] + 0.25 * side[1][d] + 0.125 * side[2][d];
+    Packing packing(b, makeShapes(n, pos.data(), rot.data()), std::make_unique<PeriodicBoundaryConditions>(),
+                    traits->getInteraction(), 1, 1);
+    return packing.getPackingFraction(traits->getGeometry().getVolume());
+}
+
 double ref_nematic_p2(std::size_t n, const double *rot) {
     SpherocylinderTraits traits(1, 1);
     std::vector<double> pos(3 * n, 0.5);

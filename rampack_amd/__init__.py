@@ -9,6 +9,7 @@ There is no CPU fallback: if the HIP library is missing or no GPU is visible, ev
 from __future__ import annotations
 
 import ctypes
+import math
 import os
 from dataclasses import dataclass
 from typing import Optional, Sequence
@@ -35,7 +36,7 @@ RP_ERR_UNSUPPORTED = -4
 # Every symbol include/rampack_amd.h declares.
 EXPORTED_SYMBOLS = (
     "rp_create", "rp_set_environment", "rp_run_cycles", "rp_count_total_overlaps", "rp_download",
-    "rp_set_timing", "rp_get_timing", "rp_destroy", "rp_pair_decisions", "rp_last_error",
+    "rp_set_timing", "rp_get_timing", "rp_destroy", "rp_pair_decisions", "rp_last_error", "rp_get_observables",
 )
 
 
@@ -72,6 +73,16 @@ class Timing(ctypes.Structure):
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class Observables(ctypes.Structure):
+    _fields_ = [("packing_fraction", ctypes.c_double), ("number_density", ctypes.c_double),
+                ("volume", ctypes.c_double), ("nematic_p2", ctypes.c_double), ("q_tensor", ctypes.c_double * 6)]
+
+    def as_dict(self):
+        d = {name: getattr(self, name) for name, _ in self._fields_ if name != "q_tensor"}
+        d["q_tensor"] = list(self.q_tensor)
+        return d
 
 
 class RampackError(RuntimeError):
@@ -116,6 +127,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.rp_destroy.restype = None
     lib.rp_pair_decisions.argtypes = [ctypes.POINTER(ShapeDesc), ctypes.c_size_t, dp,
                                       ctypes.POINTER(ctypes.c_uint8)]
+    lib.rp_get_observables.argtypes = [vp, ctypes.POINTER(Observables)]
     lib.rp_last_error.restype = ctypes.c_char_p
     _lib = lib
     return lib
@@ -239,9 +251,33 @@ class GpuPacking:
         _check(_lib.rp_get_timing(self._ctx, ctypes.byref(t)))
         return t.as_dict()
 
+    def observables(self) -> dict:
+        """PackingFraction and NematicOrder of the device-resident state (rp_get_observables)."""
+        o = Observables()
+        _check(_lib.rp_get_observables(self._ctx, ctypes.byref(o)))
+        return o.as_dict()
+
     def packing_fraction(self) -> float:
-        box, _, _ = self.download()
-        return self.n * self.shape.volume / abs(np.linalg.det(box))
+        return self.observables()["packing_fraction"]
+
+
+def mean_and_error(samples) -> tuple:
+    """Quantity::calculateFromSamples (src/utils/Quantity.cpp:45-62): the mean and the naive standard error of the
+    mean, summed in sample order as the reference does."""
+    s = [float(x) for x in samples]
+    if not s:
+        return 0.0, 0.0
+    if len(s) == 1:
+        return s[0], 0.0
+    acc = 0.0
+    for x in s:
+        acc += x
+    value = acc / len(s)
+    err = 0.0
+    for x in s:
+        err += (value - x) ** 2
+    err = err / len(s) / (len(s) - 1)
+    return value, math.sqrt(err)
 
 
 def nematic_p2(rot: np.ndarray) -> float:

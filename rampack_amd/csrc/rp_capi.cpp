@@ -84,6 +84,7 @@ struct rp_ctx {
     Proposals prop{};
     size_t propCap = 0;
     void *propMem = nullptr;
+    double *obsMem = nullptr;            // Q-tensor partial sums
     unsigned long long *dCounters = nullptr;
     unsigned long long *dPhases = nullptr;  // sweep phase timers (RAMPACK_SWEEP_PHASES=1)
     unsigned long long *hCounters = nullptr;  // pinned
@@ -731,6 +732,71 @@ int rp_count_total_overlaps(rp_ctx *c, int earlyExit, uint64_t *out) {
     return countExact(c, earlyExit, *out);
 }
 
+namespace {
+
+// NematicOrder::calculateEigenvalues (NematicOrder.cpp:31-54, the Cardano method of arXiv:1306.6291) and the
+// largest-magnitude pick of NematicOrder::calculate (NematicOrder.cpp:22-27).  A Q tensor proportional to the
+// identity (p = 0, where the reference asserts) gives 0.
+double nematicP2(const double Q[9]) {
+    double A11 = Q[0], A12 = Q[1], A13 = Q[2], A22 = Q[4], A23 = Q[5], A33 = Q[8];
+    double b = A11 + A22 + A33;
+    double c = A11 * A22 + A11 * A33 + A22 * A33 - A12 * A12 - A13 * A13 - A23 * A23;
+    double d = A11 * A23 * A23 + A22 * A13 * A13 + A33 * A12 * A12 - A11 * A22 * A33 - 2 * A12 * A13 * A23;
+    double p = b * b - 3 * c;
+    if (!(p > 0)) return 0;
+    double sp = std::sqrt(p);
+    double q = 2 * b * b * b - 9 * b * c - 27 * d;
+    double ratio = q / 2 / (sp * sp * sp);
+    if (std::fabs(ratio) > 1) ratio = ratio < 0 ? -1 : 1;
+    double delta = std::acos(ratio);
+    double e[3] = {1. / 3 * (b + 2 * sp * std::cos(delta / 3)), 1. / 3 * (b + 2 * sp * std::cos((delta + 2 * M_PI) / 3)),
+                   1. / 3 * (b + 2 * sp * std::cos((delta - 2 * M_PI) / 3))};
+    double best = e[0];
+    for (int i = 1; i < 3; i++)
+        if (std::fabs(best) < std::fabs(e[i])) best = e[i];
+    return best;
+}
+
+// ShapeGeometry::getVolume: SphereTraits.cpp:31-33, SpherocylinderTraits.cpp:48-50
+double shapeVolume(const rp_shape_desc &s) {
+    if (s.kind == RP_SHAPE_SPHERE) return 4. / 3 * M_PI * std::pow(s.radius, 3);
+    return M_PI * s.radius * s.radius * s.length + 4. / 3 * M_PI * std::pow(s.radius, 3);
+}
+
+}  // namespace
+
+int rp_get_observables(rp_ctx *c, rp_observables *out) {
+    if (!c || !out) return fail(RP_ERR_PRECONDITION, "null argument");
+    *out = rp_observables{};
+    // Packing::getPackingFraction / getNumberDensity (Packing.cpp:225-232)
+    out->volume = c->box.volume();
+    out->number_density = (double)c->n / out->volume;
+    out->packing_fraction = out->number_density * shapeVolume(c->shape);
+    if (c->shape.kind == RP_SHAPE_SPHERE) {
+        // a sphere has no primary axis (ShapeGeometry::getPrimaryAxis is not provided by SphereTraits)
+        out->nematic_p2 = std::nan("");
+        for (double &q : out->q_tensor) q = std::nan("");
+        return RP_OK;
+    }
+    if (!c->obsMem) RP_HIP(hipMalloc(&c->obsMem, sizeof(double) * nematic_scratch_doubles()));
+    RP_HIP(launch_nematic_sums(c->stream, c->cur.p, (int)c->n, c->obsMem));
+    double sums[6];
+    RP_HIP(hipMemcpyAsync(sums, c->obsMem + nematic_scratch_doubles() - 6, sizeof sums, hipMemcpyDeviceToHost,
+                          c->stream));
+    RP_HIP(hipStreamSynchronize(c->stream));
+    // Q /= N; Q = 0.5 (3 Q - I)  (NematicOrder.cpp:18-20)
+    const int idx[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
+    double Q[9];
+    for (int i = 0; i < 9; i++) {
+        double v = sums[idx[i]] / (double)c->n;
+        Q[i] = 0.5 * (3. * v - (i % 4 == 0 ? 1.0 : 0.0));
+    }
+    const int up[6] = {0, 1, 2, 4, 5, 8};
+    for (int k = 0; k < 6; k++) out->q_tensor[k] = Q[up[k]];
+    out->nematic_p2 = nematicP2(Q);
+    return RP_OK;
+}
+
 int rp_download(rp_ctx *c, double box[9], double *pos, double *rot) {
     if (!c) return fail(RP_ERR_PRECONDITION, "null context");
     RP_HIP(hipStreamSynchronize(c->stream));
@@ -808,6 +874,7 @@ void rp_destroy(rp_ctx *c) {
     if (c->cellStart) hipFree(c->cellStart);
     if (c->cellCount) hipFree(c->cellCount);
     if (c->propMem) hipFree(c->propMem);
+    if (c->obsMem) hipFree(c->obsMem);
     if (c->tmpSlot) hipFree(c->tmpSlot);
     if (c->tmpOrder) hipFree(c->tmpOrder);
     if (c->tmpPos) hipFree(c->tmpPos);

@@ -98,6 +98,11 @@ size_t count_lds_bytes(int kind, int cap);
 hipError_t launch_proposals(hipStream_t s, const Grid &g, const Proposals &pr, int moves, const int *cellOf, int n,
                             double ts, double rs, uint64_t seed, uint32_t cycle);
 // waves: wavefronts per active cell (1, 2 or 4)
+// Sums of a_i a_j over the primary axes (orientation column 2), Q_11 Q_12 Q_13 Q_22 Q_23 Q_33 order, written to
+// scratch[6 * 256 .. 6 * 256 + 6); scratch holds nematic_scratch_doubles() doubles.
+size_t nematic_scratch_doubles();
+hipError_t launch_nematic_sums(hipStream_t s, const Particles &p, int n, double *scratch);
+
 // Workgroups of the sweep kernel one compute unit can hold at this LDS size (occupancy API).
 int sweep_occupancy(int kind, int cap, int waves);
 hipError_t launch_sweep(hipStream_t s, const SweepParams &sp, int waves);

@@ -1111,6 +1111,59 @@ __global__ __launch_bounds__(WAVE *WAVES) void k_sweep(SweepParams P) {
     }
 }
 
+// ---------------------------------------------------------------------------------------------------------------
+// Nematic Q-tensor sums (NematicOrder::calculate, NematicOrder.cpp:9-17): sum over particles of a_i a_j for the
+// primary axis a = orientation column 2 (SpherocylinderTraits.cpp:52-54).  Each product is rounded once, as in the
+// reference; the sum runs as a fixed tree (block partials in a fixed order, then one workgroup), so a given state
+// always gives the same bits.
+
+constexpr int NEMATIC_THREADS = 256;
+constexpr int NEMATIC_BLOCKS = 256;
+
+__device__ __forceinline__ void block_sum6(double v[6], double *lds) {
+    const int tid = threadIdx.x;
+    for (int c = 0; c < 6; c++) lds[c * NEMATIC_THREADS + tid] = v[c];
+    __syncthreads();
+    for (int w = NEMATIC_THREADS / 2; w > 0; w >>= 1) {
+        if (tid < w)
+            for (int c = 0; c < 6; c++) lds[c * NEMATIC_THREADS + tid] += lds[c * NEMATIC_THREADS + tid + w];
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(NEMATIC_THREADS) void k_nematic_partial(Particles p, int n, double *partial) {
+    __shared__ double lds[6 * NEMATIC_THREADS];
+    double v[6] = {0, 0, 0, 0, 0, 0};
+    for (int i = blockIdx.x * NEMATIC_THREADS + threadIdx.x; i < n; i += NEMATIC_BLOCKS * NEMATIC_THREADS) {
+        const double a0 = p.o[2][i], a1 = p.o[5][i], a2 = p.o[8][i];
+        v[0] += a0 * a0;
+        v[1] += a0 * a1;
+        v[2] += a0 * a2;
+        v[3] += a1 * a1;
+        v[4] += a1 * a2;
+        v[5] += a2 * a2;
+    }
+    block_sum6(v, lds);
+    if (threadIdx.x < 6) partial[threadIdx.x * NEMATIC_BLOCKS + blockIdx.x] = lds[threadIdx.x * NEMATIC_THREADS];
+}
+
+__global__ __launch_bounds__(NEMATIC_THREADS) void k_nematic_final(const double *partial, double *out) {
+    __shared__ double lds[6 * NEMATIC_THREADS];
+    double v[6];
+    for (int c = 0; c < 6; c++) v[c] = partial[c * NEMATIC_BLOCKS + threadIdx.x];
+    block_sum6(v, lds);
+    if (threadIdx.x < 6) out[threadIdx.x] = lds[threadIdx.x * NEMATIC_THREADS];
+}
+
+size_t nematic_scratch_doubles() { return 6 * NEMATIC_BLOCKS + 6; }
+
+hipError_t launch_nematic_sums(hipStream_t s, const Particles &p, int n, double *scratch) {
+    static_assert(NEMATIC_BLOCKS == NEMATIC_THREADS, "the final pass reads one partial per thread");
+    hipLaunchKernelGGL(k_nematic_partial, dim3(NEMATIC_BLOCKS), dim3(NEMATIC_THREADS), 0, s, p, n, scratch);
+    hipLaunchKernelGGL(k_nematic_final, dim3(1), dim3(NEMATIC_THREADS), 0, s, scratch, scratch + 6 * NEMATIC_BLOCKS);
+    return hipGetLastError();
+}
+
 int sweep_occupancy(int kind, int cap, int waves) {
     int w = waves >= 4 ? 4 : (waves == 2 ? 2 : 1);
     size_t lds = sweep_lds_bytes(kind, cap, w);

@@ -105,6 +105,8 @@ def _setup_ref(lib):
                                        ctypes.c_int, _u64p, _dp]
     lib.ref_nematic_p2.argtypes = [ctypes.c_size_t, _dp]
     lib.ref_nematic_p2.restype = ctypes.c_double
+    lib.ref_packing_fraction.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, ctypes.c_size_t]
+    lib.ref_packing_fraction.restype = ctypes.c_double
     lib.ref_mc_run.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, ctypes.c_size_t, _dp, _dp,
                                ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                ctypes.c_double, ctypes.c_ulong, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,

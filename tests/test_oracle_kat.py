@@ -199,3 +199,30 @@ def test_count_equals_brute_force_on_small_boxes():
         brute = o.lib.orc_count_brute(SC, 0.5, 5.0, 0, None, None, _d(box.ravel()), n, _d(pos), _d(rot))
         assert count == brute
         assert count > 0
+
+
+def test_mean_and_error_closed_form():
+    """Quantity::calculateFromSamples (Quantity.cpp:45-62): empty -> (0, 0); one sample -> (x, 0); [1, 2, 3, 4] ->
+    mean 2.5 and sqrt(sum (x - 2.5)^2 / (4 * 3)) = sqrt(5 / 12)."""
+    import math
+    import rampack_amd as ra
+    assert ra.mean_and_error([]) == (0.0, 0.0)
+    assert ra.mean_and_error([3.25]) == (3.25, 0.0)
+    m, e = ra.mean_and_error([1.0, 2.0, 3.0, 4.0])
+    assert m == 2.5
+    assert abs(e - math.sqrt(5 / 12)) < 1e-16
+
+
+def test_reference_packing_fraction_closed_form():
+    """The reference's Packing::getPackingFraction on a triclinic box equals N v0 / |det(box)| with
+    v0 = pi r^2 L + 4/3 pi r^3 (SpherocylinderTraits.cpp:48-50)."""
+    import math
+    import numpy as np
+    import oracle_lib
+    if not oracle_lib.ref_available():
+        import pytest
+        pytest.skip("reference build absent")
+    box = np.array([[50.0, 2.0, -1.0], [0.0, 48.0, 1.5], [0.0, 0.0, 52.0]])
+    phi = oracle_lib.ref().lib.ref_packing_fraction(1, 0.5, 5.0, oracle_lib._d(box.ravel()), 1000)
+    v0 = math.pi * 0.25 * 5.0 + 4 / 3 * math.pi * 0.125
+    assert abs(phi - 1000 * v0 / abs(np.linalg.det(box))) < 1e-15
