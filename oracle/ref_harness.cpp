@@ -110,6 +110,23 @@ int ref_pair_decisions(int kind, double radius, double length, std::size_t n, co
     return 0;
 }
 
+// Interaction::calculateEnergyBetween through the reference's RepulsiveLennardJonesInteraction on a sphere
+// (CentralInteraction.h:58-67, RepulsiveLennardJonesInteraction.cpp:8-16) with a fixed cell translation.  Only
+// soft == 1 exists in the reference; other values return -1.
+int ref_pair_energies(int kind, double radius, double length, int soft, double eps, double sigma, std::size_t n,
+                      const double *rec, double *out) {
+    if (kind != 0 || soft != 1) return -1;
+    auto traits = makeTraits(kind, radius, length, 0, nullptr, nullptr, soft, eps, sigma);
+    const Interaction &inter = traits->getInteraction();
+    for (std::size_t i = 0; i < n; i++) {
+        const double *q = rec + 27 * i;
+        Vector<3> p1{q[0], q[1], q[2]}, p2{q[12], q[13], q[14]};
+        FixedTranslation bc(Vector<3>{q[24], q[25], q[26]});
+        out[i] = inter.calculateEnergyBetween(p1, mat(q + 3), 0, p2, mat(q + 15), 0, bc);
+    }
+    return 0;
+}
+
 int ref_snapshot_stats(int kind, double radius, double length, int nCentres, const double *centres,
                        const double *radii, int soft, double eps, double sigma, const double *box, std::size_t n,
                        const double *pos, const double *rot, int earlyExit, std::uint64_t *overlaps,

@@ -241,10 +241,13 @@ struct ShapeSpec {
     int soft = 0;                   // 0 none, 1 WCA on centre distance, 2 WCA on segment distance (no reference)
     double eps = 1, sigma = 1;
 
-    // SphereTraits.h:34, SpherocylinderTraits.h:81, PolysphereTraits.cpp:116-121
+    // SphereTraits.h:34, SpherocylinderTraits.h:81, PolysphereTraits.cpp:116-121; a WCA sphere takes the range of
+    // its interaction (RepulsiveLennardJonesInteraction.h:26), a WCA spherocylinder (segment distance, no reference)
+    // L + sigma 2^(1/6), the farthest centre distance at which two segments can be within the cut-off
     double range() const {
-        if (kind == SPHERE) return 2 * radius;
-        if (kind == SPHEROCYLINDER) return 2 * radius + length;
+        const double cut = sigma * std::pow(2., 1. / 6);
+        if (kind == SPHERE) return soft == 1 ? cut : 2 * radius;
+        if (kind == SPHEROCYLINDER) return soft == 2 ? length + cut : 2 * radius + length;
         double m = 0;
         for (double r : radii) m = std::max(m, r);
         double rr = 2 * m;
@@ -259,7 +262,8 @@ struct ShapeSpec {
         return 2 * std::sqrt(maxR2) + range();
     }
     int nCentres() const { return kind == POLYSPHERE ? (int)centres.size() : 0; }
-    bool hasHard() const { return true; }
+    // a WCA interaction replaces the hard core of a sphere or spherocylinder (CentralInteraction.h:48-51)
+    bool hasHard() const { return kind == POLYSPHERE || soft == 0; }
 };
 
 // Cap centre: SpherocylinderTraits.cpp:24-26
@@ -467,7 +471,11 @@ static std::uint64_t countTotal(const Snapshot &S, bool earlyExit, double *energ
             for (std::size_t j = i + 1; j < S.n; j++) {
                 if (!nc) {
                     if (sp.kind == SPHERE) {
-                        if (pbcDistance2(S.box, S.pos[i], S.pos[j]) < std::pow(2 * sp.radius, 2)) count++;
+                        // the real PBC distance (PeriodicBoundaryConditions.cpp:38-50) feeds both the overlap test
+                        // and CentralInteraction::calculateEnergyBetween (CentralInteraction.h:58-67)
+                        double d2 = pbcDistance2(S.box, S.pos[i], S.pos[j]);
+                        if (sp.hasHard() && d2 < std::pow(2 * sp.radius, 2)) count++;
+                        if (energyOut && sp.soft == 1) energy += wcaEnergy(sp.eps, sp.sigma, d2);
                     } else {
                         Vec T = pbcTranslation(S.box, S.pos[i], S.pos[j]);
                         if (pairOverlap(sp, S.pos[i], S.rot[i], 0, S.pos[j], S.rot[j], 0, T)) count++;
@@ -582,6 +590,23 @@ int orc_pair_decisions(int kind, double radius, double length, std::size_t n, co
     return 0;
 }
 
+// Soft pair energies with a fixed translation per record (27 doubles as in orc_pair_decisions): soft 1 = WCA on the
+// centre distance (CentralInteraction.h:58-67 with HardcodedTranslation), soft 2 = WCA on the segment distance.
+int orc_pair_energies(int kind, double radius, double length, int soft, double eps, double sigma, std::size_t n,
+                      const double *rec, double *out) {
+    ShapeSpec s = makeSpec(kind, radius, length, 0, nullptr, nullptr, soft, eps, sigma);
+    for (std::size_t i = 0; i < n; i++) {
+        const double *q = rec + 27 * i;
+        Vec p1, p2, T;
+        Mat O1, O2;
+        std::memcpy(p1.v, q, 24); std::memcpy(O1.m, q + 3, 72);
+        std::memcpy(p2.v, q + 12, 24); std::memcpy(O2.m, q + 15, 72);
+        std::memcpy(T.v, q + 24, 24);
+        out[i] = pairEnergy(s, p1, O1, p2, O2, T);
+    }
+    return 0;
+}
+
 // Squared segment distance for 4 points per record (12 doubles).
 void orc_segment_distance2(std::size_t n, const double *pts, double *out) {
     for (std::size_t i = 0; i < n; i++) {
@@ -598,7 +623,7 @@ int orc_snapshot_stats(int kind, double radius, double length, int nCentres, con
                        double *energy) {
     ShapeSpec s = makeSpec(kind, radius, length, nCentres, centres, radii, soft, eps, sigma);
     Snapshot S(s, box, n, pos, rot);
-    *overlaps = countTotal(S, earlyExit != 0, nullptr);
+    *overlaps = s.hasHard() ? countTotal(S, earlyExit != 0, nullptr) : 0;
     if (energy) {
         double e = 0;
         if (soft) countTotal(S, false, &e);

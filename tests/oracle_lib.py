@@ -29,6 +29,8 @@ def _setup_common(lib):
     lib.orc_count_brute.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp, _dp, _dp,
                                     ctypes.c_size_t, _dp, _dp]
     lib.orc_count_brute.restype = ctypes.c_uint64
+    lib.orc_pair_energies.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double,
+                                      ctypes.c_double, ctypes.c_size_t, _dp, _dp]
     lib.orc_segment_distance2.argtypes = [ctypes.c_size_t, _dp, _dp]
     lib.orc_segment_distance2.restype = None
     lib.orc_box_info.argtypes = [_dp, _dp, _dp, _dp]
@@ -58,6 +60,16 @@ class Checker:
         out = np.zeros(len(rec), dtype=np.uint8)
         f = getattr(self.lib, self.prefix + "pair_decisions")
         f(kind, radius, length, len(rec), rec.ctypes.data_as(_dp), out.ctypes.data_as(_u8p))
+        return out
+
+    def pair_energies(self, kind: int, radius: float, length: float, soft: int, eps: float, sigma: float,
+                      rec: np.ndarray) -> np.ndarray:
+        rec = np.ascontiguousarray(rec, dtype=np.float64).reshape(-1, 27)
+        out = np.zeros(len(rec))
+        f = getattr(self.lib, self.prefix + "pair_energies")
+        rc = f(kind, radius, length, soft, eps, sigma, len(rec), rec.ctypes.data_as(_dp), out.ctypes.data_as(_dp))
+        if rc != 0:
+            raise RuntimeError(f"{self.prefix}pair_energies does not support soft={soft} on kind={kind}")
         return out
 
     def snapshot_stats(self, kind, radius, length, box, pos, rot, early_exit=False, soft=0, eps=1.0, sigma=1.0,
@@ -103,6 +115,8 @@ def _setup_ref(lib):
     lib.ref_snapshot_stats.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp, _dp,
                                        ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, ctypes.c_size_t, _dp, _dp,
                                        ctypes.c_int, _u64p, _dp]
+    lib.ref_pair_energies.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double,
+                                      ctypes.c_double, ctypes.c_size_t, _dp, _dp]
     lib.ref_nematic_p2.argtypes = [ctypes.c_size_t, _dp]
     lib.ref_nematic_p2.restype = ctypes.c_double
     lib.ref_packing_fraction.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, ctypes.c_size_t]
