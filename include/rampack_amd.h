@@ -35,6 +35,16 @@ extern "C" {
 #define RP_SHAPE_SPHERE 0
 #define RP_SHAPE_SPHEROCYLINDER 1
 
+/* Soft interactions, which replace the hard core:
+ * RP_SOFT_WCA -- a sphere with RepulsiveLennardJonesInteraction (src/core/interactions/
+ *   RepulsiveLennardJonesInteraction.cpp:8-16 through CentralInteraction.h:58-67): WCA of the centre distance,
+ *   range sigma 2^(1/6);
+ * RP_SOFT_WCA_SEGMENT -- a spherocylinder whose energy is the WCA of the squared distance between the two axis
+ *   segments (no counterpart in the reference), range L + sigma 2^(1/6). */
+#define RP_SOFT_NONE 0
+#define RP_SOFT_WCA 1
+#define RP_SOFT_WCA_SEGMENT 2
+
 /* Precision of the overlap tests.  RP_PREC_FP64 evaluates every test in the reference's FP64 operation order. */
 #define RP_PREC_FP64 0
 
@@ -54,6 +64,9 @@ typedef struct rp_shape_desc {
     int kind;       /* RP_SHAPE_* */
     double radius;  /* sphere / spherocylinder radius */
     double length;  /* spherocylinder distance between cap centres */
+    int soft;       /* RP_SOFT_*; RP_SOFT_NONE: hard core */
+    double epsilon; /* WCA energy scale (> 0) */
+    double sigma;   /* WCA length scale (> 0) */
 } rp_shape_desc;
 
 /* RototranslationSampler descriptor (src/core/move_samplers/RototranslationSampler.h:17-60). */
@@ -129,6 +142,12 @@ int rp_run_cycles(rp_ctx *ctx, size_t n_cycles, int adapt_steps, rp_counters *ou
  * (Packing.cpp:858-899); early_exit returns 0 or 1. */
 int rp_count_total_overlaps(rp_ctx *ctx, int early_exit, uint64_t *out);
 
+/* Packing::getTotalEnergy (src/core/Packing.cpp:690-709): the soft energy summed over every pair once on the
+ * reference's neighbour grid; 0 for hard shapes.  The device sums per cell and then in a fixed order, so a state
+ * always gives the same bits; the reference's OpenMP reduction order differs, so values agree to the rounding of the
+ * sum. */
+int rp_total_energy(rp_ctx *ctx, double *out);
+
 /* Current state, in the caller's particle order (Packing::operator[], Packing::getBox). */
 int rp_download(rp_ctx *ctx, double box[9], double *pos, double *rot);
 
@@ -148,6 +167,10 @@ void rp_destroy(rp_ctx *ctx);
  * Packing does (Packing.cpp:21-44, 381-398).  Each record is 27 doubles: pos1[3] O1[9] pos2[3] O2[9] T[3].
  * decisions[i] = 1 for an overlap. */
 int rp_pair_decisions(const rp_shape_desc *shape, size_t n, const double *records, uint8_t *decisions);
+
+/* Batched soft pair energies on the device: Interaction::calculateEnergyBetween with a fixed cell translation
+ * (CentralInteraction.h:58-67 for RP_SOFT_WCA); records as in rp_pair_decisions. */
+int rp_pair_energies(const rp_shape_desc *shape, size_t n, const double *records, double *energies);
 
 const char *rp_last_error(void);
 

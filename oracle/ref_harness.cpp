@@ -36,6 +36,7 @@
 #include "core/volume_scalers/TriclinicAdapter.h"
 #include "core/volume_scalers/AnisotropicVolumeScaler.h"
 #include "core/volume_scalers/LogScalingFactorSampler.h"
+#include "core/volume_scalers/DeltaVolumeScaler.h"
 #include "core/observables/NematicOrder.h"
 
 namespace {
@@ -170,18 +171,20 @@ double ref_nematic_p2(std::size_t n, const double *rot) {
 }
 
 // One NpT (or NVT) run of the reference's hot path with the restated cycle loop.  box_kind: 0 none (NVT), 1
-// delta_triclinic(scale_together=True), 2 log(spec="isotropic").  rot_step <= 0 selects the automatic rotation step
-// (RototranslationSampler.cpp:95-114).  Samples of (packing fraction, P2) are taken every avg_every cycles of the
-// averaging phase.  The return value is the number of samples, or a negative number on error.
-long ref_mc_run(int kind, double radius, double length, const double *box, std::size_t n, const double *pos,
-                const double *rot, double temperature, double pressure, double transStep, double rotStep,
-                int boxKind, double boxStep, unsigned long seed, int ddx, int ddy, int ddz, int threads,
-                std::size_t thermCycles, std::size_t avgCycles, std::size_t avgEvery, double *samples,
-                double *outBox, double *outPos, double *outRot, double *seconds, double *finalSteps) {
+// delta_triclinic(scale_together=True), 2 log(spec="isotropic"), 3 delta_v.  soft 1: WCA sphere (eps, sigma).
+// rot_step <= 0 selects the automatic rotation step (RototranslationSampler.cpp:95-114).  Samples of (packing
+// fraction, P2 (spherocylinders, else 0), total energy / N) are taken every avg_every cycles of the averaging
+// phase.  The return value is the number of samples, or a negative number on error.
+long ref_mc_run(int kind, double radius, double length, int soft, double eps, double sigma, const double *box,
+                std::size_t n, const double *pos, const double *rot, double temperature, double pressure,
+                double transStep, double rotStep, int boxKind, double boxStep, unsigned long seed, int ddx, int ddy,
+                int ddz, int threads, std::size_t thermCycles, std::size_t avgCycles, std::size_t avgEvery,
+                double *samples, double *outBox, double *outPos, double *outRot, double *seconds,
+                double *finalSteps) {
     try {
         std::array<std::size_t, 3> dd{(std::size_t)ddx, (std::size_t)ddy, (std::size_t)ddz};
         std::size_t numDomains = dd[0] * dd[1] * dd[2];
-        auto traits = makeTraits(kind, radius, length, 0, nullptr, nullptr, 0, 1, 1);
+        auto traits = makeTraits(kind, radius, length, 0, nullptr, nullptr, soft, eps, sigma);
         const Interaction &inter = traits->getInteraction();
         auto packing = std::make_unique<Packing>(TriclinicBox(mat(box)), makeShapes(n, pos, rot),
                                                  std::make_unique<PeriodicBoundaryConditions>(), inter,
@@ -196,6 +199,8 @@ long ref_mc_run(int kind, double radius, double length, const double *box, std::
             scaler = std::make_unique<TriclinicAdapter>(
                 std::make_unique<AnisotropicVolumeScaler>(std::make_unique<LogScalingFactorSampler>(), dir, false),
                 boxStep);
+        } else if (boxKind == 3) {
+            scaler = std::make_unique<TriclinicAdapter>(std::make_unique<DeltaVolumeScaler>(), boxStep);
         }
 
         std::vector<std::mt19937> mts;
@@ -316,7 +321,13 @@ long ref_mc_run(int kind, double radius, double length, const double *box, std::
             totalCycles++;
             if (totalCycles % 10000 == 0) packing->renormalizeOrientations(inter, false);
             if (!therm && avgEvery && (totalCycles - thermCycles) % avgEvery == 0 && samples) {
-                samples[2 * nSamples] = packing->getPackingFraction(volume0);
+                samples[3 * nSamples] = packing->getPackingFraction(volume0);
+                samples[3 * nSamples + 2] = packing->getTotalEnergy(inter) / (double)n;
+                samples[3 * nSamples + 1] = 0;
+                if (kind != 1) {
+                    nSamples++;
+                    continue;
+                }
                 std::vector<double> r(9 * n);
                 for (std::size_t i = 0; i < n; i++) {
                     const auto &O = (*packing)[i].getOrientation();
@@ -325,7 +336,7 @@ long ref_mc_run(int kind, double radius, double length, const double *box, std::
                 }
                 NematicOrder q;
                 q.calculate(*packing, temperature, pressure, *traits);
-                samples[2 * nSamples + 1] = q.getIntervalValues().front();
+                samples[3 * nSamples + 1] = q.getIntervalValues().front();
                 nSamples++;
             }
         }

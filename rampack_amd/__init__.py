@@ -21,6 +21,9 @@ LIB_PATH = os.path.join(_HERE, "lib", "librampack_amd.so")
 
 SHAPE_SPHERE = 0
 SHAPE_SPHEROCYLINDER = 1
+SOFT_NONE = 0
+SOFT_WCA = 1
+SOFT_WCA_SEGMENT = 2
 PREC_FP64 = 0
 BOX_NONE = 0
 BOX_DELTA_TRICLINIC = 1
@@ -37,11 +40,13 @@ RP_ERR_UNSUPPORTED = -4
 EXPORTED_SYMBOLS = (
     "rp_create", "rp_set_environment", "rp_run_cycles", "rp_count_total_overlaps", "rp_download",
     "rp_set_timing", "rp_get_timing", "rp_destroy", "rp_pair_decisions", "rp_last_error", "rp_get_observables",
+    "rp_total_energy", "rp_pair_energies",
 )
 
 
 class ShapeDesc(ctypes.Structure):
-    _fields_ = [("kind", ctypes.c_int), ("radius", ctypes.c_double), ("length", ctypes.c_double)]
+    _fields_ = [("kind", ctypes.c_int), ("radius", ctypes.c_double), ("length", ctypes.c_double),
+                ("soft", ctypes.c_int), ("epsilon", ctypes.c_double), ("sigma", ctypes.c_double)]
 
 
 class MoveDesc(ctypes.Structure):
@@ -128,6 +133,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.rp_pair_decisions.argtypes = [ctypes.POINTER(ShapeDesc), ctypes.c_size_t, dp,
                                       ctypes.POINTER(ctypes.c_uint8)]
     lib.rp_get_observables.argtypes = [vp, ctypes.POINTER(Observables)]
+    lib.rp_total_energy.argtypes = [vp, dp]
+    lib.rp_pair_energies.argtypes = [ctypes.POINTER(ShapeDesc), ctypes.c_size_t, dp, dp]
     lib.rp_last_error.restype = ctypes.c_char_p
     _lib = lib
     return lib
@@ -150,16 +157,25 @@ def _dptr(a: np.ndarray):
 
 @dataclass
 class Shape:
-    """ShapeTraits descriptor: 'sphere' (radius) or 'spherocylinder' (radius, length between cap centres)."""
+    """ShapeTraits descriptor: 'sphere' (radius) or 'spherocylinder' (radius, length between cap centres).
+    soft: None (hard core), 'wca' (a sphere with RepulsiveLennardJonesInteraction(epsilon, sigma)) or 'wca_segment'
+    (a spherocylinder with WCA on the distance between the axis segments)."""
     kind: str
     radius: float = 0.5
     length: float = 0.0
+    soft: Optional[str] = None
+    epsilon: float = 1.0
+    sigma: float = 1.0
 
     def desc(self) -> ShapeDesc:
         kinds = {"sphere": SHAPE_SPHERE, "spherocylinder": SHAPE_SPHEROCYLINDER}
+        softs = {None: SOFT_NONE, "wca": SOFT_WCA, "wca_segment": SOFT_WCA_SEGMENT}
         if self.kind not in kinds:
             raise ValueError(f"unknown shape {self.kind}")
-        return ShapeDesc(kinds[self.kind], float(self.radius), float(self.length))
+        if self.soft not in softs:
+            raise ValueError(f"unknown soft interaction {self.soft}")
+        return ShapeDesc(kinds[self.kind], float(self.radius), float(self.length), softs[self.soft],
+                         float(self.epsilon), float(self.sigma))
 
     @property
     def volume(self) -> float:
@@ -171,6 +187,11 @@ class Shape:
 
     @property
     def range(self) -> float:
+        cut = self.sigma * 2 ** (1 / 6)
+        if self.soft == "wca":
+            return cut
+        if self.soft == "wca_segment":
+            return self.length + cut
         return 2 * self.radius + (self.length if self.kind == "spherocylinder" else 0.0)
 
 
@@ -182,6 +203,16 @@ def pair_decisions(shape: Shape, records: np.ndarray) -> np.ndarray:
     d = shape.desc()
     _check(lib.rp_pair_decisions(ctypes.byref(d), len(rec), _dptr(rec),
                                  out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))))
+    return out
+
+
+def pair_energies(shape: Shape, records: np.ndarray) -> np.ndarray:
+    """Device soft pair energies on records of 27 doubles: pos1, O1 (row-major), pos2, O2, T."""
+    lib = load_library()
+    rec = np.ascontiguousarray(records, dtype=np.float64).reshape(-1, 27)
+    out = np.zeros(len(rec))
+    d = shape.desc()
+    _check(lib.rp_pair_energies(ctypes.byref(d), len(rec), _dptr(rec), _dptr(out)))
     return out
 
 
@@ -250,6 +281,12 @@ class GpuPacking:
         t = Timing()
         _check(_lib.rp_get_timing(self._ctx, ctypes.byref(t)))
         return t.as_dict()
+
+    def total_energy(self) -> float:
+        """Packing::getTotalEnergy of the device-resident state (0 for hard shapes)."""
+        v = ctypes.c_double()
+        _check(_lib.rp_total_energy(self._ctx, ctypes.byref(v)))
+        return float(v.value)
 
     def observables(self) -> dict:
         """PackingFraction and NematicOrder of the device-resident state (rp_get_observables)."""

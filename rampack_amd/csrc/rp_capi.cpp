@@ -85,6 +85,9 @@ struct rp_ctx {
     size_t propCap = 0;
     void *propMem = nullptr;
     double *obsMem = nullptr;            // Q-tensor partial sums
+    double *cellEnergy = nullptr;        // per-cell soft energy partials
+    int cellEnergyCap = 0;
+    double *sumScratch = nullptr;        // fixed-order sum of the partials
     unsigned long long *dCounters = nullptr;
     unsigned long long *dPhases = nullptr;  // sweep phase timers (RAMPACK_SWEEP_PHASES=1)
     unsigned long long *hCounters = nullptr;  // pinned
@@ -145,9 +148,13 @@ int allocParticles(DeviceParticles &d, size_t n) {
     return RP_OK;
 }
 
+// sigma * pow(2, 1/6), as RepulsiveLennardJonesInteraction's constructor computes it
+double wcaCut(const rp_shape_desc &s) { return s.sigma * std::pow(2., 1. / 6); }
+
 PairConsts makeConsts(const rp_shape_desc &s) {
     PairConsts k{};
     k.kind = s.kind;
+    k.soft = s.soft;
     if (s.kind == RP_SHAPE_SPHERE) {
         k.contact2 = std::pow(2 * s.radius, 2);              // SphereTraits.cpp:64
     } else {
@@ -155,11 +162,24 @@ PairConsts makeConsts(const rp_shape_desc &s) {
         k.range2 = std::pow(2 * s.radius + s.length, 2);     // SpherocylinderTraits.cpp:37
         k.halfLen = 0.5 * s.length;                          // SpherocylinderTraits.cpp:24-26 (0.5 * 1 * L)
     }
+    if (s.soft) {
+        const double cut = wcaCut(s);
+        k.eps = s.epsilon;
+        k.sigma2 = s.sigma * s.sigma;                        // RepulsiveLennardJonesInteraction.cpp:12
+        k.cut2 = cut * cut;                                  // RepulsiveLennardJonesInteraction.cpp:9
+        // segment distance >= centre distance - L: beyond (L + cut) the energy is 0; the margin keeps every pair
+        // whose rounded segment distance could still fall below the cut-off
+        k.softRange2 = s.soft == 2 ? std::pow(s.length + cut, 2) * (1 + 1e-9) : k.cut2;
+    }
     return k;
 }
 
+// Interaction range: SphereTraits.h:34, SpherocylinderTraits.h:81; a WCA sphere the range of its interaction
+// (RepulsiveLennardJonesInteraction.h:26); a WCA spherocylinder L + sigma 2^(1/6)
 double shapeRange(const rp_shape_desc &s) {
-    return s.kind == RP_SHAPE_SPHERE ? 2 * s.radius : 2 * s.radius + s.length;  // SphereTraits.h:34, SpherocylinderTraits.h:81
+    if (s.soft == RP_SOFT_WCA) return wcaCut(s);
+    if (s.soft == RP_SOFT_WCA_SEGMENT) return s.length + wcaCut(s);
+    return s.kind == RP_SHAPE_SPHERE ? 2 * s.radius : 2 * s.radius + s.length;
 }
 
 void fillTranslations(Grid &g) {
@@ -192,11 +212,12 @@ int ensureProposals(rp_ctx *c, size_t need) {
     c->propMem = nullptr;
     c->propCap = 0;
     size_t cap = need + need / 4 + 256;
-    RP_HIP(hipMalloc(&c->propMem, cap * (sizeof(int) + 12 * sizeof(double))));
+    RP_HIP(hipMalloc(&c->propMem, cap * (sizeof(int) + 13 * sizeof(double))));
     char *q = static_cast<char *>(c->propMem);
     c->prop.t = reinterpret_cast<double *>(q);
     c->prop.R = c->prop.t + 3 * cap;
-    c->prop.k = reinterpret_cast<int *>(c->prop.R + 9 * cap);
+    c->prop.u = c->k.soft ? c->prop.R + 9 * cap : nullptr;
+    c->prop.k = reinterpret_cast<int *>(c->prop.R + 10 * cap);
     c->prop.stride = (int)cap;
     c->propCap = cap;
     return RP_OK;
@@ -357,6 +378,61 @@ double minHeight(const HostBox &b) {
 // Simulation::tryScaling for a hard interaction (Simulation.cpp:493-531) + Packing::tryScaling
 // (Packing.cpp:172-208).  On acceptance the rescaled arrays, already sorted by the next cycle's grid, become
 // current.
+// Packing::getTotalEnergy (Packing.cpp:690-709) of particles sorted by grid g.
+int totalEnergy(rp_ctx *c, const Grid &g, const Particles &p, double &out);
+
+// Simulation::tryScaling with a soft interaction (Simulation.cpp:508-517): the scaling is always tried, and
+// accepted if u <= exp(N ln(V'/V) - dE/T - p dV/T) with dE = getTotalEnergy(new) - getTotalEnergy(old)
+// (Packing.cpp:172-208).  The old energy is summed on the cycle's grid, which the colour passes keep valid.
+int softBoxMove(rp_ctx *c, const HostBox &nb, double exponent, double u, const double shiftUnit[3], bool &accepted) {
+    double eOld = 0, eNew = 0;
+    int r = totalEnergy(c, c->grid, c->cur.p, eOld);
+    if (r) return r;
+    Grid g{};
+    r = chooseSweepGrid(c, nb, shiftUnit, g);
+    if (r == RP_ERR_UNSUPPORTED) {
+        c->boxTooSmall++;
+        return RP_OK;
+    }
+    if (r) return r;
+    r = rebuild(c, g, 1, c->box.inv);
+    if (r) return r;
+    c->boxScans++;
+    r = totalEnergy(c, g, c->alt.p, eNew);
+    if (r) return r;
+    if (!(u <= std::exp(exponent - (eNew - eOld) / c->temperature))) return RP_OK;
+    c->box = nb;
+    swapBuffers(c);
+    c->grid = g;
+    c->sortedForCycle = true;
+    accepted = true;
+    return RP_OK;
+}
+
+int totalEnergy(rp_ctx *c, const Grid &g, const Particles &p, double &out) {
+    if (g.ncell > c->cellEnergyCap) {
+        if (c->cellEnergy) hipFree(c->cellEnergy);
+        c->cellEnergy = nullptr;
+        c->cellEnergyCap = 0;
+        RP_HIP(hipMalloc(&c->cellEnergy, sizeof(double) * (size_t)g.ncell));
+        c->cellEnergyCap = g.ncell;
+    }
+    if (!c->sumScratch) RP_HIP(hipMalloc(&c->sumScratch, sizeof(double) * fixed_sum_scratch_doubles()));
+    double lambda = (double)c->n / (double)g.ncell;
+    const int side = 2 * g.reach + 1;
+    double expect = ((side * side * side - 1) / 2 + 1) * lambda;
+    int cap = (int)std::ceil(expect + 8 * std::sqrt(expect) + 64);
+    cap = std::min((cap + 63) / 64 * 64, capForBytes(c, count_lds_bytes(c->k.kind, 64) / 64.0));
+    RP_HIP(launch_total_energy(c->stream, g, (int)c->n, p, c->k, c->cellEnergy, c->sumScratch, c->dCounters, cap));
+    RP_HIP(hipMemcpyAsync(&out, c->sumScratch + fixed_sum_scratch_doubles() - 1, sizeof(double),
+                          hipMemcpyDeviceToHost, c->stream));
+    int r = readCounters(c);  // synchronises the stream
+    if (r) return r;
+    if (c->hCounters[CNT_UNSTAGED_BLOCKS] >> 32)
+        return fail(RP_ERR_UNSUPPORTED, "a grid cell holds more particles than the energy sum can stage");
+    return RP_OK;
+}
+
 int boxMove(rp_ctx *c, bool &accepted) {
     accepted = false;
     HostBox nb;
@@ -387,6 +463,7 @@ int boxMove(rp_ctx *c, bool &accepted) {
     double exponent = N * std::log(factor) - c->pressure * dV / c->temperature;
     double u = philox_unit(c->hostRng);
     double shiftUnit[3] = {philox_unit(c->hostRng), philox_unit(c->hostRng), philox_unit(c->hostRng)};
+    if (c->k.soft) return softBoxMove(c, nb, exponent, u, shiftUnit, accepted);
     if (u > std::exp(exponent)) return RP_OK;
 
     Grid g{};
@@ -471,6 +548,7 @@ int runCycle(rp_ctx *c) {
     sp.counters = c->dCounters;
     sp.phases = c->dPhases;
     sp.pipelined = c->pipelined;
+    sp.temperature = c->temperature;
     const int colours = g.period * g.period * g.period;
     int order[27];
     for (int i = 0; i < colours; i++) order[i] = i;
@@ -571,6 +649,14 @@ int rp_create(rp_ctx **out, const rp_shape_desc *shape, const double box[9], siz
     if (shape->kind != RP_SHAPE_SPHERE && shape->kind != RP_SHAPE_SPHEROCYLINDER)
         return fail(RP_ERR_UNSUPPORTED, "unsupported shape kind");
     if (!(shape->radius > 0)) return fail(RP_ERR_PRECONDITION, "radius must be > 0");
+    if (shape->soft != RP_SOFT_NONE) {
+        if (!((shape->soft == RP_SOFT_WCA && shape->kind == RP_SHAPE_SPHERE) ||
+              (shape->soft == RP_SOFT_WCA_SEGMENT && shape->kind == RP_SHAPE_SPHEROCYLINDER)))
+            return fail(RP_ERR_UNSUPPORTED, "soft interaction not available for this shape");
+        // RepulsiveLennardJonesInteraction.cpp:21-22
+        if (!(shape->epsilon > 0)) return fail(RP_ERR_PRECONDITION, "epsilon must be > 0");
+        if (!(shape->sigma > 0)) return fail(RP_ERR_PRECONDITION, "sigma must be > 0");
+    }
     if (shape->kind == RP_SHAPE_SPHEROCYLINDER && !(shape->length >= 0))
         return fail(RP_ERR_PRECONDITION, "length must be >= 0");
     HostBox hb;
@@ -592,11 +678,12 @@ int rp_create(rp_ctx **out, const rp_shape_desc *shape, const double box[9], siz
     if (const char *p = std::getenv("RAMPACK_COLOURING")) c->colourPeriod = std::atoi(p);
     if (const char *p = std::getenv("RAMPACK_SWEEP_PIPELINE")) c->pipelined = std::atoi(p) != 0;
     if (const char *ph = std::getenv("RAMPACK_SWEEP_PHASES"); ph && std::atoi(ph) != 0) {
-        if (hipMalloc(&c->dPhases, sizeof(unsigned long long) * RP_PHASE_NUM) != hipSuccess)
-            return fail(RP_ERR_DEVICE, "phase timer allocation failed");
         unsigned long long init[RP_PHASE_NUM] = {};
-        if (hipMemcpy(c->dPhases, init, sizeof init, hipMemcpyHostToDevice) != hipSuccess)
-            return fail(RP_ERR_DEVICE, "phase timer init failed");
+        if (hipMalloc(&c->dPhases, sizeof(unsigned long long) * RP_PHASE_NUM) != hipSuccess ||
+            hipMemcpy(c->dPhases, init, sizeof init, hipMemcpyHostToDevice) != hipSuccess) {
+            rp_destroy(c);
+            return fail(RP_ERR_DEVICE, "phase timer set-up failed");
+        }
     }
     c->shape = *shape;
     c->k = makeConsts(*shape);
@@ -678,7 +765,9 @@ int rp_run_cycles(rp_ctx *c, size_t nCycles, int adapt, rp_counters *out) {
     if (!c) return fail(RP_ERR_PRECONDITION, "null context");
     if (!c->envSet) return fail(RP_ERR_PRECONDITION, "environment not set");
     uint64_t overl = 0;
-    int r = countExact(c, 1, overl);
+    int r = RP_OK;
+    // a WCA interaction has no hard part: there is nothing to validate (Simulation.cpp:127-132)
+    if (!c->k.soft) r = countExact(c, 1, overl);
     if (r == RP_ERR_UNSUPPORTED) {
         // the reference's grid is off for this box; check on the checkerboard grid instead
         double zero[3] = {0, 0, 0};
@@ -875,6 +964,8 @@ void rp_destroy(rp_ctx *c) {
     if (c->cellCount) hipFree(c->cellCount);
     if (c->propMem) hipFree(c->propMem);
     if (c->obsMem) hipFree(c->obsMem);
+    if (c->cellEnergy) hipFree(c->cellEnergy);
+    if (c->sumScratch) hipFree(c->sumScratch);
     if (c->tmpSlot) hipFree(c->tmpSlot);
     if (c->tmpOrder) hipFree(c->tmpOrder);
     if (c->tmpPos) hipFree(c->tmpPos);
@@ -884,10 +975,56 @@ void rp_destroy(rp_ctx *c) {
     delete c;
 }
 
+int rp_total_energy(rp_ctx *c, double *out) {
+    if (!c || !out) return fail(RP_ERR_PRECONDITION, "null argument");
+    *out = 0;
+    if (!c->k.soft) return RP_OK;  // Packing::getTotalEnergy returns 0 without a soft part (Packing.cpp:691-692)
+    c->sortedForCycle = false;
+    Grid g{};
+    int disabled = 0;
+    int r = makeExactGrid(c, c->box, g, disabled);
+    if (r) return r;
+    if (disabled) {
+        // the reference would sum all pairs; every pair within range is also a half-shell pair of the
+        // checkerboard grid, which exists whenever the box is at least two ranges high
+        double zero[3] = {0, 0, 0};
+        r = chooseSweepGrid(c, c->box, zero, g);
+        if (r) return r;
+    }
+    r = rebuild(c, g, 0, nullptr);
+    if (r) return r;
+    return totalEnergy(c, g, c->alt.p, *out);
+}
+
+int rp_pair_energies(const rp_shape_desc *shape, size_t n, const double *records, double *energies) {
+    if (!shape || (n && (!records || !energies))) return fail(RP_ERR_PRECONDITION, "null argument");
+    if (!((shape->soft == RP_SOFT_WCA && shape->kind == RP_SHAPE_SPHERE) ||
+          (shape->soft == RP_SOFT_WCA_SEGMENT && shape->kind == RP_SHAPE_SPHEROCYLINDER)))
+        return fail(RP_ERR_UNSUPPORTED, "no soft interaction for this shape descriptor");
+    if (!(shape->epsilon > 0) || !(shape->sigma > 0)) return fail(RP_ERR_PRECONDITION, "epsilon, sigma must be > 0");
+    if (n == 0) return RP_OK;
+    PairConsts k = makeConsts(*shape);
+    double *dRec = nullptr, *dOut = nullptr;
+    RP_HIP(hipMalloc(&dRec, sizeof(double) * 27 * n));
+    if (hipMalloc(&dOut, sizeof(double) * n) != hipSuccess) {
+        hipFree(dRec);
+        return fail(RP_ERR_DEVICE, "hipMalloc failed");
+    }
+    int r = RP_OK;
+    hipError_t e = hipMemcpy(dRec, records, sizeof(double) * 27 * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_pair_energies(nullptr, k, n, dRec, dOut);
+    if (e == hipSuccess) e = hipMemcpy(energies, dOut, sizeof(double) * n, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) r = fail(RP_ERR_DEVICE, std::string("pair energies: ") + hipGetErrorString(e));
+    hipFree(dRec);
+    hipFree(dOut);
+    return r;
+}
+
 int rp_pair_decisions(const rp_shape_desc *shape, size_t n, const double *records, uint8_t *decisions) {
     if (!shape || (n && (!records || !decisions))) return fail(RP_ERR_PRECONDITION, "null argument");
     if (shape->kind != RP_SHAPE_SPHERE && shape->kind != RP_SHAPE_SPHEROCYLINDER)
         return fail(RP_ERR_UNSUPPORTED, "unsupported shape kind");
+    if (shape->soft != RP_SOFT_NONE) return fail(RP_ERR_UNSUPPORTED, "a soft interaction has no hard part");
     if (n == 0) return RP_OK;
     PairConsts k = makeConsts(*shape);
     double *dRec = nullptr;

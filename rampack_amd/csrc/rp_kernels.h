@@ -41,6 +41,7 @@ struct Proposals {
     int *k;         // particle index within the cell
     double *t;      // translation, 3 x stride
     double *R;      // rotation matrix, row-major, 9 x stride
+    double *u;      // Metropolis uniform in [0, 1) for soft interactions; nullptr for hard ones
     int stride;
 };
 
@@ -54,6 +55,7 @@ struct SweepParams {
     int na[3];              // active cells per axis (n / period)
     int cap;                // stencil entries that fit in LDS
     int pipelined;          // 1: wavefront 0 plans the next move while the others test (needs >= 2 wavefronts)
+    double temperature;     // soft interactions: the move is accepted if u <= exp(-dE / temperature)
     unsigned long long *counters;
     unsigned long long *phases;  // optional per-phase wall-clock sums (RP_PHASE_*), nullptr when not profiling
 };
@@ -98,6 +100,16 @@ size_t count_lds_bytes(int kind, int cap);
 hipError_t launch_proposals(hipStream_t s, const Grid &g, const Proposals &pr, int moves, const int *cellOf, int n,
                             double ts, double rs, uint64_t seed, uint32_t cycle);
 // waves: wavefronts per active cell (1, 2 or 4)
+// Soft pair energies on records pos1[3] O1[9] pos2[3] O2[9] T[3].
+hipError_t launch_pair_energies(hipStream_t s, PairConsts k, size_t n, const double *rec, double *out);
+
+// Total soft energy over the half-shell pairs of grid g (Packing::getTotalEnergy, Packing.cpp:690-709):
+// cellEnergy (g.ncell doubles) receives one partial per cell, scratch (fixed_sum_scratch_doubles()) the fixed-order
+// sum, whose last element is the total.  counters[CNT_UNSTAGED_BLOCKS] >> 32 counts cells too full to stage.
+hipError_t launch_total_energy(hipStream_t s, const Grid &g, int n, Particles p, PairConsts k, double *cellEnergy,
+                               double *scratch, unsigned long long *counters, int cap);
+size_t fixed_sum_scratch_doubles();
+
 // Sums of a_i a_j over the primary axes (orientation column 2), Q_11 Q_12 Q_13 Q_22 Q_23 Q_33 order, written to
 // scratch[6 * 256 .. 6 * 256 + 6); scratch holds nematic_scratch_doubles() doubles.
 size_t nematic_scratch_doubles();

@@ -119,6 +119,29 @@ hipError_t launch_pair_decisions(hipStream_t s, PairConsts k, size_t n, const do
     return hipGetLastError();
 }
 
+__global__ __launch_bounds__(256) void k_pair_energies(PairConsts k, size_t n, const double *__restrict__ rec,
+                                                       double *__restrict__ out) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double *q = rec + 27 * i;
+    V3 p1 = v3(q[0], q[1], q[2]);
+    V3 a1 = v3(q[5], q[8], q[11]);
+    V3 p2 = v3(q[12], q[13], q[14]);
+    V3 a2 = v3(q[17], q[20], q[23]);
+    V3 T = v3(q[24], q[25], q[26]);
+    SCFirst f;
+    if (k.kind == 0) f.pos = p1;
+    else f = sc_first(k, p1, a1);
+    out[i] = soft_energy(k, f, p2, a2, T);
+}
+
+hipError_t launch_pair_energies(hipStream_t s, PairConsts k, size_t n, const double *rec, double *out) {
+    if (n == 0) return hipSuccess;
+    unsigned blocks = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(k_pair_energies, dim3(blocks), dim3(256), 0, s, k, n, rec, out);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------------------------------------------
 // Fill kernels (used instead of hipMemsetAsync so the cycle stays capturable in a graph)
 
@@ -491,6 +514,130 @@ hipError_t launch_count_overlaps(hipStream_t s, const Grid &g, int n, Particles 
 }
 
 // ---------------------------------------------------------------------------------------------------------------
+// Total soft energy (Packing::getTotalEnergy / getTotalEnergyNGCellHelper, Packing.cpp:690-709, 766-839): the same
+// half-shell traversal as the overlap count, every pair once; one partial per cell, then a fixed-order sum.
+
+constexpr int SUM_THREADS = 256;
+constexpr int SUM_BLOCKS = 256;
+
+// Sums SUM_THREADS values held one per thread in a fixed tree; the result is in lds[0].
+__device__ __forceinline__ double block_sum(double v, double *lds) {
+    const int tid = threadIdx.x;
+    lds[tid] = v;
+    __syncthreads();
+    for (int w = SUM_THREADS / 2; w > 0; w >>= 1) {
+        if (tid < w) lds[tid] += lds[tid + w];
+        __syncthreads();
+    }
+    return lds[0];
+}
+
+template <int KIND, bool STAGED>
+__device__ double energy_cell(const Particles &p, const PairConsts &k, const Stencil &S, const double *sT,
+                              const int *sTag, int nOwn, int total) {
+    const int tid = threadIdx.x;
+    double e = 0;
+    for (int a = 0; a < nOwn; a++) {
+        V3 pos1, ax1;
+        int c1;
+        load_candidate<KIND, STAGED>(p, S, a, pos1, ax1, c1);
+        const int wx = (c1 & 3) - 1, wy = ((c1 >> 2) & 3) - 1, wz = ((c1 >> 4) & 3) - 1;
+        const int tagA = sTag[a];
+        SCFirst f = make_first<KIND>(k, pos1, ax1);
+        for (int b = tid; b < total; b += COUNT_THREADS) {
+            if (b < nOwn && sTag[b] >= tagA) continue;
+            V3 p2, a2;
+            int code;
+            load_candidate<KIND, STAGED>(p, S, b, p2, a2, code);
+            V3 T = b < nOwn && code == c1 ? v3(0, 0, 0) : image_translation(sT, code, wx, wy, wz);
+            e += soft_energy(k, f, p2, a2, T);
+        }
+    }
+    return e;
+}
+
+template <int KIND>
+__global__ __launch_bounds__(COUNT_THREADS) void k_total_energy(Grid g, Particles p, PairConsts k, double *cellEnergy,
+                                                                unsigned long long *counters, int cap) {
+    extern __shared__ double smem[];
+    __shared__ int s_start[MAX_HALF + 1], s_base[MAX_HALF], s_ecode[MAX_HALF];
+    __shared__ double sT[81];
+    __shared__ double sSum[COUNT_THREADS];
+    const int tid = threadIdx.x;
+    const int cell = blockIdx.x;
+    const int own0 = g.cellStart[cell];
+    const int nOwn = g.cellStart[cell + 1] - own0;
+    if (nOwn == 0) {
+        if (tid == 0) cellEnergy[cell] = 0;
+        return;
+    }
+    const int cx = cell % g.n[0], cy = (cell / g.n[0]) % g.n[1], cz = cell / (g.n[0] * g.n[1]);
+    const int ncells = stencil_cells(g.reach, true);
+    for (int i = tid; i < 81; i += COUNT_THREADS) sT[i] = g.T[i];
+    build_stencil(g, cx, cy, cz, ncells, true, s_start, s_base, s_ecode);
+    const int total = s_start[ncells];
+    Stencil S;
+    S.start = s_start;
+    S.base = s_base;
+    S.ecode = s_ecode;
+    S.ncells = ncells;
+    double *rest;
+    carve_stencil(S, smem, cap, KIND, rest);
+    int *sTag = reinterpret_cast<int *>(rest);
+    for (int a = tid; a < nOwn && a < cap; a += COUNT_THREADS) sTag[a] = p.tag[own0 + a];
+    double e = 0;
+    if (total <= cap) {
+        stage_stencil<KIND>(p, S, total, tid, COUNT_THREADS);
+        __syncthreads();
+        e = energy_cell<KIND, true>(p, k, S, sT, sTag, nOwn, total);
+    } else if (nOwn <= cap) {
+        __syncthreads();
+        e = energy_cell<KIND, false>(p, k, S, sT, sTag, nOwn, total);
+    } else if (tid == 0) {
+        atomicAdd(&counters[CNT_UNSTAGED_BLOCKS], 1ull << 32);
+    }
+    static_assert(COUNT_THREADS == SUM_THREADS, "one partial per thread");
+    double sum = block_sum(e, sSum);
+    if (tid == 0) cellEnergy[cell] = sum;
+}
+
+// Fixed-order sum of n doubles: SUM_BLOCKS strided partials, then one workgroup; out = scratch[SUM_BLOCKS].
+__global__ __launch_bounds__(SUM_THREADS) void k_sum_partial(const double *in, int n, double *scratch) {
+    __shared__ double lds[SUM_THREADS];
+    double v = 0;
+    for (int i = blockIdx.x * SUM_THREADS + threadIdx.x; i < n; i += SUM_BLOCKS * SUM_THREADS) v += in[i];
+    double s = block_sum(v, lds);
+    if (threadIdx.x == 0) scratch[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(SUM_THREADS) void k_sum_final(double *scratch) {
+    __shared__ double lds[SUM_THREADS];
+    static_assert(SUM_BLOCKS == SUM_THREADS, "one partial per thread");
+    double s = block_sum(scratch[threadIdx.x], lds);
+    if (threadIdx.x == 0) scratch[SUM_BLOCKS] = s;
+}
+
+size_t fixed_sum_scratch_doubles() { return SUM_BLOCKS + 1; }
+
+hipError_t launch_total_energy(hipStream_t s, const Grid &g, int n, Particles p, PairConsts k, double *cellEnergy,
+                               double *scratch, unsigned long long *counters, int cap) {
+    size_t lds = count_lds_bytes(k.kind, cap);
+    if (n > 0 && g.ncell > 0) {
+        if (k.kind == 0)
+            hipLaunchKernelGGL(k_total_energy<0>, dim3((unsigned)g.ncell), dim3(COUNT_THREADS), lds, s, g, p, k,
+                               cellEnergy, counters, cap);
+        else
+            hipLaunchKernelGGL(k_total_energy<1>, dim3((unsigned)g.ncell), dim3(COUNT_THREADS), lds, s, g, p, k,
+                               cellEnergy, counters, cap);
+    }
+    // with no particles the partials are sums of nothing (zeros)
+    hipLaunchKernelGGL(k_sum_partial, dim3(SUM_BLOCKS), dim3(SUM_THREADS), 0, s, cellEnergy, n > 0 ? g.ncell : 0,
+                       scratch);
+    hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(SUM_THREADS), 0, s, scratch);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------------------------------------
 // Checkerboard sweep.  Workgroup = one active cell of colour `colour` (cells with coordinates of the colour's
 // parity on every axis; same-colour cells are separated by a full inactive cell, wider than the interaction
 // range).  The cell's particles are moved n_cell times: uniform random pick + rototranslation
@@ -548,6 +695,7 @@ __global__ __launch_bounds__(256) void k_proposals(Grid g, Proposals pr, int mov
     double R[9];
     rodrigues(axis, philox_uniform(rng, -rs, rs), R);
     for (int c = 0; c < 9; c++) pr.R[c * pr.stride + i] = R[c];
+    if (pr.u) pr.u[i] = philox_unit(rng);  // Simulation.cpp:483, drawn after the move as there
 }
 
 hipError_t launch_proposals(hipStream_t s, const Grid &g, const Proposals &pr, int moves, const int *cellOf, int n,
@@ -563,7 +711,7 @@ hipError_t launch_proposals(hipStream_t s, const Grid &g, const Proposals &pr, i
 // own orientations, one batch of proposals and the translation table.
 size_t sweep_lds_bytes(int kind, int cap, int waves) {
     return stencil_lds_bytes(kind, cap) + sizeof(int) * (cap + WAVE * waves) +
-           sizeof(double) * (9 * OWN_CAP + 12 * MOVE_CHUNK + 81 + 2 * 24) + sizeof(int) * (MOVE_CHUNK + 8);
+           sizeof(double) * (9 * OWN_CAP + 12 * MOVE_CHUNK + 81 + 80) + sizeof(int) * (MOVE_CHUNK + 8);
 }
 
 __device__ __forceinline__ void flush_counters(const SweepParams &P, unsigned long long accepted, int moves,
@@ -978,6 +1126,147 @@ __device__ void sweep_cell_unstaged(const SweepParams &P, const Stencil &S, doub
     flush_counters(P, accepted, moves, leftCell, tests, false);
 }
 
+// Soft sweep of one active cell (Packing::tryMove with a soft interaction, Packing.cpp:136-164, and the Metropolis
+// test of Simulation.cpp:483).  Every trial move sums the moved particle's energy with all stencil candidates at
+// its current pose (initialEnergy) and at the trial pose (finalEnergy) in one pass; the lanes' sums are reduced per
+// wavefront (xor butterfly: every lane ends with the same bits), the wavefronts' sums in index order, and the move is
+// accepted if it stays in the cell and u <= exp(-(finalEnergy - initialEnergy) / T).  STAGED: stencil and own
+// orientations in LDS; otherwise read from HBM and accepted poses written back there by lane 0.
+constexpr int SWEEP_AUX = 80;  // doubles of the planner / soft scratch area
+static_assert(SWEEP_AUX >= 2 * PLAN_D && SWEEP_AUX >= MOVE_CHUNK + 2 * 2 * 4, "sweep scratch area too small");
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int off = WAVE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+template <int KIND, int WAVES, bool TRANSLATED, bool STAGED>
+__device__ void sweep_cell_soft(const SweepParams &P, const Stencil &S, double *oo, double *pt, double *pR, int *pk,
+                                double *aux, const double *sT, int cell, int own0, int nOwn, int total) {
+    constexpr int NT = WAVE * WAVES;
+    const Grid &g = P.g;
+    const int tid = threadIdx.x;
+    const int wave = tid / WAVE;
+    double *pu = aux;                 // MOVE_CHUNK uniforms
+    double *sums = aux + MOVE_CHUNK;  // [move parity][final, initial][wave]
+    unsigned long long accepted = 0, leftCell = 0, tests = 0;
+    int move = 0;
+    const int moves = P.moves > 0 ? P.moves : nOwn;
+    const int pbase = P.moves > 0 ? cell * P.moves : own0;
+    for (int chunk = 0; chunk < moves; chunk += MOVE_CHUNK) {
+        const int cn = min(MOVE_CHUNK, moves - chunk);
+        load_proposals(P, pbase, chunk, cn, pt, pR, pk);
+        if (tid < cn) pu[tid] = P.pr.u[pbase + chunk + tid];
+        __syncthreads();
+        for (int mi = 0; mi < cn; mi++, move++) {
+            const int kIdx = pk[mi];
+            const int gi = own0 + kIdx;
+            V3 opos, oax;
+            double O[9];
+            int ow[3];
+            if (STAGED) {
+                opos = v3(S.x[kIdx], S.y[kIdx], S.z[kIdx]);
+                for (int c = 0; c < 9; c++) O[c] = oo[c * OWN_CAP + kIdx];
+                const int code = S.code[kIdx];
+                ow[0] = (code & 3) - 1;
+                ow[1] = ((code >> 2) & 3) - 1;
+                ow[2] = ((code >> 4) & 3) - 1;
+            } else {
+                opos = load3(P.p.x, P.p.y, P.p.z, gi);
+                for (int c = 0; c < 9; c++) O[c] = P.p.o[c][gi];
+                const unsigned w = P.p.wbits[gi];
+                ow[0] = w & 1;
+                ow[1] = (w >> 1) & 1;
+                ow[2] = (w >> 2) & 1;
+            }
+            oax = v3(O[2], O[5], O[8]);
+            V3 t = v3(pt[mi], pt[MOVE_CHUNK + mi], pt[2 * MOVE_CHUNK + mi]);
+            V3 npos = translate_wrapped(g.dims, g.inv, opos, t);
+            int nc[3];
+            unsigned nw;
+            bin_position(g, npos, nc, nw);
+            const bool inside = cell_index(g, nc[0], nc[1], nc[2]) == cell;
+            double NO[9], R[9];
+            for (int c = 0; c < 9; c++) R[c] = pR[c * MOVE_CHUNK + mi];
+            mulm(R, O, NO);  // Shape::rotate (Shape.cpp:14-16)
+            const V3 nax = v3(NO[2], NO[5], NO[8]);
+            const int nwx = nw & 1, nwy = (nw >> 1) & 1, nwz = (nw >> 2) & 1;
+            double eFinal = 0, eInitial = 0;
+            if (inside) {
+                SCFirst fN = make_first<KIND>(P.k, npos, nax);
+                SCFirst fO = make_first<KIND>(P.k, opos, oax);
+                for (int j = tid; j < total; j += NT) {
+                    if (j == kIdx) continue;
+                    V3 p2, a2;
+                    int code;
+                    load_candidate<KIND, STAGED>(P.p, S, j, p2, a2, code);
+                    V3 TN = v3(0, 0, 0), TO = v3(0, 0, 0);
+                    if (!TRANSLATED) {
+                        TN = image_translation(sT, code, nwx, nwy, nwz);
+                        TO = image_translation(sT, code, ow[0], ow[1], ow[2]);
+                    }
+                    eFinal += soft_energy(P.k, fN, p2, a2, TN);
+                    eInitial += soft_energy(P.k, fO, p2, a2, TO);
+                }
+                tests += (unsigned long long)total;
+            } else {
+                leftCell++;
+            }
+            eFinal = wave_sum(eFinal);
+            eInitial = wave_sum(eInitial);
+            double *slot = sums + (move & 1) * 2 * WAVES;
+            if ((tid & (WAVE - 1)) == 0) {
+                slot[wave] = eFinal;
+                slot[WAVES + wave] = eInitial;
+            }
+            __syncthreads();  // one barrier per move: the two slots alternate
+            double fin = 0, ini = 0;
+            for (int w = 0; w < WAVES; w++) {
+                fin += slot[w];
+                ini += slot[WAVES + w];
+            }
+            if (!inside || !(pu[mi] <= exp(-(fin - ini) / P.temperature))) continue;
+
+            accepted++;
+            if (STAGED) {
+                S.x[kIdx] = npos.x;
+                S.y[kIdx] = npos.y;
+                S.z[kIdx] = npos.z;
+                if (KIND == 1) {
+                    S.ax[kIdx] = nax.x;
+                    S.ay[kIdx] = nax.y;
+                    S.az[kIdx] = nax.z;
+                }
+                S.code[kIdx] = (nwx + 1) | ((nwy + 1) << 2) | ((nwz + 1) << 4);
+                for (int c = 0; c < 9; c++) oo[c * OWN_CAP + kIdx] = NO[c];
+            } else {
+                if (tid == 0) {
+                    P.p.x[gi] = npos.x;
+                    P.p.y[gi] = npos.y;
+                    P.p.z[gi] = npos.z;
+                    for (int c = 0; c < 9; c++) P.p.o[c][gi] = NO[c];
+                    P.p.wbits[gi] = (uint8_t)nw;
+                }
+                __syncthreads();
+            }
+        }
+        __syncthreads();
+    }
+    if (STAGED) {
+        for (int a = tid; a < nOwn; a += NT) {
+            const int gi = own0 + a;
+            P.p.x[gi] = S.x[a];
+            P.p.y[gi] = S.y[a];
+            P.p.z[gi] = S.z[a];
+            for (int c = 0; c < 9; c++) P.p.o[c][gi] = oo[c * OWN_CAP + a];
+            int code = S.code[a];
+            P.p.wbits[gi] =
+                (uint8_t)(((code & 3) - 1) | ((((code >> 2) & 3) - 1) << 1) | ((((code >> 4) & 3) - 1) << 2));
+        }
+    }
+    flush_counters(P, accepted, moves, leftCell, tests, STAGED);
+}
+
 // Stages the stencil: every lane resolves up to SLOT_GROUP entries, issues all their HBM loads, then writes LDS.
 template <int KIND, int NT>
 __device__ __forceinline__ void stage_stencil_batched(const SweepParams &P, const Stencil &S, const double *sT,
@@ -1065,8 +1354,8 @@ __global__ __launch_bounds__(WAVE *WAVES) void k_sweep(SweepParams P) {
     double *pt = oo + 9 * OWN_CAP;      // 3 x MOVE_CHUNK proposal translations
     double *pR = pt + 3 * MOVE_CHUNK;   // 9 x MOVE_CHUNK proposal rotations
     double *sT = pR + 9 * MOVE_CHUNK;   // 81 translation table
-    double *plans = sT + 81;            // 2 x PLAN_D planner output
-    int *pk = reinterpret_cast<int *>(plans + 2 * 24);
+    double *plans = sT + 81;            // SWEEP_AUX: 2 x PLAN_D planner output, or soft uniforms and sums
+    int *pk = reinterpret_cast<int *>(plans + SWEEP_AUX);
     int *plansI = pk + MOVE_CHUNK;      // 2 x 4 planner output
     int *queue = plansI + 8;            // cap + 64*WAVES ambiguous-candidate slots
     for (int i = tid; i < 81; i += NT) sT[i] = g.T[i];
@@ -1083,7 +1372,13 @@ __global__ __launch_bounds__(WAVE *WAVES) void k_sweep(SweepParams P) {
         __syncthreads();
         if (P.phases) clk[2] = wall_clock64();
         unsigned long long *c3 = P.phases ? clk : nullptr;
-        if (WAVES >= 2 && P.pipelined) {
+        if (P.k.soft) {
+            if (translated)
+                sweep_cell_soft<KIND, WAVES, true, true>(P, S, oo, pt, pR, pk, plans, sT, cell, own0, nOwn, total);
+            else
+                sweep_cell_soft<KIND, WAVES, false, true>(P, S, oo, pt, pR, pk, plans, sT, cell, own0, nOwn, total);
+            if (c3) c3[3] = wall_clock64();
+        } else if (WAVES >= 2 && P.pipelined) {
             if (translated)
                 sweep_cell_pipelined<KIND, WAVES, true>(P, S, oo, pt, pR, pk, queue, sT, s_vote, plans, plansI, own0,
                                                         nOwn, total, cell, c3);
@@ -1097,7 +1392,10 @@ __global__ __launch_bounds__(WAVE *WAVES) void k_sweep(SweepParams P) {
         }
     } else {
         if (P.phases) clk[2] = wall_clock64();
-        sweep_cell_unstaged<KIND, WAVES>(P, S, pt, pR, pk, sT, cell, own0, nOwn, total);
+        if (P.k.soft)
+            sweep_cell_soft<KIND, WAVES, false, false>(P, S, oo, pt, pR, pk, plans, sT, cell, own0, nOwn, total);
+        else
+            sweep_cell_unstaged<KIND, WAVES>(P, S, pt, pR, pk, sT, cell, own0, nOwn, total);
         if (P.phases) clk[3] = wall_clock64();
     }
     if (P.phases && tid == 0) {

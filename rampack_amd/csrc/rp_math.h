@@ -134,9 +134,12 @@ RP_HD double segment_distance2(const V3 &s11, const V3 &u, double a, const V3 &s
 // (SphereTraits.cpp:64, SpherocylinderTraits.cpp:34-43).
 struct PairConsts {
     int kind;            // 0 sphere, 1 spherocylinder
+    int soft;            // 0 hard core; 1 WCA on the centre distance; 2 WCA on the segment distance
     double contact2;     // sphere: pow(2r, 2); spherocylinder: 4*r*r
     double range2;       // spherocylinder: pow(2r + L, 2)
     double halfLen;      // spherocylinder: 0.5*L (cap offset, SpherocylinderTraits.cpp:24-26)
+    double eps, sigma2, cut2;  // WCA: epsilon, sigma*sigma, (sigma*pow(2, 1/6))^2 (RepulsiveLennardJonesInteraction)
+    double softRange2;   // centre distance^2 at and beyond which the soft energy is 0
 };
 
 // Spherocylinder first-particle data that does not depend on the candidate.
@@ -181,6 +184,21 @@ RP_HD double wca_energy(double eps, double sigma2, double cut2, double d2) {
     double x6 = x2 * x2 * x2;
     double x12 = x6 * x6;
     return 4 * eps * (x12 - x6) + eps;
+}
+
+// Soft pair energy of the first particle f against pos2 + T (HardcodedTranslation).  soft 1: WCA of the centre
+// distance (CentralInteraction.h:58-67); soft 2: WCA of the squared distance of the two segments, the caps built as
+// in the overlap test.  Beyond softRange2 the segments cannot be within the cut-off (segment distance >= centre
+// distance - L), so the energy is 0 without the segment evaluation.
+RP_HD double soft_energy(const PairConsts &k, const SCFirst &f, const V3 &pos2, const V3 &axis2, const V3 &T) {
+    V3 p2 = add(pos2, T);
+    V3 d = sub(p2, f.pos);
+    double d2 = dot(d, d);
+    if (k.soft == 1) return wca_energy(k.eps, k.sigma2, k.cut2, d2);
+    if (d2 >= k.softRange2) return 0;
+    V3 c2a = add(p2, scal(axis2, -k.halfLen));
+    V3 c2b = add(p2, scal(axis2, k.halfLen));
+    return wca_energy(k.eps, k.sigma2, k.cut2, segment_distance2(f.capA, f.u, f.a, c2a, c2b));
 }
 
 // Philox4x32-10 (Salmon et al., SC'11), counter-based: one call maps (counter, key) to four 32-bit words.

@@ -169,3 +169,15 @@ def random_snapshot(n: int, seed: int, box: np.ndarray, kind: str = "spherocylin
     else:
         rot = random_rotations(rng, n)
     return np.ascontiguousarray(pos), np.ascontiguousarray(rot)
+
+
+def jittered_lattice(n_side: int, spacing: float, jitter: float, seed: int):
+    """n_side^3 particles on a simple-cubic lattice of the given spacing, each displaced by U(-jitter, jitter) per
+    axis, in the cubic box of side n_side * spacing; identity orientations.  Neighbours stay about `spacing` apart,
+    so soft energies are moderate and every pair contributes."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    idx = np.stack(np.meshgrid(*[np.arange(n_side)] * 3, indexing="ij"), -1).reshape(-1, 3).astype(np.float64)
+    pos = (idx + 0.5) * spacing + (rng.random((len(idx), 3)) * 2.0 - 1.0) * jitter
+    box = np.diag([n_side * spacing] * 3)
+    rot = np.tile(np.eye(3).reshape(1, 9), (len(idx), 1))
+    return box, np.ascontiguousarray(pos), np.ascontiguousarray(rot)

@@ -18,6 +18,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 sys.path.insert(0, os.path.dirname(HERE))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 
 import corpus  # noqa: E402
 import oracle_lib  # noqa: E402
@@ -43,6 +44,40 @@ SNAPSHOTS = [
     ("sphere_triclinic", 0, 0.5, 0.0, TRICLINIC, 20000, 15, False),
     ("sphere_cubic", 0, 0.5, 0.0, np.diag([12.0, 12.0, 12.0]), 1500, 16, False),
 ]
+
+
+CUT = 2 ** (1 / 6)
+
+SOFT_CORPORA = [
+    # name, n, seed, eps, sigma, delta: WCA sphere pairs with centre distances within +-delta of the cut-off
+    ("wca_sphere_pairs", 100000, 41, 1.0, 1.0, 0.25),
+    ("wca_sphere_pairs_scaled", 50000, 42, 3.0, 2.0, 0.5),
+]
+
+SOFT_SNAPSHOTS = [
+    # name, n_side, spacing, jitter, seed: WCA spheres (epsilon = sigma = 1) on a jittered simple-cubic lattice
+    ("wca_sphere_lattice", 12, 1.08, 0.12, 43),
+    ("wca_sphere_lattice_dense", 10, 1.0, 0.05, 44),
+]
+
+# NpT WCA spheres from a simple-cubic start, delta_v box moves (the reference's cycle loop restated on its Packing)
+WCA_NPT = dict(name="wca_sphere_npt", n_side=8, phi0=0.30, temperature=1.0, pressure=2.0, box_step=5.0,
+               therm=4000, avg=12000, every=20, seed=1234)
+
+
+def reference_statistics(ref):
+    from rampack_amd import lattice
+    st = dict(WCA_NPT)
+    box, pos, rot = lattice.hard_sphere_lattice(st["n_side"], st["phi0"])
+    r = oracle_lib.ref_mc_run(ref, 0, 0.5, 0.0, box, pos, rot, st["temperature"], st["pressure"], 0.1, 0.0, 3,
+                              st["box_step"], st["seed"], therm=st["therm"], avg=st["avg"], avg_every=st["every"],
+                              soft=1, eps=1.0, sigma=1.0)
+    s = r["samples"]
+    rho = s[:, 0] / (np.pi / 6)   # packing fraction / v0 with the shape radius 0.5
+    e = s[:, 2]
+    st.update(rho=float(rho.mean()), rho_err=float(rho.std(ddof=1) / np.sqrt(len(rho))),
+              energy=float(e.mean()), energy_err=float(e.std(ddof=1) / np.sqrt(len(e))), samples=int(len(s)))
+    return st
 
 
 def corpus_records(kind, radius, length, n, seed, delta):
@@ -74,6 +109,24 @@ def main():
                                          np.concatenate([pos.ravel(), rot.ravel()])),
                                      overlaps=full, overlaps_early_exit=early))
         print(name, full, early)
+    out["soft_corpora"] = []
+    for name, n, seed, eps, sigma, delta in SOFT_CORPORA:
+        rec = corpus.sphere_contact_corpus(n, seed, radius=sigma * CUT / 2, delta=delta)
+        e = ref.pair_energies(0, 0.5, 0.0, 1, eps, sigma, rec)
+        out["soft_corpora"].append(dict(name=name, n=n, seed=seed, eps=eps, sigma=sigma, delta=delta,
+                                        sha256=corpus.records_sha256(rec), energies_sha256=corpus.records_sha256(e),
+                                        nonzero=int((e != 0).sum())))
+        print(name, int((e != 0).sum()), "/", n)
+    out["soft_snapshots"] = []
+    for name, n_side, spacing, jitter, seed in SOFT_SNAPSHOTS:
+        box, pos, rot = corpus.jittered_lattice(n_side, spacing, jitter, seed)
+        _, energy = ref.snapshot_stats(0, 0.5, 0.0, box, pos, rot, soft=1, eps=1.0, sigma=1.0)
+        out["soft_snapshots"].append(dict(name=name, n_side=n_side, spacing=spacing, jitter=jitter, seed=seed,
+                                          sha256=corpus.records_sha256(np.concatenate([pos.ravel(), rot.ravel()])),
+                                          energy=energy))
+        print(name, energy)
+    out["statistics"] = [reference_statistics(ref)]
+    print(out["statistics"])
     with open(os.path.join(HERE, "golden.json"), "w") as f:
         json.dump(out, f, indent=1)
 

@@ -121,7 +121,8 @@ def _setup_ref(lib):
     lib.ref_nematic_p2.restype = ctypes.c_double
     lib.ref_packing_fraction.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, ctypes.c_size_t]
     lib.ref_packing_fraction.restype = ctypes.c_double
-    lib.ref_mc_run.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, ctypes.c_size_t, _dp, _dp,
+    lib.ref_mc_run.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double,
+                               ctypes.c_double, _dp, ctypes.c_size_t, _dp, _dp,
                                ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                ctypes.c_double, ctypes.c_ulong, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t, _dp, _dp, _dp, _dp, _dp, _dp]
@@ -145,23 +146,26 @@ def ref(perf: bool = False) -> Checker:
 
 
 def ref_mc_run(checker: Checker, kind, radius, length, box, pos, rot, temperature, pressure, trans_step, rot_step,
-               box_kind, box_step, seed, dd=(1, 1, 1), threads=1, therm=0, avg=0, avg_every=1):
-    """Runs the reference hot path with the restated cycle loop; returns dict(samples, box, pos, rot, seconds, steps)."""
+               box_kind, box_step, seed, dd=(1, 1, 1), threads=1, therm=0, avg=0, avg_every=1, soft=0, eps=1.0,
+               sigma=1.0):
+    """Runs the reference hot path with the restated cycle loop; returns dict(samples, box, pos, rot, seconds, steps)
+    with samples rows (packing fraction, P2, total energy / N)."""
     n = len(pos)
     nsamp = (avg // avg_every) if avg_every else 0
-    samples = np.zeros(2 * max(nsamp, 1))
+    samples = np.zeros(3 * max(nsamp, 1))
     obox = np.zeros(9); opos = np.zeros((n, 3)); orot = np.zeros((n, 9))
     secs = ctypes.c_double()
     steps = np.zeros(3)
     box = np.ascontiguousarray(box, dtype=np.float64).reshape(9)
     pos = np.ascontiguousarray(pos, dtype=np.float64)
     rot = np.ascontiguousarray(rot, dtype=np.float64)
-    r = checker.lib.ref_mc_run(kind, radius, length, box.ctypes.data_as(_dp), n, pos.ctypes.data_as(_dp),
+    r = checker.lib.ref_mc_run(kind, radius, length, soft, eps, sigma, box.ctypes.data_as(_dp), n,
+                               pos.ctypes.data_as(_dp),
                                rot.ctypes.data_as(_dp), temperature, pressure, trans_step, rot_step, box_kind,
                                box_step, seed, dd[0], dd[1], dd[2], threads, therm, avg, avg_every,
                                samples.ctypes.data_as(_dp), obox.ctypes.data_as(_dp), opos.ctypes.data_as(_dp),
                                orot.ctypes.data_as(_dp), ctypes.byref(secs), steps.ctypes.data_as(_dp))
     if r < 0:
         raise RuntimeError(f"ref_mc_run failed ({r})")
-    return dict(samples=samples[:2 * r].reshape(-1, 2), box=obox.reshape(3, 3), pos=opos, rot=orot,
+    return dict(samples=samples[:3 * r].reshape(-1, 3), box=obox.reshape(3, 3), pos=opos, rot=orot,
                 seconds=secs.value, steps=steps)

@@ -44,7 +44,7 @@ def test_last_error_is_a_string():
 
 
 def test_structure_layouts_match_header():
-    assert ctypes.sizeof(ra.ShapeDesc) == 24
+    assert ctypes.sizeof(ra.ShapeDesc) == 48
     assert ctypes.sizeof(ra.MoveDesc) == 24
     assert ctypes.sizeof(ra.BoxDesc) == 16
     assert ctypes.sizeof(ra.Counters) == 10 * 8 + 3 * 8

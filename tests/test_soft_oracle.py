@@ -158,3 +158,12 @@ def test_segment_wca_total_energy_matches_brute_force():
     assert n == 0
     assert e == pytest.approx(brute_total_energy(box, pos, rot, 1, 1.0, 1.0, 5.0), rel=1e-9)
     assert e > 0
+
+
+def test_oracle_total_energy_matches_reference_fixture(golden):
+    for s in golden.get("soft_snapshots", []):
+        box, pos, rot = corpus.jittered_lattice(s["n_side"], s["spacing"], s["jitter"], s["seed"])
+        assert corpus.records_sha256(np.concatenate([pos.ravel(), rot.ravel()])) == s["sha256"], s["name"]
+        _, e = oracle_lib.oracle().snapshot_stats(0, 0.5, 0.0, box, pos, rot, soft=1, eps=1.0, sigma=1.0)
+        assert e == pytest.approx(s["energy"], rel=1e-12)
+        assert e > 0
