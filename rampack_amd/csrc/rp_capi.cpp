@@ -235,12 +235,18 @@ int makeSweepGrid(rp_ctx *c, const HostBox &b, const double shiftUnit[3], Grid &
     double cellSize = std::max(c->range / reach, std::cbrt(b.volume() / (double)c->n) / 5);
     for (int d = 0; d < 3; d++) {
         int n = period * (int)std::floor(h[d] / (period * cellSize));
-        if (n < period || (reach > 1 && n < 2 * reach + 1))
+        if (period == 2 && n < 2 && h[d] >= cellSize) {
+            // one cell across this axis: it is its own neighbour under the images -1, 0, +1 (as in the reference's
+            // grid with one cell per axis, NeighbourGrid.cpp:80-100), only colours of even parity on the axis
+            // exist, and the origin shift is 0 so that image offsets stay in {-1, 0, 1}
+            n = 1;
+        } else if (n < period || (reach > 1 && n < 2 * reach + 1)) {
             return fail(RP_ERR_UNSUPPORTED, "box height " + std::to_string(h[d]) + " is too small for a checkerboard "
                                             "of period " + std::to_string(period));
+        }
         g.n[d] = n;
         g.relCell[d] = 1 / (double)n;
-        g.shift[d] = shiftUnit[d] * g.relCell[d];
+        g.shift[d] = n == 1 ? 0.0 : shiftUnit[d] * g.relCell[d];
     }
     g.ncell = g.n[0] * g.n[1] * g.n[2];
     g.exact = 0;
@@ -517,7 +523,7 @@ int runCycle(rp_ctx *c) {
     sp.k = c->k;
     double maxT = c->maxTransImplicit ? minHeight(c->box) / 2 : c->maxTransStep;
     const double ts = std::min(maxT, c->transStep), rs = std::min(c->rotStep, M_PI);
-    for (int d = 0; d < 3; d++) sp.na[d] = g.n[d] / g.period;
+    for (int d = 0; d < 3; d++) sp.na[d] = std::max(1, g.n[d] / g.period);
     double lambda = (double)c->n / (double)g.ncell;
     // Dense cells make a fixed number of moves each (the reference's floor(N / domains) per domain,
     // Simulation.cpp:431-433), which balances the workgroups; sparse cells move as many times as they hold
@@ -562,6 +568,9 @@ int runCycle(rp_ctx *c) {
     c->tstats.sweep_period = (uint64_t)g.period;
     for (int i = 0; i < colours; i++) {
         sp.colour = order[i];
+        // a colour with odd parity on a one-cell axis has no cells
+        const int par[3] = {sp.colour % g.period, (sp.colour / g.period) % g.period, sp.colour / (g.period * g.period)};
+        if ((g.n[0] == 1 && par[0]) || (g.n[1] == 1 && par[1]) || (g.n[2] == 1 && par[2])) continue;
         timingBegin(c, 0);
         RP_HIP(launch_sweep(c->stream, sp, waves));
         timingEnd(c);

@@ -756,7 +756,7 @@ __device__ __forceinline__ bool block_vote(bool hit, int move, volatile int *slo
 //   one workgroup vote decides the move.  An accepted move is written into LDS by every lane (same values), so
 //   no second barrier is needed before the next move reads it.
 template <int KIND, int WAVES, bool TRANSLATED>
-__device__ void sweep_cell_staged(const SweepParams &P, const Stencil &S, double *oo, double *pt, double *pR,
+__device__ __forceinline__ void sweep_cell_staged(const SweepParams &P, const Stencil &S, double *oo, double *pt, double *pR,
                                   int *pk, int *queue, const double *sT, volatile int *vote, int cell, int own0,
                                   int nOwn, int total, unsigned long long *clk) {
     constexpr int NT = WAVE * WAVES;
@@ -921,7 +921,7 @@ __device__ __forceinline__ void plan_move(const SweepParams &P, const Stencil &S
 }
 
 template <int KIND, int WAVES, bool TRANSLATED>
-__device__ void sweep_cell_pipelined(const SweepParams &P, const Stencil &S, double *oo, double *pt, double *pR,
+__device__ __forceinline__ void sweep_cell_pipelined(const SweepParams &P, const Stencil &S, double *oo, double *pt, double *pR,
                                      int *pk, int *queue, const double *sT, volatile int *vote, double *plans,
                                      int *plansI, int own0, int nOwn, int total, int cell,
                                      unsigned long long *clk) {
@@ -1057,7 +1057,7 @@ __device__ void sweep_cell_pipelined(const SweepParams &P, const Stencil &S, dou
 // Sweep of one active cell whose stencil does not fit the LDS area: candidates are read from the cell-sorted
 // arrays in HBM, accepted moves are written there at once.
 template <int KIND, int WAVES>
-__device__ void sweep_cell_unstaged(const SweepParams &P, const Stencil &S, double *pt, double *pR, int *pk,
+__device__ __forceinline__ void sweep_cell_unstaged(const SweepParams &P, const Stencil &S, double *pt, double *pR, int *pk,
                                     const double *sT, int cell, int own0, int nOwn, int total) {
     constexpr int NT = WAVE * WAVES;
     const Grid &g = P.g;
@@ -1141,7 +1141,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 template <int KIND, int WAVES, bool TRANSLATED, bool STAGED>
-__device__ void sweep_cell_soft(const SweepParams &P, const Stencil &S, double *oo, double *pt, double *pR, int *pk,
+__device__ __forceinline__ void sweep_cell_soft(const SweepParams &P, const Stencil &S, double *oo, double *pt, double *pR, int *pk,
                                 double *aux, const double *sT, int cell, int own0, int nOwn, int total) {
     constexpr int NT = WAVE * WAVES;
     const Grid &g = P.g;
@@ -1314,7 +1314,7 @@ __device__ __forceinline__ void stage_stencil_batched(const SweepParams &P, cons
     }
 }
 
-template <int KIND, int WAVES>
+template <int KIND, int WAVES, bool SOFT>
 __global__ __launch_bounds__(WAVE *WAVES) void k_sweep(SweepParams P) {
     constexpr int NT = WAVE * WAVES;
     extern __shared__ double smem[];
@@ -1330,6 +1330,7 @@ __global__ __launch_bounds__(WAVE *WAVES) void k_sweep(SweepParams P) {
     const int per = g.period;
     const int cx = per * bx + P.colour % per, cy = per * by + (P.colour / per) % per,
               cz = per * bz + P.colour / (per * per);
+    if (cx >= g.n[0] || cy >= g.n[1] || cz >= g.n[2]) return;  // odd parity on a one-cell axis
     const int cell = cell_index(g, cx, cy, cz);
     const int own0 = g.cellStart[cell];
     const int nOwn = g.cellStart[cell + 1] - own0;
@@ -1361,7 +1362,10 @@ __global__ __launch_bounds__(WAVE *WAVES) void k_sweep(SweepParams P) {
     for (int i = tid; i < 81; i += NT) sT[i] = g.T[i];
     __syncthreads();
 
-    if (total <= P.cap && nOwn <= OWN_CAP) {
+    // with one cell across an axis the own cell recurs in its stencil under other images; staged copies would go
+    // stale as particles move, so such grids read the stencil from HBM
+    const bool ownRecurs = g.n[0] == 1 || g.n[1] == 1 || g.n[2] == 1;
+    if (total <= P.cap && nOwn <= OWN_CAP && !ownRecurs) {
         // a cell off the last row on every axis holds no wrapped particle: stage pos + T directly
         const bool translated = cx < g.n[0] - 1 && cy < g.n[1] - 1 && cz < g.n[2] - 1;
         stage_stencil_batched<KIND, NT>(P, S, sT, total, translated);
@@ -1372,7 +1376,7 @@ __global__ __launch_bounds__(WAVE *WAVES) void k_sweep(SweepParams P) {
         __syncthreads();
         if (P.phases) clk[2] = wall_clock64();
         unsigned long long *c3 = P.phases ? clk : nullptr;
-        if (P.k.soft) {
+        if (SOFT) {
             if (translated)
                 sweep_cell_soft<KIND, WAVES, true, true>(P, S, oo, pt, pR, pk, plans, sT, cell, own0, nOwn, total);
             else
@@ -1392,7 +1396,7 @@ __global__ __launch_bounds__(WAVE *WAVES) void k_sweep(SweepParams P) {
         }
     } else {
         if (P.phases) clk[2] = wall_clock64();
-        if (P.k.soft)
+        if (SOFT)
             sweep_cell_soft<KIND, WAVES, false, false>(P, S, oo, pt, pR, pk, plans, sT, cell, own0, nOwn, total);
         else
             sweep_cell_unstaged<KIND, WAVES>(P, S, pt, pR, pk, sT, cell, own0, nOwn, total);
@@ -1466,7 +1470,7 @@ int sweep_occupancy(int kind, int cap, int waves) {
     int w = waves >= 4 ? 4 : (waves == 2 ? 2 : 1);
     size_t lds = sweep_lds_bytes(kind, cap, w);
     int blocks = 0;
-#define RP_OCC(KIND, W) hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_sweep<KIND, W>, WAVE * W, lds)
+#define RP_OCC(KIND, W) hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_sweep<KIND, W, false>, WAVE * W, lds)
     if (kind == 0) {
         if (w == 4) RP_OCC(0, 4);
         else if (w == 2) RP_OCC(0, 2);
@@ -1484,16 +1488,18 @@ hipError_t launch_sweep(hipStream_t s, const SweepParams &sp, int waves) {
     unsigned blocks = (unsigned)(sp.na[0] * sp.na[1] * sp.na[2]);
     int w = waves >= 4 ? 4 : (waves == 2 ? 2 : 1);
     size_t lds = sweep_lds_bytes(sp.k.kind, sp.cap, w);
-#define RP_SWEEP(KIND, W) hipLaunchKernelGGL((k_sweep<KIND, W>), dim3(blocks), dim3(WAVE * W), lds, s, sp)
+    // soft and hard sweeps are separate kernels: each instantiation carries one path only
+#define RP_SWEEP(KIND, W, SOFT) hipLaunchKernelGGL((k_sweep<KIND, W, SOFT>), dim3(blocks), dim3(WAVE * W), lds, s, sp)
+#define RP_SWEEP_W(KIND, SOFT)         \
+    if (w == 4) RP_SWEEP(KIND, 4, SOFT); \
+    else if (w == 2) RP_SWEEP(KIND, 2, SOFT); \
+    else RP_SWEEP(KIND, 1, SOFT);
     if (sp.k.kind == 0) {
-        if (w == 4) RP_SWEEP(0, 4);
-        else if (w == 2) RP_SWEEP(0, 2);
-        else RP_SWEEP(0, 1);
+        if (sp.k.soft) { RP_SWEEP_W(0, true) } else { RP_SWEEP_W(0, false) }
     } else {
-        if (w == 4) RP_SWEEP(1, 4);
-        else if (w == 2) RP_SWEEP(1, 2);
-        else RP_SWEEP(1, 1);
+        if (sp.k.soft) { RP_SWEEP_W(1, true) } else { RP_SWEEP_W(1, false) }
     }
+#undef RP_SWEEP_W
 #undef RP_SWEEP
     return hipGetLastError();
 }

@@ -226,3 +226,45 @@ def test_launch_shapes_give_identical_trajectories(kind, monkeypatch):
     for key, ((b, p, r), acc) in results.items():
         assert acc == acc0, key
         assert np.array_equal(b, b0) and np.array_equal(p, p0) and np.array_equal(r, r0), key
+
+
+def test_one_cell_axis_sweep_stays_overlap_free():
+    """A box less than two interaction ranges high on one axis: the grid has one cell across it, the cell is its
+    own neighbour under the images -1, 0, +1, and the sweep reads the stencil from HBM."""
+    box, pos, rot = lattice.spherocylinder_lattice(n_side=(3, 6, 3), spacing=(2.3, 2.2, 8.3))
+    assert box[0, 0] < 2 * SC.range
+    pk = ra.GpuPacking(SC, box, pos, rot, seed=8)
+    pk.set_environment(1.0, 1.0, 0.3, None, box_move=("delta_triclinic", 0.01))
+    c = pk.run_cycles(200, adapt_steps=True)
+    assert c["unstaged_blocks"] > 0
+    assert c["box_too_small"] == 0
+    assert 0 < c["moves_accepted"] < c["moves_attempted"]
+    b, p, r = pk.download()
+    assert overlaps_by_oracle(SC, b, p, r) == 0
+
+
+def test_degenerate_spherocylinder_gas_reference_value():
+    """SimulationTest.cpp:103-148: 50 spherocylinders L = 3, r = 0.5 from a 5 x 5 x 2 arrangement (spacing 2, 2, 5)
+    in a 10^3 box, T = 1, p = 0.5, rototranslation(0.5, 1), delta_v(10); the reference's long-run density is
+    0.09454087.  The equilibrium box (about 8.1 high for a range of 4) moves between one- and two-cell grids.
+    The volume of 50 particles decorrelates slowly: over 10^4 cycles the reference's own runs scatter by 5 %
+    between seeds (0.0929, 0.0973), far beyond the naive standard error; over 6 x 10^4 cycles sampled every 100,
+    three reference seeds give 0.0946, 0.0948, 0.0945 with block errors (10 blocks) of 5-6 x 10^-4.  The same
+    protocol and error estimate are used here."""
+    box, pos, rot = lattice.orthorhombic_arrange(50, (10, 10, 10), (5, 5, 2), (2, 2, 5))
+    shape = ra.Shape("spherocylinder", radius=0.5, length=3.0)
+    pk = ra.GpuPacking(shape, box, pos, rot, seed=1234)
+    pk.set_environment(1.0, 0.5, 0.5, 1.0, box_move=("delta_v", 10.0))
+    pk.run_cycles(5000, adapt_steps=True)
+    rho = []
+    for _ in range(600):
+        pk.run_cycles(100)
+        rho.append(pk.observables()["number_density"])
+    c = pk.run_cycles(0)
+    m = float(np.mean(rho))
+    block_err = float(np.std(np.reshape(rho, (10, -1)).mean(axis=1), ddof=1) / np.sqrt(10))
+    print(f"spherocylinder gas: {m} +- {block_err} (reference 0.09454087); box moves too small "
+          f"{c['box_too_small']}", flush=True)
+    assert c["box_too_small"] == 0
+    assert abs(m - 0.09454087) <= 3 * block_err + 0.001 * 0.09454087
+    assert block_err / m < 0.01
