@@ -10,7 +10,9 @@ p* = 2 with step adaptation; the timed cycles run without adaptation.  Inputs ar
 Roofline: the dominant kernel is the sweep (k_sweep).  Its algorithmic traffic per trial move is
 (27 + a) * 96 B (the reference's 27-cell stencil of FP64 particle records plus the write-back of accepted moves, a =
 acceptance rate); achieved GB/s = that figure times the moves of one launch divided by the launch's average duration,
-measured with HIP events on the launch stream.
+measured with HIP events on the launch stream over the timed cycles (events are read once, after them).  traffic is
+the HBM bytes one sweep launch moves, from rocprofv3 FETCH_SIZE and WRITE_SIZE counters on this workload scaled by
+the same counters on a 256 MiB device copy (SWEEP_TRAFFIC below).
 
 CPU baseline: the reference's own hot path (oracle/_ref/libref_perf.so: Packing, NeighbourGrid,
 SpherocylinderTraits, RototranslationSampler, TriclinicDeltaScaler and DomainDecomposition compiled from the
@@ -33,6 +35,19 @@ sys.path.insert(0, ROOT)
 METRIC = "MC trial moves/s @N=1e5 spherocyl.; plain FP64 vs guarded-FP32+learned order"
 HBM_PEAK_GBS = 8000.0
 RECORD_BYTES = 96  # FP64 position + FP64 3x3 orientation, the reference's particle record
+
+# HBM traffic of one k_sweep launch on this workload (profiles/k_sweep_hard_spherocylinders_N1e5_hbm_traffic.txt):
+# rocprofv3 FETCH_SIZE / WRITE_SIZE means in KiB per dispatch, and the same counters on a 256 MiB device-to-device
+# copy, which reads and writes exactly 256 MiB.
+SWEEP_TRAFFIC = dict(fetch_kib=10827.29, write_kib=1402.11, copy_fetch_kib=131084.19, copy_write_kib=262144.0,
+                     copy_bytes=268435456)
+
+
+def sweep_traffic_bytes() -> float:
+    t = SWEEP_TRAFFIC
+    fetch = t["fetch_kib"] * t["copy_bytes"] / t["copy_fetch_kib"]
+    write = t["write_kib"] * t["copy_bytes"] / t["copy_write_kib"]
+    return fetch + write
 
 
 def log(*a):
@@ -95,7 +110,6 @@ def main():
     ap.add_argument("--warmup", type=int, default=300)
     ap.add_argument("--n", type=int, default=100000)
     ap.add_argument("--seed", type=int, default=20261202)
-    ap.add_argument("--timing-cycles", type=int, default=50)
     ap.add_argument("--cpu-cycles", type=int, default=-1, help="-1: sized to ~20 s of CPU work; 0: skip")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -123,9 +137,13 @@ def main():
     phi_start = pk.packing_fraction()
     before = pk.run_cycles(0)
 
+    # HIP events around every kernel launch on the context's stream, read once after the timed cycles
+    pk.set_timing(True)
     t0 = time.perf_counter()
     after = pk.run_cycles(args.steps)  # returns after the device finished (counters are read back)
     elapsed = time.perf_counter() - t0
+    tm = pk.timing()
+    pk.set_timing(False)
     moves = after["moves_attempted"] - before["moves_attempted"]
     accepted = after["moves_accepted"] - before["moves_accepted"]
     acc_rate = accepted / moves
@@ -135,13 +153,8 @@ def main():
         f"phi {phi_end:.4f}, box moves {after['box_accepted'] - before['box_accepted']}/"
         f"{after['box_attempted'] - before['box_attempted']}, unstaged blocks {after['unstaged_blocks']}")
 
-    # event-timed leg for the roofline (syncs once per cycle, so it is not the throughput measurement)
-    args.timing_cycles = max(args.timing_cycles, 1)
-    pk.set_timing(True)
-    tb = pk.run_cycles(args.timing_cycles)
-    tm = pk.timing()
-    pk.set_timing(False)
-    moves_per_launch = (tb["moves_attempted"] - after["moves_attempted"]) / max(tm["sweep_launches"], 1)
+    tb = after
+    moves_per_launch = moves / max(tm["sweep_launches"], 1)
     avg_sweep_ms = tm["sweep_ms"] / max(tm["sweep_launches"], 1)
     bytes_per_launch = (27 + acc_rate) * RECORD_BYTES * moves_per_launch
     achieved = bytes_per_launch / (avg_sweep_ms * 1e-3) / 1e9
@@ -197,7 +210,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": sweep_traffic_bytes(),
             "kernel": "k_sweep",
             "avg_launch_us": avg_sweep_ms * 1e3,
             "algorithmic_bytes_per_launch": bytes_per_launch,

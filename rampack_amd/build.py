@@ -19,6 +19,9 @@ HIPCC_FLAGS = [
     "-ffp-contract=off",
     # no packed FP32 instructions anywhere in the device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # the sweep kernel reads its ~1.2 KB by-value argument through several hundred uses; above InstCombine's default
+    # limit (300) the argument stays in a private copy, written to scratch by every lane at launch
+    "-mllvm", "-instcombine-max-copied-from-constant-users=4096",
     "-Wno-unused-result", "-Wno-unused-value",
 ]
 

@@ -117,7 +117,7 @@ struct rp_ctx {
     size_t ldsLimit = 64 * 1024 - 1024;  // dynamic LDS a workgroup may use
     int sweepWaves = 0;                  // wavefronts per active cell; 0 = chosen from the stencil size
     int colourPeriod = 0;                // checkerboard colour period (2 or 3); 0 = period 2
-    int pipelined = 1;                   // planner/tester split of the staged sweep
+    int pipelined = 1;                   // staged sweep: 0 plain, 1 planner/tester split, 2 grouped moves
 
     // timing
     bool timing = false;
@@ -685,7 +685,7 @@ int rp_create(rp_ctx **out, const rp_shape_desc *shape, const double box[9], siz
         c->ldsLimit = (size_t)std::min(ldsBytes, 160 * 1024) - 1024;
     if (const char *w = std::getenv("RAMPACK_SWEEP_WAVES")) c->sweepWaves = std::atoi(w);
     if (const char *p = std::getenv("RAMPACK_COLOURING")) c->colourPeriod = std::atoi(p);
-    if (const char *p = std::getenv("RAMPACK_SWEEP_PIPELINE")) c->pipelined = std::atoi(p) != 0;
+    if (const char *p = std::getenv("RAMPACK_SWEEP_PIPELINE")) c->pipelined = std::atoi(p);
     if (const char *ph = std::getenv("RAMPACK_SWEEP_PHASES"); ph && std::atoi(ph) != 0) {
         unsigned long long init[RP_PHASE_NUM] = {};
         if (hipMalloc(&c->dPhases, sizeof(unsigned long long) * RP_PHASE_NUM) != hipSuccess ||
@@ -799,11 +799,13 @@ int rp_run_cycles(rp_ctx *c, size_t nCycles, int adapt, rp_counters *out) {
             r = adaptSteps(c);
             if (r) return r;
         }
-        if (c->timing) {
+        if (c->timing && c->evUsed >= 8192) {  // bounded event pool; otherwise read once at the end
             r = timingHarvest(c);
             if (r) return r;
         }
     }
+    r = timingHarvest(c);
+    if (r) return r;
     r = readCounters(c);
     if (r) return r;
     if (out) {

@@ -54,7 +54,8 @@ struct SweepParams {
     int colour;
     int na[3];              // active cells per axis (n / period)
     int cap;                // stencil entries that fit in LDS
-    int pipelined;          // 1: wavefront 0 plans the next move while the others test (needs >= 2 wavefronts)
+    int pipelined;          // 1: wavefront 0 plans the next move while the others test; 2: grouped moves
+                            // (both need >= 2 wavefronts)
     double temperature;     // soft interactions: the move is accepted if u <= exp(-dE / temperature)
     unsigned long long *counters;
     unsigned long long *phases;  // optional per-phase wall-clock sums (RP_PHASE_*), nullptr when not profiling

@@ -646,6 +646,9 @@ hipError_t launch_total_energy(hipStream_t s, const Grid &g, int n, Particles p,
 // step with a workgroup vote after every step, and the first step with an overlap rejects the move.
 
 constexpr int OWN_CAP = 64;   // own-cell particles kept in LDS
+constexpr int SWEEP_AUX = 288;   // doubles of the planner / soft scratch area (grouped: 3 buffers x 4 plans x 24)
+constexpr int SWEEP_AUX_I = 32;  // ints of planner output (grouped: 3 buffers x 8, plus 3 hit words)
+constexpr int GROUPED_QUEUE = 512;  // queued (candidate, move) pairs per tester wavefront in the grouped sweep
 constexpr int MOVE_CHUNK = 64; // proposals staged per batch
 constexpr int SLOT_GROUP = 4;  // candidate slots loaded per lane before any is tested
 
@@ -710,8 +713,9 @@ hipError_t launch_proposals(hipStream_t s, const Grid &g, const Proposals &pr, i
 // LDS of one sweep workgroup: the stencil (cap entries), the ambiguous-candidate queues (cap + 64*waves ints),
 // own orientations, one batch of proposals and the translation table.
 size_t sweep_lds_bytes(int kind, int cap, int waves) {
-    return stencil_lds_bytes(kind, cap) + sizeof(int) * (cap + WAVE * waves) +
-           sizeof(double) * (9 * OWN_CAP + 12 * MOVE_CHUNK + 81 + 80) + sizeof(int) * (MOVE_CHUNK + 8);
+    const int queueInts = max(cap + WAVE * waves, (waves - 1) * GROUPED_QUEUE);
+    return stencil_lds_bytes(kind, cap) + sizeof(int) * queueInts +
+           sizeof(double) * (9 * OWN_CAP + 12 * MOVE_CHUNK + 81 + SWEEP_AUX) + sizeof(int) * (MOVE_CHUNK + SWEEP_AUX_I);
 }
 
 __device__ __forceinline__ void flush_counters(const SweepParams &P, unsigned long long accepted, int moves,
@@ -1126,13 +1130,309 @@ __device__ __forceinline__ void sweep_cell_unstaged(const SweepParams &P, const 
     flush_counters(P, accepted, moves, leftCell, tests, false);
 }
 
+// Grouped sweep of one staged cell (two or more wavefronts).  Up to GROUP consecutive trial moves on distinct
+// particles are tested against the stencil at once, each candidate loaded once for all of them, and then decided in
+// their order exactly as the sequential sweep decides them: move k is rejected if it overlaps a particle outside the
+// group, or a group particle i in the pose that particle has when move k runs -- the old pose for i > k or for a
+// rejected move i < k (found by the stencil scan), the trial pose for an accepted move i < k (found by the planner's
+// pair tests between trial poses).  Same decisions, same trajectory as one move at a time.  Wavefront 0 plans the
+// next group (one lane per move) while the others scan; one workgroup barrier per group.  Plans and hit words are
+// triple-buffered: a buffer is rewritten two groups after its use, when every wavefront has passed the barrier that
+// follows its last read.
+constexpr int GROUP = 4;
+constexpr int GQ = GROUPED_QUEUE;
+constexpr int GPLAN_I = 8;       // ints per group plan: kIdx[4], size, trial-pair hits, packed (nw | inside << 3)
+constexpr int GBUF = 3;          // plan / hit-word buffers
+static_assert(SWEEP_AUX >= GBUF * GROUP * PLAN_D && SWEEP_AUX_I >= GBUF * GPLAN_I + GBUF, "grouped layout");
+
+template <int KIND>
+__device__ __forceinline__ bool hard_pair_hit(const PairConsts &k, const SCFirst &f, const V3 &npos, const V3 &p2,
+                                              const V3 &a2) {
+    V3 d = sub(p2, npos);
+    double d2 = dot(d, d);
+    if (d2 < k.contact2) return true;
+    if (KIND == 0 || !(d2 < k.range2)) return false;
+    V3 c2a = add(p2, scal(a2, -k.halfLen));
+    V3 c2b = add(p2, scal(a2, k.halfLen));
+    return segment_distance2(f.capA, f.u, f.a, c2a, c2b) < k.contact2;
+}
+
+// Size of the group starting at move m0 of the chunk: the longest prefix of at most GROUP moves on distinct
+// particles.
+__device__ __forceinline__ int group_size(const int *pk, int m0, int cn) {
+    int gs = 1;
+    while (gs < GROUP && m0 + gs < cn) {
+        const int kk = pk[m0 + gs];
+        bool dup = false;
+        for (int i = 0; i < gs; i++) dup |= pk[m0 + i] == kk;
+        if (dup) break;
+        gs++;
+    }
+    return gs;
+}
+
+__device__ __forceinline__ SCFirst plan_first(const double *q) {
+    SCFirst f;
+    f.pos = v3(q[0], q[1], q[2]);
+    f.capA = v3(q[6], q[7], q[8]);
+    f.u = v3(q[9], q[10], q[11]);
+    f.a = q[12];
+    return f;
+}
+
+__device__ __forceinline__ int wrap_code(unsigned nw) {
+    return (int)((nw & 1) + 1) | ((int)(((nw >> 1) & 1) + 1) << 2) | ((int)(((nw >> 2) & 1) + 1) << 4);
+}
+
+// Planner wavefront: lane t < gs plans move m0 + t into plan slot t; then lanes 0-5 test the trial poses against
+// each other (bit 4k + i: move k's trial pose overlaps move i's trial pose, i < k).
+template <int KIND, bool TRANSLATED>
+__device__ __forceinline__ void plan_group(const SweepParams &P, const Stencil &S, const double *oo, const double *pt,
+                                           const double *pR, const int *pk, const double *sT, int m0, int gs,
+                                           int cell, double *plan, int *planI) {
+    const Grid &g = P.g;
+    const int lane = threadIdx.x & (WAVE - 1);
+    unsigned byte = 0;
+    if (lane < gs) {
+        const int mi = m0 + lane;
+        const int kIdx = pk[mi];
+        V3 pos = v3(S.x[kIdx], S.y[kIdx], S.z[kIdx]);
+        V3 t = v3(pt[mi], pt[MOVE_CHUNK + mi], pt[2 * MOVE_CHUNK + mi]);
+        V3 npos = translate_wrapped(g.dims, g.inv, pos, t);
+        int nc[3];
+        unsigned nw;
+        bin_position(g, npos, nc, nw);
+        const unsigned inside = cell_index(g, nc[0], nc[1], nc[2]) == cell ? 1u : 0u;
+        byte = nw | (inside << 3);
+        double O[9], R[9], NO[9];
+        for (int c = 0; c < 9; c++) {
+            O[c] = oo[c * OWN_CAP + kIdx];
+            R[c] = pR[c * MOVE_CHUNK + mi];
+        }
+        mulm(R, O, NO);  // Shape::rotate (Shape.cpp:14-16)
+        const V3 nax = v3(NO[2], NO[5], NO[8]);
+        SCFirst f = make_first<KIND>(P.k, npos, nax);
+        double *q = plan + lane * PLAN_D;
+        q[0] = npos.x; q[1] = npos.y; q[2] = npos.z;
+        q[3] = nax.x; q[4] = nax.y; q[5] = nax.z;
+        if (KIND == 1) {
+            q[6] = f.capA.x; q[7] = f.capA.y; q[8] = f.capA.z;
+            q[9] = f.u.x; q[10] = f.u.y; q[11] = f.u.z;
+            q[12] = f.a;
+        }
+        for (int c = 0; c < 9; c++) q[13 + c] = NO[c];
+        planI[lane] = kIdx;
+    }
+    unsigned word = 0;
+    for (int t = 0; t < gs; t++) word |= ((unsigned)__shfl((int)byte, t) & 15u) << (4 * t);
+    __builtin_amdgcn_wave_barrier();  // the plans were written by other lanes of this wavefront
+    unsigned hit = 0;
+    if (lane < 6) {
+        const int kk = lane < 1 ? 1 : (lane < 3 ? 2 : 3);
+        const int ii = lane < 1 ? 0 : (lane < 3 ? lane - 1 : lane - 3);
+        if (kk < gs) {
+            const double *qk = plan + kk * PLAN_D, *qi = plan + ii * PLAN_D;
+            SCFirst f;
+            if (KIND == 1) f = plan_first(qk);
+            else f.pos = v3(qk[0], qk[1], qk[2]);
+            V3 p2 = v3(qi[0], qi[1], qi[2]);
+            const V3 a2 = v3(qi[3], qi[4], qi[5]);
+            if (!TRANSLATED) {
+                const unsigned wk = (word >> (4 * kk)) & 7, wi = (word >> (4 * ii)) & 7;
+                p2 = add(p2, image_translation(sT, wrap_code(wi), wk & 1, (wk >> 1) & 1, (wk >> 2) & 1));
+            }
+            if (hard_pair_hit<KIND>(P.k, f, f.pos, p2, a2)) hit = 1u << (4 * kk + ii);
+        }
+    }
+    for (int off = 1; off < 8; off <<= 1) hit |= (unsigned)__shfl_xor((int)hit, off);
+    if (lane == 0) {
+        planI[4] = gs;
+        planI[5] = (int)hit;
+        planI[6] = (int)word;
+    }
+}
+
+template <int KIND, int WAVES, bool TRANSLATED>
+__device__ __forceinline__ void sweep_cell_grouped(const SweepParams &P, const Stencil &S, double *oo, double *pt,
+                                                   double *pR, int *pk, int *queue, const double *sT, double *plans,
+                                                   int *plansI, int *hitWords, int own0, int nOwn, int total, int cell,
+                                                   unsigned long long *clk) {
+    constexpr int NTT = WAVE * (WAVES - 1);
+    const int tid = threadIdx.x;
+    const int lane = tid & (WAVE - 1);
+    const int wave = tid / WAVE;
+    const bool planner = wave == 0;
+    const int ttid = tid - WAVE;
+    int *myQueue = queue + (wave > 0 ? wave - 1 : 0) * GQ;
+    unsigned long long accepted = 0, leftCell = 0, tests = 0;
+    const int moves = P.moves > 0 ? P.moves : nOwn;
+    const int pbase = P.moves > 0 ? cell * P.moves : own0;
+
+    for (int chunk = 0; chunk < moves; chunk += MOVE_CHUNK) {
+        const int cn = min(MOVE_CHUNK, moves - chunk);
+        load_proposals(P, pbase, chunk, cn, pt, pR, pk);
+        if (tid < GBUF) hitWords[tid] = 0;
+        __syncthreads();
+        int m0 = 0, gidx = 0;
+        int gs = group_size(pk, 0, cn);
+        if (planner) plan_group<KIND, TRANSLATED>(P, S, oo, pt, pR, pk, sT, 0, gs, cell, plans, plansI);
+        __syncthreads();
+        while (m0 < cn) {
+            const int buf = gidx % GBUF, nbuf = (gidx + 1) % GBUF;
+            const double *plan = plans + buf * GROUP * PLAN_D;
+            const int *planI = plansI + buf * GPLAN_I;
+            // this group's plan in registers
+            int kIdx[GROUP];
+            V3 npos[GROUP];
+#pragma unroll
+            for (int k = 0; k < GROUP; k++) {
+                kIdx[k] = k < gs ? planI[k] : -1;
+                npos[k] = k < gs ? v3(plan[k * PLAN_D], plan[k * PLAN_D + 1], plan[k * PLAN_D + 2]) : v3(0, 0, 0);
+            }
+            const unsigned newHit = (unsigned)planI[5];
+            const unsigned word = (unsigned)planI[6];  // 4 bits per move: nw | inside << 3
+            const int m1 = m0 + gs;
+            const int gsNext = m1 < cn ? group_size(pk, m1, cn) : 0;
+            if (planner) {
+                if (gsNext)
+                    plan_group<KIND, TRANSLATED>(P, S, oo, pt, pR, pk, sT, m1, gsNext, cell,
+                                                 plans + nbuf * GROUP * PLAN_D, plansI + nbuf * GPLAN_I);
+                if (lane == 0) hitWords[nbuf] = 0;
+            } else {
+                // 5 bits per move: bits 0-3 overlap with group particle i's current pose, bit 4 with a particle
+                // outside the group
+                unsigned hitAll = 0;
+                int queued = 0;
+                const int slots = (total + NTT - 1) / NTT;
+                for (int s = 0; s < slots; s++) {
+                    const int j = s * NTT + ttid;
+                    const bool valid = j < total;
+                    const int jj = valid ? j : 0;
+                    const V3 p2 = v3(S.x[jj], S.y[jj], S.z[jj]);
+                    const int code = TRANSLATED ? 0 : S.code[jj];
+                    unsigned groupBit = 16;
+#pragma unroll
+                    for (int i = 0; i < GROUP; i++)
+                        if (jj == kIdx[i]) groupBit = 1u << i;
+#pragma unroll
+                    for (int k = 0; k < GROUP; k++) {
+                        if (k < gs) {
+                            const unsigned wk = (word >> (4 * k)) & 15;
+                            bool amb = false;
+                            if (valid && (wk & 8) && jj != kIdx[k]) {
+                                V3 q2 = p2;
+                                if (!TRANSLATED)
+                                    q2 = add(p2, image_translation(sT, code, wk & 1, (wk >> 1) & 1, (wk >> 2) & 1));
+                                V3 d = sub(q2, npos[k]);
+                                double d2 = dot(d, d);
+                                if (d2 < P.k.contact2) hitAll |= groupBit << (5 * k);
+                                else if (KIND == 1 && d2 < P.k.range2) amb = true;
+                            }
+                            if (KIND == 1) {
+                                unsigned long long m = __ballot(amb);
+                                if (amb) myQueue[queued + __popcll(m & ((1ull << lane) - 1))] = (jj << 2) | k;
+                                queued += __popcll(m);
+                            }
+                        }
+                    }
+                    if (KIND == 1 && (queued > GQ - GROUP * WAVE || s + 1 == slots)) {
+                        // exact segment tests of the queued pairs (pass B)
+                        __builtin_amdgcn_wave_barrier();  // the queue was written by other lanes of this wavefront
+                        for (int q = lane; q < queued; q += WAVE) {
+                            const int e = myQueue[q];
+                            const int jq = e >> 2, k = e & 3;
+                            const unsigned wk = (word >> (4 * k)) & 15;
+                            V3 c2 = v3(S.x[jq], S.y[jq], S.z[jq]);
+                            if (!TRANSLATED)
+                                c2 = add(c2, image_translation(sT, S.code[jq], wk & 1, (wk >> 1) & 1, (wk >> 2) & 1));
+                            const V3 a2 = v3(S.ax[jq], S.ay[jq], S.az[jq]);
+                            const SCFirst f = plan_first(plan + k * PLAN_D);
+                            const V3 c2a = add(c2, scal(a2, -P.k.halfLen));
+                            const V3 c2b = add(c2, scal(a2, P.k.halfLen));
+                            if (segment_distance2(f.capA, f.u, f.a, c2a, c2b) < P.k.contact2) {
+                                unsigned bit = 16;
+#pragma unroll
+                                for (int i = 0; i < GROUP; i++)
+                                    if (jq == kIdx[i]) bit = 1u << i;
+                                hitAll |= bit << (5 * k);
+                            }
+                        }
+                        __builtin_amdgcn_wave_barrier();
+                        queued = 0;
+                    }
+                }
+                for (int off = 1; off < WAVE; off <<= 1) hitAll |= (unsigned)__shfl_xor((int)hitAll, off);
+                if (lane == 0 && hitAll) atomicOr(&hitWords[buf], (int)hitAll);
+            }
+            __syncthreads();  // the group's only barrier: hit words and the next plan are complete
+            // decide the group's moves in order
+            const unsigned hw = (unsigned)hitWords[buf];
+            unsigned acc = 0;
+            for (int k = 0; k < gs; k++) {
+                const unsigned b = (hw >> (5 * k)) & 31;
+                const bool inside = (word >> (4 * k + 3)) & 1;
+                bool hit = !inside || (b & 16);
+                for (int i = 0; i < gs; i++) {
+                    if (i == k) continue;
+                    if (i < k && ((acc >> i) & 1)) hit |= (newHit >> (4 * k + i)) & 1;
+                    else hit |= (b >> i) & 1;
+                }
+                if (!inside) leftCell++;
+                else tests += (unsigned long long)total;
+                if (!hit) acc |= 1u << k;
+            }
+            bool replan = false;
+            for (int k = 0; k < gs; k++) {
+                if (!((acc >> k) & 1)) continue;
+                accepted++;
+                const int ki = planI[k];
+                const double *q = plan + k * PLAN_D;
+                S.x[ki] = q[0];
+                S.y[ki] = q[1];
+                S.z[ki] = q[2];
+                if (KIND == 1) {
+                    S.ax[ki] = q[3];
+                    S.ay[ki] = q[4];
+                    S.az[ki] = q[5];
+                }
+                S.code[ki] = wrap_code((word >> (4 * k)) & 7);
+                if (planner)
+                    for (int c = 0; c < 9; c++) oo[c * OWN_CAP + ki] = q[13 + c];
+                for (int t = 0; t < gsNext; t++) replan |= pk[m1 + t] == ki;
+            }
+            if (replan) {
+                // the next group was planned from a pose this group just replaced
+                __syncthreads();
+                if (planner)
+                    plan_group<KIND, TRANSLATED>(P, S, oo, pt, pR, pk, sT, m1, gsNext, cell,
+                                                 plans + nbuf * GROUP * PLAN_D, plansI + nbuf * GPLAN_I);
+                __syncthreads();
+            }
+            m0 = m1;
+            gs = gsNext;
+            gidx++;
+        }
+        __syncthreads();
+    }
+    if (clk) clk[3] = wall_clock64();
+    for (int a = tid; a < nOwn; a += WAVE * WAVES) {
+        const int gi = own0 + a;
+        P.p.x[gi] = S.x[a];
+        P.p.y[gi] = S.y[a];
+        P.p.z[gi] = S.z[a];
+        for (int c = 0; c < 9; c++) P.p.o[c][gi] = oo[c * OWN_CAP + a];
+        int code = S.code[a];
+        P.p.wbits[gi] = (uint8_t)(((code & 3) - 1) | ((((code >> 2) & 3) - 1) << 1) | ((((code >> 4) & 3) - 1) << 2));
+    }
+    flush_counters(P, accepted, moves, leftCell, tests, true);
+}
+
 // Soft sweep of one active cell (Packing::tryMove with a soft interaction, Packing.cpp:136-164, and the Metropolis
 // test of Simulation.cpp:483).  Every trial move sums the moved particle's energy with all stencil candidates at
 // its current pose (initialEnergy) and at the trial pose (finalEnergy) in one pass; the lanes' sums are reduced per
 // wavefront (xor butterfly: every lane ends with the same bits), the wavefronts' sums in index order, and the move is
 // accepted if it stays in the cell and u <= exp(-(finalEnergy - initialEnergy) / T).  STAGED: stencil and own
 // orientations in LDS; otherwise read from HBM and accepted poses written back there by lane 0.
-constexpr int SWEEP_AUX = 80;  // doubles of the planner / soft scratch area
 static_assert(SWEEP_AUX >= 2 * PLAN_D && SWEEP_AUX >= MOVE_CHUNK + 2 * 2 * 4, "sweep scratch area too small");
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -1357,8 +1657,8 @@ __global__ __launch_bounds__(WAVE *WAVES) void k_sweep(SweepParams P) {
     double *sT = pR + 9 * MOVE_CHUNK;   // 81 translation table
     double *plans = sT + 81;            // SWEEP_AUX: 2 x PLAN_D planner output, or soft uniforms and sums
     int *pk = reinterpret_cast<int *>(plans + SWEEP_AUX);
-    int *plansI = pk + MOVE_CHUNK;      // 2 x 4 planner output
-    int *queue = plansI + 8;            // cap + 64*WAVES ambiguous-candidate slots
+    int *plansI = pk + MOVE_CHUNK;      // SWEEP_AUX_I planner output ints
+    int *queue = plansI + SWEEP_AUX_I;  // max(cap + 64*WAVES, (WAVES-1)*GROUPED_QUEUE) queued candidates
     for (int i = tid; i < 81; i += NT) sT[i] = g.T[i];
     __syncthreads();
 
@@ -1382,6 +1682,14 @@ __global__ __launch_bounds__(WAVE *WAVES) void k_sweep(SweepParams P) {
             else
                 sweep_cell_soft<KIND, WAVES, false, true>(P, S, oo, pt, pR, pk, plans, sT, cell, own0, nOwn, total);
             if (c3) c3[3] = wall_clock64();
+        } else if (WAVES >= 2 && P.pipelined == 2) {
+            int *hitWords = plansI + GBUF * GPLAN_I;
+            if (translated)
+                sweep_cell_grouped<KIND, WAVES, true>(P, S, oo, pt, pR, pk, queue, sT, plans, plansI, hitWords, own0,
+                                                      nOwn, total, cell, c3);
+            else
+                sweep_cell_grouped<KIND, WAVES, false>(P, S, oo, pt, pR, pk, queue, sT, plans, plansI, hitWords, own0,
+                                                       nOwn, total, cell, c3);
         } else if (WAVES >= 2 && P.pipelined) {
             if (translated)
                 sweep_cell_pipelined<KIND, WAVES, true>(P, S, oo, pt, pR, pk, queue, sT, s_vote, plans, plansI, own0,

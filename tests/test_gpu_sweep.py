@@ -159,10 +159,10 @@ def test_degenerate_hard_sphere_gas_carnahan_starling():
     assert err / rho < 0.03
 
 
-@pytest.mark.parametrize("waves,pipe", [(1, 0), (2, 0), (4, 0), (2, 1), (4, 1)])
+@pytest.mark.parametrize("waves,pipe", [(1, 0), (2, 0), (4, 0), (2, 1), (4, 1), (2, 2), (4, 2)])
 @pytest.mark.parametrize("kind", ["sphere", "spherocylinder"])
 def test_every_launch_shape_is_overlap_free(kind, waves, pipe, monkeypatch):
-    """Each workgroup size (1, 2 or 4 wavefronts per active cell, with and without the planner/tester split) for
+    """Each workgroup size (1, 2 or 4 wavefronts per active cell; plain, planner/tester split or grouped moves) for
     both shapes keeps the packing overlap-free."""
     monkeypatch.setenv("RAMPACK_SWEEP_WAVES", str(waves))
     monkeypatch.setenv("RAMPACK_SWEEP_PIPELINE", str(pipe))
@@ -203,7 +203,8 @@ def test_both_colourings_are_overlap_free(period, monkeypatch):
 @pytest.mark.parametrize("kind", ["sphere", "spherocylinder"])
 def test_launch_shapes_give_identical_trajectories(kind, monkeypatch):
     """Proposals are drawn per (cycle, cell, move) and every decision is the same FP64 evaluation, so the
-    single-wavefront, multi-wavefront and pipelined (planner + testers) sweeps produce bit-identical packings."""
+    single-wavefront, multi-wavefront, pipelined (planner + testers) and grouped (several moves tested at once,
+    decided in order) sweeps produce bit-identical packings."""
     if kind == "sphere":
         shape = HS
         box, pos, rot = lattice.hard_sphere_lattice(10, 0.45)
@@ -213,7 +214,7 @@ def test_launch_shapes_give_identical_trajectories(kind, monkeypatch):
         box, pos, rot = rod_start()
         move = ("delta_triclinic", 0.002)
     results = {}
-    for waves, pipe in ((1, 0), (2, 0), (4, 0), (2, 1), (4, 1)):
+    for waves, pipe in ((1, 0), (2, 0), (4, 0), (2, 1), (4, 1), (2, 2), (4, 2)):
         monkeypatch.setenv("RAMPACK_SWEEP_WAVES", str(waves))
         monkeypatch.setenv("RAMPACK_SWEEP_PIPELINE", str(pipe))
         pk = ra.GpuPacking(shape, box, pos, rot, seed=77)
