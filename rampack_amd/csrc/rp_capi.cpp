@@ -304,9 +304,11 @@ int makeExactGrid(rp_ctx *c, const HostBox &b, Grid &g, int &disabled) {
 void timingBegin(rp_ctx *c, int kind) {
     if (!c->timing) return;
     if (c->evUsed == c->evStart.size()) {
+        // timing only, read after a stream synchronisation: no system-scope fence (cache write-back and
+        // invalidation) per record, which would slow the work between the events
         hipEvent_t a, b;
-        hipEventCreate(&a);
-        hipEventCreate(&b);
+        hipEventCreateWithFlags(&a, hipEventDisableSystemFence);
+        hipEventCreateWithFlags(&b, hipEventDisableSystemFence);
         c->evStart.push_back(a);
         c->evStop.push_back(b);
         c->evKind.push_back(0);
