@@ -148,6 +148,13 @@ int rp_count_total_overlaps(rp_ctx *ctx, int early_exit, uint64_t *out);
  * sum. */
 int rp_total_energy(rp_ctx *ctx, double *out);
 
+/* Packing::renormalizeOrientations (src/core/Packing.cpp:1178-1216) as Simulation::fixRotationMatrices runs it
+ * (Simulation.cpp:750-769): every orientation becomes three iterations of R <- 1.5 R - 0.5 R R^T R unless that
+ * creates an overlap; *rejected receives the number of particles left unchanged.  rp_run_cycles runs it after every
+ * `every`-th cycle (the reference's rotation_matrix_fix_every, default here 10000; 0 disables). */
+int rp_fix_orientations(rp_ctx *ctx, uint64_t *rejected);
+int rp_set_orientation_fix_every(rp_ctx *ctx, uint64_t every);
+
 /* Current state, in the caller's particle order (Packing::operator[], Packing::getBox). */
 int rp_download(rp_ctx *ctx, double box[9], double *pos, double *rot);
 

@@ -160,6 +160,35 @@ double ref_packing_fraction(int kind, double radius, double length, const double
     return packing.getPackingFraction(traits->getGeometry().getVolume());
 }
 
+// Packing::fixRotationMatrix (Packing.cpp:1211-1216, private) through the public
+// Packing::renormalizeOrientations(interaction, false) on particles 10 apart, where no fix can create an overlap, so
+// every fix is accepted; n row-major matrices in place.
+int ref_fix_rotation(std::size_t n, double *rot) {
+    try {
+        SpherocylinderTraits traits(1, 0.5);
+        std::size_t side = 1;
+        while (side * side * side < n) side++;
+        const double L = 10.0 * (double)side;
+        std::vector<double> pos(3 * n);
+        for (std::size_t i = 0; i < n; i++) {
+            pos[3 * i] = 10.0 * (double)(i % side) + 5;
+            pos[3 * i + 1] = 10.0 * (double)((i / side) % side) + 5;
+            pos[3 * i + 2] = 10.0 * (double)(i / (side * side)) + 5;
+        }
+        Packing packing(TriclinicBox(std::array<double, 3>{L, L, L}), makeShapes(n, pos.data(), rot),
+                        std::make_unique<PeriodicBoundaryConditions>(), traits.getInteraction(), 1, 1);
+        if (packing.renormalizeOrientations(traits.getInteraction(), false) != 0) return -2;
+        for (std::size_t i = 0; i < n; i++) {
+            const auto &O = packing[i].getOrientation();
+            for (int a = 0; a < 3; a++)
+                for (int b = 0; b < 3; b++) rot[9 * i + 3 * a + b] = O(a, b);
+        }
+        return 0;
+    } catch (const std::exception &) {
+        return -1;
+    }
+}
+
 double ref_nematic_p2(std::size_t n, const double *rot) {
     SpherocylinderTraits traits(1, 1);
     std::vector<double> pos(3 * n, 0.5);

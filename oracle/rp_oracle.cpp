@@ -607,6 +607,34 @@ int orc_pair_energies(int kind, double radius, double length, int soft, double e
     return 0;
 }
 
+// Packing::fixRotationMatrix (Packing.cpp:1211-1216) on n row-major 3x3 matrices in place:
+// R <- (1.5 R) - (((0.5 R) R^T) R), three times.
+void orc_fix_rotation(std::size_t n, double *rot) {
+    for (std::size_t k = 0; k < n; k++) {
+        double *R = rot + 9 * k;
+        for (int it = 0; it < 3; it++) {
+            double A[9], B[9], C[9], D[9];
+            for (int i = 0; i < 9; i++) {
+                A[i] = R[i] * 1.5;
+                B[i] = R[i] * 0.5;
+            }
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) {
+                    double s = 0;
+                    for (int m = 0; m < 3; m++) s += B[3 * i + m] * R[3 * j + m];  // (B R^T)_ij
+                    C[3 * i + j] = s;
+                }
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) {
+                    double s = 0;
+                    for (int m = 0; m < 3; m++) s += C[3 * i + m] * R[3 * m + j];
+                    D[3 * i + j] = s;
+                }
+            for (int i = 0; i < 9; i++) R[i] = A[i] - D[i];
+        }
+    }
+}
+
 // Squared segment distance for 4 points per record (12 doubles).
 void orc_segment_distance2(std::size_t n, const double *pts, double *out) {
     for (std::size_t i = 0; i < n; i++) {

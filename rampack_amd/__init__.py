@@ -40,7 +40,7 @@ RP_ERR_UNSUPPORTED = -4
 EXPORTED_SYMBOLS = (
     "rp_create", "rp_set_environment", "rp_run_cycles", "rp_count_total_overlaps", "rp_download",
     "rp_set_timing", "rp_get_timing", "rp_destroy", "rp_pair_decisions", "rp_last_error", "rp_get_observables",
-    "rp_total_energy", "rp_pair_energies",
+    "rp_total_energy", "rp_pair_energies", "rp_fix_orientations", "rp_set_orientation_fix_every",
 )
 
 
@@ -134,6 +134,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
                                       ctypes.POINTER(ctypes.c_uint8)]
     lib.rp_get_observables.argtypes = [vp, ctypes.POINTER(Observables)]
     lib.rp_total_energy.argtypes = [vp, dp]
+    lib.rp_fix_orientations.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64)]
+    lib.rp_set_orientation_fix_every.argtypes = [vp, ctypes.c_uint64]
     lib.rp_pair_energies.argtypes = [ctypes.POINTER(ShapeDesc), ctypes.c_size_t, dp, dp]
     lib.rp_last_error.restype = ctypes.c_char_p
     _lib = lib
@@ -281,6 +283,15 @@ class GpuPacking:
         t = Timing()
         _check(_lib.rp_get_timing(self._ctx, ctypes.byref(t)))
         return t.as_dict()
+
+    def fix_orientations(self) -> int:
+        """Packing::renormalizeOrientations: returns the number of particles whose fix was skipped (overlap)."""
+        v = ctypes.c_uint64()
+        _check(_lib.rp_fix_orientations(self._ctx, ctypes.byref(v)))
+        return int(v.value)
+
+    def set_orientation_fix_every(self, every: int):
+        _check(_lib.rp_set_orientation_fix_every(self._ctx, int(every)))
 
     def total_energy(self) -> float:
         """Packing::getTotalEnergy of the device-resident state (0 for hard shapes)."""

@@ -88,6 +88,9 @@ struct rp_ctx {
     double *cellEnergy = nullptr;        // per-cell soft energy partials
     int cellEnergyCap = 0;
     double *sumScratch = nullptr;        // fixed-order sum of the partials
+    unsigned long long *dFixCounters = nullptr;  // counters of the orientation renormalisation pass
+    uint64_t fixEvery = 10000;           // cycles between orientation renormalisations (0: never)
+    uint64_t fixRejected = 0;            // renormalisations skipped because they would create an overlap
     unsigned long long *dCounters = nullptr;
     unsigned long long *dPhases = nullptr;  // sweep phase timers (RAMPACK_SWEEP_PHASES=1)
     unsigned long long *hCounters = nullptr;  // pinned
@@ -797,6 +800,10 @@ int rp_run_cycles(rp_ctx *c, size_t nCycles, int adapt, rp_counters *out) {
     for (size_t i = 0; i < nCycles; i++) {
         r = runCycle(c);
         if (r) return r;
+        if (c->fixEvery && c->cycles % c->fixEvery == 0) {  // Simulation.cpp:144-145
+            r = rp_fix_orientations(c, nullptr);
+            if (r) return r;
+        }
         if (adapt) {
             r = adaptSteps(c);
             if (r) return r;
@@ -979,6 +986,7 @@ void rp_destroy(rp_ctx *c) {
     if (c->obsMem) hipFree(c->obsMem);
     if (c->cellEnergy) hipFree(c->cellEnergy);
     if (c->sumScratch) hipFree(c->sumScratch);
+    if (c->dFixCounters) hipFree(c->dFixCounters);
     if (c->tmpSlot) hipFree(c->tmpSlot);
     if (c->tmpOrder) hipFree(c->tmpOrder);
     if (c->tmpPos) hipFree(c->tmpPos);
@@ -986,6 +994,76 @@ void rp_destroy(rp_ctx *c) {
     if (c->hCounters) hipHostFree(c->hCounters);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
+}
+
+namespace {
+
+// Packing::renormalizeOrientations (Packing.cpp:1178-1209) through Simulation::fixRotationMatrices
+// (Simulation.cpp:750-769): every orientation is replaced by fix_rotation(O) unless that creates an overlap; a shape
+// without a hard part takes every fix.  The particles are visited cell by cell in the checkerboard colours (the
+// reference visits them in index order; only a fix that itself creates an overlap can tell the orders apart).
+int fixOrientations(rp_ctx *c, uint64_t &rejected) {
+    rejected = 0;
+    if (c->k.soft) {
+        RP_HIP(launch_fix_all(c->stream, c->cur.p, (int)c->n));
+        RP_HIP(hipStreamSynchronize(c->stream));
+        return RP_OK;
+    }
+    int r = prepareCycleGrid(c);
+    if (r) return r;
+    if (!c->dFixCounters) RP_HIP(hipMalloc(&c->dFixCounters, sizeof(unsigned long long) * CNT_NUM));
+    RP_HIP(launch_fill_u64(c->stream, c->dFixCounters, CNT_NUM, 0));
+    const Grid &g = c->grid;
+    SweepParams sp{};
+    sp.g = g;
+    sp.p = c->cur.p;
+    sp.k = c->k;
+    for (int d = 0; d < 3; d++) sp.na[d] = std::max(1, g.n[d] / g.period);
+    sp.moves = 0;
+    sp.fixMode = 1;
+    sp.pipelined = 0;
+    sp.counters = c->dFixCounters;
+    const double lambda = (double)c->n / (double)g.ncell;
+    const int side = 2 * g.reach + 1;
+    const double expect = side * side * side * lambda;
+    const int waves = 4;
+    int cap = (int)std::ceil(expect + 8 * std::sqrt(expect) + 64);
+    cap = (cap + 63) / 64 * 64;
+    const double perEntry =
+        (double)(sweep_lds_bytes(c->k.kind, 1024, waves) - sweep_lds_bytes(c->k.kind, 0, waves)) / 1024.0;
+    const int maxCap = (int)((double)(c->ldsLimit - sweep_lds_bytes(c->k.kind, 0, waves)) / perEntry) / 64 * 64;
+    sp.cap = std::max(64, std::min(cap, maxCap));
+    const int colours = g.period * g.period * g.period;
+    for (int colour = 0; colour < colours; colour++) {
+        const int par[3] = {colour % g.period, (colour / g.period) % g.period, colour / (g.period * g.period)};
+        if ((g.n[0] == 1 && par[0]) || (g.n[1] == 1 && par[1]) || (g.n[2] == 1 && par[2])) continue;
+        sp.colour = colour;
+        RP_HIP(launch_sweep(c->stream, sp, waves));
+    }
+    unsigned long long h[CNT_NUM];
+    RP_HIP(hipMemcpyAsync(h, c->dFixCounters, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    RP_HIP(hipStreamSynchronize(c->stream));
+    rejected = h[CNT_ATTEMPTED] - h[CNT_ACCEPTED];
+    c->fixRejected += rejected;
+    c->sortedForCycle = false;  // the next cycle draws its own grid origin
+    return RP_OK;
+}
+
+}  // namespace
+
+int rp_fix_orientations(rp_ctx *c, uint64_t *rejected) {
+    if (!c) return fail(RP_ERR_PRECONDITION, "null context");
+    uint64_t rej = 0;
+    int r = fixOrientations(c, rej);
+    if (r) return r;
+    if (rejected) *rejected = rej;
+    return RP_OK;
+}
+
+int rp_set_orientation_fix_every(rp_ctx *c, uint64_t every) {
+    if (!c) return fail(RP_ERR_PRECONDITION, "null context");
+    c->fixEvery = every;
+    return RP_OK;
 }
 
 int rp_total_energy(rp_ctx *c, double *out) {

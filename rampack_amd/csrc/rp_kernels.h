@@ -57,6 +57,8 @@ struct SweepParams {
     int pipelined;          // 1: wavefront 0 plans the next move while the others test; 2: grouped moves
                             // (both need >= 2 wavefronts)
     double temperature;     // soft interactions: the move is accepted if u <= exp(-dE / temperature)
+    int fixMode;            // orientation renormalisation pass: one trial per own particle in index order, no
+                            // translation, orientation fix_rotation(O) instead of R O (Packing::renormalizeOrientations)
     unsigned long long *counters;
     unsigned long long *phases;  // optional per-phase wall-clock sums (RP_PHASE_*), nullptr when not profiling
 };
@@ -115,6 +117,9 @@ size_t fixed_sum_scratch_doubles();
 // scratch[6 * 256 .. 6 * 256 + 6); scratch holds nematic_scratch_doubles() doubles.
 size_t nematic_scratch_doubles();
 hipError_t launch_nematic_sums(hipStream_t s, const Particles &p, int n, double *scratch);
+
+// Applies fix_rotation to every orientation (no overlap test: soft shapes have no hard part).
+hipError_t launch_fix_all(hipStream_t s, const Particles &p, int n);
 
 // Workgroups of the sweep kernel one compute unit can hold at this LDS size (occupancy API).
 int sweep_occupancy(int kind, int cap, int waves);

@@ -777,14 +777,21 @@ __device__ __forceinline__ void sweep_cell_staged(const SweepParams &P, const St
 
     for (int chunk = 0; chunk < moves; chunk += MOVE_CHUNK) {
         const int cn = min(MOVE_CHUNK, moves - chunk);
-        load_proposals(P, pbase, chunk, cn, pt, pR, pk);
+        if (P.fixMode) {
+            if (tid < cn) pk[tid] = chunk + tid;
+        } else {
+            load_proposals(P, pbase, chunk, cn, pt, pR, pk);
+        }
         __syncthreads();
 
         for (int mi = 0; mi < cn; mi++, move++) {
             const int kIdx = pk[mi];
             V3 pos = v3(S.x[kIdx], S.y[kIdx], S.z[kIdx]);
-            V3 t = v3(pt[mi], pt[MOVE_CHUNK + mi], pt[2 * MOVE_CHUNK + mi]);
-            V3 npos = translate_wrapped(g.dims, g.inv, pos, t);
+            V3 npos = pos;
+            if (!P.fixMode) {
+                V3 t = v3(pt[mi], pt[MOVE_CHUNK + mi], pt[2 * MOVE_CHUNK + mi]);
+                npos = translate_wrapped(g.dims, g.inv, pos, t);
+            }
             int nc[3];
             unsigned nw;
             bin_position(g, npos, nc, nw);
@@ -794,9 +801,14 @@ __device__ __forceinline__ void sweep_cell_staged(const SweepParams &P, const St
                 double O[9], R[9];
                 for (int c = 0; c < 9; c++) {
                     O[c] = oo[c * OWN_CAP + kIdx];
-                    R[c] = pR[c * MOVE_CHUNK + mi];
+                    R[c] = P.fixMode ? 0.0 : pR[c * MOVE_CHUNK + mi];
                 }
-                mulm(R, O, NO);  // Shape::rotate (Shape.cpp:14-16)
+                if (P.fixMode) {
+                    for (int c = 0; c < 9; c++) NO[c] = O[c];
+                    fix_rotation(NO);  // Packing::tryOrientationFix (Packing.cpp:1218-1240)
+                } else {
+                    mulm(R, O, NO);  // Shape::rotate (Shape.cpp:14-16)
+                }
             }
             const V3 nax = v3(NO[2], NO[5], NO[8]);
             const int wx = nw & 1, wy = (nw >> 1) & 1, wz = (nw >> 2) & 1;
@@ -1071,14 +1083,21 @@ __device__ __forceinline__ void sweep_cell_unstaged(const SweepParams &P, const 
     const int pbase = P.moves > 0 ? cell * P.moves : own0;
     for (int chunk = 0; chunk < moves; chunk += MOVE_CHUNK) {
         const int cn = min(MOVE_CHUNK, moves - chunk);
-        load_proposals(P, pbase, chunk, cn, pt, pR, pk);
+        if (P.fixMode) {
+            if (tid < cn) pk[tid] = chunk + tid;
+        } else {
+            load_proposals(P, pbase, chunk, cn, pt, pR, pk);
+        }
         __syncthreads();
         for (int mi = 0; mi < cn; mi++) {
             const int kIdx = pk[mi];
             const int gi = own0 + kIdx;
             V3 pos = load3(P.p.x, P.p.y, P.p.z, gi);
-            V3 t = v3(pt[mi], pt[MOVE_CHUNK + mi], pt[2 * MOVE_CHUNK + mi]);
-            V3 npos = translate_wrapped(g.dims, g.inv, pos, t);
+            V3 npos = pos;
+            if (!P.fixMode) {
+                V3 t = v3(pt[mi], pt[MOVE_CHUNK + mi], pt[2 * MOVE_CHUNK + mi]);
+                npos = translate_wrapped(g.dims, g.inv, pos, t);
+            }
             int nc[3];
             unsigned nw;
             bin_position(g, npos, nc, nw);
@@ -1089,9 +1108,14 @@ __device__ __forceinline__ void sweep_cell_unstaged(const SweepParams &P, const 
             double O[9], R[9], NO[9];
             for (int c = 0; c < 9; c++) {
                 O[c] = P.p.o[c][gi];
-                R[c] = pR[c * MOVE_CHUNK + mi];
+                R[c] = P.fixMode ? 0.0 : pR[c * MOVE_CHUNK + mi];
             }
-            mulm(R, O, NO);
+            if (P.fixMode) {
+                for (int c = 0; c < 9; c++) NO[c] = O[c];
+                fix_rotation(NO);
+            } else {
+                mulm(R, O, NO);
+            }
             V3 nax = v3(NO[2], NO[5], NO[8]);
             SCFirst f = make_first<KIND>(P.k, npos, nax);
             const int wx = nw & 1, wy = (nw >> 1) & 1, wz = (nw >> 2) & 1;
@@ -1682,6 +1706,11 @@ __global__ __launch_bounds__(WAVE *WAVES) void k_sweep(SweepParams P) {
             else
                 sweep_cell_soft<KIND, WAVES, false, true>(P, S, oo, pt, pR, pk, plans, sT, cell, own0, nOwn, total);
             if (c3) c3[3] = wall_clock64();
+        } else if (P.fixMode) {
+            if (translated)
+                sweep_cell_staged<KIND, WAVES, true>(P, S, oo, pt, pR, pk, queue, sT, s_vote, cell, own0, nOwn, total, c3);
+            else
+                sweep_cell_staged<KIND, WAVES, false>(P, S, oo, pt, pR, pk, queue, sT, s_vote, cell, own0, nOwn, total, c3);
         } else if (WAVES >= 2 && P.pipelined == 2) {
             int *hitWords = plansI + GBUF * GPLAN_I;
             if (translated)
@@ -1771,6 +1800,21 @@ hipError_t launch_nematic_sums(hipStream_t s, const Particles &p, int n, double 
     static_assert(NEMATIC_BLOCKS == NEMATIC_THREADS, "the final pass reads one partial per thread");
     hipLaunchKernelGGL(k_nematic_partial, dim3(NEMATIC_BLOCKS), dim3(NEMATIC_THREADS), 0, s, p, n, scratch);
     hipLaunchKernelGGL(k_nematic_final, dim3(1), dim3(NEMATIC_THREADS), 0, s, scratch, scratch + 6 * NEMATIC_BLOCKS);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_fix_all(Particles p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double R[9];
+    for (int c = 0; c < 9; c++) R[c] = p.o[c][i];
+    fix_rotation(R);
+    for (int c = 0; c < 9; c++) p.o[c][i] = R[c];
+}
+
+hipError_t launch_fix_all(hipStream_t s, const Particles &p, int n) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fix_all, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n);
     return hipGetLastError();
 }
 

@@ -59,6 +59,23 @@ RP_HD void mulm(const double *a, const double *b, double *r) {
         }
 }
 
+// Packing::fixRotationMatrix (Packing.cpp:1211-1216): three iterations of R <- 1.5 R - 0.5 R R^T R, evaluated as
+// (1.5 R) - (((0.5 R) R^T) R) with Matrix.tpp's elementwise scalar products and row-by-column sums.
+RP_HD void fix_rotation(double *R) {
+    for (int it = 0; it < 3; it++) {
+        double A[9], B[9], Rt[9], C[9], D[9];
+        for (int i = 0; i < 9; i++) {
+            A[i] = R[i] * 1.5;
+            B[i] = R[i] * 0.5;
+        }
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) Rt[3 * i + j] = R[3 * j + i];
+        mulm(B, Rt, C);
+        mulm(C, R, D);
+        for (int i = 0; i < 9; i++) R[i] = A[i] - D[i];
+    }
+}
+
 // Matrix.tpp:352-359
 RP_HD double det3(const double *a) {
     return a[0] * a[4] * a[8] + a[1] * a[5] * a[6] + a[2] * a[3] * a[7]

@@ -31,6 +31,8 @@ def _setup_common(lib):
     lib.orc_count_brute.restype = ctypes.c_uint64
     lib.orc_pair_energies.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double,
                                       ctypes.c_double, ctypes.c_size_t, _dp, _dp]
+    lib.orc_fix_rotation.argtypes = [ctypes.c_size_t, _dp]
+    lib.orc_fix_rotation.restype = None
     lib.orc_segment_distance2.argtypes = [ctypes.c_size_t, _dp, _dp]
     lib.orc_segment_distance2.restype = None
     lib.orc_box_info.argtypes = [_dp, _dp, _dp, _dp]
@@ -117,6 +119,8 @@ def _setup_ref(lib):
                                        ctypes.c_int, _u64p, _dp]
     lib.ref_pair_energies.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double,
                                       ctypes.c_double, ctypes.c_size_t, _dp, _dp]
+    lib.ref_fix_rotation.argtypes = [ctypes.c_size_t, _dp]
+    lib.ref_fix_rotation.restype = ctypes.c_int
     lib.ref_nematic_p2.argtypes = [ctypes.c_size_t, _dp]
     lib.ref_nematic_p2.restype = ctypes.c_double
     lib.ref_packing_fraction.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, ctypes.c_size_t]

@@ -269,3 +269,22 @@ def test_degenerate_spherocylinder_gas_reference_value():
     assert c["box_too_small"] == 0
     assert abs(m - 0.09454087) <= 3 * block_err + 0.001 * 0.09454087
     assert block_err / m < 0.01
+
+
+@pytest.mark.parametrize("shape_name", ["hard", "wca"])
+def test_fix_orientations_matches_oracle(shape_name):
+    """Packing::renormalizeOrientations: orientations perturbed by 1e-7 are replaced by the oracle's fix_rotation
+    bit for bit; in a packing with room to spare no fix is rejected.  A WCA shape has no hard part: every fix
+    applies."""
+    box, pos, rot = lattice.spherocylinder_lattice(n_side=(10, 10, 3), spacing=(1.6, 1.6, 6.4))
+    rng = np.random.Generator(np.random.PCG64(12))
+    rot = np.ascontiguousarray(rot + (rng.random(rot.shape) - 0.5) * 2e-7)
+    shape = SC if shape_name == "hard" else ra.Shape("spherocylinder", 0.5, 5.0, soft="wca_segment")
+    pk = ra.GpuPacking(shape, box, pos, rot)
+    assert pk.fix_orientations() == 0
+    _, p, r = pk.download()
+    expected = np.ascontiguousarray(rot.copy())
+    oracle_lib.oracle().lib.orc_fix_rotation(len(expected), oracle_lib._d(expected))
+    assert np.array_equal(p, pos)
+    assert np.array_equal(r, expected)
+    pk.close()

@@ -226,3 +226,27 @@ def test_reference_packing_fraction_closed_form():
     phi = oracle_lib.ref().lib.ref_packing_fraction(1, 0.5, 5.0, oracle_lib._d(box.ravel()), 1000)
     v0 = math.pi * 0.25 * 5.0 + 4 / 3 * math.pi * 0.125
     assert abs(phi - 1000 * v0 / abs(np.linalg.det(box))) < 1e-15
+
+
+def test_fix_rotation_matches_reference_and_orthonormalises():
+    """Packing::fixRotationMatrix (Packing.cpp:1211-1216): the oracle equals the reference build bit for bit on
+    perturbed rotations; the result is orthonormal to rounding, and an exact rotation matrix with entries 0 and +-1
+    is a fixed point."""
+    import numpy as np
+    import corpus
+    import oracle_lib
+    rng = np.random.Generator(np.random.PCG64(3))
+    rot = corpus.random_rotations(rng, 2000) + (rng.random((2000, 9)) - 0.5) * 2e-6
+    mine = np.ascontiguousarray(rot.copy())
+    oracle_lib.oracle().lib.orc_fix_rotation(len(mine), oracle_lib._d(mine))
+    if oracle_lib.ref_available():
+        theirs = np.ascontiguousarray(rot.copy())
+        assert oracle_lib.ref().lib.ref_fix_rotation(len(theirs), oracle_lib._d(theirs)) == 0
+        assert np.array_equal(mine, theirs)
+    m = mine.reshape(-1, 3, 3)
+    dev = np.abs(np.einsum("nij,nkj->nik", m, m) - np.eye(3)).max()
+    assert dev < 1e-14
+    perm = np.ascontiguousarray(np.array([[0, 1, 0, 0, 0, -1, -1, 0, 0]], dtype=float))
+    before = perm.copy()
+    oracle_lib.oracle().lib.orc_fix_rotation(1, oracle_lib._d(perm))
+    assert np.array_equal(perm, before)
