@@ -155,6 +155,18 @@ int rp_total_energy(rp_ctx *ctx, double *out);
 int rp_fix_orientations(rp_ctx *ctx, uint64_t *rejected);
 int rp_set_orientation_fix_every(rp_ctx *ctx, uint64_t every);
 
+/* RAMSNAP snapshots, the reference's text snapshot format (RamsnapWriter::write, src/core/io/RamsnapWriter.cpp:8-48;
+ * RamsnapReader::read, src/core/io/RamsnapReader.cpp:15-92).  aux_text holds "key value" lines (the key is the
+ * first whitespace-free token, the value the rest of the line); entries are written sorted by key.  A broken file
+ * is RP_ERR_VALIDATION (RamsnapException is a ValidationException). */
+int rp_ramsnap_write(const char *path, const double box[9], size_t n, const double *pos, const double *rot,
+                     const char *aux_text);
+int rp_ramsnap_save(rp_ctx *ctx, const char *path, const char *aux_text);  /* the device-resident state */
+/* Number of particles and bytes of aux text (with the terminating NUL) of a RAMSNAP file. */
+int rp_ramsnap_info(const char *path, size_t *n, size_t *aux_bytes);
+int rp_ramsnap_read(const char *path, double box[9], size_t n, double *pos, double *rot, char *aux_text,
+                    size_t aux_cap);
+
 /* Current state, in the caller's particle order (Packing::operator[], Packing::getBox). */
 int rp_download(rp_ctx *ctx, double box[9], double *pos, double *rot);
 

@@ -15,6 +15,7 @@
 // std::mt19937 streams follows Simulation.cpp:401-404, 456-458, 483, 499-525.
 
 #include <cmath>
+#include <cstdio>
 #include <cstdint>
 #include <cstring>
 #include <chrono>
@@ -38,6 +39,10 @@
 #include "core/volume_scalers/LogScalingFactorSampler.h"
 #include "core/volume_scalers/DeltaVolumeScaler.h"
 #include "core/observables/NematicOrder.h"
+#include "core/io/RamsnapReader.h"
+#include "core/io/RamsnapWriter.h"
+#include <fstream>
+#include <sstream>
 
 namespace {
 
@@ -184,6 +189,56 @@ int ref_fix_rotation(std::size_t n, double *rot) {
                 for (int b = 0; b < 3; b++) rot[9 * i + 3 * a + b] = O(a, b);
         }
         return 0;
+    } catch (const std::exception &) {
+        return -1;
+    }
+}
+
+// RamsnapWriter::write of a packing of spheres with the given state; aux_text as "key value" lines.
+int ref_ramsnap_write(const char *path, const double *box, std::size_t n, const double *pos, const double *rot,
+                      const char *aux_text) {
+    try {
+        SphereTraits traits(0.5);
+        Packing packing(TriclinicBox(mat(box)), makeShapes(n, pos, rot), std::make_unique<PeriodicBoundaryConditions>(),
+                        traits.getInteraction(), 1, 1);
+        std::map<std::string, std::string> aux;
+        std::istringstream in(aux_text ? aux_text : "");
+        std::string line;
+        while (std::getline(in, line)) {
+            std::istringstream ls(line);
+            std::string key, value;
+            if (!(ls >> key)) continue;
+            ls >> std::ws;
+            std::getline(ls, value);
+            aux[key] = value;
+        }
+        std::ofstream f(path);
+        RamsnapWriter().write(f, packing, aux);
+        return f ? 0 : -1;
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "ref_ramsnap_write: %s\n", e.what());
+        return -1;
+    }
+}
+
+// RamsnapReader::read into n-particle buffers; returns the particle count, or -1 on a broken file.
+long ref_ramsnap_read(const char *path, double *box, std::size_t n, double *pos, double *rot) {
+    try {
+        SphereTraits traits(0.5);
+        Packing packing(std::make_unique<PeriodicBoundaryConditions>());
+        std::ifstream f(path);
+        RamsnapReader().read(f, packing, traits.getInteraction());
+        const auto &d = packing.getBox().getDimensions();
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) box[3 * a + b] = d(a, b);
+        for (std::size_t i = 0; i < packing.size() && i < n; i++) {
+            const auto &p = packing[i].getPosition();
+            const auto &O = packing[i].getOrientation();
+            for (int a = 0; a < 3; a++) pos[3 * i + a] = p[a];
+            for (int a = 0; a < 3; a++)
+                for (int b = 0; b < 3; b++) rot[9 * i + 3 * a + b] = O(a, b);
+        }
+        return (long)packing.size();
     } catch (const std::exception &) {
         return -1;
     }

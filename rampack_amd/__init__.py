@@ -41,6 +41,7 @@ EXPORTED_SYMBOLS = (
     "rp_create", "rp_set_environment", "rp_run_cycles", "rp_count_total_overlaps", "rp_download",
     "rp_set_timing", "rp_get_timing", "rp_destroy", "rp_pair_decisions", "rp_last_error", "rp_get_observables",
     "rp_total_energy", "rp_pair_energies", "rp_fix_orientations", "rp_set_orientation_fix_every",
+    "rp_ramsnap_write", "rp_ramsnap_save", "rp_ramsnap_info", "rp_ramsnap_read",
 )
 
 
@@ -134,6 +135,10 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
                                       ctypes.POINTER(ctypes.c_uint8)]
     lib.rp_get_observables.argtypes = [vp, ctypes.POINTER(Observables)]
     lib.rp_total_energy.argtypes = [vp, dp]
+    lib.rp_ramsnap_write.argtypes = [ctypes.c_char_p, dp, ctypes.c_size_t, dp, dp, ctypes.c_char_p]
+    lib.rp_ramsnap_save.argtypes = [vp, ctypes.c_char_p, ctypes.c_char_p]
+    lib.rp_ramsnap_info.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]
+    lib.rp_ramsnap_read.argtypes = [ctypes.c_char_p, dp, ctypes.c_size_t, dp, dp, ctypes.c_char_p, ctypes.c_size_t]
     lib.rp_fix_orientations.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64)]
     lib.rp_set_orientation_fix_every.argtypes = [vp, ctypes.c_uint64]
     lib.rp_pair_energies.argtypes = [ctypes.POINTER(ShapeDesc), ctypes.c_size_t, dp, dp]
@@ -293,6 +298,10 @@ class GpuPacking:
     def set_orientation_fix_every(self, every: int):
         _check(_lib.rp_set_orientation_fix_every(self._ctx, int(every)))
 
+    def save_ramsnap(self, path: str, aux: Optional[dict] = None):
+        """RamsnapWriter::write of the device-resident state."""
+        _check(_lib.rp_ramsnap_save(self._ctx, os.fsencode(path), _aux_text(aux)))
+
     def total_energy(self) -> float:
         """Packing::getTotalEnergy of the device-resident state (0 for hard shapes)."""
         v = ctypes.c_double()
@@ -307,6 +316,45 @@ class GpuPacking:
 
     def packing_fraction(self) -> float:
         return self.observables()["packing_fraction"]
+
+
+def _aux_text(aux) -> bytes:
+    lines = []
+    for k, v in (aux or {}).items():
+        k = str(k)
+        if not k or any(ch.isspace() for ch in k):
+            raise ValueError(f"RAMSNAP aux key {k!r} must be non-empty and free of whitespace")
+        lines.append(f"{k} {v}\n")
+    return "".join(lines).encode()
+
+
+def ramsnap_write(path: str, box: np.ndarray, pos: np.ndarray, rot: np.ndarray, aux: Optional[dict] = None):
+    """RamsnapWriter::write (RamsnapWriter.cpp:8-48): a RAMSNAP snapshot of the given state."""
+    lib = load_library()
+    box = np.ascontiguousarray(box, dtype=np.float64).reshape(9)
+    pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+    rot = np.ascontiguousarray(rot, dtype=np.float64).reshape(-1, 9)
+    if len(pos) != len(rot):
+        raise ValueError("positions and orientations differ in length")
+    _check(lib.rp_ramsnap_write(os.fsencode(path), _dptr(box), len(pos), _dptr(pos), _dptr(rot), _aux_text(aux)))
+
+
+def ramsnap_read(path: str):
+    """RamsnapReader::read (RamsnapReader.cpp:15-92): (box 3x3, positions n x 3, orientations n x 9, aux dict)."""
+    lib = load_library()
+    n = ctypes.c_size_t()
+    nbytes = ctypes.c_size_t()
+    _check(lib.rp_ramsnap_info(os.fsencode(path), ctypes.byref(n), ctypes.byref(nbytes)))
+    box = np.zeros(9)
+    pos = np.zeros((n.value, 3))
+    rot = np.zeros((n.value, 9))
+    buf = ctypes.create_string_buffer(nbytes.value)
+    _check(lib.rp_ramsnap_read(os.fsencode(path), _dptr(box), n.value, _dptr(pos), _dptr(rot), buf, nbytes.value))
+    aux = {}
+    for line in buf.value.decode().splitlines():
+        key, _, value = line.partition(" ")
+        aux[key] = value
+    return box.reshape(3, 3), pos, rot, aux
 
 
 def mean_and_error(samples) -> tuple:

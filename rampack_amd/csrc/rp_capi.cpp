@@ -651,7 +651,16 @@ int countExact(rp_ctx *c, int earlyExit, uint64_t &out) {
 
 extern "C" {
 
+int rp_ramsnap_save(rp_ctx *c, const char *path, const char *aux_text) {
+    if (!c || !path) return fail(RP_ERR_PRECONDITION, "null argument");
+    std::vector<double> box(9), pos(3 * c->n), rot(9 * c->n);
+    int r = rp_download(c, box.data(), pos.data(), rot.data());
+    if (r) return r;
+    return rp_ramsnap_write(path, box.data(), c->n, pos.data(), rot.data(), aux_text);
+}
+
 const char *rp_last_error(void) { return g_lastError.c_str(); }
+
 
 int rp_create(rp_ctx **out, const rp_shape_desc *shape, const double box[9], size_t n, const double *pos,
               const double *rot, uint64_t seed, int precision) {
@@ -1136,3 +1145,8 @@ int rp_pair_decisions(const rp_shape_desc *shape, size_t n, const double *record
 }
 
 }  // extern "C"
+
+// Error reporting for the library's other host sources (C++ linkage).
+namespace rp {
+int set_error(int code, const std::string &msg) { return fail(code, msg); }
+}  // namespace rp

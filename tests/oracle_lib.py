@@ -121,6 +121,9 @@ def _setup_ref(lib):
                                       ctypes.c_double, ctypes.c_size_t, _dp, _dp]
     lib.ref_fix_rotation.argtypes = [ctypes.c_size_t, _dp]
     lib.ref_fix_rotation.restype = ctypes.c_int
+    lib.ref_ramsnap_write.argtypes = [ctypes.c_char_p, _dp, ctypes.c_size_t, _dp, _dp, ctypes.c_char_p]
+    lib.ref_ramsnap_read.argtypes = [ctypes.c_char_p, _dp, ctypes.c_size_t, _dp, _dp]
+    lib.ref_ramsnap_read.restype = ctypes.c_long
     lib.ref_nematic_p2.argtypes = [ctypes.c_size_t, _dp]
     lib.ref_nematic_p2.restype = ctypes.c_double
     lib.ref_packing_fraction.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, ctypes.c_size_t]

@@ -99,3 +99,17 @@ def test_observables_follow_the_state_during_a_run():
     assert abs(o["nematic_p2"] - w[np.argmax(np.abs(w))]) < 1e-10
     assert 0 < o["nematic_p2"] < 1
     pk.close()
+
+
+def test_ramsnap_save_of_the_device_state_reads_back_exactly(tmp_path):
+    box, pos, rot = lattice.spherocylinder_lattice(n_side=(8, 8, 3), spacing=(1.6, 1.6, 6.4))
+    pk = ra.GpuPacking(SC, box, pos, rot, seed=2)
+    pk.set_environment(1.0, 2.0, 0.2, None, box_move=("delta_triclinic", 0.002))
+    pk.run_cycles(20)
+    path = tmp_path / "state.ramsnap"
+    pk.save_ramsnap(str(path), {"cycles": 20})
+    b, p, r = pk.download()
+    b2, p2, r2, aux = ra.ramsnap_read(str(path))
+    assert np.array_equal(b, b2) and np.array_equal(p, p2) and np.array_equal(r, r2)
+    assert aux == {"cycles": "20"}
+    pk.close()
