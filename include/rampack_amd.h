@@ -155,6 +155,18 @@ int rp_total_energy(rp_ctx *ctx, double *out);
 int rp_fix_orientations(rp_ctx *ctx, uint64_t *rejected);
 int rp_set_orientation_fix_every(rp_ctx *ctx, uint64_t every);
 
+/* Overlap-counting mode (Packing::toggleOverlapCounting, src/core/Packing.cpp:1049-1060; moves as
+ * Packing::calculateMoveOverlapEnergy, Packing.cpp:304-323; box moves as Packing::tryScaling's counting branch,
+ * Packing.cpp:190-198): a particle move is accepted if it does not raise the particle's overlap count, and the
+ * overlap count is kept up to date; rp_run_cycles does not reject an overlapping packing while it is on.
+ * rp_cached_overlaps reads the count (an error when counting is off, as the reference throws). */
+int rp_set_overlap_counting(rp_ctx *ctx, int enable);
+int rp_cached_overlaps(rp_ctx *ctx, uint64_t *out);
+
+/* Simulation::relaxOverlaps (src/core/Simulation.cpp:222-284): cycles with step adaptation in overlap-counting
+ * mode until no overlap is left or max_cycles have run; counting is switched off afterwards. */
+int rp_relax_overlaps(rp_ctx *ctx, size_t max_cycles, size_t *cycles_run, uint64_t *overlaps_left);
+
 /* RAMSNAP snapshots, the reference's text snapshot format (RamsnapWriter::write, src/core/io/RamsnapWriter.cpp:8-48;
  * RamsnapReader::read, src/core/io/RamsnapReader.cpp:15-92).  aux_text holds "key value" lines (the key is the
  * first whitespace-free token, the value the rest of the line); entries are written sorted by key.  A broken file

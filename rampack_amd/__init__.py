@@ -42,6 +42,7 @@ EXPORTED_SYMBOLS = (
     "rp_set_timing", "rp_get_timing", "rp_destroy", "rp_pair_decisions", "rp_last_error", "rp_get_observables",
     "rp_total_energy", "rp_pair_energies", "rp_fix_orientations", "rp_set_orientation_fix_every",
     "rp_ramsnap_write", "rp_ramsnap_save", "rp_ramsnap_info", "rp_ramsnap_read",
+    "rp_set_overlap_counting", "rp_cached_overlaps", "rp_relax_overlaps",
 )
 
 
@@ -135,6 +136,10 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
                                       ctypes.POINTER(ctypes.c_uint8)]
     lib.rp_get_observables.argtypes = [vp, ctypes.POINTER(Observables)]
     lib.rp_total_energy.argtypes = [vp, dp]
+    lib.rp_set_overlap_counting.argtypes = [vp, ctypes.c_int]
+    lib.rp_cached_overlaps.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64)]
+    lib.rp_relax_overlaps.argtypes = [vp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t),
+                                      ctypes.POINTER(ctypes.c_uint64)]
     lib.rp_ramsnap_write.argtypes = [ctypes.c_char_p, dp, ctypes.c_size_t, dp, dp, ctypes.c_char_p]
     lib.rp_ramsnap_save.argtypes = [vp, ctypes.c_char_p, ctypes.c_char_p]
     lib.rp_ramsnap_info.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]
@@ -297,6 +302,21 @@ class GpuPacking:
 
     def set_orientation_fix_every(self, every: int):
         _check(_lib.rp_set_orientation_fix_every(self._ctx, int(every)))
+
+    def set_overlap_counting(self, enable: bool):
+        _check(_lib.rp_set_overlap_counting(self._ctx, 1 if enable else 0))
+
+    def cached_overlaps(self) -> int:
+        v = ctypes.c_uint64()
+        _check(_lib.rp_cached_overlaps(self._ctx, ctypes.byref(v)))
+        return int(v.value)
+
+    def relax_overlaps(self, max_cycles: int = 1 << 62):
+        """Simulation::relaxOverlaps: returns (cycles run, overlaps left)."""
+        done = ctypes.c_size_t()
+        left = ctypes.c_uint64()
+        _check(_lib.rp_relax_overlaps(self._ctx, int(max_cycles), ctypes.byref(done), ctypes.byref(left)))
+        return int(done.value), int(left.value)
 
     def save_ramsnap(self, path: str, aux: Optional[dict] = None):
         """RamsnapWriter::write of the device-resident state."""

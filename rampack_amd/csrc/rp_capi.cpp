@@ -91,6 +91,8 @@ struct rp_ctx {
     unsigned long long *dFixCounters = nullptr;  // counters of the orientation renormalisation pass
     uint64_t fixEvery = 10000;           // cycles between orientation renormalisations (0: never)
     uint64_t fixRejected = 0;            // renormalisations skipped because they would create an overlap
+    bool counting = false;               // overlap-counting mode (Packing::toggleOverlapCounting)
+    long long numOverlaps = 0;           // cached overlap count in counting mode (Packing::getCachedNumberOfOverlaps)
     unsigned long long *dCounters = nullptr;
     unsigned long long *dPhases = nullptr;  // sweep phase timers (RAMPACK_SWEEP_PHASES=1)
     unsigned long long *hCounters = nullptr;  // pinned
@@ -392,6 +394,35 @@ double minHeight(const HostBox &b) {
 // Packing::getTotalEnergy (Packing.cpp:690-709) of particles sorted by grid g.
 int totalEnergy(rp_ctx *c, const Grid &g, const Particles &p, double &out);
 
+// Simulation::tryScaling in overlap-counting mode (Simulation.cpp:508-517 with Packing::tryScaling's counting branch,
+// Packing.cpp:190-198): the rescaled packing is always counted in full; fewer overlaps accept the move, more reject
+// it, the same number leaves the ordinary Metropolis test.
+int countingBoxMove(rp_ctx *c, const HostBox &nb, double exponent, double u, const double shiftUnit[3],
+                    bool &accepted) {
+    Grid g{};
+    int r = chooseSweepGrid(c, nb, shiftUnit, g);
+    if (r == RP_ERR_UNSUPPORTED) {
+        c->boxTooSmall++;
+        return RP_OK;
+    }
+    if (r) return r;
+    r = rebuild(c, g, 1, c->box.inv);
+    if (r) return r;
+    uint64_t after = 0;
+    c->boxScans++;
+    r = scanOverlaps(c, g, 0, after);
+    if (r) return r;
+    const long long delta = (long long)after - c->numOverlaps;
+    if (delta > 0 || (delta == 0 && !(u <= std::exp(exponent)))) return RP_OK;
+    c->box = nb;
+    swapBuffers(c);
+    c->grid = g;
+    c->sortedForCycle = true;
+    c->numOverlaps = (long long)after;
+    accepted = true;
+    return RP_OK;
+}
+
 // Simulation::tryScaling with a soft interaction (Simulation.cpp:508-517): the scaling is always tried, and
 // accepted if u <= exp(N ln(V'/V) - dE/T - p dV/T) with dE = getTotalEnergy(new) - getTotalEnergy(old)
 // (Packing.cpp:172-208).  The old energy is summed on the cycle's grid, which the colour passes keep valid.
@@ -475,6 +506,7 @@ int boxMove(rp_ctx *c, bool &accepted) {
     double u = philox_unit(c->hostRng);
     double shiftUnit[3] = {philox_unit(c->hostRng), philox_unit(c->hostRng), philox_unit(c->hostRng)};
     if (c->k.soft) return softBoxMove(c, nb, exponent, u, shiftUnit, accepted);
+    if (c->counting) return countingBoxMove(c, nb, exponent, u, shiftUnit, accepted);
     if (u > std::exp(exponent)) return RP_OK;
 
     Grid g{};
@@ -560,6 +592,7 @@ int runCycle(rp_ctx *c) {
     sp.phases = c->dPhases;
     sp.pipelined = c->pipelined;
     sp.temperature = c->temperature;
+    sp.counting = c->counting && !c->k.soft ? 1 : 0;
     const int colours = g.period * g.period * g.period;
     int order[27];
     for (int i = 0; i < colours; i++) order[i] = i;
@@ -581,6 +614,14 @@ int runCycle(rp_ctx *c) {
         timingEnd(c);
     }
     c->sortedForCycle = false;
+    if (sp.counting) {
+        // Packing::acceptMove adds each accepted move's overlap change to the cached count (Packing.cpp:298-301)
+        unsigned long long d = 0;
+        RP_HIP(hipMemcpyAsync(&d, c->dCounters + CNT_OVERLAP_DELTA, sizeof d, hipMemcpyDeviceToHost, c->stream));
+        RP_HIP(launch_fill_u64(c->stream, c->dCounters + CNT_OVERLAP_DELTA, 1, 0));
+        RP_HIP(hipStreamSynchronize(c->stream));
+        c->numOverlaps += (long long)d;
+    }
     if (c->boxKind != RP_BOX_NONE) {
         bool acc = false;
         r = boxMove(c, acc);
@@ -789,8 +830,9 @@ int rp_run_cycles(rp_ctx *c, size_t nCycles, int adapt, rp_counters *out) {
     if (!c->envSet) return fail(RP_ERR_PRECONDITION, "environment not set");
     uint64_t overl = 0;
     int r = RP_OK;
-    // a WCA interaction has no hard part: there is nothing to validate (Simulation.cpp:127-132)
-    if (!c->k.soft) r = countExact(c, 1, overl);
+    // a WCA interaction has no hard part, and counting mode runs on overlapping packings: nothing to validate
+    // (Simulation.cpp:127-132)
+    if (!c->k.soft && !c->counting) r = countExact(c, 1, overl);
     if (r == RP_ERR_UNSUPPORTED) {
         // the reference's grid is off for this box; check on the checkerboard grid instead
         double zero[3] = {0, 0, 0};
@@ -809,7 +851,10 @@ int rp_run_cycles(rp_ctx *c, size_t nCycles, int adapt, rp_counters *out) {
     for (size_t i = 0; i < nCycles; i++) {
         r = runCycle(c);
         if (r) return r;
-        if (c->fixEvery && c->cycles % c->fixEvery == 0) {  // Simulation.cpp:144-145
+        // Simulation.cpp:144-145; in counting mode the reference's renormalisation keeps the orientations
+        // (Packing::renormalizeOrientations with allowOverlaps resets the packing to its unfixed copy,
+        // Packing.cpp:1179-1190), so none is run there
+        if (c->fixEvery && !c->counting && c->cycles % c->fixEvery == 0) {
             r = rp_fix_orientations(c, nullptr);
             if (r) return r;
         }
@@ -1066,6 +1111,56 @@ int rp_fix_orientations(rp_ctx *c, uint64_t *rejected) {
     int r = fixOrientations(c, rej);
     if (r) return r;
     if (rejected) *rejected = rej;
+    return RP_OK;
+}
+
+int rp_set_overlap_counting(rp_ctx *c, int enable) {
+    if (!c) return fail(RP_ERR_PRECONDITION, "null context");
+    if (!enable || c->k.soft) {
+        c->counting = false;
+        c->numOverlaps = 0;
+        return RP_OK;
+    }
+    uint64_t full = 0;
+    int r = countExact(c, 0, full);
+    if (r == RP_ERR_UNSUPPORTED) {
+        double zero[3] = {0, 0, 0};
+        Grid g{};
+        c->sortedForCycle = false;
+        r = chooseSweepGrid(c, c->box, zero, g);
+        if (r) return r;
+        r = rebuild(c, g, 0, nullptr);
+        if (r) return r;
+        r = scanOverlaps(c, g, 0, full);
+    }
+    if (r) return r;
+    RP_HIP(launch_fill_u64(c->stream, c->dCounters + CNT_OVERLAP_DELTA, 1, 0));
+    c->counting = true;
+    c->numOverlaps = (long long)full;
+    return RP_OK;
+}
+
+int rp_cached_overlaps(rp_ctx *c, uint64_t *out) {
+    if (!c || !out) return fail(RP_ERR_PRECONDITION, "null argument");
+    if (!c->counting) return fail(RP_ERR_PRECONDITION, "overlap counting is off; no count is cached (Packing.cpp:1059)");
+    *out = (uint64_t)c->numOverlaps;
+    return RP_OK;
+}
+
+int rp_relax_overlaps(rp_ctx *c, size_t max_cycles, size_t *cycles_run, uint64_t *overlaps_left) {
+    if (!c) return fail(RP_ERR_PRECONDITION, "null context");
+    if (!c->envSet) return fail(RP_ERR_PRECONDITION, "environment not set");
+    int r = rp_set_overlap_counting(c, 1);
+    if (r) return r;
+    size_t done = 0;
+    while (c->numOverlaps > 0 && done < max_cycles) {  // Simulation::relaxOverlaps (Simulation.cpp:249-256)
+        r = rp_run_cycles(c, 1, 1, nullptr);
+        if (r) return r;
+        done++;
+    }
+    if (cycles_run) *cycles_run = done;
+    if (overlaps_left) *overlaps_left = (uint64_t)c->numOverlaps;
+    c->counting = false;
     return RP_OK;
 }
 

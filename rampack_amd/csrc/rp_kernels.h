@@ -57,6 +57,8 @@ struct SweepParams {
     int pipelined;          // 1: wavefront 0 plans the next move while the others test; 2: grouped moves
                             // (both need >= 2 wavefronts)
     double temperature;     // soft interactions: the move is accepted if u <= exp(-dE / temperature)
+    int counting;           // overlap-counting mode (Packing::toggleOverlapCounting): a move is accepted if it does not
+                            // raise the moved particle's overlap count; runs in the soft sweep with 0/1 pair values
     int fixMode;            // orientation renormalisation pass: one trial per own particle in index order, no
                             // translation, orientation fix_rotation(O) instead of R O (Packing::renormalizeOrientations)
     unsigned long long *counters;
@@ -82,6 +84,7 @@ enum Counter : int {
     CNT_OVERLAPS = 4,
     CNT_MAX_STENCIL = 5,
     CNT_TESTS = 6,
+    CNT_OVERLAP_DELTA = 7,  // overlap-counting mode: sum of the accepted moves' overlap changes (two's complement)
     CNT_NUM = 8
 };
 

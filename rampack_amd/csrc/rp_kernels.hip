@@ -1452,7 +1452,9 @@ __device__ __forceinline__ void sweep_cell_grouped(const SweepParams &P, const S
 }
 
 // Soft sweep of one active cell (Packing::tryMove with a soft interaction, Packing.cpp:136-164, and the Metropolis
-// test of Simulation.cpp:483).  Every trial move sums the moved particle's energy with all stencil candidates at
+// test of Simulation.cpp:483).  In overlap-counting mode (Packing::calculateMoveOverlapEnergy, Packing.cpp:304-323)
+// the summed pair value is the overlap indicator, the move is accepted if it stays in the cell and its overlap
+// count does not grow, and the accepted changes are summed into counters[CNT_OVERLAP_DELTA].  Every trial move sums the moved particle's energy with all stencil candidates at
 // its current pose (initialEnergy) and at the trial pose (finalEnergy) in one pass; the lanes' sums are reduced per
 // wavefront (xor butterfly: every lane ends with the same bits), the wavefronts' sums in index order, and the move is
 // accepted if it stays in the cell and u <= exp(-(finalEnergy - initialEnergy) / T).  STAGED: stencil and own
@@ -1474,13 +1476,14 @@ __device__ __forceinline__ void sweep_cell_soft(const SweepParams &P, const Sten
     double *pu = aux;                 // MOVE_CHUNK uniforms
     double *sums = aux + MOVE_CHUNK;  // [move parity][final, initial][wave]
     unsigned long long accepted = 0, leftCell = 0, tests = 0;
+    long long overlapDelta = 0;
     int move = 0;
     const int moves = P.moves > 0 ? P.moves : nOwn;
     const int pbase = P.moves > 0 ? cell * P.moves : own0;
     for (int chunk = 0; chunk < moves; chunk += MOVE_CHUNK) {
         const int cn = min(MOVE_CHUNK, moves - chunk);
         load_proposals(P, pbase, chunk, cn, pt, pR, pk);
-        if (tid < cn) pu[tid] = P.pr.u[pbase + chunk + tid];
+        if (tid < cn && P.pr.u) pu[tid] = P.pr.u[pbase + chunk + tid];
         __syncthreads();
         for (int mi = 0; mi < cn; mi++, move++) {
             const int kIdx = pk[mi];
@@ -1529,8 +1532,13 @@ __device__ __forceinline__ void sweep_cell_soft(const SweepParams &P, const Sten
                         TN = image_translation(sT, code, nwx, nwy, nwz);
                         TO = image_translation(sT, code, ow[0], ow[1], ow[2]);
                     }
-                    eFinal += soft_energy(P.k, fN, p2, a2, TN);
-                    eInitial += soft_energy(P.k, fO, p2, a2, TO);
+                    if (P.counting) {
+                        eFinal += pair_overlap<KIND>(P.k, fN, p2, a2, TN) ? 1.0 : 0.0;
+                        eInitial += pair_overlap<KIND>(P.k, fO, p2, a2, TO) ? 1.0 : 0.0;
+                    } else {
+                        eFinal += soft_energy(P.k, fN, p2, a2, TN);
+                        eInitial += soft_energy(P.k, fO, p2, a2, TO);
+                    }
                 }
                 tests += (unsigned long long)total;
             } else {
@@ -1549,7 +1557,13 @@ __device__ __forceinline__ void sweep_cell_soft(const SweepParams &P, const Sten
                 fin += slot[w];
                 ini += slot[WAVES + w];
             }
-            if (!inside || !(pu[mi] <= exp(-(fin - ini) / P.temperature))) continue;
+            if (!inside) continue;
+            if (P.counting) {
+                if (fin - ini > 0) continue;  // counts are exact in doubles
+                overlapDelta += (long long)(fin - ini);
+            } else if (!(pu[mi] <= exp(-(fin - ini) / P.temperature))) {
+                continue;
+            }
 
             accepted++;
             if (STAGED) {
@@ -1589,6 +1603,8 @@ __device__ __forceinline__ void sweep_cell_soft(const SweepParams &P, const Sten
         }
     }
     flush_counters(P, accepted, moves, leftCell, tests, STAGED);
+    if (P.counting && tid == 0 && overlapDelta)
+        atomicAdd(&P.counters[CNT_OVERLAP_DELTA], (unsigned long long)overlapDelta);
 }
 
 // Stages the stencil: every lane resolves up to SLOT_GROUP entries, issues all their HBM loads, then writes LDS.
@@ -1847,9 +1863,9 @@ hipError_t launch_sweep(hipStream_t s, const SweepParams &sp, int waves) {
     else if (w == 2) RP_SWEEP(KIND, 2, SOFT); \
     else RP_SWEEP(KIND, 1, SOFT);
     if (sp.k.kind == 0) {
-        if (sp.k.soft) { RP_SWEEP_W(0, true) } else { RP_SWEEP_W(0, false) }
+        if (sp.k.soft || sp.counting) { RP_SWEEP_W(0, true) } else { RP_SWEEP_W(0, false) }
     } else {
-        if (sp.k.soft) { RP_SWEEP_W(1, true) } else { RP_SWEEP_W(1, false) }
+        if (sp.k.soft || sp.counting) { RP_SWEEP_W(1, true) } else { RP_SWEEP_W(1, false) }
     }
 #undef RP_SWEEP_W
 #undef RP_SWEEP

@@ -288,3 +288,51 @@ def test_fix_orientations_matches_oracle(shape_name):
     assert np.array_equal(p, pos)
     assert np.array_equal(r, expected)
     pk.close()
+
+
+def random_rod_lattice(n_side, spacing, seed):
+    """Rods on a simple-cubic lattice with uniformly random orientations: overlapping when spacing < L + 2r."""
+    idx = np.stack(np.meshgrid(*[np.arange(n_side)] * 3, indexing="ij"), -1).reshape(-1, 3).astype(float)
+    pos = (idx + 0.5) * spacing
+    rng = np.random.Generator(np.random.PCG64(seed))
+    rot = corpus.random_rotations(rng, len(pos))
+    return np.diag([n_side * spacing] * 3), np.ascontiguousarray(pos), np.ascontiguousarray(rot)
+
+
+def test_cached_overlap_count_stays_exact_in_counting_mode():
+    """Overlap-counting mode (Packing::toggleOverlapCounting): after particle and box moves the cached count equals
+    a full recount by the oracle, exactly."""
+    box, pos, rot = random_rod_lattice(8, 3.0, 4)
+    start, _ = oracle_lib.oracle().snapshot_stats(1, 0.5, 5.0, box, pos, rot)
+    assert start > 0
+    pk = ra.GpuPacking(SC, box, pos, rot, seed=6)
+    pk.set_environment(1.0, 1.0, 0.3, None, box_move=("delta_triclinic", 0.01))
+    pk.set_overlap_counting(True)
+    assert pk.cached_overlaps() == start
+    for _ in range(4):
+        c = pk.run_cycles(5)
+        b, p, r = pk.download()
+        full, _ = oracle_lib.oracle().snapshot_stats(1, 0.5, 5.0, b, p, r)
+        assert pk.cached_overlaps() == full
+    assert c["box_attempted"] > 0 and c["moves_accepted"] > 0
+    pk.set_overlap_counting(False)
+    with pytest.raises(ra.PreconditionError):
+        pk.cached_overlaps()
+
+
+def test_overlap_relaxation_reaches_a_legal_packing():
+    """Simulation::relaxOverlaps from randomly oriented rods on a lattice (the survey's C2 start, 1000 rods at the
+    same spacing): the relaxation ends with no overlap, confirmed by the oracle, and ordinary integration accepts
+    the result."""
+    box, pos, rot = random_rod_lattice(10, 52.22 / 20, 2024)
+    before, _ = oracle_lib.oracle().snapshot_stats(1, 0.5, 5.0, box, pos, rot)
+    assert before > 100
+    pk = ra.GpuPacking(SC, box, pos, rot, seed=2024)
+    pk.set_environment(1.0, 1.0, 0.5, None, box_move=("delta_triclinic", 0.01))
+    cycles, left = pk.relax_overlaps(20000)
+    print(f"relaxation: {before} overlaps removed in {cycles} cycles", flush=True)
+    assert left == 0
+    b, p, r = pk.download()
+    assert overlaps_by_oracle(SC, b, p, r) == 0
+    pk.run_cycles(10)
+    pk.close()
