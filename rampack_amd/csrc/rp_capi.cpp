@@ -582,7 +582,8 @@ int runCycle(rp_ctx *c) {
     cap = (cap + 63) / 64 * 64;
     int waves = c->sweepWaves;
     if (waves <= 0) waves = expect > 384 ? 4 : (expect > 128 ? 2 : 1);
-    waves = waves >= 4 ? 4 : (waves == 2 ? 2 : 1);
+    waves = waves >= 8 ? 8 : (waves >= 4 ? 4 : (waves == 2 ? 2 : 1));
+    if (waves == 8 && (c->k.soft || c->counting)) waves = 4;  // eight wavefronts only for the hard sweep
     const double perEntry =
         (double)(sweep_lds_bytes(c->k.kind, 1024, waves) - sweep_lds_bytes(c->k.kind, 0, waves)) / 1024.0;
     const size_t fixed = sweep_lds_bytes(c->k.kind, 0, waves);

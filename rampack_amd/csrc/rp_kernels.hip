@@ -1654,8 +1654,9 @@ __device__ __forceinline__ void stage_stencil_batched(const SweepParams &P, cons
     }
 }
 
+// Two workgroups of eight wavefronts share a compute unit only within 128 registers per lane (4 wavefronts per SIMD)
 template <int KIND, int WAVES, bool SOFT>
-__global__ __launch_bounds__(WAVE *WAVES) void k_sweep(SweepParams P) {
+__global__ __launch_bounds__(WAVE *WAVES, WAVES >= 8 ? 4 : 1) void k_sweep(SweepParams P) {
     constexpr int NT = WAVE * WAVES;
     extern __shared__ double smem[];
     __shared__ int s_start[MAX_STENCIL + 1];
@@ -1835,16 +1836,18 @@ hipError_t launch_fix_all(hipStream_t s, const Particles &p, int n) {
 }
 
 int sweep_occupancy(int kind, int cap, int waves) {
-    int w = waves >= 4 ? 4 : (waves == 2 ? 2 : 1);
+    int w = waves >= 8 ? 8 : (waves >= 4 ? 4 : (waves == 2 ? 2 : 1));
     size_t lds = sweep_lds_bytes(kind, cap, w);
     int blocks = 0;
 #define RP_OCC(KIND, W) hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_sweep<KIND, W, false>, WAVE * W, lds)
     if (kind == 0) {
-        if (w == 4) RP_OCC(0, 4);
+        if (w == 8) RP_OCC(0, 8);
+        else if (w == 4) RP_OCC(0, 4);
         else if (w == 2) RP_OCC(0, 2);
         else RP_OCC(0, 1);
     } else {
-        if (w == 4) RP_OCC(1, 4);
+        if (w == 8) RP_OCC(1, 8);
+        else if (w == 4) RP_OCC(1, 4);
         else if (w == 2) RP_OCC(1, 2);
         else RP_OCC(1, 1);
     }
@@ -1854,7 +1857,8 @@ int sweep_occupancy(int kind, int cap, int waves) {
 
 hipError_t launch_sweep(hipStream_t s, const SweepParams &sp, int waves) {
     unsigned blocks = (unsigned)(sp.na[0] * sp.na[1] * sp.na[2]);
-    int w = waves >= 4 ? 4 : (waves == 2 ? 2 : 1);
+    int w = waves >= 8 ? 8 : (waves >= 4 ? 4 : (waves == 2 ? 2 : 1));
+    if (w == 8 && (sp.k.soft || sp.counting)) w = 4;  // eight wavefronts only for the hard sweep
     size_t lds = sweep_lds_bytes(sp.k.kind, sp.cap, w);
     // soft and hard sweeps are separate kernels: each instantiation carries one path only
 #define RP_SWEEP(KIND, W, SOFT) hipLaunchKernelGGL((k_sweep<KIND, W, SOFT>), dim3(blocks), dim3(WAVE * W), lds, s, sp)
@@ -1863,9 +1867,13 @@ hipError_t launch_sweep(hipStream_t s, const SweepParams &sp, int waves) {
     else if (w == 2) RP_SWEEP(KIND, 2, SOFT); \
     else RP_SWEEP(KIND, 1, SOFT);
     if (sp.k.kind == 0) {
-        if (sp.k.soft || sp.counting) { RP_SWEEP_W(0, true) } else { RP_SWEEP_W(0, false) }
+        if (sp.k.soft || sp.counting) { RP_SWEEP_W(0, true) }
+        else if (w == 8) RP_SWEEP(0, 8, false);
+        else { RP_SWEEP_W(0, false) }
     } else {
-        if (sp.k.soft || sp.counting) { RP_SWEEP_W(1, true) } else { RP_SWEEP_W(1, false) }
+        if (sp.k.soft || sp.counting) { RP_SWEEP_W(1, true) }
+        else if (w == 8) RP_SWEEP(1, 8, false);
+        else { RP_SWEEP_W(1, false) }
     }
 #undef RP_SWEEP_W
 #undef RP_SWEEP

@@ -159,10 +159,10 @@ def test_degenerate_hard_sphere_gas_carnahan_starling():
     assert err / rho < 0.03
 
 
-@pytest.mark.parametrize("waves,pipe", [(1, 0), (2, 0), (4, 0), (2, 1), (4, 1), (2, 2), (4, 2)])
+@pytest.mark.parametrize("waves,pipe", [(1, 0), (2, 0), (4, 0), (2, 1), (4, 1), (2, 2), (4, 2), (8, 1), (8, 2)])
 @pytest.mark.parametrize("kind", ["sphere", "spherocylinder"])
 def test_every_launch_shape_is_overlap_free(kind, waves, pipe, monkeypatch):
-    """Each workgroup size (1, 2 or 4 wavefronts per active cell; plain, planner/tester split or grouped moves) for
+    """Each workgroup size (1, 2, 4 or 8 wavefronts per active cell; plain, planner/tester split or grouped moves) for
     both shapes keeps the packing overlap-free."""
     monkeypatch.setenv("RAMPACK_SWEEP_WAVES", str(waves))
     monkeypatch.setenv("RAMPACK_SWEEP_PIPELINE", str(pipe))
@@ -214,7 +214,7 @@ def test_launch_shapes_give_identical_trajectories(kind, monkeypatch):
         box, pos, rot = rod_start()
         move = ("delta_triclinic", 0.002)
     results = {}
-    for waves, pipe in ((1, 0), (2, 0), (4, 0), (2, 1), (4, 1), (2, 2), (4, 2)):
+    for waves, pipe in ((1, 0), (2, 0), (4, 0), (2, 1), (4, 1), (2, 2), (4, 2), (8, 0), (8, 1), (8, 2)):
         monkeypatch.setenv("RAMPACK_SWEEP_WAVES", str(waves))
         monkeypatch.setenv("RAMPACK_SWEEP_PIPELINE", str(pipe))
         pk = ra.GpuPacking(shape, box, pos, rot, seed=77)
