@@ -361,6 +361,13 @@ int readCounters(rp_ctx *c) {
 // Largest multiple of 64 entries whose LDS footprint (perEntry bytes each) fits the workgroup limit.
 int capForBytes(rp_ctx *c, double perEntry) { return std::max(64, (int)((double)c->ldsLimit / perEntry) / 64 * 64); }
 
+// Largest stencil the overlap-count / energy kernels can stage within the LDS limit.
+int countCapLimit(rp_ctx *c) {
+    const size_t fixed = count_lds_bytes(c->k.kind, 0);
+    const double per = (double)(count_lds_bytes(c->k.kind, 1024) - fixed) / 1024.0;
+    return std::max(64, (int)((double)(c->ldsLimit - fixed) / per) / 64 * 64);
+}
+
 // Early-exit overlap scan of the `alt` arrays sorted by `g`; sets found.
 int scanOverlaps(rp_ctx *c, const Grid &g, int earlyExit, uint64_t &count) {
     RP_HIP(launch_fill_u64(c->stream, c->dCounters + CNT_OVERLAPS, 1, 0));
@@ -369,7 +376,7 @@ int scanOverlaps(rp_ctx *c, const Grid &g, int earlyExit, uint64_t &count) {
     const int side = 2 * g.reach + 1;
     double expect = ((side * side * side - 1) / 2 + 1) * lambda;
     int cap = (int)std::ceil(expect + 8 * std::sqrt(expect) + 64);
-    cap = std::min((cap + 63) / 64 * 64, capForBytes(c, count_lds_bytes(c->k.kind, 64) / 64.0));
+    cap = std::min((cap + 63) / 64 * 64, countCapLimit(c));
     RP_HIP(launch_count_overlaps(c->stream, g, (int)c->n, c->alt.p, c->k, earlyExit, c->dCounters, cap));
     timingEnd(c);
     int r = readCounters(c);
@@ -464,7 +471,7 @@ int totalEnergy(rp_ctx *c, const Grid &g, const Particles &p, double &out) {
     const int side = 2 * g.reach + 1;
     double expect = ((side * side * side - 1) / 2 + 1) * lambda;
     int cap = (int)std::ceil(expect + 8 * std::sqrt(expect) + 64);
-    cap = std::min((cap + 63) / 64 * 64, capForBytes(c, count_lds_bytes(c->k.kind, 64) / 64.0));
+    cap = std::min((cap + 63) / 64 * 64, countCapLimit(c));
     RP_HIP(launch_total_energy(c->stream, g, (int)c->n, p, c->k, c->cellEnergy, c->sumScratch, c->dCounters, cap));
     RP_HIP(hipMemcpyAsync(&out, c->sumScratch + fixed_sum_scratch_doubles() - 1, sizeof(double),
                           hipMemcpyDeviceToHost, c->stream));

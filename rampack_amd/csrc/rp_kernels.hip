@@ -456,6 +456,97 @@ __device__ void count_cell(const Grid &g, const Particles &p, const PairConsts &
     if (tid == 0 && *sFound) atomicAdd(&counters[CNT_OVERLAPS], (unsigned long long)*sFound);
 }
 
+// Flat form of count_cell for a staged stencil: the (own particle, candidate) pairs of the cell are processed in
+// chunks of COUNT_CHUNK by all threads at once -- pass A classifies each pair by its centre distance (definite
+// overlap, ambiguous, far), pass B runs the segment test on the chunk's ambiguous pairs, compacted block-wide -- so
+// no thread walks the own particles one by one.  Same arithmetic as pair_overlap, split in two.
+constexpr int COUNT_OWN_CAP = 128;             // own particles whose first-operand data fits the table
+constexpr int COUNT_CHUNK = COUNT_THREADS * 8;  // pairs per chunk (and queue entries)
+
+template <int KIND>
+__device__ void count_cell_flat(const PairConsts &k, const Stencil &S, const double *sT, const int *sTag, int nOwn,
+                                int total, int earlyExit, unsigned long long *counters, int *sFound, double *fTab,
+                                int *queue, int *sQueued) {
+    const int tid = threadIdx.x;
+    const int lane = tid & (WAVE - 1);
+    volatile unsigned long long *flag = counters + CNT_OVERLAPS;
+    // first operands: position, cap A, u, a (10 doubles) per own particle
+    for (int a = tid; a < nOwn; a += COUNT_THREADS) {
+        const V3 pos1 = v3(S.x[a], S.y[a], S.z[a]);
+        SCFirst f = make_first<KIND>(k, pos1, KIND == 1 ? v3(S.ax[a], S.ay[a], S.az[a]) : v3(0, 0, 0));
+        double *q = fTab + 10 * a;
+        q[0] = f.pos.x; q[1] = f.pos.y; q[2] = f.pos.z;
+        if (KIND == 1) {
+            q[3] = f.capA.x; q[4] = f.capA.y; q[5] = f.capA.z;
+            q[6] = f.u.x; q[7] = f.u.y; q[8] = f.u.z;
+            q[9] = f.a;
+        }
+    }
+    if (tid == 0) *sQueued = 0;
+    __syncthreads();
+    unsigned long long found = 0;
+    const int pairs = nOwn * total;  // < COUNT_OWN_CAP * 65536
+    // pair index q = a * total + b, advanced by COUNT_THREADS per step without a division
+    int qa = tid / total, qb = tid - (tid / total) * total;
+    for (int base = 0; base < pairs; base += COUNT_CHUNK) {
+        // pass A
+        for (int i = 0; i < COUNT_CHUNK / COUNT_THREADS; i++) {
+            const int qi = base + i * COUNT_THREADS + tid;
+            bool amb = false;
+            const int a = qa, b = qb;
+            qb += COUNT_THREADS;
+            while (qb >= total) {
+                qb -= total;
+                qa++;
+            }
+            if (qi < pairs) {
+                if (!(b < nOwn && sTag[b] >= sTag[a])) {
+                    const int c1 = S.code[a], code = S.code[b];
+                    const int wx = (c1 & 3) - 1, wy = ((c1 >> 2) & 3) - 1, wz = ((c1 >> 4) & 3) - 1;
+                    const V3 T = b < nOwn && code == c1 ? v3(0, 0, 0) : image_translation(sT, code, wx, wy, wz);
+                    const double *q = fTab + 10 * a;
+                    const V3 d = sub(add(v3(S.x[b], S.y[b], S.z[b]), T), v3(q[0], q[1], q[2]));
+                    const double d2 = dot(d, d);
+                    if (d2 < k.contact2) found++;
+                    else if (KIND == 1 && d2 < k.range2) amb = true;
+                }
+            }
+            if (KIND == 1) {
+                const unsigned long long m = __ballot(amb);
+                int slot = 0;
+                if (lane == 0 && m) slot = atomicAdd(sQueued, __popcll(m));
+                slot = __shfl(slot, 0);
+                if (amb) queue[slot + __popcll(m & ((1ull << lane) - 1))] = (a << 16) | b;
+            }
+        }
+        __syncthreads();
+        // pass B
+        if (KIND == 1) {
+            const int queued = *sQueued;
+            for (int e = tid; e < queued; e += COUNT_THREADS) {
+                const int a = queue[e] >> 16, b = queue[e] & 0xffff;
+                const int c1 = S.code[a], code = S.code[b];
+                const int wx = (c1 & 3) - 1, wy = ((c1 >> 2) & 3) - 1, wz = ((c1 >> 4) & 3) - 1;
+                const V3 T = b < nOwn && code == c1 ? v3(0, 0, 0) : image_translation(sT, code, wx, wy, wz);
+                const double *q = fTab + 10 * a;
+                const V3 p2 = add(v3(S.x[b], S.y[b], S.z[b]), T);
+                const V3 a2 = v3(S.ax[b], S.ay[b], S.az[b]);
+                const V3 c2a = add(p2, scal(a2, -k.halfLen));
+                const V3 c2b = add(p2, scal(a2, k.halfLen));
+                if (segment_distance2(v3(q[3], q[4], q[5]), v3(q[6], q[7], q[8]), q[9], c2a, c2b) < k.contact2)
+                    found++;
+            }
+        }
+        const int stop = earlyExit ? __syncthreads_or(found != 0 || *flag != 0) : (__syncthreads(), 0);
+        if (tid == 0) *sQueued = 0;
+        __syncthreads();
+        if (stop) break;
+    }
+    if (found) atomicAdd(sFound, (int)found);
+    __syncthreads();
+    if (tid == 0 && *sFound) atomicAdd(&counters[CNT_OVERLAPS], (unsigned long long)*sFound);
+}
+
 template <int KIND>
 __global__ __launch_bounds__(COUNT_THREADS) void k_count_overlaps(Grid g, Particles p, PairConsts k, int earlyExit,
                                                                   unsigned long long *counters, int cap) {
@@ -485,7 +576,14 @@ __global__ __launch_bounds__(COUNT_THREADS) void k_count_overlaps(Grid g, Partic
     carve_stencil(S, smem, cap, KIND, rest);
     int *sTag = reinterpret_cast<int *>(rest);  // own-cell tags, up to cap entries
     for (int a = tid; a < nOwn && a < cap; a += COUNT_THREADS) sTag[a] = p.tag[own0 + a];
-    if (total <= cap) {
+    if (total <= cap && nOwn <= COUNT_OWN_CAP && total <= 0xffff) {
+        __shared__ int sQueued;
+        stage_stencil<KIND>(p, S, total, tid, COUNT_THREADS);
+        double *fTab = reinterpret_cast<double *>(sTag + ((cap + 1) & ~1));  // 10 doubles per own particle
+        int *queue = reinterpret_cast<int *>(fTab + 10 * COUNT_OWN_CAP);        // COUNT_CHUNK entries
+        __syncthreads();
+        count_cell_flat<KIND>(k, S, sT, sTag, nOwn, total, earlyExit, counters, &sFound, fTab, queue, &sQueued);
+    } else if (total <= cap) {
         stage_stencil<KIND>(p, S, total, tid, COUNT_THREADS);
         __syncthreads();
         count_cell<KIND, true>(g, p, k, S, sT, sTag, nOwn, total, earlyExit, counters, &sFound);
@@ -498,7 +596,10 @@ __global__ __launch_bounds__(COUNT_THREADS) void k_count_overlaps(Grid g, Partic
     }
 }
 
-size_t count_lds_bytes(int kind, int cap) { return stencil_lds_bytes(kind, cap) + sizeof(int) * cap; }
+size_t count_lds_bytes(int kind, int cap) {
+    return stencil_lds_bytes(kind, cap) + sizeof(int) * (((cap + 1) & ~1) + COUNT_CHUNK) +
+           sizeof(double) * 10 * COUNT_OWN_CAP;
+}
 
 hipError_t launch_count_overlaps(hipStream_t s, const Grid &g, int n, Particles p, PairConsts k, int earlyExit,
                                  unsigned long long *counters, int cap) {
