@@ -179,6 +179,47 @@ int rp_ramsnap_info(const char *path, size_t *n, size_t *aux_bytes);
 int rp_ramsnap_read(const char *path, double box[9], size_t n, double *pos, double *rot, char *aux_text,
                     size_t aux_cap);
 
+/* RAMTRJ trajectories, the reference's binary trajectory format (RamtrjIO, src/core/io/RamtrjIO.cpp:16-105): a
+ * 33-byte header, then per snapshot the box (9 doubles) and per particle its position and the extrinsic XYZ
+ * Tait-Bryan angles of its orientation (EulerAngles::forMatrix, src/geometry/EulerAngles.cpp:12-45; read back with
+ * Matrix<3,3>::rotation, src/geometry/Matrix.tpp:408-433).  Files are byte-identical to the reference's.
+ *
+ * Recorder -- RamtrjRecorder (src/core/io/RamtrjRecorder.cpp:8-66) opened as SimulationRecorderFactory does
+ * (src/frontend/SimulationRecorderFactory.cpp:13-31): append 0 truncates the file, append 1 continues a recording
+ * with the same particle count and cycle step.  Snapshot k (1, 2, ...) must be recorded at cycle k * cycle_step;
+ * an orientation that is not a rotation matrix to 1e-12 is a precondition error (EulerAngles.cpp:47-53).  Closing
+ * writes the snapshot count into the header. */
+typedef struct rp_ramtrj_recorder rp_ramtrj_recorder;
+int rp_ramtrj_recorder_open(rp_ramtrj_recorder **out, const char *path, size_t num_particles, size_t cycle_step,
+                            int append);
+int rp_ramtrj_record(rp_ramtrj_recorder *rec, const double box[9], size_t n, const double *pos, const double *rot,
+                     size_t cycle);
+int rp_ramtrj_record_packing(rp_ramtrj_recorder *rec, rp_ctx *ctx, size_t cycle);  /* the device-resident state */
+int rp_ramtrj_recorder_last_cycle(rp_ramtrj_recorder *rec, size_t *cycle);      /* getLastCycleNumber */
+int rp_ramtrj_recorder_close(rp_ramtrj_recorder *rec);                           /* also frees rec */
+
+/* Player -- RamtrjPlayer (src/core/io/RamtrjPlayer.cpp:11-128) opened as IO::loadRamtrjPlayer does
+ * (src/frontend/IO.cpp:32-64).  expected_particles 0 reads strictly: a file whose length disagrees with its header
+ * is RP_ERR_VALIDATION.  Otherwise a truncated file (a recorder that never closed) is repaired in memory as
+ * RamtrjPlayer::AutoFix does (RamtrjPlayer.cpp:146-189) and *fix, if given, reports it.  next_snapshot reads the
+ * following snapshot; jump_to_snapshot the one recorded at `cycle` (a positive multiple of the cycle step). */
+typedef struct rp_ramtrj_player rp_ramtrj_player;
+typedef struct rp_ramtrj_header {
+    uint8_t version_major, version_minor;
+    uint64_t num_particles, num_snapshots, cycle_step;
+    uint64_t current_snapshot; /* snapshots read; current_snapshot * cycle_step = getCurrentSnapshotCycles */
+} rp_ramtrj_header;
+typedef struct rp_ramtrj_fix {
+    int fixing_needed, fixing_successful;
+    uint64_t header_snapshots, inferred_snapshots, bytes_remainder, bytes_per_snapshot;
+} rp_ramtrj_fix;
+int rp_ramtrj_player_open(rp_ramtrj_player **out, const char *path, size_t expected_particles, rp_ramtrj_fix *fix);
+int rp_ramtrj_player_header(rp_ramtrj_player *player, rp_ramtrj_header *out);
+int rp_ramtrj_next_snapshot(rp_ramtrj_player *player, double box[9], size_t n, double *pos, double *rot);
+int rp_ramtrj_jump_to_snapshot(rp_ramtrj_player *player, size_t cycle, double box[9], size_t n, double *pos,
+                               double *rot);
+void rp_ramtrj_player_close(rp_ramtrj_player *player);
+
 /* Current state, in the caller's particle order (Packing::operator[], Packing::getBox). */
 int rp_download(rp_ctx *ctx, double box[9], double *pos, double *rot);
 

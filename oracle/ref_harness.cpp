@@ -41,6 +41,8 @@
 #include "core/observables/NematicOrder.h"
 #include "core/io/RamsnapReader.h"
 #include "core/io/RamsnapWriter.h"
+#include "core/io/RamtrjRecorder.h"
+#include "core/io/RamtrjPlayer.h"
 #include <fstream>
 #include <sstream>
 
@@ -240,6 +242,94 @@ long ref_ramsnap_read(const char *path, double *box, std::size_t n, double *pos,
         }
         return (long)packing.size();
     } catch (const std::exception &) {
+        return -1;
+    }
+}
+
+// RamtrjRecorder over a file opened as SimulationRecorderFactory does: snapshots k = 0..n_snap-1 (boxes 9 doubles,
+// positions n x 3, orientations n x 9 each, stacked) recorded at cycles first_cycle + k * cycle_step; append
+// continues an existing file.  The recorder is closed (header rewritten) on return.  0, or -1 on an exception.
+int ref_ramtrj_write(const char *path, std::size_t n, std::size_t cycle_step, std::size_t n_snap, const double *boxes,
+                     const double *pos, const double *rot, int append, std::size_t first_cycle) {
+    try {
+        SphereTraits traits(0.5);
+        auto mode = std::ios_base::in | std::ios_base::out | std::ios_base::binary;
+        auto f = std::make_unique<std::fstream>(path, append ? mode : mode | std::ios_base::trunc);
+        if (!*f) return -1;
+        RamtrjRecorder recorder(std::move(f), n, cycle_step, append != 0);
+        for (std::size_t k = 0; k < n_snap; k++) {
+            Packing packing(TriclinicBox(mat(boxes + 9 * k)), makeShapes(n, pos + 3 * n * k, rot + 9 * n * k),
+                            std::make_unique<PeriodicBoundaryConditions>(), traits.getInteraction(), 1, 1);
+            recorder.recordSnapshot(packing, first_cycle + k * cycle_step);
+        }
+        recorder.close();
+        return 0;
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "ref_ramtrj_write: %s\n", e.what());
+        return -1;
+    }
+}
+
+// RamtrjPlayer over the file (expected_n 0: strict constructor; else with AutoFix(expected_n)); jump_cycle 0 reads
+// every snapshot in order with nextSnapshot, otherwise jumpToSnapshot(jump_cycle) once.  Up to max_snap snapshots
+// go to boxes/pos/rot (stacked as in ref_ramtrj_write); info receives particles, snapshots, cycle step, total
+// cycles, fixing needed, fixing successful, inferred snapshots, bytes remainder.  Returns the number of snapshots
+// read, or -1 on an exception (info is still filled from the AutoFix report).
+long ref_ramtrj_read(const char *path, std::size_t expected_n, std::size_t jump_cycle, std::size_t n,
+                     std::size_t max_snap, double *boxes, double *pos, double *rot, unsigned long long *info) {
+    std::unique_ptr<RamtrjPlayer::AutoFix> autoFix;
+    if (expected_n) autoFix = std::make_unique<RamtrjPlayer::AutoFix>(expected_n);
+    auto report = [&]() {
+        if (!autoFix) return;
+        info[4] = autoFix->wasFixingNeeded();
+        info[5] = autoFix->wasFixingSuccessful();
+        info[6] = autoFix->getInferredSnapshots();
+        info[7] = autoFix->getBytesRemainder();
+    };
+    try {
+        SphereTraits traits(0.5);
+        auto in = std::make_unique<std::ifstream>(path, std::ios_base::in | std::ios_base::binary);
+        std::unique_ptr<RamtrjPlayer> player;
+        if (autoFix) player = std::make_unique<RamtrjPlayer>(std::move(in), *autoFix);
+        else player = std::make_unique<RamtrjPlayer>(std::move(in));
+        info[0] = player->getNumMolecules();
+        info[2] = player->getCycleStep();
+        info[3] = player->getTotalCycles();
+        info[1] = info[2] ? info[3] / info[2] : 0;
+        report();
+        // nextSnapshot expects a packing of the trajectory's size (RamtrjPlayer.cpp:56); its state is replaced
+        std::vector<double> p0(3 * n, 0.5), r0(9 * n, 0.0);
+        for (std::size_t i = 0; i < n; i++) r0[9 * i] = r0[9 * i + 4] = r0[9 * i + 8] = 1;
+        Packing packing(TriclinicBox(std::array<double, 3>{1, 1, 1}), makeShapes(n, p0.data(), r0.data()),
+                        std::make_unique<PeriodicBoundaryConditions>(), traits.getInteraction(), 1, 1);
+        long got = 0;
+        auto store = [&]() {
+            if ((std::size_t)got >= max_snap) return;
+            const auto &d = packing.getBox().getDimensions();
+            for (int a = 0; a < 3; a++)
+                for (int b = 0; b < 3; b++) boxes[9 * got + 3 * a + b] = d(a, b);
+            for (std::size_t i = 0; i < packing.size() && i < n; i++) {
+                const auto &p = packing[i].getPosition();
+                const auto &O = packing[i].getOrientation();
+                for (int a = 0; a < 3; a++) pos[3 * n * got + 3 * i + a] = p[a];
+                for (int a = 0; a < 3; a++)
+                    for (int b = 0; b < 3; b++) rot[9 * n * got + 9 * i + 3 * a + b] = O(a, b);
+            }
+            got++;
+        };
+        if (jump_cycle) {
+            player->jumpToSnapshot(packing, traits.getInteraction(), jump_cycle);
+            store();
+        } else {
+            while (player->hasNext()) {
+                player->nextSnapshot(packing, traits.getInteraction());
+                store();
+            }
+        }
+        return got;
+    } catch (const std::exception &e) {
+        report();
+        std::fprintf(stderr, "ref_ramtrj_read: %s\n", e.what());
         return -1;
     }
 }

@@ -43,6 +43,9 @@ EXPORTED_SYMBOLS = (
     "rp_total_energy", "rp_pair_energies", "rp_fix_orientations", "rp_set_orientation_fix_every",
     "rp_ramsnap_write", "rp_ramsnap_save", "rp_ramsnap_info", "rp_ramsnap_read",
     "rp_set_overlap_counting", "rp_cached_overlaps", "rp_relax_overlaps",
+    "rp_ramtrj_recorder_open", "rp_ramtrj_record", "rp_ramtrj_record_packing", "rp_ramtrj_recorder_last_cycle",
+    "rp_ramtrj_recorder_close", "rp_ramtrj_player_open", "rp_ramtrj_player_header", "rp_ramtrj_next_snapshot",
+    "rp_ramtrj_jump_to_snapshot", "rp_ramtrj_player_close",
 )
 
 
@@ -90,6 +93,21 @@ class Observables(ctypes.Structure):
         d = {name: getattr(self, name) for name, _ in self._fields_ if name != "q_tensor"}
         d["q_tensor"] = list(self.q_tensor)
         return d
+
+
+class RamtrjHeader(ctypes.Structure):
+    _fields_ = [("version_major", ctypes.c_uint8), ("version_minor", ctypes.c_uint8),
+                ("num_particles", ctypes.c_uint64), ("num_snapshots", ctypes.c_uint64),
+                ("cycle_step", ctypes.c_uint64), ("current_snapshot", ctypes.c_uint64)]
+
+
+class RamtrjFix(ctypes.Structure):
+    _fields_ = [("fixing_needed", ctypes.c_int), ("fixing_successful", ctypes.c_int),
+                ("header_snapshots", ctypes.c_uint64), ("inferred_snapshots", ctypes.c_uint64),
+                ("bytes_remainder", ctypes.c_uint64), ("bytes_per_snapshot", ctypes.c_uint64)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 class RampackError(RuntimeError):
@@ -144,6 +162,18 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.rp_ramsnap_save.argtypes = [vp, ctypes.c_char_p, ctypes.c_char_p]
     lib.rp_ramsnap_info.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]
     lib.rp_ramsnap_read.argtypes = [ctypes.c_char_p, dp, ctypes.c_size_t, dp, dp, ctypes.c_char_p, ctypes.c_size_t]
+    sz = ctypes.c_size_t
+    lib.rp_ramtrj_recorder_open.argtypes = [ctypes.POINTER(vp), ctypes.c_char_p, sz, sz, ctypes.c_int]
+    lib.rp_ramtrj_record.argtypes = [vp, dp, sz, dp, dp, sz]
+    lib.rp_ramtrj_record_packing.argtypes = [vp, vp, sz]
+    lib.rp_ramtrj_recorder_last_cycle.argtypes = [vp, ctypes.POINTER(sz)]
+    lib.rp_ramtrj_recorder_close.argtypes = [vp]
+    lib.rp_ramtrj_player_open.argtypes = [ctypes.POINTER(vp), ctypes.c_char_p, sz, ctypes.POINTER(RamtrjFix)]
+    lib.rp_ramtrj_player_header.argtypes = [vp, ctypes.POINTER(RamtrjHeader)]
+    lib.rp_ramtrj_next_snapshot.argtypes = [vp, dp, sz, dp, dp]
+    lib.rp_ramtrj_jump_to_snapshot.argtypes = [vp, sz, dp, sz, dp, dp]
+    lib.rp_ramtrj_player_close.argtypes = [vp]
+    lib.rp_ramtrj_player_close.restype = None
     lib.rp_fix_orientations.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64)]
     lib.rp_set_orientation_fix_every.argtypes = [vp, ctypes.c_uint64]
     lib.rp_pair_energies.argtypes = [ctypes.POINTER(ShapeDesc), ctypes.c_size_t, dp, dp]
@@ -375,6 +405,138 @@ def ramsnap_read(path: str):
         key, _, value = line.partition(" ")
         aux[key] = value
     return box.reshape(3, 3), pos, rot, aux
+
+
+class RamtrjRecorder:
+    """RamtrjRecorder (src/core/io/RamtrjRecorder.cpp:8-66), opened as SimulationRecorderFactory does
+    (src/frontend/SimulationRecorderFactory.cpp:13-31): append=False truncates the file, append=True continues a
+    recording of the same particle count and cycle step.  Snapshot k (k = 1, 2, ...) is recorded at cycle
+    k * cycle_step; close() writes the snapshot count into the header."""
+
+    def __init__(self, path: str, num_particles: int, cycle_step: int, append: bool = False):
+        lib = load_library()
+        self._rec = ctypes.c_void_p()
+        self.num_particles = int(num_particles)
+        _check(lib.rp_ramtrj_recorder_open(ctypes.byref(self._rec), os.fsencode(path), self.num_particles,
+                                           int(cycle_step), int(bool(append))))
+
+    def record_snapshot(self, state, cycle: int):
+        """Records a GpuPacking (its device-resident state) or a (box, positions, orientations) triple."""
+        if self._rec is None:
+            raise PreconditionError(RP_ERR_PRECONDITION, "RAMTRJ recorder is closed")
+        if isinstance(state, GpuPacking):
+            _check(_lib.rp_ramtrj_record_packing(self._rec, state._ctx, int(cycle)))
+            return
+        box, pos, rot = state
+        box = np.ascontiguousarray(box, dtype=np.float64).reshape(9)
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+        rot = np.ascontiguousarray(rot, dtype=np.float64).reshape(-1, 9)
+        if len(pos) != len(rot):
+            raise ValueError("positions and orientations differ in length")
+        _check(_lib.rp_ramtrj_record(self._rec, _dptr(box), len(pos), _dptr(pos), _dptr(rot), int(cycle)))
+
+    def get_last_cycle_number(self) -> int:
+        out = ctypes.c_size_t()
+        _check(_lib.rp_ramtrj_recorder_last_cycle(self._rec, ctypes.byref(out)))
+        return out.value
+
+    def close(self):
+        if self._rec is not None:
+            rec, self._rec = self._rec, None
+            _check(_lib.rp_ramtrj_recorder_close(rec))
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        if getattr(self, "_rec", None) is not None and _lib is not None:
+            _lib.rp_ramtrj_recorder_close(self._rec)
+            self._rec = None
+
+
+class RamtrjPlayer:
+    """RamtrjPlayer (src/core/io/RamtrjPlayer.cpp:11-128), opened as IO::loadRamtrjPlayer does
+    (src/frontend/IO.cpp:32-64).  With auto_fix_particles=None the file is read strictly (a length that disagrees
+    with the header is a ValidationError); with the expected particle count a truncated file is repaired in memory
+    (RamtrjPlayer::AutoFix) and `auto_fix` holds the report.  Snapshots come back as (box 3x3, positions n x 3,
+    orientations n x 9)."""
+
+    def __init__(self, path: str, auto_fix_particles: Optional[int] = None):
+        lib = load_library()
+        self._p = ctypes.c_void_p()
+        fix = RamtrjFix()
+        try:
+            _check(lib.rp_ramtrj_player_open(ctypes.byref(self._p), os.fsencode(path),
+                                             int(auto_fix_particles or 0), ctypes.byref(fix)))
+        finally:
+            self.auto_fix = fix.as_dict() if auto_fix_particles else None
+        h = self._header()
+        self.version = (h.version_major, h.version_minor)
+        self.num_particles = h.num_particles
+        self.num_snapshots = h.num_snapshots
+        self.cycle_step = h.cycle_step
+
+    def _header(self) -> RamtrjHeader:
+        if self._p is None:
+            raise PreconditionError(RP_ERR_PRECONDITION, "RAMTRJ player is closed")
+        h = RamtrjHeader()
+        _check(_lib.rp_ramtrj_player_header(self._p, ctypes.byref(h)))
+        return h
+
+    def get_total_cycles(self) -> int:
+        return self.cycle_step * self.num_snapshots
+
+    def get_cycle_step(self) -> int:
+        return self.cycle_step
+
+    def get_num_molecules(self) -> int:
+        return self.num_particles
+
+    def get_current_snapshot_cycles(self) -> int:
+        return self._header().current_snapshot * self.cycle_step
+
+    def has_next(self) -> bool:
+        return self._p is not None and self._header().current_snapshot < self.num_snapshots
+
+    def _buffers(self):
+        n = self.num_particles
+        return np.zeros(9), np.zeros((n, 3)), np.zeros((n, 9))
+
+    def next_snapshot(self):
+        box, pos, rot = self._buffers()
+        if self._p is None:
+            raise PreconditionError(RP_ERR_PRECONDITION, "RAMTRJ player is closed")
+        _check(_lib.rp_ramtrj_next_snapshot(self._p, _dptr(box), len(pos), _dptr(pos), _dptr(rot)))
+        return box.reshape(3, 3), pos, rot
+
+    def jump_to_snapshot(self, cycle: int):
+        box, pos, rot = self._buffers()
+        if self._p is None:
+            raise PreconditionError(RP_ERR_PRECONDITION, "RAMTRJ player is closed")
+        _check(_lib.rp_ramtrj_jump_to_snapshot(self._p, int(cycle), _dptr(box), len(pos), _dptr(pos), _dptr(rot)))
+        return box.reshape(3, 3), pos, rot
+
+    def last_snapshot(self):
+        return self.jump_to_snapshot(self.num_snapshots * self.cycle_step)
+
+    def close(self):
+        if self._p is not None:
+            _lib.rp_ramtrj_player_close(self._p)
+            self._p = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        if getattr(self, "_p", None) is not None and _lib is not None:
+            _lib.rp_ramtrj_player_close(self._p)
+            self._p = None
 
 
 def mean_and_error(samples) -> tuple:

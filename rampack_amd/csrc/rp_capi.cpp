@@ -709,6 +709,14 @@ int rp_ramsnap_save(rp_ctx *c, const char *path, const char *aux_text) {
     return rp_ramsnap_write(path, box.data(), c->n, pos.data(), rot.data(), aux_text);
 }
 
+int rp_ramtrj_record_packing(rp_ramtrj_recorder *r, rp_ctx *c, size_t cycle) {
+    if (!r || !c) return fail(RP_ERR_PRECONDITION, "null argument");
+    std::vector<double> box(9), pos(3 * c->n), rot(9 * c->n);
+    int e = rp_download(c, box.data(), pos.data(), rot.data());
+    if (e) return e;
+    return rp_ramtrj_record(r, box.data(), c->n, pos.data(), rot.data(), cycle);
+}
+
 const char *rp_last_error(void) { return g_lastError.c_str(); }
 
 

@@ -12,6 +12,9 @@
 //   Packing(box, shapes, interaction)                 src/core/Packing.h:154-160
 //   Simulation(packing, moveSamplers, seed, scaler)   src/core/Simulation.h:281-283
 //   Simulation::integrate(T, p, therm, avg, every)    src/core/Simulation.h:321-340
+//   RamtrjRecorder(path, n, cycleStep, append)        src/core/io/RamtrjRecorder.h:30-50 (a file path in place
+//                                                     of the stream, opened as SimulationRecorderFactory.cpp:13-31)
+//   RamtrjPlayer(path[, autoFix])                     src/core/io/RamtrjPlayer.h:19-120
 // Physics plug-ins cannot run per pair across a device boundary, so every plug-in exports a plain descriptor and
 // the whole sweep runs behind the C ABI.  Errors map to the reference's exception types (src/utils/Exceptions.h).
 #pragma once
@@ -227,16 +230,18 @@ class Packing {
     std::vector<Shape> shapes;
     rp_shape_desc shape;
     rp_ctx *ctx = nullptr;
+    uint64_t seed = 0;
     bool stale = false;
 
-    void upload(uint64_t seed) {
+    void upload(uint64_t seed_) {
+        this->seed = seed_;
         std::vector<double> pos(3 * shapes.size()), rot(9 * shapes.size());
         for (std::size_t i = 0; i < shapes.size(); i++) {
             for (int k = 0; k < 3; k++) pos[3 * i + k] = shapes[i].position[k];
             for (int k = 0; k < 9; k++) rot[9 * i + k] = shapes[i].orientation[k];
         }
         rp_ctx *c = nullptr;
-        check(rp_create(&c, &shape, box.data(), shapes.size(), pos.data(), rot.data(), seed, RP_PREC_FP64));
+        check(rp_create(&c, &shape, box.data(), shapes.size(), pos.data(), rot.data(), this->seed, RP_PREC_FP64));
         if (ctx) rp_destroy(ctx);
         ctx = c;
     }
@@ -269,6 +274,14 @@ public:
         }
         self->stale = false;
     }
+    // Packing::reset (src/core/Packing.cpp:67-75): a new state for the same shape, e.g. a trajectory snapshot
+    void reset(std::vector<Shape> newShapes, const Box &newBox) {
+        if (newShapes.empty()) throw PreconditionException("Packing: no shapes");
+        this->shapes = std::move(newShapes);
+        this->box = newBox;
+        this->stale = false;
+        this->upload(this->seed);
+    }
     void markStale() { stale = true; }
     [[nodiscard]] rp_ctx *context() const { return ctx; }
     [[nodiscard]] std::size_t size() const { return shapes.size(); }
@@ -298,6 +311,120 @@ public:
     }
 };
 
+// ---------------------------------------------------------------------------------------------- trajectories
+
+// SimulationRecorder (src/core/SimulationRecorder.h:15-33)
+class SimulationRecorder {
+public:
+    virtual ~SimulationRecorder() = default;
+    virtual void recordSnapshot(const Packing &packing, std::size_t cycle) = 0;
+    [[nodiscard]] virtual std::size_t getLastCycleNumber() const = 0;
+    virtual void close() = 0;
+};
+
+// RamtrjRecorder (src/core/io/RamtrjRecorder.cpp:8-66): snapshots of the device-resident state in RAMTRJ format
+class RamtrjRecorder : public SimulationRecorder {
+    rp_ramtrj_recorder *rec = nullptr;
+
+public:
+    RamtrjRecorder(const std::string &path, std::size_t numParticles, std::size_t cycleStep, bool append) {
+        check(rp_ramtrj_recorder_open(&rec, path.c_str(), numParticles, cycleStep, append ? 1 : 0));
+    }
+    RamtrjRecorder(const RamtrjRecorder &) = delete;
+    RamtrjRecorder &operator=(const RamtrjRecorder &) = delete;
+    ~RamtrjRecorder() override {
+        if (rec) rp_ramtrj_recorder_close(rec);
+    }
+    void recordSnapshot(const Packing &packing, std::size_t cycle) override {
+        if (!rec) throw PreconditionException("RAMTRJ recorder is closed");
+        check(rp_ramtrj_record_packing(rec, packing.context(), cycle));
+    }
+    [[nodiscard]] std::size_t getLastCycleNumber() const override {
+        std::size_t c = 0;
+        if (rec) check(rp_ramtrj_recorder_last_cycle(rec, &c));
+        return c;
+    }
+    void close() override {
+        if (!rec) return;
+        rp_ramtrj_recorder *r = rec;
+        rec = nullptr;
+        check(rp_ramtrj_recorder_close(r));
+    }
+};
+
+// RamtrjPlayer (src/core/io/RamtrjPlayer.cpp:11-128); AutoFix as RamtrjPlayer::AutoFix (RamtrjPlayer.h:29-77)
+class RamtrjPlayer {
+    rp_ramtrj_player *player = nullptr;
+    rp_ramtrj_header header{};
+
+    void open(const std::string &path, std::size_t expected, rp_ramtrj_fix *fix) {
+        check(rp_ramtrj_player_open(&player, path.c_str(), expected, fix));
+        check(rp_ramtrj_player_header(player, &header));
+    }
+    void load(Packing &packing, int (*read)(RamtrjPlayer &, Box &, std::vector<double> &, std::vector<double> &)) {
+        if (packing.size() != header.num_particles) throw PreconditionException("RAMTRJ player: packing size");
+        Box box{};
+        std::vector<double> pos(3 * packing.size()), rot(9 * packing.size());
+        check(read(*this, box, pos, rot));
+        std::vector<Shape> shapes(packing.size());
+        for (std::size_t i = 0; i < shapes.size(); i++) {
+            for (int k = 0; k < 3; k++) shapes[i].position[k] = pos[3 * i + k];
+            for (int k = 0; k < 9; k++) shapes[i].orientation[k] = rot[9 * i + k];
+        }
+        packing.reset(std::move(shapes), box);
+        check(rp_ramtrj_player_header(player, &header));
+    }
+
+public:
+    class AutoFix {
+        std::size_t expected;
+        rp_ramtrj_fix fix{};
+        friend class RamtrjPlayer;
+
+    public:
+        explicit AutoFix(std::size_t expectedNumMolecules) : expected{expectedNumMolecules} {
+            if (expected == 0) throw PreconditionException("AutoFix: expected number of molecules must be > 0");
+        }
+        [[nodiscard]] bool wasFixingSuccessful() const { return fix.fixing_successful != 0; }
+        [[nodiscard]] bool wasFixingNeeded() const { return fix.fixing_needed != 0; }
+        [[nodiscard]] std::size_t getInferredSnapshots() const { return fix.inferred_snapshots; }
+        [[nodiscard]] std::size_t getBytesRemainder() const { return fix.bytes_remainder; }
+    };
+
+    explicit RamtrjPlayer(const std::string &path) { open(path, 0, nullptr); }
+    RamtrjPlayer(const std::string &path, AutoFix &autoFix) { open(path, autoFix.expected, &autoFix.fix); }
+    RamtrjPlayer(const RamtrjPlayer &) = delete;
+    RamtrjPlayer &operator=(const RamtrjPlayer &) = delete;
+    ~RamtrjPlayer() { close(); }
+
+    [[nodiscard]] bool hasNext() const { return player && header.current_snapshot < header.num_snapshots; }
+    void nextSnapshot(Packing &packing) {
+        if (!player) throw PreconditionException("RAMTRJ player is closed");
+        load(packing, [](RamtrjPlayer &p, Box &b, std::vector<double> &x, std::vector<double> &r) {
+            return rp_ramtrj_next_snapshot(p.player, b.data(), x.size() / 3, x.data(), r.data());
+        });
+    }
+    void jumpToSnapshot(Packing &packing, std::size_t cycleNumber) {
+        if (!player) throw PreconditionException("RAMTRJ player is closed");
+        this->jumpCycle = cycleNumber;
+        load(packing, [](RamtrjPlayer &p, Box &b, std::vector<double> &x, std::vector<double> &r) {
+            return rp_ramtrj_jump_to_snapshot(p.player, p.jumpCycle, b.data(), x.size() / 3, x.data(), r.data());
+        });
+    }
+    void lastSnapshot(Packing &packing) { jumpToSnapshot(packing, header.num_snapshots * header.cycle_step); }
+    [[nodiscard]] std::size_t getCurrentSnapshotCycles() const { return header.current_snapshot * header.cycle_step; }
+    [[nodiscard]] std::size_t getTotalCycles() const { return header.cycle_step * header.num_snapshots; }
+    [[nodiscard]] std::size_t getCycleStep() const { return header.cycle_step; }
+    [[nodiscard]] std::size_t getNumMolecules() const { return header.num_particles; }
+    void close() {
+        if (player) rp_ramtrj_player_close(player);
+        player = nullptr;
+    }
+
+private:
+    std::size_t jumpCycle = 0;
+};
+
 // ---------------------------------------------------------------------------------------------- simulation
 
 class Simulation {
@@ -319,9 +446,13 @@ public:
 
     // Simulation::integrate (Simulation.cpp:98-199): thermalisation with step adaptation, then averaging; the
     // observer is called every `averagingEvery` averaging cycles with the current packing.
+    // With snapshotEvery > 0 every recorder takes a snapshot whenever the cycle count is a multiple of it, in both
+    // phases (Simulation.cpp:146-150, 175-179).
     void integrate(double temperature_, double pressure_, std::size_t thermalisationCycles,
                    std::size_t averagingCycles, std::size_t averagingEvery,
-                   const std::function<void(const Packing &, std::size_t)> &observer = {}) {
+                   const std::function<void(const Packing &, std::size_t)> &observer = {},
+                   std::size_t snapshotEvery = 0,
+                   const std::vector<std::unique_ptr<SimulationRecorder>> &recorders = {}) {
         if (thermalisationCycles == 0 && averagingCycles == 0) throw PreconditionException("no cycles");
         if (averagingCycles > 0 && (averagingEvery == 0 || averagingEvery > averagingCycles))
             throw PreconditionException("averagingEvery");
@@ -331,11 +462,21 @@ public:
         rp_box_desc bd{RP_BOX_NONE, 0};
         if (this->boxScaler) bd = this->boxScaler->descriptor();
         check(rp_set_environment(this->packing->context(), temperature_, pressure_, &md, &bd));
+        auto advance = [&](std::size_t cycles, bool adapt) {
+            while (cycles > 0) {
+                std::size_t chunk = cycles;
+                if (snapshotEvery) chunk = std::min(chunk, snapshotEvery - this->counters.cycles % snapshotEvery);
+                this->run(chunk, adapt);
+                cycles -= chunk;
+                if (snapshotEvery && this->counters.cycles % snapshotEvery == 0)
+                    for (const auto &recorder : recorders) recorder->recordSnapshot(*this->packing, this->counters.cycles);
+            }
+        };
         if (thermalisationCycles > 0)
-            this->run(thermalisationCycles, true);
+            advance(thermalisationCycles, true);
         for (std::size_t done = 0; done < averagingCycles; done += averagingEvery) {
             std::size_t chunk = std::min(averagingEvery, averagingCycles - done);
-            this->run(chunk, false);
+            advance(chunk, false);
             if (observer && chunk == averagingEvery) observer(*this->packing, this->counters.cycles);
         }
     }

@@ -3,7 +3,9 @@
 // the packing back.  Needs a GPU; prints one line per check and exits non-zero on the first failure.
 #include "rampack_gpu.hpp"
 
+#include <algorithm>
 #include <cstdio>
+#include <string>
 #include <cstdlib>
 
 using namespace rampack_amd;
@@ -64,6 +66,46 @@ int main() {
     CHECK(phis.back() > 0.34 && phis.back() < 0.42);  // a hard-sphere fluid at p = 5 (CS: 0.393)
     CHECK(sim.getCounters().box_accepted > 0);
     CHECK(sim.getMoveAcceptanceRate() > 0.05 && sim.getMoveAcceptanceRate() < 0.6);
+
+    // a RAMTRJ recording of a run, replayed into another packing (SimulationIOTest.cpp:37-98)
+    {
+        std::string path = "/tmp/rampack_amd_facade_test.ramtrj";
+        auto p2 = std::make_unique<Packing>(box, shapes, traits.getInteraction());
+        std::vector<std::unique_ptr<MoveSampler>> s3;
+        s3.push_back(std::make_unique<RototranslationSampler>(0.1, std::nullopt));
+        Simulation recSim(std::move(p2), std::move(s3), 99,
+                          std::make_unique<TriclinicAdapter>(std::make_unique<LogIsotropicVolumeScaler>(), 1e-4));
+        std::vector<std::unique_ptr<SimulationRecorder>> recorders;
+        recorders.push_back(std::make_unique<RamtrjRecorder>(path, shapes.size(), 100, false));
+        recSim.integrate(1.0, 5.0, 250, 250, 50, {}, 100, recorders);
+        CHECK(recorders.front()->getLastCycleNumber() == 500);
+        recorders.front()->close();
+        Packing replay(box, shapes, traits.getInteraction());
+        RamtrjPlayer::AutoFix autoFix(shapes.size());
+        RamtrjPlayer player(path, autoFix);
+        CHECK(!autoFix.wasFixingNeeded() && autoFix.wasFixingSuccessful());
+        CHECK(player.getTotalCycles() == 500 && player.getCycleStep() == 100);
+        int snapshots = 0;
+        while (player.hasNext()) {
+            player.nextSnapshot(replay);
+            snapshots++;
+        }
+        CHECK(snapshots == 5);
+        const Packing &live = recSim.getPacking();
+        double worst = 0;
+        bool exact = live.getBox() == replay.getBox();
+        for (std::size_t i = 0; i < live.size(); i++) {
+            exact = exact && live[i].position == replay[i].position;
+            for (int k = 0; k < 9; k++)
+                worst = std::max(worst, std::abs(live[i].orientation[k] - replay[i].orientation[k]));
+        }
+        CHECK(exact);
+        CHECK(worst < 1e-12);
+        player.jumpToSnapshot(replay, 200);
+        CHECK(player.getCurrentSnapshotCycles() == 200);
+        CHECK(throws<PreconditionException>([&] { player.jumpToSnapshot(replay, 250); }));
+        std::remove(path.c_str());
+    }
 
     // overlapping start: Simulation::integrate refuses it (Simulation.cpp:131-132)
     std::vector<Shape> bad(2);

@@ -126,6 +126,11 @@ def _setup_ref(lib):
     lib.ref_ramsnap_read.restype = ctypes.c_long
     lib.ref_nematic_p2.argtypes = [ctypes.c_size_t, _dp]
     lib.ref_nematic_p2.restype = ctypes.c_double
+    lib.ref_ramtrj_write.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t, _dp, _dp,
+                                     _dp, ctypes.c_int, ctypes.c_size_t]
+    lib.ref_ramtrj_read.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t,
+                                    ctypes.c_size_t, _dp, _dp, _dp, _u64p]
+    lib.ref_ramtrj_read.restype = ctypes.c_long
     lib.ref_packing_fraction.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, ctypes.c_size_t]
     lib.ref_packing_fraction.restype = ctypes.c_double
     lib.ref_mc_run.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double,
@@ -176,3 +181,28 @@ def ref_mc_run(checker: Checker, kind, radius, length, box, pos, rot, temperatur
         raise RuntimeError(f"ref_mc_run failed ({r})")
     return dict(samples=samples[:3 * r].reshape(-1, 3), box=obox.reshape(3, 3), pos=opos, rot=orot,
                 seconds=secs.value, steps=steps)
+
+
+def ref_ramtrj_write(checker: Checker, path, cycle_step, boxes, pos, rot, append=False, first_cycle=None):
+    """The reference's RamtrjRecorder: snapshots k of (boxes[k], pos[k], rot[k]) at cycles first_cycle + k * step
+    (first_cycle defaults to cycle_step); returns the harness status (0 ok, -1 exception)."""
+    boxes = np.ascontiguousarray(boxes, dtype=np.float64).reshape(-1, 9)
+    pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(len(boxes), -1, 3)
+    rot = np.ascontiguousarray(rot, dtype=np.float64).reshape(len(boxes), -1, 9)
+    first = cycle_step if first_cycle is None else first_cycle
+    return checker.lib.ref_ramtrj_write(os.fsencode(str(path)), pos.shape[1], cycle_step, len(boxes),
+                                        boxes.ctypes.data_as(_dp), pos.ctypes.data_as(_dp), rot.ctypes.data_as(_dp),
+                                        int(append), first)
+
+
+def ref_ramtrj_read(checker: Checker, path, n, max_snap, expected_n=0, jump_cycle=0):
+    """The reference's RamtrjPlayer (AutoFix when expected_n > 0): (count or -1, boxes, pos, rot, info) with info =
+    particles, snapshots, cycle step, total cycles, fixing needed, fixing successful, inferred, remainder."""
+    boxes = np.zeros((max(max_snap, 1), 9))
+    pos = np.zeros((max(max_snap, 1), n, 3))
+    rot = np.zeros((max(max_snap, 1), n, 9))
+    info = np.zeros(8, dtype=np.uint64)
+    got = checker.lib.ref_ramtrj_read(os.fsencode(str(path)), expected_n, jump_cycle, n, max_snap,
+                                      boxes.ctypes.data_as(_dp), pos.ctypes.data_as(_dp), rot.ctypes.data_as(_dp),
+                                      info.ctypes.data_as(_u64p))
+    return got, boxes[:max(got, 0)], pos[:max(got, 0)], rot[:max(got, 0)], info

@@ -113,3 +113,32 @@ def test_ramsnap_save_of_the_device_state_reads_back_exactly(tmp_path):
     assert np.array_equal(b, b2) and np.array_equal(p, p2) and np.array_equal(r, r2)
     assert aux == {"cycles": "20"}
     pk.close()
+
+
+def test_ramtrj_recording_of_a_running_packing_replays_it(tmp_path):
+    """RamtrjRecorder::recordSnapshot on the device-resident state every 10 cycles of an NpT run (as Simulation
+    records, Simulation.cpp:146-150): the file is byte-identical to the reference's recorder given the downloaded
+    states, and the replay matches each state to 1e-12 (SimulationIOTest.cpp:22-34), positions and boxes exactly."""
+    box, pos, rot = lattice.spherocylinder_lattice(n_side=(8, 8, 3), spacing=(1.6, 1.6, 6.4))
+    pk = ra.GpuPacking(SC, box, pos, rot, seed=3)
+    pk.set_environment(1.0, 2.0, 0.2, None, box_move=("delta_triclinic", 0.002))
+    path = tmp_path / "run.ramtrj"
+    states = []
+    with ra.RamtrjRecorder(str(path), len(pos), 10) as rec:
+        for k in range(1, 5):
+            pk.run_cycles(10)
+            rec.record_snapshot(pk, 10 * k)
+            states.append(pk.download())
+        assert rec.get_last_cycle_number() == 40
+    with ra.RamtrjPlayer(str(path)) as pl:
+        assert pl.get_total_cycles() == 40 and pl.num_snapshots == 4
+        for b, p, r in states:
+            b2, p2, r2 = pl.next_snapshot()
+            assert np.array_equal(b, b2) and np.array_equal(p, p2)
+            np.testing.assert_allclose(r2, r, atol=1e-12, rtol=0)
+    if oracle_lib.ref_available():
+        theirs = tmp_path / "ref.ramtrj"
+        assert oracle_lib.ref_ramtrj_write(oracle_lib.ref(), theirs, 10, np.array([s[0] for s in states]),
+                                           np.array([s[1] for s in states]), np.array([s[2] for s in states])) == 0
+        assert path.read_bytes() == theirs.read_bytes()
+    pk.close()
