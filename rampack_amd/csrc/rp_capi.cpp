@@ -95,6 +95,8 @@ struct rp_ctx {
     long long numOverlaps = 0;           // cached overlap count in counting mode (Packing::getCachedNumberOfOverlaps)
     unsigned long long *dCounters = nullptr;
     unsigned long long *dPhases = nullptr;  // sweep phase timers (RAMPACK_SWEEP_PHASES=1)
+    std::string tracePath;                  // per-workgroup trace of one sweep launch (RAMPACK_SWEEP_TRACE)
+    uint64_t traceLaunch = 0, sweepLaunches = 0;
     unsigned long long *hCounters = nullptr;  // pinned
     uint64_t seed = 0;
     uint64_t cycle = 0;
@@ -557,6 +559,43 @@ int prepareCycleGrid(rp_ctx *c) {
 }
 
 // One cycle: 8 colour passes in random order (N trial moves in total), then one box move.
+// One sweep launch with the per-workgroup trace on, written to the context's trace file as lines of
+// "block start_us end_us own stencil cell cu" (times from the first workgroup's start).
+int traceSweep(rp_ctx *c, SweepParams sp, int waves) {
+    const size_t blocks = (size_t)sp.na[0] * sp.na[1] * sp.na[2];
+    unsigned long long *d = nullptr;
+    RP_HIP(hipMalloc(&d, sizeof(unsigned long long) * 4 * blocks));
+    std::vector<unsigned long long> h(4 * blocks);
+    int r = RP_OK;
+    if (launch_fill_u64(c->stream, d, 4 * blocks, 0) != hipSuccess) r = fail(RP_ERR_DEVICE, "trace fill failed");
+    sp.trace = d;
+    if (!r && launch_sweep(c->stream, sp, waves) != hipSuccess) r = fail(RP_ERR_DEVICE, "traced sweep failed");
+    if (!r && (hipMemcpyAsync(h.data(), d, sizeof(unsigned long long) * 4 * blocks, hipMemcpyDeviceToHost,
+                              c->stream) != hipSuccess ||
+               hipStreamSynchronize(c->stream) != hipSuccess))
+        r = fail(RP_ERR_DEVICE, "trace copy failed");
+    hipFree(d);
+    if (r) return r;
+    int khz = 100000, dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0)
+        khz = 100000;
+    unsigned long long first = ~0ull;
+    for (size_t i = 0; i < blocks; i++)
+        if (h[4 * i]) first = std::min(first, h[4 * i]);
+    FILE *f = std::fopen(c->tracePath.c_str(), "w");
+    if (!f) return fail(RP_ERR_PRECONDITION, "cannot write " + c->tracePath);
+    const double perUs = khz / 1000.0;
+    for (size_t i = 0; i < blocks; i++) {
+        if (!h[4 * i]) continue;  // no particle in the cell
+        std::fprintf(f, "%zu %.2f %.2f %llu %llu %llu %llu\n", i, (h[4 * i] - first) / perUs,
+                     (h[4 * i + 1] - first) / perUs, h[4 * i + 2] & 0xffff, (h[4 * i + 2] >> 16) & 0xffff,
+                     h[4 * i + 2] >> 32, h[4 * i + 3]);
+    }
+    std::fclose(f);
+    return RP_OK;
+}
+
 int runCycle(rp_ctx *c) {
     int r = prepareCycleGrid(c);
     if (r) return r;
@@ -619,7 +658,13 @@ int runCycle(rp_ctx *c) {
         const int par[3] = {sp.colour % g.period, (sp.colour / g.period) % g.period, sp.colour / (g.period * g.period)};
         if ((g.n[0] == 1 && par[0]) || (g.n[1] == 1 && par[1]) || (g.n[2] == 1 && par[2])) continue;
         timingBegin(c, 0);
-        RP_HIP(launch_sweep(c->stream, sp, waves));
+        const bool trace = !c->tracePath.empty() && ++c->sweepLaunches == c->traceLaunch;
+        if (trace) {
+            r = traceSweep(c, sp, waves);
+            if (r) return r;
+        } else {
+            RP_HIP(launch_sweep(c->stream, sp, waves));
+        }
         timingEnd(c);
     }
     c->sortedForCycle = false;
@@ -758,6 +803,15 @@ int rp_create(rp_ctx **out, const rp_shape_desc *shape, const double box[9], siz
     if (const char *w = std::getenv("RAMPACK_SWEEP_WAVES")) c->sweepWaves = std::atoi(w);
     if (const char *p = std::getenv("RAMPACK_COLOURING")) c->colourPeriod = std::atoi(p);
     if (const char *p = std::getenv("RAMPACK_SWEEP_PIPELINE")) c->pipelined = std::atoi(p);
+    if (const char *tr = std::getenv("RAMPACK_SWEEP_TRACE")) {
+        // path:launch -- the workgroups of the launch-th sweep launch of this context, one line each
+        std::string v(tr);
+        size_t colon = v.rfind(':');
+        if (colon != std::string::npos && colon > 0) {
+            c->tracePath = v.substr(0, colon);
+            c->traceLaunch = std::strtoull(v.c_str() + colon + 1, nullptr, 10);
+        }
+    }
     if (const char *ph = std::getenv("RAMPACK_SWEEP_PHASES"); ph && std::atoi(ph) != 0) {
         unsigned long long init[RP_PHASE_NUM] = {};
         if (hipMalloc(&c->dPhases, sizeof(unsigned long long) * RP_PHASE_NUM) != hipSuccess ||
@@ -1044,6 +1098,14 @@ void rp_destroy(rp_ctx *c) {
                          (unsigned long long)ph[RP_PHASE_BLOCKS], ph[RP_PHASE_LAST] / perUs, khz,
                          sweep_occupancy(c->shape.kind, c->tstats.sweep_cap ? (int)c->tstats.sweep_cap : 64,
                                          c->tstats.sweep_waves ? (int)c->tstats.sweep_waves : 4));
+            if (ph[RP_PHASE_GROUPS]) {
+                const double ng = (double)ph[RP_PHASE_GROUPS];
+                std::fprintf(stderr,
+                             "grouped sweep per group (us): planner busy %.3f, tester busy %.3f, barrier to end %.3f; "
+                             "%.2f groups per workgroup, %.1f %% replanned\n",
+                             ph[RP_PHASE_PLAN] / ng / perUs, ph[RP_PHASE_SCAN] / ng / perUs,
+                             ph[RP_PHASE_DECIDE] / ng / perUs, ng / b, 100.0 * ph[RP_PHASE_REPLANS] / ng);
+            }
         }
         hipFree(c->dPhases);
     }

@@ -63,6 +63,8 @@ struct SweepParams {
                             // translation, orientation fix_rotation(O) instead of R O (Packing::renormalizeOrientations)
     unsigned long long *counters;
     unsigned long long *phases;  // optional per-phase wall-clock sums (RP_PHASE_*), nullptr when not profiling
+    unsigned long long *trace;   // optional per-workgroup record of one launch: start and end wall clock,
+                                 // own | stencil << 16 | cell << 32, compute-unit id; nullptr when not tracing
 };
 
 // Phases of a sweep workgroup timed with the device wall clock when SweepParams::phases is set.
@@ -73,7 +75,12 @@ enum SweepPhase : int {
     RP_PHASE_WRITE = 3,    // write-back and counters
     RP_PHASE_BLOCKS = 4,   // workgroups timed
     RP_PHASE_LAST = 6,     // longest workgroup (max)
-    RP_PHASE_NUM = 8
+    RP_PHASE_PLAN = 8,     // grouped sweep: planner wavefront busy (planning the next group)
+    RP_PHASE_SCAN = 9,     // grouped sweep: first tester wavefront busy (stencil scan and segment tests)
+    RP_PHASE_DECIDE = 10,  // grouped sweep: group barrier to the end of the group (decisions, write-back, replans)
+    RP_PHASE_GROUPS = 11,  // grouped sweep: groups
+    RP_PHASE_REPLANS = 12, // grouped sweep: groups planned twice
+    RP_PHASE_NUM = 16
 };
 
 enum Counter : int {

@@ -1369,10 +1369,12 @@ __device__ __forceinline__ void plan_group(const SweepParams &P, const Stencil &
             if (hard_pair_hit<KIND>(P.k, f, f.pos, p2, a2)) hit = 1u << (4 * kk + ii);
         }
     }
-    for (int off = 1; off < 8; off <<= 1) hit |= (unsigned)__shfl_xor((int)hit, off);
+    // lane l < 6 tests pair (kk, ii) = (1,0) (2,0) (2,1) (3,0) (3,1) (3,2): bits 4, 8, 9, 12, 13, 14
+    const unsigned long long hm = __ballot(hit != 0);
+    const unsigned pairs = (unsigned)(((hm & 1) << 4) | ((hm & 6) << 7) | ((hm & 56) << 9));
     if (lane == 0) {
         planI[4] = gs;
-        planI[5] = (int)hit;
+        planI[5] = (int)pairs;
         planI[6] = (int)word;
     }
 }
@@ -1383,13 +1385,19 @@ __device__ __forceinline__ void sweep_cell_grouped(const SweepParams &P, const S
                                                    int *plansI, int *hitWords, int own0, int nOwn, int total, int cell,
                                                    unsigned long long *clk) {
     constexpr int NTT = WAVE * (WAVES - 1);
+    // a tester's queue: entries [0, GQ - WAVE), then one discard slot per lane for lanes with nothing to queue; a
+    // slot adds at most GROUP * WAVE entries, so the queue is drained once it holds more than QFLUSH
+    constexpr int QFLUSH = GQ - WAVE - GROUP * WAVE;
+    static_assert(QFLUSH > 0, "grouped queue");
     const int tid = threadIdx.x;
     const int lane = tid & (WAVE - 1);
     const int wave = tid / WAVE;
     const bool planner = wave == 0;
     const int ttid = tid - WAVE;
     int *myQueue = queue + (wave > 0 ? wave - 1 : 0) * GQ;
+    int *discard = myQueue + (GQ - WAVE) + lane;
     unsigned long long accepted = 0, leftCell = 0, tests = 0;
+    unsigned long long tBusy = 0, tDecide = 0, nGroups = 0, nReplans = 0;  // phase timers (clk set)
     const int moves = P.moves > 0 ? P.moves : nOwn;
     const int pbase = P.moves > 0 ? cell * P.moves : own0;
 
@@ -1399,25 +1407,33 @@ __device__ __forceinline__ void sweep_cell_grouped(const SweepParams &P, const S
         if (tid < GBUF) hitWords[tid] = 0;
         __syncthreads();
         int m0 = 0, gidx = 0;
-        int gs = group_size(pk, 0, cn);
+        // group sizes, plan words and particle indices are the same in every lane: kept in scalar registers
+        int gs = __builtin_amdgcn_readfirstlane(group_size(pk, 0, cn));
         if (planner) plan_group<KIND, TRANSLATED>(P, S, oo, pt, pR, pk, sT, 0, gs, cell, plans, plansI);
         __syncthreads();
         while (m0 < cn) {
             const int buf = gidx % GBUF, nbuf = (gidx + 1) % GBUF;
             const double *plan = plans + buf * GROUP * PLAN_D;
             const int *planI = plansI + buf * GPLAN_I;
-            // this group's plan in registers
             int kIdx[GROUP];
             V3 npos[GROUP];
 #pragma unroll
             for (int k = 0; k < GROUP; k++) {
-                kIdx[k] = k < gs ? planI[k] : -1;
+                kIdx[k] = k < gs ? __builtin_amdgcn_readfirstlane(planI[k]) : -1;
                 npos[k] = k < gs ? v3(plan[k * PLAN_D], plan[k * PLAN_D + 1], plan[k * PLAN_D + 2]) : v3(0, 0, 0);
             }
-            const unsigned newHit = (unsigned)planI[5];
-            const unsigned word = (unsigned)planI[6];  // 4 bits per move: nw | inside << 3
+            const unsigned newHit = (unsigned)__builtin_amdgcn_readfirstlane(planI[5]);
+            const unsigned word = (unsigned)__builtin_amdgcn_readfirstlane(planI[6]);  // nw | inside << 3 per move
             const int m1 = m0 + gs;
-            const int gsNext = m1 < cn ? group_size(pk, m1, cn) : 0;
+            const int gsNext = m1 < cn ? __builtin_amdgcn_readfirstlane(group_size(pk, m1, cn)) : 0;
+            // bit k: group particle k is also moved by the next group, whose plan then needs its new pose
+            unsigned nextUses = 0;
+            for (int t = 0; t < gsNext; t++) {
+                const int kn = __builtin_amdgcn_readfirstlane(pk[m1 + t]);
+#pragma unroll
+                for (int k = 0; k < GROUP; k++) nextUses |= kn == kIdx[k] ? 1u << k : 0u;
+            }
+            const unsigned long long tg0 = clk ? wall_clock64() : 0;
             if (planner) {
                 if (gsNext)
                     plan_group<KIND, TRANSLATED>(P, S, oo, pt, pR, pk, sT, m1, gsNext, cell,
@@ -1425,7 +1441,8 @@ __device__ __forceinline__ void sweep_cell_grouped(const SweepParams &P, const S
                 if (lane == 0) hitWords[nbuf] = 0;
             } else {
                 // 5 bits per move: bits 0-3 overlap with group particle i's current pose, bit 4 with a particle
-                // outside the group
+                // outside the group.  Each (candidate, move) pair is evaluated without branching on the lane: the
+                // ballot of the ambiguous pairs is their compare mask, and lanes without one write a discard slot.
                 unsigned hitAll = 0;
                 int queued = 0;
                 const int slots = (total + NTT - 1) / NTT;
@@ -1437,39 +1454,41 @@ __device__ __forceinline__ void sweep_cell_grouped(const SweepParams &P, const S
                     const int code = TRANSLATED ? 0 : S.code[jj];
                     unsigned groupBit = 16;
 #pragma unroll
-                    for (int i = 0; i < GROUP; i++)
-                        if (jj == kIdx[i]) groupBit = 1u << i;
+                    for (int i = 0; i < GROUP; i++) groupBit = jj == kIdx[i] ? 1u << i : groupBit;
 #pragma unroll
                     for (int k = 0; k < GROUP; k++) {
-                        if (k < gs) {
-                            const unsigned wk = (word >> (4 * k)) & 15;
-                            bool amb = false;
-                            if (valid && (wk & 8) && jj != kIdx[k]) {
-                                V3 q2 = p2;
-                                if (!TRANSLATED)
-                                    q2 = add(p2, image_translation(sT, code, wk & 1, (wk >> 1) & 1, (wk >> 2) & 1));
-                                V3 d = sub(q2, npos[k]);
-                                double d2 = dot(d, d);
-                                if (d2 < P.k.contact2) hitAll |= groupBit << (5 * k);
-                                else if (KIND == 1 && d2 < P.k.range2) amb = true;
+                        if (k < gs && ((word >> (4 * k + 3)) & 1)) {  // moves leaving the cell are rejected
+                            V3 q2 = p2;
+                            if (!TRANSLATED) {
+                                const unsigned wk = (word >> (4 * k)) & 7;
+                                q2 = add(p2, image_translation(sT, code, wk & 1, (wk >> 1) & 1, (wk >> 2) & 1));
                             }
+                            const V3 d = sub(q2, npos[k]);
+                            const double d2 = dot(d, d);
+                            const bool other = valid && jj != kIdx[k];
+                            const bool close = d2 < P.k.contact2;
+                            hitAll |= other && close ? groupBit << (5 * k) : 0u;
                             if (KIND == 1) {
-                                unsigned long long m = __ballot(amb);
-                                if (amb) myQueue[queued + __popcll(m & ((1ull << lane) - 1))] = (jj << 2) | k;
+                                const bool amb = other && !close && d2 < P.k.range2;
+                                const unsigned long long m = __ballot(amb);
+                                const int at = queued + (int)__builtin_amdgcn_mbcnt_hi(
+                                                            (unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+                                *(amb ? myQueue + at : discard) = (jj << 2) | k;
                                 queued += __popcll(m);
                             }
                         }
                     }
-                    if (KIND == 1 && (queued > GQ - GROUP * WAVE || s + 1 == slots)) {
+                    if (KIND == 1 && queued > 0 && (queued > QFLUSH || s + 1 == slots)) {
                         // exact segment tests of the queued pairs (pass B)
                         __builtin_amdgcn_wave_barrier();  // the queue was written by other lanes of this wavefront
                         for (int q = lane; q < queued; q += WAVE) {
                             const int e = myQueue[q];
                             const int jq = e >> 2, k = e & 3;
-                            const unsigned wk = (word >> (4 * k)) & 15;
                             V3 c2 = v3(S.x[jq], S.y[jq], S.z[jq]);
-                            if (!TRANSLATED)
+                            if (!TRANSLATED) {
+                                const unsigned wk = (word >> (4 * k)) & 7;
                                 c2 = add(c2, image_translation(sT, S.code[jq], wk & 1, (wk >> 1) & 1, (wk >> 2) & 1));
+                            }
                             const V3 a2 = v3(S.ax[jq], S.ay[jq], S.az[jq]);
                             const SCFirst f = plan_first(plan + k * PLAN_D);
                             const V3 c2a = add(c2, scal(a2, -P.k.halfLen));
@@ -1477,8 +1496,7 @@ __device__ __forceinline__ void sweep_cell_grouped(const SweepParams &P, const S
                             if (segment_distance2(f.capA, f.u, f.a, c2a, c2b) < P.k.contact2) {
                                 unsigned bit = 16;
 #pragma unroll
-                                for (int i = 0; i < GROUP; i++)
-                                    if (jq == kIdx[i]) bit = 1u << i;
+                                for (int i = 0; i < GROUP; i++) bit = jq == kIdx[i] ? 1u << i : bit;
                                 hitAll |= bit << (5 * k);
                             }
                         }
@@ -1486,45 +1504,47 @@ __device__ __forceinline__ void sweep_cell_grouped(const SweepParams &P, const S
                         queued = 0;
                     }
                 }
-                for (int off = 1; off < WAVE; off <<= 1) hitAll |= (unsigned)__shfl_xor((int)hitAll, off);
-                if (lane == 0 && hitAll) atomicOr(&hitWords[buf], (int)hitAll);
+                if (hitAll) atomicOr(&hitWords[buf], (int)hitAll);
             }
+            const unsigned long long tg1 = clk ? wall_clock64() : 0;
             __syncthreads();  // the group's only barrier: hit words and the next plan are complete
-            // decide the group's moves in order
-            const unsigned hw = (unsigned)hitWords[buf];
-            unsigned acc = 0;
+            const unsigned long long tg2 = clk ? wall_clock64() : 0;
+            // decide the group's moves in order (uniform: every lane of every wavefront reaches the same decisions)
+            // move k is rejected if it leaves the cell, overlaps a particle outside the group, overlaps the old
+            // pose of a group particle i not moved before it (bit i of the stencil hits: i > k, or i < k rejected),
+            // or the trial pose of an accepted move i < k (bit i of the planner's trial-pair hits); bit k of the
+            // stencil hits is never set (a move is not tested against its own particle)
+            const unsigned hw = (unsigned)__builtin_amdgcn_readfirstlane(hitWords[buf]);
+            unsigned acc = 0, insideMask = 0;
             for (int k = 0; k < gs; k++) {
                 const unsigned b = (hw >> (5 * k)) & 31;
-                const bool inside = (word >> (4 * k + 3)) & 1;
-                bool hit = !inside || (b & 16);
-                for (int i = 0; i < gs; i++) {
-                    if (i == k) continue;
-                    if (i < k && ((acc >> i) & 1)) hit |= (newHit >> (4 * k + i)) & 1;
-                    else hit |= (b >> i) & 1;
-                }
-                if (!inside) leftCell++;
-                else tests += (unsigned long long)total;
+                const unsigned inside = (word >> (4 * k + 3)) & 1;
+                const unsigned trialHits = (newHit >> (4 * k)) & 15;
+                const bool hit = !inside || (b & 16) || ((b & 15) & ~acc) || (trialHits & acc);
+                insideMask |= inside << k;
                 if (!hit) acc |= 1u << k;
             }
-            bool replan = false;
+            accepted += (unsigned long long)__popc(acc);
+            leftCell += (unsigned long long)(gs - __popc(insideMask));
+            tests += (unsigned long long)total * (unsigned long long)__popc(insideMask);
+            const bool replan = (acc & nextUses) != 0;
+            // accepted poses into this wavefront's view of the staged copy, one lane per component; LDS accesses of
+            // a wavefront complete in order, so its next scan reads them
             for (int k = 0; k < gs; k++) {
                 if (!((acc >> k) & 1)) continue;
-                accepted++;
-                const int ki = planI[k];
+                const int ki = kIdx[k];
                 const double *q = plan + k * PLAN_D;
-                S.x[ki] = q[0];
-                S.y[ki] = q[1];
-                S.z[ki] = q[2];
-                if (KIND == 1) {
-                    S.ax[ki] = q[3];
-                    S.ay[ki] = q[4];
-                    S.az[ki] = q[5];
+                if (lane < (KIND == 1 ? 6 : 3)) {
+                    double *dst = lane == 0 ? S.x : lane == 1 ? S.y : lane == 2 ? S.z
+                                : lane == 3 ? S.ax : lane == 4 ? S.ay : S.az;
+                    dst[ki] = q[lane];
+                } else if (lane == 6) {
+                    S.code[ki] = wrap_code((word >> (4 * k)) & 7);
+                } else if (planner && lane >= 8 && lane < 17) {
+                    oo[(lane - 8) * OWN_CAP + ki] = q[13 + (lane - 8)];
                 }
-                S.code[ki] = wrap_code((word >> (4 * k)) & 7);
-                if (planner)
-                    for (int c = 0; c < 9; c++) oo[c * OWN_CAP + ki] = q[13 + c];
-                for (int t = 0; t < gsNext; t++) replan |= pk[m1 + t] == ki;
             }
+            __builtin_amdgcn_wave_barrier();
             if (replan) {
                 // the next group was planned from a pose this group just replaced
                 __syncthreads();
@@ -1533,13 +1553,27 @@ __device__ __forceinline__ void sweep_cell_grouped(const SweepParams &P, const S
                                                  plans + nbuf * GROUP * PLAN_D, plansI + nbuf * GPLAN_I);
                 __syncthreads();
             }
+            if (clk) {
+                tBusy += tg1 - tg0;
+                tDecide += wall_clock64() - tg2;
+                nGroups++;
+                nReplans += replan ? 1 : 0;
+            }
             m0 = m1;
             gs = gsNext;
             gidx++;
         }
         __syncthreads();
     }
-    if (clk) clk[3] = wall_clock64();
+    if (clk) {
+        clk[3] = wall_clock64();
+        if (lane == 0 && wave <= 1) atomicAdd(&P.phases[planner ? RP_PHASE_PLAN : RP_PHASE_SCAN], tBusy);
+        if (tid == 0) {
+            atomicAdd(&P.phases[RP_PHASE_DECIDE], tDecide);
+            atomicAdd(&P.phases[RP_PHASE_GROUPS], nGroups);
+            atomicAdd(&P.phases[RP_PHASE_REPLANS], nReplans);
+        }
+    }
     for (int a = tid; a < nOwn; a += WAVE * WAVES) {
         const int gi = own0 + a;
         P.p.x[gi] = S.x[a];
@@ -1768,6 +1802,7 @@ __global__ __launch_bounds__(WAVE *WAVES, WAVES >= 8 ? 4 : 1) void k_sweep(Sweep
     const Grid &g = P.g;
     const int tid = threadIdx.x;
     const int b = blockIdx.x;
+    const unsigned long long t0 = P.trace ? wall_clock64() : 0;
     const int bx = b % P.na[0], by = (b / P.na[0]) % P.na[1], bz = b / (P.na[0] * P.na[1]);
     const int per = g.period;
     const int cx = per * bx + P.colour % per, cy = per * by + (P.colour / per) % per,
@@ -1865,6 +1900,13 @@ __global__ __launch_bounds__(WAVE *WAVES, WAVES >= 8 ? 4 : 1) void k_sweep(Sweep
         atomicAdd(&P.phases[RP_PHASE_WRITE], clk[4] - clk[3]);
         atomicAdd(&P.phases[RP_PHASE_BLOCKS], 1ull);
         atomicMax(&P.phases[RP_PHASE_LAST], clk[4] - clk[0]);
+    }
+    if (P.trace && tid == 0) {
+        unsigned long long *t = P.trace + 4 * b;
+        t[0] = t0;
+        t[1] = wall_clock64();
+        t[2] = (unsigned long long)nOwn | ((unsigned long long)total << 16) | ((unsigned long long)cell << 32);
+        t[3] = (unsigned long long)__smid();
     }
 }
 
