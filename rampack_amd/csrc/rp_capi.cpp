@@ -95,6 +95,7 @@ struct rp_ctx {
     long long numOverlaps = 0;           // cached overlap count in counting mode (Packing::getCachedNumberOfOverlaps)
     unsigned long long *dCounters = nullptr;
     unsigned long long *dPhases = nullptr;  // sweep phase timers (RAMPACK_SWEEP_PHASES=1)
+    int sweepTuning = 0;                    // grouped sweep wave priorities (RAMPACK_SWEEP_TUNING)
     std::string tracePath;                  // per-workgroup trace of one sweep launch (RAMPACK_SWEEP_TRACE)
     uint64_t traceLaunch = 0, sweepLaunches = 0;
     unsigned long long *hCounters = nullptr;  // pinned
@@ -638,6 +639,7 @@ int runCycle(rp_ctx *c) {
     sp.cap = std::max(64, std::min(cap, maxCap));
     sp.counters = c->dCounters;
     sp.phases = c->dPhases;
+    sp.tuning = c->sweepTuning;
     sp.pipelined = c->pipelined >= 0 ? c->pipelined : (waves >= 8 ? 2 : 1);
     sp.temperature = c->temperature;
     sp.counting = c->counting && !c->k.soft ? 1 : 0;
@@ -803,6 +805,7 @@ int rp_create(rp_ctx **out, const rp_shape_desc *shape, const double box[9], siz
     if (const char *w = std::getenv("RAMPACK_SWEEP_WAVES")) c->sweepWaves = std::atoi(w);
     if (const char *p = std::getenv("RAMPACK_COLOURING")) c->colourPeriod = std::atoi(p);
     if (const char *p = std::getenv("RAMPACK_SWEEP_PIPELINE")) c->pipelined = std::atoi(p);
+    if (const char *t = std::getenv("RAMPACK_SWEEP_TUNING")) c->sweepTuning = std::atoi(t);
     if (const char *tr = std::getenv("RAMPACK_SWEEP_TRACE")) {
         // path:launch -- the workgroups of the launch-th sweep launch of this context, one line each
         std::string v(tr);
@@ -1101,10 +1104,12 @@ void rp_destroy(rp_ctx *c) {
             if (ph[RP_PHASE_GROUPS]) {
                 const double ng = (double)ph[RP_PHASE_GROUPS];
                 std::fprintf(stderr,
-                             "grouped sweep per group (us): planner busy %.3f, tester busy %.3f, barrier to end %.3f; "
-                             "%.2f groups per workgroup, %.1f %% replanned\n",
-                             ph[RP_PHASE_PLAN] / ng / perUs, ph[RP_PHASE_SCAN] / ng / perUs,
-                             ph[RP_PHASE_DECIDE] / ng / perUs, ng / b, 100.0 * ph[RP_PHASE_REPLANS] / ng);
+                             "grouped sweep per group (us): loop top %.3f, planner busy %.3f, tester busy %.3f "
+                             "(segment tests %.3f), barrier to end %.3f; %.2f groups per workgroup, %.1f %% replanned\n",
+                             ph[RP_PHASE_TOP] / ng / perUs, ph[RP_PHASE_PLAN] / ng / perUs,
+                             ph[RP_PHASE_SCAN] / ng / perUs, ph[RP_PHASE_PASSB] / ng / perUs,
+                             ph[RP_PHASE_DECIDE] / ng / perUs, ng / b,
+                             100.0 * ph[RP_PHASE_REPLANS] / ng);
             }
         }
         hipFree(c->dPhases);

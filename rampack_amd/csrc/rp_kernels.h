@@ -63,6 +63,8 @@ struct SweepParams {
                             // translation, orientation fix_rotation(O) instead of R O (Packing::renormalizeOrientations)
     unsigned long long *counters;
     unsigned long long *phases;  // optional per-phase wall-clock sums (RP_PHASE_*), nullptr when not profiling
+    int tuning;                  // grouped sweep wave priorities: bit 0 raises the planner, bit 1 the testers
+                                 // that scan one slot more than the others
     unsigned long long *trace;   // optional per-workgroup record of one launch: start and end wall clock,
                                  // own | stencil << 16 | cell << 32, compute-unit id; nullptr when not tracing
 };
@@ -80,6 +82,8 @@ enum SweepPhase : int {
     RP_PHASE_DECIDE = 10,  // grouped sweep: group barrier to the end of the group (decisions, write-back, replans)
     RP_PHASE_GROUPS = 11,  // grouped sweep: groups
     RP_PHASE_REPLANS = 12, // grouped sweep: groups planned twice
+    RP_PHASE_TOP = 13,     // grouped sweep: start of a group to the start of planning / scanning (plan words)
+    RP_PHASE_PASSB = 14,   // grouped sweep: first tester wavefront in the segment tests of queued pairs
     RP_PHASE_NUM = 16
 };
 

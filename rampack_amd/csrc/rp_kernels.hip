@@ -1397,41 +1397,65 @@ __device__ __forceinline__ void sweep_cell_grouped(const SweepParams &P, const S
     int *myQueue = queue + (wave > 0 ? wave - 1 : 0) * GQ;
     int *discard = myQueue + (GQ - WAVE) + lane;
     unsigned long long accepted = 0, leftCell = 0, tests = 0;
-    unsigned long long tBusy = 0, tDecide = 0, nGroups = 0, nReplans = 0;  // phase timers (clk set)
+    unsigned long long tBusy = 0, tDecide = 0, tTop = 0, tPassB = 0, nGroups = 0, nReplans = 0;  // phase timers
     const int moves = P.moves > 0 ? P.moves : nOwn;
     const int pbase = P.moves > 0 ? cell * P.moves : own0;
+    // issue priority: the planner and the testers with an extra slot bound each group's time
+    if ((P.tuning & 1) && planner) __builtin_amdgcn_s_setprio(2);
+    if ((P.tuning & 2) && !planner && (wave - 1) * WAVE < total - ((total - 1) / NTT) * NTT) __builtin_amdgcn_s_setprio(1);
 
     for (int chunk = 0; chunk < moves; chunk += MOVE_CHUNK) {
         const int cn = min(MOVE_CHUNK, moves - chunk);
         load_proposals(P, pbase, chunk, cn, pt, pR, pk);
         if (tid < GBUF) hitWords[tid] = 0;
         __syncthreads();
-        int m0 = 0, gidx = 0;
-        // group sizes, plan words and particle indices are the same in every lane: kept in scalar registers
-        int gs = __builtin_amdgcn_readfirstlane(group_size(pk, 0, cn));
+        // The chunk's groups, in registers of every wavefront: lane m holds the particle of move m and the length
+        // of the run of moves on distinct particles that starts there (at most GROUP); the groups follow from
+        // those runs (group_size), and lane g of `groups` holds group g as start | size << 8.
+        const int pkv = lane < cn ? pk[lane] : -1;
+        const int q1 = __shfl_down(pkv, 1), q2 = __shfl_down(pkv, 2), q3 = __shfl_down(pkv, 3);
+        const bool d1 = lane + 1 < cn && q1 != pkv;
+        const bool d2 = d1 && lane + 2 < cn && q2 != pkv && q2 != q1;
+        const bool d3 = d2 && lane + 3 < cn && q3 != pkv && q3 != q1 && q3 != q2;
+        const int runv = 1 + (int)d1 + (int)d2 + (int)d3;
+        int groups = 0, ngroups = 0;
+        for (int m = 0; m < cn; ngroups++) {
+            const int r = __builtin_amdgcn_readlane(runv, m);
+            groups = lane == ngroups ? m | (r << 8) : groups;
+            m += r;
+        }
+        int gidx = 0;
+        int gs = __builtin_amdgcn_readlane(groups, 0) >> 8;
         if (planner) plan_group<KIND, TRANSLATED>(P, S, oo, pt, pR, pk, sT, 0, gs, cell, plans, plansI);
         __syncthreads();
-        while (m0 < cn) {
+        for (; gidx < ngroups; gidx++) {
+            const unsigned long long tgTop = clk ? wall_clock64() : 0;
             const int buf = gidx % GBUF, nbuf = (gidx + 1) % GBUF;
             const double *plan = plans + buf * GROUP * PLAN_D;
             const int *planI = plansI + buf * GPLAN_I;
+            // group sizes, plan words and particle indices are the same in every lane: kept in scalar registers
+            const int m0 = __builtin_amdgcn_readlane(groups, gidx) & 255;
+            gs = __builtin_amdgcn_readlane(groups, gidx) >> 8;
+            const int m1 = m0 + gs;
+            const int gsNext = gidx + 1 < ngroups ? __builtin_amdgcn_readlane(groups, gidx + 1) >> 8 : 0;
             int kIdx[GROUP];
             V3 npos[GROUP];
 #pragma unroll
             for (int k = 0; k < GROUP; k++) {
-                kIdx[k] = k < gs ? __builtin_amdgcn_readfirstlane(planI[k]) : -1;
+                kIdx[k] = k < gs ? __builtin_amdgcn_readlane(pkv, m0 + k) : -1;
                 npos[k] = k < gs ? v3(plan[k * PLAN_D], plan[k * PLAN_D + 1], plan[k * PLAN_D + 2]) : v3(0, 0, 0);
             }
             const unsigned newHit = (unsigned)__builtin_amdgcn_readfirstlane(planI[5]);
             const unsigned word = (unsigned)__builtin_amdgcn_readfirstlane(planI[6]);  // nw | inside << 3 per move
-            const int m1 = m0 + gs;
-            const int gsNext = m1 < cn ? __builtin_amdgcn_readfirstlane(group_size(pk, m1, cn)) : 0;
             // bit k: group particle k is also moved by the next group, whose plan then needs its new pose
             unsigned nextUses = 0;
-            for (int t = 0; t < gsNext; t++) {
-                const int kn = __builtin_amdgcn_readfirstlane(pk[m1 + t]);
 #pragma unroll
-                for (int k = 0; k < GROUP; k++) nextUses |= kn == kIdx[k] ? 1u << k : 0u;
+            for (int t = 0; t < GROUP; t++) {
+                if (t < gsNext) {
+                    const int kn = __builtin_amdgcn_readlane(pkv, m1 + t);
+#pragma unroll
+                    for (int k = 0; k < GROUP; k++) nextUses |= kn == kIdx[k] ? 1u << k : 0u;
+                }
             }
             const unsigned long long tg0 = clk ? wall_clock64() : 0;
             if (planner) {
@@ -1455,21 +1479,26 @@ __device__ __forceinline__ void sweep_cell_grouped(const SweepParams &P, const S
                     unsigned groupBit = 16;
 #pragma unroll
                     for (int i = 0; i < GROUP; i++) groupBit = jj == kIdx[i] ? 1u << i : groupBit;
+                    // the moves' distances first, as independent chains, then their hit bits and queue entries
+                    double d2[GROUP];
+#pragma unroll
+                    for (int k = 0; k < GROUP; k++) {
+                        V3 q2 = p2;
+                        if (!TRANSLATED) {
+                            const unsigned wk = (word >> (4 * k)) & 7;
+                            q2 = add(p2, image_translation(sT, code, wk & 1, (wk >> 1) & 1, (wk >> 2) & 1));
+                        }
+                        const V3 d = sub(q2, npos[k]);
+                        d2[k] = dot(d, d);
+                    }
 #pragma unroll
                     for (int k = 0; k < GROUP; k++) {
                         if (k < gs && ((word >> (4 * k + 3)) & 1)) {  // moves leaving the cell are rejected
-                            V3 q2 = p2;
-                            if (!TRANSLATED) {
-                                const unsigned wk = (word >> (4 * k)) & 7;
-                                q2 = add(p2, image_translation(sT, code, wk & 1, (wk >> 1) & 1, (wk >> 2) & 1));
-                            }
-                            const V3 d = sub(q2, npos[k]);
-                            const double d2 = dot(d, d);
                             const bool other = valid && jj != kIdx[k];
-                            const bool close = d2 < P.k.contact2;
+                            const bool close = d2[k] < P.k.contact2;
                             hitAll |= other && close ? groupBit << (5 * k) : 0u;
                             if (KIND == 1) {
-                                const bool amb = other && !close && d2 < P.k.range2;
+                                const bool amb = other && !close && d2[k] < P.k.range2;
                                 const unsigned long long m = __ballot(amb);
                                 const int at = queued + (int)__builtin_amdgcn_mbcnt_hi(
                                                             (unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
@@ -1480,6 +1509,7 @@ __device__ __forceinline__ void sweep_cell_grouped(const SweepParams &P, const S
                     }
                     if (KIND == 1 && queued > 0 && (queued > QFLUSH || s + 1 == slots)) {
                         // exact segment tests of the queued pairs (pass B)
+                        const unsigned long long tb0 = clk ? wall_clock64() : 0;
                         __builtin_amdgcn_wave_barrier();  // the queue was written by other lanes of this wavefront
                         for (int q = lane; q < queued; q += WAVE) {
                             const int e = myQueue[q];
@@ -1502,6 +1532,7 @@ __device__ __forceinline__ void sweep_cell_grouped(const SweepParams &P, const S
                         }
                         __builtin_amdgcn_wave_barrier();
                         queued = 0;
+                        if (clk) tPassB += wall_clock64() - tb0;
                     }
                 }
                 if (hitAll) atomicOr(&hitWords[buf], (int)hitAll);
@@ -1515,11 +1546,12 @@ __device__ __forceinline__ void sweep_cell_grouped(const SweepParams &P, const S
             // or the trial pose of an accepted move i < k (bit i of the planner's trial-pair hits); bit k of the
             // stencil hits is never set (a move is not tested against its own particle)
             const unsigned hw = (unsigned)__builtin_amdgcn_readfirstlane(hitWords[buf]);
+            const unsigned word0 = word, newHit0 = newHit;
             unsigned acc = 0, insideMask = 0;
             for (int k = 0; k < gs; k++) {
                 const unsigned b = (hw >> (5 * k)) & 31;
-                const unsigned inside = (word >> (4 * k + 3)) & 1;
-                const unsigned trialHits = (newHit >> (4 * k)) & 15;
+                const unsigned inside = (word0 >> (4 * k + 3)) & 1;
+                const unsigned trialHits = (newHit0 >> (4 * k)) & 15;
                 const bool hit = !inside || (b & 16) || ((b & 15) & ~acc) || (trialHits & acc);
                 insideMask |= inside << k;
                 if (!hit) acc |= 1u << k;
@@ -1539,7 +1571,7 @@ __device__ __forceinline__ void sweep_cell_grouped(const SweepParams &P, const S
                                 : lane == 3 ? S.ax : lane == 4 ? S.ay : S.az;
                     dst[ki] = q[lane];
                 } else if (lane == 6) {
-                    S.code[ki] = wrap_code((word >> (4 * k)) & 7);
+                    S.code[ki] = wrap_code((word0 >> (4 * k)) & 7);
                 } else if (planner && lane >= 8 && lane < 17) {
                     oo[(lane - 8) * OWN_CAP + ki] = q[13 + (lane - 8)];
                 }
@@ -1556,22 +1588,22 @@ __device__ __forceinline__ void sweep_cell_grouped(const SweepParams &P, const S
             if (clk) {
                 tBusy += tg1 - tg0;
                 tDecide += wall_clock64() - tg2;
+                tTop += tg0 - tgTop;
                 nGroups++;
                 nReplans += replan ? 1 : 0;
             }
-            m0 = m1;
-            gs = gsNext;
-            gidx++;
         }
         __syncthreads();
     }
     if (clk) {
         clk[3] = wall_clock64();
         if (lane == 0 && wave <= 1) atomicAdd(&P.phases[planner ? RP_PHASE_PLAN : RP_PHASE_SCAN], tBusy);
+        if (lane == 0 && wave == 1) atomicAdd(&P.phases[RP_PHASE_PASSB], tPassB);
         if (tid == 0) {
             atomicAdd(&P.phases[RP_PHASE_DECIDE], tDecide);
             atomicAdd(&P.phases[RP_PHASE_GROUPS], nGroups);
             atomicAdd(&P.phases[RP_PHASE_REPLANS], nReplans);
+            atomicAdd(&P.phases[RP_PHASE_TOP], tTop);
         }
     }
     for (int a = tid; a < nOwn; a += WAVE * WAVES) {
