@@ -95,6 +95,7 @@ struct rp_ctx {
     long long numOverlaps = 0;           // cached overlap count in counting mode (Packing::getCachedNumberOfOverlaps)
     unsigned long long *dCounters = nullptr;
     unsigned long long *dPhases = nullptr;  // sweep phase timers (RAMPACK_SWEEP_PHASES=1)
+    int device = 0;                         // the HIP device the context was created on
     int sweepTuning = 0;                    // grouped sweep wave priorities (RAMPACK_SWEEP_TUNING)
     std::string tracePath;                  // per-workgroup trace of one sweep launch (RAMPACK_SWEEP_TRACE)
     uint64_t traceLaunch = 0, sweepLaunches = 0;
@@ -799,6 +800,7 @@ int rp_create(rp_ctx **out, const rp_shape_desc *shape, const double box[9], siz
     if (hipGetDevice(&dev) != hipSuccess) return fail(RP_ERR_DEVICE, "no HIP device");
 
     rp_ctx *c = new rp_ctx();
+    c->device = dev;
     int ldsBytes = 0;
     if (hipDeviceGetAttribute(&ldsBytes, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && ldsBytes > 0)
         c->ldsLimit = (size_t)std::min(ldsBytes, 160 * 1024) - 1024;

@@ -1485,7 +1485,9 @@ __device__ __forceinline__ void sweep_cell_grouped(const SweepParams &P, const S
                     for (int k = 0; k < GROUP; k++) {
                         V3 q2 = p2;
                         if (!TRANSLATED) {
-                            const unsigned wk = (word >> (4 * k)) & 7;
+                            // a move that leaves the cell is not tested; its wrap bits are zeroed so that the
+                            // translation index stays inside the 27-entry table
+                            const unsigned wk = (word >> (4 * k + 3)) & 1 ? (word >> (4 * k)) & 7 : 0u;
                             q2 = add(p2, image_translation(sT, code, wk & 1, (wk >> 1) & 1, (wk >> 2) & 1));
                         }
                         const V3 d = sub(q2, npos[k]);
