@@ -211,6 +211,7 @@ int ensureCells(rp_ctx *c, int ncell) {
     int cap = ncell + ncell / 4 + 64;
     RP_HIP(hipMalloc(&c->cellStart, sizeof(int) * (cap + 1)));
     RP_HIP(hipMalloc(&c->cellCount, sizeof(int) * cap));
+    RP_HIP(launch_fill_i32(c->stream, c->cellCount, (size_t)cap, 0));  // rebuilds keep them zero from here on
     c->ncellCap = cap;
     return RP_OK;
 }
@@ -1112,6 +1113,9 @@ void rp_destroy(rp_ctx *c) {
                              ph[RP_PHASE_SCAN] / ng / perUs, ph[RP_PHASE_PASSB] / ng / perUs,
                              ph[RP_PHASE_DECIDE] / ng / perUs, ng / b,
                              100.0 * ph[RP_PHASE_REPLANS] / ng);
+                std::fprintf(stderr, "grouped planner per group (us): poses %.3f, plan words %.3f, trial pairs %.3f\n",
+                             ph[RP_PHASE_PLAN_POSE] / ng / perUs, ph[RP_PHASE_PLAN_WORD] / ng / perUs,
+                             ph[RP_PHASE_PLAN_PAIRS] / ng / perUs);
             }
         }
         hipFree(c->dPhases);

@@ -173,24 +173,44 @@ hipError_t launch_fill_i32(hipStream_t s, int *p, size_t n, int v) {
 __global__ __launch_bounds__(256) void k_bin(Grid g, int n, Particles src, int rescale, M3 oldInv,
                                              double *__restrict__ tmpPos, int *__restrict__ tmpCell,
                                              int *__restrict__ tmpSlot, uint8_t *__restrict__ tmpW) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    V3 pos = load3(src.x, src.y, src.z, i);
-    if (rescale) pos = mulv(g.dims, mulv(oldInv.m, pos));
-    tmpPos[i] = pos.x;
-    tmpPos[n + i] = pos.y;
-    tmpPos[2 * n + i] = pos.z;
-    int c[3];
-    unsigned w;
-    bin_position(g, pos, c, w);
-    int cell = cell_index(g, c[0], c[1], c[2]);
-    tmpCell[i] = cell;
-    tmpW[i] = (uint8_t)w;
-    tmpSlot[i] = atomicAdd(&g.cellCount[cell], 1);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = i < n;
+    int cell = -1;
+    if (valid) {
+        V3 pos = load3(src.x, src.y, src.z, i);
+        if (rescale) pos = mulv(g.dims, mulv(oldInv.m, pos));
+        tmpPos[i] = pos.x;
+        tmpPos[n + i] = pos.y;
+        tmpPos[2 * n + i] = pos.z;
+        int c[3];
+        unsigned w;
+        bin_position(g, pos, c, w);
+        cell = cell_index(g, c[0], c[1], c[2]);
+        tmpCell[i] = cell;
+        tmpW[i] = (uint8_t)w;
+    }
+    // The particles arrive in the previous grid's cell order, so a wavefront's lanes fall into a few cells: one
+    // counter update per distinct cell, each lane's slot its rank among the lanes of that cell.  (Slots order the
+    // members only until the gather, which ranks them by source index.)
+    unsigned long long active = __ballot(valid);
+    int slot = 0;
+    while (active) {
+        const int leader = __ffsll((long long)active) - 1;
+        const int lc = __builtin_amdgcn_readlane(cell, leader);
+        const unsigned long long same = __ballot(valid && cell == lc);
+        int base = 0;
+        if ((int)(threadIdx.x & (WAVE - 1)) == leader) base = atomicAdd(&g.cellCount[lc], (int)__popcll(same));
+        base = __builtin_amdgcn_readlane(base, leader);
+        if (valid && cell == lc)
+            slot = base + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(same >> 32),
+                                                         __builtin_amdgcn_mbcnt_lo((unsigned)same, 0u));
+        active &= ~same;
+    }
+    if (valid) tmpSlot[i] = slot;
 }
 
 // Exclusive scan of the cell counts by one workgroup: contiguous chunk per thread, then a scan of the chunk sums.
-__global__ __launch_bounds__(1024) void k_scan(int ncell, const int *__restrict__ count, int *__restrict__ start,
+__global__ __launch_bounds__(1024) void k_scan(int ncell, int *__restrict__ count, int *__restrict__ start,
                                                unsigned long long *counters) {
     __shared__ int part[1024];
     __shared__ int pmax[1024];
@@ -216,8 +236,10 @@ __global__ __launch_bounds__(1024) void k_scan(int ncell, const int *__restrict_
     }
     int run = t == 0 ? 0 : part[t - 1];
     for (int i = b; i < e; i++) {
+        const int c = count[i];
         start[i] = run;
-        run += count[i];
+        run += c;
+        count[i] = 0;  // cleared for the next rebuild, so the counts are zero whenever a rebuild starts
     }
     if (t == 1023) {
         start[ncell] = part[1023];
@@ -272,10 +294,9 @@ __global__ __launch_bounds__(WAVE) void k_gather(Grid g, int n, Particles src, P
 hipError_t launch_rebuild(hipStream_t s, const Grid &g, int n, Particles src, Particles dst, int *tmpSlot,
                           int *tmpOrder, double *tmpPos, int rescale, const double *oldInv) {
     // tmpSlot holds 2n ints (slot, cell); tmpOrder holds n ints followed by n bytes of wrap bits.
+    // the cell counts are zero here: cleared when allocated and by the scan of every rebuild
     int *tmpCell = tmpSlot + n;
     uint8_t *tmpW = reinterpret_cast<uint8_t *>(tmpOrder + n);
-    hipError_t err = launch_fill_i32(s, g.cellCount, (size_t)g.ncell, 0);
-    if (err != hipSuccess) return err;
     M3 inv{};
     if (rescale)
         for (int i = 0; i < 9; i++) inv.m[i] = oldInv[i];
@@ -1314,9 +1335,11 @@ __device__ __forceinline__ int wrap_code(unsigned nw) {
 template <int KIND, bool TRANSLATED>
 __device__ __forceinline__ void plan_group(const SweepParams &P, const Stencil &S, const double *oo, const double *pt,
                                            const double *pR, const int *pk, const double *sT, int m0, int gs,
-                                           int cell, double *plan, int *planI) {
+                                           int cell, double *plan, int *planI,
+                                           unsigned long long *phase = nullptr) {
     const Grid &g = P.g;
     const int lane = threadIdx.x & (WAVE - 1);
+    const unsigned long long t0 = phase ? wall_clock64() : 0;
     unsigned byte = 0;
     if (lane < gs) {
         const int mi = m0 + lane;
@@ -1348,9 +1371,11 @@ __device__ __forceinline__ void plan_group(const SweepParams &P, const Stencil &
         for (int c = 0; c < 9; c++) q[13 + c] = NO[c];
         planI[lane] = kIdx;
     }
+    const unsigned long long t1 = phase ? wall_clock64() : 0;
     unsigned word = 0;
     for (int t = 0; t < gs; t++) word |= ((unsigned)__shfl((int)byte, t) & 15u) << (4 * t);
     __builtin_amdgcn_wave_barrier();  // the plans were written by other lanes of this wavefront
+    const unsigned long long t2 = phase ? wall_clock64() : 0;
     unsigned hit = 0;
     if (lane < 6) {
         const int kk = lane < 1 ? 1 : (lane < 3 ? 2 : 3);
@@ -1377,6 +1402,11 @@ __device__ __forceinline__ void plan_group(const SweepParams &P, const Stencil &
         planI[5] = (int)pairs;
         planI[6] = (int)word;
     }
+    if (phase) {
+        phase[0] += t1 - t0;
+        phase[1] += t2 - t1;
+        phase[2] += wall_clock64() - t2;
+    }
 }
 
 template <int KIND, int WAVES, bool TRANSLATED>
@@ -1398,6 +1428,8 @@ __device__ __forceinline__ void sweep_cell_grouped(const SweepParams &P, const S
     int *discard = myQueue + (GQ - WAVE) + lane;
     unsigned long long accepted = 0, leftCell = 0, tests = 0;
     unsigned long long tBusy = 0, tDecide = 0, tTop = 0, tPassB = 0, nGroups = 0, nReplans = 0;  // phase timers
+    unsigned long long tPlan[3] = {0, 0, 0};
+    unsigned long long *planPhase = clk ? tPlan : nullptr;
     const int moves = P.moves > 0 ? P.moves : nOwn;
     const int pbase = P.moves > 0 ? cell * P.moves : own0;
     // issue priority: the planner and the testers with an extra slot bound each group's time
@@ -1461,7 +1493,7 @@ __device__ __forceinline__ void sweep_cell_grouped(const SweepParams &P, const S
             if (planner) {
                 if (gsNext)
                     plan_group<KIND, TRANSLATED>(P, S, oo, pt, pR, pk, sT, m1, gsNext, cell,
-                                                 plans + nbuf * GROUP * PLAN_D, plansI + nbuf * GPLAN_I);
+                                                 plans + nbuf * GROUP * PLAN_D, plansI + nbuf * GPLAN_I, planPhase);
                 if (lane == 0) hitWords[nbuf] = 0;
             } else {
                 // 5 bits per move: bits 0-3 overlap with group particle i's current pose, bit 4 with a particle
@@ -1601,6 +1633,11 @@ __device__ __forceinline__ void sweep_cell_grouped(const SweepParams &P, const S
         clk[3] = wall_clock64();
         if (lane == 0 && wave <= 1) atomicAdd(&P.phases[planner ? RP_PHASE_PLAN : RP_PHASE_SCAN], tBusy);
         if (lane == 0 && wave == 1) atomicAdd(&P.phases[RP_PHASE_PASSB], tPassB);
+        if (tid == 0) {
+            atomicAdd(&P.phases[RP_PHASE_PLAN_POSE], tPlan[0]);
+            atomicAdd(&P.phases[RP_PHASE_PLAN_WORD], tPlan[1]);
+            atomicAdd(&P.phases[RP_PHASE_PLAN_PAIRS], tPlan[2]);
+        }
         if (tid == 0) {
             atomicAdd(&P.phases[RP_PHASE_DECIDE], tDecide);
             atomicAdd(&P.phases[RP_PHASE_GROUPS], nGroups);
