@@ -1113,9 +1113,6 @@ void rp_destroy(rp_ctx *c) {
                              ph[RP_PHASE_SCAN] / ng / perUs, ph[RP_PHASE_PASSB] / ng / perUs,
                              ph[RP_PHASE_DECIDE] / ng / perUs, ng / b,
                              100.0 * ph[RP_PHASE_REPLANS] / ng);
-                std::fprintf(stderr, "grouped planner per group (us): poses %.3f, plan words %.3f, trial pairs %.3f\n",
-                             ph[RP_PHASE_PLAN_POSE] / ng / perUs, ph[RP_PHASE_PLAN_WORD] / ng / perUs,
-                             ph[RP_PHASE_PLAN_PAIRS] / ng / perUs);
             }
         }
         hipFree(c->dPhases);

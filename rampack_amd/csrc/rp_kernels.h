@@ -84,10 +84,7 @@ enum SweepPhase : int {
     RP_PHASE_REPLANS = 12, // grouped sweep: groups planned twice
     RP_PHASE_TOP = 13,     // grouped sweep: start of a group to the start of planning / scanning (plan words)
     RP_PHASE_PASSB = 14,   // grouped sweep: first tester wavefront in the segment tests of queued pairs
-    RP_PHASE_PLAN_POSE = 15,   // grouped sweep: planner, trial poses written
-    RP_PHASE_PLAN_WORD = 16,   // grouped sweep: planner, plan words gathered
-    RP_PHASE_PLAN_PAIRS = 17,  // grouped sweep: planner, trial-pair tests
-    RP_PHASE_NUM = 20
+    RP_PHASE_NUM = 16
 };
 
 enum Counter : int {

@@ -1335,11 +1335,9 @@ __device__ __forceinline__ int wrap_code(unsigned nw) {
 template <int KIND, bool TRANSLATED>
 __device__ __forceinline__ void plan_group(const SweepParams &P, const Stencil &S, const double *oo, const double *pt,
                                            const double *pR, const int *pk, const double *sT, int m0, int gs,
-                                           int cell, double *plan, int *planI,
-                                           unsigned long long *phase = nullptr) {
+                                           int cell, double *plan, int *planI) {
     const Grid &g = P.g;
     const int lane = threadIdx.x & (WAVE - 1);
-    const unsigned long long t0 = phase ? wall_clock64() : 0;
     unsigned byte = 0;
     if (lane < gs) {
         const int mi = m0 + lane;
@@ -1371,11 +1369,9 @@ __device__ __forceinline__ void plan_group(const SweepParams &P, const Stencil &
         for (int c = 0; c < 9; c++) q[13 + c] = NO[c];
         planI[lane] = kIdx;
     }
-    const unsigned long long t1 = phase ? wall_clock64() : 0;
     unsigned word = 0;
     for (int t = 0; t < gs; t++) word |= ((unsigned)__shfl((int)byte, t) & 15u) << (4 * t);
     __builtin_amdgcn_wave_barrier();  // the plans were written by other lanes of this wavefront
-    const unsigned long long t2 = phase ? wall_clock64() : 0;
     unsigned hit = 0;
     if (lane < 6) {
         const int kk = lane < 1 ? 1 : (lane < 3 ? 2 : 3);
@@ -1402,11 +1398,6 @@ __device__ __forceinline__ void plan_group(const SweepParams &P, const Stencil &
         planI[5] = (int)pairs;
         planI[6] = (int)word;
     }
-    if (phase) {
-        phase[0] += t1 - t0;
-        phase[1] += t2 - t1;
-        phase[2] += wall_clock64() - t2;
-    }
 }
 
 template <int KIND, int WAVES, bool TRANSLATED>
@@ -1428,8 +1419,6 @@ __device__ __forceinline__ void sweep_cell_grouped(const SweepParams &P, const S
     int *discard = myQueue + (GQ - WAVE) + lane;
     unsigned long long accepted = 0, leftCell = 0, tests = 0;
     unsigned long long tBusy = 0, tDecide = 0, tTop = 0, tPassB = 0, nGroups = 0, nReplans = 0;  // phase timers
-    unsigned long long tPlan[3] = {0, 0, 0};
-    unsigned long long *planPhase = clk ? tPlan : nullptr;
     const int moves = P.moves > 0 ? P.moves : nOwn;
     const int pbase = P.moves > 0 ? cell * P.moves : own0;
     // issue priority: the planner and the testers with an extra slot bound each group's time
@@ -1493,7 +1482,7 @@ __device__ __forceinline__ void sweep_cell_grouped(const SweepParams &P, const S
             if (planner) {
                 if (gsNext)
                     plan_group<KIND, TRANSLATED>(P, S, oo, pt, pR, pk, sT, m1, gsNext, cell,
-                                                 plans + nbuf * GROUP * PLAN_D, plansI + nbuf * GPLAN_I, planPhase);
+                                                 plans + nbuf * GROUP * PLAN_D, plansI + nbuf * GPLAN_I);
                 if (lane == 0) hitWords[nbuf] = 0;
             } else {
                 // 5 bits per move: bits 0-3 overlap with group particle i's current pose, bit 4 with a particle
@@ -1633,11 +1622,6 @@ __device__ __forceinline__ void sweep_cell_grouped(const SweepParams &P, const S
         clk[3] = wall_clock64();
         if (lane == 0 && wave <= 1) atomicAdd(&P.phases[planner ? RP_PHASE_PLAN : RP_PHASE_SCAN], tBusy);
         if (lane == 0 && wave == 1) atomicAdd(&P.phases[RP_PHASE_PASSB], tPassB);
-        if (tid == 0) {
-            atomicAdd(&P.phases[RP_PHASE_PLAN_POSE], tPlan[0]);
-            atomicAdd(&P.phases[RP_PHASE_PLAN_WORD], tPlan[1]);
-            atomicAdd(&P.phases[RP_PHASE_PLAN_PAIRS], tPlan[2]);
-        }
         if (tid == 0) {
             atomicAdd(&P.phases[RP_PHASE_DECIDE], tDecide);
             atomicAdd(&P.phases[RP_PHASE_GROUPS], nGroups);
