@@ -240,6 +240,26 @@ void rp_destroy(rp_ctx *ctx);
  * decisions[i] = 1 for an overlap. */
 int rp_pair_decisions(const rp_shape_desc *shape, size_t n, const double *records, uint8_t *decisions);
 
+/* Guarded FP32 form of rp_pair_decisions (no counterpart in the reference; its FP64 decisions are the target).  Each
+ * pair's centre distance, and for spherocylinders within range the segment distance, is evaluated in FP32 with a
+ * rigorous bound on its error against the FP64 value; a pair farther from the threshold than the bound is decided in
+ * FP32, any other pair by the FP64 test, so every decision equals rp_pair_decisions'.  codes (n bytes, or NULL):
+ * bits 0-1 = 0 FP32 no overlap, 1 FP32 overlap, 2 FP64 fallback; bit 2 = the segment stage ran; bit 3 = differs
+ * from FP64 (never set).  stats (or NULL) counts the fallbacks and reports the largest FP32 error.  With codes and
+ * stats both NULL only the undecided pairs are evaluated in FP64 (the form to time); otherwise every pair is, for the
+ * statistics. */
+typedef struct rp_guard_stats {
+    uint64_t pairs;
+    uint64_t fp64_fallback;      /* pairs inside a guard band, decided by the FP64 test */
+    uint64_t segment_stage;      /* pairs that reached the FP32 segment-distance stage */
+    uint64_t segment_fallback;   /* of those, pairs inside the segment guard band */
+    uint64_t mismatches;         /* guarded decision != FP64 decision: 0 */
+    double max_err_centre;       /* max |FP32 - FP64| squared centre distance */
+    double max_err_segment;      /* max |FP32 - FP64| squared segment distance (FP32 at its own (s, t)) */
+} rp_guard_stats;
+int rp_pair_decisions_guarded(const rp_shape_desc *shape, size_t n, const double *records, uint8_t *decisions,
+                              uint8_t *codes, rp_guard_stats *stats);
+
 /* Batched soft pair energies on the device: Interaction::calculateEnergyBetween with a fixed cell translation
  * (CentralInteraction.h:58-67 for RP_SOFT_WCA); records as in rp_pair_decisions. */
 int rp_pair_energies(const rp_shape_desc *shape, size_t n, const double *records, double *energies);

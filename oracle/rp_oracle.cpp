@@ -760,4 +760,144 @@ double orc_nematic_p2(std::size_t n, const double *rot) {
     return best;
 }
 
+// Guarded FP32 pair decisions, restated for the tests (no counterpart in the reference: the FP64 decisions above are
+// what it must reproduce).  The same FP32 arithmetic as the device (each operation rounded on its own, correctly
+// rounded division), so the classification codes can be compared bit for bit:
+//   centre stage: coordinates relative to r0 = floor(pos1), d2f = |fj - fm|^2 with a bound on |d2f - d2_64|;
+//   segment stage: the FP32 form of the segment-distance algorithm picks (s, t); the value at (s, t) bounds the
+//   minimum from above, the value plus the minimum of the tangent plane over [0, 1]^2 from below (convexity); the
+//   FP32 cap points lie within delta of the reference's, which moves the minimum distance by at most 2 delta.
+// bound_scale multiplies the unit roundoff every error bound is built from (1 in use; the tests set it to 0 to show
+// that bounds too small are caught by the FP64 comparison).
+namespace orcguard {
+struct F { float x, y, z; };
+static inline F mk(float x, float y, float z) { return {x, y, z}; }
+static inline F fadd(const F &a, const F &b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline F fsub(const F &a, const F &b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline F fscal(const F &a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
+static inline float fdot(const F &a, const F &b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+static inline F fabs3(const F &a) { return mk(std::fabs(a.x), std::fabs(a.y), std::fabs(a.z)); }
+static float U = 5.9604645e-08f * 1.01f;
+static const double SLACK64 = 1e-6;
+
+static float centre(const F &fm, const F &em, const F &fj, const F &ej, float &bound) {
+    const F d = fsub(fj, fm);
+    const F ad = fabs3(d);
+    const F e = fadd(fadd(em, ej), fscal(ad, U));
+    const float d2 = fdot(d, d);
+    bound = ((2.f * ad.x + e.x) * e.x + (2.f * ad.y + e.y) * e.y + (2.f * ad.z + e.z) * e.z) * 1.01f +
+            3.1f * U * d2 + 1e-12f;
+    return d2;
+}
+
+static float capErr(const F &c, const F &ep, const F &a, const F &hh, float h, float eh) {
+    const F aa = fabs3(a);
+    const float x = ep.x + U * aa.x * h + eh * aa.x + U * std::fabs(hh.x) + U * std::fabs(c.x);
+    const float y = ep.y + U * aa.y * h + eh * aa.y + U * std::fabs(hh.y) + U * std::fabs(c.y);
+    const float z = ep.z + U * aa.z * h + eh * aa.z + U * std::fabs(hh.z) + U * std::fabs(c.z);
+    return (x + y + z) * 1.01f + 1e-9f;
+}
+
+// 0 no overlap, 1 overlap, 2 undecided
+static int segment(double contact2, const F &fm, const F &am, const F &em, const F &fj, const F &aj, const F &ej,
+                   float h, float eh, float &d2s) {
+    const F ha = fscal(am, h), hj = fscal(aj, h);
+    const F ca = fsub(fm, ha), cb = fadd(fm, ha), da = fsub(fj, hj), db = fadd(fj, hj);
+    const float delta = std::max(std::max(capErr(ca, em, am, ha, h, eh), capErr(cb, em, am, ha, h, eh)),
+                                 std::max(capErr(da, ej, aj, hj, h, eh), capErr(db, ej, aj, hj, h, eh)));
+    const F u = fsub(cb, ca), v = fsub(db, da), w = fsub(ca, da);
+    const float a = fdot(u, u), b = fdot(u, v), c = fdot(v, v), d = fdot(u, w), e = fdot(v, w);
+    const float D = a * c - b * b;
+    if (!(D >= 1e-3f * a * c)) return 2;
+    float sN = b * e - c * d, sD = D, tN = a * e - b * d, tD = D;
+    if (sN < 0.f) { sN = 0.f; tN = e; tD = c; }
+    else if (sN > sD) { sN = sD; tN = e + b; tD = c; }
+    if (tN < 0.f) {
+        tN = 0.f;
+        if (-d < 0.f) sN = 0.f;
+        else if (-d > a) sN = sD;
+        else { sN = -d; sD = a; }
+    } else if (tN > tD) {
+        tN = tD;
+        if ((-d + b) < 0.f) sN = 0.f;
+        else if ((-d + b) > a) sN = sD;
+        else { sN = -d + b; sD = a; }
+    }
+    const float sc = std::min(std::max(sN / sD, 0.f), 1.f), tc = std::min(std::max(tN / tD, 0.f), 1.f);
+    const F wsu = fadd(w, fscal(u, sc));
+    const F r = fsub(wsu, fscal(v, tc));
+    const F aw = fabs3(w), au = fabs3(u), av = fabs3(v), ar = fabs3(r);
+    const F er = fscal(fadd(fadd(fadd(aw, au), fadd(av, fabs3(fscal(u, sc)))), fadd(fadd(fabs3(wsu), fabs3(fscal(v, tc))), ar)),
+                       U * 1.01f);
+    const float f = fdot(r, r);
+    const float ef = ((2.f * ar.x + er.x) * er.x + (2.f * ar.y + er.y) * er.y + (2.f * ar.z + er.z) * er.z) * 1.01f +
+                     3.1f * U * f;
+    d2s = f;
+    const float gs = 2.f * fdot(u, r), gt = -2.f * fdot(v, r);
+    const float egs = 2.f * (fdot(au, er) + fdot(fadd(ar, er), fscal(au, U)) + 3.1f * U * fdot(au, ar));
+    const float egt = 2.f * (fdot(av, er) + fdot(fadd(ar, er), fscal(av, U)) + 3.1f * U * fdot(av, ar));
+    const float ls = std::min(-gs * sc, gs * (1.f - sc)), lt = std::min(-gt * tc, gt * (1.f - tc));
+    const double ub = (double)f + (double)ef;
+    const double lb = (double)f - (double)ef + (double)ls + (double)lt - (double)egs - (double)egt -
+                      2.1 * (double)U * ((double)std::fabs(gs) + (double)std::fabs(gt)) -
+                      4.0 * (double)U * ((double)f + (double)std::fabs(ls) + (double)std::fabs(lt));
+    const double dd = 2.0 * (double)delta;
+    const double hiDist = std::sqrt(ub) + dd;
+    if (hiDist * hiDist + SLACK64 < contact2) return 1;
+    if (lb > 0) {
+        const double loDist = std::sqrt(lb) - dd;
+        if (loDist > 0 && loDist * loDist - SLACK64 >= contact2) return 0;
+    }
+    return 2;
+}
+}  // namespace orcguard
+
+int orc_pair_guarded(int kind, double radius, double length, std::size_t n, const double *rec, std::uint8_t *dec,
+                     std::uint8_t *codes, double bound_scale) {
+    using namespace orcguard;
+    const float scale = (float)bound_scale;
+    U = 5.9604645e-08f * 1.01f * scale;
+    const double contact2 = kind == SPHERE ? std::pow(2 * radius, 2) : 4 * radius * radius;
+    const double range2 = std::pow(2 * radius + length, 2);
+    const double halfLen = 0.5 * length;
+    for (std::size_t i = 0; i < n; i++) {
+        const double *q = rec + 27 * i;
+        Vec p1, p2, T;
+        Mat O1, O2;
+        std::memcpy(p1.v, q, 24); std::memcpy(O1.m, q + 3, 72);
+        std::memcpy(p2.v, q + 12, 24); std::memcpy(O2.m, q + 15, 72);
+        std::memcpy(T.v, q + 24, 24);
+        const bool o64 = (kind == SPHERE) ? sphereOverlap(radius, p1, p2, T)
+                                          : spherocylinderOverlap(radius, length, p1, O1, p2, O2, T);
+        const Vec p2bc = add(p2, T);
+        const Vec r0 = {{std::floor(p1.v[0]), std::floor(p1.v[1]), std::floor(p1.v[2])}};
+        const Vec dm = sub(p1, r0), dj = sub(p2bc, r0);
+        const F fm = mk((float)dm.v[0], (float)dm.v[1], (float)dm.v[2]);
+        const F fj = mk((float)dj.v[0], (float)dj.v[1], (float)dj.v[2]);
+        const F em = fscal(fabs3(fm), U), ej = fscal(fabs3(fj), U);
+        float bound;
+        const float d2f = centre(fm, em, fj, ej, bound);
+        const double lo = (double)d2f - (double)bound, hi = (double)d2f + (double)bound;
+        int g = 2, flags = 0;
+        float segf = -1;
+        if (hi < contact2) g = 1;
+        else if (kind == SPHERE && lo >= contact2) g = 0;
+        else if (kind != SPHERE && lo >= range2) g = 0;
+        else if (kind != SPHERE && lo >= contact2 && hi < range2) {
+            flags |= 4;
+            const float h = (float)halfLen;
+            const float eh = (float)std::fabs((double)h - halfLen);
+            const F am = mk((float)O1.m[2], (float)O1.m[5], (float)O1.m[8]);
+            const F aj = mk((float)O2.m[2], (float)O2.m[5], (float)O2.m[8]);
+            g = segment(contact2, fm, am, em, fj, aj, ej, h, eh, segf);
+        }
+        const bool o = g == 2 ? o64 : g == 1;
+        if (o != o64) flags |= 8;
+        dec[i] = o ? 1 : 0;
+        if (codes) codes[i] = (std::uint8_t)(g | flags);
+    }
+    U = 5.9604645e-08f * 1.01f;
+    return 0;
+}
+
 }  // extern "C"

@@ -45,7 +45,7 @@ EXPORTED_SYMBOLS = (
     "rp_set_overlap_counting", "rp_cached_overlaps", "rp_relax_overlaps",
     "rp_ramtrj_recorder_open", "rp_ramtrj_record", "rp_ramtrj_record_packing", "rp_ramtrj_recorder_last_cycle",
     "rp_ramtrj_recorder_close", "rp_ramtrj_player_open", "rp_ramtrj_player_header", "rp_ramtrj_next_snapshot",
-    "rp_ramtrj_jump_to_snapshot", "rp_ramtrj_player_close",
+    "rp_ramtrj_jump_to_snapshot", "rp_ramtrj_player_close", "rp_pair_decisions_guarded",
 )
 
 
@@ -105,6 +105,15 @@ class RamtrjFix(ctypes.Structure):
     _fields_ = [("fixing_needed", ctypes.c_int), ("fixing_successful", ctypes.c_int),
                 ("header_snapshots", ctypes.c_uint64), ("inferred_snapshots", ctypes.c_uint64),
                 ("bytes_remainder", ctypes.c_uint64), ("bytes_per_snapshot", ctypes.c_uint64)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class GuardStats(ctypes.Structure):
+    _fields_ = [("pairs", ctypes.c_uint64), ("fp64_fallback", ctypes.c_uint64), ("segment_stage", ctypes.c_uint64),
+                ("segment_fallback", ctypes.c_uint64), ("mismatches", ctypes.c_uint64),
+                ("max_err_centre", ctypes.c_double), ("max_err_segment", ctypes.c_double)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -177,6 +186,9 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.rp_fix_orientations.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64)]
     lib.rp_set_orientation_fix_every.argtypes = [vp, ctypes.c_uint64]
     lib.rp_pair_energies.argtypes = [ctypes.POINTER(ShapeDesc), ctypes.c_size_t, dp, dp]
+    lib.rp_pair_decisions_guarded.argtypes = [ctypes.POINTER(ShapeDesc), ctypes.c_size_t, dp,
+                                              ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(ctypes.c_uint8),
+                                              ctypes.POINTER(GuardStats)]
     lib.rp_last_error.restype = ctypes.c_char_p
     _lib = lib
     return lib
@@ -246,6 +258,27 @@ def pair_decisions(shape: Shape, records: np.ndarray) -> np.ndarray:
     _check(lib.rp_pair_decisions(ctypes.byref(d), len(rec), _dptr(rec),
                                  out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))))
     return out
+
+
+def pair_decisions_guarded(shape: Shape, records: np.ndarray, verify: bool = True):
+    """Guarded FP32 pair decisions (rp_pair_decisions_guarded): (decisions, codes, stats dict), or only the decisions
+    with verify=False (FP64 then runs only where the FP32 bounds leave a pair undecided).  The decisions equal
+    pair_decisions'; codes bits 0-1 = 0 FP32 no overlap / 1 FP32 overlap / 2 FP64 fallback, bit 2 = segment stage,
+    bit 3 = differs from FP64."""
+    lib = load_library()
+    rec = np.ascontiguousarray(records, dtype=np.float64).reshape(-1, 27)
+    out = np.zeros(len(rec), dtype=np.uint8)
+    d = shape.desc()
+    u8 = ctypes.POINTER(ctypes.c_uint8)
+    if not verify:
+        _check(lib.rp_pair_decisions_guarded(ctypes.byref(d), len(rec), _dptr(rec), out.ctypes.data_as(u8), None,
+                                             None))
+        return out
+    codes = np.zeros(len(rec), dtype=np.uint8)
+    st = GuardStats()
+    _check(lib.rp_pair_decisions_guarded(ctypes.byref(d), len(rec), _dptr(rec), out.ctypes.data_as(u8),
+                                         codes.ctypes.data_as(u8), ctypes.byref(st)))
+    return out, codes, st.as_dict()
 
 
 def pair_energies(shape: Shape, records: np.ndarray) -> np.ndarray:

@@ -1326,6 +1326,57 @@ int rp_pair_decisions(const rp_shape_desc *shape, size_t n, const double *record
     return r;
 }
 
+int rp_pair_decisions_guarded(const rp_shape_desc *shape, size_t n, const double *records, uint8_t *decisions,
+                              uint8_t *codes, rp_guard_stats *stats) {
+    if (!shape || (n && (!records || !decisions))) return fail(RP_ERR_PRECONDITION, "null argument");
+    if (shape->kind != RP_SHAPE_SPHERE && shape->kind != RP_SHAPE_SPHEROCYLINDER)
+        return fail(RP_ERR_UNSUPPORTED, "unsupported shape kind");
+    if (shape->soft != RP_SOFT_NONE) return fail(RP_ERR_UNSUPPORTED, "a soft interaction has no hard part");
+    if (stats) *stats = rp_guard_stats{};
+    if (n == 0) return RP_OK;
+    PairConsts k = makeConsts(*shape);
+    double *dRec = nullptr;
+    uint8_t *dOut = nullptr;
+    float *dErr = nullptr;
+    const size_t outBytes = 2 * n, errBytes = sizeof(float) * 2 * n;
+    RP_HIP(hipMalloc(&dRec, sizeof(double) * 27 * n));
+    if (hipMalloc(&dOut, outBytes) != hipSuccess || hipMalloc(&dErr, errBytes) != hipSuccess) {
+        hipFree(dRec);
+        if (dOut) hipFree(dOut);
+        return fail(RP_ERR_DEVICE, "hipMalloc failed");
+    }
+    std::vector<uint8_t> hOut(outBytes);
+    std::vector<float> hErr(2 * n);
+    int r = RP_OK;
+    hipError_t e = hipMemcpy(dRec, records, sizeof(double) * 27 * n, hipMemcpyHostToDevice);
+    // without codes or statistics the kernel runs FP64 only on the undecided pairs
+    const bool verify = codes || stats;
+    if (e == hipSuccess)
+        e = launch_pair_decisions_guarded(nullptr, k, n, dRec, dOut, verify ? dOut + n : nullptr, dErr, dErr + n);
+    if (e == hipSuccess) e = hipMemcpy(hOut.data(), dOut, verify ? outBytes : n, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && verify) e = hipMemcpy(hErr.data(), dErr, errBytes, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) r = fail(RP_ERR_DEVICE, std::string("guarded pair decisions: ") + hipGetErrorString(e));
+    hipFree(dRec);
+    hipFree(dOut);
+    hipFree(dErr);
+    if (r) return r;
+    std::memcpy(decisions, hOut.data(), n);
+    if (codes) std::memcpy(codes, hOut.data() + n, n);
+    if (stats) {
+        stats->pairs = n;
+        for (size_t i = 0; i < n; i++) {
+            const uint8_t c = hOut[n + i];
+            if ((c & 3) == 2) stats->fp64_fallback++;
+            if (c & 4) stats->segment_stage++;
+            if ((c & 4) && (c & 3) == 2) stats->segment_fallback++;
+            if (c & 8) stats->mismatches++;
+            stats->max_err_centre = std::max(stats->max_err_centre, (double)hErr[i]);
+            if (hErr[n + i] >= 0) stats->max_err_segment = std::max(stats->max_err_segment, (double)hErr[n + i]);
+        }
+    }
+    return RP_OK;
+}
+
 }  // extern "C"
 
 // Error reporting for the library's other host sources (C++ linkage).

@@ -101,6 +101,8 @@ enum Counter : int {
 
 // Launch wrappers (rp_kernels.hip).  All enqueue on `s` and return the launch status.
 hipError_t launch_pair_decisions(hipStream_t s, PairConsts k, size_t n, const double *rec, uint8_t *out);
+hipError_t launch_pair_decisions_guarded(hipStream_t s, PairConsts k, size_t n, const double *rec, uint8_t *out,
+                                         uint8_t *code, float *errC, float *errS);
 hipError_t launch_fill_u64(hipStream_t s, unsigned long long *p, size_t n, unsigned long long v);
 hipError_t launch_fill_i32(hipStream_t s, int *p, size_t n, int v);
 // Cell-list rebuild: src -> dst sorted by cell of `g`.  When rescale != 0 the positions are first mapped

@@ -119,6 +119,85 @@ hipError_t launch_pair_decisions(hipStream_t s, PairConsts k, size_t n, const do
     return hipGetLastError();
 }
 
+// Guarded FP32 decisions of the same records.  Coordinates are taken relative to r0 = floor(pos1) (FP32 roundings
+// of FP64 differences, as the staged values of a sweep would be); a pair the FP32 bounds decide keeps that decision,
+// any other pair takes the FP64 decision.  Per record: out = the decision; code = GuardResult of the deciding FP32
+// stage | 4 if the segment stage ran | 8 if the decision differs from the FP64 one (must never happen); errC =
+// |d2f - d2_64|; errS = |segment d2 in FP32 - in FP64| at the FP32 (s, t), or -1 without a segment stage.
+// VERIFY: also evaluate every pair in FP64 for the codes and error statistics; otherwise FP64 runs only for the
+// pairs the FP32 stages leave undecided.
+template <bool VERIFY>
+__global__ __launch_bounds__(256) void k_pair_decisions_guarded(PairConsts k, size_t n, const double *__restrict__ rec,
+                                                                uint8_t *__restrict__ out, uint8_t *__restrict__ code,
+                                                                float *__restrict__ errC, float *__restrict__ errS) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double *q = rec + 27 * i;
+    const V3 p1 = v3(q[0], q[1], q[2]);
+    const V3 a1 = v3(q[3 + 2], q[3 + 5], q[3 + 8]);
+    const V3 p2 = v3(q[12], q[13], q[14]);
+    const V3 a2 = v3(q[15 + 2], q[15 + 5], q[15 + 8]);
+    const V3 T = v3(q[24], q[25], q[26]);
+    const V3 p2bc = add(p2, T);
+    // FP64 reference decision (and values, for the statistics)
+    auto fp64 = [&](double &d2_64, double &seg64) {
+        const V3 dd = sub(p2bc, p1);
+        d2_64 = dot(dd, dd);
+        seg64 = -1;
+        if (k.kind == 0) return sphere_overlap(k, p1, p2, T);
+        const SCFirst f1 = sc_first(k, p1, a1);
+        if (VERIFY && !(d2_64 < k.contact2) && d2_64 < k.range2)
+            seg64 = segment_distance2(f1.capA, f1.u, f1.a, add(p2bc, scal(a2, -k.halfLen)),
+                                      add(p2bc, scal(a2, k.halfLen)));
+        return sc_overlap(k, f1, p2, a2, T);
+    };
+    double d2_64 = 0, seg64 = -1;
+    bool o64 = false;
+    if (VERIFY) o64 = fp64(d2_64, seg64);
+    // guarded FP32
+    const V3 r0 = v3(floor(p1.x), floor(p1.y), floor(p1.z));
+    const F3 fm = to_f3(sub(p1, r0)), fj = to_f3(sub(p2bc, r0));
+    const F3 em = round_err(fm), ej = round_err(fj);
+    float bound;
+    const float d2f = guard_centre(fm, em, fj, ej, bound);
+    const double lo = (double)d2f - (double)bound, hi = (double)d2f + (double)bound;
+    int g = GUARD_UNDECIDED, flags = 0;
+    float segf = -1;
+    if (hi < k.contact2) g = GUARD_OVERLAP;
+    else if (k.kind == 0 && lo >= k.contact2) g = GUARD_NO;
+    else if (k.kind == 1 && lo >= k.range2) g = GUARD_NO;
+    else if (k.kind == 1 && lo >= k.contact2 && hi < k.range2) {
+        flags |= 4;
+        const float h = (float)k.halfLen;
+        const float eh = (float)fabs((double)h - k.halfLen);
+        g = guard_segment(k.contact2, fm, to_f3(a1), em, fj, to_f3(a2), ej, h, eh, segf);
+    }
+    if (!VERIFY) {
+        if (g == GUARD_UNDECIDED) o64 = fp64(d2_64, seg64);
+        out[i] = (g == GUARD_UNDECIDED ? o64 : g == GUARD_OVERLAP) ? 1 : 0;
+        return;
+    }
+    const bool o = g == GUARD_UNDECIDED ? o64 : g == GUARD_OVERLAP;
+    if (o != o64) flags |= 8;
+    out[i] = o ? 1 : 0;
+    code[i] = (uint8_t)(g | flags);
+    errC[i] = (float)fabs((double)d2f - d2_64);
+    errS[i] = (flags & 4) && seg64 >= 0 && segf >= 0 ? (float)fabs((double)segf - seg64) : -1.f;
+}
+
+hipError_t launch_pair_decisions_guarded(hipStream_t s, PairConsts k, size_t n, const double *rec, uint8_t *out,
+                                         uint8_t *code, float *errC, float *errS) {
+    if (n == 0) return hipSuccess;
+    unsigned blocks = (unsigned)((n + 255) / 256);
+    if (code)
+        hipLaunchKernelGGL(k_pair_decisions_guarded<true>, dim3(blocks), dim3(256), 0, s, k, n, rec, out, code, errC,
+                           errS);
+    else
+        hipLaunchKernelGGL(k_pair_decisions_guarded<false>, dim3(blocks), dim3(256), 0, s, k, n, rec, out, code,
+                           errC, errS);
+    return hipGetLastError();
+}
+
 __global__ __launch_bounds__(256) void k_pair_energies(PairConsts k, size_t n, const double *__restrict__ rec,
                                                        double *__restrict__ out) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -282,4 +282,120 @@ RP_HD double philox_uniform(Philox &p, double a, double b) { return a + philox_u
 // Uniform integer in [0, n) (multiply-shift; bias < n / 2^32).
 RP_HD uint32_t philox_below(Philox &p, uint32_t n) { return (uint32_t)(((uint64_t)philox_u32(p) * n) >> 32); }
 
+// ---------------------------------------------------------------------------------------------------------------
+// Guarded FP32 pair decisions.  Distances are evaluated in FP32 on coordinates taken relative to a reference point,
+// each with a bound on its error against the reference's FP64 value; a pair whose FP32 value lies farther from the
+// decision threshold than the bound is decided in FP32, any other pair is left to the FP64 path, so the decision is
+// always the FP64 one.  Every bound is per component and uses u = 2^-24 (FP32 unit roundoff) with a 1 % margin.
+//   Centre distance: d2f = |fj - fm|^2 from FP32 coordinates fm, fj that are roundings of FP64 differences; the bound
+//   covers both input roundings, the subtraction and the three-term sum.
+//   Segment distance (spherocylinders): the squared distance between points s, t of the two axis segments is a convex
+//   quadratic on [0, 1]^2, so at ANY (s, t) its value bounds the minimum from above and its value plus the minimum of
+//   its linear part over the corners bounds the minimum from below.  (s, t) come from the FP32 form of the reference's
+//   algorithm (SegmentDistanceCalculator.h:22-85) but need not be exact; only the evaluation at (s, t) and the
+//   perturbation of the four cap points by the FP32 roundings (which moves the minimum distance by at most twice the
+//   largest cap displacement) enter the bounds.  Nearly parallel pairs (D < 1e-3 a c) go to FP64: there the reference's
+//   own parallel branch (D < 1e-10) need not return the minimum.
+constexpr float GUARD_U = 5.9604645e-08f * 1.01f;  // 2^-24 with margin
+constexpr double GUARD_SLACK64 = 1e-6;             // the reference's FP64 rounding and EPS-clamped parameters
+
+enum GuardResult : int { GUARD_NO = 0, GUARD_OVERLAP = 1, GUARD_UNDECIDED = 2 };
+
+struct F3 { float x, y, z; };
+RP_HD F3 f3(float x, float y, float z) { F3 r; r.x = x; r.y = y; r.z = z; return r; }
+RP_HD F3 fadd(const F3 &a, const F3 &b) { return f3(a.x + b.x, a.y + b.y, a.z + b.z); }
+RP_HD F3 fsub(const F3 &a, const F3 &b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
+RP_HD F3 fscal(const F3 &a, float s) { return f3(a.x * s, a.y * s, a.z * s); }
+RP_HD float fdot(const F3 &a, const F3 &b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+RP_HD F3 fabs3(const F3 &a) { return f3(fabsf(a.x), fabsf(a.y), fabsf(a.z)); }
+// FP32 rounding of an FP64 coordinate and the bound on |result - exact|: u |x| (the FP64 input is exact data, or an
+// FP64 difference whose own rounding, 2^-53 |x|, the margin covers)
+RP_HD F3 to_f3(const V3 &a) { return f3((float)a.x, (float)a.y, (float)a.z); }
+RP_HD F3 round_err(const F3 &a) { return fscal(fabs3(a), GUARD_U); }
+
+// Centre-distance stage.  fm, fj: FP32 roundings of (pos - r0); em, ej their error bounds.  Returns d2f and sets
+// `bound` so that |d2f - d2_64| <= bound for the reference's FP64 d2 of the same pair.
+RP_HD float guard_centre(const F3 &fm, const F3 &em, const F3 &fj, const F3 &ej, float &bound) {
+    const F3 d = fsub(fj, fm);
+    const F3 ad = fabs3(d);
+    const F3 e = fadd(fadd(em, ej), fscal(ad, GUARD_U));  // inputs + the subtraction
+    const float d2 = fdot(d, d);
+    bound = ((2.f * ad.x + e.x) * e.x + (2.f * ad.y + e.y) * e.y + (2.f * ad.z + e.z) * e.z) * 1.01f +
+            3.1f * GUARD_U * d2 + 1e-12f;
+    return d2;
+}
+
+// Segment stage for a spherocylinder pair whose centre distance is certainly within (contact, range).  fm, am: FP32
+// centre (relative) and axis of the first particle, em its position error bound; fj, aj, ej likewise; h = L / 2 in
+// FP32 with its rounding error eh.  Returns GUARD_NO, GUARD_OVERLAP or GUARD_UNDECIDED; d2s receives the FP32
+// squared segment distance at the chosen (s, t) (for error statistics).
+RP_HD int guard_segment(double contact2, const F3 &fm, const F3 &am, const F3 &em, const F3 &fj, const F3 &aj,
+                        const F3 &ej, float h, float eh, float &d2s) {
+    // cap points in FP32 and the bound on their displacement from the reference's caps (relative to r0)
+    const F3 ha = fscal(am, h), hj = fscal(aj, h);
+    const F3 ca = fsub(fm, ha), cb = fadd(fm, ha), da = fsub(fj, hj), db = fadd(fj, hj);
+    auto capErr = [&](const F3 &c, const F3 &p, const F3 &ep, const F3 &a, const F3 &hh) {
+        // position, axis rounding times h, h rounding times |axis|, the product and the sum
+        const F3 aa = fabs3(a);
+        const float x = ep.x + GUARD_U * aa.x * h + eh * aa.x + GUARD_U * fabsf(hh.x) + GUARD_U * fabsf(c.x);
+        const float y = ep.y + GUARD_U * aa.y * h + eh * aa.y + GUARD_U * fabsf(hh.y) + GUARD_U * fabsf(c.y);
+        const float z = ep.z + GUARD_U * aa.z * h + eh * aa.z + GUARD_U * fabsf(hh.z) + GUARD_U * fabsf(c.z);
+        (void)p;
+        return (x + y + z) * 1.01f + 1e-9f;  // >= the Euclidean displacement
+    };
+    const float delta = fmaxf(fmaxf(capErr(ca, fm, em, am, ha), capErr(cb, fm, em, am, ha)),
+                              fmaxf(capErr(da, fj, ej, aj, hj), capErr(db, fj, ej, aj, hj)));
+    // the reference's parametrisation: S1 = ca + s u, S2 = da + t v, w = ca - da
+    const F3 u = fsub(cb, ca), v = fsub(db, da), w = fsub(ca, da);
+    const float a = fdot(u, u), b = fdot(u, v), c = fdot(v, v), d = fdot(u, w), e = fdot(v, w);
+    const float D = a * c - b * b;
+    if (!(D >= 1e-3f * a * c)) return GUARD_UNDECIDED;  // nearly parallel (or degenerate)
+    // FP32 form of SegmentDistanceCalculator.h:22-85 (general branch); any (s, t) in [0, 1]^2 gives valid bounds
+    float sN = b * e - c * d, sD = D, tN = a * e - b * d, tD = D;
+    if (sN < 0.f) { sN = 0.f; tN = e; tD = c; }
+    else if (sN > sD) { sN = sD; tN = e + b; tD = c; }
+    if (tN < 0.f) {
+        tN = 0.f;
+        if (-d < 0.f) sN = 0.f;
+        else if (-d > a) sN = sD;
+        else { sN = -d; sD = a; }
+    } else if (tN > tD) {
+        tN = tD;
+        if ((-d + b) < 0.f) sN = 0.f;
+        else if ((-d + b) > a) sN = sD;
+        else { sN = -d + b; sD = a; }
+    }
+    const float sc = fminf(fmaxf(sN / sD, 0.f), 1.f), tc = fminf(fmaxf(tN / tD, 0.f), 1.f);
+    // residual at (sc, tc) and its error against the exact residual of the FP32 cap points
+    const F3 wsu = fadd(w, fscal(u, sc));
+    const F3 r = fsub(wsu, fscal(v, tc));
+    const F3 aw = fabs3(w), au = fabs3(u), av = fabs3(v), ar = fabs3(r);
+    const F3 er = fscal(fadd(fadd(fadd(aw, au), fadd(av, fabs3(fscal(u, sc)))),
+                             fadd(fadd(fabs3(wsu), fabs3(fscal(v, tc))), ar)),
+                        GUARD_U * 1.01f);
+    const float f = fdot(r, r);
+    const float ef = ((2.f * ar.x + er.x) * er.x + (2.f * ar.y + er.y) * er.y + (2.f * ar.z + er.z) * er.z) * 1.01f +
+                     3.1f * GUARD_U * f;
+    d2s = f;
+    // gradient of |w + s u - t v|^2 at (sc, tc): (2 u.r, -2 v.r), and the minimum of its linear part over [0, 1]^2
+    const float gs = 2.f * fdot(u, r), gt = -2.f * fdot(v, r);
+    const float egs = 2.f * (fdot(au, er) + fdot(fadd(ar, er), fscal(au, GUARD_U)) + 3.1f * GUARD_U * fdot(au, ar));
+    const float egt = 2.f * (fdot(av, er) + fdot(fadd(ar, er), fscal(av, GUARD_U)) + 3.1f * GUARD_U * fdot(av, ar));
+    const float ls = fminf(-gs * sc, gs * (1.f - sc)), lt = fminf(-gt * tc, gt * (1.f - tc));
+    // bounds on the minimum squared distance of the FP32 cap geometry, in FP64 from here on
+    const double ub = (double)f + (double)ef;
+    const double lb = (double)f - (double)ef + (double)ls + (double)lt - (double)egs - (double)egt -
+                      2.1 * (double)GUARD_U * ((double)fabsf(gs) + (double)fabsf(gt)) -
+                      4.0 * (double)GUARD_U * ((double)f + (double)fabsf(ls) + (double)fabsf(lt));
+    // minimum distance of the reference geometry: within 2 delta of the FP32 geometry's
+    const double dd = 2.0 * (double)delta;
+    const double hiDist = sqrt(ub) + dd;
+    if (hiDist * hiDist + GUARD_SLACK64 < contact2) return GUARD_OVERLAP;
+    if (lb > 0) {
+        const double loDist = sqrt(lb) - dd;
+        if (loDist > 0 && loDist * loDist - GUARD_SLACK64 >= contact2) return GUARD_NO;
+    }
+    return GUARD_UNDECIDED;
+}
+
 }  // namespace rp

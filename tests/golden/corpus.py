@@ -143,6 +143,26 @@ def spherocylinder_contact_corpus(n: int, seed: int, radius: float = 0.5, length
     return np.ascontiguousarray(rec)
 
 
+def chunked_contact_corpus(n: int, seed: int, radius: float = 0.5, length: float = 5.0, delta: float = 1e-6,
+                           chunk: int = 1000000, threads: int = 8, box: np.ndarray = BOX) -> np.ndarray:
+    """A large spherocylinder corpus made of chunks of `chunk` records; chunk k is
+    spherocylinder_contact_corpus(chunk, [seed, k]) (its own PCG64 stream), so the result is the same for any number
+    of threads.  numpy releases the interpreter lock inside the array operations, so threads run the bisection of
+    different chunks concurrently without starting processes."""
+    from concurrent.futures import ThreadPoolExecutor
+    rec = np.empty((n, 27))
+    starts = list(range(0, n, chunk))
+
+    def make(k):
+        m = min(chunk, n - starts[k])
+        rec[starts[k]:starts[k] + m] = spherocylinder_contact_corpus(m, [seed, k], radius=radius, length=length,
+                                                                     delta=delta, box=box)
+
+    with ThreadPoolExecutor(max_workers=max(1, threads)) as ex:
+        list(ex.map(make, range(len(starts))))
+    return rec
+
+
 def sphere_contact_corpus(n: int, seed: int, radius: float = 0.5, delta: float = 1e-6,
                           box: np.ndarray = BOX) -> np.ndarray:
     rng = np.random.Generator(np.random.PCG64(seed))

@@ -1,7 +1,7 @@
 """Writes tests/golden/golden.json from the reference's own hot path (oracle/_ref/libref.so, compiled from the
 reference sources by oracle/Makefile).  Run in the container that holds the reference tree:
 
-    make -C oracle ref oracle && python tests/golden/make_golden.py
+    make -C oracle ref oracle && python tests/golden/make_golden.py [--large-only]
 
 The fixture stores only data: generator parameters, SHA-256 of the generated inputs, the reference's decision
 bitmaps and the reference's overlap counts / energies for deterministic snapshots.
@@ -9,6 +9,7 @@ bitmaps and the reference's overlap counts / energies for deterministic snapshot
 from __future__ import annotations
 
 import base64
+import hashlib
 import json
 import os
 import sys
@@ -29,6 +30,12 @@ CORPORA = [
     ("sc_contact_ulp", 1, 0.5, 5.0, 200000, 99, 2e-15),
     ("sphere_contact_1e-6", 0, 0.5, 0.0, 100000, 20261203, 1e-6),
     ("sphere_contact_ulp", 0, 0.5, 0.0, 200000, 5, 2e-16),
+]
+
+# name, kind, radius, length, n, seed, delta, chunk: corpora too large to commit as bitmaps; the fixture keeps the
+# SHA-256 of the inputs and of the reference's packed decision bitmap (corpus.chunked_contact_corpus)
+LARGE_CORPORA = [
+    ("sc_contact_1e7", 1, 0.5, 5.0, 10000000, 20261202, 1e-6, 250000),
 ]
 
 TRICLINIC = np.array([[30.0, 1.5, -0.5], [0.0, 29.0, 0.75], [0.0, 0.0, 31.0]])
@@ -90,8 +97,28 @@ def snapshot_arrays(kind, box, n, seed, parallel):
     return corpus.random_snapshot(n, seed, box, kind="sphere" if kind == 0 else "spherocylinder", parallel=parallel)
 
 
+def large_corpora(ref):
+    out = []
+    for name, kind, radius, length, n, seed, delta, chunk in LARGE_CORPORA:
+        rec = corpus.chunked_contact_corpus(n, seed, radius=radius, length=length, delta=delta, chunk=chunk)
+        dec = ref.pair_decisions(kind, radius, length, rec)
+        out.append(dict(name=name, kind=kind, radius=radius, length=length, n=n, seed=seed, delta=delta, chunk=chunk,
+                        sha256=corpus.records_sha256(rec), overlaps=int(dec.sum()),
+                        decisions_sha256=hashlib.sha256(np.packbits(dec).tobytes()).hexdigest()))
+        print(name, int(dec.sum()), "/", n)
+    return out
+
+
 def main():
     ref = oracle_lib.ref()
+    path = os.path.join(HERE, "golden.json")
+    if "--large-only" in sys.argv:
+        with open(path) as f:
+            out = json.load(f)
+        out["large_corpora"] = large_corpora(ref)
+        with open(path, "w") as f:
+            json.dump(out, f, indent=1)
+        return
     out = {"corpora": [], "snapshots": []}
     for name, kind, radius, length, n, seed, delta in CORPORA:
         rec = corpus_records(kind, radius, length, n, seed, delta)
@@ -125,9 +152,10 @@ def main():
                                           sha256=corpus.records_sha256(np.concatenate([pos.ravel(), rot.ravel()])),
                                           energy=energy))
         print(name, energy)
+    out["large_corpora"] = large_corpora(ref)
     out["statistics"] = [reference_statistics(ref)]
     print(out["statistics"])
-    with open(os.path.join(HERE, "golden.json"), "w") as f:
+    with open(path, "w") as f:
         json.dump(out, f, indent=1)
 
 

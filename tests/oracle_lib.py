@@ -48,6 +48,8 @@ def _setup_common(lib):
     lib.orc_rotation.restype = None
     lib.orc_nematic_p2.argtypes = [ctypes.c_size_t, _dp]
     lib.orc_nematic_p2.restype = ctypes.c_double
+    lib.orc_pair_guarded.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_size_t, _dp, _u8p,
+                                     _u8p, ctypes.c_double]
 
 
 class Checker:
@@ -206,3 +208,13 @@ def ref_ramtrj_read(checker: Checker, path, n, max_snap, expected_n=0, jump_cycl
                                       boxes.ctypes.data_as(_dp), pos.ctypes.data_as(_dp), rot.ctypes.data_as(_dp),
                                       info.ctypes.data_as(_u64p))
     return got, boxes[:max(got, 0)], pos[:max(got, 0)], rot[:max(got, 0)], info
+
+
+def pair_guarded(kind: int, radius: float, length: float, rec, bound_scale: float = 1.0):
+    """The oracle's guarded FP32 pair decisions: (decisions, codes) as rp_pair_decisions_guarded returns them."""
+    rec = np.ascontiguousarray(rec, dtype=np.float64).reshape(-1, 27)
+    dec = np.zeros(len(rec), dtype=np.uint8)
+    codes = np.zeros(len(rec), dtype=np.uint8)
+    oracle().lib.orc_pair_guarded(kind, radius, length, len(rec), rec.ctypes.data_as(_dp), dec.ctypes.data_as(_u8p),
+                                  codes.ctypes.data_as(_u8p), bound_scale)
+    return dec, codes
