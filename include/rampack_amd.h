@@ -228,7 +228,11 @@ int rp_download(rp_ctx *ctx, double box[9], double *pos, double *rot);
  * results agree to the rounding of an N-term sum), the 3x3 eigenvalue step on the host. */
 int rp_get_observables(rp_ctx *ctx, rp_observables *out);
 
-/* Turns event timing of the sweep kernels on or off and reads the accumulated statistics. */
+/* Event timing of the cycle's kernels (no reference counterpart; the reference's CasinoMode.cpp:400-442 timers
+ * measure whole phases on the host).  enable: 0 off; 1 every sweep, rebuild, overlap-scan and proposal launch;
+ * RP_TIMING_SAMPLED one sweep launch per cycle (the colour pass whose index is the cycle number modulo the colour
+ * count), which leaves the other launches untouched.  rp_get_timing reads the accumulated statistics. */
+#define RP_TIMING_SAMPLED 2
 int rp_set_timing(rp_ctx *ctx, int enable);
 int rp_get_timing(rp_ctx *ctx, rp_timing *out);
 

@@ -349,8 +349,10 @@ class GpuPacking:
         _check(_lib.rp_download(self._ctx, _dptr(box), _dptr(pos), _dptr(rot)))
         return box.reshape(3, 3), pos, rot
 
-    def set_timing(self, enable: bool):
-        _check(_lib.rp_set_timing(self._ctx, 1 if enable else 0))
+    def set_timing(self, enable):
+        """True: event timing of every sweep, rebuild, scan and proposal launch; "sampled": one sweep launch per cycle
+        (rotating through the colours); False: off."""
+        _check(_lib.rp_set_timing(self._ctx, 2 if enable == "sampled" else (1 if enable else 0)))
 
     def timing(self) -> dict:
         t = Timing()

@@ -129,7 +129,7 @@ struct rp_ctx {
     int pipelined = -1;                  // staged sweep: 0 plain, 1 planner/tester split, 2 grouped moves, -1 auto
 
     // timing
-    bool timing = false;
+    int timing = 0;  // 0 off, 1 every launch, 2 one sweep launch per cycle (RP_TIMING_SAMPLED)
     std::vector<hipEvent_t> evStart, evStop;
     std::vector<int> evKind;
     size_t evUsed = 0;
@@ -311,11 +311,13 @@ int makeExactGrid(rp_ctx *c, const HostBox &b, Grid &g, int &disabled) {
     return RP_OK;
 }
 
-void timingBegin(rp_ctx *c, int kind) {
-    if (!c->timing) return;
+// Events for one timed launch of the given kind (0 sweep, 1 rebuild, 2 overlap scan, 3 proposals), or none when
+// timing is off.  The launch itself records them (hipExtLaunchKernel), so timing adds no packets between kernels.
+Timed timingSlot(rp_ctx *c, int kind) {
+    if (!c->timing || (c->timing == RP_TIMING_SAMPLED && kind != 0)) return Timed{};
     if (c->evUsed == c->evStart.size()) {
         // timing only, read after a stream synchronisation: no system-scope fence (cache write-back and
-        // invalidation) per record, which would slow the work between the events
+        // invalidation) per record
         hipEvent_t a, b;
         hipEventCreateWithFlags(&a, hipEventDisableSystemFence);
         hipEventCreateWithFlags(&b, hipEventDisableSystemFence);
@@ -324,13 +326,11 @@ void timingBegin(rp_ctx *c, int kind) {
         c->evKind.push_back(0);
     }
     c->evKind[c->evUsed] = kind;
-    hipEventRecord(c->evStart[c->evUsed], c->stream);
-}
-
-void timingEnd(rp_ctx *c) {
-    if (!c->timing) return;
-    hipEventRecord(c->evStop[c->evUsed], c->stream);
+    Timed t;
+    t.start = c->evStart[c->evUsed];
+    t.stop = c->evStop[c->evUsed];
     c->evUsed++;
+    return t;
 }
 
 int timingHarvest(rp_ctx *c) {
@@ -349,10 +349,8 @@ int timingHarvest(rp_ctx *c) {
 }
 
 int rebuild(rp_ctx *c, const Grid &g, int rescale, const double *oldInv) {
-    timingBegin(c, 1);
     RP_HIP(launch_rebuild(c->stream, g, (int)c->n, c->cur.p, c->alt.p, c->tmpSlot, c->tmpOrder, c->tmpPos, rescale,
-                          oldInv));
-    timingEnd(c);
+                          oldInv, timingSlot(c, 1)));
     return RP_OK;
 }
 
@@ -376,14 +374,13 @@ int countCapLimit(rp_ctx *c) {
 // Early-exit overlap scan of the `alt` arrays sorted by `g`; sets found.
 int scanOverlaps(rp_ctx *c, const Grid &g, int earlyExit, uint64_t &count) {
     RP_HIP(launch_fill_u64(c->stream, c->dCounters + CNT_OVERLAPS, 1, 0));
-    timingBegin(c, 2);
     double lambda = (double)c->n / (double)g.ncell;
     const int side = 2 * g.reach + 1;
     double expect = ((side * side * side - 1) / 2 + 1) * lambda;
     int cap = (int)std::ceil(expect + 8 * std::sqrt(expect) + 64);
     cap = std::min((cap + 63) / 64 * 64, countCapLimit(c));
-    RP_HIP(launch_count_overlaps(c->stream, g, (int)c->n, c->alt.p, c->k, earlyExit, c->dCounters, cap));
-    timingEnd(c);
+    RP_HIP(launch_count_overlaps(c->stream, g, (int)c->n, c->alt.p, c->k, earlyExit, c->dCounters, cap,
+                                 timingSlot(c, 2)));
     int r = readCounters(c);
     if (r) return r;
     if (c->hCounters[CNT_UNSTAGED_BLOCKS] >> 32)
@@ -621,10 +618,8 @@ int runCycle(rp_ctx *c) {
         if (r2) return r2;
     }
     sp.pr = c->prop;
-    timingBegin(c, 3);
     RP_HIP(launch_proposals(c->stream, g, sp.pr, sp.moves, c->cur.p.cell, (int)c->n, ts, rs, c->seed,
-                            (uint32_t)c->cycle));
-    timingEnd(c);
+                            (uint32_t)c->cycle, timingSlot(c, 3)));
     const int side = 2 * g.reach + 1;
     double expect = side * side * side * lambda;
     int cap = (int)std::ceil(expect + 8 * std::sqrt(expect) + 64);
@@ -661,15 +656,15 @@ int runCycle(rp_ctx *c) {
         // a colour with odd parity on a one-cell axis has no cells
         const int par[3] = {sp.colour % g.period, (sp.colour / g.period) % g.period, sp.colour / (g.period * g.period)};
         if ((g.n[0] == 1 && par[0]) || (g.n[1] == 1 && par[1]) || (g.n[2] == 1 && par[2])) continue;
-        timingBegin(c, 0);
         const bool trace = !c->tracePath.empty() && ++c->sweepLaunches == c->traceLaunch;
         if (trace) {
             r = traceSweep(c, sp, waves);
             if (r) return r;
         } else {
-            RP_HIP(launch_sweep(c->stream, sp, waves));
+            // sampled timing: the i-th colour pass of cycle c is timed when i == c mod (colour count)
+            const bool timed = c->timing == 1 || (c->timing == RP_TIMING_SAMPLED && i == (int)(c->cycle % colours));
+            RP_HIP(launch_sweep(c->stream, sp, waves, timed ? timingSlot(c, 0) : Timed{}));
         }
-        timingEnd(c);
     }
     c->sortedForCycle = false;
     if (sp.counting) {
@@ -1064,7 +1059,8 @@ int rp_set_timing(rp_ctx *c, int enable) {
     if (!c) return fail(RP_ERR_PRECONDITION, "null context");
     int r = timingHarvest(c);
     if (r) return r;
-    c->timing = enable != 0;
+    if (enable < 0 || enable > RP_TIMING_SAMPLED) return fail(RP_ERR_PRECONDITION, "unknown timing mode");
+    c->timing = enable;
     rp_timing keep = c->tstats;
     c->tstats = rp_timing{};
     c->tstats.lds_limit = keep.lds_limit;

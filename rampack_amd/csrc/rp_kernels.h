@@ -103,21 +103,26 @@ enum Counter : int {
 hipError_t launch_pair_decisions(hipStream_t s, PairConsts k, size_t n, const double *rec, uint8_t *out);
 hipError_t launch_pair_decisions_guarded(hipStream_t s, PairConsts k, size_t n, const double *rec, uint8_t *out,
                                          uint8_t *code, float *errC, float *errS);
+// Start / stop events carried by a launch's own dispatch (hipExtLaunchKernel): the kernel's begin and end times
+// are recorded with no marker packets between kernels.  Null events: an untimed launch.
+struct Timed {
+    hipEvent_t start = nullptr, stop = nullptr;
+};
 hipError_t launch_fill_u64(hipStream_t s, unsigned long long *p, size_t n, unsigned long long v);
 hipError_t launch_fill_i32(hipStream_t s, int *p, size_t n, int v);
 // Cell-list rebuild: src -> dst sorted by cell of `g`.  When rescale != 0 the positions are first mapped
 // newDims * (oldInv * pos) (Packing.cpp:183-184) with newDims = g.dims.
 hipError_t launch_rebuild(hipStream_t s, const Grid &g, int n, Particles src, Particles dst, int *tmpSlot,
-                          int *tmpOrder, double *tmpPos, int rescale, const double *oldInv);
+                          int *tmpOrder, double *tmpPos, int rescale, const double *oldInv, Timed t = {});
 // Half-shell overlap count over the grid (Packing.cpp:431-559).  Adds to counters[CNT_OVERLAPS]; with
 // earlyExit the workgroups stop once any overlap is found.  cap: LDS entries per workgroup (>= largest cell).
 hipError_t launch_count_overlaps(hipStream_t s, const Grid &g, int n, Particles p, PairConsts k, int earlyExit,
-                                 unsigned long long *counters, int cap);
+                                 unsigned long long *counters, int cap, Timed t = {});
 size_t count_lds_bytes(int kind, int cap);
 // Proposals of one cycle (translation step ts, rotation step rs): `moves` per cell when moves > 0, else one per
 // particle of its cell (cellOf = cell of every sorted record, n records).
 hipError_t launch_proposals(hipStream_t s, const Grid &g, const Proposals &pr, int moves, const int *cellOf, int n,
-                            double ts, double rs, uint64_t seed, uint32_t cycle);
+                            double ts, double rs, uint64_t seed, uint32_t cycle, Timed t = {});
 // waves: wavefronts per active cell (1, 2 or 4)
 // Soft pair energies on records pos1[3] O1[9] pos2[3] O2[9] T[3].
 hipError_t launch_pair_energies(hipStream_t s, PairConsts k, size_t n, const double *rec, double *out);
@@ -139,7 +144,7 @@ hipError_t launch_fix_all(hipStream_t s, const Particles &p, int n);
 
 // Workgroups of the sweep kernel one compute unit can hold at this LDS size (occupancy API).
 int sweep_occupancy(int kind, int cap, int waves);
-hipError_t launch_sweep(hipStream_t s, const SweepParams &sp, int waves);
+hipError_t launch_sweep(hipStream_t s, const SweepParams &sp, int waves, Timed t = {});
 size_t sweep_lds_bytes(int kind, int cap, int waves);
 
 }  // namespace rp

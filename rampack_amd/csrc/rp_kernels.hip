@@ -15,6 +15,8 @@
 //                        Packing.cpp:147-148)
 #include "rp_kernels.h"
 
+#include <hip/hip_ext.h>
+
 #include <cfloat>
 
 namespace rp {
@@ -371,7 +373,7 @@ __global__ __launch_bounds__(WAVE) void k_gather(Grid g, int n, Particles src, P
 }
 
 hipError_t launch_rebuild(hipStream_t s, const Grid &g, int n, Particles src, Particles dst, int *tmpSlot,
-                          int *tmpOrder, double *tmpPos, int rescale, const double *oldInv) {
+                          int *tmpOrder, double *tmpPos, int rescale, const double *oldInv, Timed t) {
     // tmpSlot holds 2n ints (slot, cell); tmpOrder holds n ints followed by n bytes of wrap bits.
     // the cell counts are zero here: cleared when allocated and by the scan of every rebuild
     int *tmpCell = tmpSlot + n;
@@ -380,11 +382,13 @@ hipError_t launch_rebuild(hipStream_t s, const Grid &g, int n, Particles src, Pa
     if (rescale)
         for (int i = 0; i < 9; i++) inv.m[i] = oldInv[i];
     unsigned blocks = (unsigned)((n + 255) / 256);
-    hipLaunchKernelGGL(k_bin, dim3(blocks), dim3(256), 0, s, g, n, src, rescale, inv, tmpPos, tmpCell, tmpSlot, tmpW);
+    hipExtLaunchKernelGGL(k_bin, dim3(blocks), dim3(256), 0, s, t.start, nullptr, 0, g, n, src, rescale, inv, tmpPos,
+                          tmpCell, tmpSlot, tmpW);
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, g.ncell, g.cellCount, g.cellStart,
                        (unsigned long long *)nullptr);
     hipLaunchKernelGGL(k_scatter, dim3(blocks), dim3(256), 0, s, n, g.cellStart, tmpCell, tmpSlot, tmpOrder);
-    hipLaunchKernelGGL(k_gather, dim3((unsigned)g.ncell), dim3(WAVE), 0, s, g, n, src, dst, tmpPos, tmpW, tmpOrder);
+    hipExtLaunchKernelGGL(k_gather, dim3((unsigned)g.ncell), dim3(WAVE), 0, s, nullptr, t.stop, 0, g, n, src, dst,
+                          tmpPos, tmpW, tmpOrder);
     return hipGetLastError();
 }
 
@@ -702,15 +706,15 @@ size_t count_lds_bytes(int kind, int cap) {
 }
 
 hipError_t launch_count_overlaps(hipStream_t s, const Grid &g, int n, Particles p, PairConsts k, int earlyExit,
-                                 unsigned long long *counters, int cap) {
+                                 unsigned long long *counters, int cap, Timed t) {
     if (n == 0) return hipSuccess;
     size_t lds = count_lds_bytes(k.kind, cap);
     if (k.kind == 0)
-        hipLaunchKernelGGL(k_count_overlaps<0>, dim3((unsigned)g.ncell), dim3(COUNT_THREADS), lds, s, g, p, k,
-                           earlyExit, counters, cap);
+        hipExtLaunchKernelGGL(k_count_overlaps<0>, dim3((unsigned)g.ncell), dim3(COUNT_THREADS), (uint32_t)lds, s,
+                              t.start, t.stop, 0, g, p, k, earlyExit, counters, cap);
     else
-        hipLaunchKernelGGL(k_count_overlaps<1>, dim3((unsigned)g.ncell), dim3(COUNT_THREADS), lds, s, g, p, k,
-                           earlyExit, counters, cap);
+        hipExtLaunchKernelGGL(k_count_overlaps<1>, dim3((unsigned)g.ncell), dim3(COUNT_THREADS), (uint32_t)lds, s,
+                              t.start, t.stop, 0, g, p, k, earlyExit, counters, cap);
     return hipGetLastError();
 }
 
@@ -903,11 +907,11 @@ __global__ __launch_bounds__(256) void k_proposals(Grid g, Proposals pr, int mov
 }
 
 hipError_t launch_proposals(hipStream_t s, const Grid &g, const Proposals &pr, int moves, const int *cellOf, int n,
-                            double ts, double rs, uint64_t seed, uint32_t cycle) {
+                            double ts, double rs, uint64_t seed, uint32_t cycle, Timed t) {
     size_t total = moves > 0 ? (size_t)g.ncell * moves : (size_t)n;
     if (total == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_proposals, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, g, pr, moves, cellOf, n,
-                       ts, rs, seed, cycle);
+    hipExtLaunchKernelGGL(k_proposals, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, t.start, t.stop, 0, g,
+                          pr, moves, cellOf, n, ts, rs, seed, cycle);
     return hipGetLastError();
 }
 
@@ -2132,13 +2136,14 @@ int sweep_occupancy(int kind, int cap, int waves) {
     return blocks;
 }
 
-hipError_t launch_sweep(hipStream_t s, const SweepParams &sp, int waves) {
+hipError_t launch_sweep(hipStream_t s, const SweepParams &sp, int waves, Timed t) {
     unsigned blocks = (unsigned)(sp.na[0] * sp.na[1] * sp.na[2]);
     int w = waves >= 8 ? 8 : (waves >= 4 ? 4 : (waves == 2 ? 2 : 1));
     if (w == 8 && (sp.k.soft || sp.counting)) w = 4;  // eight wavefronts only for the hard sweep
     size_t lds = sweep_lds_bytes(sp.k.kind, sp.cap, w);
     // soft and hard sweeps are separate kernels: each instantiation carries one path only
-#define RP_SWEEP(KIND, W, SOFT) hipLaunchKernelGGL((k_sweep<KIND, W, SOFT>), dim3(blocks), dim3(WAVE * W), lds, s, sp)
+#define RP_SWEEP(KIND, W, SOFT) \
+    hipExtLaunchKernelGGL((k_sweep<KIND, W, SOFT>), dim3(blocks), dim3(WAVE * W), (uint32_t)lds, s, t.start, t.stop, 0, sp)
 #define RP_SWEEP_W(KIND, SOFT)         \
     if (w == 4) RP_SWEEP(KIND, 4, SOFT); \
     else if (w == 2) RP_SWEEP(KIND, 2, SOFT); \

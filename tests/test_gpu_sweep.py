@@ -200,6 +200,25 @@ def test_both_colourings_are_overlap_free(period, monkeypatch):
     assert overlaps_by_oracle(SC, b, p, r) == 0
 
 
+def test_sampled_timing_times_one_sweep_per_cycle():
+    """Sampled timing: one sweep launch per cycle carries events, no other launch does, and the trajectory is the
+    untimed one."""
+    box, pos, rot = rod_start()
+    runs = []
+    for mode in ("sampled", False):
+        pk = ra.GpuPacking(SC, box, pos, rot, seed=77)
+        pk.set_environment(1.0, 2.0, 0.2, None, box_move=("delta_triclinic", 0.002))
+        pk.set_timing(mode)
+        pk.run_cycles(40)
+        t = pk.timing()
+        if mode:
+            assert t["sweep_launches"] == 40 and t["sweep_ms"] > 0
+            assert t["rebuild_launches"] == 0 and t["scan_launches"] == 0 and t["proposal_launches"] == 0
+        runs.append(pk.download())
+    for x, y in zip(runs[0], runs[1]):
+        assert np.array_equal(x, y)
+
+
 @pytest.mark.parametrize("kind", ["sphere", "spherocylinder"])
 def test_launch_shapes_give_identical_trajectories(kind, monkeypatch):
     """Proposals are drawn per (cycle, cell, move) and every decision is the same FP64 evaluation, so the
