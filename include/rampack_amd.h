@@ -148,6 +148,18 @@ int rp_count_total_overlaps(rp_ctx *ctx, int early_exit, uint64_t *out);
  * sum. */
 int rp_total_energy(rp_ctx *ctx, double *out);
 
+/* Packing::tryScaling(newBox, interaction) (src/core/Packing.cpp:172-208): positions become
+ * newBox * (oldBox^-1 * pos) and the scaled state is current on return.  *dE receives +INFINITY if the scaled
+ * packing has an overlap (the early-exit count, skipped when the box only grew), in overlap-counting mode
+ * -INFINITY / +INFINITY when the full count fell / grew, otherwise getTotalEnergy(new) - getTotalEnergy(old)
+ * (0 for hard shapes).  The state before the call is kept for rp_revert_scaling.  RP_ERR_PRECONDITION for a
+ * zero-volume box (Expects at Packing.cpp:173); RP_ERR_UNSUPPORTED when a box height is below two interaction
+ * ranges (the checkerboard grid needs two cells per axis).
+ * rp_revert_scaling: Packing::revertScaling (Packing.cpp:363-371), the state saved by the last rp_try_scaling
+ * (box, particles, cached overlap count) becomes current; RP_ERR_PRECONDITION if no scaling was tried. */
+int rp_try_scaling(rp_ctx *ctx, const double new_box[9], double *dE);
+int rp_revert_scaling(rp_ctx *ctx);
+
 /* Packing::renormalizeOrientations (src/core/Packing.cpp:1178-1216) as Simulation::fixRotationMatrices runs it
  * (Simulation.cpp:750-769): every orientation becomes three iterations of R <- 1.5 R - 0.5 R R^T R unless that
  * creates an overlap; *rejected receives the number of particles left unchanged.  rp_run_cycles runs it after every

@@ -46,6 +46,7 @@ EXPORTED_SYMBOLS = (
     "rp_ramtrj_recorder_open", "rp_ramtrj_record", "rp_ramtrj_record_packing", "rp_ramtrj_recorder_last_cycle",
     "rp_ramtrj_recorder_close", "rp_ramtrj_player_open", "rp_ramtrj_player_header", "rp_ramtrj_next_snapshot",
     "rp_ramtrj_jump_to_snapshot", "rp_ramtrj_player_close", "rp_pair_decisions_guarded",
+    "rp_try_scaling", "rp_revert_scaling",
 )
 
 
@@ -163,6 +164,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
                                       ctypes.POINTER(ctypes.c_uint8)]
     lib.rp_get_observables.argtypes = [vp, ctypes.POINTER(Observables)]
     lib.rp_total_energy.argtypes = [vp, dp]
+    lib.rp_try_scaling.argtypes = [vp, dp, dp]
+    lib.rp_revert_scaling.argtypes = [vp]
     lib.rp_set_overlap_counting.argtypes = [vp, ctypes.c_int]
     lib.rp_cached_overlaps.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64)]
     lib.rp_relax_overlaps.argtypes = [vp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t),
@@ -392,6 +395,19 @@ class GpuPacking:
         v = ctypes.c_double()
         _check(_lib.rp_total_energy(self._ctx, ctypes.byref(v)))
         return float(v.value)
+
+    def try_scaling(self, new_box) -> float:
+        """Packing::tryScaling (Packing.cpp:172-208): scales the device-resident state to new_box (3x3, columns are
+        the box vectors) and returns +inf on an overlap (-inf / +inf by the count change in overlap-counting mode),
+        else the total energy change (0 for hard shapes).  The scaled state stays current until revert_scaling."""
+        b = np.ascontiguousarray(new_box, dtype=np.float64).reshape(9)
+        v = ctypes.c_double()
+        _check(_lib.rp_try_scaling(self._ctx, _dptr(b), ctypes.byref(v)))
+        return float(v.value)
+
+    def revert_scaling(self):
+        """Packing::revertScaling (Packing.cpp:363-371): the state before the last try_scaling becomes current."""
+        _check(_lib.rp_revert_scaling(self._ctx))
 
     def observables(self) -> dict:
         """PackingFraction and NematicOrder of the device-resident state (rp_get_observables)."""

@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cmath>
+#include <limits>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -65,6 +66,7 @@ bool isBoxUpscaled(const HostBox &o, const HostBox &nb) {
 struct DeviceParticles {
     Particles p{};
     void *mem = nullptr;
+    size_t bytes = 0;  // the whole block; the arrays sit at the same offsets in every block of n particles
 };
 
 }  // namespace
@@ -116,6 +118,12 @@ struct rp_ctx {
     Grid grid{};
     bool sortedForCycle = false;
 
+    // state saved by rp_try_scaling for rp_revert_scaling (Packing::lastBox / lastShapes / lastScalingNumOverlaps)
+    DeviceParticles last;
+    bool scalingSaved = false;
+    HostBox lastBox;
+    long long lastNumOverlaps = 0;
+
     // counters
     uint64_t boxAttempted = 0, boxAccepted = 0, boxScans = 0, boxTooSmall = 0, cycles = 0;
     uint64_t accSinceEval = 0, attSinceEval = 0;      // host view used by the step adaptation
@@ -141,6 +149,7 @@ namespace {
 int allocParticles(DeviceParticles &d, size_t n) {
     size_t bytes = n * (12 * sizeof(double) + 2 * sizeof(int) + 1) + 4096;
     RP_HIP(hipMalloc(&d.mem, bytes));
+    d.bytes = bytes;
     char *q = static_cast<char *>(d.mem);
     auto take = [&](size_t b) {
         char *r = q;
@@ -1117,6 +1126,7 @@ void rp_destroy(rp_ctx *c) {
     for (auto e : c->evStop) hipEventDestroy(e);
     if (c->cur.mem) hipFree(c->cur.mem);
     if (c->alt.mem) hipFree(c->alt.mem);
+    if (c->last.mem) hipFree(c->last.mem);
     if (c->cellStart) hipFree(c->cellStart);
     if (c->cellCount) hipFree(c->cellCount);
     if (c->propMem) hipFree(c->propMem);
@@ -1250,6 +1260,80 @@ int rp_relax_overlaps(rp_ctx *c, size_t max_cycles, size_t *cycles_run, uint64_t
 int rp_set_orientation_fix_every(rp_ctx *c, uint64_t every) {
     if (!c) return fail(RP_ERR_PRECONDITION, "null context");
     c->fixEvery = every;
+    return RP_OK;
+}
+
+// Packing::tryScaling (Packing.cpp:172-208) on the device-resident state.  The state before the call is kept for
+// rp_revert_scaling; the scaled state becomes current whatever the result, as in the reference.
+int rp_try_scaling(rp_ctx *c, const double new_box[9], double *dE) {
+    if (!c || !new_box || !dE) return fail(RP_ERR_PRECONDITION, "null argument");
+    HostBox nb;
+    nb.set(new_box);
+    if (nb.volume() == 0) return fail(RP_ERR_PRECONDITION, "tryScaling: the new box has zero volume");
+    if (!c->last.mem) {
+        int r = allocParticles(c->last, c->n);
+        if (r) return r;
+    }
+    static constexpr double INF = std::numeric_limits<double>::infinity();
+    // energy of the current state, on a grid the current arrays are sorted by
+    double eOld = 0;
+    if (c->k.soft) {
+        int r = prepareCycleGrid(c);
+        if (r) return r;
+        r = totalEnergy(c, c->grid, c->cur.p, eOld);
+        if (r) return r;
+    }
+    // the scaled state, sorted by a checkerboard grid of the new box (no origin shift), into `alt`
+    Grid g{};
+    const double zero[3] = {0, 0, 0};
+    int r = chooseSweepGrid(c, nb, zero, g);
+    if (r) return r;
+    RP_HIP(hipMemcpyAsync(c->last.mem, c->cur.mem, c->cur.bytes, hipMemcpyDeviceToDevice, c->stream));
+    r = rebuild(c, g, 1, c->box.inv);
+    if (r) return r;
+    double result = 0;
+    long long count = c->numOverlaps;
+    if (!c->k.soft) {
+        if (c->counting) {
+            uint64_t after = 0;
+            r = scanOverlaps(c, g, 0, after);
+            if (r) return r;
+            count = (long long)after;
+            if (count < c->numOverlaps) result = -INF;
+            else if (count > c->numOverlaps) result = INF;
+        } else if (!isBoxUpscaled(c->box, nb)) {  // spheres and spherocylinders are convex
+            uint64_t found = 0;
+            r = scanOverlaps(c, g, 1, found);
+            if (r) return r;
+            if (found) result = INF;
+        }
+    } else {
+        double eNew = 0;
+        r = totalEnergy(c, g, c->alt.p, eNew);
+        if (r) return r;
+        result = eNew - eOld;
+    }
+    c->lastBox = c->box;
+    c->lastNumOverlaps = c->numOverlaps;
+    c->scalingSaved = true;
+    c->box = nb;
+    c->numOverlaps = count;
+    swapBuffers(c);
+    c->grid = g;
+    c->sortedForCycle = true;
+    *dE = result;
+    return RP_OK;
+}
+
+// Packing::revertScaling (Packing.cpp:363-371): the state saved by the last rp_try_scaling becomes current.
+int rp_revert_scaling(rp_ctx *c) {
+    if (!c) return fail(RP_ERR_PRECONDITION, "null context");
+    if (!c->scalingSaved) return fail(RP_ERR_PRECONDITION, "revertScaling without a preceding tryScaling");
+    RP_HIP(hipMemcpyAsync(c->cur.mem, c->last.mem, c->cur.bytes, hipMemcpyDeviceToDevice, c->stream));
+    c->box = c->lastBox;
+    c->numOverlaps = c->lastNumOverlaps;
+    c->sortedForCycle = false;  // the cell arrays describe the scaled state's grid: the next cycle re-sorts
+    RP_HIP(hipStreamSynchronize(c->stream));
     return RP_OK;
 }
 

@@ -303,6 +303,19 @@ public:
     [[nodiscard]] double getPackingFraction(double shapeVolume) const {
         return (double)this->size() / this->getVolume() * shapeVolume;
     }
+    // Packing::tryScaling(newBox, interaction) (Packing.cpp:172-208): +inf on an overlap, else the energy change;
+    // the scaled state is current until revertScaling
+    double tryScaling(const Box &newBox, const Interaction &) {
+        double dE = 0;
+        check(rp_try_scaling(ctx, newBox.data(), &dE));
+        stale = true;
+        return dE;
+    }
+    // Packing::revertScaling (Packing.cpp:363-371)
+    void revertScaling() {
+        check(rp_revert_scaling(ctx));
+        stale = true;
+    }
     // Packing::countTotalOverlaps (Packing.cpp:519-559)
     [[nodiscard]] std::size_t countTotalOverlaps(const Interaction &, bool earlyExit = true) const {
         uint64_t n = 0;

@@ -4,6 +4,7 @@
 #include "rampack_gpu.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <string>
 #include <cstdlib>
@@ -66,6 +67,21 @@ int main() {
     CHECK(phis.back() > 0.34 && phis.back() < 0.42);  // a hard-sphere fluid at p = 5 (CS: 0.393)
     CHECK(sim.getCounters().box_accepted > 0);
     CHECK(sim.getMoveAcceptanceRate() > 0.05 && sim.getMoveAcceptanceRate() < 0.6);
+
+    // Packing::tryScaling / revertScaling (Packing.cpp:172-208, 363-371) on the lattice
+    {
+        Packing pk(box, shapes, traits.getInteraction());
+        Box grown = box, crushed = box;
+        for (double &x : grown) x *= 1.05;
+        for (double &x : crushed) x *= 0.8;  // lattice spacing below the sphere diameter
+        CHECK(pk.tryScaling(grown, traits.getInteraction()) == 0);
+        CHECK(pk.getBox() == grown);
+        CHECK(std::isinf(pk.tryScaling(crushed, traits.getInteraction())));
+        CHECK(pk.getBox() == crushed);
+        pk.revertScaling();
+        CHECK(pk.getBox() == grown);
+        CHECK(pk.countTotalOverlaps(traits.getInteraction(), false) == 0);
+    }
 
     // a RAMTRJ recording of a run, replayed into another packing (SimulationIOTest.cpp:37-98)
     {
