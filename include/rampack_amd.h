@@ -31,9 +31,12 @@ extern "C" {
 #define RP_ERR_UNSUPPORTED (-4)
 
 /* Shape kinds: SphereTraits (src/core/shapes/SphereTraits.h:17-70), SpherocylinderTraits
- * (src/core/shapes/SpherocylinderTraits.h:16-86). */
+ * (src/core/shapes/SpherocylinderTraits.h:16-86), PolysphereTraits (src/core/shapes/PolysphereTraits.h, hard beads
+ * at fixed body-frame positions, PolysphereTraits.cpp:97-106). */
 #define RP_SHAPE_SPHERE 0
 #define RP_SHAPE_SPHEROCYLINDER 1
+#define RP_SHAPE_POLYSPHERE 2
+#define RP_MAX_CENTRES 16       /* beads per polysphere */
 
 /* Soft interactions, which replace the hard core:
  * RP_SOFT_WCA -- a sphere with RepulsiveLennardJonesInteraction (src/core/interactions/
@@ -67,6 +70,9 @@ typedef struct rp_shape_desc {
     int soft;       /* RP_SOFT_*; RP_SOFT_NONE: hard core */
     double epsilon; /* WCA energy scale (> 0) */
     double sigma;   /* WCA length scale (> 0) */
+    int n_centres;          /* polysphere: number of beads, 1..RP_MAX_CENTRES */
+    const double *centres;  /* polysphere: n_centres body-frame bead positions (x, y, z), host memory */
+    const double *radii;    /* polysphere: n_centres bead radii (> 0), host memory */
 } rp_shape_desc;
 
 /* RototranslationSampler descriptor (src/core/move_samplers/RototranslationSampler.h:17-60). */

@@ -590,6 +590,31 @@ int orc_pair_decisions(int kind, double radius, double length, std::size_t n, co
     return 0;
 }
 
+// Polysphere pair decisions on the same records: bead centres pos + O * c (PolysphereTraits::SphereData::
+// centreForShape, PolysphereTraits.cpp:93-95) and any bead pair overlapping by beadOverlap with the record's T.
+int orc_poly_pair_decisions(int nCentres, const double *centres, const double *radii, std::size_t n, const double *rec,
+                            std::uint8_t *out) {
+    for (std::size_t i = 0; i < n; i++) {
+        const double *q = rec + 27 * i;
+        Vec p1, p2, T;
+        Mat O1, O2;
+        std::memcpy(p1.v, q, 24); std::memcpy(O1.m, q + 3, 72);
+        std::memcpy(p2.v, q + 12, 24); std::memcpy(O2.m, q + 15, 72);
+        std::memcpy(T.v, q + 24, 24);
+        bool o = false;
+        for (int b = 0; b < nCentres && !o; b++) {
+            Vec cb{{centres[3 * b], centres[3 * b + 1], centres[3 * b + 2]}};
+            Vec c2 = add(p2, mulv(O2, cb));
+            for (int a = 0; a < nCentres && !o; a++) {
+                Vec ca{{centres[3 * a], centres[3 * a + 1], centres[3 * a + 2]}};
+                o = beadOverlap(radii[a], radii[b], add(p1, mulv(O1, ca)), c2, T);
+            }
+        }
+        out[i] = o ? 1 : 0;
+    }
+    return 0;
+}
+
 // Soft pair energies with a fixed translation per record (27 doubles as in orc_pair_decisions): soft 1 = WCA on the
 // centre distance (CentralInteraction.h:58-67 with HardcodedTranslation), soft 2 = WCA on the segment distance.
 int orc_pair_energies(int kind, double radius, double length, int soft, double eps, double sigma, std::size_t n,

@@ -21,6 +21,8 @@ LIB_PATH = os.path.join(_HERE, "lib", "librampack_amd.so")
 
 SHAPE_SPHERE = 0
 SHAPE_SPHEROCYLINDER = 1
+SHAPE_POLYSPHERE = 2
+MAX_CENTRES = 16
 SOFT_NONE = 0
 SOFT_WCA = 1
 SOFT_WCA_SEGMENT = 2
@@ -52,7 +54,9 @@ EXPORTED_SYMBOLS = (
 
 class ShapeDesc(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int), ("radius", ctypes.c_double), ("length", ctypes.c_double),
-                ("soft", ctypes.c_int), ("epsilon", ctypes.c_double), ("sigma", ctypes.c_double)]
+                ("soft", ctypes.c_int), ("epsilon", ctypes.c_double), ("sigma", ctypes.c_double),
+                ("n_centres", ctypes.c_int), ("centres", ctypes.POINTER(ctypes.c_double)),
+                ("radii", ctypes.POINTER(ctypes.c_double))]
 
 
 class MoveDesc(ctypes.Structure):
@@ -214,7 +218,8 @@ def _dptr(a: np.ndarray):
 
 @dataclass
 class Shape:
-    """ShapeTraits descriptor: 'sphere' (radius) or 'spherocylinder' (radius, length between cap centres).
+    """ShapeTraits descriptor: 'sphere' (radius), 'spherocylinder' (radius, length between cap centres) or
+    'polysphere' (hard beads: body-frame `centres`, n x 3, and `radii`, n; PolysphereTraits).
     soft: None (hard core), 'wca' (a sphere with RepulsiveLennardJonesInteraction(epsilon, sigma)) or 'wca_segment'
     (a spherocylinder with WCA on the distance between the axis segments)."""
     kind: str
@@ -223,23 +228,38 @@ class Shape:
     soft: Optional[str] = None
     epsilon: float = 1.0
     sigma: float = 1.0
+    centres: Optional[tuple] = None
+    radii: Optional[tuple] = None
 
     def desc(self) -> ShapeDesc:
-        kinds = {"sphere": SHAPE_SPHERE, "spherocylinder": SHAPE_SPHEROCYLINDER}
+        kinds = {"sphere": SHAPE_SPHERE, "spherocylinder": SHAPE_SPHEROCYLINDER, "polysphere": SHAPE_POLYSPHERE}
         softs = {None: SOFT_NONE, "wca": SOFT_WCA, "wca_segment": SOFT_WCA_SEGMENT}
         if self.kind not in kinds:
             raise ValueError(f"unknown shape {self.kind}")
         if self.soft not in softs:
             raise ValueError(f"unknown soft interaction {self.soft}")
-        return ShapeDesc(kinds[self.kind], float(self.radius), float(self.length), softs[self.soft],
-                         float(self.epsilon), float(self.sigma))
+        d = ShapeDesc(kinds[self.kind], float(self.radius), float(self.length), softs[self.soft],
+                      float(self.epsilon), float(self.sigma))
+        if self.kind == "polysphere":
+            c = np.ascontiguousarray(self.centres, dtype=np.float64).reshape(-1, 3)
+            r = np.ascontiguousarray(self.radii, dtype=np.float64).reshape(-1)
+            if len(c) != len(r):
+                raise ValueError("polysphere: one radius per centre")
+            d.n_centres = len(c)
+            d.centres = _dptr(c)
+            d.radii = _dptr(r)
+            d._keep = (c, r)  # the arrays must outlive the descriptor's use
+        return d
 
     @property
     def volume(self) -> float:
-        """SphereTraits::getVolume / SpherocylinderTraits::getVolume (SpherocylinderTraits.cpp:48-50)."""
+        """SphereTraits::getVolume / SpherocylinderTraits::getVolume (SpherocylinderTraits.cpp:48-50); a polysphere
+        of beads that do not overlap: the sum of the bead volumes (PolysphereTraits.cpp:124-131)."""
         r, l = self.radius, self.length
         if self.kind == "sphere":
             return 4.0 / 3 * np.pi * r ** 3
+        if self.kind == "polysphere":
+            return float(sum(4.0 / 3 * np.pi * x ** 3 for x in self.radii))
         return np.pi * r * r * l + 4.0 / 3 * np.pi * r ** 3
 
     @property
@@ -249,7 +269,17 @@ class Shape:
             return cut
         if self.soft == "wca_segment":
             return self.length + cut
+        if self.kind == "polysphere":
+            return 2 * max(self.radii)
         return 2 * self.radius + (self.length if self.kind == "spherocylinder" else 0.0)
+
+    @property
+    def total_range(self) -> float:
+        """Interaction::getTotalRangeRadius (Interaction.cpp:55-70)."""
+        if self.kind != "polysphere":
+            return self.range
+        c = np.asarray(self.centres, dtype=np.float64).reshape(-1, 3)
+        return 2 * float(np.sqrt((c * c).sum(axis=1).max())) + self.range
 
 
 def pair_decisions(shape: Shape, records: np.ndarray) -> np.ndarray:

@@ -118,6 +118,12 @@ struct rp_ctx {
     Grid grid{};
     bool sortedForCycle = false;
 
+    // polyspheres: bead table on the device, total range, bead items for overlap counts
+    double *dBeads = nullptr;
+    double totalRange = 0;
+    double beadVolume = 0;  // sum of bead volumes (NaN when beads overlap)
+    DeviceParticles beadA, beadB;
+
     // state saved by rp_try_scaling for rp_revert_scaling (Packing::lastBox / lastShapes / lastScalingNumOverlaps)
     DeviceParticles last;
     bool scalingSaved = false;
@@ -197,7 +203,73 @@ PairConsts makeConsts(const rp_shape_desc &s) {
 double shapeRange(const rp_shape_desc &s) {
     if (s.soft == RP_SOFT_WCA) return wcaCut(s);
     if (s.soft == RP_SOFT_WCA_SEGMENT) return s.length + wcaCut(s);
+    if (s.kind == RP_SHAPE_POLYSPHERE) {
+        // PolysphereTraits.cpp:116-121: twice the largest bead radius
+        double m = 0;
+        for (int i = 0; i < s.n_centres; i++) m = std::max(m, s.radii[i]);
+        return 2 * m;
+    }
     return s.kind == RP_SHAPE_SPHERE ? 2 * s.radius : 2 * s.radius + s.length;
+}
+
+// Interaction::getTotalRangeRadius (Interaction.cpp:55-70): 2 max |centre| + range for multi-centre shapes
+double shapeTotalRange(const rp_shape_desc &s) {
+    if (s.kind != RP_SHAPE_POLYSPHERE) return shapeRange(s);
+    double maxR2 = 0;
+    for (int i = 0; i < s.n_centres; i++) {
+        const double *q = s.centres + 3 * i;
+        double r2 = 0;
+        r2 += q[0] * q[0];
+        r2 += q[1] * q[1];
+        r2 += q[2] * q[2];
+        maxR2 = std::max(maxR2, r2);
+    }
+    return 2 * std::sqrt(maxR2) + shapeRange(s);
+}
+
+// Shape descriptor checks shared by rp_create and the pair entry points.
+int validateShape(const rp_shape_desc &s) {
+    if (s.kind == RP_SHAPE_POLYSPHERE) {
+        if (s.n_centres < 1 || s.n_centres > RP_MAX_CENTRES)
+            return fail(RP_ERR_PRECONDITION, "a polysphere needs 1 to " + std::to_string(RP_MAX_CENTRES) + " beads");
+        if (!s.centres || !s.radii) return fail(RP_ERR_PRECONDITION, "polysphere centres or radii missing");
+        for (int i = 0; i < s.n_centres; i++)  // SphereData constructor, PolysphereTraits.cpp:83-87
+            if (!(s.radii[i] > 0)) return fail(RP_ERR_PRECONDITION, "bead radius must be > 0");
+        if (s.soft != RP_SOFT_NONE) return fail(RP_ERR_UNSUPPORTED, "soft interaction not available for this shape");
+        return RP_OK;
+    }
+    if (s.kind != RP_SHAPE_SPHERE && s.kind != RP_SHAPE_SPHEROCYLINDER)
+        return fail(RP_ERR_UNSUPPORTED, "unsupported shape kind");
+    if (!(s.radius > 0)) return fail(RP_ERR_PRECONDITION, "radius must be > 0");
+    if (s.soft != RP_SOFT_NONE) {
+        if (!((s.soft == RP_SOFT_WCA && s.kind == RP_SHAPE_SPHERE) ||
+              (s.soft == RP_SOFT_WCA_SEGMENT && s.kind == RP_SHAPE_SPHEROCYLINDER)))
+            return fail(RP_ERR_UNSUPPORTED, "soft interaction not available for this shape");
+        // RepulsiveLennardJonesInteraction.cpp:21-22
+        if (!(s.epsilon > 0)) return fail(RP_ERR_PRECONDITION, "epsilon must be > 0");
+        if (!(s.sigma > 0)) return fail(RP_ERR_PRECONDITION, "sigma must be > 0");
+    }
+    if (s.kind == RP_SHAPE_SPHEROCYLINDER && !(s.length >= 0))
+        return fail(RP_ERR_PRECONDITION, "length must be >= 0");
+    return RP_OK;
+}
+
+// Device copy of a polysphere's beads (body-frame x, y, z, radius per bead) for k.beads; nullptr otherwise.
+int uploadBeads(const rp_shape_desc &s, double **dBeads) {
+    *dBeads = nullptr;
+    if (s.kind != RP_SHAPE_POLYSPHERE) return RP_OK;
+    std::vector<double> h(4 * (size_t)s.n_centres);
+    for (int i = 0; i < s.n_centres; i++) {
+        for (int d = 0; d < 3; d++) h[4 * i + d] = s.centres[3 * i + d];
+        h[4 * i + 3] = s.radii[i];
+    }
+    RP_HIP(hipMalloc(dBeads, sizeof(double) * h.size()));
+    if (hipMemcpy(*dBeads, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        hipFree(*dBeads);
+        *dBeads = nullptr;
+        return fail(RP_ERR_DEVICE, "bead upload failed");
+    }
+    return RP_OK;
 }
 
 void fillTranslations(Grid &g) {
@@ -381,14 +453,14 @@ int countCapLimit(rp_ctx *c) {
 }
 
 // Early-exit overlap scan of the `alt` arrays sorted by `g`; sets found.
-int scanOverlaps(rp_ctx *c, const Grid &g, int earlyExit, uint64_t &count) {
+int scanItems(rp_ctx *c, const Grid &g, const Particles &items, size_t nItems, int earlyExit, uint64_t &count) {
     RP_HIP(launch_fill_u64(c->stream, c->dCounters + CNT_OVERLAPS, 1, 0));
-    double lambda = (double)c->n / (double)g.ncell;
+    double lambda = (double)nItems / (double)g.ncell;
     const int side = 2 * g.reach + 1;
     double expect = ((side * side * side - 1) / 2 + 1) * lambda;
     int cap = (int)std::ceil(expect + 8 * std::sqrt(expect) + 64);
     cap = std::min((cap + 63) / 64 * 64, countCapLimit(c));
-    RP_HIP(launch_count_overlaps(c->stream, g, (int)c->n, c->alt.p, c->k, earlyExit, c->dCounters, cap,
+    RP_HIP(launch_count_overlaps(c->stream, g, (int)nItems, items, c->k, earlyExit, c->dCounters, cap,
                                  timingSlot(c, 2)));
     int r = readCounters(c);
     if (r) return r;
@@ -396,6 +468,10 @@ int scanOverlaps(rp_ctx *c, const Grid &g, int earlyExit, uint64_t &count) {
         return fail(RP_ERR_UNSUPPORTED, "a grid cell holds more particles than the overlap count can stage");
     count = c->hCounters[CNT_OVERLAPS];
     return RP_OK;
+}
+
+int scanOverlaps(rp_ctx *c, const Grid &g, int earlyExit, uint64_t &count) {
+    return scanItems(c, g, c->alt.p, c->n, earlyExit, count);
 }
 
 void swapBuffers(rp_ctx *c) { std::swap(c->cur, c->alt); }
@@ -733,7 +809,36 @@ int adaptSteps(rp_ctx *c) {
     return RP_OK;
 }
 
+// Packing::countTotalOverlaps for polyspheres (Packing.cpp:519-559 over the grid of interaction centres,
+// Packing.cpp:901-929): bead items from the current particles, sorted by the reference's grid of bead range.
+int countExactBeads(rp_ctx *c, int earlyExit, uint64_t &out) {
+    c->sortedForCycle = false;
+    const size_t items = c->n * (size_t)c->k.nc;
+    if (!c->beadA.mem) {
+        int r = allocParticles(c->beadA, items);
+        if (!r) r = allocParticles(c->beadB, items);
+        if (r) return r;
+        RP_HIP(hipMemsetAsync(c->beadA.mem, 0, c->beadA.bytes, c->stream));
+        RP_HIP(hipMemsetAsync(c->beadB.mem, 0, c->beadB.bytes, c->stream));
+    }
+    Grid g{};
+    int disabled = 0;
+    int r = makeExactGrid(c, c->box, g, disabled);
+    if (r) return r;
+    if (disabled)
+        return fail(RP_ERR_UNSUPPORTED, "the reference would count without a neighbour grid for this box; "
+                                        "the all-pairs path is not implemented on the device");
+    RP_HIP(launch_centres(c->stream, c->cur.p, (int)c->n, c->k, g, c->beadA.p));
+    RP_HIP(launch_rebuild(c->stream, g, (int)items, c->beadA.p, c->beadB.p, c->tmpSlot, c->tmpOrder, c->tmpPos, 0,
+                          nullptr, timingSlot(c, 1)));
+    r = scanItems(c, g, c->beadB.p, items, earlyExit, out);
+    if (r) return r;
+    if (earlyExit && out) out = 1;
+    return RP_OK;
+}
+
 int countExact(rp_ctx *c, int earlyExit, uint64_t &out) {
+    if (c->k.kind == RP_SHAPE_POLYSPHERE) return countExactBeads(c, earlyExit, out);
     c->sortedForCycle = false;  // the shared cell arrays are about to describe another grid
     Grid g{};
     int disabled = 0;
@@ -780,19 +885,7 @@ int rp_create(rp_ctx **out, const rp_shape_desc *shape, const double box[9], siz
     if (n == 0) return fail(RP_ERR_PRECONDITION, "empty packing (Packing.cpp:69)");
     if (n > (size_t)0x3fffffff) return fail(RP_ERR_PRECONDITION, "too many particles");
     if (precision != RP_PREC_FP64) return fail(RP_ERR_UNSUPPORTED, "unknown precision");
-    if (shape->kind != RP_SHAPE_SPHERE && shape->kind != RP_SHAPE_SPHEROCYLINDER)
-        return fail(RP_ERR_UNSUPPORTED, "unsupported shape kind");
-    if (!(shape->radius > 0)) return fail(RP_ERR_PRECONDITION, "radius must be > 0");
-    if (shape->soft != RP_SOFT_NONE) {
-        if (!((shape->soft == RP_SOFT_WCA && shape->kind == RP_SHAPE_SPHERE) ||
-              (shape->soft == RP_SOFT_WCA_SEGMENT && shape->kind == RP_SHAPE_SPHEROCYLINDER)))
-            return fail(RP_ERR_UNSUPPORTED, "soft interaction not available for this shape");
-        // RepulsiveLennardJonesInteraction.cpp:21-22
-        if (!(shape->epsilon > 0)) return fail(RP_ERR_PRECONDITION, "epsilon must be > 0");
-        if (!(shape->sigma > 0)) return fail(RP_ERR_PRECONDITION, "sigma must be > 0");
-    }
-    if (shape->kind == RP_SHAPE_SPHEROCYLINDER && !(shape->length >= 0))
-        return fail(RP_ERR_PRECONDITION, "length must be >= 0");
+    if (int v = validateShape(*shape)) return v;
     HostBox hb;
     hb.set(box);
     if (hb.volume() == 0) return fail(RP_ERR_PRECONDITION, "box volume is 0 (Packing.cpp:68)");
@@ -831,8 +924,11 @@ int rp_create(rp_ctx **out, const rp_shape_desc *shape, const double box[9], siz
         }
     }
     c->shape = *shape;
+    c->shape.centres = nullptr;  // the caller's arrays are not kept; the device copy is k.beads
+    c->shape.radii = nullptr;
     c->k = makeConsts(*shape);
     c->range = shapeRange(*shape);
+    c->totalRange = shapeTotalRange(*shape);
     c->n = n;
     c->box = hb;
     c->seed = seed;
@@ -842,13 +938,37 @@ int rp_create(rp_ctx **out, const rp_shape_desc *shape, const double box[9], siz
         return code;
     };
     if (hipStreamCreate(&c->stream) != hipSuccess) return cleanup(fail(RP_ERR_DEVICE, "hipStreamCreate failed"));
-    int r = allocParticles(c->cur, n);
+    int r = uploadBeads(*shape, &c->dBeads);
+    if (r) return cleanup(r);
+    c->k.beads = c->dBeads;
+    c->k.nc = shape->kind == RP_SHAPE_POLYSPHERE ? shape->n_centres : 0;
+    if (shape->kind == RP_SHAPE_POLYSPHERE) {
+        // PolysphereGeometry::calculateVolume (PolysphereTraits.cpp:138-146) needs beads that do not overlap by
+        // spheresOverlap's test (PolysphereTraits.cpp:211-224): distance^2 (1 + 1e-12) < (r_i + r_j)^2
+        double v = 0;
+        bool overlapping = false;
+        for (int i = 0; i < shape->n_centres; i++) {
+            const double r = shape->radii[i];
+            v = v + 4 * M_PI / 3 * r * r * r;
+            for (int j = i + 1; j < shape->n_centres; j++) {
+                double d2 = 0;
+                for (int q = 0; q < 3; q++) {
+                    double d = shape->centres[3 * i + q] - shape->centres[3 * j + q];
+                    d2 += d * d;
+                }
+                overlapping = overlapping || d2 * (1e-12 + 1) < std::pow(r + shape->radii[j], 2);
+            }
+        }
+        c->beadVolume = overlapping ? std::nan("") : v;
+    }
+    const size_t items = n * (size_t)std::max(1, c->k.nc);  // rebuild scratch: particles or bead items
+    r = allocParticles(c->cur, n);
     if (r) return cleanup(r);
     r = allocParticles(c->alt, n);
     if (r) return cleanup(r);
-    if (hipMalloc(&c->tmpSlot, sizeof(int) * 2 * n) != hipSuccess ||
-        hipMalloc(&c->tmpOrder, sizeof(int) * n + n + 256) != hipSuccess ||
-        hipMalloc(&c->tmpPos, sizeof(double) * 3 * n) != hipSuccess ||
+    if (hipMalloc(&c->tmpSlot, sizeof(int) * 2 * items) != hipSuccess ||
+        hipMalloc(&c->tmpOrder, sizeof(int) * items + items + 256) != hipSuccess ||
+        hipMalloc(&c->tmpPos, sizeof(double) * 3 * items) != hipSuccess ||
         hipMalloc(&c->dCounters, sizeof(unsigned long long) * CNT_NUM) != hipSuccess ||
         hipHostMalloc(&c->hCounters, sizeof(unsigned long long) * CNT_NUM, hipHostMallocDefault) != hipSuccess)
         return cleanup(fail(RP_ERR_DEVICE, "device allocation failed"));
@@ -908,6 +1028,7 @@ int rp_set_environment(rp_ctx *c, double temperature, double pressure, const rp_
 
 int rp_run_cycles(rp_ctx *c, size_t nCycles, int adapt, rp_counters *out) {
     if (!c) return fail(RP_ERR_PRECONDITION, "null context");
+    if (c->k.kind == RP_SHAPE_POLYSPHERE) return fail(RP_ERR_UNSUPPORTED, "the polysphere sweep is not built");
     if (!c->envSet) return fail(RP_ERR_PRECONDITION, "environment not set");
     uint64_t overl = 0;
     int r = RP_OK;
@@ -1015,9 +1136,11 @@ int rp_get_observables(rp_ctx *c, rp_observables *out) {
     // Packing::getPackingFraction / getNumberDensity (Packing.cpp:225-232)
     out->volume = c->box.volume();
     out->number_density = (double)c->n / out->volume;
-    out->packing_fraction = out->number_density * shapeVolume(c->shape);
-    if (c->shape.kind == RP_SHAPE_SPHERE) {
-        // a sphere has no primary axis (ShapeGeometry::getPrimaryAxis is not provided by SphereTraits)
+    out->packing_fraction =
+        out->number_density * (c->shape.kind == RP_SHAPE_POLYSPHERE ? c->beadVolume : shapeVolume(c->shape));
+    if (c->shape.kind != RP_SHAPE_SPHEROCYLINDER) {
+        // a sphere has no primary axis (ShapeGeometry::getPrimaryAxis is not provided by SphereTraits), and a
+        // polysphere given by its beads alone has none either (the axis of PolysphereGeometry is optional)
         out->nematic_p2 = std::nan("");
         for (double &q : out->q_tensor) q = std::nan("");
         return RP_OK;
@@ -1127,6 +1250,9 @@ void rp_destroy(rp_ctx *c) {
     if (c->cur.mem) hipFree(c->cur.mem);
     if (c->alt.mem) hipFree(c->alt.mem);
     if (c->last.mem) hipFree(c->last.mem);
+    if (c->beadA.mem) hipFree(c->beadA.mem);
+    if (c->beadB.mem) hipFree(c->beadB.mem);
+    if (c->dBeads) hipFree(c->dBeads);
     if (c->cellStart) hipFree(c->cellStart);
     if (c->cellCount) hipFree(c->cellCount);
     if (c->propMem) hipFree(c->propMem);
@@ -1200,6 +1326,7 @@ int fixOrientations(rp_ctx *c, uint64_t &rejected) {
 
 int rp_fix_orientations(rp_ctx *c, uint64_t *rejected) {
     if (!c) return fail(RP_ERR_PRECONDITION, "null context");
+    if (c->k.kind == RP_SHAPE_POLYSPHERE) return fail(RP_ERR_UNSUPPORTED, "the polysphere sweep is not built");
     uint64_t rej = 0;
     int r = fixOrientations(c, rej);
     if (r) return r;
@@ -1267,6 +1394,7 @@ int rp_set_orientation_fix_every(rp_ctx *c, uint64_t every) {
 // rp_revert_scaling; the scaled state becomes current whatever the result, as in the reference.
 int rp_try_scaling(rp_ctx *c, const double new_box[9], double *dE) {
     if (!c || !new_box || !dE) return fail(RP_ERR_PRECONDITION, "null argument");
+    if (c->k.kind == RP_SHAPE_POLYSPHERE) return fail(RP_ERR_UNSUPPORTED, "polysphere box moves are not built");
     HostBox nb;
     nb.set(new_box);
     if (nb.volume() == 0) return fail(RP_ERR_PRECONDITION, "tryScaling: the new box has zero volume");
@@ -1384,11 +1512,20 @@ int rp_pair_energies(const rp_shape_desc *shape, size_t n, const double *records
 
 int rp_pair_decisions(const rp_shape_desc *shape, size_t n, const double *records, uint8_t *decisions) {
     if (!shape || (n && (!records || !decisions))) return fail(RP_ERR_PRECONDITION, "null argument");
-    if (shape->kind != RP_SHAPE_SPHERE && shape->kind != RP_SHAPE_SPHEROCYLINDER)
-        return fail(RP_ERR_UNSUPPORTED, "unsupported shape kind");
+    if (int v = validateShape(*shape)) return v;
     if (shape->soft != RP_SOFT_NONE) return fail(RP_ERR_UNSUPPORTED, "a soft interaction has no hard part");
     if (n == 0) return RP_OK;
     PairConsts k = makeConsts(*shape);
+    double *dBeads = nullptr;
+    if (int r = uploadBeads(*shape, &dBeads)) return r;
+    k.beads = dBeads;
+    k.nc = shape->kind == RP_SHAPE_POLYSPHERE ? shape->n_centres : 0;
+    struct FreeBeads {
+        double *p;
+        ~FreeBeads() {
+            if (p) hipFree(p);
+        }
+    } freeBeads{dBeads};
     double *dRec = nullptr;
     uint8_t *dOut = nullptr;
     RP_HIP(hipMalloc(&dRec, sizeof(double) * 27 * n));

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include "rampack_amd.h"  // RP_MAX_CENTRES
 #include "rp_math.h"
 
 namespace rp {
@@ -108,6 +109,10 @@ hipError_t launch_pair_decisions_guarded(hipStream_t s, PairConsts k, size_t n, 
 struct Timed {
     hipEvent_t start = nullptr, stop = nullptr;
 };
+// Absolute bead centres (Packing.cpp:1036-1047) of the n particles of src into items of dst (n * k.nc items: x, y,
+// z and tag = item index); g supplies the box (dims, inv) for the periodic correction.
+hipError_t launch_centres(hipStream_t s, const Particles &src, int n, const PairConsts &k, const Grid &g,
+                          const Particles &dst);
 hipError_t launch_fill_u64(hipStream_t s, unsigned long long *p, size_t n, unsigned long long v);
 hipError_t launch_fill_i32(hipStream_t s, int *p, size_t n, int v);
 // Cell-list rebuild: src -> dst sorted by cell of `g`.  When rescale != 0 the positions are first mapped

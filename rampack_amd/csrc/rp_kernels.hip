@@ -81,7 +81,7 @@ __device__ __forceinline__ bool pair_overlap(const PairConsts &k, const SCFirst 
 
 template <int KIND>
 __device__ __forceinline__ SCFirst make_first(const PairConsts &k, const V3 &pos, const V3 &axis) {
-    if (KIND == 0) {
+    if (KIND != 1) {
         SCFirst f;
         f.pos = pos;
         return f;
@@ -104,14 +104,51 @@ __global__ __launch_bounds__(256) void k_pair_decisions(PairConsts k, size_t n, 
     V3 p2 = v3(q[12], q[13], q[14]);
     V3 a2 = v3(q[15 + 2], q[15 + 5], q[15 + 8]);
     V3 T = v3(q[24], q[25], q[26]);
-    bool o;
+    bool o = false;
     if (k.kind == 0) {
         o = sphere_overlap(k, p1, p2, T);
-    } else {
+    } else if (k.kind == 1) {
         SCFirst f = sc_first(k, p1, a1);
         o = sc_overlap(k, f, p2, a2, T);
+    } else {
+        // bead centres pos + O * c (PolysphereTraits::SphereData::centreForShape, PolysphereTraits.cpp:93-95);
+        // bead pairs as PolysphereTraits::HardInteraction::overlapBetween (PolysphereTraits.cpp:97-106)
+        for (int b = 0; b < k.nc && !o; b++) {
+            const double *cb = k.beads + 4 * b;
+            const V3 c2 = add(p2, mulv(q + 15, v3(cb[0], cb[1], cb[2])));
+            for (int a = 0; a < k.nc && !o; a++) {
+                const double *ca = k.beads + 4 * a;
+                const V3 c1 = add(p1, mulv(q + 3, v3(ca[0], ca[1], ca[2])));
+                o = bead_overlap(ca[3], cb[3], c1, c2, T);
+            }
+        }
     }
     out[i] = o ? 1 : 0;
+}
+
+// Absolute bead centres of every particle (Packing::recalculateAbsoluteInteractionCentres, Packing.cpp:1036-1047):
+// item i * nc + c is position + orientation * centre_c, wrapped by the periodic correction of that sum; its tag is
+// the item index (the reference's centre index, the order in which its grid lists centres).
+__global__ __launch_bounds__(256) void k_centres(Particles src, int n, PairConsts k, Grid g, Particles dst) {
+    const int item = blockIdx.x * blockDim.x + threadIdx.x;
+    if (item >= n * k.nc) return;
+    const int i = item / k.nc, c = item - i * k.nc;
+    double O[9];
+    for (int e = 0; e < 9; e++) O[e] = src.o[e][i];
+    const double *cb = k.beads + 4 * c;
+    const V3 pos = translate_wrapped(g.dims, g.inv, load3(src.x, src.y, src.z, i), mulv(O, v3(cb[0], cb[1], cb[2])));
+    dst.x[item] = pos.x;
+    dst.y[item] = pos.y;
+    dst.z[item] = pos.z;
+    dst.tag[item] = item;
+}
+
+hipError_t launch_centres(hipStream_t s, const Particles &src, int n, const PairConsts &k, const Grid &g,
+                          const Particles &dst) {
+    const size_t items = (size_t)n * k.nc;
+    if (items == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_centres, dim3((unsigned)((items + 255) / 256)), dim3(256), 0, s, src, n, k, g, dst);
+    return hipGetLastError();
 }
 
 hipError_t launch_pair_decisions(hipStream_t s, PairConsts k, size_t n, const double *rec, uint8_t *out) {
@@ -500,6 +537,7 @@ __device__ __forceinline__ void stage_stencil(const Particles &p, const Stencil 
             S.ay[j] = p.o[5][gi];
             S.az[j] = p.o[8][gi];
         }
+        if (KIND == 2) S.ax[j] = (double)p.tag[gi];  // bead item index: particle * nc + bead
         S.code[j] = image_code(S.ecode[q], p.wbits[gi]);
     }
 }
@@ -511,7 +549,7 @@ __device__ __forceinline__ void carve_stencil(Stencil &S, double *smem, int cap,
     S.ax = S.z + cap;
     S.ay = S.ax + cap;
     S.az = S.ay + cap;
-    double *after = kind == 0 ? S.ax : S.az + cap;
+    double *after = kind == 0 ? S.ax : S.az + cap;  // kind 2 keeps the bead item index in ax
     S.code = reinterpret_cast<int *>(after);
     rest = after + (cap + 1) / 2;
 }
@@ -567,10 +605,12 @@ __device__ void count_cell(const Grid &g, const Particles &p, const PairConsts &
 constexpr int COUNT_OWN_CAP = 128;             // own particles whose first-operand data fits the table
 constexpr int COUNT_CHUNK = COUNT_THREADS * 8;  // pairs per chunk (and queue entries)
 
+// KIND 2 (bead items of polyspheres): pairs of beads of one particle are skipped (Packing.cpp:482-483, 500-501)
+// and the contact distance is the sum of the two bead radii (sBeadR, indexed by bead).
 template <int KIND>
 __device__ void count_cell_flat(const PairConsts &k, const Stencil &S, const double *sT, const int *sTag, int nOwn,
                                 int total, int earlyExit, unsigned long long *counters, int *sFound, double *fTab,
-                                int *queue, int *sQueued) {
+                                int *queue, int *sQueued, const double *sBeadR) {
     const int tid = threadIdx.x;
     const int lane = tid & (WAVE - 1);
     volatile unsigned long long *flag = counters + CNT_OVERLAPS;
@@ -609,10 +649,19 @@ __device__ void count_cell_flat(const PairConsts &k, const Stencil &S, const dou
                     const int wx = (c1 & 3) - 1, wy = ((c1 >> 2) & 3) - 1, wz = ((c1 >> 4) & 3) - 1;
                     const V3 T = b < nOwn && code == c1 ? v3(0, 0, 0) : image_translation(sT, code, wx, wy, wz);
                     const double *q = fTab + 10 * a;
-                    const V3 d = sub(add(v3(S.x[b], S.y[b], S.z[b]), T), v3(q[0], q[1], q[2]));
-                    const double d2 = dot(d, d);
-                    if (d2 < k.contact2) found++;
-                    else if (KIND == 1 && d2 < k.range2) amb = true;
+                    if (KIND == 2) {
+                        const int ta = (int)S.ax[a], tb = (int)S.ax[b];
+                        const int pa = ta / k.nc, pb = tb / k.nc;
+                        if (pa != pb &&
+                            bead_overlap(sBeadR[ta - pa * k.nc], sBeadR[tb - pb * k.nc], v3(q[0], q[1], q[2]),
+                                         v3(S.x[b], S.y[b], S.z[b]), T))
+                            found++;
+                    } else {
+                        const V3 d = sub(add(v3(S.x[b], S.y[b], S.z[b]), T), v3(q[0], q[1], q[2]));
+                        const double d2 = dot(d, d);
+                        if (d2 < k.contact2) found++;
+                        else if (KIND == 1 && d2 < k.range2) amb = true;
+                    }
                 }
             }
             if (KIND == 1) {
@@ -680,13 +729,20 @@ __global__ __launch_bounds__(COUNT_THREADS) void k_count_overlaps(Grid g, Partic
     carve_stencil(S, smem, cap, KIND, rest);
     int *sTag = reinterpret_cast<int *>(rest);  // own-cell tags, up to cap entries
     for (int a = tid; a < nOwn && a < cap; a += COUNT_THREADS) sTag[a] = p.tag[own0 + a];
+    __shared__ double sBeadR[RP_MAX_CENTRES];
+    if (KIND == 2)
+        for (int c = tid; c < k.nc; c += COUNT_THREADS) sBeadR[c] = k.beads[4 * c + 3];
     if (total <= cap && nOwn <= COUNT_OWN_CAP && total <= 0xffff) {
         __shared__ int sQueued;
         stage_stencil<KIND>(p, S, total, tid, COUNT_THREADS);
         double *fTab = reinterpret_cast<double *>(sTag + ((cap + 1) & ~1));  // 10 doubles per own particle
         int *queue = reinterpret_cast<int *>(fTab + 10 * COUNT_OWN_CAP);        // COUNT_CHUNK entries
         __syncthreads();
-        count_cell_flat<KIND>(k, S, sT, sTag, nOwn, total, earlyExit, counters, &sFound, fTab, queue, &sQueued);
+        count_cell_flat<KIND>(k, S, sT, sTag, nOwn, total, earlyExit, counters, &sFound, fTab, queue, &sQueued,
+                              sBeadR);
+    } else if (KIND == 2) {
+        // bead grids hold a few beads per cell; a cell beyond the flat path's limits is reported as an error
+        if (tid == 0) atomicAdd(&counters[CNT_UNSTAGED_BLOCKS], 1ull << 32);
     } else if (total <= cap) {
         stage_stencil<KIND>(p, S, total, tid, COUNT_THREADS);
         __syncthreads();
@@ -712,8 +768,11 @@ hipError_t launch_count_overlaps(hipStream_t s, const Grid &g, int n, Particles 
     if (k.kind == 0)
         hipExtLaunchKernelGGL(k_count_overlaps<0>, dim3((unsigned)g.ncell), dim3(COUNT_THREADS), (uint32_t)lds, s,
                               t.start, t.stop, 0, g, p, k, earlyExit, counters, cap);
-    else
+    else if (k.kind == 1)
         hipExtLaunchKernelGGL(k_count_overlaps<1>, dim3((unsigned)g.ncell), dim3(COUNT_THREADS), (uint32_t)lds, s,
+                              t.start, t.stop, 0, g, p, k, earlyExit, counters, cap);
+    else
+        hipExtLaunchKernelGGL(k_count_overlaps<2>, dim3((unsigned)g.ncell), dim3(COUNT_THREADS), (uint32_t)lds, s,
                               t.start, t.stop, 0, g, p, k, earlyExit, counters, cap);
     return hipGetLastError();
 }

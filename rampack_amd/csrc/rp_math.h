@@ -112,6 +112,14 @@ RP_HD V3 translate_wrapped(const double *dims, const double *inv, const V3 &pos,
     return add(p, mulv(dims, v3(c[0], c[1], c[2])));
 }
 
+// PolysphereTraits::HardInteraction::overlapBetween (PolysphereTraits.cpp:97-106) with HardcodedTranslation
+// (Packing.cpp:36-42): r = r1 + r2, then (c2 + T - c1)^2 < r * r.
+RP_HD bool bead_overlap(double r1, double r2, const V3 &c1, const V3 &c2, const V3 &T) {
+    const double r = r1 + r2;
+    const V3 d = sub(add(c2, T), c1);
+    return dot(d, d) < r * r;
+}
+
 // SegmentDistanceCalculator.h:22-85, with u = s12 - s11 and a = u.u passed in (both depend only on the first
 // segment, so the sweep computes them once per trial move).
 RP_HD double segment_distance2(const V3 &s11, const V3 &u, double a, const V3 &s21, const V3 &s22) {
@@ -150,13 +158,15 @@ RP_HD double segment_distance2(const V3 &s11, const V3 &u, double a, const V3 &s
 // Constants of one pair-overlap test, computed on the host exactly as the reference computes them
 // (SphereTraits.cpp:64, SpherocylinderTraits.cpp:34-43).
 struct PairConsts {
-    int kind;            // 0 sphere, 1 spherocylinder
+    int kind;            // 0 sphere, 1 spherocylinder, 2 polysphere
     int soft;            // 0 hard core; 1 WCA on the centre distance; 2 WCA on the segment distance
     double contact2;     // sphere: pow(2r, 2); spherocylinder: 4*r*r
     double range2;       // spherocylinder: pow(2r + L, 2)
     double halfLen;      // spherocylinder: 0.5*L (cap offset, SpherocylinderTraits.cpp:24-26)
     double eps, sigma2, cut2;  // WCA: epsilon, sigma*sigma, (sigma*pow(2, 1/6))^2 (RepulsiveLennardJonesInteraction)
     double softRange2;   // centre distance^2 at and beyond which the soft energy is 0
+    int nc;              // polysphere: beads per particle
+    const double *beads; // polysphere: device array, per bead body-frame x, y, z and radius
 };
 
 // Spherocylinder first-particle data that does not depend on the candidate.

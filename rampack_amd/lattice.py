@@ -89,3 +89,23 @@ def orthorhombic_arrange(n: int, dims, particles_in_line=None, cell=None):
                 pos.append([i * cell[0] + offset[0], j * cell[1] + offset[1], k * cell[2] + offset[2]])
     pos = np.asarray(pos[:n], dtype=float)
     return np.diag(dims), pos, np.tile(np.eye(3).reshape(1, 9), (len(pos), 1))
+
+
+def kinked_chain(n_beads: int = 11, diameter: float = 1.0, kink_deg: float = 30.0, kink_at: int = None):
+    """Body-frame bead centres and radii of a chain of touching spheres with one kink: beads 0..kink_at lie on the
+    z axis at unit spacing (times the diameter), the rest continue in the x-z plane deflected by kink_deg from the z
+    axis.  The centres are shifted so that their mean is the origin.  (The deflection reading of a "30 degree kink";
+    the interior angle between the two arms is 180 - kink_deg.)"""
+    if kink_at is None:
+        kink_at = n_beads // 2
+    pts = []
+    for i in range(n_beads):
+        if i <= kink_at:
+            pts.append((0.0, 0.0, i * diameter))
+        else:
+            t = np.radians(kink_deg)
+            k = (i - kink_at) * diameter
+            pts.append((k * np.sin(t), 0.0, kink_at * diameter + k * np.cos(t)))
+    c = np.array(pts)
+    c -= c.mean(axis=0)
+    return c, np.full(n_beads, diameter / 2)

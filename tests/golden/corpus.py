@@ -201,3 +201,15 @@ def jittered_lattice(n_side: int, spacing: float, jitter: float, seed: int):
     box = np.diag([n_side * spacing] * 3)
     rot = np.tile(np.eye(3).reshape(1, 9), (len(idx), 1))
     return box, np.ascontiguousarray(pos), np.ascontiguousarray(rot)
+
+
+def bead_set(name: str):
+    """Body-frame centres and radii of the polysphere fixtures: 'kinked11' (11 touching beads of diameter 1 with a
+    30 degree kink), 'dimer' (two touching beads of radius 0.5 on the z axis), 'mixed' (three beads of radii 0.5,
+    0.4, 0.3 that do not overlap)."""
+    if name == "kinked11":
+        from rampack_amd import lattice
+        return lattice.kinked_chain(11, 1.0, 30.0)
+    if name == "dimer":
+        return np.array([[0.0, 0.0, -0.5], [0.0, 0.0, 0.5]]), np.array([0.5, 0.5])
+    return np.array([[0.0, 0.0, 0.0], [0.0, 0.0, 0.9], [0.5, 0.3, -0.7]]), np.array([0.5, 0.4, 0.3])

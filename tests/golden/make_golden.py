@@ -1,7 +1,7 @@
 """Writes tests/golden/golden.json from the reference's own hot path (oracle/_ref/libref.so, compiled from the
 reference sources by oracle/Makefile).  Run in the container that holds the reference tree:
 
-    make -C oracle ref oracle && python tests/golden/make_golden.py [--large-only]
+    make -C oracle ref oracle && python tests/golden/make_golden.py [--large-only] [--poly-only]
 
 The fixture stores only data: generator parameters, SHA-256 of the generated inputs, the reference's decision
 bitmaps and the reference's overlap counts / energies for deterministic snapshots.
@@ -37,6 +37,30 @@ CORPORA = [
 LARGE_CORPORA = [
     ("sc_contact_1e7", 1, 0.5, 5.0, 10000000, 20261202, 1e-6, 250000),
 ]
+
+# name, bead set, box, n, seed: random polysphere snapshots (positions uniform in the box, random orientations);
+# the reference counts overlapping bead pairs of different particles on its grid of interaction centres
+POLY_SNAPSHOTS = [
+    ("kinked11_cubic", "kinked11", np.diag([30.0, 30.0, 30.0]), 600, 31),
+    ("kinked11_triclinic", "kinked11", np.array([[26.0, 2.0, -1.5], [0.0, 25.0, 1.25], [0.0, 0.0, 27.0]]), 500, 32),
+    ("dimer_triclinic", "dimer", np.array([[20.0, 1.5, -0.5], [0.0, 19.0, 0.75], [0.0, 0.0, 21.0]]), 3000, 33),
+    ("mixed_radii_cubic", "mixed", np.diag([18.0, 18.0, 18.0]), 1500, 34),
+]
+
+
+def poly_snapshots(ref):
+    out = []
+    for name, beads, box, n, seed in POLY_SNAPSHOTS:
+        centres, radii = corpus.bead_set(beads)
+        pos, rot = corpus.random_snapshot(n, seed, box)
+        full, _ = ref.snapshot_stats(2, 0.0, 0.0, box, pos, rot, centres=centres, radii=radii)
+        early, _ = ref.snapshot_stats(2, 0.0, 0.0, box, pos, rot, early_exit=True, centres=centres, radii=radii)
+        out.append(dict(name=name, beads=beads, box=box.tolist(), n=n, seed=seed,
+                        sha256=corpus.records_sha256(np.concatenate([pos.ravel(), rot.ravel()])),
+                        overlaps=full, overlaps_early_exit=early))
+        print(name, full, early)
+    return out
+
 
 TRICLINIC = np.array([[30.0, 1.5, -0.5], [0.0, 29.0, 0.75], [0.0, 0.0, 31.0]])
 CUBIC = np.diag([30.0, 30.0, 30.0])
@@ -112,10 +136,13 @@ def large_corpora(ref):
 def main():
     ref = oracle_lib.ref()
     path = os.path.join(HERE, "golden.json")
-    if "--large-only" in sys.argv:
+    if "--large-only" in sys.argv or "--poly-only" in sys.argv:
         with open(path) as f:
             out = json.load(f)
-        out["large_corpora"] = large_corpora(ref)
+        if "--large-only" in sys.argv:
+            out["large_corpora"] = large_corpora(ref)
+        if "--poly-only" in sys.argv:
+            out["poly_snapshots"] = poly_snapshots(ref)
         with open(path, "w") as f:
             json.dump(out, f, indent=1)
         return
@@ -153,6 +180,7 @@ def main():
                                           energy=energy))
         print(name, energy)
     out["large_corpora"] = large_corpora(ref)
+    out["poly_snapshots"] = poly_snapshots(ref)
     out["statistics"] = [reference_statistics(ref)]
     print(out["statistics"])
     with open(path, "w") as f:

@@ -42,6 +42,7 @@ def _setup_common(lib):
     lib.orc_is_box_upscaled.argtypes = [_dp, _dp]
     lib.orc_translate_wrapped.argtypes = [_dp, ctypes.c_size_t, _dp, _dp, _dp]
     lib.orc_translate_wrapped.restype = None
+    lib.orc_poly_pair_decisions.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_size_t, _dp, _u8p]
     lib.orc_rescale.argtypes = [_dp, _dp, ctypes.c_size_t, _dp, _dp]
     lib.orc_rescale.restype = None
     lib.orc_rotation.argtypes = [_dp, ctypes.c_double, _dp]
@@ -218,3 +219,14 @@ def pair_guarded(kind: int, radius: float, length: float, rec, bound_scale: floa
     oracle().lib.orc_pair_guarded(kind, radius, length, len(rec), rec.ctypes.data_as(_dp), dec.ctypes.data_as(_u8p),
                                   codes.ctypes.data_as(_u8p), bound_scale)
     return dec, codes
+
+
+def poly_pair_decisions(centres, radii, rec) -> np.ndarray:
+    """The oracle's polysphere pair decisions (records of 27 doubles as for the other shapes)."""
+    c = np.ascontiguousarray(centres, dtype=np.float64).reshape(-1, 3)
+    r = np.ascontiguousarray(radii, dtype=np.float64).reshape(-1)
+    rec = np.ascontiguousarray(rec, dtype=np.float64).reshape(-1, 27)
+    out = np.zeros(len(rec), dtype=np.uint8)
+    oracle().lib.orc_poly_pair_decisions(len(c), c.ctypes.data_as(_dp), r.ctypes.data_as(_dp), len(rec),
+                                         rec.ctypes.data_as(_dp), out.ctypes.data_as(_u8p))
+    return out

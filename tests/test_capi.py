@@ -43,9 +43,27 @@ def test_last_error_is_a_string():
     assert isinstance(lib.rp_last_error(), bytes)
 
 
-def test_structure_layouts_match_header():
-    assert ctypes.sizeof(ra.ShapeDesc) == 48
-    assert ctypes.sizeof(ra.MoveDesc) == 24
-    assert ctypes.sizeof(ra.BoxDesc) == 16
-    assert ctypes.sizeof(ra.Counters) == 10 * 8 + 3 * 8
-    assert ctypes.sizeof(ra.Timing) == 12 * 8
+def test_structure_layouts_match_header(tmp_path):
+    """Every ctypes mirror has the size and field offsets the C compiler gives the header's struct."""
+    mirrors = {"rp_shape_desc": ra.ShapeDesc, "rp_move_desc": ra.MoveDesc, "rp_box_desc": ra.BoxDesc,
+               "rp_counters": ra.Counters, "rp_timing": ra.Timing, "rp_guard_stats": ra.GuardStats}
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "rampack_amd.h"', "int main(void) {"]
+    for cname, py in mirrors.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for field, _ in py._fields_:
+            lines.append(f'printf("{cname} {field} %zu\\n", offsetof({cname}, {field}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    import subprocess
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    seen = 0
+    for line in filter(None, out):
+        cname, field, value = line.split()
+        py = mirrors[cname]
+        got = ctypes.sizeof(py) if field == "size" else getattr(py, field).offset
+        assert got == int(value), (cname, field)
+        seen += 1
+    assert seen == sum(len(py._fields_) + 1 for py in mirrors.values())
