@@ -122,6 +122,8 @@ struct rp_ctx {
     double *dBeads = nullptr;
     double totalRange = 0;
     double beadVolume = 0;  // sum of bead volumes (NaN when beads overlap)
+    int *beadCellStart = nullptr, *beadCellCount = nullptr;  // cell arrays of bead grids (kept apart from the
+    int beadNcellCap = 0;                                     // sweep grid's, which box moves must not disturb)
     DeviceParticles beadA, beadB;
 
     // state saved by rp_try_scaling for rp_revert_scaling (Packing::lastBox / lastShapes / lastScalingNumOverlaps)
@@ -284,16 +286,20 @@ void fillTranslations(Grid &g) {
             }
 }
 
-int ensureCells(rp_ctx *c, int ncell) {
-    if (ncell <= c->ncellCap) return RP_OK;
-    if (c->cellStart) hipFree(c->cellStart);
-    if (c->cellCount) hipFree(c->cellCount);
-    c->cellStart = c->cellCount = nullptr;
+// Cell arrays of at least ncell cells: the shared ones, or (beads) those of the bead grids.
+int ensureCells(rp_ctx *c, int ncell, bool beads = false) {
+    int *&start = beads ? c->beadCellStart : c->cellStart;
+    int *&count = beads ? c->beadCellCount : c->cellCount;
+    int &have = beads ? c->beadNcellCap : c->ncellCap;
+    if (ncell <= have) return RP_OK;
+    if (start) hipFree(start);
+    if (count) hipFree(count);
+    start = count = nullptr;
     int cap = ncell + ncell / 4 + 64;
-    RP_HIP(hipMalloc(&c->cellStart, sizeof(int) * (cap + 1)));
-    RP_HIP(hipMalloc(&c->cellCount, sizeof(int) * cap));
-    RP_HIP(launch_fill_i32(c->stream, c->cellCount, (size_t)cap, 0));  // rebuilds keep them zero from here on
-    c->ncellCap = cap;
+    RP_HIP(hipMalloc(&start, sizeof(int) * (cap + 1)));
+    RP_HIP(hipMalloc(&count, sizeof(int) * cap));
+    RP_HIP(launch_fill_i32(c->stream, count, (size_t)cap, 0));  // rebuilds keep them zero from here on
+    have = cap;
     return RP_OK;
 }
 
@@ -323,7 +329,9 @@ int makeSweepGrid(rp_ctx *c, const HostBox &b, const double shiftUnit[3], Grid &
     std::memcpy(g.dims, b.dims, sizeof g.dims);
     std::memcpy(g.inv, b.inv, sizeof g.inv);
     const int reach = period - 1;
-    double cellSize = std::max(c->range / reach, std::cbrt(b.volume() / (double)c->n) / 5);
+    // a particle-level grid: a polysphere reaches other particles within its total range
+    const double gridRange = c->k.kind == RP_SHAPE_POLYSPHERE ? c->totalRange : c->range;
+    double cellSize = std::max(gridRange / reach, std::cbrt(b.volume() / (double)c->n) / 5);
     for (int d = 0; d < 3; d++) {
         int n = period * (int)std::floor(h[d] / (period * cellSize));
         if (period == 2 && n < 2 && h[d] >= cellSize) {
@@ -364,7 +372,7 @@ int chooseSweepGrid(rp_ctx *c, const HostBox &b, const double shiftUnit[3], Grid
 
 // The reference's neighbour grid (Packing.cpp:858-883, NeighbourGrid.cpp:178-201).  Returns 1 when the
 // reference would run without a grid.
-int makeExactGrid(rp_ctx *c, const HostBox &b, Grid &g, int &disabled) {
+int makeExactGrid(rp_ctx *c, const HostBox &b, Grid &g, int &disabled, bool beads = false) {
     double h[3];
     b.heights(h);
     double cell = c->range;
@@ -385,10 +393,10 @@ int makeExactGrid(rp_ctx *c, const HostBox &b, Grid &g, int &disabled) {
     g.reach = 1;
     g.period = 2;
     fillTranslations(g);
-    int r = ensureCells(c, g.ncell);
+    int r = ensureCells(c, g.ncell, beads);
     if (r) return r;
-    g.cellStart = c->cellStart;
-    g.cellCount = c->cellCount;
+    g.cellStart = beads ? c->beadCellStart : c->cellStart;
+    g.cellCount = beads ? c->beadCellCount : c->cellCount;
     return RP_OK;
 }
 
@@ -487,6 +495,7 @@ double minHeight(const HostBox &b) {
 // current.
 // Packing::getTotalEnergy (Packing.cpp:690-709) of particles sorted by grid g.
 int totalEnergy(rp_ctx *c, const Grid &g, const Particles &p, double &out);
+int countBeads(rp_ctx *c, const Particles &src, const HostBox &box, int earlyExit, uint64_t &out);
 
 // Simulation::tryScaling in overlap-counting mode (Simulation.cpp:508-517 with Packing::tryScaling's counting branch,
 // Packing.cpp:190-198): the rescaled packing is always counted in full; fewer overlaps accept the move, more reject
@@ -613,11 +622,14 @@ int boxMove(rp_ctx *c, bool &accepted) {
     if (r) return r;
     r = rebuild(c, g, 1, c->box.inv);
     if (r) return r;
-    bool cannotOverlap = isBoxUpscaled(c->box, nb);  // spheres and spherocylinders are convex
+    // spheres and spherocylinders are convex; polyspheres are not (PolysphereTraits.h:133) and are always scanned,
+    // bead by bead on the rescaled particles
+    const bool poly = c->k.kind == RP_SHAPE_POLYSPHERE;
+    bool cannotOverlap = !poly && isBoxUpscaled(c->box, nb);
     if (!cannotOverlap) {
         uint64_t found = 0;
         c->boxScans++;
-        r = scanOverlaps(c, g, 1, found);
+        r = poly ? countBeads(c, c->alt.p, nb, 1, found) : scanOverlaps(c, g, 1, found);
         if (r) return r;
         if (found) return RP_OK;
     }
@@ -713,7 +725,8 @@ int runCycle(rp_ctx *c) {
     // eight wavefronts with grouped moves for dense stencils (measured fastest on 1e5 rods), else fewer
     if (waves <= 0) waves = expect > 384 ? 8 : (expect > 128 ? 2 : 1);
     waves = waves >= 8 ? 8 : (waves >= 4 ? 4 : (waves == 2 ? 2 : 1));
-    if (waves == 8 && (c->k.soft || c->counting)) waves = 4;  // eight wavefronts only for the hard sweep
+    // eight wavefronts only for the staged hard sweep; polyspheres read their stencil from HBM
+    if (waves == 8 && (c->k.soft || c->counting || c->k.kind == RP_SHAPE_POLYSPHERE)) waves = 4;
     const double perEntry =
         (double)(sweep_lds_bytes(c->k.kind, 1024, waves) - sweep_lds_bytes(c->k.kind, 0, waves)) / 1024.0;
     const size_t fixed = sweep_lds_bytes(c->k.kind, 0, waves);
@@ -810,9 +823,10 @@ int adaptSteps(rp_ctx *c) {
 }
 
 // Packing::countTotalOverlaps for polyspheres (Packing.cpp:519-559 over the grid of interaction centres,
-// Packing.cpp:901-929): bead items from the current particles, sorted by the reference's grid of bead range.
-int countExactBeads(rp_ctx *c, int earlyExit, uint64_t &out) {
-    c->sortedForCycle = false;
+// Packing.cpp:901-929): bead items of the particles `src` in box `box`, sorted by the reference's grid of bead range
+// in the bead cell arrays (the sweep grid's arrays stay valid).  RP_ERR_UNSUPPORTED when the reference would count
+// without a grid.
+int countBeads(rp_ctx *c, const Particles &src, const HostBox &box, int earlyExit, uint64_t &out) {
     const size_t items = c->n * (size_t)c->k.nc;
     if (!c->beadA.mem) {
         int r = allocParticles(c->beadA, items);
@@ -823,12 +837,12 @@ int countExactBeads(rp_ctx *c, int earlyExit, uint64_t &out) {
     }
     Grid g{};
     int disabled = 0;
-    int r = makeExactGrid(c, c->box, g, disabled);
+    int r = makeExactGrid(c, box, g, disabled, true);
     if (r) return r;
     if (disabled)
         return fail(RP_ERR_UNSUPPORTED, "the reference would count without a neighbour grid for this box; "
                                         "the all-pairs path is not implemented on the device");
-    RP_HIP(launch_centres(c->stream, c->cur.p, (int)c->n, c->k, g, c->beadA.p));
+    RP_HIP(launch_centres(c->stream, src, (int)c->n, c->k, g, c->beadA.p));
     RP_HIP(launch_rebuild(c->stream, g, (int)items, c->beadA.p, c->beadB.p, c->tmpSlot, c->tmpOrder, c->tmpPos, 0,
                           nullptr, timingSlot(c, 1)));
     r = scanItems(c, g, c->beadB.p, items, earlyExit, out);
@@ -838,7 +852,7 @@ int countExactBeads(rp_ctx *c, int earlyExit, uint64_t &out) {
 }
 
 int countExact(rp_ctx *c, int earlyExit, uint64_t &out) {
-    if (c->k.kind == RP_SHAPE_POLYSPHERE) return countExactBeads(c, earlyExit, out);
+    if (c->k.kind == RP_SHAPE_POLYSPHERE) return countBeads(c, c->cur.p, c->box, earlyExit, out);
     c->sortedForCycle = false;  // the shared cell arrays are about to describe another grid
     Grid g{};
     int disabled = 0;
@@ -942,6 +956,9 @@ int rp_create(rp_ctx **out, const rp_shape_desc *shape, const double box[9], siz
     if (r) return cleanup(r);
     c->k.beads = c->dBeads;
     c->k.nc = shape->kind == RP_SHAPE_POLYSPHERE ? shape->n_centres : 0;
+    // polysphere sweep prefilter: particles whose centres are at least the total range apart share no bead
+    // contact; the 1e-9 margin keeps every pair whose rounded bead distances could decide otherwise
+    if (shape->kind == RP_SHAPE_POLYSPHERE) c->k.range2 = std::pow(c->totalRange * (1 + 1e-9), 2);
     if (shape->kind == RP_SHAPE_POLYSPHERE) {
         // PolysphereGeometry::calculateVolume (PolysphereTraits.cpp:138-146) needs beads that do not overlap by
         // spheresOverlap's test (PolysphereTraits.cpp:211-224): distance^2 (1 + 1e-12) < (r_i + r_j)^2
@@ -1011,7 +1028,7 @@ int rp_set_environment(rp_ctx *c, double temperature, double pressure, const rp_
         c->rotStep = move->rot_step;
     } else {
         // RototranslationSampler::setupForShapeTraits (RototranslationSampler.cpp:95-114)
-        double ratio = c->transStep / c->range;
+        double ratio = c->transStep / c->totalRange;
         c->rotStep = 0.133183 - 1.18634 * ratio + 264.37 * ratio * ratio;
     }
     c->boxKind = box ? box->kind : RP_BOX_NONE;
@@ -1028,7 +1045,6 @@ int rp_set_environment(rp_ctx *c, double temperature, double pressure, const rp_
 
 int rp_run_cycles(rp_ctx *c, size_t nCycles, int adapt, rp_counters *out) {
     if (!c) return fail(RP_ERR_PRECONDITION, "null context");
-    if (c->k.kind == RP_SHAPE_POLYSPHERE) return fail(RP_ERR_UNSUPPORTED, "the polysphere sweep is not built");
     if (!c->envSet) return fail(RP_ERR_PRECONDITION, "environment not set");
     uint64_t overl = 0;
     int r = RP_OK;
@@ -1253,6 +1269,8 @@ void rp_destroy(rp_ctx *c) {
     if (c->beadA.mem) hipFree(c->beadA.mem);
     if (c->beadB.mem) hipFree(c->beadB.mem);
     if (c->dBeads) hipFree(c->dBeads);
+    if (c->beadCellStart) hipFree(c->beadCellStart);
+    if (c->beadCellCount) hipFree(c->beadCellCount);
     if (c->cellStart) hipFree(c->cellStart);
     if (c->cellCount) hipFree(c->cellCount);
     if (c->propMem) hipFree(c->propMem);
@@ -1326,7 +1344,6 @@ int fixOrientations(rp_ctx *c, uint64_t &rejected) {
 
 int rp_fix_orientations(rp_ctx *c, uint64_t *rejected) {
     if (!c) return fail(RP_ERR_PRECONDITION, "null context");
-    if (c->k.kind == RP_SHAPE_POLYSPHERE) return fail(RP_ERR_UNSUPPORTED, "the polysphere sweep is not built");
     uint64_t rej = 0;
     int r = fixOrientations(c, rej);
     if (r) return r;
@@ -1336,6 +1353,8 @@ int rp_fix_orientations(rp_ctx *c, uint64_t *rejected) {
 
 int rp_set_overlap_counting(rp_ctx *c, int enable) {
     if (!c) return fail(RP_ERR_PRECONDITION, "null context");
+    if (enable && c->k.kind == RP_SHAPE_POLYSPHERE)
+        return fail(RP_ERR_UNSUPPORTED, "overlap counting for polyspheres is not built");
     if (!enable || c->k.soft) {
         c->counting = false;
         c->numOverlaps = 0;
@@ -1394,7 +1413,6 @@ int rp_set_orientation_fix_every(rp_ctx *c, uint64_t every) {
 // rp_revert_scaling; the scaled state becomes current whatever the result, as in the reference.
 int rp_try_scaling(rp_ctx *c, const double new_box[9], double *dE) {
     if (!c || !new_box || !dE) return fail(RP_ERR_PRECONDITION, "null argument");
-    if (c->k.kind == RP_SHAPE_POLYSPHERE) return fail(RP_ERR_UNSUPPORTED, "polysphere box moves are not built");
     HostBox nb;
     nb.set(new_box);
     if (nb.volume() == 0) return fail(RP_ERR_PRECONDITION, "tryScaling: the new box has zero volume");
@@ -1429,6 +1447,11 @@ int rp_try_scaling(rp_ctx *c, const double new_box[9], double *dE) {
             count = (long long)after;
             if (count < c->numOverlaps) result = -INF;
             else if (count > c->numOverlaps) result = INF;
+        } else if (c->k.kind == RP_SHAPE_POLYSPHERE) {  // not convex (PolysphereTraits.h:133): always counted
+            uint64_t found = 0;
+            r = countBeads(c, c->alt.p, nb, 1, found);
+            if (r) return r;
+            if (found) result = INF;
         } else if (!isBoxUpscaled(c->box, nb)) {  // spheres and spherocylinders are convex
             uint64_t found = 0;
             r = scanOverlaps(c, g, 1, found);

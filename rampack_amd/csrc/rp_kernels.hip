@@ -1336,6 +1336,26 @@ __device__ __forceinline__ void sweep_cell_pipelined(const SweepParams &P, const
 
 // Sweep of one active cell whose stencil does not fit the LDS area: candidates are read from the cell-sorted
 // arrays in HBM, accepted moves are written there at once.
+// Polysphere candidate test: particle j at p2 (orientation in p) under translation T against the trial beads of the
+// moved particle (sTrial, centred at npos).  Candidates whose centre is at least the prefilter distance away
+// (k.range2 = (total range (1 + 1e-9))^2) cannot reach any bead; otherwise every bead pair is tested as
+// PolysphereTraits::HardInteraction::overlapBetween with the candidate's beads at p2 + O_j c_b.
+__device__ __forceinline__ bool poly_hit(const PairConsts &k, const double *sBeads, const double *sTrial,
+                                         const V3 &npos, const V3 &p2, const Particles &p, int gj, const V3 &T) {
+    const V3 d = sub(add(p2, T), npos);
+    if (!(dot(d, d) < k.range2)) return false;
+    double O[9];
+    for (int e = 0; e < 9; e++) O[e] = p.o[e][gj];
+    for (int b = 0; b < k.nc; b++) {
+        const V3 c2 = add(p2, mulv(O, v3(sBeads[4 * b], sBeads[4 * b + 1], sBeads[4 * b + 2])));
+        for (int a = 0; a < k.nc; a++) {
+            const V3 c1 = v3(sTrial[3 * a], sTrial[3 * a + 1], sTrial[3 * a + 2]);
+            if (bead_overlap(sBeads[4 * a + 3], sBeads[4 * b + 3], c1, c2, T)) return true;
+        }
+    }
+    return false;
+}
+
 template <int KIND, int WAVES>
 __device__ __forceinline__ void sweep_cell_unstaged(const SweepParams &P, const Stencil &S, double *pt, double *pR, int *pk,
                                     const double *sT, int cell, int own0, int nOwn, int total) {
@@ -1345,6 +1365,12 @@ __device__ __forceinline__ void sweep_cell_unstaged(const SweepParams &P, const 
     unsigned long long accepted = 0, leftCell = 0, tests = 0;
     const int moves = P.moves > 0 ? P.moves : nOwn;
     const int pbase = P.moves > 0 ? cell * P.moves : own0;
+    // polyspheres: the bead table, and the trial pose's bead centres of the current move
+    __shared__ double sBeads[KIND == 2 ? 4 * RP_MAX_CENTRES : 1], sTrial[KIND == 2 ? 3 * RP_MAX_CENTRES : 1];
+    if (KIND == 2) {
+        for (int e = tid; e < 4 * P.k.nc; e += NT) sBeads[e] = P.k.beads[e];
+        __syncthreads();
+    }
     for (int chunk = 0; chunk < moves; chunk += MOVE_CHUNK) {
         const int cn = min(MOVE_CHUNK, moves - chunk);
         if (P.fixMode) {
@@ -1383,15 +1409,33 @@ __device__ __forceinline__ void sweep_cell_unstaged(const SweepParams &P, const 
             V3 nax = v3(NO[2], NO[5], NO[8]);
             SCFirst f = make_first<KIND>(P.k, npos, nax);
             const int wx = nw & 1, wy = (nw >> 1) & 1, wz = (nw >> 2) & 1;
+            if (KIND == 2) {
+                // trial bead centres npos + NO * c (PolysphereTraits.cpp:93-95)
+                if (tid < P.k.nc) {
+                    const V3 c1 = add(npos, mulv(NO, v3(sBeads[4 * tid], sBeads[4 * tid + 1], sBeads[4 * tid + 2])));
+                    sTrial[3 * tid] = c1.x;
+                    sTrial[3 * tid + 1] = c1.y;
+                    sTrial[3 * tid + 2] = c1.z;
+                }
+                __syncthreads();
+            }
             bool hit = false;
             for (int base = 0; base < total; base += NT) {
                 const int j = base + tid;
                 bool h = false;
                 if (j < total && j != kIdx) {
-                    V3 p2, a2;
-                    int code;
-                    load_candidate<KIND, false>(P.p, S, j, p2, a2, code);
-                    h = pair_overlap<KIND>(P.k, f, p2, a2, image_translation(sT, code, wx, wy, wz));
+                    if (KIND == 2) {
+                        const int q = stencil_cell(S.start, S.ncells, j);
+                        const int gj = S.base[q] + (j - S.start[q]);
+                        const V3 p2 = load3(P.p.x, P.p.y, P.p.z, gj);
+                        const V3 T = image_translation(sT, image_code(S.ecode[q], P.p.wbits[gj]), wx, wy, wz);
+                        h = poly_hit(P.k, sBeads, sTrial, npos, p2, P.p, gj, T);
+                    } else {
+                        V3 p2, a2;
+                        int code;
+                        load_candidate<KIND, false>(P.p, S, j, p2, a2, code);
+                        h = pair_overlap<KIND>(P.k, f, p2, a2, image_translation(sT, code, wx, wy, wz));
+                    }
                 }
                 tests += (unsigned long long)min(NT, total - base);
                 bool any;
@@ -2039,7 +2083,7 @@ __global__ __launch_bounds__(WAVE *WAVES, WAVES >= 8 ? 4 : 1) void k_sweep(Sweep
     // with one cell across an axis the own cell recurs in its stencil under other images; staged copies would go
     // stale as particles move, so such grids read the stencil from HBM
     const bool ownRecurs = g.n[0] == 1 || g.n[1] == 1 || g.n[2] == 1;
-    if (total <= P.cap && nOwn <= OWN_CAP && !ownRecurs) {
+    if (KIND != 2 && total <= P.cap && nOwn <= OWN_CAP && !ownRecurs) {
         // a cell off the last row on every axis holds no wrapped particle: stage pos + T directly
         const bool translated = cx < g.n[0] - 1 && cy < g.n[1] - 1 && cz < g.n[2] - 1;
         stage_stencil_batched<KIND, NT>(P, S, sT, total, translated);
@@ -2198,7 +2242,7 @@ int sweep_occupancy(int kind, int cap, int waves) {
 hipError_t launch_sweep(hipStream_t s, const SweepParams &sp, int waves, Timed t) {
     unsigned blocks = (unsigned)(sp.na[0] * sp.na[1] * sp.na[2]);
     int w = waves >= 8 ? 8 : (waves >= 4 ? 4 : (waves == 2 ? 2 : 1));
-    if (w == 8 && (sp.k.soft || sp.counting)) w = 4;  // eight wavefronts only for the hard sweep
+    if (w == 8 && (sp.k.soft || sp.counting || sp.k.kind == 2)) w = 4;  // eight only for the staged hard sweep
     size_t lds = sweep_lds_bytes(sp.k.kind, sp.cap, w);
     // soft and hard sweeps are separate kernels: each instantiation carries one path only
 #define RP_SWEEP(KIND, W, SOFT) \
@@ -2207,7 +2251,10 @@ hipError_t launch_sweep(hipStream_t s, const SweepParams &sp, int waves, Timed t
     if (w == 4) RP_SWEEP(KIND, 4, SOFT); \
     else if (w == 2) RP_SWEEP(KIND, 2, SOFT); \
     else RP_SWEEP(KIND, 1, SOFT);
-    if (sp.k.kind == 0) {
+    if (sp.k.kind == 2) {
+        // polyspheres: the hard sweep reading the stencil from HBM, at most four wavefronts
+        RP_SWEEP_W(2, false)
+    } else if (sp.k.kind == 0) {
         if (sp.k.soft || sp.counting) { RP_SWEEP_W(0, true) }
         else if (w == 8) RP_SWEEP(0, 8, false);
         else { RP_SWEEP_W(0, false) }

@@ -52,3 +52,68 @@ def test_polysphere_observables_and_validation():
         ra.GpuPacking(ra.Shape("polysphere", centres=c, radii=-r), box, pos, rot)
     with pytest.raises(ra.PreconditionError):
         ra.GpuPacking(ra.Shape("polysphere", centres=np.zeros((17, 3)), radii=np.ones(17)), box, pos, rot)
+
+
+def chain_start(n_side=(8, 8, 2), spacing=(3.0, 3.0, 11.5)):
+    """Kinked 11-bead chains on an orthorhombic lattice, all in the body frame's orientation (overlap-free: the
+    oracle confirms it in the tests)."""
+    idx = np.stack(np.meshgrid(*[np.arange(k) for k in n_side], indexing="ij"), -1).reshape(-1, 3).astype(float)
+    pos = (idx + 0.5) * np.array(spacing)
+    box = np.diag(np.array(n_side) * np.array(spacing))
+    rot = np.tile(np.eye(3).reshape(1, 9), (len(pos), 1))
+    return box, np.ascontiguousarray(pos), np.ascontiguousarray(rot)
+
+
+def poly_overlaps(beads, box, pos, rot, early=False):
+    c, r = corpus.bead_set(beads)
+    return oracle_lib.oracle().snapshot_stats(2, 0.0, 0.0, box, pos, rot, early_exit=early, centres=c, radii=r)[0]
+
+
+def test_polysphere_sweep_stays_overlap_free():
+    box, pos, rot = chain_start()
+    assert poly_overlaps("kinked11", box, pos, rot) == 0
+    pk = ra.GpuPacking(poly_shape("kinked11"), box, pos, rot, seed=21)
+    pk.set_environment(1.0, 1.0, 0.3, None, box_move=("delta_triclinic", 0.05))
+    for _ in range(3):
+        c = pk.run_cycles(20, adapt_steps=True)
+        b, p, r = pk.download()
+        assert poly_overlaps("kinked11", b, p, r) == 0
+        assert pk.count_total_overlaps() == 0
+    print(c, flush=True)
+    assert c["moves_accepted"] > 0 and c["box_accepted"] > 0
+    assert abs(c["moves_attempted"] - 60 * len(pos)) <= 0.15 * 60 * len(pos)
+    # orientations stay orthonormal and the particles move
+    R = r.reshape(-1, 3, 3)
+    assert np.allclose(R @ R.transpose(0, 2, 1), np.eye(3), atol=1e-12)
+    assert not np.allclose(p, pos)
+    pk.close()
+
+
+@pytest.mark.parametrize("waves", ["1", "2"])
+def test_polysphere_launch_shapes_give_identical_trajectories(waves, monkeypatch):
+    box, pos, rot = chain_start()
+    runs = []
+    for w in (waves, "4"):
+        monkeypatch.setenv("RAMPACK_SWEEP_WAVES", w)
+        pk = ra.GpuPacking(poly_shape("kinked11"), box, pos, rot, seed=5)
+        pk.set_environment(1.0, 1.0, 0.3, None, box_move=("delta_triclinic", 0.05))
+        pk.run_cycles(15)
+        runs.append(pk.download())
+        pk.close()
+    for x, y in zip(*runs):
+        assert np.array_equal(x, y)
+
+
+def test_polysphere_try_scaling():
+    box, pos, rot = chain_start()
+    pk = ra.GpuPacking(poly_shape("kinked11"), box, pos, rot, seed=8)
+    start = pk.download()
+    for f in (1.02, 0.999, 0.7):
+        dE = pk.try_scaling(box * f)
+        b, p, r = pk.download()
+        n = poly_overlaps("kinked11", b, p, r, early=True)
+        assert dE == (np.inf if n else 0.0), f  # polyspheres are not convex: every scaling is counted
+        pk.revert_scaling()
+        assert all(np.array_equal(x, y) for x, y in zip(pk.download(), start))
+    assert pk.try_scaling(box * 0.7) == np.inf
+    pk.close()
