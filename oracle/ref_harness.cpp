@@ -349,6 +349,13 @@ double ref_nematic_p2(std::size_t n, const double *rot) {
 // rot_step <= 0 selects the automatic rotation step (RototranslationSampler.cpp:95-114).  Samples of (packing
 // fraction, P2 (spherocylinders, else 0), total energy / N) are taken every avg_every cycles of the averaging
 // phase.  The return value is the number of samples, or a negative number on error.
+// Bead data for polyspheres (kind 2) in ref_mc_run, which takes no centres of its own.
+static std::vector<double> gBeadCentres, gBeadRadii;
+void ref_set_beads(int nCentres, const double *centres, const double *radii) {
+    gBeadCentres.assign(centres, centres + 3 * nCentres);
+    gBeadRadii.assign(radii, radii + nCentres);
+}
+
 long ref_mc_run(int kind, double radius, double length, int soft, double eps, double sigma, const double *box,
                 std::size_t n, const double *pos, const double *rot, double temperature, double pressure,
                 double transStep, double rotStep, int boxKind, double boxStep, unsigned long seed, int ddx, int ddy,
@@ -358,7 +365,8 @@ long ref_mc_run(int kind, double radius, double length, int soft, double eps, do
     try {
         std::array<std::size_t, 3> dd{(std::size_t)ddx, (std::size_t)ddy, (std::size_t)ddz};
         std::size_t numDomains = dd[0] * dd[1] * dd[2];
-        auto traits = makeTraits(kind, radius, length, 0, nullptr, nullptr, soft, eps, sigma);
+        auto traits = makeTraits(kind, radius, length, kind == 2 ? (int)gBeadRadii.size() : 0, gBeadCentres.data(),
+                                 gBeadRadii.data(), soft, eps, sigma);
         const Interaction &inter = traits->getInteraction();
         auto packing = std::make_unique<Packing>(TriclinicBox(mat(box)), makeShapes(n, pos, rot),
                                                  std::make_unique<PeriodicBoundaryConditions>(), inter,

@@ -213,3 +213,12 @@ def bead_set(name: str):
     if name == "dimer":
         return np.array([[0.0, 0.0, -0.5], [0.0, 0.0, 0.5]]), np.array([0.5, 0.5])
     return np.array([[0.0, 0.0, 0.0], [0.0, 0.0, 0.9], [0.5, 0.3, -0.7]]), np.array([0.5, 0.4, 0.3])
+
+
+def dimer_start(st: dict):
+    """Aligned polyspheres on an orthorhombic lattice: st["n_side"] per axis at st["spacing"], identity
+    orientations (the NpT dumbbell statistics' start state)."""
+    from rampack_amd import lattice
+    box, pos = lattice.sc_positions((st["n_side"],) * 3, st["spacing"])
+    rot = np.tile(np.eye(3).reshape(1, 9), (len(pos), 1))
+    return box, np.ascontiguousarray(pos), np.ascontiguousarray(rot)

@@ -96,6 +96,37 @@ WCA_NPT = dict(name="wca_sphere_npt", n_side=8, phi0=0.30, temperature=1.0, pres
                therm=4000, avg=12000, every=20, seed=1234)
 
 
+# NpT hard dumbbells (two touching beads of diameter 1) from an aligned orthorhombic start, log isotropic box moves
+# (independent runs: samples within one run are correlated far beyond the sampling interval, so the fixture keeps the
+# mean over runs and the spread between them)
+DIMER_NPT = dict(name="dimer_npt", beads="dimer", n_side=6, spacing=(1.6, 1.6, 2.05), temperature=1.0,
+                 pressure=1.5, trans_step=0.2, box_step=0.02, therm=4000, avg=12000, every=20,
+                 seeds=[2468, 11, 12, 13, 14, 15, 16, 17] + list(range(200, 224)))
+
+
+def reference_dimer_statistics(ref):
+    st = dict(DIMER_NPT)
+    from concurrent.futures import ProcessPoolExecutor
+    with ProcessPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
+        means = list(ex.map(dimer_run, [(st, seed) for seed in st["seeds"]]))
+    st.update(phi_runs=means, phi=float(np.mean(means)), phi_run_sd=float(np.std(means, ddof=1)))
+    return st
+
+
+def dimer_run(args):
+    """One reference NpT run of the dumbbell state point; returns its mean packing fraction."""
+    st, seed = args
+    ref = oracle_lib.ref()
+    centres, radii = corpus.bead_set(st["beads"])
+    c = np.ascontiguousarray(centres)
+    r = np.ascontiguousarray(radii)
+    ref.lib.ref_set_beads(len(c), c.ctypes.data_as(oracle_lib._dp), r.ctypes.data_as(oracle_lib._dp))
+    box, pos, rot = corpus.dimer_start(st)
+    out = oracle_lib.ref_mc_run(ref, 2, 0.0, 0.0, box, pos, rot, st["temperature"], st["pressure"], st["trans_step"],
+                                0.0, 2, st["box_step"], seed, therm=st["therm"], avg=st["avg"], avg_every=st["every"])
+    return float(out["samples"][:, 0].mean())
+
+
 def reference_statistics(ref):
     from rampack_amd import lattice
     st = dict(WCA_NPT)
@@ -143,6 +174,9 @@ def main():
             out["large_corpora"] = large_corpora(ref)
         if "--poly-only" in sys.argv:
             out["poly_snapshots"] = poly_snapshots(ref)
+            out["statistics"] = [s for s in out["statistics"] if s["name"] != DIMER_NPT["name"]]
+            out["statistics"].append(reference_dimer_statistics(ref))
+            print(out["statistics"][-1])
         with open(path, "w") as f:
             json.dump(out, f, indent=1)
         return
@@ -181,7 +215,7 @@ def main():
         print(name, energy)
     out["large_corpora"] = large_corpora(ref)
     out["poly_snapshots"] = poly_snapshots(ref)
-    out["statistics"] = [reference_statistics(ref)]
+    out["statistics"] = [reference_statistics(ref), reference_dimer_statistics(ref)]
     print(out["statistics"])
     with open(path, "w") as f:
         json.dump(out, f, indent=1)

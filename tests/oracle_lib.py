@@ -117,6 +117,8 @@ def ref_available() -> bool:
 
 def _setup_ref(lib):
     lib.ref_pair_decisions.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_size_t, _dp, _u8p]
+    lib.ref_set_beads.argtypes = [ctypes.c_int, _dp, _dp]
+    lib.ref_set_beads.restype = None
     lib.ref_snapshot_stats.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp, _dp,
                                        ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, ctypes.c_size_t, _dp, _dp,
                                        ctypes.c_int, _u64p, _dp]
