@@ -75,11 +75,20 @@ typedef struct rp_shape_desc {
     const double *radii;    /* polysphere: n_centres bead radii (> 0), host memory */
 } rp_shape_desc;
 
-/* RototranslationSampler descriptor (src/core/move_samplers/RototranslationSampler.h:17-60). */
+/* Move samplers: RototranslationSampler (src/core/move_samplers/RototranslationSampler.cpp:22-49),
+ * TranslationSampler (TranslationSampler.cpp:20-37), RotationSampler (RotationSampler.cpp:12-33). */
+#define RP_MOVE_ROTOTRANSLATION 0
+#define RP_MOVE_TRANSLATION 1
+#define RP_MOVE_ROTATION 2
+#define RP_MAX_MOVE_SAMPLERS 4
+
+/* Move sampler descriptor (RototranslationSampler.h:17-60, TranslationSampler.h, RotationSampler.h).  kind 0 (the
+ * zero default) is the rototranslation sampler. */
 typedef struct rp_move_desc {
     double trans_step;     /* translation step (> 0) */
     double rot_step;       /* rotation step; <= 0 selects the automatic rule of RototranslationSampler.cpp:95-114 */
     double max_trans_step; /* 0: implicit, half the smallest box height (RototranslationSampler.cpp:78-85) */
+    int kind;              /* RP_MOVE_*; the translation sampler ignores rot_step, the rotation sampler trans_step */
 } rp_move_desc;
 
 /* TriclinicBoxScaler descriptor (src/core/TriclinicBoxScaler.h:15-55). */
@@ -138,6 +147,14 @@ int rp_create(rp_ctx **out, const rp_shape_desc *shape, const double box[9], siz
  * scaler (NULL or RP_BOX_NONE: NVT). */
 int rp_set_environment(rp_ctx *ctx, double temperature, double pressure, const rp_move_desc *move,
                        const rp_box_desc *box);
+
+/* The environment's list of move samplers (Simulation.h:113-165 holds a vector of MoveSampler): each cycle makes
+ * n_moves * N trial moves, each with a sampler drawn uniformly (every sampler requests N moves,
+ * Simulation.cpp:456-465, 683-691); each sampler adapts its steps on its own acceptance counter
+ * (Simulation.cpp:561-607).  Replaces the sampler that rp_set_environment set; 1 <= n_moves <= RP_MAX_MOVE_SAMPLERS.
+ * rp_get_move_samplers copies the samplers with their current step sizes (n entries at most). */
+int rp_set_move_samplers(rp_ctx *ctx, const rp_move_desc *moves, size_t n_moves);
+int rp_get_move_samplers(rp_ctx *ctx, rp_move_desc *out, size_t n, size_t *n_moves);
 
 /* Simulation::performCycle x n_cycles (src/core/Simulation.cpp:325-363): N trial rototranslations as checkerboard
  * colour passes, then one box move; adapt_steps != 0 applies the thermalisation step adaptation of

@@ -22,6 +22,9 @@ LIB_PATH = os.path.join(_HERE, "lib", "librampack_amd.so")
 SHAPE_SPHERE = 0
 SHAPE_SPHEROCYLINDER = 1
 SHAPE_POLYSPHERE = 2
+MOVE_ROTOTRANSLATION = 0
+MOVE_TRANSLATION = 1
+MOVE_ROTATION = 2
 MAX_CENTRES = 16
 SOFT_NONE = 0
 SOFT_WCA = 1
@@ -48,7 +51,7 @@ EXPORTED_SYMBOLS = (
     "rp_ramtrj_recorder_open", "rp_ramtrj_record", "rp_ramtrj_record_packing", "rp_ramtrj_recorder_last_cycle",
     "rp_ramtrj_recorder_close", "rp_ramtrj_player_open", "rp_ramtrj_player_header", "rp_ramtrj_next_snapshot",
     "rp_ramtrj_jump_to_snapshot", "rp_ramtrj_player_close", "rp_pair_decisions_guarded",
-    "rp_try_scaling", "rp_revert_scaling",
+    "rp_try_scaling", "rp_revert_scaling", "rp_set_move_samplers", "rp_get_move_samplers",
 )
 
 
@@ -61,7 +64,7 @@ class ShapeDesc(ctypes.Structure):
 
 class MoveDesc(ctypes.Structure):
     _fields_ = [("trans_step", ctypes.c_double), ("rot_step", ctypes.c_double),
-                ("max_trans_step", ctypes.c_double)]
+                ("max_trans_step", ctypes.c_double), ("kind", ctypes.c_int)]
 
 
 class BoxDesc(ctypes.Structure):
@@ -169,6 +172,9 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.rp_get_observables.argtypes = [vp, ctypes.POINTER(Observables)]
     lib.rp_total_energy.argtypes = [vp, dp]
     lib.rp_try_scaling.argtypes = [vp, dp, dp]
+    lib.rp_set_move_samplers.argtypes = [vp, ctypes.POINTER(MoveDesc), ctypes.c_size_t]
+    lib.rp_get_move_samplers.argtypes = [vp, ctypes.POINTER(MoveDesc), ctypes.c_size_t,
+                                         ctypes.POINTER(ctypes.c_size_t)]
     lib.rp_revert_scaling.argtypes = [vp]
     lib.rp_set_overlap_counting.argtypes = [vp, ctypes.c_int]
     lib.rp_cached_overlaps.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64)]
@@ -425,6 +431,40 @@ class GpuPacking:
         v = ctypes.c_double()
         _check(_lib.rp_total_energy(self._ctx, ctypes.byref(v)))
         return float(v.value)
+
+    def set_move_samplers(self, samplers):
+        """The environment's move samplers (after set_environment): a list of (kind, step) with kind 'rototranslation'
+        (step = (translation, rotation or None for the automatic rule[, maximum translation])), 'translation'
+        (step = translation or (translation, maximum)) or 'rotation' (step = rotation angle).  Each cycle makes
+        len(samplers) * N moves, each with a sampler drawn uniformly."""
+        kinds = {"rototranslation": MOVE_ROTOTRANSLATION, "translation": MOVE_TRANSLATION, "rotation": MOVE_ROTATION}
+        arr = (MoveDesc * len(samplers))()
+        for i, (kind, step) in enumerate(samplers):
+            if kind not in kinds:
+                raise ValueError(f"unknown move sampler {kind}")
+            d = arr[i]
+            d.kind = kinds[kind]
+            st = step if isinstance(step, (tuple, list)) else (step,)
+            if kind == "rotation":
+                d.rot_step = float(st[0])
+            else:
+                d.trans_step = float(st[0])
+                if kind == "rototranslation" and len(st) > 1 and st[1] is not None:
+                    d.rot_step = float(st[1])
+                mx = st[2] if kind == "rototranslation" and len(st) > 2 else (st[1] if kind == "translation"
+                                                                              and len(st) > 1 else 0.0)
+                d.max_trans_step = float(mx or 0.0)
+        _check(_lib.rp_set_move_samplers(self._ctx, arr, len(samplers)))
+
+    def move_samplers(self) -> list:
+        """The move samplers with their current step sizes: dicts of kind, trans_step, rot_step, max_trans_step."""
+        n = ctypes.c_size_t()
+        _check(_lib.rp_get_move_samplers(self._ctx, None, 0, ctypes.byref(n)))
+        arr = (MoveDesc * max(n.value, 1))()
+        _check(_lib.rp_get_move_samplers(self._ctx, arr, n.value, None))
+        names = {MOVE_ROTOTRANSLATION: "rototranslation", MOVE_TRANSLATION: "translation", MOVE_ROTATION: "rotation"}
+        return [dict(kind=names[d.kind], trans_step=d.trans_step, rot_step=d.rot_step,
+                     max_trans_step=d.max_trans_step) for d in arr[:n.value]]
 
     def try_scaling(self, new_box) -> float:
         """Packing::tryScaling (Packing.cpp:172-208): scales the device-resident state to new_box (3x3, columns are

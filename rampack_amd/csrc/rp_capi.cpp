@@ -109,8 +109,14 @@ struct rp_ctx {
     // environment
     bool envSet = false;
     double temperature = 1, pressure = 0;
-    double transStep = 0.1, rotStep = 0.1, maxTransStep = 0;
-    bool maxTransImplicit = true;
+    // move samplers (Simulation.h:113-165 keeps a vector of them): kind, steps, maximum translation step
+    struct Sampler {
+        int kind = RP_MOVE_ROTOTRANSLATION;
+        double transStep = 0.1, rotStep = 0.1, maxTransStep = 0;
+        bool maxTransImplicit = true;
+    };
+    std::vector<Sampler> samplers;
+    uint64_t typeAccBase[RP_MAX_MOVE_SAMPLERS] = {}, typeAttBase[RP_MAX_MOVE_SAMPLERS] = {};
     int boxKind = RP_BOX_NONE;
     double boxStep = 0;
 
@@ -309,12 +315,14 @@ int ensureProposals(rp_ctx *c, size_t need) {
     c->propMem = nullptr;
     c->propCap = 0;
     size_t cap = need + need / 4 + 256;
-    RP_HIP(hipMalloc(&c->propMem, cap * (sizeof(int) + 13 * sizeof(double))));
+    RP_HIP(hipMalloc(&c->propMem, cap * (sizeof(int) + 13 * sizeof(double) + 2)));
     char *q = static_cast<char *>(c->propMem);
     c->prop.t = reinterpret_cast<double *>(q);
     c->prop.R = c->prop.t + 3 * cap;
     c->prop.u = c->k.soft ? c->prop.R + 9 * cap : nullptr;
     c->prop.k = reinterpret_cast<int *>(c->prop.R + 10 * cap);
+    c->prop.type = reinterpret_cast<uint8_t *>(c->prop.k + cap);
+    c->prop.acc = c->prop.type + cap;
     c->prop.stride = (int)cap;
     c->propCap = cap;
     return RP_OK;
@@ -701,21 +709,34 @@ int runCycle(rp_ctx *c) {
     sp.g = g;
     sp.p = c->cur.p;
     sp.k = c->k;
-    double maxT = c->maxTransImplicit ? minHeight(c->box) / 2 : c->maxTransStep;
-    const double ts = std::min(maxT, c->transStep), rs = std::min(c->rotStep, M_PI);
+    // the samplers' steps in use: min(step, maximum step) for translations (TranslationSampler.cpp:29,
+    // RototranslationSampler.cpp:31-33), min(step, pi) for rototranslation angles (RototranslationSampler.cpp:35);
+    // a rotation sampler's step is capped at pi when it grows (RotationSampler.cpp:36-44)
+    MoveSet ms{};
+    ms.n = (int)c->samplers.size();
+    for (int i = 0; i < ms.n; i++) {
+        const auto &sm = c->samplers[i];
+        const double maxT = sm.maxTransImplicit ? minHeight(c->box) / 2 : sm.maxTransStep;
+        ms.kind[i] = sm.kind;
+        ms.ts[i] = std::min(maxT, sm.transStep);
+        ms.rs[i] = sm.kind == RP_MOVE_ROTATION ? sm.rotStep : std::min(sm.rotStep, M_PI);
+    }
     for (int d = 0; d < 3; d++) sp.na[d] = std::max(1, g.n[d] / g.period);
     double lambda = (double)c->n / (double)g.ncell;
     // Dense cells make a fixed number of moves each (the reference's floor(N / domains) per domain,
     // Simulation.cpp:431-433), which balances the workgroups; sparse cells move as many times as they hold
     // particles, so a cycle is exactly N moves.
-    sp.moves = lambda >= 3 ? (int)std::lround(lambda) : 0;
+    // (with k samplers, k times as many: each requests N moves)
+    sp.moves = lambda >= 3 ? (int)std::lround(lambda) * ms.n : 0;
+    sp.mult = ms.n;
+    const size_t proposals = sp.moves > 0 ? (size_t)g.ncell * sp.moves : c->n * (size_t)ms.n;
     {
-        size_t need = sp.moves > 0 ? (size_t)g.ncell * sp.moves : c->n;
-        int r2 = ensureProposals(c, need);
+        int r2 = ensureProposals(c, proposals);
         if (r2) return r2;
     }
     sp.pr = c->prop;
-    RP_HIP(launch_proposals(c->stream, g, sp.pr, sp.moves, c->cur.p.cell, (int)c->n, ts, rs, c->seed,
+    if (ms.n == 1) sp.pr.type = sp.pr.acc = nullptr;  // one sampler: the totals are its counters
+    RP_HIP(launch_proposals(c->stream, g, sp.pr, sp.moves, c->cur.p.cell, (int)c->n, ms, c->seed,
                             (uint32_t)c->cycle, timingSlot(c, 3)));
     const int side = 2 * g.reach + 1;
     double expect = side * side * side * lambda;
@@ -727,6 +748,9 @@ int runCycle(rp_ctx *c) {
     waves = waves >= 8 ? 8 : (waves >= 4 ? 4 : (waves == 2 ? 2 : 1));
     // eight wavefronts only for the staged hard sweep; polyspheres read their stencil from HBM
     if (waves == 8 && (c->k.soft || c->counting || c->k.kind == RP_SHAPE_POLYSPHERE)) waves = 4;
+    // per-sampler acceptance flags are compiled into the kernels of at most four wavefronts only (the eight-wavefront
+    // kernel has no registers to spare for them)
+    if (waves == 8 && ms.n > 1) waves = 4;
     const double perEntry =
         (double)(sweep_lds_bytes(c->k.kind, 1024, waves) - sweep_lds_bytes(c->k.kind, 0, waves)) / 1024.0;
     const size_t fixed = sweep_lds_bytes(c->k.kind, 0, waves);
@@ -765,6 +789,7 @@ int runCycle(rp_ctx *c) {
         }
     }
     c->sortedForCycle = false;
+    RP_HIP(launch_type_counts(c->stream, sp.pr, proposals, c->dCounters));
     if (sp.counting) {
         // Packing::acceptMove adds each accepted move's overlap change to the cached count (Packing.cpp:298-301)
         unsigned long long d = 0;
@@ -796,20 +821,32 @@ int adaptSteps(rp_ctx *c) {
     if (c->cycles % 100 == 0) {
         int r = readCounters(c);
         if (r) return r;
-        uint64_t acc = c->hCounters[CNT_ACCEPTED] - c->accBase;
-        uint64_t att = c->hCounters[CNT_ATTEMPTED] - c->attBase;
-        c->accBase = c->hCounters[CNT_ACCEPTED];
-        c->attBase = c->hCounters[CNT_ATTEMPTED];
-        if (att > 0) {
-            double rate = (double)acc / (double)att;
-            double maxT = c->maxTransImplicit ? minHeight(c->box) / 2 : c->maxTransStep;
+        const bool several = c->samplers.size() > 1;
+        for (size_t i = 0; i < c->samplers.size(); i++) {
+            // one sampler: the move totals; several: each sampler's own counter (Simulation.cpp:561-607)
+            const int ia = several ? CNT_TYPE_ACC + (int)i : CNT_ACCEPTED;
+            const int it = several ? CNT_TYPE_ATT + (int)i : CNT_ATTEMPTED;
+            uint64_t &accBase = several ? c->typeAccBase[i] : c->accBase;
+            uint64_t &attBase = several ? c->typeAttBase[i] : c->attBase;
+            const uint64_t acc = c->hCounters[ia] - accBase, att = c->hCounters[it] - attBase;
+            accBase = c->hCounters[ia];
+            attBase = c->hCounters[it];
+            if (att == 0) continue;
+            const double rate = (double)acc / (double)att;
+            auto &sm = c->samplers[i];
+            const double maxT = sm.maxTransImplicit ? minHeight(c->box) / 2 : sm.maxTransStep;
             if (rate > 0.2) {
-                double old = c->transStep;
-                if (!(maxT > 0 && c->transStep >= maxT)) c->transStep *= 1.1;
-                c->rotStep *= c->transStep / old;
+                if (sm.kind == RP_MOVE_ROTATION) {
+                    sm.rotStep = std::min(sm.rotStep * 1.1, M_PI);  // RotationSampler.cpp:36-44
+                } else {
+                    // TranslationSampler.cpp:48-54; RototranslationSampler.cpp:51-56 scales the rotation too
+                    double old = sm.transStep;
+                    if (!(maxT > 0 && sm.transStep >= maxT)) sm.transStep *= 1.1;
+                    if (sm.kind == RP_MOVE_ROTOTRANSLATION) sm.rotStep *= sm.transStep / old;
+                }
             } else if (rate < 0.1) {
-                c->transStep /= 1.1;
-                c->rotStep /= 1.1;
+                if (sm.kind != RP_MOVE_ROTATION) sm.transStep /= 1.1;
+                if (sm.kind != RP_MOVE_TRANSLATION) sm.rotStep /= 1.1;
             }
         }
     }
@@ -1016,30 +1053,81 @@ int rp_set_environment(rp_ctx *c, double temperature, double pressure, const rp_
                        const rp_box_desc *box) {
     if (!c || !move) return fail(RP_ERR_PRECONDITION, "null argument");
     if (!(temperature > 0)) return fail(RP_ERR_PRECONDITION, "temperature must be > 0");
-    if (!(move->trans_step > 0)) return fail(RP_ERR_PRECONDITION, "translation step must be > 0");
-    if (move->max_trans_step < 0 || (move->max_trans_step > 0 && move->trans_step > move->max_trans_step))
-        return fail(RP_ERR_PRECONDITION, "invalid maximal translation step (RototranslationSampler.cpp:14-16)");
+    const int boxKind = box ? box->kind : RP_BOX_NONE;
+    if (boxKind != RP_BOX_NONE && boxKind != RP_BOX_DELTA_TRICLINIC && boxKind != RP_BOX_LOG_ISOTROPIC &&
+        boxKind != RP_BOX_DELTA_V)
+        return fail(RP_ERR_UNSUPPORTED, "unknown box move");
+    if (boxKind != RP_BOX_NONE && !(box->step > 0))
+        return fail(RP_ERR_PRECONDITION, "box step must be > 0 (TriclinicBoxScaler.cpp:9-11)");
+    int r = rp_set_move_samplers(c, move, 1);
+    if (r) return r;
     c->temperature = temperature;
     c->pressure = pressure;
-    c->transStep = move->trans_step;
-    c->maxTransStep = move->max_trans_step;
-    c->maxTransImplicit = move->max_trans_step == 0;
-    if (move->rot_step > 0) {
-        c->rotStep = move->rot_step;
-    } else {
-        // RototranslationSampler::setupForShapeTraits (RototranslationSampler.cpp:95-114)
-        double ratio = c->transStep / c->totalRange;
-        c->rotStep = 0.133183 - 1.18634 * ratio + 264.37 * ratio * ratio;
-    }
-    c->boxKind = box ? box->kind : RP_BOX_NONE;
-    if (c->boxKind != RP_BOX_NONE && c->boxKind != RP_BOX_DELTA_TRICLINIC && c->boxKind != RP_BOX_LOG_ISOTROPIC &&
-        c->boxKind != RP_BOX_DELTA_V)
-        return fail(RP_ERR_UNSUPPORTED, "unknown box move");
-    if (c->boxKind != RP_BOX_NONE) {
-        if (!(box->step > 0)) return fail(RP_ERR_PRECONDITION, "box step must be > 0 (TriclinicBoxScaler.cpp:9-11)");
-        c->boxStep = box->step;
-    }
+    c->boxKind = boxKind;
+    if (boxKind != RP_BOX_NONE) c->boxStep = box->step;
     c->envSet = true;
+    return RP_OK;
+}
+
+int rp_set_move_samplers(rp_ctx *c, const rp_move_desc *moves, size_t n) {
+    if (!c || !moves) return fail(RP_ERR_PRECONDITION, "null argument");
+    if (n < 1 || n > RP_MAX_MOVE_SAMPLERS)
+        return fail(RP_ERR_PRECONDITION, "1 to " + std::to_string(RP_MAX_MOVE_SAMPLERS) + " move samplers");
+    std::vector<rp_ctx::Sampler> list;
+    for (size_t i = 0; i < n; i++) {
+        const rp_move_desc &m = moves[i];
+        rp_ctx::Sampler sm;
+        sm.kind = m.kind;
+        if (m.kind == RP_MOVE_ROTATION) {
+            if (!(m.rot_step > 0)) return fail(RP_ERR_PRECONDITION, "rotation step must be > 0 (RotationSampler.cpp:10)");
+            sm.rotStep = m.rot_step;
+            sm.transStep = 0;
+        } else if (m.kind == RP_MOVE_TRANSLATION || m.kind == RP_MOVE_ROTOTRANSLATION) {
+            // TranslationSampler.cpp:12-18, RototranslationSampler.cpp:14-16
+            if (!(m.trans_step > 0)) return fail(RP_ERR_PRECONDITION, "translation step must be > 0");
+            if (m.max_trans_step < 0 || (m.max_trans_step > 0 && m.trans_step > m.max_trans_step))
+                return fail(RP_ERR_PRECONDITION, "invalid maximal translation step (RototranslationSampler.cpp:14-16)");
+            sm.transStep = m.trans_step;
+            sm.maxTransStep = m.max_trans_step;
+            sm.maxTransImplicit = m.max_trans_step == 0;
+            if (m.kind == RP_MOVE_TRANSLATION) {
+                sm.rotStep = 0;
+            } else if (m.rot_step > 0) {
+                sm.rotStep = m.rot_step;
+            } else {
+                // RototranslationSampler::setupForShapeTraits (RototranslationSampler.cpp:95-114)
+                double ratio = sm.transStep / c->totalRange;
+                sm.rotStep = 0.133183 - 1.18634 * ratio + 264.37 * ratio * ratio;
+            }
+        } else {
+            return fail(RP_ERR_UNSUPPORTED, "unknown move sampler");
+        }
+        list.push_back(sm);
+    }
+    c->samplers = list;
+    if (c->dCounters) {
+        // the per-sampler counters restart with the new list
+        int r = readCounters(c);
+        if (r) return r;
+        for (int i = 0; i < RP_MAX_MOVE_SAMPLERS; i++) {
+            c->typeAccBase[i] = c->hCounters[CNT_TYPE_ACC + i];
+            c->typeAttBase[i] = c->hCounters[CNT_TYPE_ATT + i];
+        }
+    }
+    return RP_OK;
+}
+
+int rp_get_move_samplers(rp_ctx *c, rp_move_desc *out, size_t n, size_t *nMoves) {
+    if (!c || (n && !out)) return fail(RP_ERR_PRECONDITION, "null argument");
+    if (nMoves) *nMoves = c->samplers.size();
+    for (size_t i = 0; i < n && i < c->samplers.size(); i++) {
+        const auto &sm = c->samplers[i];
+        out[i] = rp_move_desc{};
+        out[i].kind = sm.kind;
+        out[i].trans_step = sm.transStep;
+        out[i].rot_step = sm.rotStep;
+        out[i].max_trans_step = sm.maxTransImplicit ? 0 : sm.maxTransStep;
+    }
     return RP_OK;
 }
 
@@ -1100,9 +1188,13 @@ int rp_run_cycles(rp_ctx *c, size_t nCycles, int adapt, rp_counters *out) {
         out->box_too_small = c->boxTooSmall;
         out->unstaged_blocks = c->hCounters[CNT_UNSTAGED_BLOCKS];
         out->pair_tests = c->hCounters[CNT_TESTS];
-        double maxT = c->maxTransImplicit ? minHeight(c->box) / 2 : c->maxTransStep;
-        out->trans_step = std::min(maxT, c->transStep);
-        out->rot_step = c->rotStep;
+        // the first sampler that translates and the first that rotates
+        out->trans_step = out->rot_step = 0;
+        for (auto it = c->samplers.rbegin(); it != c->samplers.rend(); ++it) {
+            const double maxT = it->maxTransImplicit ? minHeight(c->box) / 2 : it->maxTransStep;
+            if (it->kind != RP_MOVE_ROTATION) out->trans_step = std::min(maxT, it->transStep);
+            if (it->kind != RP_MOVE_TRANSLATION) out->rot_step = it->rotStep;
+        }
         out->box_step = c->boxStep;
     }
     return RP_OK;
@@ -1311,6 +1403,7 @@ int fixOrientations(rp_ctx *c, uint64_t &rejected) {
     sp.k = c->k;
     for (int d = 0; d < 3; d++) sp.na[d] = std::max(1, g.n[d] / g.period);
     sp.moves = 0;
+    sp.mult = 1;
     sp.fixMode = 1;
     sp.pipelined = 0;
     sp.counters = c->dFixCounters;

@@ -43,7 +43,18 @@ struct Proposals {
     double *t;      // translation, 3 x stride
     double *R;      // rotation matrix, row-major, 9 x stride
     double *u;      // Metropolis uniform in [0, 1) for soft interactions; nullptr for hard ones
+    uint8_t *type;  // with several move samplers: the sampler of each proposal (255: no move); else nullptr
+    uint8_t *acc;   // with several move samplers: 1 once the sweep accepts the proposal; else nullptr
     int stride;
+};
+
+// The move samplers of a cycle (RototranslationSampler, TranslationSampler, RotationSampler): each proposal picks
+// one uniformly (every sampler requests N moves, Simulation.cpp:456-465, 683-691) and uses its step sizes.
+struct MoveSet {
+    int n;
+    int kind[RP_MAX_MOVE_SAMPLERS];   // RP_MOVE_*
+    double ts[RP_MAX_MOVE_SAMPLERS];  // translation half-width in use (already capped by the maximum step)
+    double rs[RP_MAX_MOVE_SAMPLERS];  // rotation half-angle in use
 };
 
 struct SweepParams {
@@ -51,7 +62,8 @@ struct SweepParams {
     Particles p;
     PairConsts k;
     Proposals pr;
-    int moves;              // trial moves per active cell and colour pass; 0: as many as the cell holds
+    int moves;              // trial moves per active cell and colour pass; 0: mult per particle the cell holds
+    int mult;               // moves per particle in that case: the number of move samplers
     int colour;
     int na[3];              // active cells per axis (n / period)
     int cap;                // stencil entries that fit in LDS
@@ -97,7 +109,9 @@ enum Counter : int {
     CNT_MAX_STENCIL = 5,
     CNT_TESTS = 6,
     CNT_OVERLAP_DELTA = 7,  // overlap-counting mode: sum of the accepted moves' overlap changes (two's complement)
-    CNT_NUM = 8
+    CNT_TYPE_ATT = 8,       // per move sampler (RP_MAX_MOVE_SAMPLERS entries): attempted moves
+    CNT_TYPE_ACC = 12,      // per move sampler: accepted moves
+    CNT_NUM = 16
 };
 
 // Launch wrappers (rp_kernels.hip).  All enqueue on `s` and return the launch status.
@@ -124,10 +138,12 @@ hipError_t launch_rebuild(hipStream_t s, const Grid &g, int n, Particles src, Pa
 hipError_t launch_count_overlaps(hipStream_t s, const Grid &g, int n, Particles p, PairConsts k, int earlyExit,
                                  unsigned long long *counters, int cap, Timed t = {});
 size_t count_lds_bytes(int kind, int cap);
-// Proposals of one cycle (translation step ts, rotation step rs): `moves` per cell when moves > 0, else one per
-// particle of its cell (cellOf = cell of every sorted record, n records).
+// Proposals of one cycle: `moves` per cell when moves > 0, else ms.n per particle of its cell (cellOf = cell of every
+// sorted record, n records).
 hipError_t launch_proposals(hipStream_t s, const Grid &g, const Proposals &pr, int moves, const int *cellOf, int n,
-                            double ts, double rs, uint64_t seed, uint32_t cycle, Timed t = {});
+                            const MoveSet &ms, uint64_t seed, uint32_t cycle, Timed t = {});
+// Adds each proposal's attempt and acceptance (pr.type, pr.acc; `total` proposals) to the per-sampler counters.
+hipError_t launch_type_counts(hipStream_t s, const Proposals &pr, size_t total, unsigned long long *counters);
 // waves: wavefronts per active cell (1, 2 or 4)
 // Soft pair energies on records pos1[3] O1[9] pos2[3] O2[9] T[3].
 hipError_t launch_pair_energies(hipStream_t s, PairConsts k, size_t n, const double *rec, double *out);

@@ -933,10 +933,12 @@ __device__ __forceinline__ void rodrigues(V3 axis, double angle, double *R) {
     for (int c = 0; c < 9; c++) R[c] = ((c % 4 == 0 ? 1.0 : 0.0) + K[c] * sa) + fKK[c];
 }
 
-// moves > 0: `moves` proposals per cell at [cell * moves, ...); moves == 0: one per particle, cell c's at
-// [cellStart[c], cellStart[c + 1]) (the caller passes the cell of every sorted record in `cellOf`).
+// moves > 0: `moves` proposals per cell at [cell * moves, ...); moves == 0: ms.n per particle, cell c's at
+// [ms.n cellStart[c], ms.n cellStart[c + 1]) (the caller passes the cell of every sorted record in `cellOf`).
+// With several samplers the first draw picks the sampler (Simulation.cpp:456-465); a translation sampler's move
+// has the identity rotation and a rotation sampler's a zero translation, which leave the pose bit-identical.
 __global__ __launch_bounds__(256) void k_proposals(Grid g, Proposals pr, int moves, const int *__restrict__ cellOf,
-                                                   int n, double ts, double rs, uint64_t seed, uint32_t cycle) {
+                                                   int n, MoveSet ms, uint64_t seed, uint32_t cycle) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     int cell, m;
     if (moves > 0) {
@@ -944,33 +946,72 @@ __global__ __launch_bounds__(256) void k_proposals(Grid g, Proposals pr, int mov
         cell = i / moves;
         m = i - cell * moves;
     } else {
-        if (i >= n) return;
-        cell = cellOf[i];
-        m = i - g.cellStart[cell];
+        if (i >= n * ms.n) return;
+        cell = cellOf[i / ms.n];
+        m = i - g.cellStart[cell] * ms.n;
     }
     const int nOwn = g.cellStart[cell + 1] - g.cellStart[cell];
-    if (nOwn == 0) return;
+    if (nOwn == 0) {
+        if (pr.type) pr.type[i] = 255;
+        if (pr.acc) pr.acc[i] = 0;
+        return;
+    }
     Philox rng = philox_init(seed, cycle, (uint32_t)cell, (uint32_t)m);
+    const int type = ms.n > 1 ? (int)philox_below(rng, (uint32_t)ms.n) : 0;
+    const int kind = ms.kind[type];
+    const double ts = ms.ts[type], rs = ms.rs[type];
     pr.k[i] = (int)philox_below(rng, (uint32_t)nOwn);
-    for (int d = 0; d < 3; d++) pr.t[d * pr.stride + i] = philox_uniform(rng, -ts, ts);
-    V3 axis;
-    do {
-        axis.x = philox_uniform(rng, -1, 1);
-        axis.y = philox_uniform(rng, -1, 1);
-        axis.z = philox_uniform(rng, -1, 1);
-    } while (dot(axis, axis) > 1 || dot(axis, axis) == 0);
-    double R[9];
-    rodrigues(axis, philox_uniform(rng, -rs, rs), R);
+    for (int d = 0; d < 3; d++)
+        pr.t[d * pr.stride + i] = kind == RP_MOVE_ROTATION ? 0.0 : philox_uniform(rng, -ts, ts);
+    double R[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    if (kind != RP_MOVE_TRANSLATION) {
+        V3 axis;
+        do {
+            axis.x = philox_uniform(rng, -1, 1);
+            axis.y = philox_uniform(rng, -1, 1);
+            axis.z = philox_uniform(rng, -1, 1);
+        } while (dot(axis, axis) > 1 || dot(axis, axis) == 0);
+        rodrigues(axis, philox_uniform(rng, -rs, rs), R);
+    }
     for (int c = 0; c < 9; c++) pr.R[c * pr.stride + i] = R[c];
     if (pr.u) pr.u[i] = philox_unit(rng);  // Simulation.cpp:483, drawn after the move as there
+    if (pr.type) pr.type[i] = (uint8_t)type;
+    if (pr.acc) pr.acc[i] = 0;
 }
 
 hipError_t launch_proposals(hipStream_t s, const Grid &g, const Proposals &pr, int moves, const int *cellOf, int n,
-                            double ts, double rs, uint64_t seed, uint32_t cycle, Timed t) {
-    size_t total = moves > 0 ? (size_t)g.ncell * moves : (size_t)n;
+                            const MoveSet &ms, uint64_t seed, uint32_t cycle, Timed t) {
+    size_t total = moves > 0 ? (size_t)g.ncell * moves : (size_t)n * ms.n;
     if (total == 0) return hipSuccess;
     hipExtLaunchKernelGGL(k_proposals, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, t.start, t.stop, 0, g,
-                          pr, moves, cellOf, n, ts, rs, seed, cycle);
+                          pr, moves, cellOf, n, ms, seed, cycle);
+    return hipGetLastError();
+}
+
+// Per-sampler attempts and acceptances of one cycle's proposals, summed per workgroup in LDS.
+__global__ __launch_bounds__(256) void k_type_counts(Proposals pr, size_t total, unsigned long long *counters) {
+    __shared__ unsigned s[2 * RP_MAX_MOVE_SAMPLERS];
+    if (threadIdx.x < 2 * RP_MAX_MOVE_SAMPLERS) s[threadIdx.x] = 0;
+    __syncthreads();
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < total) {
+        const int t = pr.type[i];
+        if (t < RP_MAX_MOVE_SAMPLERS) {
+            atomicAdd(&s[t], 1u);
+            if (pr.acc[i]) atomicAdd(&s[RP_MAX_MOVE_SAMPLERS + t], 1u);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < RP_MAX_MOVE_SAMPLERS) {
+        if (s[threadIdx.x]) atomicAdd(&counters[CNT_TYPE_ATT + threadIdx.x], (unsigned long long)s[threadIdx.x]);
+        const unsigned a = s[RP_MAX_MOVE_SAMPLERS + threadIdx.x];
+        if (a) atomicAdd(&counters[CNT_TYPE_ACC + threadIdx.x], (unsigned long long)a);
+    }
+}
+
+hipError_t launch_type_counts(hipStream_t s, const Proposals &pr, size_t total, unsigned long long *counters) {
+    if (total == 0 || !pr.type || !pr.acc) return hipSuccess;
+    hipLaunchKernelGGL(k_type_counts, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, pr, total, counters);
     return hipGetLastError();
 }
 
@@ -1036,8 +1077,8 @@ __device__ __forceinline__ void sweep_cell_staged(const SweepParams &P, const St
     int *myQueue = queue + wave * WAVE * slots;
     unsigned long long accepted = 0, leftCell = 0, tests = 0;
     int move = 0;
-    const int moves = P.moves > 0 ? P.moves : nOwn;
-    const int pbase = P.moves > 0 ? cell * P.moves : own0;
+    const int moves = P.moves > 0 ? P.moves : nOwn * P.mult;
+    const int pbase = P.moves > 0 ? cell * P.moves : own0 * P.mult;
 
     for (int chunk = 0; chunk < moves; chunk += MOVE_CHUNK) {
         const int cn = min(MOVE_CHUNK, moves - chunk);
@@ -1131,6 +1172,7 @@ __device__ __forceinline__ void sweep_cell_staged(const SweepParams &P, const St
 
             // Packing::acceptMove: the cell belongs to this workgroup for the whole colour pass
             accepted++;
+            if (WAVES < 8 && P.pr.acc && !P.fixMode && tid == 0) P.pr.acc[pbase + chunk + mi] = 1;
             S.x[kIdx] = npos.x;
             S.y[kIdx] = npos.y;
             S.z[kIdx] = npos.z;
@@ -1215,8 +1257,8 @@ __device__ __forceinline__ void sweep_cell_pipelined(const SweepParams &P, const
     int *myQueue = queue + (wave > 0 ? (wave - 1) : 0) * WAVE * slots;
     unsigned long long accepted = 0, leftCell = 0, tests = 0;
     int move = 0;
-    const int moves = P.moves > 0 ? P.moves : nOwn;
-    const int pbase = P.moves > 0 ? cell * P.moves : own0;
+    const int moves = P.moves > 0 ? P.moves : nOwn * P.mult;
+    const int pbase = P.moves > 0 ? cell * P.moves : own0 * P.mult;
 
     for (int chunk = 0; chunk < moves; chunk += MOVE_CHUNK) {
         const int cn = min(MOVE_CHUNK, moves - chunk);
@@ -1298,6 +1340,7 @@ __device__ __forceinline__ void sweep_cell_pipelined(const SweepParams &P, const
 
             // Packing::acceptMove: every lane writes the same values, so the next move sees them without a barrier
             accepted++;
+            if (WAVES < 8 && P.pr.acc && tid == 0) P.pr.acc[pbase + chunk + mi] = 1;
             S.x[kIdx] = npos.x;
             S.y[kIdx] = npos.y;
             S.z[kIdx] = npos.z;
@@ -1363,8 +1406,8 @@ __device__ __forceinline__ void sweep_cell_unstaged(const SweepParams &P, const 
     const Grid &g = P.g;
     const int tid = threadIdx.x;
     unsigned long long accepted = 0, leftCell = 0, tests = 0;
-    const int moves = P.moves > 0 ? P.moves : nOwn;
-    const int pbase = P.moves > 0 ? cell * P.moves : own0;
+    const int moves = P.moves > 0 ? P.moves : nOwn * P.mult;
+    const int pbase = P.moves > 0 ? cell * P.moves : own0 * P.mult;
     // polyspheres: the bead table, and the trial pose's bead centres of the current move
     __shared__ double sBeads[KIND == 2 ? 4 * RP_MAX_CENTRES : 1], sTrial[KIND == 2 ? 3 * RP_MAX_CENTRES : 1];
     if (KIND == 2) {
@@ -1448,6 +1491,7 @@ __device__ __forceinline__ void sweep_cell_unstaged(const SweepParams &P, const 
             }
             if (hit) continue;
             accepted++;
+            if (WAVES < 8 && P.pr.acc && !P.fixMode && tid == 0) P.pr.acc[pbase + chunk + mi] = 1;
             if (tid == 0) {
                 P.p.x[gi] = npos.x;
                 P.p.y[gi] = npos.y;
@@ -1605,8 +1649,8 @@ __device__ __forceinline__ void sweep_cell_grouped(const SweepParams &P, const S
     int *discard = myQueue + (GQ - WAVE) + lane;
     unsigned long long accepted = 0, leftCell = 0, tests = 0;
     unsigned long long tBusy = 0, tDecide = 0, tTop = 0, tPassB = 0, nGroups = 0, nReplans = 0;  // phase timers
-    const int moves = P.moves > 0 ? P.moves : nOwn;
-    const int pbase = P.moves > 0 ? cell * P.moves : own0;
+    const int moves = P.moves > 0 ? P.moves : nOwn * P.mult;
+    const int pbase = P.moves > 0 ? cell * P.moves : own0 * P.mult;
     // issue priority: the planner and the testers with an extra slot bound each group's time
     if ((P.tuning & 1) && planner) __builtin_amdgcn_s_setprio(2);
     if ((P.tuning & 2) && !planner && (wave - 1) * WAVE < total - ((total - 1) / NTT) * NTT) __builtin_amdgcn_s_setprio(1);
@@ -1615,6 +1659,7 @@ __device__ __forceinline__ void sweep_cell_grouped(const SweepParams &P, const S
         const int cn = min(MOVE_CHUNK, moves - chunk);
         load_proposals(P, pbase, chunk, cn, pt, pR, pk);
         if (tid < GBUF) hitWords[tid] = 0;
+        unsigned long long chunkAcc = 0;  // bit m: move m of the chunk was accepted (several samplers only)
         __syncthreads();
         // The chunk's groups, in registers of every wavefront: lane m holds the particle of move m and the length
         // of the run of moves on distinct particles that starts there (at most GROUP); the groups follow from
@@ -1766,6 +1811,7 @@ __device__ __forceinline__ void sweep_cell_grouped(const SweepParams &P, const S
                 if (!hit) acc |= 1u << k;
             }
             accepted += (unsigned long long)__popc(acc);
+            if (WAVES < 8) chunkAcc |= (unsigned long long)acc << m0;
             leftCell += (unsigned long long)(gs - __popc(insideMask));
             tests += (unsigned long long)total * (unsigned long long)__popc(insideMask);
             const bool replan = (acc & nextUses) != 0;
@@ -1802,6 +1848,8 @@ __device__ __forceinline__ void sweep_cell_grouped(const SweepParams &P, const S
                 nReplans += replan ? 1 : 0;
             }
         }
+        // several samplers: the accepted moves of the chunk, for the per-sampler counters
+        if (WAVES < 8 && P.pr.acc && tid < cn && ((chunkAcc >> tid) & 1)) P.pr.acc[pbase + chunk + tid] = 1;
         __syncthreads();
     }
     if (clk) {
@@ -1854,8 +1902,8 @@ __device__ __forceinline__ void sweep_cell_soft(const SweepParams &P, const Sten
     unsigned long long accepted = 0, leftCell = 0, tests = 0;
     long long overlapDelta = 0;
     int move = 0;
-    const int moves = P.moves > 0 ? P.moves : nOwn;
-    const int pbase = P.moves > 0 ? cell * P.moves : own0;
+    const int moves = P.moves > 0 ? P.moves : nOwn * P.mult;
+    const int pbase = P.moves > 0 ? cell * P.moves : own0 * P.mult;
     for (int chunk = 0; chunk < moves; chunk += MOVE_CHUNK) {
         const int cn = min(MOVE_CHUNK, moves - chunk);
         load_proposals(P, pbase, chunk, cn, pt, pR, pk);
@@ -1942,6 +1990,7 @@ __device__ __forceinline__ void sweep_cell_soft(const SweepParams &P, const Sten
             }
 
             accepted++;
+            if (WAVES < 8 && P.pr.acc && tid == 0) P.pr.acc[pbase + chunk + mi] = 1;
             if (STAGED) {
                 S.x[kIdx] = npos.x;
                 S.y[kIdx] = npos.y;

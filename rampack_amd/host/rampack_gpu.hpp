@@ -157,12 +157,54 @@ public:
         return {{"translation", this->translationStepSize}, {"rotation", this->rotationStepSize.value_or(0)}};
     }
     [[nodiscard]] rp_move_desc descriptor() const override {
-        return {this->translationStepSize, this->rotationStepSize.value_or(0), this->maxTranslationStepSize};
+        return {this->translationStepSize, this->rotationStepSize.value_or(0), this->maxTranslationStepSize,
+                RP_MOVE_ROTOTRANSLATION};
     }
     void updateStepSizes(double translation, double rotation) override {
         this->translationStepSize = translation;
         this->rotationStepSize = rotation;
     }
+};
+
+// TranslationSampler (src/core/move_samplers/TranslationSampler.cpp:8-59)
+class TranslationSampler : public MoveSampler {
+    double translationStepSize;
+    double maxTranslationStepSize;
+
+public:
+    explicit TranslationSampler(double translationStepSize, double maxTranslationStepSize = 0)
+            : translationStepSize{translationStepSize}, maxTranslationStepSize{maxTranslationStepSize} {
+        if (!(maxTranslationStepSize >= 0)) throw PreconditionException("maxTranslationStepSize >= 0");
+        if (maxTranslationStepSize > 0 && translationStepSize > maxTranslationStepSize)
+            throw PreconditionException("translationStepSize <= maxTranslationStepSize");
+        if (!(translationStepSize > 0)) throw PreconditionException("translationStepSize > 0");
+    }
+    [[nodiscard]] std::string getName() const override { return "translation"; }
+    [[nodiscard]] std::size_t getNumOfRequestedMoves(std::size_t numParticles) const override { return numParticles; }
+    [[nodiscard]] std::vector<std::pair<std::string, double>> getStepSizes() const override {
+        return {{"translation", this->translationStepSize}};
+    }
+    [[nodiscard]] rp_move_desc descriptor() const override {
+        return {this->translationStepSize, 0, this->maxTranslationStepSize, RP_MOVE_TRANSLATION};
+    }
+    void updateStepSizes(double translation, double) override { this->translationStepSize = translation; }
+};
+
+// RotationSampler (src/core/move_samplers/RotationSampler.cpp:8-60)
+class RotationSampler : public MoveSampler {
+    double rotationStepSize;
+
+public:
+    explicit RotationSampler(double rotationStepSize) : rotationStepSize{rotationStepSize} {
+        if (!(rotationStepSize > 0)) throw PreconditionException("rotationStepSize > 0");
+    }
+    [[nodiscard]] std::string getName() const override { return "rotation"; }
+    [[nodiscard]] std::size_t getNumOfRequestedMoves(std::size_t numParticles) const override { return numParticles; }
+    [[nodiscard]] std::vector<std::pair<std::string, double>> getStepSizes() const override {
+        return {{"rotation", this->rotationStepSize}};
+    }
+    [[nodiscard]] rp_move_desc descriptor() const override { return {0, this->rotationStepSize, 0, RP_MOVE_ROTATION}; }
+    void updateStepSizes(double, double rotation) override { this->rotationStepSize = rotation; }
 };
 
 class TriclinicBoxScaler {
@@ -452,8 +494,10 @@ public:
                unsigned long seed, std::unique_ptr<TriclinicBoxScaler> boxScaler)
             : packing{std::move(packing)}, moveSamplers{std::move(moveSamplers)}, boxScaler{std::move(boxScaler)} {
         if (!this->packing) throw PreconditionException("Simulation: no packing");
-        if (this->moveSamplers.size() != 1 || this->moveSamplers.front()->getName() != "rototranslation")
-            throw PreconditionException("Simulation: the device sweep runs one rototranslation sampler");
+        if (this->moveSamplers.empty() || this->moveSamplers.size() > RP_MAX_MOVE_SAMPLERS)
+            throw PreconditionException("Simulation: 1 to " + std::to_string(RP_MAX_MOVE_SAMPLERS) + " move samplers");
+        for (const auto &sampler : this->moveSamplers)
+            if (!sampler) throw PreconditionException("Simulation: null move sampler");
         this->packing->reseed(seed);
     }
 
@@ -471,10 +515,12 @@ public:
             throw PreconditionException("averagingEvery");
         this->temperature = temperature_;
         this->pressure = pressure_;
-        rp_move_desc md = this->moveSamplers.front()->descriptor();
+        std::vector<rp_move_desc> md;
+        for (const auto &sampler : this->moveSamplers) md.push_back(sampler->descriptor());
         rp_box_desc bd{RP_BOX_NONE, 0};
         if (this->boxScaler) bd = this->boxScaler->descriptor();
-        check(rp_set_environment(this->packing->context(), temperature_, pressure_, &md, &bd));
+        check(rp_set_environment(this->packing->context(), temperature_, pressure_, md.data(), &bd));
+        check(rp_set_move_samplers(this->packing->context(), md.data(), md.size()));
         auto advance = [&](std::size_t cycles, bool adapt) {
             while (cycles > 0) {
                 std::size_t chunk = cycles;
@@ -505,7 +551,10 @@ private:
     void run(std::size_t cycles, bool adapt) {
         check(rp_run_cycles(this->packing->context(), cycles, adapt ? 1 : 0, &this->counters));
         this->packing->markStale();
-        this->moveSamplers.front()->updateStepSizes(this->counters.trans_step, this->counters.rot_step);
+        std::vector<rp_move_desc> steps(this->moveSamplers.size());
+        check(rp_get_move_samplers(this->packing->context(), steps.data(), steps.size(), nullptr));
+        for (std::size_t i = 0; i < steps.size(); i++)
+            this->moveSamplers[i]->updateStepSizes(steps[i].trans_step, steps[i].rot_step);
         if (this->boxScaler) this->boxScaler->setStepSize(this->counters.box_step);
     }
 };

@@ -68,6 +68,20 @@ int main() {
     CHECK(sim.getCounters().box_accepted > 0);
     CHECK(sim.getMoveAcceptanceRate() > 0.05 && sim.getMoveAcceptanceRate() < 0.6);
 
+    // separate translation and rotation samplers (SimulationTest.cpp:134-146 uses them the same way)
+    {
+        auto p3 = std::make_unique<Packing>(box, shapes, traits.getInteraction());
+        std::vector<std::unique_ptr<MoveSampler>> s4;
+        s4.push_back(std::make_unique<TranslationSampler>(0.1));
+        s4.push_back(std::make_unique<RotationSampler>(0.5));
+        Simulation twoSamplers(std::move(p3), std::move(s4), 77,
+                               std::make_unique<TriclinicAdapter>(std::make_unique<LogIsotropicVolumeScaler>(), 1e-4));
+        twoSamplers.integrate(1.0, 5.0, 300, 300, 100);
+        CHECK(twoSamplers.getCounters().moves_attempted == 2 * 600 * 1000);
+        CHECK(twoSamplers.getPacking().countTotalOverlaps(traits.getInteraction(), false) == 0);
+        CHECK(twoSamplers.getPacking().getPackingFraction(traits.getVolume()) > 0.34);
+    }
+
     // Packing::tryScaling / revertScaling (Packing.cpp:172-208, 363-371) on the lattice
     {
         Packing pk(box, shapes, traits.getInteraction());
