@@ -47,6 +47,14 @@ extern "C" {
 #define RP_SOFT_NONE 0
 #define RP_SOFT_WCA 1
 #define RP_SOFT_WCA_SEGMENT 2
+/* RP_SOFT_LJ -- a sphere with LennardJonesInteraction (src/core/interactions/LennardJonesInteraction.cpp:8-13):
+ *   4 eps ((s/r)^12 - (s/r)^6), range 3 sigma (LennardJonesInteraction.h:25).  The reference adds it for every pair
+ *   its neighbour grid visits, so its truncation depends on the grid; sweeps and total energies here truncate at the
+ *   range (pairs at and beyond 3 sigma contribute 0), rp_pair_energies does not truncate.
+ * RP_SOFT_SIC -- a sphere with SquareInverseCoreInteraction (SquareInverseCoreInteraction.cpp:13-15):
+ *   eps max(0, sigma^2 / r^2 - 1), range sigma; eps may be negative. */
+#define RP_SOFT_LJ 3
+#define RP_SOFT_SIC 4
 
 /* Precision of the overlap tests.  RP_PREC_FP64 evaluates every test in the reference's FP64 operation order. */
 #define RP_PREC_FP64 0

@@ -32,6 +32,8 @@
 #include "core/shapes/SpherocylinderTraits.h"
 #include "core/shapes/PolysphereTraits.h"
 #include "core/interactions/RepulsiveLennardJonesInteraction.h"
+#include "core/interactions/LennardJonesInteraction.h"
+#include "core/interactions/SquareInverseCoreInteraction.h"
 #include "core/move_samplers/RototranslationSampler.h"
 #include "core/volume_scalers/TriclinicDeltaScaler.h"
 #include "core/volume_scalers/TriclinicAdapter.h"
@@ -68,6 +70,10 @@ std::unique_ptr<ShapeTraits> makeTraits(int kind, double radius, double length, 
     if (kind == 0) {
         if (soft == 1)
             return std::make_unique<SphereTraits>(radius, std::make_shared<RepulsiveLennardJonesInteraction>(eps, sigma));
+        if (soft == 3)
+            return std::make_unique<SphereTraits>(radius, std::make_shared<LennardJonesInteraction>(eps, sigma));
+        if (soft == 4)
+            return std::make_unique<SphereTraits>(radius, std::make_shared<SquareInverseCoreInteraction>(eps, sigma));
         return std::make_unique<SphereTraits>(radius);
     }
     if (kind == 1)
@@ -120,10 +126,10 @@ int ref_pair_decisions(int kind, double radius, double length, std::size_t n, co
 
 // Interaction::calculateEnergyBetween through the reference's RepulsiveLennardJonesInteraction on a sphere
 // (CentralInteraction.h:58-67, RepulsiveLennardJonesInteraction.cpp:8-16) with a fixed cell translation.  Only
-// soft == 1 exists in the reference; other values return -1.
+// soft == 1 (WCA), 3 (Lennard-Jones) and 4 (square-inverse core) exist in the reference; other values return -1.
 int ref_pair_energies(int kind, double radius, double length, int soft, double eps, double sigma, std::size_t n,
                       const double *rec, double *out) {
-    if (kind != 0 || soft != 1) return -1;
+    if (kind != 0 || (soft != 1 && soft != 3 && soft != 4)) return -1;
     auto traits = makeTraits(kind, radius, length, 0, nullptr, nullptr, soft, eps, sigma);
     const Interaction &inter = traits->getInteraction();
     for (std::size_t i = 0; i < n; i++) {

@@ -246,7 +246,7 @@ struct ShapeSpec {
     // L + sigma 2^(1/6), the farthest centre distance at which two segments can be within the cut-off
     double range() const {
         const double cut = sigma * std::pow(2., 1. / 6);
-        if (kind == SPHERE) return soft == 1 ? cut : 2 * radius;
+        if (kind == SPHERE) return soft == 1 ? cut : soft == 3 ? 3 * sigma : soft == 4 ? sigma : 2 * radius;
         if (kind == SPHEROCYLINDER) return soft == 2 ? length + cut : 2 * radius + length;
         double m = 0;
         for (double r : radii) m = std::max(m, r);
@@ -320,12 +320,36 @@ static inline bool pairOverlap(const ShapeSpec &s, const Vec &p1, const Mat &O1,
     }
 }
 
-// Soft energy with an explicit translation.  soft == 1 follows CentralInteraction.h:58-67 on the centre distance;
-// soft == 2 (WCA on the spherocylinder segment distance) has no reference counterpart and is defined as
-// wca(segmentDistance2(caps)) with the same cap construction as the overlap test.
+// LennardJonesInteraction.cpp:8-13; truncated: zero at and beyond the range 3 sigma (LennardJonesInteraction.h:25),
+// the definition sweeps and totals use, where the reference adds whatever pairs its grid visits
+static inline double ljEnergy(double eps, double sigma, double d2, bool truncated) {
+    if (truncated && d2 >= std::pow(3 * sigma, 2)) return 0;
+    double x2 = sigma * sigma / d2;
+    double x6 = x2 * x2 * x2;
+    double x12 = x6 * x6;
+    return 4 * eps * (x12 - x6);
+}
+
+// SquareInverseCoreInteraction.cpp:13-15
+static inline double sicEnergy(double eps, double sigma, double d2) {
+    return eps * std::max(0.0, std::pow(sigma, 2) / d2 - 1);
+}
+
+// CentralInteraction::calculateEnergyForDistance2 of the sphere interactions (soft 1, 3, 4)
+static inline double centralEnergy(const ShapeSpec &s, double d2, bool truncated) {
+    if (s.soft == 1) return wcaEnergy(s.eps, s.sigma, d2);
+    if (s.soft == 3) return ljEnergy(s.eps, s.sigma, d2, truncated);
+    if (s.soft == 4) return sicEnergy(s.eps, s.sigma, d2);
+    return 0;
+}
+
+// Soft energy with an explicit translation.  soft 1, 3, 4 follow CentralInteraction.h:58-67 on the centre distance
+// (Lennard-Jones truncated at its range unless `truncated` is false); soft == 2 (WCA on the spherocylinder segment
+// distance) has no reference counterpart and is defined as wca(segmentDistance2(caps)) with the same cap
+// construction as the overlap test.
 static inline double pairEnergy(const ShapeSpec &s, const Vec &p1, const Mat &O1, const Vec &p2, const Mat &O2,
-                                const Vec &T) {
-    if (s.soft == 1) return wcaEnergy(s.eps, s.sigma, hcDistance2(p1, p2, T));
+                                const Vec &T, bool truncated = true) {
+    if (s.soft == 1 || s.soft == 3 || s.soft == 4) return centralEnergy(s, hcDistance2(p1, p2, T), truncated);
     if (s.soft == 2) {
         Vec p2bc = add(p2, T);
         double l = s.length;
@@ -475,7 +499,7 @@ static std::uint64_t countTotal(const Snapshot &S, bool earlyExit, double *energ
                         // and CentralInteraction::calculateEnergyBetween (CentralInteraction.h:58-67)
                         double d2 = pbcDistance2(S.box, S.pos[i], S.pos[j]);
                         if (sp.hasHard() && d2 < std::pow(2 * sp.radius, 2)) count++;
-                        if (energyOut && sp.soft == 1) energy += wcaEnergy(sp.eps, sp.sigma, d2);
+                        if (energyOut && sp.soft) energy += centralEnergy(sp, d2, true);
                     } else {
                         Vec T = pbcTranslation(S.box, S.pos[i], S.pos[j]);
                         if (pairOverlap(sp, S.pos[i], S.rot[i], 0, S.pos[j], S.rot[j], 0, T)) count++;
@@ -627,7 +651,7 @@ int orc_pair_energies(int kind, double radius, double length, int soft, double e
         std::memcpy(p1.v, q, 24); std::memcpy(O1.m, q + 3, 72);
         std::memcpy(p2.v, q + 12, 24); std::memcpy(O2.m, q + 15, 72);
         std::memcpy(T.v, q + 24, 24);
-        out[i] = pairEnergy(s, p1, O1, p2, O2, T);
+        out[i] = pairEnergy(s, p1, O1, p2, O2, T, false);
     }
     return 0;
 }

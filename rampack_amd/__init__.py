@@ -29,6 +29,8 @@ MAX_CENTRES = 16
 SOFT_NONE = 0
 SOFT_WCA = 1
 SOFT_WCA_SEGMENT = 2
+SOFT_LJ = 3
+SOFT_SIC = 4
 PREC_FP64 = 0
 BOX_NONE = 0
 BOX_DELTA_TRICLINIC = 1
@@ -226,8 +228,10 @@ def _dptr(a: np.ndarray):
 class Shape:
     """ShapeTraits descriptor: 'sphere' (radius), 'spherocylinder' (radius, length between cap centres) or
     'polysphere' (hard beads: body-frame `centres`, n x 3, and `radii`, n; PolysphereTraits).
-    soft: None (hard core), 'wca' (a sphere with RepulsiveLennardJonesInteraction(epsilon, sigma)) or 'wca_segment'
-    (a spherocylinder with WCA on the distance between the axis segments)."""
+    soft: None (hard core), 'wca' (a sphere with RepulsiveLennardJonesInteraction(epsilon, sigma)), 'lj' (a sphere with
+    LennardJonesInteraction, truncated at its range 3 sigma in sweeps and total energies), 'sic' (a sphere with
+    SquareInverseCoreInteraction) or 'wca_segment' (a spherocylinder with WCA on the distance between the axis
+    segments)."""
     kind: str
     radius: float = 0.5
     length: float = 0.0
@@ -239,7 +243,7 @@ class Shape:
 
     def desc(self) -> ShapeDesc:
         kinds = {"sphere": SHAPE_SPHERE, "spherocylinder": SHAPE_SPHEROCYLINDER, "polysphere": SHAPE_POLYSPHERE}
-        softs = {None: SOFT_NONE, "wca": SOFT_WCA, "wca_segment": SOFT_WCA_SEGMENT}
+        softs = {None: SOFT_NONE, "wca": SOFT_WCA, "wca_segment": SOFT_WCA_SEGMENT, "lj": SOFT_LJ, "sic": SOFT_SIC}
         if self.kind not in kinds:
             raise ValueError(f"unknown shape {self.kind}")
         if self.soft not in softs:
@@ -273,6 +277,10 @@ class Shape:
         cut = self.sigma * 2 ** (1 / 6)
         if self.soft == "wca":
             return cut
+        if self.soft == "lj":
+            return 3 * self.sigma
+        if self.soft == "sic":
+            return self.sigma
         if self.soft == "wca_segment":
             return self.length + cut
         if self.kind == "polysphere":

@@ -202,6 +202,15 @@ PairConsts makeConsts(const rp_shape_desc &s) {
         // segment distance >= centre distance - L: beyond (L + cut) the energy is 0; the margin keeps every pair
         // whose rounded segment distance could still fall below the cut-off
         k.softRange2 = s.soft == 2 ? std::pow(s.length + cut, 2) * (1 + 1e-9) : k.cut2;
+        if (s.soft == RP_SOFT_LJ) {
+            // LennardJonesInteraction.cpp:9 (sigma * sigma / d2); truncated at the range 3 sigma
+            k.cut2 = std::pow(3 * s.sigma, 2);
+            k.softRange2 = k.cut2;
+        } else if (s.soft == RP_SOFT_SIC) {
+            k.sigma2 = std::pow(s.sigma, 2);                // SquareInverseCoreInteraction.cpp:14
+            k.cut2 = k.sigma2;                              // zero beyond sigma
+            k.softRange2 = k.cut2;
+        }
     }
     return k;
 }
@@ -210,6 +219,8 @@ PairConsts makeConsts(const rp_shape_desc &s) {
 // (RepulsiveLennardJonesInteraction.h:26); a WCA spherocylinder L + sigma 2^(1/6)
 double shapeRange(const rp_shape_desc &s) {
     if (s.soft == RP_SOFT_WCA) return wcaCut(s);
+    if (s.soft == RP_SOFT_LJ) return 3 * s.sigma;   // LennardJonesInteraction.h:25
+    if (s.soft == RP_SOFT_SIC) return s.sigma;      // SquareInverseCoreInteraction.h:22
     if (s.soft == RP_SOFT_WCA_SEGMENT) return s.length + wcaCut(s);
     if (s.kind == RP_SHAPE_POLYSPHERE) {
         // PolysphereTraits.cpp:116-121: twice the largest bead radius
@@ -250,11 +261,14 @@ int validateShape(const rp_shape_desc &s) {
         return fail(RP_ERR_UNSUPPORTED, "unsupported shape kind");
     if (!(s.radius > 0)) return fail(RP_ERR_PRECONDITION, "radius must be > 0");
     if (s.soft != RP_SOFT_NONE) {
-        if (!((s.soft == RP_SOFT_WCA && s.kind == RP_SHAPE_SPHERE) ||
+        const bool sphereSoft = s.soft == RP_SOFT_WCA || s.soft == RP_SOFT_LJ || s.soft == RP_SOFT_SIC;
+        if (!((sphereSoft && s.kind == RP_SHAPE_SPHERE) ||
               (s.soft == RP_SOFT_WCA_SEGMENT && s.kind == RP_SHAPE_SPHEROCYLINDER)))
             return fail(RP_ERR_UNSUPPORTED, "soft interaction not available for this shape");
-        // RepulsiveLennardJonesInteraction.cpp:21-22
-        if (!(s.epsilon > 0)) return fail(RP_ERR_PRECONDITION, "epsilon must be > 0");
+        // RepulsiveLennardJonesInteraction.cpp:21-22, LennardJonesInteraction.cpp:16-17; the square-inverse core
+        // only needs a nonzero epsilon (SquareInverseCoreInteraction.cpp:8-9)
+        if (s.soft == RP_SOFT_SIC ? !(s.epsilon != 0 && std::isfinite(s.epsilon)) : !(s.epsilon > 0))
+            return fail(RP_ERR_PRECONDITION, s.soft == RP_SOFT_SIC ? "epsilon must be nonzero" : "epsilon must be > 0");
         if (!(s.sigma > 0)) return fail(RP_ERR_PRECONDITION, "sigma must be > 0");
     }
     if (s.kind == RP_SHAPE_SPHEROCYLINDER && !(s.length >= 0))
@@ -1604,12 +1618,12 @@ int rp_total_energy(rp_ctx *c, double *out) {
 
 int rp_pair_energies(const rp_shape_desc *shape, size_t n, const double *records, double *energies) {
     if (!shape || (n && (!records || !energies))) return fail(RP_ERR_PRECONDITION, "null argument");
-    if (!((shape->soft == RP_SOFT_WCA && shape->kind == RP_SHAPE_SPHERE) ||
-          (shape->soft == RP_SOFT_WCA_SEGMENT && shape->kind == RP_SHAPE_SPHEROCYLINDER)))
-        return fail(RP_ERR_UNSUPPORTED, "no soft interaction for this shape descriptor");
-    if (!(shape->epsilon > 0) || !(shape->sigma > 0)) return fail(RP_ERR_PRECONDITION, "epsilon, sigma must be > 0");
+    if (shape->soft == RP_SOFT_NONE) return fail(RP_ERR_UNSUPPORTED, "no soft interaction for this shape descriptor");
+    if (int v = validateShape(*shape)) return v;
     if (n == 0) return RP_OK;
     PairConsts k = makeConsts(*shape);
+    // Interaction::calculateEnergyBetween for one pair: the Lennard-Jones formula is not truncated
+    if (shape->soft == RP_SOFT_LJ) k.cut2 = std::numeric_limits<double>::infinity();
     double *dRec = nullptr, *dOut = nullptr;
     RP_HIP(hipMalloc(&dRec, sizeof(double) * 27 * n));
     if (hipMalloc(&dOut, sizeof(double) * n) != hipSuccess) {

@@ -159,7 +159,7 @@ RP_HD double segment_distance2(const V3 &s11, const V3 &u, double a, const V3 &s
 // (SphereTraits.cpp:64, SpherocylinderTraits.cpp:34-43).
 struct PairConsts {
     int kind;            // 0 sphere, 1 spherocylinder, 2 polysphere
-    int soft;            // 0 hard core; 1 WCA on the centre distance; 2 WCA on the segment distance
+    int soft;            // 0 hard core; 1 WCA on the centre distance; 2 WCA on the segment distance; 3 LJ; 4 SIC
     double contact2;     // sphere: pow(2r, 2); spherocylinder: 4*r*r
     double range2;       // spherocylinder: pow(2r + L, 2)
     double halfLen;      // spherocylinder: 0.5*L (cap offset, SpherocylinderTraits.cpp:24-26)
@@ -213,15 +213,32 @@ RP_HD double wca_energy(double eps, double sigma2, double cut2, double d2) {
     return 4 * eps * (x12 - x6) + eps;
 }
 
+// LennardJonesInteraction.cpp:8-13, zero at and beyond cut2 (infinite cut2: the reference's formula for any d)
+RP_HD double lj_energy(double eps, double sigma2, double cut2, double d2) {
+    if (d2 >= cut2) return 0;
+    double x2 = sigma2 / d2;
+    double x6 = x2 * x2 * x2;
+    double x12 = x6 * x6;
+    return 4 * eps * (x12 - x6);
+}
+
+// SquareInverseCoreInteraction.cpp:13-15 (sigma2 = pow(sigma, 2))
+RP_HD double sic_energy(double eps, double sigma2, double d2) {
+    const double v = sigma2 / d2 - 1;
+    return eps * (v > 0.0 ? v : 0.0);
+}
+
 // Soft pair energy of the first particle f against pos2 + T (HardcodedTranslation).  soft 1: WCA of the centre
-// distance (CentralInteraction.h:58-67); soft 2: WCA of the squared distance of the two segments, the caps built as
-// in the overlap test.  Beyond softRange2 the segments cannot be within the cut-off (segment distance >= centre
+// distance (CentralInteraction.h:58-67); 3: Lennard-Jones and 4: square-inverse-core of the centre distance; soft 2:
+// WCA of the squared distance of the two segments, the caps built as in the overlap test.  Beyond softRange2 the segments cannot be within the cut-off (segment distance >= centre
 // distance - L), so the energy is 0 without the segment evaluation.
 RP_HD double soft_energy(const PairConsts &k, const SCFirst &f, const V3 &pos2, const V3 &axis2, const V3 &T) {
     V3 p2 = add(pos2, T);
     V3 d = sub(p2, f.pos);
     double d2 = dot(d, d);
     if (k.soft == 1) return wca_energy(k.eps, k.sigma2, k.cut2, d2);
+    if (k.soft == 3) return lj_energy(k.eps, k.sigma2, k.cut2, d2);
+    if (k.soft == 4) return sic_energy(k.eps, k.sigma2, d2);
     if (d2 >= k.softRange2) return 0;
     V3 c2a = add(p2, scal(axis2, -k.halfLen));
     V3 c2b = add(p2, scal(axis2, k.halfLen));

@@ -123,3 +123,41 @@ def test_soft_shape_validation():
         ra.GpuPacking(ra.Shape("sphere", 0.5, soft="wca", sigma=0.0), box, pos, rot)
     with pytest.raises(ra.RampackError):
         ra.pair_decisions(WCA_SPHERE, np.zeros((1, 27)))
+
+
+@pytest.mark.parametrize("soft,name,eps,sigma,seed", [(3, "lj", 1.0, 1.0, 41), (3, "lj", 0.5, 1.4, 42),
+                                                       (4, "sic", 2.0, 1.0, 43), (4, "sic", -1.5, 1.2, 44)])
+def test_lj_and_sic_pair_energies_are_bit_identical_to_oracle(soft, name, eps, sigma, seed):
+    """Interaction::calculateEnergyBetween per pair (Lennard-Jones untruncated), centre distances 0.5-3.5 sigma."""
+    rec = corpus.sphere_contact_corpus(50000, seed, radius=sigma, delta=1.5 * sigma)
+    shape = ra.Shape("sphere", radius=0.5, soft=name, epsilon=eps, sigma=sigma)
+    gpu = ra.pair_energies(shape, rec)
+    cpu = oracle_lib.oracle().pair_energies(0, 0.5, 0.0, soft, eps, sigma, rec)
+    assert np.array_equal(gpu, cpu)
+    assert (gpu != 0).sum() > 5000
+
+
+@pytest.mark.parametrize("soft,name,eps,sigma,temperature", [(3, "lj", 1.0, 1.0, 1.5), (4, "sic", 1.0, 1.0, 1.0),
+                                                              (4, "sic", -0.5, 1.0, 2.0)])
+def test_lj_and_sic_totals_and_sweep_bookkeeping(soft, name, eps, sigma, temperature):
+    """Total energy (truncated at the range) equals the oracle's on a jittered lattice and, after NVT Metropolis
+    sweeps, on the downloaded state; same seed, same trajectory."""
+    shape = ra.Shape("sphere", radius=0.5, soft=name, epsilon=eps, sigma=sigma)
+    box, pos, rot = corpus.jittered_lattice(12, 14.0 / 12, 0.2, 20 + soft)
+    pk = ra.GpuPacking(shape, box, pos, rot, seed=3)
+    _, expected = oracle_lib.oracle().snapshot_stats(0, 0.5, 0.0, box, pos, rot, soft=soft, eps=eps, sigma=sigma)
+    assert pk.total_energy() == pytest.approx(expected, rel=1e-11, abs=1e-9)
+    out = []
+    for _ in range(2):
+        pk = ra.GpuPacking(shape, box, pos, rot, seed=9)
+        pk.set_environment(temperature, 1.0, 0.15, None)
+        c = pk.run_cycles(60, adapt_steps=True)
+        assert 0 < c["moves_accepted"] < c["moves_attempted"]
+        b, p, r = pk.download()
+        e = pk.total_energy()
+        _, expected = oracle_lib.oracle().snapshot_stats(0, 0.5, 0.0, b, p, r, soft=soft, eps=eps, sigma=sigma)
+        print(f"{name} eps {eps}: E/N {e / len(p)}", flush=True)
+        assert np.isfinite(e) and e == pytest.approx(expected, rel=1e-10, abs=1e-8)
+        out.append(p)
+        pk.close()
+    assert np.array_equal(out[0], out[1])

@@ -167,3 +167,66 @@ def test_oracle_total_energy_matches_reference_fixture(golden):
         _, e = oracle_lib.oracle().snapshot_stats(0, 0.5, 0.0, box, pos, rot, soft=1, eps=1.0, sigma=1.0)
         assert e == pytest.approx(s["energy"], rel=1e-12)
         assert e > 0
+
+
+# Lennard-Jones (soft 3) and square-inverse core (soft 4) on spheres
+@pytest.mark.parametrize("soft,eps,sigma,seed", [(3, 1.0, 1.0, 4), (3, 0.5, 1.4, 5), (4, 2.0, 1.0, 6),
+                                                  (4, -1.5, 1.2, 7)])
+def test_lj_and_sic_pair_energies_match_reference_bit_for_bit(soft, eps, sigma, seed):
+    """Per pair, Interaction::calculateEnergyBetween of the reference build (LennardJonesInteraction,
+    SquareInverseCoreInteraction on a sphere) at centre distances from 0.5 to 3.5 sigma, untruncated."""
+    ref = need_ref()
+    rec = corpus.sphere_contact_corpus(20000, seed, radius=sigma, delta=1.5 * sigma)
+    e_orc = oracle_lib.oracle().pair_energies(0, 0.5, 0.0, soft, eps, sigma, rec)
+    e_ref = ref.pair_energies(0, 0.5, 0.0, soft, eps, sigma, rec)
+    assert np.array_equal(e_orc, e_ref)
+    assert (e_orc != 0).sum() > 2500  # SIC: nonzero within sigma only, a sixth of the corpus
+    if soft == 3:
+        d = np.linalg.norm(rec[:, 12:15] + rec[:, 24:27] - rec[:, 0:3], axis=1)
+        assert ((d > 3 * sigma) & (e_orc != 0)).sum() > 1000  # the pair formula has no cut-off
+
+
+def test_lj_and_sic_closed_forms():
+    """LJ: U(sigma) = 0 and the minimum U(2^(1/6) sigma) = -eps; SIC: eps (sigma^2 / d^2 - 1) inside sigma (eps at
+    d = sigma / sqrt 2, attractive for eps < 0), 0 outside."""
+    o = np.eye(3)
+    orc = oracle_lib.oracle()
+    e = lambda soft, eps, sigma, d: orc.pair_energies(0, 0.5, 0, soft, eps, sigma,
+                                                      record([1, 1, 1], o, [1 + d, 1, 1], o))[0]
+    assert e(3, 2.0, 1.5, 1.5) == 0.0
+    assert e(3, 2.0, 1.5, 1.5 * CUT) == pytest.approx(-2.0, rel=1e-13)
+    assert e(4, 3.0, 2.0, 2.0 / math.sqrt(2)) == pytest.approx(3.0, rel=1e-14)
+    assert e(4, -3.0, 2.0, 2.0 / math.sqrt(2)) == pytest.approx(-3.0, rel=1e-14)
+    assert e(4, 3.0, 2.0, 2.0) == 0.0 and e(4, 3.0, 2.0, 2.5) == 0.0
+
+
+def brute_central_total(box, pos, soft, eps, sigma):
+    """All pairs over all 27 images, each pair's energy truncated at the interaction range (LJ 3 sigma, SIC
+    sigma), valid while at most one image of a pair is within range."""
+    rng2 = (3 * sigma) ** 2 if soft == 3 else sigma ** 2
+    tr = np.array([box @ np.array([i, j, k]) for i in (-1, 0, 1) for j in (-1, 0, 1) for k in (-1, 0, 1)], float)
+    total = 0.0
+    for i in range(len(pos)):
+        for t in tr:
+            d2 = ((pos[i + 1:] + t - pos[i]) ** 2).sum(1)
+            d2 = d2[d2 < rng2]
+            if soft == 3:
+                x6 = (sigma * sigma / d2) ** 3
+                total += float((4 * eps * (x6 * x6 - x6)).sum())
+            else:
+                total += float((eps * np.maximum(0.0, sigma ** 2 / d2 - 1)).sum())
+    return total
+
+
+@pytest.mark.parametrize("soft,eps,sigma", [(3, 1.0, 1.0), (4, 1.5, 1.0), (4, -0.8, 1.1)])
+def test_lj_and_sic_total_energies(soft, eps, sigma):
+    """Total energies: the oracle (truncated at the range) equals an all-images brute force; for the square-inverse
+    core, which vanishes beyond its range, also the reference build's getTotalEnergy."""
+    box = np.array([[14.0, 0.5, -0.3], [0.0, 13.5, 0.4], [0.0, 0.0, 14.5]])
+    box_, pos, rot = corpus.jittered_lattice(12, 14.0 / 12, 0.2, 20 + soft)
+    _, e_orc = oracle_lib.oracle().snapshot_stats(0, 0.5, 0.0, box_, pos, rot, soft=soft, eps=eps, sigma=sigma)
+    assert e_orc == pytest.approx(brute_central_total(box_, pos, soft, eps, sigma), rel=1e-9)
+    assert e_orc != 0
+    if soft == 4 and oracle_lib.ref_available():
+        _, e_ref = oracle_lib.ref().snapshot_stats(0, 0.5, 0.0, box_, pos, rot, soft=soft, eps=eps, sigma=sigma)
+        assert e_orc == pytest.approx(e_ref, rel=1e-12)
