@@ -65,8 +65,10 @@ extern "C" {
  * LogScalingFactorSampler.cpp:8-19). */
 #define RP_BOX_NONE 0
 #define RP_BOX_DELTA_TRICLINIC 1
-#define RP_BOX_LOG_ISOTROPIC 2
+#define RP_BOX_LOG_ISOTROPIC 2  /* log scaling; anisotropic with rp_box_desc.groups */
 #define RP_BOX_DELTA_V 3        /* TriclinicAdapter + DeltaVolumeScaler (src/core/volume_scalers/DeltaVolumeScaler.cpp:10-19) */
+#define RP_BOX_LINEAR 4         /* TriclinicAdapter + AnisotropicVolumeScaler + LinearScalingFactorSampler
+                                 * (src/core/volume_scalers/LinearScalingFactorSampler.cpp:8-22) */
 
 typedef struct rp_ctx rp_ctx;
 
@@ -99,10 +101,18 @@ typedef struct rp_move_desc {
     int kind;              /* RP_MOVE_*; the translation sampler ignores rot_step, the rotation sampler trans_step */
 } rp_move_desc;
 
-/* TriclinicBoxScaler descriptor (src/core/TriclinicBoxScaler.h:15-55). */
+/* TriclinicBoxScaler descriptor (src/core/TriclinicBoxScaler.h:15-55).  The fields after `step` default to zero:
+ * an isotropic log / linear scaler and a delta_triclinic scaler that moves all nine entries. */
 typedef struct rp_box_desc {
     int kind;      /* RP_BOX_* */
     double step;   /* initial step size (> 0) */
+    int groups[3]; /* log / linear: scaling group (0-2) of the box sides x, y, z -- AnisotropicVolumeScaler's list
+                    * of ScalingDirections (AnisotropicVolumeScaler.cpp:13-31); sides with one group share a factor,
+                    * all zero is isotropic */
+    int independent; /* log / linear with several groups: 1 = one group per move, drawn uniformly; 0 = a factor for
+                      * every group on each move (AnisotropicVolumeScaler.cpp:16-27) */
+    int one_side;    /* delta_triclinic: 1 = one box side per move (scaleTogether = false,
+                      * TriclinicDeltaScaler.cpp:17-23); 0 = all nine entries */
 } rp_box_desc;
 
 /* Counters and step sizes after a call of rp_run_cycles (cumulative over the context's lifetime). */

@@ -22,6 +22,7 @@ LIB_PATH = os.path.join(_HERE, "lib", "librampack_amd.so")
 SHAPE_SPHERE = 0
 SHAPE_SPHEROCYLINDER = 1
 SHAPE_POLYSPHERE = 2
+BOX_LINEAR = 4
 MOVE_ROTOTRANSLATION = 0
 MOVE_TRANSLATION = 1
 MOVE_ROTATION = 2
@@ -70,7 +71,8 @@ class MoveDesc(ctypes.Structure):
 
 
 class BoxDesc(ctypes.Structure):
-    _fields_ = [("kind", ctypes.c_int), ("step", ctypes.c_double)]
+    _fields_ = [("kind", ctypes.c_int), ("step", ctypes.c_double), ("groups", ctypes.c_int * 3),
+                ("independent", ctypes.c_int), ("one_side", ctypes.c_int)]
 
 
 class Counters(ctypes.Structure):
@@ -370,12 +372,40 @@ class GpuPacking:
     def set_environment(self, temperature: float, pressure: float, trans_step: float,
                         rot_step: Optional[float] = None, max_trans_step: float = 0.0,
                         box_move: Optional[tuple] = None):
-        """box_move: None (NVT), ('delta_triclinic', step), ('log', step) (isotropic) or ('delta_v', step)."""
+        """box_move: None (NVT) or (kind, step[, spec]): 'delta_triclinic' (spec 'one_side' moves one box side per
+        move, TriclinicDeltaScaler's scaleTogether = false), 'log' or 'linear' (spec as the reference's
+        AnisotropicVolumeScaler: 'isotropic' (default), 'anisotropic <groups>' or 'independent <groups>' with groups
+        such as 'x y z' or 'xy z'), or 'delta_v'."""
         m = MoveDesc(float(trans_step), float(rot_step) if rot_step else 0.0, float(max_trans_step))
         bd = None
         if box_move is not None:
-            kinds = {"delta_triclinic": BOX_DELTA_TRICLINIC, "log": BOX_LOG_ISOTROPIC, "delta_v": BOX_DELTA_V}
+            kinds = {"delta_triclinic": BOX_DELTA_TRICLINIC, "log": BOX_LOG_ISOTROPIC, "delta_v": BOX_DELTA_V,
+                     "linear": BOX_LINEAR}
             bd = BoxDesc(kinds[box_move[0]], float(box_move[1]))
+            spec = box_move[2] if len(box_move) > 2 else None
+            if spec and box_move[0] == "delta_triclinic":
+                if spec not in ("one_side", "scale_together"):
+                    raise ValueError(f"unknown delta_triclinic spec {spec}")
+                bd.one_side = 1 if spec == "one_side" else 0
+            elif spec and box_move[0] in ("log", "linear"):
+                words = spec.split()
+                if words[0] == "isotropic" and len(words) == 1:
+                    pass
+                elif words[0] in ("anisotropic", "independent") and len(words) > 1:
+                    bd.independent = 1 if words[0] == "independent" else 0
+                    seen = set()
+                    for g, axes in enumerate(words[1:]):
+                        for a in axes:
+                            if a not in "xyz" or a in seen:
+                                raise ValueError(f"bad scaling spec {spec}")
+                            seen.add(a)
+                            bd.groups["xyz".index(a)] = g
+                    if seen != set("xyz"):
+                        raise ValueError(f"scaling spec {spec} must name x, y and z once each")
+                else:
+                    raise ValueError(f"bad scaling spec {spec}")
+            elif spec:
+                raise ValueError(f"{box_move[0]} takes no spec")
         _check(_lib.rp_set_environment(self._ctx, float(temperature), float(pressure), ctypes.byref(m),
                                        ctypes.byref(bd) if bd is not None else None))
 

@@ -119,6 +119,8 @@ struct rp_ctx {
     uint64_t typeAccBase[RP_MAX_MOVE_SAMPLERS] = {}, typeAttBase[RP_MAX_MOVE_SAMPLERS] = {};
     int boxKind = RP_BOX_NONE;
     double boxStep = 0;
+    int boxGroups[3] = {0, 0, 0};  // scaling group of each box side (log / linear)
+    int boxIndependent = 0, boxOneSide = 0;
 
     // grid the current arrays are sorted by, and whether it is ready for the next cycle
     Grid grid{};
@@ -606,10 +608,41 @@ int boxMove(rp_ctx *c, bool &accepted) {
     double nd[9];
     std::memcpy(nd, c->box.dims, sizeof nd);
     if (c->boxKind == RP_BOX_DELTA_TRICLINIC) {
-        for (int i = 0; i < 9; i++) nd[i] += philox_uniform(c->hostRng, -c->boxStep, c->boxStep);
-    } else if (c->boxKind == RP_BOX_LOG_ISOTROPIC) {
-        double f = std::exp(philox_uniform(c->hostRng, -c->boxStep, c->boxStep));
-        for (int i = 0; i < 9; i++) nd[i] = nd[i] * f;
+        if (c->boxOneSide) {
+            // TriclinicDeltaScaler.cpp:17-23: one side (column) drawn uniformly
+            const int side = (int)philox_below(c->hostRng, 3);
+            for (int i = 0; i < 3; i++) nd[3 * i + side] += philox_uniform(c->hostRng, -c->boxStep, c->boxStep);
+        } else {
+            for (int i = 0; i < 9; i++) nd[i] += philox_uniform(c->hostRng, -c->boxStep, c->boxStep);
+        }
+    } else if (c->boxKind == RP_BOX_LOG_ISOTROPIC || c->boxKind == RP_BOX_LINEAR) {
+        // TriclinicAdapter::updateBox (TriclinicAdapter.h:27-33): side c (column c) times factor c, the factors from
+        // AnisotropicVolumeScaler::sampleScalingFactors (AnisotropicVolumeScaler.cpp:13-31) with the log sampler
+        // (exp(U), LogScalingFactorSampler.cpp:8-19) or the linear one ((h + U) / h, LinearScalingFactorSampler.cpp)
+        double h[3];
+        c->box.heights(h);
+        int groupList[3], ngroups = 0;
+        for (int a = 0; a < 3; a++) {
+            bool seen = false;
+            for (int g = 0; g < ngroups; g++) seen = seen || groupList[g] == c->boxGroups[a];
+            if (!seen) groupList[ngroups++] = c->boxGroups[a];
+        }
+        double factors[3] = {1, 1, 1};
+        auto sampleGroup = [&](int group) {
+            const double u = philox_uniform(c->hostRng, -c->boxStep, c->boxStep);
+            const double logFactor = std::exp(u);
+            for (int a = 0; a < 3; a++)
+                if (c->boxGroups[a] == group)
+                    factors[a] *= c->boxKind == RP_BOX_LOG_ISOTROPIC ? logFactor : (h[a] + u) / h[a];
+        };
+        if (c->boxIndependent && ngroups > 1) sampleGroup(groupList[philox_below(c->hostRng, (uint32_t)ngroups)]);
+        else
+            for (int g = 0; g < ngroups; g++) sampleGroup(groupList[g]);
+        for (int a = 0; a < 3; a++)
+            if (!(factors[a] > 0)) return fail(RP_ERR_PRECONDITION, "box scaling factor <= 0 (step too large)");
+        for (int r = 0; r < 3; r++)
+            for (int col = 0; col < 3; col++)
+                if (factors[col] != 1) nd[3 * r + col] = nd[3 * r + col] * factors[col];
     } else {
         // DeltaVolumeScaler: factor = cbrt((dV + V) / V) with V the product of the heights
         double h[3];
@@ -1069,16 +1102,25 @@ int rp_set_environment(rp_ctx *c, double temperature, double pressure, const rp_
     if (!(temperature > 0)) return fail(RP_ERR_PRECONDITION, "temperature must be > 0");
     const int boxKind = box ? box->kind : RP_BOX_NONE;
     if (boxKind != RP_BOX_NONE && boxKind != RP_BOX_DELTA_TRICLINIC && boxKind != RP_BOX_LOG_ISOTROPIC &&
-        boxKind != RP_BOX_DELTA_V)
+        boxKind != RP_BOX_DELTA_V && boxKind != RP_BOX_LINEAR)
         return fail(RP_ERR_UNSUPPORTED, "unknown box move");
     if (boxKind != RP_BOX_NONE && !(box->step > 0))
         return fail(RP_ERR_PRECONDITION, "box step must be > 0 (TriclinicBoxScaler.cpp:9-11)");
+    if (boxKind != RP_BOX_NONE)
+        for (int a = 0; a < 3; a++)
+            if (box->groups[a] < 0 || box->groups[a] > 2)
+                return fail(RP_ERR_PRECONDITION, "box scaling groups are 0, 1 or 2");
     int r = rp_set_move_samplers(c, move, 1);
     if (r) return r;
     c->temperature = temperature;
     c->pressure = pressure;
     c->boxKind = boxKind;
-    if (boxKind != RP_BOX_NONE) c->boxStep = box->step;
+    if (boxKind != RP_BOX_NONE) {
+        c->boxStep = box->step;
+        for (int a = 0; a < 3; a++) c->boxGroups[a] = box->groups[a];
+        c->boxIndependent = box->independent ? 1 : 0;
+        c->boxOneSide = box->one_side ? 1 : 0;
+    }
     c->envSet = true;
     return RP_OK;
 }

@@ -224,13 +224,19 @@ public:
     [[nodiscard]] virtual rp_box_desc descriptor() const = 0;
 };
 
+// TriclinicDeltaScaler (src/core/volume_scalers/TriclinicDeltaScaler.cpp:7-25): all nine entries, or one box side
+// per move when scaleTogether is false
 class TriclinicDeltaScaler : public TriclinicBoxScaler {
+    bool scaleTogether;
+
 public:
-    TriclinicDeltaScaler(double stepSize, bool scaleTogether) : TriclinicBoxScaler(stepSize) {
-        if (!scaleTogether)
-            throw PreconditionException("TriclinicDeltaScaler: only scale_together = True runs on the device");
+    TriclinicDeltaScaler(double stepSize, bool scaleTogether)
+            : TriclinicBoxScaler(stepSize), scaleTogether{scaleTogether} {}
+    [[nodiscard]] rp_box_desc descriptor() const override {
+        rp_box_desc d{RP_BOX_DELTA_TRICLINIC, this->stepSize};
+        d.one_side = this->scaleTogether ? 0 : 1;
+        return d;
     }
-    [[nodiscard]] rp_box_desc descriptor() const override { return {RP_BOX_DELTA_TRICLINIC, this->stepSize}; }
 };
 
 // VolumeScaler kinds accepted by TriclinicAdapter.
@@ -238,6 +244,29 @@ class VolumeScaler {
 public:
     virtual ~VolumeScaler() = default;
     [[nodiscard]] virtual int kind() const = 0;
+    virtual void describe(rp_box_desc &) const {}
+};
+
+// AnisotropicVolumeScaler (src/core/volume_scalers/AnisotropicVolumeScaler.cpp:13-31) with the log or the linear
+// factor sampler: groups[a] is the scaling group of box side a (sides of one group share a factor); independent:
+// one group per move instead of every group
+class AnisotropicVolumeScaler : public VolumeScaler {
+    int sampler;
+    std::array<int, 3> groups;
+    bool independent;
+
+public:
+    enum FactorSampler { LOG = RP_BOX_LOG_ISOTROPIC, LINEAR = RP_BOX_LINEAR };
+    AnisotropicVolumeScaler(FactorSampler sampler, std::array<int, 3> groups, bool independent)
+            : sampler{sampler}, groups{groups}, independent{independent} {
+        for (int g : groups)
+            if (g < 0 || g > 2) throw PreconditionException("AnisotropicVolumeScaler: groups are 0, 1 or 2");
+    }
+    [[nodiscard]] int kind() const override { return this->sampler; }
+    void describe(rp_box_desc &d) const override {
+        for (int a = 0; a < 3; a++) d.groups[a] = this->groups[a];
+        d.independent = this->independent ? 1 : 0;
+    }
 };
 class LogIsotropicVolumeScaler : public VolumeScaler {  // log(spec="isotropic"), AnisotropicVolumeScaler + LogScaling
 public:
@@ -254,7 +283,11 @@ class TriclinicAdapter : public TriclinicBoxScaler {
 public:
     TriclinicAdapter(std::unique_ptr<VolumeScaler> volumeScaler, double stepSize)
             : TriclinicBoxScaler(stepSize), volumeScaler{std::move(volumeScaler)} {}
-    [[nodiscard]] rp_box_desc descriptor() const override { return {this->volumeScaler->kind(), this->stepSize}; }
+    [[nodiscard]] rp_box_desc descriptor() const override {
+        rp_box_desc d{this->volumeScaler->kind(), this->stepSize};
+        this->volumeScaler->describe(d);
+        return d;
+    }
 };
 
 // ---------------------------------------------------------------------------------------------- packing

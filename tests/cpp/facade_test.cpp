@@ -33,7 +33,8 @@ bool throws(F f) {
 int main() {
     CHECK(throws<PreconditionException>([] { SpherocylinderTraits(-1, 1); }));
     CHECK(throws<PreconditionException>([] { RototranslationSampler(0.5, std::nullopt, 0.1); }));
-    CHECK(throws<PreconditionException>([] { TriclinicDeltaScaler(0.1, false); }));
+    CHECK(throws<PreconditionException>([] { AnisotropicVolumeScaler(AnisotropicVolumeScaler::LOG, {0, 3, 0}, false); }));
+    CHECK(TriclinicDeltaScaler(0.1, false).descriptor().one_side == 1);
 
     // 1000 hard spheres of radius 0.5 on a 10^3 sc lattice at packing fraction 0.36, NpT p = 5 with log moves
     SphereTraits traits(0.5);

@@ -355,3 +355,40 @@ def test_overlap_relaxation_reaches_a_legal_packing():
     assert overlaps_by_oracle(SC, b, p, r) == 0
     pk.run_cycles(10)
     pk.close()
+
+
+def test_anisotropic_linear_and_one_side_box_moves():
+    """AnisotropicVolumeScaler (AnisotropicVolumeScaler.cpp:13-31) through TriclinicAdapter: 'independent xy z'
+    scales the x and y sides by one factor (their lengths stay equal bit for bit) and z on its own, 'anisotropic x y
+    z' scales the three sides apart; a diagonal box stays diagonal (sides are only scaled), and the isotropic linear
+    sampler keeps a cube a cube.  TriclinicDeltaScaler with scaleTogether false shears the box.  Every run stays
+    overlap-free."""
+    box, pos, rot = lattice.hard_sphere_lattice(8, 0.30)
+    runs = [("log", "independent xy z"), ("log", "anisotropic x y z"), ("linear", "isotropic"),
+            ("delta_triclinic", "one_side")]
+    for i, (kind, spec) in enumerate(runs):
+        pk = ra.GpuPacking(HS, box, pos, rot, seed=60 + i)
+        # steps of about 0.1 % of the box: a 1 % compression of this fluid almost always creates a contact
+        step = {"log": 0.001, "linear": 0.01, "delta_triclinic": 0.01}[kind]
+        pk.set_environment(1.0, 5.0, 0.1, None, box_move=(kind, step, spec))
+        c = pk.run_cycles(300, adapt_steps=True)
+        b, p, r = pk.download()
+        print(kind, spec, c["box_accepted"], np.diag(b), flush=True)
+        assert c["box_accepted"] > 10
+        assert overlaps_by_oracle(HS, b, p, r) == 0
+        off = b - np.diag(np.diag(b))
+        if kind == "delta_triclinic":
+            assert np.abs(off).max() > 0
+            continue
+        assert not off.any()
+        d = np.diag(b)
+        if spec == "independent xy z":
+            assert d[0] == d[1] and d[2] != d[0]
+        elif spec == "anisotropic x y z":
+            assert d[0] != d[1] and d[1] != d[2] and d[0] != d[2]
+        else:
+            assert d[0] == d[1] == d[2]
+        pk.close()
+    pk = ra.GpuPacking(HS, box, pos, rot)
+    with pytest.raises(ValueError):
+        pk.set_environment(1.0, 5.0, 0.1, None, box_move=("log", 0.01, "anisotropic xy"))
