@@ -108,7 +108,7 @@ typedef struct rp_box_desc {
     double step;   /* initial step size (> 0) */
     int groups[3]; /* log / linear: scaling group (0-2) of the box sides x, y, z -- AnisotropicVolumeScaler's list
                     * of ScalingDirections (AnisotropicVolumeScaler.cpp:13-31); sides with one group share a factor,
-                    * all zero is isotropic */
+                    * all zero is isotropic; 3: the side is not scaled */
     int independent; /* log / linear with several groups: 1 = one group per move, drawn uniformly; 0 = a factor for
                       * every group on each move (AnisotropicVolumeScaler.cpp:16-27) */
     int one_side;    /* delta_triclinic: 1 = one box side per move (scaleTogether = false,

@@ -375,7 +375,7 @@ class GpuPacking:
         """box_move: None (NVT) or (kind, step[, spec]): 'delta_triclinic' (spec 'one_side' moves one box side per
         move, TriclinicDeltaScaler's scaleTogether = false), 'log' or 'linear' (spec as the reference's
         AnisotropicVolumeScaler: 'isotropic' (default), 'anisotropic <groups>' or 'independent <groups>' with groups
-        such as 'x y z' or 'xy z'), or 'delta_v'."""
+        such as 'x y z' or 'xy z', and '[y]' for a side that is not scaled), or 'delta_v'."""
         m = MoveDesc(float(trans_step), float(rot_step) if rot_step else 0.0, float(max_trans_step))
         bd = None
         if box_move is not None:
@@ -395,11 +395,12 @@ class GpuPacking:
                     bd.independent = 1 if words[0] == "independent" else 0
                     seen = set()
                     for g, axes in enumerate(words[1:]):
-                        for a in axes:
+                        fixed = axes.startswith("[") and axes.endswith("]")  # sides that are not scaled
+                        for a in axes.strip("[]"):
                             if a not in "xyz" or a in seen:
                                 raise ValueError(f"bad scaling spec {spec}")
                             seen.add(a)
-                            bd.groups["xyz".index(a)] = g
+                            bd.groups["xyz".index(a)] = 3 if fixed else g
                     if seen != set("xyz"):
                         raise ValueError(f"scaling spec {spec} must name x, y and z once each")
                 else:

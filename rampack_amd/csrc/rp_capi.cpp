@@ -623,6 +623,7 @@ int boxMove(rp_ctx *c, bool &accepted) {
         c->box.heights(h);
         int groupList[3], ngroups = 0;
         for (int a = 0; a < 3; a++) {
+            if (c->boxGroups[a] == 3) continue;  // a side that is not scaled ("[y]" in the reference's spec)
             bool seen = false;
             for (int g = 0; g < ngroups; g++) seen = seen || groupList[g] == c->boxGroups[a];
             if (!seen) groupList[ngroups++] = c->boxGroups[a];
@@ -1108,8 +1109,8 @@ int rp_set_environment(rp_ctx *c, double temperature, double pressure, const rp_
         return fail(RP_ERR_PRECONDITION, "box step must be > 0 (TriclinicBoxScaler.cpp:9-11)");
     if (boxKind != RP_BOX_NONE)
         for (int a = 0; a < 3; a++)
-            if (box->groups[a] < 0 || box->groups[a] > 2)
-                return fail(RP_ERR_PRECONDITION, "box scaling groups are 0, 1 or 2");
+            if (box->groups[a] < 0 || box->groups[a] > 3)
+                return fail(RP_ERR_PRECONDITION, "box scaling groups are 0, 1, 2 or 3 (not scaled)");
     int r = rp_set_move_samplers(c, move, 1);
     if (r) return r;
     c->temperature = temperature;
