@@ -392,3 +392,38 @@ def test_anisotropic_linear_and_one_side_box_moves():
     pk = ra.GpuPacking(HS, box, pos, rot)
     with pytest.raises(ValueError):
         pk.set_environment(1.0, 5.0, 0.1, None, box_move=("log", 0.01, "anisotropic xy"))
+
+
+def test_spherocylinder_liquid_npt_matches_reference_runs(golden):
+    """NpT hard spherocylinders L/D = 5 at p* = 1 (an isotropic liquid near packing fraction 0.31, 300 rods from an
+    aligned lattice, delta_triclinic box moves) against eight runs of the reference build (its Packing, samplers and
+    scaler under the restated cycle loop): the mean packing fraction and nematic P2 over independent device runs
+    agree with the reference's means within 4 standard errors of the difference, each side's error taken from the
+    spread between its runs."""
+    st = next((s for s in golden.get("statistics", []) if s["name"] == "sc_npt_p1"), None)
+    if st is None:
+        pytest.skip("no reference statistics in the fixture")
+    box, pos, rot = lattice.spherocylinder_lattice(n_side=tuple(st["n_side"]), spacing=tuple(st["spacing"]))
+    phis, p2s = [], []
+    for seed in (1, 2, 3, 4, 5, 6):
+        pk = ra.GpuPacking(SC, box, pos, rot, seed=seed)
+        pk.set_environment(st["temperature"], st["pressure"], st["trans_step"], st["rot_step"],
+                           box_move=("delta_triclinic", st["box_step"]))
+        pk.run_cycles(st["therm"], adapt_steps=True)
+        phi, p2 = [], []
+        for _ in range(st["avg"] // st["every"]):
+            pk.run_cycles(st["every"])
+            o = pk.observables()
+            phi.append(o["packing_fraction"])
+            p2.append(o["nematic_p2"])
+        phis.append(float(np.mean(phi)))
+        p2s.append(float(np.mean(p2)))
+        b, p, r = pk.download()
+        assert overlaps_by_oracle(SC, b, p, r) == 0
+        pk.close()
+    for name, dev, ref in (("phi", phis, st["phi_runs"]), ("P2", p2s, st["p2_runs"])):
+        dev, ref = np.array(dev), np.array(ref)
+        err = np.hypot(dev.std(ddof=1) / np.sqrt(len(dev)), ref.std(ddof=1) / np.sqrt(len(ref)))
+        print(f"{name}: device {dev.mean()} (runs {np.round(dev, 4)}), reference {ref.mean()} "
+              f"(runs {np.round(ref, 4)}), error of the difference {err}", flush=True)
+        assert abs(dev.mean() - ref.mean()) <= 4 * err
