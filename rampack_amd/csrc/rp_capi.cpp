@@ -537,7 +537,7 @@ int countingBoxMove(rp_ctx *c, const HostBox &nb, double exponent, double u, con
     if (r) return r;
     uint64_t after = 0;
     c->boxScans++;
-    r = scanOverlaps(c, g, 0, after);
+    r = c->k.kind == RP_SHAPE_POLYSPHERE ? countBeads(c, c->alt.p, nb, 0, after) : scanOverlaps(c, g, 0, after);
     if (r) return r;
     const long long delta = (long long)after - c->numOverlaps;
     if (delta > 0 || (delta == 0 && !(u <= std::exp(exponent)))) return RP_OK;
@@ -1503,8 +1503,6 @@ int rp_fix_orientations(rp_ctx *c, uint64_t *rejected) {
 
 int rp_set_overlap_counting(rp_ctx *c, int enable) {
     if (!c) return fail(RP_ERR_PRECONDITION, "null context");
-    if (enable && c->k.kind == RP_SHAPE_POLYSPHERE)
-        return fail(RP_ERR_UNSUPPORTED, "overlap counting for polyspheres is not built");
     if (!enable || c->k.soft) {
         c->counting = false;
         c->numOverlaps = 0;
@@ -1592,7 +1590,7 @@ int rp_try_scaling(rp_ctx *c, const double new_box[9], double *dE) {
     if (!c->k.soft) {
         if (c->counting) {
             uint64_t after = 0;
-            r = scanOverlaps(c, g, 0, after);
+            r = c->k.kind == RP_SHAPE_POLYSPHERE ? countBeads(c, c->alt.p, nb, 0, after) : scanOverlaps(c, g, 0, after);
             if (r) return r;
             count = (long long)after;
             if (count < c->numOverlaps) result = -INF;

@@ -1399,6 +1399,26 @@ __device__ __forceinline__ bool poly_hit(const PairConsts &k, const double *sBea
     return false;
 }
 
+// Overlapping bead pairs of particle j (p2 + O_j c_b under translation T) with beads sBeadPos (the moved
+// particle's beads, centred at pos); the count of Packing::countParticleOverlaps over interaction centres
+// (Packing.cpp:381-409, 619-647).  Candidates at least the prefilter distance away contribute none.
+__device__ __forceinline__ int poly_count(const PairConsts &k, const double *sBeads, const double *sBeadPos,
+                                          const V3 &pos, const V3 &p2, const Particles &p, int gj, const V3 &T) {
+    const V3 d = sub(add(p2, T), pos);
+    if (!(dot(d, d) < k.range2)) return 0;
+    double O[9];
+    for (int e = 0; e < 9; e++) O[e] = p.o[e][gj];
+    int count = 0;
+    for (int b = 0; b < k.nc; b++) {
+        const V3 c2 = add(p2, mulv(O, v3(sBeads[4 * b], sBeads[4 * b + 1], sBeads[4 * b + 2])));
+        for (int a = 0; a < k.nc; a++) {
+            const V3 c1 = v3(sBeadPos[3 * a], sBeadPos[3 * a + 1], sBeadPos[3 * a + 2]);
+            count += bead_overlap(sBeads[4 * a + 3], sBeads[4 * b + 3], c1, c2, T) ? 1 : 0;
+        }
+    }
+    return count;
+}
+
 template <int KIND, int WAVES>
 __device__ __forceinline__ void sweep_cell_unstaged(const SweepParams &P, const Stencil &S, double *pt, double *pR, int *pk,
                                     const double *sT, int cell, int own0, int nOwn, int total) {
@@ -1904,6 +1924,13 @@ __device__ __forceinline__ void sweep_cell_soft(const SweepParams &P, const Sten
     int move = 0;
     const int moves = P.moves > 0 ? P.moves : nOwn * P.mult;
     const int pbase = P.moves > 0 ? cell * P.moves : own0 * P.mult;
+    // polyspheres (counting mode): the bead table, the moved particle's bead centres before and after the move
+    __shared__ double sBeads[KIND == 2 ? 4 * RP_MAX_CENTRES : 1], sOld[KIND == 2 ? 3 * RP_MAX_CENTRES : 1],
+        sNew[KIND == 2 ? 3 * RP_MAX_CENTRES : 1];
+    if (KIND == 2) {
+        for (int e = tid; e < 4 * P.k.nc; e += NT) sBeads[e] = P.k.beads[e];
+        __syncthreads();
+    }
     for (int chunk = 0; chunk < moves; chunk += MOVE_CHUNK) {
         const int cn = min(MOVE_CHUNK, moves - chunk);
         load_proposals(P, pbase, chunk, cn, pt, pR, pk);
@@ -1943,6 +1970,16 @@ __device__ __forceinline__ void sweep_cell_soft(const SweepParams &P, const Sten
             const V3 nax = v3(NO[2], NO[5], NO[8]);
             const int nwx = nw & 1, nwy = (nw >> 1) & 1, nwz = (nw >> 2) & 1;
             double eFinal = 0, eInitial = 0;
+            if (KIND == 2) {
+                // bead centres pos + O c before and after the move (PolysphereTraits.cpp:93-95)
+                if (tid < P.k.nc) {
+                    const V3 c = v3(sBeads[4 * tid], sBeads[4 * tid + 1], sBeads[4 * tid + 2]);
+                    const V3 cn2 = add(npos, mulv(NO, c)), co = add(opos, mulv(O, c));
+                    sNew[3 * tid] = cn2.x; sNew[3 * tid + 1] = cn2.y; sNew[3 * tid + 2] = cn2.z;
+                    sOld[3 * tid] = co.x; sOld[3 * tid + 1] = co.y; sOld[3 * tid + 2] = co.z;
+                }
+                __syncthreads();
+            }
             if (inside) {
                 SCFirst fN = make_first<KIND>(P.k, npos, nax);
                 SCFirst fO = make_first<KIND>(P.k, opos, oax);
@@ -1956,7 +1993,12 @@ __device__ __forceinline__ void sweep_cell_soft(const SweepParams &P, const Sten
                         TN = image_translation(sT, code, nwx, nwy, nwz);
                         TO = image_translation(sT, code, ow[0], ow[1], ow[2]);
                     }
-                    if (P.counting) {
+                    if (KIND == 2) {
+                        const int q = stencil_cell(S.start, S.ncells, j);
+                        const int gj = S.base[q] + (j - S.start[q]);
+                        eFinal += (double)poly_count(P.k, sBeads, sNew, npos, p2, P.p, gj, TN);
+                        eInitial += (double)poly_count(P.k, sBeads, sOld, opos, p2, P.p, gj, TO);
+                    } else if (P.counting) {
                         eFinal += pair_overlap<KIND>(P.k, fN, p2, a2, TN) ? 1.0 : 0.0;
                         eInitial += pair_overlap<KIND>(P.k, fO, p2, a2, TO) ? 1.0 : 0.0;
                     } else {
@@ -2301,8 +2343,10 @@ hipError_t launch_sweep(hipStream_t s, const SweepParams &sp, int waves, Timed t
     else if (w == 2) RP_SWEEP(KIND, 2, SOFT); \
     else RP_SWEEP(KIND, 1, SOFT);
     if (sp.k.kind == 2) {
-        // polyspheres: the hard sweep reading the stencil from HBM, at most four wavefronts
-        RP_SWEEP_W(2, false)
+        // polyspheres: the hard sweep reading the stencil from HBM, at most four wavefronts; counting mode through
+        // the soft sweep's counting branch
+        if (sp.counting) { RP_SWEEP_W(2, true) }
+        else { RP_SWEEP_W(2, false) }
     } else if (sp.k.kind == 0) {
         if (sp.k.soft || sp.counting) { RP_SWEEP_W(0, true) }
         else if (w == 8) RP_SWEEP(0, 8, false);

@@ -147,3 +147,35 @@ def test_dimer_npt_packing_fraction_matches_reference(golden):
     print(f"gpu phi {np.mean(means)} (runs {np.round(means, 5)}), reference {ref.mean()} (runs {np.round(ref, 5)}), "
           f"error of the difference {err}", flush=True)
     assert abs(np.mean(means) - ref.mean()) <= 4 * err
+
+
+def random_chains(n_side=(8, 8, 2), spacing=(3.0, 3.0, 11.5), seed=12):
+    box, pos, _ = chain_start(n_side, spacing)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    rot = corpus.random_rotations(rng, len(pos))
+    return box, pos, np.ascontiguousarray(rot)
+
+
+def test_polysphere_counting_mode_and_overlap_relaxation():
+    """Overlap counting for polyspheres (Packing::toggleOverlapCounting, countParticleOverlaps over interaction
+    centres): the cached count of overlapping bead pairs equals the oracle's full recount after sweeps and box moves,
+    and Simulation::relaxOverlaps from randomly oriented chains ends with no overlap."""
+    box, pos, rot = random_chains()
+    start = poly_overlaps("kinked11", box, pos, rot)
+    assert start > 0
+    pk = ra.GpuPacking(poly_shape("kinked11"), box, pos, rot, seed=13)
+    pk.set_environment(1.0, 1.0, 0.3, None, box_move=("delta_triclinic", 0.05))
+    pk.set_overlap_counting(True)
+    assert pk.cached_overlaps() == start
+    for _ in range(3):
+        pk.run_cycles(5)
+        b, p, r = pk.download()
+        assert pk.cached_overlaps() == poly_overlaps("kinked11", b, p, r)
+    pk.set_overlap_counting(False)
+    cycles, left = pk.relax_overlaps(20000)
+    print(f"polysphere relaxation: {start} overlapping bead pairs removed in {cycles} cycles", flush=True)
+    assert left == 0
+    b, p, r = pk.download()
+    assert poly_overlaps("kinked11", b, p, r) == 0
+    pk.run_cycles(10)
+    pk.close()
