@@ -147,7 +147,7 @@ struct rp_ctx {
     uint64_t boxSinceEval = 0, boxAccSinceEval = 0;
 
     // launch shape
-    size_t ldsLimit = 64 * 1024 - 1024;  // dynamic LDS a workgroup may use
+    size_t ldsLimit = 64 * 1024 - 4096;  // dynamic LDS a workgroup may use
     int sweepWaves = 0;                  // wavefronts per active cell; 0 = chosen from the stencil size
     int colourPeriod = 0;                // checkerboard colour period (2 or 3); 0 = period 2
     int pipelined = -1;                  // staged sweep: 0 plain, 1 planner/tester split, 2 grouped moves, -1 auto
@@ -804,6 +804,7 @@ int runCycle(rp_ctx *c) {
     const size_t fixed = sweep_lds_bytes(c->k.kind, 0, waves);
     int maxCap = (int)((double)(c->ldsLimit - fixed) / perEntry) / 64 * 64;
     sp.cap = std::max(64, std::min(cap, maxCap));
+    if (c->k.kind == RP_SHAPE_POLYSPHERE) sp.cap = 64;  // the polysphere sweep reads its stencil from HBM
     sp.counters = c->dCounters;
     sp.phases = c->dPhases;
     sp.tuning = c->sweepTuning;
@@ -1000,7 +1001,9 @@ int rp_create(rp_ctx **out, const rp_shape_desc *shape, const double box[9], siz
     c->device = dev;
     int ldsBytes = 0;
     if (hipDeviceGetAttribute(&ldsBytes, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && ldsBytes > 0)
-        c->ldsLimit = (size_t)std::min(ldsBytes, 160 * 1024) - 1024;
+        // dynamic LDS a workgroup may use: the device limit less room for the kernels' static LDS tables (stencil
+        // cell tables, bead tables, vote slots: under 4 KB)
+        c->ldsLimit = (size_t)std::min(ldsBytes, 160 * 1024) - 4096;
     if (const char *w = std::getenv("RAMPACK_SWEEP_WAVES")) c->sweepWaves = std::atoi(w);
     if (const char *p = std::getenv("RAMPACK_COLOURING")) c->colourPeriod = std::atoi(p);
     if (const char *p = std::getenv("RAMPACK_SWEEP_PIPELINE")) c->pipelined = std::atoi(p);
@@ -1474,6 +1477,7 @@ int fixOrientations(rp_ctx *c, uint64_t &rejected) {
         (double)(sweep_lds_bytes(c->k.kind, 1024, waves) - sweep_lds_bytes(c->k.kind, 0, waves)) / 1024.0;
     const int maxCap = (int)((double)(c->ldsLimit - sweep_lds_bytes(c->k.kind, 0, waves)) / perEntry) / 64 * 64;
     sp.cap = std::max(64, std::min(cap, maxCap));
+    if (c->k.kind == RP_SHAPE_POLYSPHERE) sp.cap = 64;  // read from HBM
     const int colours = g.period * g.period * g.period;
     for (int colour = 0; colour < colours; colour++) {
         const int par[3] = {colour % g.period, (colour / g.period) % g.period, colour / (g.period * g.period)};
