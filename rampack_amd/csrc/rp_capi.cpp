@@ -149,7 +149,7 @@ struct rp_ctx {
     // launch shape
     size_t ldsLimit = 64 * 1024 - 4096;  // dynamic LDS a workgroup may use
     int sweepWaves = 0;                  // wavefronts per active cell; 0 = chosen from the stencil size
-    int colourPeriod = 0;                // checkerboard colour period (2 or 3); 0 = period 2
+    int colourPeriod = 0;                // checkerboard colour period (2 or 3); 0 = by shape
     int pipelined = -1;                  // staged sweep: 0 plain, 1 planner/tester split, 2 grouped moves, -1 auto
 
     // timing
@@ -383,14 +383,15 @@ int makeSweepGrid(rp_ctx *c, const HostBox &b, const double shiftUnit[3], Grid &
     return RP_OK;
 }
 
-// Colour period of the checkerboard: RAMPACK_COLOURING (2 or 3) if set, otherwise 3 -- finer cells, 27 colours,
-// more active cells per pass and a tighter 125-cell stencil -- when those cells still hold two particles on
-// average, else 2.
-// Period 2 (8 colours, cells >= range) unless period 3 is asked for: at the same density the period-3 grid has
-// 27 launches per cycle of workgroups with a few particles each and 125-cell stencils, which measured about four
-// times slower per trial move on 1e5 rods.
+// Colour period of the checkerboard: RAMPACK_COLOURING (2 or 3) when set; otherwise 3 for polyspheres and 2 for
+// the other shapes.  Period 2 has 8 colours and cells >= range.  Period 3 has 27 colours and cells >= range / 2:
+// at the same density that is 27 launches per cycle of workgroups with a few particles each and 125-cell
+// stencils, which measured about four times slower per trial move on 1e5 rods.  A polysphere's sweep cell spans
+// the whole molecule, so period 2 leaves few active cells per colour and stencils of thousands of beads; period 3
+// measured 1.9x faster on 50 653 kinked 11-bead chains.  Period 3 falls back to 2 when the box is too small for it.
 int chooseSweepGrid(rp_ctx *c, const HostBox &b, const double shiftUnit[3], Grid &g) {
-    if (c->colourPeriod == 3 && makeSweepGrid(c, b, shiftUnit, g, 3) == RP_OK) return RP_OK;
+    const int period = c->colourPeriod ? c->colourPeriod : (c->k.kind == RP_SHAPE_POLYSPHERE ? 3 : 2);
+    if (period == 3 && makeSweepGrid(c, b, shiftUnit, g, 3) == RP_OK) return RP_OK;
     return makeSweepGrid(c, b, shiftUnit, g, 2);
 }
 

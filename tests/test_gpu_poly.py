@@ -104,6 +104,23 @@ def test_polysphere_launch_shapes_give_identical_trajectories(waves, monkeypatch
         assert np.array_equal(x, y)
 
 
+@pytest.mark.parametrize("period", ["", "2", "3"])
+def test_polysphere_colourings_are_overlap_free(period, monkeypatch):
+    """Both checkerboard periods on a box wide enough for period 3 (cells of half the total range, at least six
+    per axis); without RAMPACK_COLOURING polyspheres take period 3."""
+    monkeypatch.setenv("RAMPACK_COLOURING", period)
+    box, pos, rot = chain_start((12, 12, 3), (3.0, 3.0, 11.5))
+    pk = ra.GpuPacking(poly_shape("kinked11"), box, pos, rot, seed=31)
+    pk.set_environment(1.0, 1.0, 0.3, None, box_move=("delta_triclinic", 0.02))
+    pk.set_timing(True)
+    c = pk.run_cycles(20, adapt_steps=True)
+    assert pk.timing()["sweep_period"] == int(period or 3)
+    b, p, r = pk.download()
+    assert poly_overlaps("kinked11", b, p, r) == 0 and pk.count_total_overlaps() == 0
+    assert c["moves_accepted"] > 0 and not np.allclose(p, pos)
+    pk.close()
+
+
 def test_polysphere_try_scaling():
     box, pos, rot = chain_start()
     pk = ra.GpuPacking(poly_shape("kinked11"), box, pos, rot, seed=8)
