@@ -179,6 +179,14 @@ int rp_get_move_samplers(rp_ctx *ctx, rp_move_desc *out, size_t n, size_t *n_mov
  * Simulation.cpp:533-607.  Fails with RP_ERR_VALIDATION if the packing has overlaps (Simulation.cpp:131-132). */
 int rp_run_cycles(rp_ctx *ctx, size_t n_cycles, int adapt_steps, rp_counters *out);
 
+/* rp_run_cycles on n independent packings at once: the replicas of an equation-of-state scan (the reference runs one
+ * `rampack casino` per state point; Simulation::integrate, src/core/Simulation.cpp:92-176, per replica), meant to
+ * run each context on its own host thread and HIP stream so that small replicas fill the GPU together.  out
+ * (nullable) receives n counter records.  A context may appear only once (RP_ERR_PRECONDITION otherwise).
+ * SWITCHED OFF: concurrent replicas faulted the GPU on a 32-replica scan and the cause is not found, so after the
+ * argument checks every call with n > 0 returns RP_ERR_UNSUPPORTED; run the replicas with rp_run_cycles. */
+int rp_run_replicas(rp_ctx *const *ctxs, size_t n, size_t n_cycles, int adapt_steps, rp_counters *out);
+
 /* Packing::countTotalOverlaps (src/core/Packing.cpp:519-559) on the reference's own neighbour grid
  * (Packing.cpp:858-899); early_exit returns 0 or 1. */
 int rp_count_total_overlaps(rp_ctx *ctx, int early_exit, uint64_t *out);

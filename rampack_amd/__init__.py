@@ -54,7 +54,7 @@ EXPORTED_SYMBOLS = (
     "rp_ramtrj_recorder_open", "rp_ramtrj_record", "rp_ramtrj_record_packing", "rp_ramtrj_recorder_last_cycle",
     "rp_ramtrj_recorder_close", "rp_ramtrj_player_open", "rp_ramtrj_player_header", "rp_ramtrj_next_snapshot",
     "rp_ramtrj_jump_to_snapshot", "rp_ramtrj_player_close", "rp_pair_decisions_guarded",
-    "rp_try_scaling", "rp_revert_scaling", "rp_set_move_samplers", "rp_get_move_samplers",
+    "rp_try_scaling", "rp_revert_scaling", "rp_set_move_samplers", "rp_get_move_samplers", "rp_run_replicas",
 )
 
 
@@ -165,6 +165,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.rp_set_environment.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.POINTER(MoveDesc),
                                        ctypes.POINTER(BoxDesc)]
     lib.rp_run_cycles.argtypes = [vp, ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(Counters)]
+    lib.rp_run_replicas.argtypes = [ctypes.POINTER(vp), ctypes.c_size_t, ctypes.c_size_t, ctypes.c_int,
+                                    ctypes.POINTER(Counters)]
     lib.rp_count_total_overlaps.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64)]
     lib.rp_download.argtypes = [vp, dp, dp, dp]
     lib.rp_set_timing.argtypes = [vp, ctypes.c_int]
@@ -536,6 +538,17 @@ def _aux_text(aux) -> bytes:
             raise ValueError(f"RAMSNAP aux key {k!r} must be non-empty and free of whitespace")
         lines.append(f"{k} {v}\n")
     return "".join(lines).encode()
+
+
+def run_replicas(packings, n_cycles: int, adapt_steps: bool = False) -> list:
+    """GpuPacking.run_cycles on every packing at once (rp_run_replicas), one counter dict per packing.  Switched off:
+    concurrent replicas faulted the GPU on a 32-replica scan with the cause not found, so any non-empty list raises
+    RampackError (RP_ERR_UNSUPPORTED); run the packings one by one with run_cycles."""
+    packings = list(packings)
+    ctxs = (ctypes.c_void_p * len(packings))(*[p._ctx.value for p in packings])
+    out = (Counters * len(packings))()
+    _check(_lib.rp_run_replicas(ctxs, len(packings), int(n_cycles), 1 if adapt_steps else 0, out))
+    return [c.as_dict() for c in out]
 
 
 def ramsnap_write(path: str, box: np.ndarray, pos: np.ndarray, rot: np.ndarray, aux: Optional[dict] = None):

@@ -109,3 +109,27 @@ def kinked_chain(n_beads: int = 11, diameter: float = 1.0, kink_deg: float = 30.
     c = np.array(pts)
     c -= c.mean(axis=0)
     return c, np.full(n_beads, diameter / 2)
+
+
+def rod_replica_start(n: int, packing_fraction: float, radius: float = 0.5, length: float = 5.0):
+    """n parallel spherocylinders along z filling the first n sites of a k x k x m lattice, the box scaled so that
+    the packing fraction is exact: the start of one replica of an equation-of-state scan (sc lattice, parallel
+    orientations).  The cell is c = 1.5 (length + 2 radius) along z (end-to-end gaps wide enough for box moves
+    to compress the start) and k ~ c/a times as many sites across, so
+    the box is near cubic.  The unfilled sites shrink the scaled cell; a start whose scaled cell leaves rods less
+    than 0.1 % apart is refused."""
+    v0 = np.pi * radius ** 2 * (length + 4.0 / 3 * radius)
+    c = 1.5 * (length + 2 * radius)
+    a = np.sqrt(v0 / (packing_fraction * c))
+    if a < 2 * radius:
+        raise ValueError("packing fraction too high for a parallel lattice start")
+    m = max(1, int(np.ceil(np.cbrt(n * a * a / (c * c)))))
+    k = int(np.ceil(np.sqrt(n / m)))
+    box, pos = sc_positions((k, k, m), (a, a, c))
+    pos = pos[:n]
+    scale = np.cbrt(n * v0 / packing_fraction / np.linalg.det(box))
+    if c * scale < 1.001 * (length + 2 * radius) or a * scale < 1.001 * 2 * radius:
+        raise ValueError("lattice too sparse for an overlap-free parallel start")
+    box, pos = box * scale, np.ascontiguousarray(pos * scale)
+    rot = np.tile(np.eye(3).reshape(1, 9), (n, 1))
+    return box, pos, rot

@@ -67,3 +67,15 @@ def test_structure_layouts_match_header(tmp_path):
         assert got == int(value), (cname, field)
         seen += 1
     assert seen == sum(len(py._fields_) + 1 for py in mirrors.values())
+
+
+def test_run_replicas_is_switched_off():
+    """The argument checks compare pointers only (nothing is dereferenced); past them every non-empty call is
+    refused, and an empty list is a no-op."""
+    lib = ra.load_library()
+    fake = (ctypes.c_void_p * 3)(0x1000, 0x2000, 0x1000)
+    assert lib.rp_run_replicas(fake, 0, 1, 0, None) == ra.RP_OK
+    assert lib.rp_run_replicas(fake, 3, 1, 0, None) == ra.RP_ERR_PRECONDITION
+    assert b"same context" in lib.rp_last_error()
+    assert lib.rp_run_replicas(fake, 2, 1, 0, None) == ra.RP_ERR_UNSUPPORTED
+    assert b"switched off" in lib.rp_last_error()
