@@ -69,21 +69,51 @@ def near_cubic_divisions(cores: int):
     return best
 
 
-def cpu_baseline(box, pos, rot, trans_step, rot_step, box_step, cycles, threads, seed):
+def cpu_baseline(box, pos, rot, trans_step, rot_step, box_step, cycles, threads, seed, wl):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import numpy as np
     import oracle_lib
     if not os.path.exists(oracle_lib.REF_PERF_PATH):
         return None
     ref = oracle_lib.ref(perf=True)
-    dd = near_cubic_divisions(threads)
+    if wl["beads"] is not None:
+        c, r = (np.ascontiguousarray(x) for x in wl["beads"])
+        ref.lib.ref_set_beads(len(c), c.ctypes.data_as(oracle_lib._dp), r.ctypes.data_as(oracle_lib._dp))
+    dd = wl["divisions"](threads)
     os.environ["OMP_NUM_THREADS"] = str(threads)
-    res = oracle_lib.ref_mc_run(ref, 1, 0.5, 5.0, box, pos, rot, 1.0, 2.0, trans_step, rot_step, 1, box_step, seed,
-                                dd=dd, threads=threads, therm=0, avg=cycles, avg_every=0)
+    res = oracle_lib.ref_mc_run(ref, wl["kind"], wl["radius"], wl["length"], box, pos, rot, 1.0, wl["pressure"],
+                                trans_step, rot_step, 1, box_step, seed, dd=dd, threads=threads, therm=0, avg=cycles,
+                                avg_every=0)
     rate = len(pos) * cycles / res["seconds"]
     return {"value": rate, "unit": "trial moves/s", "cores": threads, "kind": "reference",
-            "sample": f"{cycles} cycles of the same N={len(pos)} state (after GPU warm-up), NpT p*=2, "
+            "sample": f"{cycles} cycles of the same N={len(pos)} state (after GPU warm-up), NpT p*={wl['pressure']}, "
                       f"rototranslation + delta_triclinic, OpenMP domain divisions {list(dd)}, "
                       f"{res['seconds']:.1f} s"}
+
+
+def workload(name: str, n: int):
+    """The packing, its environment and what the roofline and CPU legs need for one bench workload."""
+    import numpy as np
+    import rampack_amd as ra
+    from rampack_amd import lattice
+    if name == "spherocylinders":
+        box, pos, rot = lattice.spherocylinder_smectic(n, 0.45)
+        return dict(shape=ra.Shape("spherocylinder", radius=0.5, length=5.0), state=(box, pos, rot), pressure=2.0,
+                    trans_step=0.1, box_step=1e-4, kind=1, radius=0.5, length=5.0, beads=None, centres=1,
+                    divisions=near_cubic_divisions,
+                    text=f"hard spherocylinders L/D=5 (r=0.5, L=5), N={len(pos)}, NpT p*=2.0 T=1, rototranslation "
+                         f"+ delta_triclinic box move per cycle, plain FP64 checkerboard sweep")
+    # rigid chains of 11 touching beads of diameter 1 with a 30 degree kink at the middle bead, aligned on a
+    # 37 x 37 x 37 lattice (N = 50 653, phi = 0.425), NpT p* = 3
+    c, r = lattice.kinked_chain(11, 1.0, 30.0)
+    box, pos = lattice.sc_positions((37, 37, 37), (1.1, 1.1, 11.2))
+    rot = np.tile(np.eye(3).reshape(1, 9), (len(pos), 1))
+    return dict(shape=ra.Shape("polysphere", centres=c, radii=r), state=(box, pos, rot), pressure=3.0,
+                trans_step=0.1, box_step=0.01, kind=2, radius=0.0, length=0.0, beads=(c, r), centres=len(c),
+                divisions=lambda t: (2, 2, max(1, t // 4)),
+                text=f"hard polyspheres: rigid chains of 11 touching beads (d=1) kinked 30 degrees at bead 5, "
+                     f"N={len(pos)} from an aligned lattice at phi=0.425 in a 40.7 x 40.7 x 414 box, NpT p*=3.0 T=1, "
+                     f"rototranslation + delta_triclinic box move per cycle, plain FP64 checkerboard sweep")
 
 
 def copy_probe(nbytes: int):
@@ -112,6 +142,9 @@ def main():
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--cpu-cycles", type=int, default=-1, help="-1: sized to ~20 s of CPU work; 0: skip")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--workload", choices=["spherocylinders", "kinked-chains"], default="spherocylinders",
+                    help="kinked-chains: the polysphere sweep on 50 653 kinked 11-bead chains (not the headline "
+                         "metric's workload)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a device-to-device copy of this many bytes (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -124,12 +157,11 @@ def main():
         sys.exit(2)
 
     import rampack_amd as ra
-    from rampack_amd import lattice
 
-    shape = ra.Shape("spherocylinder", radius=0.5, length=5.0)
-    box, pos, rot = lattice.spherocylinder_smectic(args.n, 0.45)
-    pk = ra.GpuPacking(shape, box, pos, rot, seed=args.seed)
-    pk.set_environment(1.0, 2.0, 0.1, None, box_move=("delta_triclinic", 1e-4))
+    wl = workload(args.workload, args.n)
+    box, pos, rot = wl["state"]
+    pk = ra.GpuPacking(wl["shape"], box, pos, rot, seed=args.seed)
+    pk.set_environment(1.0, wl["pressure"], wl["trans_step"], None, box_move=("delta_triclinic", wl["box_step"]))
     t0 = time.perf_counter()
     warm = pk.run_cycles(args.warmup, adapt_steps=True)
     log(f"warm-up {args.warmup} cycles in {time.perf_counter() - t0:.2f} s: phi={pk.packing_fraction():.4f} "
@@ -165,7 +197,15 @@ def main():
     # every cycle launches one sweep per colour (period^3 colours; all hold cells at this size)
     moves_per_launch = moves / (args.steps * tm["sweep_period"] ** 3)
     avg_sweep_ms = tm["sweep_ms"] / max(tm["sweep_launches"], 1)
-    bytes_per_launch = (27 + acc_rate) * RECORD_BYTES * moves_per_launch
+    # algorithmic bytes per trial move: the reference's 27-cell stencil of particle records, plus the write-back of
+    # accepted moves; with nc > 1 centres, nc (24 B centre + 72 B orientation) per stencil entry and nc 48 B
+    # bead records written back
+    nc = wl["centres"]
+    if nc == 1:
+        bytes_per_move = (27 + acc_rate) * RECORD_BYTES
+    else:
+        bytes_per_move = nc * 27 * (24 + 72) + acc_rate * (RECORD_BYTES + nc * 48)
+    bytes_per_launch = bytes_per_move * moves_per_launch
     achieved = bytes_per_launch / (avg_sweep_ms * 1e-3) / 1e9
     log(f"launch shape: colour period {tm['sweep_period']}, {tm['sweep_waves']} wavefronts per cell, "
         f"{tm['sweep_cap']} staged entries, "
@@ -183,17 +223,20 @@ def main():
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         cycles = args.cpu_cycles
         if cycles < 0:
-            probe = cpu_baseline(b, p, r, tb["trans_step"], tb["rot_step"], tb["box_step"], 2, threads, args.seed)
+            probe = cpu_baseline(b, p, r, tb["trans_step"], tb["rot_step"], tb["box_step"], 2, threads, args.seed,
+                                 wl)
             if probe is not None:
                 per_cycle = len(p) / probe["value"]
                 cycles = max(3, min(400, int(20.0 / per_cycle)))
         if cycles > 0:
-            cpu = cpu_baseline(b, p, r, tb["trans_step"], tb["rot_step"], tb["box_step"], cycles, threads, args.seed)
+            cpu = cpu_baseline(b, p, r, tb["trans_step"], tb["rot_step"], tb["box_step"], cycles, threads, args.seed,
+                               wl)
         if cpu:
             log(f"cpu baseline: {cpu['value']:.4g} trial moves/s on {cpu['cores']} threads ({cpu['sample']})")
 
     out = {
-        "metric": METRIC,
+        "metric": METRIC if args.workload == "spherocylinders" else
+        "MC trial moves/s, kinked 11-bead polysphere chains N=50653 (not the headline workload)",
         "value": rate,
         "unit": "trial moves/s",
         "n_gpus": 1,
@@ -206,10 +249,9 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": f"hard spherocylinders L/D=5 (r=0.5, L=5), N={len(pos)}, NpT p*=2.0 T=1, rototranslation "
-                        f"+ delta_triclinic box move per cycle, plain FP64 checkerboard sweep",
+            "workload": wl["text"],
             "n_particles": len(pos),
-            "pressure": 2.0,
+            "pressure": wl["pressure"],
             "packing_fraction_timed_start": round(phi_start, 5),
             "packing_fraction_timed_end": round(phi_end, 5),
             "acceptance": round(acc_rate, 4),
@@ -221,12 +263,12 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": sweep_traffic_bytes(),
+            "traffic": sweep_traffic_bytes() if args.workload == "spherocylinders" and args.n == 100000 else None,
             "kernel": "k_sweep",
             "avg_launch_us": avg_sweep_ms * 1e3,
             "timed_launches": tm["sweep_launches"],
-            "timing": "HIP events carried by one k_sweep dispatch per timed cycle (colour pass = cycle mod 8) on "
-                      "the context's stream",
+            "timing": f"HIP events carried by one k_sweep dispatch per timed cycle (colour pass = cycle mod "
+                      f"{tm['sweep_period'] ** 3}) on the context's stream",
             "algorithmic_bytes_per_launch": bytes_per_launch,
         },
         "cpu_baseline": cpu,

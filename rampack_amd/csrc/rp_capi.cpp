@@ -248,6 +248,17 @@ double shapeTotalRange(const rp_shape_desc &s) {
     return 2 * std::sqrt(maxR2) + shapeRange(s);
 }
 
+// Radius of the sphere about the particle centre that holds every bead: max |centre| + radius, raised by a relative
+// 1e-12 so that the rounding of the square root cannot make it smaller than any bead's reach.
+double shapeBound(const rp_shape_desc &s) {
+    double bound = 0;
+    for (int i = 0; i < s.n_centres; i++) {
+        const double *q = s.centres + 3 * i;
+        bound = std::max(bound, std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]) + s.radii[i]);
+    }
+    return bound * (1 + 1e-12);
+}
+
 // Shape descriptor checks shared by rp_create and the pair entry points.
 int validateShape(const rp_shape_desc &s) {
     if (s.kind == RP_SHAPE_POLYSPHERE) {
@@ -1048,6 +1059,7 @@ int rp_create(rp_ctx **out, const rp_shape_desc *shape, const double box[9], siz
     // polysphere sweep prefilter: particles whose centres are at least the total range apart share no bead
     // contact; the 1e-9 margin keeps every pair whose rounded bead distances could decide otherwise
     if (shape->kind == RP_SHAPE_POLYSPHERE) c->k.range2 = std::pow(c->totalRange * (1 + 1e-9), 2);
+    if (shape->kind == RP_SHAPE_POLYSPHERE) c->k.bound = shapeBound(*shape);
     if (shape->kind == RP_SHAPE_POLYSPHERE) {
         // PolysphereGeometry::calculateVolume (PolysphereTraits.cpp:138-146) needs beads that do not overlap by
         // spheresOverlap's test (PolysphereTraits.cpp:211-224): distance^2 (1 + 1e-12) < (r_i + r_j)^2

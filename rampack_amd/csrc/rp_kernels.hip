@@ -1383,15 +1383,34 @@ __device__ __forceinline__ void sweep_cell_pipelined(const SweepParams &P, const
 // moved particle (sTrial, centred at npos).  Candidates whose centre is at least the prefilter distance away
 // (k.range2 = (total range (1 + 1e-9))^2) cannot reach any bead; otherwise every bead pair is tested as
 // PolysphereTraits::HardInteraction::overlapBetween with the candidate's beads at p2 + O_j c_b.
+// Whether any bead pair of the moved particle (beads sTrial, centre npos) and candidate j (p2 + O_j c_b under
+// translation T) overlaps.  Two exact filters skip pairs that cannot: each particle's beads lie within k.bound of
+// its centre, so a bead of either particle at least bound + its own radius from the other particle's centre is
+// at least the sum of radii from every bead of the other.  The skip needs that distance squared to exceed
+// (bound + r)^2 by a relative 1e-9, far above the rounding of positions and of the bead test's own distance,
+// so the decision equals the full 11 x 11 test's.
 __device__ __forceinline__ bool poly_hit(const PairConsts &k, const double *sBeads, const double *sTrial,
                                          const V3 &npos, const V3 &p2, const Particles &p, int gj, const V3 &T) {
-    const V3 d = sub(add(p2, T), npos);
+    const V3 q2 = add(p2, T);
+    const V3 d = sub(q2, npos);
     if (!(dot(d, d) < k.range2)) return false;
+    // moved beads within reach of the candidate's bounding sphere (nc <= 16)
+    unsigned near = 0;
+    for (int a = 0; a < k.nc; a++) {
+        const V3 e = sub(v3(sTrial[3 * a], sTrial[3 * a + 1], sTrial[3 * a + 2]), q2);
+        const double t = k.bound + sBeads[4 * a + 3];
+        if (!(dot(e, e) > (t * t) * (1 + 1e-9))) near |= 1u << a;
+    }
+    if (!near) return false;
     double O[9];
     for (int e = 0; e < 9; e++) O[e] = p.o[e][gj];
     for (int b = 0; b < k.nc; b++) {
         const V3 c2 = add(p2, mulv(O, v3(sBeads[4 * b], sBeads[4 * b + 1], sBeads[4 * b + 2])));
-        for (int a = 0; a < k.nc; a++) {
+        const V3 f = sub(add(c2, T), npos);
+        const double t = k.bound + sBeads[4 * b + 3];
+        if (dot(f, f) > (t * t) * (1 + 1e-9)) continue;  // candidate bead out of reach of the moved particle
+        for (unsigned m = near; m; m &= m - 1) {
+            const int a = __builtin_ctz(m);
             const V3 c1 = v3(sTrial[3 * a], sTrial[3 * a + 1], sTrial[3 * a + 2]);
             if (bead_overlap(sBeads[4 * a + 3], sBeads[4 * b + 3], c1, c2, T)) return true;
         }

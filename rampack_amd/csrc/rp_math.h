@@ -167,6 +167,7 @@ struct PairConsts {
     double softRange2;   // centre distance^2 at and beyond which the soft energy is 0
     int nc;              // polysphere: beads per particle
     const double *beads; // polysphere: device array, per bead body-frame x, y, z and radius
+    double bound;        // polysphere: max over beads of |centre| + radius, rounded up (bounding-sphere radius)
 };
 
 // Spherocylinder first-particle data that does not depend on the candidate.
