@@ -4,7 +4,7 @@ Only IEEE-exact numpy operations are used (+, -, *, /, sqrt, floor, comparisons 
 the same seed gives bit-identical inputs on any x86-64 host; `records_sha256` lets a test confirm that before it
 compares decisions against the committed golden bitmaps.
 
-Corpus (north-star decision check): a random orientation pair and a random direction; the centre separation
+Corpus (near-contact decision check): a random orientation pair and a random direction; the centre separation
 along that direction is bisected so the FP64 segment distance of the two spherocylinder axes equals the contact
 distance 2r, then offset by delta ~ U(-1e-6, 1e-6); the second particle is wrapped into a triclinic box and the
 image translation T = box * k is recorded, as the reference's Packing passes it (Packing.cpp:21-44, 381-398).

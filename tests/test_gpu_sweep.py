@@ -340,8 +340,8 @@ def test_cached_overlap_count_stays_exact_in_counting_mode():
 
 
 def test_overlap_relaxation_reaches_a_legal_packing():
-    """Simulation::relaxOverlaps from randomly oriented rods on a lattice (the survey's C2 start, 1000 rods at the
-    same spacing): the relaxation ends with no overlap, confirmed by the oracle, and ordinary integration accepts
+    """Simulation::relaxOverlaps from randomly oriented rods on a lattice (the 20 x 20 x 20 random-orientation
+    start at phi = 0.25 cut to 1000 rods at the same spacing): the relaxation ends with no overlap, confirmed by the oracle, and ordinary integration accepts
     the result."""
     box, pos, rot = random_rod_lattice(10, 52.22 / 20, 2024)
     before, _ = oracle_lib.oracle().snapshot_stats(1, 0.5, 5.0, box, pos, rot)
