@@ -494,12 +494,23 @@ __device__ __forceinline__ int stencil_cells(int reach, bool half) {
 // Fills one slot of the stencil cell table of cell (cx, cy, cz): slot 0 is the own cell, then either all other
 // offsets in [-k, k]^3 (sweep) or only the positive ones, (o0, o1, o2) > 0 lexicographically with o0 the most
 // significant -- for k = 1 the reference's 9*n0 + 3*n1 + n2 > 13 (NeighbourGrid.cpp:130-131).
+// Reach-2 stencil cells (index (dx + 2) * 25 + (dy + 2) * 5 + dz + 2) nearest first: the own cell, the 26 cells one
+// step away (Chebyshev distance), then the 98 two steps away, each shell in natural order.  Read-only.
+__constant__ unsigned char kShellOrder[125] = {
+    62, 31, 32, 33, 36, 37, 38, 41, 42, 43, 56, 57, 58, 61, 63, 66, 67, 68, 81, 82, 83, 86, 87, 88, 91,
+    92, 93, 0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22,
+    23, 24, 25, 26, 27, 28, 29, 30, 34, 35, 39, 40, 44, 45, 46, 47, 48, 49, 50, 51, 52, 53, 54, 55, 59,
+    60, 64, 65, 69, 70, 71, 72, 73, 74, 75, 76, 77, 78, 79, 80, 84, 85, 89, 90, 94, 95, 96, 97, 98, 99,
+    100, 101, 102, 103, 104, 105, 106, 107, 108, 109, 110, 111, 112, 113, 114, 115, 116, 117, 118, 119, 120, 121, 122, 123, 124
+};
+
 __device__ __forceinline__ void stencil_table(const Grid &g, int cx, int cy, int cz, int slot, bool half,
-                                              int *s_start, int *s_base, int *s_ecode) {
+                                              int *s_start, int *s_base, int *s_ecode, bool byShell = false) {
     const int k = g.reach, side = 2 * k + 1, centre = (side * side * side - 1) / 2;
     int t;
     if (slot == 0) t = centre;
     else if (half) t = centre + slot;
+    else if (byShell && k == 2) t = kShellOrder[slot];
     else t = slot <= centre ? slot - 1 : slot;
     int ex, ey, ez;
     int nx = wrap_axis(cx, t / (side * side) - k, g.n[0], ex);
@@ -512,10 +523,12 @@ __device__ __forceinline__ void stencil_table(const Grid &g, int cx, int cy, int
 }
 
 // Builds the table of `ncells` stencil cells and its prefix sums (s_start[ncells] = total entries).
+// byShell (full stencils of reach 2 only) orders the cells by distance from the own cell, so that a search that
+// stops at the first overlap meets the nearest candidates first; the set of cells is the same.
 __device__ __forceinline__ void build_stencil(const Grid &g, int cx, int cy, int cz, int ncells, bool half,
-                                              int *s_start, int *s_base, int *s_ecode) {
+                                              int *s_start, int *s_base, int *s_ecode, bool byShell = false) {
     for (int slot = threadIdx.x; slot < ncells; slot += blockDim.x)
-        stencil_table(g, cx, cy, cz, slot, half, s_start, s_base, s_ecode);
+        stencil_table(g, cx, cy, cz, slot, half, s_start, s_base, s_ecode, byShell);
     __syncthreads();
     if (threadIdx.x == 0) {
         s_start[0] = 0;
@@ -2168,7 +2181,8 @@ __global__ __launch_bounds__(WAVE *WAVES, WAVES >= 8 ? 4 : 1) void k_sweep(Sweep
 
     const int ncells = stencil_cells(g.reach, false);
     if (tid < 2) s_vote[tid] = 0;
-    build_stencil(g, cx, cy, cz, ncells, false, s_start, s_base, s_ecode);
+    // polyspheres reject most moves on a near neighbour: their stencil is searched nearest cells first
+    build_stencil(g, cx, cy, cz, ncells, false, s_start, s_base, s_ecode, KIND == 2);
     const int total = s_start[ncells];
     if (P.phases) clk[1] = wall_clock64();
 
