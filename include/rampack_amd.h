@@ -55,6 +55,11 @@ extern "C" {
  *   eps max(0, sigma^2 / r^2 - 1), range sigma; eps may be negative. */
 #define RP_SOFT_LJ 3
 #define RP_SOFT_SIC 4
+/* RP_SOFT_HARD_CORE -- or-ed with a soft kind: CompoundInteraction of the shape's hard interaction and the soft one
+ *   (src/core/interactions/CompoundInteraction.cpp:9-60): a pair overlaps when the hard shapes do, its energy is the
+ *   soft part's, the range is the larger of the two, and the compound is convex (the soft part has no hard core).
+ *   E.g. RP_SOFT_WCA_SEGMENT | RP_SOFT_HARD_CORE: hard spherocylinders that also repel by WCA on the segment distance. */
+#define RP_SOFT_HARD_CORE 0x100
 
 /* Precision of the overlap tests.  RP_PREC_FP64 evaluates every test in the reference's FP64 operation order. */
 #define RP_PREC_FP64 0
@@ -186,6 +191,30 @@ int rp_run_cycles(rp_ctx *ctx, size_t n_cycles, int adapt_steps, rp_counters *ou
  * SWITCHED OFF: concurrent replicas faulted the GPU on a 32-replica scan and the cause is not found, so after the
  * argument checks every call with n > 0 returns RP_ERR_UNSUPPORTED; run the replicas with rp_run_cycles. */
 int rp_run_replicas(rp_ctx *const *ctxs, size_t n, size_t n_cycles, int adapt_steps, rp_counters *out);
+
+/* Decision record of one cycle's particle moves (no counterpart in the reference: a test hook through which a CPU
+ * restatement of Simulation::tryMove + Packing::tryMove / countParticleOverlaps / acceptMove (src/core/
+ * Simulation.cpp:447-491, src/core/Packing.cpp:136-164, 373-429, 278-302) replays every decision of the device sweep).
+ * rp_record_sweep runs the particle moves of one cycle (every checkerboard colour pass, no box move) and returns
+ * the state before and after them in the sweep's storage order (records sorted by checkerboard cell), each
+ * proposal (particle index within its cell, translation, rotation matrix) and its acceptance flag.  A call whose
+ * capacity is below rec->n_proposals fills rec, runs nothing and returns RP_ERR_PRECONDITION; the next call runs
+ * the same grid and proposals.  Hard interactions outside overlap-counting mode only. */
+typedef struct rp_sweep_record {
+    double box[9];
+    double shift[3];        /* checkerboard origin shift per axis, relative coordinates in [0, 1/cells) */
+    int cells[3];           /* checkerboard cells per axis; cell index = cx + cells[0] (cy + cells[1] cz) */
+    int period;             /* colour period: a colour's cells have coordinates = (px, py, pz) mod period */
+    int n_colours;
+    int colour_order[27];   /* colour passes in launch order, colour = px + period (py + period pz) */
+    int moves_per_cell;     /* > 0: that many proposals per cell, cell c's at [c * moves_per_cell, ...); 0: n_samplers
+                             * per record of the cell, cell c's at [n_samplers * (first record of c), ...) */
+    int n_samplers;
+    size_t n_proposals;
+} rp_sweep_record;
+int rp_record_sweep(rp_ctx *ctx, size_t capacity, rp_sweep_record *rec, double *pre_pos, double *pre_rot,
+                    int *pre_tag, int *pre_cell, int *prop_k, double *prop_t, double *prop_rot, uint8_t *accepted,
+                    double *post_pos, double *post_rot);
 
 /* Packing::countTotalOverlaps (src/core/Packing.cpp:519-559) on the reference's own neighbour grid
  * (Packing.cpp:858-899); early_exit returns 0 or 1. */
@@ -324,6 +353,22 @@ typedef struct rp_guard_stats {
 } rp_guard_stats;
 int rp_pair_decisions_guarded(const rp_shape_desc *shape, size_t n, const double *records, uint8_t *decisions,
                               uint8_t *codes, rp_guard_stats *stats);
+
+/* The box quantities the device path derives on the host (TriclinicBox, src/core/TriclinicBox.h:49-70 and
+ * TriclinicBox.cpp:27-51): the inverse of the box matrix (Matrix::inverse, src/geometry/Matrix.tpp:258-271), the three
+ * heights and the volume. */
+int rp_box_info(const double box[9], double inv[9], double heights[3], double *volume);
+
+/* The sweep's move arithmetic on explicit inputs, evaluated on the device by the functions the sweep kernels use:
+ * for each of n moves the trial position pos + t wrapped into the box (Shape::translate with
+ * PeriodicBoundaryConditions::getCorrection, src/core/Shape.cpp:9-12, PeriodicBoundaryConditions.cpp:11-22), the
+ * rotation matrix of axis / |axis| and angle (Matrix<3,3>::rotation as RototranslationSampler::sampleMove calls it,
+ * src/geometry/Matrix.tpp:438-450, RototranslationSampler.cpp:39-43), the trial orientation R O (Shape::rotate,
+ * Shape.cpp:14-16) and the device's sin and cos of the angle.  Arrays: pos / t / axis / out_pos n x 3, rot / out_R /
+ * out_rot n x 9 (row-major), angle n, out_sincos n x 2. */
+int rp_apply_moves(const double box[9], size_t n, const double *pos, const double *rot, const double *t,
+                   const double *axis, const double *angle, double *out_pos, double *out_R, double *out_rot,
+                   double *out_sincos);
 
 /* Batched soft pair energies on the device: Interaction::calculateEnergyBetween with a fixed cell translation
  * (CentralInteraction.h:58-67 for RP_SOFT_WCA); records as in rp_pair_decisions. */

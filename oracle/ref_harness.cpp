@@ -34,6 +34,7 @@
 #include "core/interactions/RepulsiveLennardJonesInteraction.h"
 #include "core/interactions/LennardJonesInteraction.h"
 #include "core/interactions/SquareInverseCoreInteraction.h"
+#include "core/interactions/CompoundInteraction.h"
 #include "core/move_samplers/RototranslationSampler.h"
 #include "core/volume_scalers/TriclinicDeltaScaler.h"
 #include "core/volume_scalers/TriclinicAdapter.h"
@@ -65,8 +66,9 @@ public:
     }
 };
 
-std::unique_ptr<ShapeTraits> makeTraits(int kind, double radius, double length, int nCentres, const double *centres,
-                                        const double *radii, int soft, double eps, double sigma) {
+std::unique_ptr<ShapeTraits> makeShapeTraits(int kind, double radius, double length, int nCentres,
+                                             const double *centres, const double *radii, int soft, double eps,
+                                             double sigma) {
     if (kind == 0) {
         if (soft == 1)
             return std::make_unique<SphereTraits>(radius, std::make_shared<RepulsiveLennardJonesInteraction>(eps, sigma));
@@ -82,6 +84,30 @@ std::unique_ptr<ShapeTraits> makeTraits(int kind, double radius, double length, 
     for (int i = 0; i < nCentres; i++)
         data.emplace_back(Vector<3>{centres[3 * i], centres[3 * i + 1], centres[3 * i + 2]}, radii[i]);
     return std::make_unique<PolysphereTraits>(PolysphereTraits::PolysphereGeometry(data));
+}
+
+// The shape's traits and the interaction the harness runs with: the traits' own, or, for a soft kind carrying the
+// hard-core flag 0x100, CompoundInteraction (CompoundInteraction.cpp:9-60) of the hard traits' interaction and the
+// soft traits' interaction (spheres only: the reference has no WCA on the segment distance).
+struct Model {
+    std::unique_ptr<ShapeTraits> traits, softTraits;
+    std::unique_ptr<CompoundInteraction> compound;
+    const Interaction &getInteraction() const { return compound ? *compound : traits->getInteraction(); }
+    const ShapeGeometry &getGeometry() const { return traits->getGeometry(); }
+};
+
+std::unique_ptr<Model> makeTraits(int kind, double radius, double length, int nCentres, const double *centres,
+                                  const double *radii, int soft, double eps, double sigma) {
+    auto m = std::make_unique<Model>();
+    if ((soft & 0x100) && (soft & 0xff)) {
+        m->traits = makeShapeTraits(kind, radius, length, nCentres, centres, radii, 0, eps, sigma);
+        m->softTraits = makeShapeTraits(kind, radius, length, nCentres, centres, radii, soft & 0xff, eps, sigma);
+        m->compound = std::make_unique<CompoundInteraction>(m->traits->getInteraction(),
+                                                            m->softTraits->getInteraction());
+    } else {
+        m->traits = makeShapeTraits(kind, radius, length, nCentres, centres, radii, soft & 0xff, eps, sigma);
+    }
+    return m;
 }
 
 Matrix<3, 3> mat(const double *m) {
@@ -152,6 +178,71 @@ int ref_snapshot_stats(int kind, double radius, double length, int nCentres, con
                         inter, 1, 1);
         *overlaps = inter.hasHardPart() ? packing.countTotalOverlaps(inter, earlyExit != 0) : 0;
         if (energy) *energy = packing.getTotalEnergy(inter);
+        return 0;
+    } catch (const std::exception &) {
+        return -1;
+    }
+}
+
+// TriclinicBox (TriclinicBox.h:49-70, TriclinicBox.cpp:27-51): the inverse the box keeps (Matrix::inverse), read
+// column by column through absoluteToRelative of the unit vectors (inv * e_j adds only exact zeros to inv(i, j)),
+// the heights and the volume.
+void ref_box_info(const double *box, double *inv, double *heights, double *volume) {
+    TriclinicBox b(mat(box));
+    for (int j = 0; j < 3; j++) {
+        Vector<3> e{};
+        e[j] = 1;
+        Vector<3> c = b.absoluteToRelative(e);
+        for (int i = 0; i < 3; i++) inv[3 * i + j] = c[i];
+    }
+    auto h = b.getHeights();
+    for (int i = 0; i < 3; i++) heights[i] = h[i];
+    *volume = b.getVolume();
+}
+
+// Shape::translate with the periodic correction (Shape.cpp:9-12, PeriodicBoundaryConditions.cpp:11-22), the step of
+// Packing::tryTranslation / tryMove (Packing.cpp:84-164), for n positions and translations.
+void ref_translate(const double *box, std::size_t n, const double *pos, const double *t, double *out) {
+    PeriodicBoundaryConditions pbc(TriclinicBox(mat(box)));
+    for (std::size_t i = 0; i < n; i++) {
+        Shape s(Vector<3>{pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]});
+        s.translate(Vector<3>{t[3 * i], t[3 * i + 1], t[3 * i + 2]}, pbc);
+        for (int d = 0; d < 3; d++) out[3 * i + d] = s.getPosition()[d];
+    }
+}
+
+// Matrix<3, 3>::rotation (Matrix.tpp:438-450) for n (axis, angle) pairs; normalize != 0 passes axis.normalized() as
+// RototranslationSampler::sampleMove does (RototranslationSampler.cpp:39-43).
+void ref_rotation(std::size_t n, const double *axis, const double *angle, int normalize, double *out) {
+    for (std::size_t i = 0; i < n; i++) {
+        Vector<3> a{axis[3 * i], axis[3 * i + 1], axis[3 * i + 2]};
+        Matrix<3, 3> R = Matrix<3, 3>::rotation(normalize ? a.normalized() : a, angle[i]);
+        for (int e = 0; e < 9; e++) out[9 * i + e] = R(e / 3, e % 3);
+    }
+}
+
+// Shape::rotate (Shape.cpp:14-16): O' = R O, for n pairs.
+void ref_rotate(std::size_t n, const double *R, const double *O, double *out) {
+    for (std::size_t i = 0; i < n; i++) {
+        Shape s(Vector<3>{0, 0, 0}, mat(O + 9 * i));
+        s.rotate(mat(R + 9 * i));
+        for (int e = 0; e < 9; e++) out[9 * i + e] = s.getOrientation()(e / 3, e % 3);
+    }
+}
+
+// Packing::tryScaling (Packing.cpp:172-208) on hard spheres of radius `radius` (kind 0) or spherocylinders (kind 1):
+// the positions it leaves (newBox.relativeToAbsolute(oldBox.absoluteToRelative(pos)), Packing.cpp:183-184) and its
+// return value (+inf for an overlap, else the energy change, 0 for hard shapes).
+int ref_try_scaling(int kind, double radius, double length, const double *oldBox, const double *newBox, std::size_t n,
+                    const double *pos, const double *rot, double *outPos, double *dE) {
+    try {
+        auto traits = makeTraits(kind, radius, length, 0, nullptr, nullptr, 0, 1, 1);
+        const Interaction &inter = traits->getInteraction();
+        Packing packing(TriclinicBox(mat(oldBox)), makeShapes(n, pos, rot),
+                        std::make_unique<PeriodicBoundaryConditions>(), inter, 1, 1);
+        *dE = packing.tryScaling(TriclinicBox(mat(newBox)), inter);
+        for (std::size_t i = 0; i < n; i++)
+            for (int d = 0; d < 3; d++) outPos[3 * i + d] = packing[i].getPosition()[d];
         return 0;
     } catch (const std::exception &) {
         return -1;
@@ -378,7 +469,7 @@ long ref_mc_run(int kind, double radius, double length, int soft, double eps, do
                                                  std::make_unique<PeriodicBoundaryConditions>(), inter,
                                                  (std::size_t)threads, (std::size_t)threads);
         RototranslationSampler sampler(transStep, rotStep > 0 ? std::optional<double>(rotStep) : std::nullopt, 0);
-        sampler.setupForShapeTraits(*traits);
+        sampler.setupForShapeTraits(*traits->traits);
         std::unique_ptr<TriclinicBoxScaler> scaler;
         if (boxKind == 1) {
             scaler = std::make_unique<TriclinicDeltaScaler>(boxStep, true);
@@ -523,7 +614,7 @@ long ref_mc_run(int kind, double radius, double length, int soft, double eps, do
                         for (int b = 0; b < 3; b++) r[9 * i + 3 * a + b] = O(a, b);
                 }
                 NematicOrder q;
-                q.calculate(*packing, temperature, pressure, *traits);
+                q.calculate(*packing, temperature, pressure, *traits->traits);
                 samples[3 * nSamples + 1] = q.getIntervalValues().front();
                 nSamples++;
             }

@@ -119,9 +119,8 @@ static inline Mat inverse3(const Mat &M) {
     return out;
 }
 
-// Matrix.tpp:438-450: Rodrigues, I + K*sin + (1 - cos)*K*K, with the matrix ops left to right
-static inline Mat rotation(const Vec &axis, double angle) {
-    double s = std::sin(angle), c = std::cos(angle);
+// Matrix.tpp:438-450: Rodrigues, I + K*sin + (1 - cos)*K*K, with the matrix ops left to right, for given sin and cos
+static inline Mat rotationSC(const Vec &axis, double s, double c) {
     Mat K{{0, -axis.v[2], axis.v[1], axis.v[2], 0, -axis.v[0], -axis.v[1], axis.v[0], 0}};
     Mat Ks{}, K2 = mulm(K, K), R{};
     // K*sina (Matrix *= x with the 0 / 1 special cases)
@@ -136,6 +135,8 @@ static inline Mat rotation(const Vec &axis, double angle) {
     for (int i = 0; i < 9; i++) R.m[i] = (I.m[i] + Ks.m[i]) + fKK.m[i];
     return R;
 }
+
+static inline Mat rotation(const Vec &axis, double angle) { return rotationSC(axis, std::sin(angle), std::cos(angle)); }
 
 // TriclinicBox.h:61-70, TriclinicBox.cpp:27-51
 struct Box {
@@ -239,6 +240,7 @@ struct ShapeSpec {
     std::vector<Vec> centres;       // polysphere bead positions (body frame)
     std::vector<double> radii;      // polysphere bead radii
     int soft = 0;                   // 0 none, 1 WCA on centre distance, 2 WCA on segment distance (no reference)
+    bool hardCore = false;          // with a soft kind: CompoundInteraction of the hard shape and the soft part
     double eps = 1, sigma = 1;
 
     // SphereTraits.h:34, SpherocylinderTraits.h:81, PolysphereTraits.cpp:116-121; a WCA sphere takes the range of
@@ -246,8 +248,15 @@ struct ShapeSpec {
     // L + sigma 2^(1/6), the farthest centre distance at which two segments can be within the cut-off
     double range() const {
         const double cut = sigma * std::pow(2., 1. / 6);
-        if (kind == SPHERE) return soft == 1 ? cut : soft == 3 ? 3 * sigma : soft == 4 ? sigma : 2 * radius;
-        if (kind == SPHEROCYLINDER) return soft == 2 ? length + cut : 2 * radius + length;
+        // CompoundInteraction.cpp:33-35: the larger of the two ranges
+        if (kind == SPHERE) {
+            const double r = soft == 1 ? cut : soft == 3 ? 3 * sigma : soft == 4 ? sigma : 2 * radius;
+            return hardCore ? std::max(r, 2 * radius) : r;
+        }
+        if (kind == SPHEROCYLINDER) {
+            const double r = soft == 2 ? length + cut : 2 * radius + length;
+            return hardCore ? std::max(r, 2 * radius + length) : r;
+        }
         double m = 0;
         for (double r : radii) m = std::max(m, r);
         double rr = 2 * m;
@@ -263,7 +272,7 @@ struct ShapeSpec {
     }
     int nCentres() const { return kind == POLYSPHERE ? (int)centres.size() : 0; }
     // a WCA interaction replaces the hard core of a sphere or spherocylinder (CentralInteraction.h:48-51)
-    bool hasHard() const { return kind == POLYSPHERE || soft == 0; }
+    bool hasHard() const { return kind == POLYSPHERE || soft == 0 || hardCore; }
 };
 
 // Cap centre: SpherocylinderTraits.cpp:24-26
@@ -580,10 +589,131 @@ static std::uint64_t countBrute(const Snapshot &S) {
     return count;
 }
 
+// Replay of one cycle of checkerboard particle moves (the device's decision record) through the reference's move
+// test: Simulation::tryMove -> Packing::tryMove (Packing.cpp:136-164: pos + t wrapped as Shape::translate +
+// PeriodicBoundaryConditions::getCorrection, O' = R O as Shape::rotate), the active-domain rule (a move that leaves
+// its domain is rejected, Packing.cpp:147-148 with ActiveDomain::isInside, ActiveDomain.cpp:9-22; the domains here
+// are the checkerboard cells), Packing::countParticleOverlaps over the reference's neighbour grid with early exit
+// (Packing.cpp:373-429, NeighbourGrid.cpp:353-370), the particle itself skipped by index in every cell
+// (Packing.cpp:383-384), and Packing::acceptMove (Packing.cpp:278-302: the record replaced, the grid updated).
+// Proposals run in the record's order: colour passes in launch order, each cell's moves in sequence.
+struct SweepReplay {
+    const ShapeSpec &spec;
+    Box box;
+    std::size_t n;
+    std::vector<Vec> pos;
+    std::vector<Mat> rot;
+    int cells[3];
+    double shift[3];
+    SweepReplay(const ShapeSpec &s, const double *b, std::size_t n_, const double *p, const double *r)
+            : spec(s), box(b), n(n_), pos(n_), rot(n_) {
+        for (std::size_t i = 0; i < n; i++) {
+            std::memcpy(pos[i].v, p + 3 * i, 24);
+            std::memcpy(rot[i].m, r + 9 * i, 72);
+        }
+    }
+    // the checkerboard cell of a position: y = x - shift, wrapped into [0, 1) (a position below the shift sits in
+    // the last cell), cell = floor(y * cells)
+    int domainOf(const Vec &p) const {
+        Vec rel = box.toRel(p);
+        int c[3];
+        for (int d = 0; d < 3; d++) {
+            const double x = rel.v[d];
+            const int w = x < shift[d] ? 1 : 0;
+            const double y = (x - shift[d]) + (w ? 1.0 : 0.0);
+            int cc = (int)(y * (double)cells[d]);
+            if (w) cc = cells[d] - 1;
+            cc = std::max(0, std::min(cells[d] - 1, cc));
+            c[d] = cc;
+        }
+        return c[0] + cells[0] * (c[1] + cells[1] * c[2]);
+    }
+};
+
+static int replaySweep(SweepReplay &R, const int *cellOf, int period, int nColours, const int *order,
+                       int movesPerCell, int nSamplers, std::size_t nProposals, const int *propK,
+                       const double *propT, const double *propR, std::uint8_t *decision) {
+    const ShapeSpec &sp = R.spec;
+    const double cellSize = ngCellSize(R.box, sp.range(), R.n);
+    if (cellSize == 0) return -2;  // the reference would test all pairs through the real PBC: not replayed here
+    Grid g(R.box, cellSize, R.n);
+    std::vector<std::size_t> where(R.n);
+    for (std::size_t i = 0; i < R.n; i++) {
+        where[i] = g.positionToCell(R.pos[i]);
+        g.add((long)i, where[i]);
+    }
+    const int ncell = R.cells[0] * R.cells[1] * R.cells[2];
+    std::vector<std::size_t> start(ncell + 1, 0);
+    for (std::size_t i = 0; i < R.n; i++) {
+        if (cellOf[i] < 0 || cellOf[i] >= ncell || (i && cellOf[i] < cellOf[i - 1])) return -3;
+        start[cellOf[i] + 1]++;
+    }
+    for (int c = 0; c < ncell; c++) start[c + 1] += start[c];
+    for (std::size_t i = 0; i < nProposals; i++) decision[i] = 3;  // not run
+    for (int ci = 0; ci < nColours; ci++) {
+        const int colour = order[ci];
+        const int px = colour % period, py = (colour / period) % period, pz = colour / (period * period);
+        for (int cz = pz; cz < R.cells[2]; cz += period)
+            for (int cy = py; cy < R.cells[1]; cy += period)
+                for (int cx = px; cx < R.cells[0]; cx += period) {
+                    const int cell = cx + R.cells[0] * (cy + R.cells[1] * cz);
+                    const std::size_t own0 = start[cell], nOwn = start[cell + 1] - own0;
+                    if (!nOwn) continue;
+                    const std::size_t moves = movesPerCell > 0 ? (std::size_t)movesPerCell : nOwn * nSamplers;
+                    const std::size_t pbase = movesPerCell > 0 ? (std::size_t)cell * movesPerCell : own0 * nSamplers;
+                    if (pbase + moves > nProposals) return -4;
+                    for (std::size_t m = 0; m < moves; m++) {
+                        const std::size_t q = pbase + m;
+                        if (propK[q] < 0 || (std::size_t)propK[q] >= nOwn) return -5;
+                        const std::size_t gi = own0 + (std::size_t)propK[q];
+                        Vec t{{propT[3 * q], propT[3 * q + 1], propT[3 * q + 2]}};
+                        Mat Rm;
+                        std::memcpy(Rm.m, propR + 9 * q, 72);
+                        const Vec np = translateWrapped(R.box, R.pos[gi], t);
+                        const Mat no = mulm(Rm, R.rot[gi]);
+                        if (R.domainOf(np) != cell) {
+                            decision[q] = 2;
+                            continue;
+                        }
+                        bool overlap = false;
+                        const std::size_t cno = g.positionToCell(np);
+                        for (long off : g.offsets) {
+                            const std::size_t nb = (std::size_t)((long)cno + off);
+                            const Vec &T = g.trans[g.transIdx[nb]];
+                            for (long j = g.head[g.reflected[nb]]; j != -1 && !overlap; j = g.next[j]) {
+                                if ((std::size_t)j == gi) continue;
+                                if (pairOverlap(sp, np, no, 0, R.pos[j], R.rot[j], 0, T)) overlap = true;
+                            }
+                            if (overlap) break;
+                        }
+                        if (overlap) {
+                            decision[q] = 0;
+                            continue;
+                        }
+                        decision[q] = 1;
+                        R.pos[gi] = np;
+                        R.rot[gi] = no;
+                        // NeighbourGrid::remove / add (NeighbourGrid.cpp:238-258, 219-227)
+                        const std::size_t nc = g.positionToCell(np);
+                        if (nc != where[gi]) {
+                            long *link = &g.head[where[gi]];
+                            while (*link != (long)gi) link = &g.next[*link];
+                            *link = g.next[gi];
+                            g.add((long)gi, nc);
+                            where[gi] = nc;
+                        }
+                    }
+                }
+    }
+    return 0;
+}
+
 static ShapeSpec makeSpec(int kind, double radius, double length, int nCentres, const double *centres,
                           const double *radii, int soft, double eps, double sigma) {
     ShapeSpec s;
-    s.kind = kind; s.radius = radius; s.length = length; s.soft = soft; s.eps = eps; s.sigma = sigma;
+    // soft kinds may carry the hard-core flag 0x100 (a CompoundInteraction with the shape's hard part)
+    s.kind = kind; s.radius = radius; s.length = length; s.soft = soft & 0xff; s.hardCore = (soft & 0x100) != 0;
+    s.eps = eps; s.sigma = sigma;
     for (int i = 0; i < nCentres; i++) {
         s.centres.push_back({{centres[3 * i], centres[3 * i + 1], centres[3 * i + 2]}});
         s.radii.push_back(radii[i]);
@@ -717,6 +847,33 @@ std::uint64_t orc_count_brute(int kind, double radius, double length, int nCentr
     return countBrute(S);
 }
 
+// Replays a device decision record (rp_record_sweep) of hard spheres (kind 0) or spherocylinders (kind 1): pos / rot
+// are the n records before the moves in the sweep's storage order, cellOf their checkerboard cells (ascending);
+// decision[i] receives 1 accepted, 0 rejected by an overlap, 2 rejected for leaving the cell, 3 not run (a cell with
+// no particle); postPos / postRot the records after the moves.  Returns 0, or negative for an input it cannot
+// replay (-2: the reference would run without a neighbour grid).
+int orc_replay_sweep(int kind, double radius, double length, const double *box, std::size_t n, const double *pos,
+                     const double *rot, const int *cellOf, const int *cells, const double *shift, int period,
+                     int nColours, const int *order, int movesPerCell, int nSamplers, std::size_t nProposals,
+                     const int *propK, const double *propT, const double *propR, std::uint8_t *decision,
+                     double *postPos, double *postRot) {
+    if (kind != SPHERE && kind != SPHEROCYLINDER) return -1;
+    ShapeSpec spec = makeSpec(kind, radius, length, 0, nullptr, nullptr, 0, 1, 1);
+    SweepReplay R(spec, box, n, pos, rot);
+    for (int d = 0; d < 3; d++) {
+        R.cells[d] = cells[d];
+        R.shift[d] = shift[d];
+    }
+    int r = replaySweep(R, cellOf, period, nColours, order, movesPerCell, nSamplers, nProposals, propK, propT, propR,
+                        decision);
+    if (r) return r;
+    for (std::size_t i = 0; i < n; i++) {
+        std::memcpy(postPos + 3 * i, R.pos[i].v, 24);
+        std::memcpy(postRot + 9 * i, R.rot[i].m, 72);
+    }
+    return 0;
+}
+
 // Box helpers: inverse, heights, volume, relative <-> absolute, PBC correction / translation / distance2.
 void orc_box_info(const double *box, double *inv, double *heights, double *volume) {
     Box b(box);
@@ -774,6 +931,23 @@ void orc_rescale(const double *oldBox, const double *newBox, std::size_t n, cons
 void orc_rotation(const double *axis, double angle, double *out) {
     Mat R = rotation({{axis[0], axis[1], axis[2]}}, angle);
     std::memcpy(out, R.m, 72);
+}
+
+// The same with sin and cos supplied (the device evaluates them with its own math library).
+void orc_rotation_sc(const double *axis, double s, double c, double *out) {
+    Mat R = rotationSC({{axis[0], axis[1], axis[2]}}, s, c);
+    std::memcpy(out, R.m, 72);
+}
+
+// Shape::rotate (Shape.cpp:14-16): R O for n pairs.
+void orc_rotate(std::size_t n, const double *R, const double *O, double *out) {
+    for (std::size_t i = 0; i < n; i++) {
+        Mat a, b;
+        std::memcpy(a.m, R + 9 * i, 72);
+        std::memcpy(b.m, O + 9 * i, 72);
+        Mat r = mulm(a, b);
+        std::memcpy(out + 9 * i, r.m, 72);
+    }
 }
 
 // Nematic order P2 (NematicOrder.cpp:9-28): Q = 0.5*(3*<a a^T> - I), a = orientation column 2

@@ -32,6 +32,7 @@ SOFT_WCA = 1
 SOFT_WCA_SEGMENT = 2
 SOFT_LJ = 3
 SOFT_SIC = 4
+SOFT_HARD_CORE = 0x100
 PREC_FP64 = 0
 BOX_NONE = 0
 BOX_DELTA_TRICLINIC = 1
@@ -55,6 +56,7 @@ EXPORTED_SYMBOLS = (
     "rp_ramtrj_recorder_close", "rp_ramtrj_player_open", "rp_ramtrj_player_header", "rp_ramtrj_next_snapshot",
     "rp_ramtrj_jump_to_snapshot", "rp_ramtrj_player_close", "rp_pair_decisions_guarded",
     "rp_try_scaling", "rp_revert_scaling", "rp_set_move_samplers", "rp_get_move_samplers", "rp_run_replicas",
+    "rp_record_sweep", "rp_box_info", "rp_apply_moves",
 )
 
 
@@ -129,6 +131,12 @@ class GuardStats(ctypes.Structure):
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class SweepRecord(ctypes.Structure):
+    _fields_ = [("box", ctypes.c_double * 9), ("shift", ctypes.c_double * 3), ("cells", ctypes.c_int * 3),
+                ("period", ctypes.c_int), ("n_colours", ctypes.c_int), ("colour_order", ctypes.c_int * 27),
+                ("moves_per_cell", ctypes.c_int), ("n_samplers", ctypes.c_int), ("n_proposals", ctypes.c_size_t)]
 
 
 class RampackError(RuntimeError):
@@ -208,6 +216,11 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.rp_pair_decisions_guarded.argtypes = [ctypes.POINTER(ShapeDesc), ctypes.c_size_t, dp,
                                               ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(ctypes.c_uint8),
                                               ctypes.POINTER(GuardStats)]
+    ip = ctypes.POINTER(ctypes.c_int)
+    lib.rp_record_sweep.argtypes = [vp, ctypes.c_size_t, ctypes.POINTER(SweepRecord), dp, dp, ip, ip, ip, dp, dp,
+                                    ctypes.POINTER(ctypes.c_uint8), dp, dp]
+    lib.rp_box_info.argtypes = [dp, dp, dp, dp]
+    lib.rp_apply_moves.argtypes = [dp, ctypes.c_size_t, dp, dp, dp, dp, dp, dp, dp, dp, dp]
     lib.rp_last_error.restype = ctypes.c_char_p
     _lib = lib
     return lib
@@ -235,7 +248,8 @@ class Shape:
     soft: None (hard core), 'wca' (a sphere with RepulsiveLennardJonesInteraction(epsilon, sigma)), 'lj' (a sphere with
     LennardJonesInteraction, truncated at its range 3 sigma in sweeps and total energies), 'sic' (a sphere with
     SquareInverseCoreInteraction) or 'wca_segment' (a spherocylinder with WCA on the distance between the axis
-    segments)."""
+    segments).  hard_core (with a soft interaction): CompoundInteraction of the shape's hard core and the soft part --
+    pairs whose hard shapes overlap are forbidden, the energy is the soft part's."""
     kind: str
     radius: float = 0.5
     length: float = 0.0
@@ -244,6 +258,7 @@ class Shape:
     sigma: float = 1.0
     centres: Optional[tuple] = None
     radii: Optional[tuple] = None
+    hard_core: bool = False
 
     def desc(self) -> ShapeDesc:
         kinds = {"sphere": SHAPE_SPHERE, "spherocylinder": SHAPE_SPHEROCYLINDER, "polysphere": SHAPE_POLYSPHERE}
@@ -252,8 +267,9 @@ class Shape:
             raise ValueError(f"unknown shape {self.kind}")
         if self.soft not in softs:
             raise ValueError(f"unknown soft interaction {self.soft}")
-        d = ShapeDesc(kinds[self.kind], float(self.radius), float(self.length), softs[self.soft],
-                      float(self.epsilon), float(self.sigma))
+        soft = softs[self.soft] | (SOFT_HARD_CORE if self.hard_core else 0)
+        d = ShapeDesc(kinds[self.kind], float(self.radius), float(self.length), soft, float(self.epsilon),
+                      float(self.sigma))
         if self.kind == "polysphere":
             c = np.ascontiguousarray(self.centres, dtype=np.float64).reshape(-1, 3)
             r = np.ascontiguousarray(self.radii, dtype=np.float64).reshape(-1)
@@ -279,17 +295,14 @@ class Shape:
     @property
     def range(self) -> float:
         cut = self.sigma * 2 ** (1 / 6)
-        if self.soft == "wca":
-            return cut
-        if self.soft == "lj":
-            return 3 * self.sigma
-        if self.soft == "sic":
-            return self.sigma
-        if self.soft == "wca_segment":
-            return self.length + cut
         if self.kind == "polysphere":
-            return 2 * max(self.radii)
-        return 2 * self.radius + (self.length if self.kind == "spherocylinder" else 0.0)
+            hard = 2 * max(self.radii)
+        else:
+            hard = 2 * self.radius + (self.length if self.kind == "spherocylinder" else 0.0)
+        soft = {"wca": cut, "lj": 3 * self.sigma, "sic": self.sigma, "wca_segment": self.length + cut}.get(self.soft)
+        if soft is None:
+            return hard
+        return max(soft, hard) if self.hard_core else soft
 
     @property
     def total_range(self) -> float:
@@ -330,6 +343,27 @@ def pair_decisions_guarded(shape: Shape, records: np.ndarray, verify: bool = Tru
     _check(lib.rp_pair_decisions_guarded(ctypes.byref(d), len(rec), _dptr(rec), out.ctypes.data_as(u8),
                                          codes.ctypes.data_as(u8), ctypes.byref(st)))
     return out, codes, st.as_dict()
+
+
+def box_info(box: np.ndarray):
+    """rp_box_info: (inverse 3x3, heights, volume) as the device path computes them."""
+    lib = load_library()
+    b = np.ascontiguousarray(box, dtype=np.float64).reshape(9)
+    inv, h, v = np.zeros(9), np.zeros(3), np.zeros(1)
+    _check(lib.rp_box_info(_dptr(b), _dptr(inv), _dptr(h), _dptr(v)))
+    return inv.reshape(3, 3), h, float(v[0])
+
+
+def apply_moves(box, pos, rot, t, axis, angle):
+    """rp_apply_moves: the sweep's move arithmetic on the device; returns (trial positions, rotation matrices, trial
+    orientations, sin/cos of the angles)."""
+    lib = load_library()
+    b = np.ascontiguousarray(box, dtype=np.float64).reshape(9)
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (pos, rot, t, axis, angle)]
+    n = len(arrs[4])
+    out = [np.zeros((n, 3)), np.zeros((n, 9)), np.zeros((n, 9)), np.zeros((n, 2))]
+    _check(lib.rp_apply_moves(_dptr(b), n, *[_dptr(a) for a in arrs], *[_dptr(o) for o in out]))
+    return tuple(out)
 
 
 def pair_energies(shape: Shape, records: np.ndarray) -> np.ndarray:
@@ -416,6 +450,32 @@ class GpuPacking:
         c = Counters()
         _check(_lib.rp_run_cycles(self._ctx, int(n_cycles), 1 if adapt_steps else 0, ctypes.byref(c)))
         return c.as_dict()
+
+    def record_sweep(self) -> dict:
+        """rp_record_sweep: one cycle's particle moves (no box move) with the state before and after them in the
+        sweep's storage order, every proposal and its acceptance flag."""
+        rec = SweepRecord()
+        n = self.n
+        ip = ctypes.POINTER(ctypes.c_int)
+        u8 = ctypes.POINTER(ctypes.c_uint8)
+        code = _lib.rp_record_sweep(self._ctx, 0, ctypes.byref(rec), None, None, None, None, None, None, None, None,
+                                    None, None)
+        if code != RP_ERR_PRECONDITION or rec.n_proposals == 0:
+            _check(code)
+        npr = int(rec.n_proposals)
+        out = {"pre_pos": np.zeros((n, 3)), "pre_rot": np.zeros((n, 9)), "pre_tag": np.zeros(n, dtype=np.int32),
+               "pre_cell": np.zeros(n, dtype=np.int32), "prop_k": np.zeros(npr, dtype=np.int32),
+               "prop_t": np.zeros((npr, 3)), "prop_rot": np.zeros((npr, 9)), "accepted": np.zeros(npr, dtype=np.uint8),
+               "post_pos": np.zeros((n, 3)), "post_rot": np.zeros((n, 9))}
+        _check(_lib.rp_record_sweep(
+            self._ctx, npr, ctypes.byref(rec), _dptr(out["pre_pos"]), _dptr(out["pre_rot"]),
+            out["pre_tag"].ctypes.data_as(ip), out["pre_cell"].ctypes.data_as(ip), out["prop_k"].ctypes.data_as(ip),
+            _dptr(out["prop_t"]), _dptr(out["prop_rot"]), out["accepted"].ctypes.data_as(u8),
+            _dptr(out["post_pos"]), _dptr(out["post_rot"])))
+        out.update(box=np.array(rec.box[:]).reshape(3, 3), shift=np.array(rec.shift[:]), cells=list(rec.cells),
+                   period=rec.period, colour_order=list(rec.colour_order[:rec.n_colours]),
+                   moves_per_cell=rec.moves_per_cell, n_samplers=rec.n_samplers, n_proposals=npr)
+        return out
 
     def count_total_overlaps(self, early_exit: bool = False) -> int:
         v = ctypes.c_uint64()

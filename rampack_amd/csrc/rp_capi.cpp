@@ -185,10 +185,16 @@ int allocParticles(DeviceParticles &d, size_t n) {
 // sigma * pow(2, 1/6), as RepulsiveLennardJonesInteraction's constructor computes it
 double wcaCut(const rp_shape_desc &s) { return s.sigma * std::pow(2., 1. / 6); }
 
+// The soft kind of a descriptor without the hard-core flag, and whether the shape's hard test applies.
+int softKind(const rp_shape_desc &s) { return s.soft & ~RP_SOFT_HARD_CORE; }
+bool compound(const rp_shape_desc &s) { return softKind(s) != RP_SOFT_NONE && (s.soft & RP_SOFT_HARD_CORE) != 0; }
+bool hasHardPart(const rp_shape_desc &s) { return softKind(s) == RP_SOFT_NONE || compound(s); }
+
 PairConsts makeConsts(const rp_shape_desc &s) {
     PairConsts k{};
     k.kind = s.kind;
-    k.soft = s.soft;
+    k.soft = softKind(s);
+    k.hardCore = compound(s) ? 1 : 0;
     if (s.kind == RP_SHAPE_SPHERE) {
         k.contact2 = std::pow(2 * s.radius, 2);              // SphereTraits.cpp:64
     } else {
@@ -196,19 +202,20 @@ PairConsts makeConsts(const rp_shape_desc &s) {
         k.range2 = std::pow(2 * s.radius + s.length, 2);     // SpherocylinderTraits.cpp:37
         k.halfLen = 0.5 * s.length;                          // SpherocylinderTraits.cpp:24-26 (0.5 * 1 * L)
     }
-    if (s.soft) {
+    const int soft = softKind(s);
+    if (soft) {
         const double cut = wcaCut(s);
         k.eps = s.epsilon;
         k.sigma2 = s.sigma * s.sigma;                        // RepulsiveLennardJonesInteraction.cpp:12
         k.cut2 = cut * cut;                                  // RepulsiveLennardJonesInteraction.cpp:9
         // segment distance >= centre distance - L: beyond (L + cut) the energy is 0; the margin keeps every pair
         // whose rounded segment distance could still fall below the cut-off
-        k.softRange2 = s.soft == 2 ? std::pow(s.length + cut, 2) * (1 + 1e-9) : k.cut2;
-        if (s.soft == RP_SOFT_LJ) {
+        k.softRange2 = soft == RP_SOFT_WCA_SEGMENT ? std::pow(s.length + cut, 2) * (1 + 1e-9) : k.cut2;
+        if (soft == RP_SOFT_LJ) {
             // LennardJonesInteraction.cpp:9 (sigma * sigma / d2); truncated at the range 3 sigma
             k.cut2 = std::pow(3 * s.sigma, 2);
             k.softRange2 = k.cut2;
-        } else if (s.soft == RP_SOFT_SIC) {
+        } else if (soft == RP_SOFT_SIC) {
             k.sigma2 = std::pow(s.sigma, 2);                // SquareInverseCoreInteraction.cpp:14
             k.cut2 = k.sigma2;                              // zero beyond sigma
             k.softRange2 = k.cut2;
@@ -219,11 +226,20 @@ PairConsts makeConsts(const rp_shape_desc &s) {
 
 // Interaction range: SphereTraits.h:34, SpherocylinderTraits.h:81; a WCA sphere the range of its interaction
 // (RepulsiveLennardJonesInteraction.h:26); a WCA spherocylinder L + sigma 2^(1/6)
+double hardRange(const rp_shape_desc &s);
+// a compound's range is the larger of its parts' (CompoundInteraction.cpp:33-35)
 double shapeRange(const rp_shape_desc &s) {
-    if (s.soft == RP_SOFT_WCA) return wcaCut(s);
-    if (s.soft == RP_SOFT_LJ) return 3 * s.sigma;   // LennardJonesInteraction.h:25
-    if (s.soft == RP_SOFT_SIC) return s.sigma;      // SquareInverseCoreInteraction.h:22
-    if (s.soft == RP_SOFT_WCA_SEGMENT) return s.length + wcaCut(s);
+    const int soft = softKind(s);
+    double r = 0;
+    if (soft == RP_SOFT_WCA) r = wcaCut(s);
+    else if (soft == RP_SOFT_LJ) r = 3 * s.sigma;   // LennardJonesInteraction.h:25
+    else if (soft == RP_SOFT_SIC) r = s.sigma;      // SquareInverseCoreInteraction.h:22
+    else if (soft == RP_SOFT_WCA_SEGMENT) r = s.length + wcaCut(s);
+    else return hardRange(s);
+    return compound(s) ? std::max(r, hardRange(s)) : r;
+}
+
+double hardRange(const rp_shape_desc &s) {
     if (s.kind == RP_SHAPE_POLYSPHERE) {
         // PolysphereTraits.cpp:116-121: twice the largest bead radius
         double m = 0;
@@ -273,15 +289,19 @@ int validateShape(const rp_shape_desc &s) {
     if (s.kind != RP_SHAPE_SPHERE && s.kind != RP_SHAPE_SPHEROCYLINDER)
         return fail(RP_ERR_UNSUPPORTED, "unsupported shape kind");
     if (!(s.radius > 0)) return fail(RP_ERR_PRECONDITION, "radius must be > 0");
-    if (s.soft != RP_SOFT_NONE) {
-        const bool sphereSoft = s.soft == RP_SOFT_WCA || s.soft == RP_SOFT_LJ || s.soft == RP_SOFT_SIC;
+    const int soft = softKind(s);
+    if (s.soft & ~(RP_SOFT_HARD_CORE | 0xff)) return fail(RP_ERR_PRECONDITION, "unknown soft interaction flags");
+    if (soft == RP_SOFT_NONE && s.soft != RP_SOFT_NONE)
+        return fail(RP_ERR_PRECONDITION, "RP_SOFT_HARD_CORE needs a soft interaction");
+    if (soft != RP_SOFT_NONE) {
+        const bool sphereSoft = soft == RP_SOFT_WCA || soft == RP_SOFT_LJ || soft == RP_SOFT_SIC;
         if (!((sphereSoft && s.kind == RP_SHAPE_SPHERE) ||
-              (s.soft == RP_SOFT_WCA_SEGMENT && s.kind == RP_SHAPE_SPHEROCYLINDER)))
+              (soft == RP_SOFT_WCA_SEGMENT && s.kind == RP_SHAPE_SPHEROCYLINDER)))
             return fail(RP_ERR_UNSUPPORTED, "soft interaction not available for this shape");
         // RepulsiveLennardJonesInteraction.cpp:21-22, LennardJonesInteraction.cpp:16-17; the square-inverse core
         // only needs a nonzero epsilon (SquareInverseCoreInteraction.cpp:8-9)
-        if (s.soft == RP_SOFT_SIC ? !(s.epsilon != 0 && std::isfinite(s.epsilon)) : !(s.epsilon > 0))
-            return fail(RP_ERR_PRECONDITION, s.soft == RP_SOFT_SIC ? "epsilon must be nonzero" : "epsilon must be > 0");
+        if (soft == RP_SOFT_SIC ? !(s.epsilon != 0 && std::isfinite(s.epsilon)) : !(s.epsilon > 0))
+            return fail(RP_ERR_PRECONDITION, soft == RP_SOFT_SIC ? "epsilon must be nonzero" : "epsilon must be > 0");
         if (!(s.sigma > 0)) return fail(RP_ERR_PRECONDITION, "sigma must be > 0");
     }
     if (s.kind == RP_SHAPE_SPHEROCYLINDER && !(s.length >= 0))
@@ -499,6 +519,7 @@ int countCapLimit(rp_ctx *c) {
 // Early-exit overlap scan of the `alt` arrays sorted by `g`; sets found.
 int scanItems(rp_ctx *c, const Grid &g, const Particles &items, size_t nItems, int earlyExit, uint64_t &count) {
     RP_HIP(launch_fill_u64(c->stream, c->dCounters + CNT_OVERLAPS, 1, 0));
+    RP_HIP(launch_fill_u64(c->stream, c->dCounters + CNT_STAGE_OVERFLOW, 1, 0));
     double lambda = (double)nItems / (double)g.ncell;
     const int side = 2 * g.reach + 1;
     double expect = ((side * side * side - 1) / 2 + 1) * lambda;
@@ -508,7 +529,7 @@ int scanItems(rp_ctx *c, const Grid &g, const Particles &items, size_t nItems, i
                                  timingSlot(c, 2)));
     int r = readCounters(c);
     if (r) return r;
-    if (c->hCounters[CNT_UNSTAGED_BLOCKS] >> 32)
+    if (c->hCounters[CNT_STAGE_OVERFLOW])
         return fail(RP_ERR_UNSUPPORTED, "a grid cell holds more particles than the overlap count can stage");
     count = c->hCounters[CNT_OVERLAPS];
     return RP_OK;
@@ -579,6 +600,14 @@ int softBoxMove(rp_ctx *c, const HostBox &nb, double exponent, double u, const d
     r = rebuild(c, g, 1, c->box.inv);
     if (r) return r;
     c->boxScans++;
+    if (c->k.hardCore && !isBoxUpscaled(c->box, nb)) {
+        // the compound's hard part: an overlap makes dE infinite (Packing.cpp:199-204); the compound is convex
+        // (CompoundInteraction.cpp:25-31)
+        uint64_t found = 0;
+        r = scanOverlaps(c, g, 1, found);
+        if (r) return r;
+        if (found) return RP_OK;
+    }
     r = totalEnergy(c, g, c->alt.p, eNew);
     if (r) return r;
     if (!(u <= std::exp(exponent - (eNew - eOld) / c->temperature))) return RP_OK;
@@ -604,12 +633,13 @@ int totalEnergy(rp_ctx *c, const Grid &g, const Particles &p, double &out) {
     double expect = ((side * side * side - 1) / 2 + 1) * lambda;
     int cap = (int)std::ceil(expect + 8 * std::sqrt(expect) + 64);
     cap = std::min((cap + 63) / 64 * 64, countCapLimit(c));
+    RP_HIP(launch_fill_u64(c->stream, c->dCounters + CNT_STAGE_OVERFLOW, 1, 0));
     RP_HIP(launch_total_energy(c->stream, g, (int)c->n, p, c->k, c->cellEnergy, c->sumScratch, c->dCounters, cap));
     RP_HIP(hipMemcpyAsync(&out, c->sumScratch + fixed_sum_scratch_doubles() - 1, sizeof(double),
                           hipMemcpyDeviceToHost, c->stream));
     int r = readCounters(c);  // synchronises the stream
     if (r) return r;
-    if (c->hCounters[CNT_UNSTAGED_BLOCKS] >> 32)
+    if (c->hCounters[CNT_STAGE_OVERFLOW])
         return fail(RP_ERR_UNSUPPORTED, "a grid cell holds more particles than the energy sum can stage");
     return RP_OK;
 }
@@ -723,7 +753,6 @@ int prepareCycleGrid(rp_ctx *c) {
     return RP_OK;
 }
 
-// One cycle: 8 colour passes in random order (N trial moves in total), then one box move.
 // One sweep launch with the per-workgroup trace on, written to the context's trace file as lines of
 // "block start_us end_us own stencil cell cu" (times from the first workgroup's start).
 int traceSweep(rp_ctx *c, SweepParams sp, int waves) {
@@ -761,7 +790,25 @@ int traceSweep(rp_ctx *c, SweepParams sp, int waves) {
     return RP_OK;
 }
 
-int runCycle(rp_ctx *c) {
+// What one cycle's particle moves used: the colour order and the proposal layout (for decision records).
+struct SweepInfo {
+    int colours = 0;
+    int order[27] = {};
+    int moves = 0, mult = 1;
+    size_t proposals = 0;
+};
+
+// Number of proposals the cycle about to run draws on the current (prepared) grid.
+size_t cycleProposals(const rp_ctx *c) {
+    const double lambda = (double)c->n / (double)c->grid.ncell;
+    const int nSamplers = (int)c->samplers.size();
+    const int moves = lambda >= 3 ? (int)std::lround(lambda) * nSamplers : 0;
+    return moves > 0 ? (size_t)c->grid.ncell * moves : c->n * (size_t)nSamplers;
+}
+
+// The particle moves of one cycle: proposals, then the colour passes in random order.  logDecisions: every
+// proposal's acceptance flag is written (prop.acc) even with a single move sampler.
+int sweepCycle(rp_ctx *c, bool logDecisions, SweepInfo *info) {
     int r = prepareCycleGrid(c);
     if (r) return r;
     const Grid &g = c->grid;
@@ -795,7 +842,10 @@ int runCycle(rp_ctx *c) {
         if (r2) return r2;
     }
     sp.pr = c->prop;
-    if (ms.n == 1) sp.pr.type = sp.pr.acc = nullptr;  // one sampler: the totals are its counters
+    if (ms.n == 1) {
+        sp.pr.type = nullptr;  // one sampler: the totals are its counters
+        if (!logDecisions) sp.pr.acc = nullptr;
+    }
     RP_HIP(launch_proposals(c->stream, g, sp.pr, sp.moves, c->cur.p.cell, (int)c->n, ms, c->seed,
                             (uint32_t)c->cycle, timingSlot(c, 3)));
     const int side = 2 * g.reach + 1;
@@ -830,6 +880,13 @@ int runCycle(rp_ctx *c) {
         int j = (int)philox_below(c->hostRng, (uint32_t)(i + 1));
         std::swap(order[i], order[j]);
     }
+    if (info) {
+        info->colours = colours;
+        std::memcpy(info->order, order, sizeof order);
+        info->moves = sp.moves;
+        info->mult = sp.mult;
+        info->proposals = proposals;
+    }
     c->tstats.lds_limit = c->ldsLimit;
     c->tstats.sweep_cap = (uint64_t)sp.cap;
     c->tstats.sweep_waves = (uint64_t)waves;
@@ -859,6 +916,13 @@ int runCycle(rp_ctx *c) {
         RP_HIP(hipStreamSynchronize(c->stream));
         c->numOverlaps += (long long)d;
     }
+    return RP_OK;
+}
+
+// One cycle: the particle moves (colour passes), then one box move.
+int runCycle(rp_ctx *c) {
+    int r = sweepCycle(c, false, nullptr);
+    if (r) return r;
     if (c->boxKind != RP_BOX_NONE) {
         bool acc = false;
         r = boxMove(c, acc);
@@ -1211,7 +1275,7 @@ int rp_run_cycles(rp_ctx *c, size_t nCycles, int adapt, rp_counters *out) {
     int r = RP_OK;
     // a WCA interaction has no hard part, and counting mode runs on overlapping packings: nothing to validate
     // (Simulation.cpp:127-132)
-    if (!c->k.soft && !c->counting) r = countExact(c, 1, overl);
+    if ((!c->k.soft || c->k.hardCore) && !c->counting) r = countExact(c, 1, overl);
     if (r == RP_ERR_UNSUPPORTED) {
         // the reference's grid is off for this box; check on the checkerboard grid instead
         double zero[3] = {0, 0, 0};
@@ -1291,6 +1355,82 @@ int rp_run_replicas(rp_ctx *const *ctxs, size_t n, size_t nCycles, int adapt, rp
     // one after another with rp_run_cycles.
     return fail(RP_ERR_UNSUPPORTED, "rp_run_replicas is switched off: concurrent replicas faulted the GPU (cause "
                                     "not found); run the replicas one by one with rp_run_cycles");
+}
+
+// Sorted device records (storage order of the current cell list) into host arrays; null arrays are skipped.
+static int downloadSorted(rp_ctx *c, double *pos, double *rot, int *tag, int *cell) {
+    const size_t n = c->n;
+    std::vector<double> col(n);
+    auto down = [&](const double *src, double *dst, int stride, int off) -> int {
+        RP_HIP(hipMemcpy(col.data(), src, sizeof(double) * n, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; i++) dst[stride * i + off] = col[i];
+        return RP_OK;
+    };
+    int r;
+    if (pos && ((r = down(c->cur.p.x, pos, 3, 0)) || (r = down(c->cur.p.y, pos, 3, 1)) ||
+                (r = down(c->cur.p.z, pos, 3, 2))))
+        return r;
+    if (rot)
+        for (int k = 0; k < 9; k++)
+            if ((r = down(c->cur.p.o[k], rot, 9, k))) return r;
+    if (tag) RP_HIP(hipMemcpy(tag, c->cur.p.tag, sizeof(int) * n, hipMemcpyDeviceToHost));
+    if (cell) RP_HIP(hipMemcpy(cell, c->cur.p.cell, sizeof(int) * n, hipMemcpyDeviceToHost));
+    return RP_OK;
+}
+
+int rp_record_sweep(rp_ctx *c, size_t capacity, rp_sweep_record *rec, double *prePos, double *preRot, int *preTag,
+                    int *preCell, int *propK, double *propT, double *propRot, uint8_t *accepted, double *postPos,
+                    double *postRot) {
+    if (!c || !rec) return fail(RP_ERR_PRECONDITION, "null argument");
+    if (!c->envSet) return fail(RP_ERR_PRECONDITION, "environment not set");
+    if (c->k.soft || c->counting)
+        return fail(RP_ERR_PRECONDITION, "decision records cover hard interactions outside overlap counting");
+    int r = prepareCycleGrid(c);  // idempotent until the cycle runs: a too-small call leaves the grid for the next
+    if (r) return r;
+    *rec = rp_sweep_record{};
+    std::memcpy(rec->box, c->box.dims, sizeof rec->box);
+    for (int d = 0; d < 3; d++) {
+        rec->shift[d] = c->grid.shift[d];
+        rec->cells[d] = c->grid.n[d];
+    }
+    rec->period = c->grid.period;
+    rec->n_samplers = (int)c->samplers.size();
+    rec->n_proposals = cycleProposals(c);
+    if (capacity < rec->n_proposals)
+        return fail(RP_ERR_PRECONDITION, "capacity " + std::to_string(capacity) + " below the cycle's " +
+                                             std::to_string(rec->n_proposals) + " proposals");
+    if (!prePos || !preRot || !preTag || !preCell || !propK || !propT || !propRot || !accepted || !postPos ||
+        !postRot)
+        return fail(RP_ERR_PRECONDITION, "null record array");
+    RP_HIP(hipStreamSynchronize(c->stream));
+    r = downloadSorted(c, prePos, preRot, preTag, preCell);
+    if (r) return r;
+    SweepInfo info;
+    r = sweepCycle(c, true, &info);
+    if (r) return r;
+    RP_HIP(hipStreamSynchronize(c->stream));
+    if (info.proposals != rec->n_proposals) return fail(RP_ERR_DEVICE, "proposal count changed");
+    rec->n_colours = info.colours;
+    std::memcpy(rec->colour_order, info.order, sizeof info.order);
+    rec->moves_per_cell = info.moves;
+    const size_t np = info.proposals, stride = (size_t)c->prop.stride;
+    RP_HIP(hipMemcpy(propK, c->prop.k, sizeof(int) * np, hipMemcpyDeviceToHost));
+    RP_HIP(hipMemcpy(accepted, c->prop.acc, np, hipMemcpyDeviceToHost));
+    std::vector<double> col(np);
+    for (int d = 0; d < 3; d++) {
+        RP_HIP(hipMemcpy(col.data(), c->prop.t + d * stride, sizeof(double) * np, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < np; i++) propT[3 * i + d] = col[i];
+    }
+    for (int e = 0; e < 9; e++) {
+        RP_HIP(hipMemcpy(col.data(), c->prop.R + e * stride, sizeof(double) * np, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < np; i++) propRot[9 * i + e] = col[i];
+    }
+    r = downloadSorted(c, postPos, postRot, nullptr, nullptr);
+    if (r) return r;
+    // a cycle without its box move: the next cycle draws new proposals and a new grid origin
+    c->cycle++;
+    c->cycles++;
+    return RP_OK;
 }
 
 int rp_count_total_overlaps(rp_ctx *c, int earlyExit, uint64_t *out) {
@@ -1480,7 +1620,7 @@ namespace {
 // reference visits them in index order; only a fix that itself creates an overlap can tell the orders apart).
 int fixOrientations(rp_ctx *c, uint64_t &rejected) {
     rejected = 0;
-    if (c->k.soft) {
+    if (c->k.soft && !c->k.hardCore) {
         RP_HIP(launch_fix_all(c->stream, c->cur.p, (int)c->n));
         RP_HIP(hipStreamSynchronize(c->stream));
         return RP_OK;
@@ -1494,6 +1634,7 @@ int fixOrientations(rp_ctx *c, uint64_t &rejected) {
     sp.g = g;
     sp.p = c->cur.p;
     sp.k = c->k;
+    sp.k.soft = 0;  // a compound interaction's fix is judged by its hard part alone (Packing.cpp:1203-1206)
     for (int d = 0; d < 3; d++) sp.na[d] = std::max(1, g.n[d] / g.period);
     sp.moves = 0;
     sp.mult = 1;
@@ -1624,7 +1765,9 @@ int rp_try_scaling(rp_ctx *c, const double new_box[9], double *dE) {
     if (r) return r;
     double result = 0;
     long long count = c->numOverlaps;
-    if (!c->k.soft) {
+    // the hard part first (Packing.cpp:190-204); a compound interaction's energy only counts without an overlap
+    const bool hard = !c->k.soft || c->k.hardCore;
+    if (hard) {
         if (c->counting) {
             uint64_t after = 0;
             r = c->k.kind == RP_SHAPE_POLYSPHERE ? countBeads(c, c->alt.p, nb, 0, after) : scanOverlaps(c, g, 0, after);
@@ -1643,7 +1786,8 @@ int rp_try_scaling(rp_ctx *c, const double new_box[9], double *dE) {
             if (r) return r;
             if (found) result = INF;
         }
-    } else {
+    }
+    if (c->k.soft && result == 0) {
         double eNew = 0;
         r = totalEnergy(c, g, c->alt.p, eNew);
         if (r) return r;
@@ -1696,12 +1840,13 @@ int rp_total_energy(rp_ctx *c, double *out) {
 
 int rp_pair_energies(const rp_shape_desc *shape, size_t n, const double *records, double *energies) {
     if (!shape || (n && (!records || !energies))) return fail(RP_ERR_PRECONDITION, "null argument");
-    if (shape->soft == RP_SOFT_NONE) return fail(RP_ERR_UNSUPPORTED, "no soft interaction for this shape descriptor");
+    if (softKind(*shape) == RP_SOFT_NONE)
+        return fail(RP_ERR_UNSUPPORTED, "no soft interaction for this shape descriptor");
     if (int v = validateShape(*shape)) return v;
     if (n == 0) return RP_OK;
     PairConsts k = makeConsts(*shape);
     // Interaction::calculateEnergyBetween for one pair: the Lennard-Jones formula is not truncated
-    if (shape->soft == RP_SOFT_LJ) k.cut2 = std::numeric_limits<double>::infinity();
+    if (softKind(*shape) == RP_SOFT_LJ) k.cut2 = std::numeric_limits<double>::infinity();
     double *dRec = nullptr, *dOut = nullptr;
     RP_HIP(hipMalloc(&dRec, sizeof(double) * 27 * n));
     if (hipMalloc(&dOut, sizeof(double) * n) != hipSuccess) {
@@ -1718,10 +1863,50 @@ int rp_pair_energies(const rp_shape_desc *shape, size_t n, const double *records
     return r;
 }
 
+int rp_box_info(const double box[9], double inv[9], double heights[3], double *volume) {
+    if (!box || !inv || !heights || !volume) return fail(RP_ERR_PRECONDITION, "null argument");
+    HostBox b;
+    b.set(box);
+    std::memcpy(inv, b.inv, sizeof b.inv);
+    b.heights(heights);
+    *volume = b.volume();
+    return RP_OK;
+}
+
+int rp_apply_moves(const double box[9], size_t n, const double *pos, const double *rot, const double *t,
+                   const double *axis, const double *angle, double *outPos, double *outR, double *outRot,
+                   double *outSinCos) {
+    if (!box || (n && (!pos || !rot || !t || !axis || !angle || !outPos || !outR || !outRot || !outSinCos)))
+        return fail(RP_ERR_PRECONDITION, "null argument");
+    HostBox b;
+    b.set(box);
+    if (b.volume() == 0) return fail(RP_ERR_PRECONDITION, "zero-volume box");
+    if (n == 0) return RP_OK;
+    // one block: inputs pos 3, rot 9, t 3, axis 3, angle 1; outputs pos 3, R 9, rot 9, sin/cos 2 (42 doubles a move)
+    double *d = nullptr;
+    RP_HIP(hipMalloc(&d, sizeof(double) * 42 * n));
+    double *dPos = d, *dRot = dPos + 3 * n, *dT = dRot + 9 * n, *dAxis = dT + 3 * n, *dAngle = dAxis + 3 * n;
+    double *oPos = dAngle + n, *oR = oPos + 3 * n, *oRot = oR + 9 * n, *oSc = oRot + 9 * n;
+    hipError_t e = hipSuccess;
+    const struct { double *dst; const double *src; size_t len; } in[] = {
+        {dPos, pos, 3 * n}, {dRot, rot, 9 * n}, {dT, t, 3 * n}, {dAxis, axis, 3 * n}, {dAngle, angle, n}};
+    for (const auto &c : in)
+        if (e == hipSuccess) e = hipMemcpy(c.dst, c.src, sizeof(double) * c.len, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = launch_apply_moves(nullptr, b.dims, b.inv, n, dPos, dRot, dT, dAxis, dAngle, oPos, oR, oRot, oSc);
+    const struct { double *dst; const double *src; size_t len; } out[] = {
+        {outPos, oPos, 3 * n}, {outR, oR, 9 * n}, {outRot, oRot, 9 * n}, {outSinCos, oSc, 2 * n}};
+    for (const auto &c : out)
+        if (e == hipSuccess) e = hipMemcpy(c.dst, c.src, sizeof(double) * c.len, hipMemcpyDeviceToHost);
+    hipFree(d);
+    if (e != hipSuccess) return fail(RP_ERR_DEVICE, std::string("apply moves: ") + hipGetErrorString(e));
+    return RP_OK;
+}
+
 int rp_pair_decisions(const rp_shape_desc *shape, size_t n, const double *records, uint8_t *decisions) {
     if (!shape || (n && (!records || !decisions))) return fail(RP_ERR_PRECONDITION, "null argument");
     if (int v = validateShape(*shape)) return v;
-    if (shape->soft != RP_SOFT_NONE) return fail(RP_ERR_UNSUPPORTED, "a soft interaction has no hard part");
+    if (!hasHardPart(*shape)) return fail(RP_ERR_UNSUPPORTED, "a soft interaction has no hard part");
     if (n == 0) return RP_OK;
     PairConsts k = makeConsts(*shape);
     double *dBeads = nullptr;
@@ -1756,7 +1941,7 @@ int rp_pair_decisions_guarded(const rp_shape_desc *shape, size_t n, const double
     if (!shape || (n && (!records || !decisions))) return fail(RP_ERR_PRECONDITION, "null argument");
     if (shape->kind != RP_SHAPE_SPHERE && shape->kind != RP_SHAPE_SPHEROCYLINDER)
         return fail(RP_ERR_UNSUPPORTED, "unsupported shape kind");
-    if (shape->soft != RP_SOFT_NONE) return fail(RP_ERR_UNSUPPORTED, "a soft interaction has no hard part");
+    if (!hasHardPart(*shape)) return fail(RP_ERR_UNSUPPORTED, "a soft interaction has no hard part");
     if (stats) *stats = rp_guard_stats{};
     if (n == 0) return RP_OK;
     PairConsts k = makeConsts(*shape);

@@ -111,7 +111,8 @@ enum Counter : int {
     CNT_OVERLAP_DELTA = 7,  // overlap-counting mode: sum of the accepted moves' overlap changes (two's complement)
     CNT_TYPE_ATT = 8,       // per move sampler (RP_MAX_MOVE_SAMPLERS entries): attempted moves
     CNT_TYPE_ACC = 12,      // per move sampler: accepted moves
-    CNT_NUM = 16
+    CNT_STAGE_OVERFLOW = 16,  // cells of an overlap count or energy sum too full to stage (cleared before each launch)
+    CNT_NUM = 24
 };
 
 // Launch wrappers (rp_kernels.hip).  All enqueue on `s` and return the launch status.
@@ -142,6 +143,11 @@ size_t count_lds_bytes(int kind, int cap);
 // sorted record, n records).
 hipError_t launch_proposals(hipStream_t s, const Grid &g, const Proposals &pr, int moves, const int *cellOf, int n,
                             const MoveSet &ms, uint64_t seed, uint32_t cycle, Timed t = {});
+// The sweep's move arithmetic on n explicit records (device arrays): pos + t wrapped into the box, the rotation matrix
+// of (axis, angle), R O, and sin / cos of the angle.
+hipError_t launch_apply_moves(hipStream_t s, const double *dims, const double *inv, size_t n, const double *pos,
+                              const double *rot, const double *t, const double *axis, const double *angle,
+                              double *outPos, double *outR, double *outRot, double *outSc);
 // Adds each proposal's attempt and acceptance (pr.type, pr.acc; `total` proposals) to the per-sampler counters.
 hipError_t launch_type_counts(hipStream_t s, const Proposals &pr, size_t total, unsigned long long *counters);
 // waves: wavefronts per active cell (1, 2 or 4)
@@ -150,7 +156,7 @@ hipError_t launch_pair_energies(hipStream_t s, PairConsts k, size_t n, const dou
 
 // Total soft energy over the half-shell pairs of grid g (Packing::getTotalEnergy, Packing.cpp:690-709):
 // cellEnergy (g.ncell doubles) receives one partial per cell, scratch (fixed_sum_scratch_doubles()) the fixed-order
-// sum, whose last element is the total.  counters[CNT_UNSTAGED_BLOCKS] >> 32 counts cells too full to stage.
+// sum, whose last element is the total.  counters[CNT_STAGE_OVERFLOW] counts cells too full to stage.
 hipError_t launch_total_energy(hipStream_t s, const Grid &g, int n, Particles p, PairConsts k, double *cellEnergy,
                                double *scratch, unsigned long long *counters, int cap);
 size_t fixed_sum_scratch_doubles();

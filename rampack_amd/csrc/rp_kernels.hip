@@ -755,7 +755,7 @@ __global__ __launch_bounds__(COUNT_THREADS) void k_count_overlaps(Grid g, Partic
                               sBeadR);
     } else if (KIND == 2) {
         // bead grids hold a few beads per cell; a cell beyond the flat path's limits is reported as an error
-        if (tid == 0) atomicAdd(&counters[CNT_UNSTAGED_BLOCKS], 1ull << 32);
+        if (tid == 0) atomicAdd(&counters[CNT_STAGE_OVERFLOW], 1ull);
     } else if (total <= cap) {
         stage_stencil<KIND>(p, S, total, tid, COUNT_THREADS);
         __syncthreads();
@@ -765,7 +765,7 @@ __global__ __launch_bounds__(COUNT_THREADS) void k_count_overlaps(Grid g, Partic
         count_cell<KIND, false>(g, p, k, S, sT, sTag, nOwn, total, earlyExit, counters, &sFound);
     } else if (tid == 0) {
         // a cell holding more than `cap` particles: the host sizes cap from the largest cell, so this marks an error
-        atomicAdd(&counters[CNT_UNSTAGED_BLOCKS], 1ull << 32);
+        atomicAdd(&counters[CNT_STAGE_OVERFLOW], 1ull);
     }
 }
 
@@ -871,7 +871,7 @@ __global__ __launch_bounds__(COUNT_THREADS) void k_total_energy(Grid g, Particle
         __syncthreads();
         e = energy_cell<KIND, false>(p, k, S, sT, sTag, nOwn, total);
     } else if (tid == 0) {
-        atomicAdd(&counters[CNT_UNSTAGED_BLOCKS], 1ull << 32);
+        atomicAdd(&counters[CNT_STAGE_OVERFLOW], 1ull);
     }
     static_assert(COUNT_THREADS == SUM_THREADS, "one partial per thread");
     double sum = block_sum(e, sSum);
@@ -946,6 +946,32 @@ __device__ __forceinline__ void rodrigues(V3 axis, double angle, double *R) {
     for (int c = 0; c < 9; c++) R[c] = ((c % 4 == 0 ? 1.0 : 0.0) + K[c] * sa) + fKK[c];
 }
 
+// The move arithmetic of the sweep on explicit inputs, one move per thread: the trial position (translate_wrapped), the
+// rotation matrix of (axis, angle) (rodrigues) and the trial orientation R O (mulm), the functions k_sweep and
+// k_proposals use.  Arrays of n records: pos 3, rot 9, t 3, axis 3, angle 1; outputs pos 3, R 9, rot 9, sin/cos 2.
+__global__ __launch_bounds__(256) void k_apply_moves(M3 dims, M3 inv, size_t n, const double *__restrict__ pos,
+                                                     const double *__restrict__ rot, const double *__restrict__ t,
+                                                     const double *__restrict__ axis, const double *__restrict__ angle,
+                                                     double *outPos, double *outR, double *outRot, double *outSc) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const V3 p = translate_wrapped(dims.m, inv.m, v3(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]),
+                                   v3(t[3 * i], t[3 * i + 1], t[3 * i + 2]));
+    outPos[3 * i] = p.x;
+    outPos[3 * i + 1] = p.y;
+    outPos[3 * i + 2] = p.z;
+    double R[9], O[9], NO[9];
+    rodrigues(v3(axis[3 * i], axis[3 * i + 1], axis[3 * i + 2]), angle[i], R);
+    for (int c = 0; c < 9; c++) O[c] = rot[9 * i + c];
+    mulm(R, O, NO);
+    for (int c = 0; c < 9; c++) {
+        outR[9 * i + c] = R[c];
+        outRot[9 * i + c] = NO[c];
+    }
+    outSc[2 * i] = sin(angle[i]);
+    outSc[2 * i + 1] = cos(angle[i]);
+}
+
 // moves > 0: `moves` proposals per cell at [cell * moves, ...); moves == 0: ms.n per particle, cell c's at
 // [ms.n cellStart[c], ms.n cellStart[c + 1]) (the caller passes the cell of every sorted record in `cellOf`).
 // With several samplers the first draw picks the sampler (Simulation.cpp:456-465); a translation sampler's move
@@ -990,6 +1016,20 @@ __global__ __launch_bounds__(256) void k_proposals(Grid g, Proposals pr, int mov
     if (pr.u) pr.u[i] = philox_unit(rng);  // Simulation.cpp:483, drawn after the move as there
     if (pr.type) pr.type[i] = (uint8_t)type;
     if (pr.acc) pr.acc[i] = 0;
+}
+
+hipError_t launch_apply_moves(hipStream_t s, const double *dims, const double *inv, size_t n, const double *pos,
+                              const double *rot, const double *t, const double *axis, const double *angle,
+                              double *outPos, double *outR, double *outRot, double *outSc) {
+    if (n == 0) return hipSuccess;
+    M3 d, v;
+    for (int c = 0; c < 9; c++) {
+        d.m[c] = dims[c];
+        v.m[c] = inv[c];
+    }
+    hipLaunchKernelGGL(k_apply_moves, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d, v, n, pos, rot, t, axis,
+                       angle, outPos, outR, outRot, outSc);
+    return hipGetLastError();
 }
 
 hipError_t launch_proposals(hipStream_t s, const Grid &g, const Proposals &pr, int moves, const int *cellOf, int n,
@@ -1518,18 +1558,20 @@ __device__ __forceinline__ void sweep_cell_unstaged(const SweepParams &P, const 
             for (int base = 0; base < total; base += NT) {
                 const int j = base + tid;
                 bool h = false;
-                if (j < total && j != kIdx) {
-                    if (KIND == 2) {
-                        const int q = stencil_cell(S.start, S.ncells, j);
-                        const int gj = S.base[q] + (j - S.start[q]);
+                if (j < total) {
+                    // the moved particle is skipped under every image: with one cell across an axis its own cell
+                    // recurs in the stencil (Packing.cpp:383-384 skips the particle's own index in every cell)
+                    const int q = stencil_cell(S.start, S.ncells, j);
+                    const int gj = S.base[q] + (j - S.start[q]);
+                    if (gj != gi) {
                         const V3 p2 = load3(P.p.x, P.p.y, P.p.z, gj);
                         const V3 T = image_translation(sT, image_code(S.ecode[q], P.p.wbits[gj]), wx, wy, wz);
-                        h = poly_hit(P.k, sBeads, sTrial, npos, p2, P.p, gj, T);
-                    } else {
-                        V3 p2, a2;
-                        int code;
-                        load_candidate<KIND, false>(P.p, S, j, p2, a2, code);
-                        h = pair_overlap<KIND>(P.k, f, p2, a2, image_translation(sT, code, wx, wy, wz));
+                        if (KIND == 2) {
+                            h = poly_hit(P.k, sBeads, sTrial, npos, p2, P.p, gj, T);
+                        } else {
+                            const V3 a2 = KIND == 1 ? v3(P.p.o[2][gj], P.p.o[5][gj], P.p.o[8][gj]) : v3(0, 0, 0);
+                            h = pair_overlap<KIND>(P.k, f, p2, a2, T);
+                        }
                     }
                 }
                 tests += (unsigned long long)min(NT, total - base);
@@ -1571,6 +1613,7 @@ constexpr int GROUP = 4;
 constexpr int GQ = GROUPED_QUEUE;
 constexpr int GPLAN_I = 8;       // ints per group plan: kIdx[4], size, trial-pair hits, packed (nw | inside << 3)
 constexpr int GBUF = 3;          // plan / hit-word buffers
+constexpr int PK_ACCEPTED = 1 << 30;  // set on a decided move's particle index in LDS once it is accepted
 static_assert(SWEEP_AUX >= GBUF * GROUP * PLAN_D && SWEEP_AUX_I >= GBUF * GPLAN_I + GBUF, "grouped layout");
 
 template <int KIND>
@@ -1711,7 +1754,6 @@ __device__ __forceinline__ void sweep_cell_grouped(const SweepParams &P, const S
         const int cn = min(MOVE_CHUNK, moves - chunk);
         load_proposals(P, pbase, chunk, cn, pt, pR, pk);
         if (tid < GBUF) hitWords[tid] = 0;
-        unsigned long long chunkAcc = 0;  // bit m: move m of the chunk was accepted (several samplers only)
         __syncthreads();
         // The chunk's groups, in registers of every wavefront: lane m holds the particle of move m and the length
         // of the run of moves on distinct particles that starts there (at most GROUP); the groups follow from
@@ -1863,7 +1905,6 @@ __device__ __forceinline__ void sweep_cell_grouped(const SweepParams &P, const S
                 if (!hit) acc |= 1u << k;
             }
             accepted += (unsigned long long)__popc(acc);
-            if (WAVES < 8) chunkAcc |= (unsigned long long)acc << m0;
             leftCell += (unsigned long long)(gs - __popc(insideMask));
             tests += (unsigned long long)total * (unsigned long long)__popc(insideMask);
             const bool replan = (acc & nextUses) != 0;
@@ -1879,6 +1920,9 @@ __device__ __forceinline__ void sweep_cell_grouped(const SweepParams &P, const S
                     dst[ki] = q[lane];
                 } else if (lane == 6) {
                     S.code[ki] = wrap_code((word0 >> (4 * k)) & 7);
+                } else if (lane == 7 && planner) {
+                    // marks the move accepted: a decided move's particle index is not read again in this chunk
+                    pk[m0 + k] |= PK_ACCEPTED;
                 } else if (planner && lane >= 8 && lane < 17) {
                     oo[(lane - 8) * OWN_CAP + ki] = q[13 + (lane - 8)];
                 }
@@ -1900,9 +1944,10 @@ __device__ __forceinline__ void sweep_cell_grouped(const SweepParams &P, const S
                 nReplans += replan ? 1 : 0;
             }
         }
-        // several samplers: the accepted moves of the chunk, for the per-sampler counters
-        if (WAVES < 8 && P.pr.acc && tid < cn && ((chunkAcc >> tid) & 1)) P.pr.acc[pbase + chunk + tid] = 1;
         __syncthreads();
+        // per-proposal acceptance flags (several samplers' counters, decision records); each lane reads the entry
+        // it alone rewrites with the next chunk's proposals
+        if (P.pr.acc && tid < cn && (pk[tid] & PK_ACCEPTED)) P.pr.acc[pbase + chunk + tid] = 1;
     }
     if (clk) {
         clk[3] = wall_clock64();
@@ -2017,6 +2062,12 @@ __device__ __forceinline__ void sweep_cell_soft(const SweepParams &P, const Sten
                 SCFirst fO = make_first<KIND>(P.k, opos, oax);
                 for (int j = tid; j < total; j += NT) {
                     if (j == kIdx) continue;
+                    // unstaged stencils may hold the own cell again under another image (one cell across an
+                    // axis): the moved particle is skipped there too (Packing.cpp:383-384)
+                    if (!STAGED) {
+                        const int q = stencil_cell(S.start, S.ncells, j);
+                        if (S.base[q] + (j - S.start[q]) == gi) continue;
+                    }
                     V3 p2, a2;
                     int code;
                     load_candidate<KIND, STAGED>(P.p, S, j, p2, a2, code);
@@ -2036,6 +2087,9 @@ __device__ __forceinline__ void sweep_cell_soft(const SweepParams &P, const Sten
                     } else {
                         eFinal += soft_energy(P.k, fN, p2, a2, TN);
                         eInitial += soft_energy(P.k, fO, p2, a2, TO);
+                        // CompoundInteraction: an overlap of the hard parts makes the move's energy infinite
+                        // (Packing::calculateMoveOverlapEnergy, Packing.cpp:304-323)
+                        if (P.k.hardCore && pair_overlap<KIND>(P.k, fN, p2, a2, TN)) eFinal = INFINITY;
                     }
                 }
                 tests += (unsigned long long)total;
@@ -2059,7 +2113,8 @@ __device__ __forceinline__ void sweep_cell_soft(const SweepParams &P, const Sten
             if (P.counting) {
                 if (fin - ini > 0) continue;  // counts are exact in doubles
                 overlapDelta += (long long)(fin - ini);
-            } else if (!(pu[mi] <= exp(-(fin - ini) / P.temperature))) {
+            } else if (!(fin < INFINITY) || !(pu[mi] <= exp(-(fin - ini) / P.temperature))) {
+                // an overlap rejects the move outright (u = 0 would pass u <= exp(-inf))
                 continue;
             }
 

@@ -160,6 +160,7 @@ RP_HD double segment_distance2(const V3 &s11, const V3 &u, double a, const V3 &s
 struct PairConsts {
     int kind;            // 0 sphere, 1 spherocylinder, 2 polysphere
     int soft;            // 0 hard core; 1 WCA on the centre distance; 2 WCA on the segment distance; 3 LJ; 4 SIC
+    int hardCore;        // with a soft kind: 1 = the hard test applies as well (CompoundInteraction)
     double contact2;     // sphere: pow(2r, 2); spherocylinder: 4*r*r
     double range2;       // spherocylinder: pow(2r + L, 2)
     double halfLen;      // spherocylinder: 0.5*L (cap offset, SpherocylinderTraits.cpp:24-26)

@@ -47,10 +47,18 @@ def _setup_common(lib):
     lib.orc_rescale.restype = None
     lib.orc_rotation.argtypes = [_dp, ctypes.c_double, _dp]
     lib.orc_rotation.restype = None
+    lib.orc_rotation_sc.argtypes = [_dp, ctypes.c_double, ctypes.c_double, _dp]
+    lib.orc_rotation_sc.restype = None
+    lib.orc_rotate.argtypes = [ctypes.c_size_t, _dp, _dp, _dp]
+    lib.orc_rotate.restype = None
     lib.orc_nematic_p2.argtypes = [ctypes.c_size_t, _dp]
     lib.orc_nematic_p2.restype = ctypes.c_double
     lib.orc_pair_guarded.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_size_t, _dp, _u8p,
                                      _u8p, ctypes.c_double]
+    _ip = ctypes.POINTER(ctypes.c_int)
+    lib.orc_replay_sweep.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, ctypes.c_size_t, _dp, _dp,
+                                     _ip, _ip, _dp, ctypes.c_int, ctypes.c_int, _ip, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_size_t, _ip, _dp, _dp, _u8p, _dp, _dp]
 
 
 class Checker:
@@ -144,6 +152,16 @@ def _setup_ref(lib):
                                ctypes.c_double, ctypes.c_ulong, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t, _dp, _dp, _dp, _dp, _dp, _dp]
     lib.ref_mc_run.restype = ctypes.c_long
+    lib.ref_box_info.argtypes = [_dp, _dp, _dp, _dp]
+    lib.ref_box_info.restype = None
+    lib.ref_translate.argtypes = [_dp, ctypes.c_size_t, _dp, _dp, _dp]
+    lib.ref_translate.restype = None
+    lib.ref_rotation.argtypes = [ctypes.c_size_t, _dp, _dp, ctypes.c_int, _dp]
+    lib.ref_rotation.restype = None
+    lib.ref_rotate.argtypes = [ctypes.c_size_t, _dp, _dp, _dp]
+    lib.ref_rotate.restype = None
+    lib.ref_try_scaling.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, _dp, ctypes.c_size_t, _dp,
+                                    _dp, _dp, _dp]
 
 
 def ref(perf: bool = False) -> Checker:
@@ -232,3 +250,31 @@ def poly_pair_decisions(centres, radii, rec) -> np.ndarray:
     oracle().lib.orc_poly_pair_decisions(len(c), c.ctypes.data_as(_dp), r.ctypes.data_as(_dp), len(rec),
                                          rec.ctypes.data_as(_dp), out.ctypes.data_as(_u8p))
     return out
+
+
+def replay_sweep(kind: int, radius: float, length: float, rec: dict):
+    """Replays a device decision record (GpuPacking.record_sweep) through the restated Packing::tryMove: returns
+    (decisions per proposal: 1 accepted, 0 overlap, 2 left its cell, 3 not run; positions after; orientations after)."""
+    lib = oracle().lib
+    ip = ctypes.POINTER(ctypes.c_int)
+    n = len(rec["pre_pos"])
+    npr = rec["n_proposals"]
+    dec = np.zeros(npr, dtype=np.uint8)
+    post_pos = np.zeros((n, 3))
+    post_rot = np.zeros((n, 9))
+    arr = {k: np.ascontiguousarray(rec[k]) for k in ("pre_pos", "pre_rot", "prop_t", "prop_rot")}
+    box = np.ascontiguousarray(rec["box"], dtype=np.float64).reshape(9)
+    cells = np.ascontiguousarray(rec["cells"], dtype=np.int32)
+    shift = np.ascontiguousarray(rec["shift"], dtype=np.float64)
+    order = np.ascontiguousarray(rec["colour_order"], dtype=np.int32)
+    cell_of = np.ascontiguousarray(rec["pre_cell"], dtype=np.int32)
+    prop_k = np.ascontiguousarray(rec["prop_k"], dtype=np.int32)
+    rc = lib.orc_replay_sweep(kind, radius, length, box.ctypes.data_as(_dp), n, arr["pre_pos"].ctypes.data_as(_dp),
+                              arr["pre_rot"].ctypes.data_as(_dp), cell_of.ctypes.data_as(ip), cells.ctypes.data_as(ip),
+                              shift.ctypes.data_as(_dp), rec["period"], len(order), order.ctypes.data_as(ip),
+                              rec["moves_per_cell"], rec["n_samplers"], npr, prop_k.ctypes.data_as(ip),
+                              arr["prop_t"].ctypes.data_as(_dp), arr["prop_rot"].ctypes.data_as(_dp),
+                              dec.ctypes.data_as(_u8p), post_pos.ctypes.data_as(_dp), post_rot.ctypes.data_as(_dp))
+    if rc != 0:
+        raise RuntimeError(f"orc_replay_sweep failed ({rc})")
+    return dec, post_pos, post_rot

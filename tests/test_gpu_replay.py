@@ -1,0 +1,92 @@
+"""Every decision of the device sweep replayed through the oracle's restatement of the reference's move test.
+
+rp_record_sweep runs one cycle's particle moves (all checkerboard colour passes, no box move) and returns the state
+before them, every proposal (particle, translation, rotation matrix) with the device's acceptance flag, and the state
+after them.  The oracle replays the proposals in the same order through Packing::tryMove / countParticleOverlaps /
+acceptMove on the reference's neighbour grid (Packing.cpp:136-164, 373-429, 278-302) and must reach the same
+decision for every proposal and the same state, bit for bit.  The eight-wavefront kernel's plain and pipelined
+variants write no flags (no registers to spare); for them the state after the moves is the check: each accepted
+move changes its particle's pose, so equal end states mean equal decisions.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib
+import rampack_amd as ra
+from rampack_amd import lattice
+
+pytestmark = pytest.mark.gpu
+
+SC = ra.Shape("spherocylinder", radius=0.5, length=5.0)
+HS = ra.Shape("sphere", radius=0.5)
+
+
+def check_record(shape, rec, flags=True):
+    kind = 0 if shape.kind == "sphere" else 1
+    dec, post_pos, post_rot = oracle_lib.replay_sweep(kind, shape.radius, shape.length, rec)
+    run = dec != 3
+    counts = np.bincount(dec[run], minlength=3)
+    print(f"proposals {run.sum()}: accepted {counts[1]}, overlap {counts[0]}, left cell {counts[2]}", flush=True)
+    assert counts[1] > 0 and counts[0] > 0
+    if flags:
+        mismatch = np.flatnonzero(rec["accepted"][run] != (dec[run] == 1))
+        assert mismatch.size == 0, f"{mismatch.size} decisions differ, first proposals {mismatch[:10]}"
+        assert not rec["accepted"][~run].any()
+    assert np.array_equal(post_pos, rec["post_pos"]), np.flatnonzero((post_pos != rec["post_pos"]).any(axis=1))[:10]
+    assert np.array_equal(post_rot, rec["post_rot"])
+    return counts
+
+
+def rod_start(n_side=(18, 18, 4)):
+    return lattice.spherocylinder_lattice(n_side=n_side, spacing=(1.45, 1.45, 6.3))
+
+
+def test_replay_hard_spheres_c1():
+    """Hard spheres N = 1000 on the sc lattice at packing fraction 0.30, NpT p* = 5 (sparse cells: one move per
+    particle per cycle), recorded at the start and after 300 cycles with box moves."""
+    box, pos, rot = lattice.hard_sphere_lattice(10, 0.30)
+    pk = ra.GpuPacking(HS, box, pos, rot, seed=1234)
+    pk.set_environment(1.0, 5.0, 0.1, None, box_move=("log", 0.01))
+    check_record(HS, pk.record_sweep())
+    pk.run_cycles(300, adapt_steps=True)
+    for _ in range(3):
+        check_record(HS, pk.record_sweep())
+
+
+@pytest.mark.parametrize("shape_case", [(1, 0), (2, 0), (4, 0), (2, 1), (4, 1), (2, 2), (4, 2), (8, 0), (8, 1),
+                                        (8, 2)])
+def test_replay_every_launch_shape(shape_case, monkeypatch):
+    """Spherocylinders L/D = 5 at p* = 2 on every launch shape: wavefronts per cell x (plain, pipelined, grouped)."""
+    waves, pipe = shape_case
+    monkeypatch.setenv("RAMPACK_SWEEP_WAVES", str(waves))
+    monkeypatch.setenv("RAMPACK_SWEEP_PIPELINE", str(pipe))
+    box, pos, rot = rod_start()
+    pk = ra.GpuPacking(SC, box, pos, rot, seed=31)
+    pk.set_environment(1.0, 2.0, 0.2, None, box_move=("delta_triclinic", 0.01))
+    pk.run_cycles(40, adapt_steps=True)
+    for _ in range(2):
+        check_record(SC, pk.record_sweep(), flags=not (waves == 8 and pipe != 2))
+
+
+def test_replay_rods_n1e5_default_launch():
+    """N = 1e5 spherocylinders from the AA-stacked smectic at packing fraction 0.45, NpT p* = 2, the default
+    (grouped eight-wavefront) launch, after 100 cycles with adaptation."""
+    box, pos, rot = lattice.spherocylinder_smectic(100000, 0.45)
+    pk = ra.GpuPacking(SC, box, pos, rot, seed=20261202)
+    pk.set_environment(1.0, 2.0, 0.1, None, box_move=("delta_triclinic", 0.05))
+    pk.run_cycles(100, adapt_steps=True)
+    assert pk.timing()["sweep_waves"] == 8
+    check_record(SC, pk.record_sweep())
+
+
+def test_replay_one_cell_axis_self_image():
+    """A box one checkerboard cell high along the rods (height 6.15, between one and two interaction ranges): the
+    own cell recurs in the stencil under the images -1 and +1, and a rod moved along its axis comes within contact of
+    its own periodic image, which the reference never tests (Packing.cpp:383-384 skips the particle by index)."""
+    box, pos, rot = lattice.spherocylinder_lattice((17, 17, 1), (1.5, 1.5, 6.15))
+    pk = ra.GpuPacking(SC, box, pos, rot, seed=5)
+    pk.set_environment(1.0, 1.0, 0.3, None, None)
+    for _ in range(3):
+        rec = pk.record_sweep()
+        assert rec["cells"][2] == 1
+        check_record(SC, rec)

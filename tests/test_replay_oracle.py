@@ -1,0 +1,179 @@
+"""The oracle's sweep replay (orc_replay_sweep, the checker of the device's decision records) pinned on the CPU.
+
+The replay searches candidates through the restated neighbour grid (cells, reflected layers, translations) and keeps
+the grid up to date as moves are accepted.  Here every decision is recomputed without any grid: each proposal's
+trial pose (pos + t wrapped back into the box, R O) is tested against every other particle under all 27 periodic
+images with the oracle's bit-exact pair test, and the active-cell rule is evaluated from its definition.  Python
+floats are IEEE doubles and every operation below is one rounding, in the reference's order (Matrix.tpp:105-113,
+Shape.cpp:9-12, PeriodicBoundaryConditions.cpp:11-22), so the trial poses are bit-identical to the restatement's.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib
+from rampack_amd import lattice
+
+
+def mulv(m, v):
+    r = []
+    for i in range(3):
+        s = 0.0
+        s += m[3 * i] * v[0]
+        s += m[3 * i + 1] * v[1]
+        s += m[3 * i + 2] * v[2]
+        r.append(s)
+    return r
+
+
+def mulm(a, b):
+    out = []
+    for i in range(3):
+        for j in range(3):
+            s = 0.0
+            s += a[3 * i] * b[j]
+            s += a[3 * i + 1] * b[3 + j]
+            s += a[3 * i + 2] * b[6 + j]
+            out.append(s)
+    return out
+
+
+def translate_wrapped(dims, inv, p, t):
+    q = [p[d] + t[d] for d in range(3)]
+    rel = mulv(inv, q)
+    c = [0.0, 0.0, 0.0]
+    for d in range(3):
+        while rel[d] + c[d] < 0:
+            c[d] += 1
+        while rel[d] + c[d] >= 1:
+            c[d] -= 1
+    k = mulv(dims, c)
+    return [q[d] + k[d] for d in range(3)]
+
+
+def domain_of(inv, p, cells, shift):
+    rel = mulv(inv, p)
+    c = []
+    for d in range(3):
+        x = rel[d]
+        w = x < shift[d]
+        y = (x - shift[d]) + (1.0 if w else 0.0)
+        cc = cells[d] - 1 if w else int(y * cells[d])
+        c.append(min(max(cc, 0), cells[d] - 1))
+    return c[0] + cells[0] * (c[1] + cells[1] * c[2])
+
+
+def brute_replay(kind, radius, length, rec):
+    """Every proposal in the record's order; overlap = any other particle under any of the 27 images."""
+    orc = oracle_lib.oracle()
+    dims = [float(x) for x in np.asarray(rec["box"]).reshape(9)]
+    inv = np.zeros(9)
+    orc.lib.orc_box_info(np.ascontiguousarray(dims).ctypes.data_as(oracle_lib._dp), inv.ctypes.data_as(oracle_lib._dp),
+                         np.zeros(3).ctypes.data_as(oracle_lib._dp), np.zeros(1).ctypes.data_as(oracle_lib._dp))
+    inv = [float(x) for x in inv]
+    T = np.array([mulv(dims, [float(i // 9 - 1), float((i // 3) % 3 - 1), float(i % 3 - 1)]) for i in range(27)])
+    pos = [list(map(float, p)) for p in rec["pre_pos"]]
+    rot = [list(map(float, r)) for r in rec["pre_rot"]]
+    n = len(pos)
+    cells, shift, period = rec["cells"], rec["shift"], rec["period"]
+    ncell = cells[0] * cells[1] * cells[2]
+    start = np.searchsorted(rec["pre_cell"], np.arange(ncell + 1))
+    dec = np.full(rec["n_proposals"], 3, dtype=np.uint8)
+    for colour in rec["colour_order"]:
+        px, py, pz = colour % period, (colour // period) % period, colour // (period * period)
+        for cz in range(pz, cells[2], period):
+            for cy in range(py, cells[1], period):
+                for cx in range(px, cells[0], period):
+                    cell = cx + cells[0] * (cy + cells[1] * cz)
+                    own0, n_own = start[cell], start[cell + 1] - start[cell]
+                    if n_own == 0:
+                        continue
+                    mpc = rec["moves_per_cell"]
+                    moves = mpc if mpc > 0 else n_own * rec["n_samplers"]
+                    pbase = cell * mpc if mpc > 0 else own0 * rec["n_samplers"]
+                    for m in range(moves):
+                        q = pbase + m
+                        gi = own0 + int(rec["prop_k"][q])
+                        npos = translate_wrapped(dims, inv, pos[gi], [float(x) for x in rec["prop_t"][q]])
+                        no = mulm([float(x) for x in rec["prop_rot"][q]], rot[gi])
+                        if domain_of(inv, npos, cells, shift) != cell:
+                            dec[q] = 2
+                            continue
+                        others = [j for j in range(n) if j != gi]
+                        recs = np.zeros((len(others) * 27, 27))
+                        recs[:, 0:3] = npos
+                        recs[:, 3:12] = no
+                        recs[:, 12:15] = np.repeat(np.array([pos[j] for j in others]), 27, axis=0)
+                        recs[:, 15:24] = np.repeat(np.array([rot[j] for j in others]), 27, axis=0)
+                        recs[:, 24:27] = np.tile(T, (len(others), 1))
+                        if orc.pair_decisions(kind, radius, length, recs).any():
+                            dec[q] = 0
+                            continue
+                        dec[q] = 1
+                        pos[gi], rot[gi] = npos, no
+    return dec, np.array(pos), np.array(rot)
+
+
+def synthetic_record(kind, box, pos, rot, cells, seed, moves_per_cell, trans, angle):
+    """A record as the device would write it: records sorted by checkerboard cell, random colour order and
+    proposals (uniform particle pick, translation in [-trans, trans)^3, rotation by up to `angle` about a random
+    axis)."""
+    orc = oracle_lib.oracle()
+    rng = np.random.Generator(np.random.PCG64(seed))
+    dims = [float(x) for x in np.asarray(box).reshape(9)]
+    inv = np.zeros(9)
+    orc.lib.orc_box_info(np.ascontiguousarray(dims).ctypes.data_as(oracle_lib._dp), inv.ctypes.data_as(oracle_lib._dp),
+                         np.zeros(3).ctypes.data_as(oracle_lib._dp), np.zeros(1).ctypes.data_as(oracle_lib._dp))
+    shift = rng.random(3) / np.array(cells)
+    cell = np.array([domain_of(list(inv), list(map(float, p)), cells, shift) for p in pos])
+    order = np.argsort(cell, kind="stable")
+    pos, rot, cell = pos[order], rot[order], cell[order]
+    ncell = int(np.prod(cells))
+    count = np.bincount(cell, minlength=ncell)
+    n_prop = ncell * moves_per_cell if moves_per_cell > 0 else len(pos)
+    prop_k = np.zeros(n_prop, dtype=np.int32)
+    for c in range(ncell):
+        lo, hi = (c * moves_per_cell, (c + 1) * moves_per_cell) if moves_per_cell > 0 else \
+            (count[:c].sum(), count[:c + 1].sum())
+        if count[c]:
+            prop_k[lo:hi] = rng.integers(0, count[c], hi - lo)
+    prop_t = rng.uniform(-trans, trans, (n_prop, 3))
+    prop_rot = np.zeros((n_prop, 9))
+    for i in range(n_prop):
+        ax = rng.normal(size=3)
+        ax /= np.linalg.norm(ax)
+        orc.lib.orc_rotation(ax.ctypes.data_as(oracle_lib._dp), float(rng.uniform(-angle, angle)),
+                             prop_rot[i].ctypes.data_as(oracle_lib._dp))
+    colours = 8
+    return {"box": np.asarray(box), "pre_pos": np.ascontiguousarray(pos), "pre_rot": np.ascontiguousarray(rot),
+            "pre_cell": cell.astype(np.int32), "cells": list(cells), "shift": shift, "period": 2,
+            "colour_order": list(rng.permutation(colours)), "moves_per_cell": moves_per_cell, "n_samplers": 1,
+            "n_proposals": n_prop, "prop_k": prop_k, "prop_t": prop_t, "prop_rot": prop_rot}
+
+
+@pytest.mark.parametrize("case", ["spheres_sparse", "rods_dense", "rods_one_cell_axis"])
+def test_replay_equals_all_image_brute_force(case):
+    if case == "spheres_sparse":
+        kind, r, l = 0, 0.5, 0.0
+        box, pos, rot = lattice.hard_sphere_lattice(5, 0.40)
+        cells, mpc, trans, angle = (4, 4, 4), 0, 0.12, 0.0
+    elif case == "rods_dense":
+        kind, r, l = 1, 0.5, 5.0
+        # the box's largest height reaches four interaction ranges: the reference keeps its neighbour grid
+        box, pos, rot = lattice.spherocylinder_lattice((20, 4, 2), (1.3, 1.6, 6.15), tilt_seed=3, max_tilt=0.15)
+        cells, mpc, trans, angle = (2, 1, 2), 30, 0.25, 0.3
+    else:
+        # one checkerboard cell across z: the box height 6.15 lies between one and two interaction ranges (6), so
+        # a rod moved by up to 0.3 along its axis comes within contact of its own periodic image
+        kind, r, l = 1, 0.5, 5.0
+        box, pos, rot = lattice.spherocylinder_lattice((17, 17, 1), (1.5, 1.5, 6.15))
+        cells, mpc, trans, angle = (2, 2, 1), 0, 0.3, 0.05
+    rec = synthetic_record(kind, box, pos, rot, cells, seed=17, moves_per_cell=mpc, trans=trans, angle=angle)
+    dec, post_pos, post_rot = oracle_lib.replay_sweep(kind, r, l, rec)
+    bdec, bpos, brot = brute_replay(kind, r, l, rec)
+    assert np.array_equal(dec, bdec)
+    assert np.array_equal(post_pos, bpos) and np.array_equal(post_rot, brot)
+    run = dec != 3
+    assert (dec[run] == 1).sum() > 0 and (dec[run] == 0).sum() > 0, np.bincount(dec)
+    if case == "rods_one_cell_axis":
+        # some moves are rejected only by the particle's own image, which the replay must not test
+        assert (dec == 1).sum() > 0

@@ -230,3 +230,39 @@ def test_lj_and_sic_total_energies(soft, eps, sigma):
     if soft == 4 and oracle_lib.ref_available():
         _, e_ref = oracle_lib.ref().snapshot_stats(0, 0.5, 0.0, box_, pos, rot, soft=soft, eps=eps, sigma=sigma)
         assert e_orc == pytest.approx(e_ref, rel=1e-12)
+
+
+HARD_CORE = 0x100
+
+
+@pytest.mark.parametrize("radius,sigma", [(0.5, 1.0), (0.45, 1.0), (0.5, 0.8)])
+def test_compound_sphere_counts_and_energies_match_reference(radius, sigma):
+    """CompoundInteraction of a hard sphere and a WCA sphere (CompoundInteraction.cpp:9-60) in the oracle against the
+    reference's own CompoundInteraction: the overlap count is the hard part's, the energy the soft part's (also over
+    overlapping pairs, which getTotalEnergy sums like any other), the range the larger one."""
+    ref = need_ref()
+    box = np.array([[9.0, 0.4, -0.3], [0.0, 8.5, 0.2], [0.0, 0.0, 9.5]])
+    pos, rot = corpus.random_snapshot(600, 21, box, kind="sphere")
+    soft = 1 | HARD_CORE
+    n_orc, e_orc = oracle_lib.oracle().snapshot_stats(0, radius, 0.0, box, pos, rot, soft=soft, eps=1.0, sigma=sigma)
+    n_ref, e_ref = ref.snapshot_stats(0, radius, 0.0, box, pos, rot, soft=soft, eps=1.0, sigma=sigma)
+    n_hard, _ = ref.snapshot_stats(0, radius, 0.0, box, pos, rot)
+    _, e_soft = ref.snapshot_stats(0, radius, 0.0, box, pos, rot, soft=1, eps=1.0, sigma=sigma)
+    assert n_orc == n_ref == n_hard and n_ref > 0
+    assert e_orc == pytest.approx(e_ref, rel=1e-12)
+    assert e_ref == pytest.approx(e_soft, rel=1e-12)
+
+
+def test_compound_segment_wca_rods_count_and_energy():
+    """Hard spherocylinders that also carry WCA on the segment distance (no reference counterpart of the soft part):
+    the count equals the plain hard rods', the energy the segment WCA's, which is pinned by brute force."""
+    box = np.array([[16.0, 0.5, -0.4], [0.0, 15.0, 0.3], [0.0, 0.0, 17.0]])
+    pos, rot = corpus.random_snapshot(120, 12, box)
+    orc = oracle_lib.oracle()
+    n_c, e_c = orc.snapshot_stats(1, 0.5, 5.0, box, pos, rot, soft=2 | HARD_CORE, eps=1.0, sigma=1.0)
+    n_h, _ = orc.snapshot_stats(1, 0.5, 5.0, box, pos, rot)
+    assert n_c == n_h and n_h > 0
+    assert e_c == pytest.approx(brute_total_energy(box, pos, rot, 1, 1.0, 1.0, 5.0), rel=1e-9)
+    if oracle_lib.ref_available():
+        n_ref, _ = oracle_lib.ref().snapshot_stats(1, 0.5, 5.0, box, pos, rot)
+        assert n_c == n_ref
