@@ -4,8 +4,11 @@ One step = one MC cycle of rampack's Simulation::performCycle (src/core/Simulati
 rototranslations (eight checkerboard colour passes) plus one delta_triclinic box move, the same unit the reference
 reports as "Cycles per second" (src/frontend/modes/CasinoMode.cpp:405-412).  Trial moves/s = N * cycles / time.
 
-Start state: AA-stacked layers of parallel rods (packing fraction 0.45) equilibrated by the untimed warm-up cycles at
-p* = 2 with step adaptation; the timed cycles run without adaptation.  Inputs are resident in HBM before timing.
+Input T: AA-stacked layers of parallel rods (packing fraction 0.45) relaxed at p* = 2 by a fixed number of cycles
+with step adaptation (--equilibrate, default 3000), seeded, so every run starts the timed region from the same state
+bit for bit (its SHA-256 is printed) whatever --warmup is; the warm-up and timed cycles then run with the adapted
+steps frozen.  The state is generated on the device rather than loaded from a file (a RAMSNAP file of 1e5 rods is
+about 20 MB).  Inputs are resident in HBM before timing.
 
 Roofline: the dominant kernel is the sweep (k_sweep).  Its algorithmic traffic per trial move is
 (27 + a) * 96 B (the reference's 27-cell stencil of FP64 particle records plus the write-back of accepted moves, a =
@@ -14,10 +17,11 @@ measured with HIP events on the launch stream over the timed cycles (events are 
 the HBM bytes one sweep launch moves, from rocprofv3 FETCH_SIZE and WRITE_SIZE counters on this workload scaled by
 the same counters on a 256 MiB device copy (SWEEP_TRAFFIC below).
 
-CPU baseline: the reference's own hot path (oracle/_ref/libref_perf.so: Packing, NeighbourGrid,
+CPU baseline: the reference's own hot path (oracle/_ref/libref_perf*.so: Packing, NeighbourGrid,
 SpherocylinderTraits, RototranslationSampler, TriclinicDeltaScaler and DomainDecomposition compiled from the
-reference sources with -O3) driven by the restated cycle loop with OpenMP domain decomposition, on the same state,
-timed for a bounded number of cycles on the host cores.
+reference sources with -O3, for x86-64-v4 where the host has AVX-512, else x86-64-v3) driven by the restated cycle
+loop with OpenMP domain decomposition over every core this process may run on, on the same state, timed for a
+bounded number of cycles.
 """
 from __future__ import annotations
 
@@ -85,10 +89,38 @@ def cpu_baseline(box, pos, rot, trans_step, rot_step, box_step, cycles, threads,
                                 trans_step, rot_step, 1, box_step, seed, dd=dd, threads=threads, therm=0, avg=cycles,
                                 avg_every=0)
     rate = len(pos) * cycles / res["seconds"]
+    build = "x86-64-v4" if "v4" in os.path.basename(oracle_lib.REF_PERF_V4_PATH) and \
+        os.path.exists(oracle_lib.REF_PERF_V4_PATH) and oracle_lib.host_has_avx512() else "x86-64-v3"
     return {"value": rate, "unit": "trial moves/s", "cores": threads, "kind": "reference",
-            "sample": f"{cycles} cycles of the same N={len(pos)} state (after GPU warm-up), NpT p*={wl['pressure']}, "
-                      f"rototranslation + delta_triclinic, OpenMP domain divisions {list(dd)}, "
-                      f"{res['seconds']:.1f} s"}
+            "cpu_model": cpu_model(), "build": f"-O3 -march={build} -fopenmp",
+            "sample": f"{cycles} cycles of the same N={len(pos)} state (the GPU's state after its timed cycles), "
+                      f"NpT p*={wl['pressure']}, rototranslation + delta_triclinic, OpenMP domain divisions "
+                      f"{list(dd)}, {res['seconds']:.1f} s"}
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            return next((line.split(":", 1)[1].strip() for line in f if line.startswith("model name")), "unknown")
+    except OSError:
+        return "unknown"
+
+
+def host_cores() -> int:
+    """The cores this process may run on (its affinity mask), not the machine's count."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def state_sha256(box, pos, rot) -> str:
+    import hashlib
+    import numpy as np
+    h = hashlib.sha256()
+    for a in (box, pos, rot):
+        h.update(np.ascontiguousarray(a, dtype=np.float64).tobytes())
+    return h.hexdigest()
 
 
 def workload(name: str, n: int):
@@ -99,7 +131,7 @@ def workload(name: str, n: int):
     if name == "spherocylinders":
         box, pos, rot = lattice.spherocylinder_smectic(n, 0.45)
         return dict(shape=ra.Shape("spherocylinder", radius=0.5, length=5.0), state=(box, pos, rot), pressure=2.0,
-                    trans_step=0.1, box_step=1e-4, kind=1, radius=0.5, length=5.0, beads=None, centres=1,
+                    trans_step=0.1, box_step=0.002, kind=1, radius=0.5, length=5.0, beads=None, centres=1,
                     divisions=near_cubic_divisions,
                     text=f"hard spherocylinders L/D=5 (r=0.5, L=5), N={len(pos)}, NpT p*=2.0 T=1, rototranslation "
                          f"+ delta_triclinic box move per cycle, plain FP64 checkerboard sweep")
@@ -137,7 +169,9 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
-    ap.add_argument("--warmup", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--equilibrate", type=int, default=3000,
+                    help="cycles with step adaptation that turn the lattice into input T (not timed, seeded)")
     ap.add_argument("--n", type=int, default=100000)
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--cpu-cycles", type=int, default=-1, help="-1: sized to ~20 s of CPU work; 0: skip")
@@ -163,9 +197,12 @@ def main():
     pk = ra.GpuPacking(wl["shape"], box, pos, rot, seed=args.seed)
     pk.set_environment(1.0, wl["pressure"], wl["trans_step"], None, box_move=("delta_triclinic", wl["box_step"]))
     t0 = time.perf_counter()
-    warm = pk.run_cycles(args.warmup, adapt_steps=True)
-    log(f"warm-up {args.warmup} cycles in {time.perf_counter() - t0:.2f} s: phi={pk.packing_fraction():.4f} "
-        f"steps trans={warm['trans_step']:.4f} rot={warm['rot_step']:.4f} box={warm['box_step']:.4f}")
+    eq = pk.run_cycles(args.equilibrate, adapt_steps=True)
+    sha = state_sha256(*pk.download())
+    log(f"input T: {args.equilibrate} adapting cycles in {time.perf_counter() - t0:.2f} s: "
+        f"phi={pk.packing_fraction():.4f} steps trans={eq['trans_step']:.4f} rot={eq['rot_step']:.4f} "
+        f"box={eq['box_step']:.5f}, box moves {eq['box_accepted']}/{eq['box_attempted']}, sha256 {sha[:16]}")
+    warm = pk.run_cycles(args.warmup)
     phi_start = pk.packing_fraction()
     before = pk.run_cycles(0)
 
@@ -220,7 +257,7 @@ def main():
     cpu = None
     if args.cpu_cycles != 0:
         b, p, r = pk.download()
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = args.cpu_threads or host_cores()
         cycles = args.cpu_cycles
         if cycles < 0:
             probe = cpu_baseline(b, p, r, tb["trans_step"], tb["rot_step"], tb["box_step"], 2, threads, args.seed,
@@ -252,6 +289,9 @@ def main():
             "workload": wl["text"],
             "n_particles": len(pos),
             "pressure": wl["pressure"],
+            "input_T": {"lattice": "AA smectic, packing fraction 0.45", "equilibration_cycles": args.equilibrate,
+                        "seed": args.seed, "sha256": sha, "trans_step": eq["trans_step"], "rot_step": eq["rot_step"],
+                        "box_step": eq["box_step"]},
             "packing_fraction_timed_start": round(phi_start, 5),
             "packing_fraction_timed_end": round(phi_end, 5),
             "acceptance": round(acc_rate, 4),

@@ -933,6 +933,14 @@ void orc_rotation(const double *axis, double angle, double *out) {
     std::memcpy(out, R.m, 72);
 }
 
+// sin and cos of n angles from the C library, as Matrix<3,3>::rotation takes them (Matrix.tpp:442-443).
+void orc_sincos(std::size_t n, const double *angle, double *out) {
+    for (std::size_t i = 0; i < n; i++) {
+        out[2 * i] = std::sin(angle[i]);
+        out[2 * i + 1] = std::cos(angle[i]);
+    }
+}
+
 // The same with sin and cos supplied (the device evaluates them with its own math library).
 void orc_rotation_sc(const double *axis, double s, double c, double *out) {
     Mat R = rotationSC({{axis[0], axis[1], axis[2]}}, s, c);

@@ -1585,7 +1585,7 @@ __device__ __forceinline__ void sweep_cell_unstaged(const SweepParams &P, const 
             }
             if (hit) continue;
             accepted++;
-            if (WAVES < 8 && P.pr.acc && !P.fixMode && tid == 0) P.pr.acc[pbase + chunk + mi] = 1;
+            if (P.pr.acc && !P.fixMode && tid == 0) P.pr.acc[pbase + chunk + mi] = 1;
             if (tid == 0) {
                 P.p.x[gi] = npos.x;
                 P.p.y[gi] = npos.y;

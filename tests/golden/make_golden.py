@@ -155,6 +155,34 @@ def dimer_run(args):
     return float(out["samples"][:, 0].mean())
 
 
+# NpT hard spheres (r = 0.5) with a WCA shell (eps = 1, sigma = 1): the reference's CompoundInteraction of SphereTraits
+# and RepulsiveLennardJonesInteraction; delta_V box moves; packing fraction and energy per particle per independent run
+COMPOUND_NPT = dict(name="compound_sphere_npt", n_side=8, phi0=0.3, temperature=1.0, pressure=2.0, trans_step=0.1,
+                    box_step=5.0, therm=3000, avg=9000, every=20, seeds=list(range(300, 316)),
+                    device_seeds=list(range(400, 412)))
+
+
+def compound_run(args):
+    """One reference NpT run of the compound-sphere state point: its mean packing fraction and energy per particle."""
+    st, seed = args
+    from rampack_amd import lattice
+    ref = oracle_lib.ref()
+    box, pos, rot = lattice.hard_sphere_lattice(st["n_side"], st["phi0"])
+    out = oracle_lib.ref_mc_run(ref, 0, 0.5, 0.0, box, pos, rot, st["temperature"], st["pressure"], st["trans_step"],
+                                0.0, 3, st["box_step"], seed, therm=st["therm"], avg=st["avg"], avg_every=st["every"],
+                                soft=1 | 0x100, eps=1.0, sigma=1.0)
+    return float(out["samples"][:, 0].mean()), float(out["samples"][:, 2].mean())
+
+
+def reference_compound_statistics(ref):
+    st = dict(COMPOUND_NPT)
+    from concurrent.futures import ProcessPoolExecutor
+    with ProcessPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
+        runs = list(ex.map(compound_run, [(st, seed) for seed in st["seeds"]]))
+    st.update(phi_runs=[r[0] for r in runs], energy_runs=[r[1] for r in runs])
+    return st
+
+
 def reference_statistics(ref):
     from rampack_amd import lattice
     st = dict(WCA_NPT)
@@ -200,6 +228,15 @@ def main():
             out = json.load(f)
         out["statistics"] = [s for s in out["statistics"] if s["name"] != SC_NPT["name"]]
         out["statistics"].append(reference_sc_statistics(ref))
+        print(out["statistics"][-1])
+        with open(path, "w") as f:
+            json.dump(out, f, indent=1)
+        return
+    if "--compound-only" in sys.argv:
+        with open(path) as f:
+            out = json.load(f)
+        out["statistics"] = [s for s in out["statistics"] if s["name"] != COMPOUND_NPT["name"]]
+        out["statistics"].append(reference_compound_statistics(ref))
         print(out["statistics"][-1])
         with open(path, "w") as f:
             json.dump(out, f, indent=1)
@@ -252,7 +289,8 @@ def main():
         print(name, energy)
     out["large_corpora"] = large_corpora(ref)
     out["poly_snapshots"] = poly_snapshots(ref)
-    out["statistics"] = [reference_statistics(ref), reference_dimer_statistics(ref), reference_sc_statistics(ref)]
+    out["statistics"] = [reference_statistics(ref), reference_dimer_statistics(ref), reference_sc_statistics(ref),
+                         reference_compound_statistics(ref)]
     print(out["statistics"])
     with open(path, "w") as f:
         json.dump(out, f, indent=1)

@@ -11,6 +11,17 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_PATH = os.path.join(ROOT, "oracle", "lib", "liboracle.so")
 REF_PATH = os.path.join(ROOT, "oracle", "_ref", "libref.so")
 REF_PERF_PATH = os.path.join(ROOT, "oracle", "_ref", "libref_perf.so")
+REF_PERF_V4_PATH = os.path.join(ROOT, "oracle", "_ref", "libref_perf_v4.so")
+
+
+def host_has_avx512() -> bool:
+    """x86-64-v4 (AVX-512 F, BW, CD, DQ, VL) on this host's CPU."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            flags = next((line.split(":", 1)[1].split() for line in f if line.startswith("flags")), [])
+    except OSError:
+        return False
+    return all(x in flags for x in ("avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl"))
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _u8p = ctypes.POINTER(ctypes.c_uint8)
@@ -51,6 +62,8 @@ def _setup_common(lib):
     lib.orc_rotation_sc.restype = None
     lib.orc_rotate.argtypes = [ctypes.c_size_t, _dp, _dp, _dp]
     lib.orc_rotate.restype = None
+    lib.orc_sincos.argtypes = [ctypes.c_size_t, _dp, _dp]
+    lib.orc_sincos.restype = None
     lib.orc_nematic_p2.argtypes = [ctypes.c_size_t, _dp]
     lib.orc_nematic_p2.restype = ctypes.c_double
     lib.orc_pair_guarded.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_size_t, _dp, _u8p,
@@ -165,8 +178,12 @@ def _setup_ref(lib):
 
 
 def ref(perf: bool = False) -> Checker:
+    """The reference build: the parity build, or (perf) the Release-optimised one, for AVX-512 hosts where one was
+    built and the host supports it."""
     global _ref
-    path = REF_PERF_PATH if perf else REF_PATH
+    path = REF_PATH
+    if perf:
+        path = REF_PERF_V4_PATH if os.path.exists(REF_PERF_V4_PATH) and host_has_avx512() else REF_PERF_PATH
     if perf:
         lib = ctypes.CDLL(path, mode=os.RTLD_LAZY)
         _setup_ref(lib)

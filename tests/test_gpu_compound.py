@@ -57,27 +57,54 @@ def test_compound_sphere_sweep_rejects_core_overlaps():
     assert np.array_equal(out[0], out[1])
 
 
-def test_compound_rods_npt_overlap_free_and_softer_than_hard_rods():
-    """Hard rods with WCA on the segment distance at p* = 2.5 (the large dense workload's soft variant, here on 4 000
-    rods): overlap-free by the oracle's count, energy equal to the oracle's sum, and a lower packing fraction than
-    plain hard rods at the same pressure (the repulsion widens every rod by up to 2^(1/6) - 1)."""
-    box, pos, rot = lattice.spherocylinder_smectic(4000, 0.45)
-    phis = {}
-    for shape in (C_ROD, ROD):
-        pk = ra.GpuPacking(shape, box, pos, rot, seed=23)
-        pk.set_environment(1.0, 2.5, 0.1, None, box_move=("delta_triclinic", 0.05))
-        pk.run_cycles(600, adapt_steps=True)
-        vals = []
-        for _ in range(20):
-            pk.run_cycles(20)
-            vals.append(pk.observables()["packing_fraction"])
-        phis[shape.soft] = np.mean(vals)
+def test_compound_rods_npt_overlap_free_and_deterministic():
+    """Hard rods with WCA on the segment distance (the large dense workload's soft variant) in NpT at p* = 1 from the
+    aligned lattice: box moves are accepted, the packing stays overlap-free by the oracle's count, the energy equals
+    the oracle's sum over the state, and the same seed gives the same trajectory."""
+    box, pos, rot = lattice.spherocylinder_lattice(n_side=(18, 18, 4), spacing=(1.45, 1.45, 6.3))
+    out = []
+    for _ in range(2):
+        pk = ra.GpuPacking(C_ROD, box, pos, rot, seed=23)
+        pk.set_environment(1.0, 1.0, 0.2, None, box_move=("delta_triclinic", 0.01))
+        c = pk.run_cycles(600, adapt_steps=True)
+        assert c["box_accepted"] > 10 and 0 < c["moves_accepted"] < c["moves_attempted"]
         b, p, r = pk.download()
-        n, e = oracle_lib.oracle().snapshot_stats(1, 0.5, 5.0, b, p, r,
-                                                  soft=(2 | HARD_CORE) if shape.soft else 0, eps=1.0, sigma=1.0)
-        assert n == 0
-        if shape.soft:
-            assert pk.total_energy() == pytest.approx(e, rel=1e-10) and e > 0
+        n, e = oracle_lib.oracle().snapshot_stats(1, 0.5, 5.0, b, p, r, soft=2 | HARD_CORE, eps=1.0, sigma=1.0)
+        assert n == 0 and pk.count_total_overlaps() == 0
+        assert pk.total_energy() == pytest.approx(e, rel=1e-10) and e > 0
+        out.append((b, p, r))
         pk.close()
-    print(f"packing fraction at p* = 2.5: compound {phis['wca_segment']:.4f}, hard {phis[None]:.4f}", flush=True)
-    assert phis["wca_segment"] < phis[None] - 0.01
+    for x, y in zip(out[0], out[1]):
+        assert np.array_equal(x, y)
+
+
+def test_compound_sphere_npt_matches_reference_runs(golden):
+    """NpT hard spheres with a WCA shell (CompoundInteraction of SphereTraits and RepulsiveLennardJonesInteraction)
+    against independent runs of the reference's own CompoundInteraction (fixture `compound_sphere_npt`): the mean
+    packing fraction and energy per particle over independent device runs within two standard errors of the
+    difference, each side's error from the spread between its runs."""
+    st = next((s for s in golden.get("statistics", []) if s["name"] == "compound_sphere_npt"), None)
+    if st is None:
+        pytest.skip("no reference statistics in the fixture")
+    box, pos, rot = lattice.hard_sphere_lattice(st["n_side"], st["phi0"])
+    phi, energy = [], []
+    for seed in st["device_seeds"]:
+        pk = ra.GpuPacking(C_SPHERE, box, pos, rot, seed=seed)
+        pk.set_environment(st["temperature"], st["pressure"], st["trans_step"], None,
+                           box_move=("delta_v", st["box_step"]))
+        pk.run_cycles(st["therm"], adapt_steps=True)
+        f, e = [], []
+        for _ in range(st["avg"] // st["every"]):
+            pk.run_cycles(st["every"])
+            f.append(pk.observables()["packing_fraction"])
+            e.append(pk.total_energy() / len(pos))
+        phi.append(np.mean(f))
+        energy.append(np.mean(e))
+        pk.close()
+    for name, dev, ref in (("phi", phi, st["phi_runs"]), ("energy", energy, st["energy_runs"])):
+        md, sd = np.mean(dev), np.std(dev, ddof=1) / np.sqrt(len(dev))
+        mr, sr = np.mean(ref), np.std(ref, ddof=1) / np.sqrt(len(ref))
+        z = (md - mr) / np.hypot(sd, sr)
+        print(f"{name}: device {md:.5f} +- {sd:.5f} ({len(dev)} runs), reference {mr:.5f} +- {sr:.5f} "
+              f"({len(ref)} runs), difference {z:+.2f} standard errors", flush=True)
+        assert abs(z) <= 2.0

@@ -21,13 +21,13 @@ SC = ra.Shape("spherocylinder", radius=0.5, length=5.0)
 HS = ra.Shape("sphere", radius=0.5)
 
 
-def check_record(shape, rec, flags=True):
+def check_record(shape, rec, flags=True, rejections=True):
     kind = 0 if shape.kind == "sphere" else 1
     dec, post_pos, post_rot = oracle_lib.replay_sweep(kind, shape.radius, shape.length, rec)
     run = dec != 3
     counts = np.bincount(dec[run], minlength=3)
     print(f"proposals {run.sum()}: accepted {counts[1]}, overlap {counts[0]}, left cell {counts[2]}", flush=True)
-    assert counts[1] > 0 and counts[0] > 0
+    assert counts[1] > 0 and (counts[0] > 0 or not rejections)
     if flags:
         mismatch = np.flatnonzero(rec["accepted"][run] != (dec[run] == 1))
         assert mismatch.size == 0, f"{mismatch.size} decisions differ, first proposals {mismatch[:10]}"
@@ -47,7 +47,7 @@ def test_replay_hard_spheres_c1():
     box, pos, rot = lattice.hard_sphere_lattice(10, 0.30)
     pk = ra.GpuPacking(HS, box, pos, rot, seed=1234)
     pk.set_environment(1.0, 5.0, 0.1, None, box_move=("log", 0.01))
-    check_record(HS, pk.record_sweep())
+    check_record(HS, pk.record_sweep(), rejections=False)  # the lattice start leaves no step-0.1 move an overlap
     pk.run_cycles(300, adapt_steps=True)
     for _ in range(3):
         check_record(HS, pk.record_sweep())
@@ -85,7 +85,7 @@ def test_replay_one_cell_axis_self_image():
     its own periodic image, which the reference never tests (Packing.cpp:383-384 skips the particle by index)."""
     box, pos, rot = lattice.spherocylinder_lattice((17, 17, 1), (1.5, 1.5, 6.15))
     pk = ra.GpuPacking(SC, box, pos, rot, seed=5)
-    pk.set_environment(1.0, 1.0, 0.3, None, None)
+    pk.set_environment(1.0, 1.0, 0.3, None)
     for _ in range(3):
         rec = pk.record_sweep()
         assert rec["cells"][2] == 1
