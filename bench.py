@@ -165,6 +165,60 @@ def copy_probe(nbytes: int):
     log(f"copy probe: 3 x {nbytes} B device-to-device")
 
 
+def timed_leg(ra, wl, input_t, eq, precision, args) -> dict:
+    """W untimed then K timed cycles from input T with the adapted steps frozen, at one precision of the sweep's
+    stencil scan; the packing stays open for the CPU leg."""
+    box, pos, rot = input_t
+    name = {ra.PREC_FP64: "fp64", ra.PREC_GUARDED_FP32: "guarded_fp32"}[precision]
+    pk = ra.GpuPacking(wl["shape"], box, pos, rot, seed=args.seed + 1, precision=precision)
+    pk.set_environment(1.0, wl["pressure"], eq["trans_step"], eq["rot_step"],
+                       box_move=("delta_triclinic", eq["box_step"]))
+    pk.run_cycles(args.warmup)
+    phi_start = pk.packing_fraction()
+    before = pk.run_cycles(0)
+    g0 = pk.guard_stats()
+    # HIP events on the context's stream around one sweep launch per timed cycle (the colour pass whose index is
+    # the cycle number modulo 8), read once after the timed cycles; events around every launch would add about 8 %
+    # to the cycle time
+    pk.set_timing("sampled")
+    t0 = time.perf_counter()
+    after = pk.run_cycles(args.steps)  # returns after the device finished (counters are read back)
+    elapsed = time.perf_counter() - t0
+    tm = pk.timing()
+    pk.set_timing(False)
+    g1 = pk.guard_stats()
+    # per-kernel diagnostics (rebuild, overlap scan, proposals) from a short pass with every launch timed
+    diag_cycles = min(100, args.steps)
+    pk.set_timing(True)
+    pk.run_cycles(diag_cycles)
+    td = pk.timing()
+    pk.set_timing(False)
+    moves = after["moves_attempted"] - before["moves_attempted"]
+    accepted = after["moves_accepted"] - before["moves_accepted"]
+    rate = moves / elapsed
+    phi_end = pk.packing_fraction()
+    avg_sweep_us = tm["sweep_ms"] / max(tm["sweep_launches"], 1) * 1e3
+    log(f"[{name}] timed {args.steps} cycles in {elapsed:.3f} s: {rate:.4g} trial moves/s, acceptance "
+        f"{accepted / moves:.3f}, phi {phi_end:.4f}, box moves {after['box_accepted'] - before['box_accepted']}/"
+        f"{after['box_attempted'] - before['box_attempted']}, unstaged blocks {after['unstaged_blocks']}")
+    log(f"[{name}] launch shape: colour period {tm['sweep_period']}, {tm['sweep_waves']} wavefronts per cell, "
+        f"{tm['sweep_cap']} staged entries, LDS limit {tm['lds_limit']} B; sweep (sampled) {tm['sweep_launches']} "
+        f"launches, avg {avg_sweep_us:.1f} us")
+    log(f"[{name}] diagnostic pass of {diag_cycles} cycles, every launch timed: sweep avg "
+        f"{td['sweep_ms'] / max(td['sweep_launches'], 1) * 1e3:.1f} us x {td['sweep_launches']}; proposals avg "
+        f"{td['proposal_ms'] / max(td['proposal_launches'], 1) * 1e3:.1f} us; rebuild avg "
+        f"{td['rebuild_ms'] / max(td['rebuild_launches'], 1) * 1e3:.1f} us x {td['rebuild_launches']}; scan avg "
+        f"{td['scan_ms'] / max(td['scan_launches'], 1) * 1e3:.1f} us x {td['scan_launches']}")
+    out = dict(rate=rate, elapsed=elapsed, moves=moves, acceptance=accepted / moves, timing=tm, counters=after,
+               packing=pk, phi_start=phi_start, phi_end=phi_end, sweep_us=avg_sweep_us)
+    if precision != ra.PREC_FP64:
+        pairs = g1["pairs"] - g0["pairs"]
+        guard = g1["fp64_fallback"] - g0["fp64_fallback"]
+        out["guard"] = {"fp32_pairs": pairs, "guard_band_pairs": guard, "guard_band_rate": guard / max(pairs, 1)}
+        log(f"[{name}] FP32 pairs {pairs}, guard band {guard} ({guard / max(pairs, 1):.2e})")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -179,6 +233,8 @@ def main():
     ap.add_argument("--workload", choices=["spherocylinders", "kinked-chains"], default="spherocylinders",
                     help="kinked-chains: the polysphere sweep on 50 653 kinked 11-bead chains (not the headline "
                          "metric's workload)")
+    ap.add_argument("--precision", choices=["both", "fp64", "guarded"], default="both",
+                    help="stencil scan of the timed legs: plain FP64, guarded FP32, or both on the same trajectory")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a device-to-device copy of this many bytes (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -198,42 +254,31 @@ def main():
     pk.set_environment(1.0, wl["pressure"], wl["trans_step"], None, box_move=("delta_triclinic", wl["box_step"]))
     t0 = time.perf_counter()
     eq = pk.run_cycles(args.equilibrate, adapt_steps=True)
-    sha = state_sha256(*pk.download())
+    input_t = pk.download()
+    sha = state_sha256(*input_t)
     log(f"input T: {args.equilibrate} adapting cycles in {time.perf_counter() - t0:.2f} s: "
         f"phi={pk.packing_fraction():.4f} steps trans={eq['trans_step']:.4f} rot={eq['rot_step']:.4f} "
         f"box={eq['box_step']:.5f}, box moves {eq['box_accepted']}/{eq['box_attempted']}, sha256 {sha[:16]}")
-    warm = pk.run_cycles(args.warmup)
-    phi_start = pk.packing_fraction()
-    before = pk.run_cycles(0)
+    pk.close()
 
-    # HIP events on the context's stream around one sweep launch per timed cycle (the colour pass whose index is
-    # the cycle number modulo 8), read once after the timed cycles; events around every launch would add about 8 %
-    # to the cycle time
-    pk.set_timing("sampled")
-    t0 = time.perf_counter()
-    after = pk.run_cycles(args.steps)  # returns after the device finished (counters are read back)
-    elapsed = time.perf_counter() - t0
-    tm = pk.timing()
-    pk.set_timing(False)
-    # per-kernel diagnostics (rebuild, overlap scan, proposals) from a short pass with every launch timed
-    diag_cycles = min(100, args.steps)
-    pk.set_timing(True)
-    pk.run_cycles(diag_cycles)
-    td = pk.timing()
-    pk.set_timing(False)
-    moves = after["moves_attempted"] - before["moves_attempted"]
-    accepted = after["moves_accepted"] - before["moves_accepted"]
-    acc_rate = accepted / moves
-    rate = moves / elapsed
-    phi_end = pk.packing_fraction()
-    log(f"timed {args.steps} cycles in {elapsed:.3f} s: {rate:.4g} trial moves/s, acceptance {acc_rate:.3f}, "
-        f"phi {phi_end:.4f}, box moves {after['box_accepted'] - before['box_accepted']}/"
-        f"{after['box_attempted'] - before['box_attempted']}, unstaged blocks {after['unstaged_blocks']}")
-
-    tb = after
+    # the legs run the same trajectory from input T (same seed and frozen steps; the guarded scan makes the FP64
+    # scan's decisions), so they time the same work
+    precisions = [("fp64", ra.PREC_FP64)]
+    if wl["kind"] in (0, 1) and args.precision in ("both", "guarded"):
+        precisions.append(("guarded_fp32", ra.PREC_GUARDED_FP32))
+    if args.precision == "guarded":
+        precisions = precisions[1:]
+    legs = {}
+    for name, prec in precisions:
+        legs[name] = timed_leg(ra, wl, input_t, eq, prec, args)
+    best = max(legs, key=lambda k: legs[k]["rate"])
+    leg = legs[best]
+    rate, elapsed, moves, acc_rate = leg["rate"], leg["elapsed"], leg["moves"], leg["acceptance"]
+    tm, tb, pk = leg["timing"], leg["counters"], leg["packing"]
+    phi_start, phi_end = leg["phi_start"], leg["phi_end"]
+    avg_sweep_ms = tm["sweep_ms"] / max(tm["sweep_launches"], 1)
     # every cycle launches one sweep per colour (period^3 colours; all hold cells at this size)
     moves_per_launch = moves / (args.steps * tm["sweep_period"] ** 3)
-    avg_sweep_ms = tm["sweep_ms"] / max(tm["sweep_launches"], 1)
     # algorithmic bytes per trial move: the reference's 27-cell stencil of particle records, plus the write-back of
     # accepted moves; with nc > 1 centres, nc (24 B centre + 72 B orientation) per stencil entry and nc 48 B
     # bead records written back
@@ -244,15 +289,6 @@ def main():
         bytes_per_move = nc * 27 * (24 + 72) + acc_rate * (RECORD_BYTES + nc * 48)
     bytes_per_launch = bytes_per_move * moves_per_launch
     achieved = bytes_per_launch / (avg_sweep_ms * 1e-3) / 1e9
-    log(f"launch shape: colour period {tm['sweep_period']}, {tm['sweep_waves']} wavefronts per cell, "
-        f"{tm['sweep_cap']} staged entries, "
-        f"LDS limit {tm['lds_limit']} B")
-    log(f"sweep kernel (timed cycles, sampled): {tm['sweep_launches']} launches timed, avg {avg_sweep_ms * 1e3:.1f} us")
-    log(f"diagnostic pass of {diag_cycles} cycles, every launch timed: sweep avg "
-        f"{td['sweep_ms'] / max(td['sweep_launches'], 1) * 1e3:.1f} us x {td['sweep_launches']}; proposals avg "
-        f"{td['proposal_ms'] / max(td['proposal_launches'], 1) * 1e3:.1f} us; rebuild avg "
-        f"{td['rebuild_ms'] / max(td['rebuild_launches'], 1) * 1e3:.1f} us x {td['rebuild_launches']}; scan avg "
-        f"{td['scan_ms'] / max(td['scan_launches'], 1) * 1e3:.1f} us x {td['scan_launches']}")
 
     cpu = None
     if args.cpu_cycles != 0:
@@ -283,7 +319,7 @@ def main():
         "higher_is_better": True,
         "scaling": "strong",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if best == "fp64" else "f64 (f32 centre distances with FP64 guard)",
         "data": "synthetic",
         "config": {
             "workload": wl["text"],
@@ -295,7 +331,14 @@ def main():
             "packing_fraction_timed_start": round(phi_start, 5),
             "packing_fraction_timed_end": round(phi_end, 5),
             "acceptance": round(acc_rate, 4),
-            "precision": "fp64",
+            "precision": best,
+            "legs": {k: {"trial_moves_per_s": v["rate"], "ms_per_step": v["elapsed"] / args.steps * 1e3,
+                         "k_sweep_avg_us": v["sweep_us"], **({"guard": v["guard"]} if "guard" in v else {})}
+                     for k, v in legs.items()},
+            "guarded_over_fp64": (legs["guarded_fp32"]["rate"] / legs["fp64"]["rate"]
+                                  if "guarded_fp32" in legs and "fp64" in legs else None),
+            "legs_note": "same seed, input T and frozen steps: both legs run the same trajectory (the guarded "
+                         "scan makes the FP64 scan's decisions); value is the faster leg",
         },
         "roofline": {
             "bound": "hbm",
@@ -313,7 +356,8 @@ def main():
         },
         "cpu_baseline": cpu,
     }
-    pk.close()
+    for v in legs.values():
+        v["packing"].close()
     print(json.dumps(out), flush=True)
 
 

@@ -61,8 +61,17 @@ extern "C" {
  *   E.g. RP_SOFT_WCA_SEGMENT | RP_SOFT_HARD_CORE: hard spherocylinders that also repel by WCA on the segment distance. */
 #define RP_SOFT_HARD_CORE 0x100
 
-/* Precision of the overlap tests.  RP_PREC_FP64 evaluates every test in the reference's FP64 operation order. */
+/* Precision of the overlap tests.  RP_PREC_FP64 evaluates every test in the reference's FP64 operation order.
+ * RP_PREC_GUARDED_FP32 (no counterpart in the reference): the stencil scan of the default hard spherocylinder sweep
+ * (eight wavefronts per cell, grouped moves; other launches and shapes keep FP64) takes squared centre distances in
+ * FP32 from positions
+ * relative to the active cell's centre, with rigorous bounds on their error; a pair whose FP32 value lies within that
+ * bound of contact or of the interaction range is decided by the FP64 value, so every decision equals RP_PREC_FP64's.
+ * Box, inverse, positions and the segment-distance stage stay FP64.  RP_PREC_GUARDED_FP32_VERIFY also evaluates every
+ * pair in FP64 and records the largest FP32 error and any class that differs (rp_get_guard_stats). */
 #define RP_PREC_FP64 0
+#define RP_PREC_GUARDED_FP32 1
+#define RP_PREC_GUARDED_FP32_VERIFY 2
 
 /* Box moves: TriclinicDeltaScaler with scale_together (src/core/volume_scalers/TriclinicDeltaScaler.cpp:7-25),
  * log isotropic through TriclinicAdapter + AnisotropicVolumeScaler + LogScalingFactorSampler
@@ -353,6 +362,13 @@ typedef struct rp_guard_stats {
 } rp_guard_stats;
 int rp_pair_decisions_guarded(const rp_shape_desc *shape, size_t n, const double *records, uint8_t *decisions,
                               uint8_t *codes, rp_guard_stats *stats);
+
+/* Precision of a context's sweeps (RP_PREC_*, as rp_create's argument), and the guarded sweep's statistics since the
+ * context was created: pairs classified in FP32, fp64_fallback = pairs inside a guard band, and in verify mode
+ * mismatches (0) and max_err_centre (largest |FP32 - FP64| squared centre distance); the segment fields stay 0 (the
+ * segment stage is FP64). */
+int rp_set_precision(rp_ctx *ctx, int precision);
+int rp_get_guard_stats(rp_ctx *ctx, rp_guard_stats *out);
 
 /* The box quantities the device path derives on the host (TriclinicBox, src/core/TriclinicBox.h:49-70 and
  * TriclinicBox.cpp:27-51): the inverse of the box matrix (Matrix::inverse, src/geometry/Matrix.tpp:258-271), the three

