@@ -386,6 +386,48 @@ int rp_apply_moves(const double box[9], size_t n, const double *pos, const doubl
                    const double *axis, const double *angle, double *out_pos, double *out_R, double *out_rot,
                    double *out_sincos);
 
+/* The learned candidate ordering (no counterpart in the reference; the reference tests candidates in cell order,
+ * src/core/Packing.cpp:381-398).  A network 12 -> 64 -> 64 -> 64 -> 1 (GELU hidden units, tanh form; sigmoid
+ * output) predicts from a candidate's pose relative to a trial pose -- its position in the trial body frame divided
+ * by the total interaction range (3) and the relative rotation O_trial^T O_candidate (9, row-major) -- the
+ * probability that the exact test finds an overlap.  RP_MLP_PARAMS floats of parameters: W1 [64][12], b1 [64],
+ * W2 [64][64], b2 [64], W3 [64][64], b3 [64], w4 [64], b4.
+ * rp_mlp_gradient: the device's FP32 forward and backward kernels on a batch of n pairs (x: n x 12, y: n labels 0/1):
+ * the mean binary cross-entropy and its gradient (RP_MLP_PARAMS floats).
+ * rp_mlp_infer: overlap probabilities of n pairs with BF16 MFMA layers and FP32 accumulation. */
+#define RP_MLP_PARAMS 9217
+int rp_mlp_gradient(const float *theta, size_t n, const float *x, const float *y, double *loss, float *grad);
+int rp_mlp_infer(const float *theta, size_t n, const float *x, float *score);
+
+/* Training and use of the ordering on a polysphere context.
+ * rp_mlp_train: the run's own trajectory supplies the data -- `snapshots` times, cycles_between cycles of the run
+ * (rp_run_cycles without adaptation), then moves_per_snapshot trial moves drawn as the sweep draws them with the first
+ * sampler's steps, and every close candidate of each trial pose (within the total interaction range) as one pair
+ * with the exact test's decision as its label.  Glorot-uniform weights, the output bias at the log-odds of the overlap
+ * fraction, then Adam (learning_rate, batch pairs per step) for `epochs` passes in a seeded random order, all on the
+ * device by the FP32 training kernels.
+ * rp_mlp_order_stats: for `moves` fresh trial moves on the current state, the mean number of exact tests per move
+ * that a search stopping at the first overlap makes in natural stencil order, in nearest-first order and in the
+ * network's order (BF16 MFMA scores, most likely overlap first); the decisions (any overlap) are the same in every
+ * order.  Moves that leave their cell or have no close candidate count 0 tests.
+ * rp_mlp_pairs: the pairs of `moves` fresh trial moves (x: 12 features, y: exact decision, meta: move, natural rank,
+ * nearest-first key) into caller arrays of `cap` pairs; *n receives the count (RP_ERR_PRECONDITION if above cap). */
+typedef struct rp_mlp_stats {
+    uint64_t pairs;              /* training pairs (rp_mlp_train) or evaluated pairs (rp_mlp_order_stats) */
+    double overlap_fraction;     /* of those, the fraction whose exact test finds an overlap */
+    double loss_initial, loss_final;  /* mean binary cross-entropy over the training pairs before / after training */
+    uint64_t steps;              /* Adam steps */
+    uint64_t moves, moves_rejected;
+    double candidates_per_move;  /* close candidates per sampled move */
+    double tests_natural, tests_nearest_first, tests_learned;  /* exact tests per sampled move, early exit */
+} rp_mlp_stats;
+int rp_mlp_train(rp_ctx *ctx, size_t snapshots, size_t cycles_between, size_t moves_per_snapshot, size_t epochs,
+                 size_t batch, double learning_rate, rp_mlp_stats *out);
+int rp_mlp_order_stats(rp_ctx *ctx, size_t moves, rp_mlp_stats *out);
+int rp_mlp_get_weights(rp_ctx *ctx, float *theta);
+int rp_mlp_set_weights(rp_ctx *ctx, const float *theta);
+int rp_mlp_pairs(rp_ctx *ctx, size_t moves, size_t cap, float *x, float *y, int *meta, size_t *n);
+
 /* Batched soft pair energies on the device: Interaction::calculateEnergyBetween with a fixed cell translation
  * (CentralInteraction.h:58-67 for RP_SOFT_WCA); records as in rp_pair_decisions. */
 int rp_pair_energies(const rp_shape_desc *shape, size_t n, const double *records, double *energies);

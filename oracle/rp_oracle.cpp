@@ -991,6 +991,93 @@ double orc_nematic_p2(std::size_t n, const double *rot) {
     return best;
 }
 
+// The learned ordering's network (no counterpart in the reference), restated in double precision as the checker
+// of the device's training kernels: x (12 features) -> GELU(W1 x + b1) -> GELU(W2 h + b2) -> GELU(W3 h + b3) ->
+// z = w4 . h + b4, p = sigmoid(z); loss = mean over the batch of the binary cross-entropy
+// max(z, 0) - z y + log(1 + exp(-|z|)); GELU(x) = x (1 + tanh(sqrt(2/pi) (x + 0.044715 x^3))) / 2.  Parameters in
+// the device's order: W1 [64][12], b1 [64], W2 [64][64], b2, W3 [64][64], b3, w4 [64], b4 (9217 floats).  grad gets
+// d loss / d parameter; z (nullable) the logits.
+namespace orcmlp {
+constexpr int IN = 12, H = 64;
+constexpr int W1 = 0, B1 = W1 + H * IN, W2 = B1 + H, B2 = W2 + H * H, W3 = B2 + H, B3 = W3 + H * H, W4 = B3 + H,
+              B4 = W4 + H, NP = B4 + 1;
+static double gelu(double x) {
+    const double c = std::sqrt(2.0 / M_PI);
+    return 0.5 * x * (1 + std::tanh(c * (x + 0.044715 * x * x * x)));
+}
+static double geluGrad(double x) {
+    const double c = std::sqrt(2.0 / M_PI);
+    const double t = std::tanh(c * (x + 0.044715 * x * x * x));
+    return 0.5 * (1 + t) + 0.5 * x * (1 - t * t) * c * (1 + 3 * 0.044715 * x * x);
+}
+}  // namespace orcmlp
+
+double orc_mlp_loss_grad(const float *theta, std::size_t n, const float *x, const float *y, double *grad,
+                         double *zOut) {
+    using namespace orcmlp;
+    std::vector<double> g(NP, 0.0);
+    double loss = 0;
+    for (std::size_t s = 0; s < n; s++) {
+        double in[IN], p1[H], a1[H], p2[H], a2[H], p3[H], a3[H];
+        for (int i = 0; i < IN; i++) in[i] = x[s * IN + i];
+        for (int o = 0; o < H; o++) {
+            double acc = theta[B1 + o];
+            for (int i = 0; i < IN; i++) acc += (double)theta[W1 + o * IN + i] * in[i];
+            p1[o] = acc;
+            a1[o] = gelu(acc);
+        }
+        for (int o = 0; o < H; o++) {
+            double acc = theta[B2 + o];
+            for (int i = 0; i < H; i++) acc += (double)theta[W2 + o * H + i] * a1[i];
+            p2[o] = acc;
+            a2[o] = gelu(acc);
+        }
+        for (int o = 0; o < H; o++) {
+            double acc = theta[B3 + o];
+            for (int i = 0; i < H; i++) acc += (double)theta[W3 + o * H + i] * a2[i];
+            p3[o] = acc;
+            a3[o] = gelu(acc);
+        }
+        double z = theta[B4];
+        for (int j = 0; j < H; j++) z += (double)theta[W4 + j] * a3[j];
+        if (zOut) zOut[s] = z;
+        const double ys = y[s];
+        loss += std::max(z, 0.0) - z * ys + std::log1p(std::exp(-std::fabs(z)));
+        const double dz = (1 / (1 + std::exp(-z)) - ys) / (double)n;
+        double d3[H], d2[H], d1[H];
+        g[B4] += dz;
+        for (int j = 0; j < H; j++) {
+            g[W4 + j] += dz * a3[j];
+            d3[j] = dz * theta[W4 + j] * geluGrad(p3[j]);
+        }
+        for (int o = 0; o < H; o++) {
+            g[B3 + o] += d3[o];
+            for (int i = 0; i < H; i++) g[W3 + o * H + i] += d3[o] * a2[i];
+        }
+        for (int i = 0; i < H; i++) {
+            double acc = 0;
+            for (int o = 0; o < H; o++) acc += (double)theta[W3 + o * H + i] * d3[o];
+            d2[i] = acc * geluGrad(p2[i]);
+        }
+        for (int o = 0; o < H; o++) {
+            g[B2 + o] += d2[o];
+            for (int i = 0; i < H; i++) g[W2 + o * H + i] += d2[o] * a1[i];
+        }
+        for (int i = 0; i < H; i++) {
+            double acc = 0;
+            for (int o = 0; o < H; o++) acc += (double)theta[W2 + o * H + i] * d2[o];
+            d1[i] = acc * geluGrad(p1[i]);
+        }
+        for (int o = 0; o < H; o++) {
+            g[B1 + o] += d1[o];
+            for (int i = 0; i < IN; i++) g[W1 + o * IN + i] += d1[o] * in[i];
+        }
+    }
+    if (grad)
+        for (int i = 0; i < NP; i++) grad[i] = g[i];
+    return loss / (double)n;
+}
+
 // Guarded FP32 pair decisions, restated for the tests (no counterpart in the reference: the FP64 decisions above are
 // what it must reproduce).  The same FP32 arithmetic as the device (each operation rounded on its own, correctly
 // rounded division), so the classification codes can be compared bit for bit:

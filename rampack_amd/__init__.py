@@ -59,6 +59,8 @@ EXPORTED_SYMBOLS = (
     "rp_ramtrj_jump_to_snapshot", "rp_ramtrj_player_close", "rp_pair_decisions_guarded",
     "rp_try_scaling", "rp_revert_scaling", "rp_set_move_samplers", "rp_get_move_samplers", "rp_run_replicas",
     "rp_record_sweep", "rp_box_info", "rp_apply_moves", "rp_set_precision", "rp_get_guard_stats",
+    "rp_mlp_gradient", "rp_mlp_infer", "rp_mlp_train", "rp_mlp_order_stats", "rp_mlp_get_weights",
+    "rp_mlp_set_weights", "rp_mlp_pairs",
 )
 
 
@@ -139,6 +141,17 @@ class SweepRecord(ctypes.Structure):
     _fields_ = [("box", ctypes.c_double * 9), ("shift", ctypes.c_double * 3), ("cells", ctypes.c_int * 3),
                 ("period", ctypes.c_int), ("n_colours", ctypes.c_int), ("colour_order", ctypes.c_int * 27),
                 ("moves_per_cell", ctypes.c_int), ("n_samplers", ctypes.c_int), ("n_proposals", ctypes.c_size_t)]
+
+
+class MlpStats(ctypes.Structure):
+    _fields_ = [("pairs", ctypes.c_uint64), ("overlap_fraction", ctypes.c_double), ("loss_initial", ctypes.c_double),
+                ("loss_final", ctypes.c_double), ("steps", ctypes.c_uint64), ("moves", ctypes.c_uint64),
+                ("moves_rejected", ctypes.c_uint64), ("candidates_per_move", ctypes.c_double),
+                ("tests_natural", ctypes.c_double), ("tests_nearest_first", ctypes.c_double),
+                ("tests_learned", ctypes.c_double)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 class RampackError(RuntimeError):
@@ -225,6 +238,15 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.rp_apply_moves.argtypes = [dp, ctypes.c_size_t, dp, dp, dp, dp, dp, dp, dp, dp, dp]
     lib.rp_set_precision.argtypes = [vp, ctypes.c_int]
     lib.rp_get_guard_stats.argtypes = [vp, ctypes.POINTER(GuardStats)]
+    fp = ctypes.POINTER(ctypes.c_float)
+    lib.rp_mlp_gradient.argtypes = [fp, ctypes.c_size_t, fp, fp, dp, fp]
+    lib.rp_mlp_infer.argtypes = [fp, ctypes.c_size_t, fp, fp]
+    sz = ctypes.c_size_t
+    lib.rp_mlp_train.argtypes = [vp, sz, sz, sz, sz, sz, ctypes.c_double, ctypes.POINTER(MlpStats)]
+    lib.rp_mlp_order_stats.argtypes = [vp, sz, ctypes.POINTER(MlpStats)]
+    lib.rp_mlp_get_weights.argtypes = [vp, fp]
+    lib.rp_mlp_set_weights.argtypes = [vp, fp]
+    lib.rp_mlp_pairs.argtypes = [vp, sz, sz, fp, fp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(sz)]
     lib.rp_last_error.restype = ctypes.c_char_p
     _lib = lib
     return lib
@@ -370,6 +392,35 @@ def apply_moves(box, pos, rot, t, axis, angle):
     return tuple(out)
 
 
+MLP_PARAMS = 9217
+
+
+def _fptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def mlp_gradient(theta, x, y):
+    """rp_mlp_gradient: (mean binary cross-entropy, gradient) of the ordering network on a batch, on the device."""
+    lib = load_library()
+    theta = np.ascontiguousarray(theta, dtype=np.float32)
+    x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 12)
+    y = np.ascontiguousarray(y, dtype=np.float32)
+    grad = np.zeros(MLP_PARAMS, dtype=np.float32)
+    loss = ctypes.c_double()
+    _check(lib.rp_mlp_gradient(_fptr(theta), len(x), _fptr(x), _fptr(y), ctypes.byref(loss), _fptr(grad)))
+    return loss.value, grad
+
+
+def mlp_infer(theta, x):
+    """rp_mlp_infer: overlap probabilities of pairs with the BF16 MFMA network."""
+    lib = load_library()
+    theta = np.ascontiguousarray(theta, dtype=np.float32)
+    x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 12)
+    out = np.zeros(len(x), dtype=np.float32)
+    _check(lib.rp_mlp_infer(_fptr(theta), len(x), _fptr(x), _fptr(out)))
+    return out
+
+
 def pair_energies(shape: Shape, records: np.ndarray) -> np.ndarray:
     """Device soft pair energies on records of 27 doubles: pos1, O1 (row-major), pos2, O2, T."""
     lib = load_library()
@@ -490,6 +541,45 @@ class GpuPacking:
         st = GuardStats()
         _check(_lib.rp_get_guard_stats(self._ctx, ctypes.byref(st)))
         return st.as_dict()
+
+    def mlp_train(self, snapshots: int, cycles_between: int, moves_per_snapshot: int, epochs: int = 3,
+                  batch: int = 4096, learning_rate: float = 1e-3) -> dict:
+        """rp_mlp_train: the learned ordering's network trained on pairs from this run's trajectory."""
+        st = MlpStats()
+        _check(_lib.rp_mlp_train(self._ctx, snapshots, cycles_between, moves_per_snapshot, epochs, batch,
+                                 float(learning_rate), ctypes.byref(st)))
+        return st.as_dict()
+
+    def mlp_order_stats(self, moves: int) -> dict:
+        """rp_mlp_order_stats: exact tests per trial move in natural, nearest-first and learned order."""
+        st = MlpStats()
+        _check(_lib.rp_mlp_order_stats(self._ctx, moves, ctypes.byref(st)))
+        return st.as_dict()
+
+    def mlp_weights(self) -> np.ndarray:
+        out = np.zeros(MLP_PARAMS, dtype=np.float32)
+        _check(_lib.rp_mlp_get_weights(self._ctx, _fptr(out)))
+        return out
+
+    def set_mlp_weights(self, theta):
+        theta = np.ascontiguousarray(theta, dtype=np.float32)
+        _check(_lib.rp_mlp_set_weights(self._ctx, _fptr(theta)))
+
+    def mlp_pairs(self, moves: int):
+        """rp_mlp_pairs: (features n x 12, exact decisions n, meta n x 3) of fresh sampled trial moves."""
+        n = ctypes.c_size_t()
+        code = _lib.rp_mlp_pairs(self._ctx, moves, 0, None, None, None, ctypes.byref(n))
+        if code != RP_ERR_PRECONDITION or n.value == 0:
+            _check(code)
+        # sample again with room for a larger draw: each call draws fresh moves
+        cap = int(n.value * 2 + 1024)
+        x = np.zeros((cap, 12), dtype=np.float32)
+        y = np.zeros(cap, dtype=np.float32)
+        meta = np.zeros((cap, 3), dtype=np.int32)
+        _check(_lib.rp_mlp_pairs(self._ctx, moves, cap, _fptr(x), _fptr(y),
+                                 meta.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), ctypes.byref(n)))
+        k = int(n.value)
+        return x[:k], y[:k], meta[:k]
 
     def count_total_overlaps(self, early_exit: bool = False) -> int:
         v = ctypes.c_uint64()

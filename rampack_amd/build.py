@@ -11,7 +11,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIB_DIR, "librampack_amd.so")
 
-SOURCES = ["rp_kernels.hip", "rp_capi.cpp", "rp_ramsnap.cpp", "rp_ramtrj.cpp"]
+SOURCES = ["rp_kernels.hip", "rp_mlp.hip", "rp_capi.cpp", "rp_ramsnap.cpp", "rp_ramtrj.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",

@@ -3,6 +3,7 @@
 // src/core/Packing.cpp:172-208, 858-899) is restated here on top of the gfx950 kernels of rp_kernels.hip.
 #include "rampack_amd.h"
 #include "rp_kernels.h"
+#include "rp_mlp.h"
 
 #include <hip/hip_runtime.h>
 
@@ -152,6 +153,18 @@ struct rp_ctx {
     int sweepWaves = 0;                  // wavefronts per active cell; 0 = chosen from the stencil size
     int colourPeriod = 0;                // checkerboard colour period (2 or 3); 0 = by shape
     int pipelined = -1;                  // staged sweep: 0 plain, 1 planner/tester split, 2 grouped moves, -1 auto
+
+    // learned candidate ordering: the network's parameters with Adam's moments and a gradient (4 x MLP_PARAMS), the
+    // training pairs and the pairs of the last order statistics (x: 12 features, y: exact decision, meta, score)
+    struct MlpData {
+        float *x = nullptr, *y = nullptr, *score = nullptr;
+        int *meta = nullptr;
+        size_t cap = 0;
+    };
+    float *mlpTheta = nullptr;
+    MlpData mlpSet, mlpEval;
+    uint32_t mlpStream = 0;
+    bool mlpTrained = false;
 
     // timing
     int timing = 0;  // 0 off, 1 every launch, 2 one sweep launch per cycle (RP_TIMING_SAMPLED)
@@ -1664,6 +1677,9 @@ void rp_destroy(rp_ctx *c) {
     if (c->cellEnergy) hipFree(c->cellEnergy);
     if (c->sumScratch) hipFree(c->sumScratch);
     if (c->dFixCounters) hipFree(c->dFixCounters);
+    if (c->mlpTheta) hipFree(c->mlpTheta);
+    for (auto *d : {&c->mlpSet, &c->mlpEval})
+        if (d->x) hipFree(d->x);
     if (c->tmpSlot) hipFree(c->tmpSlot);
     if (c->tmpOrder) hipFree(c->tmpOrder);
     if (c->tmpPos) hipFree(c->tmpPos);
@@ -1922,6 +1938,347 @@ int rp_pair_energies(const rp_shape_desc *shape, size_t n, const double *records
     hipFree(dRec);
     hipFree(dOut);
     return r;
+}
+
+namespace {
+
+// Room for `need` pairs in d (one block: x 12, y 1, score 1 floats and meta 3 ints per pair), keeping the first
+// `keep` pairs.
+int ensureMlpData(rp_ctx *c, rp_ctx::MlpData &d, size_t need, size_t keep) {
+    if (need <= d.cap) return RP_OK;
+    const size_t cap = need + need / 4 + 1024;
+    void *mem = nullptr;
+    RP_HIP(hipMalloc(&mem, cap * (sizeof(float) * (MLP_FEATURES + 2) + sizeof(int) * 3)));
+    rp_ctx::MlpData n;
+    n.x = static_cast<float *>(mem);
+    n.y = n.x + cap * MLP_FEATURES;
+    n.score = n.y + cap;
+    n.meta = reinterpret_cast<int *>(n.score + cap);
+    n.cap = cap;
+    if (d.x && keep) {
+        RP_HIP(hipMemcpyAsync(n.x, d.x, sizeof(float) * keep * MLP_FEATURES, hipMemcpyDeviceToDevice, c->stream));
+        RP_HIP(hipMemcpyAsync(n.y, d.y, sizeof(float) * keep, hipMemcpyDeviceToDevice, c->stream));
+        RP_HIP(hipMemcpyAsync(n.meta, d.meta, sizeof(int) * 3 * keep, hipMemcpyDeviceToDevice, c->stream));
+    }
+    RP_HIP(hipStreamSynchronize(c->stream));
+    if (d.x) hipFree(d.x);
+    d = n;
+    return RP_OK;
+}
+
+// Candidate pairs of M trial moves sampled on the current state (k_mlp_pairs) into d from index `at` on; P receives
+// the number written.  Moves use the first move sampler's step sizes.
+int mlpSample(rp_ctx *c, int M, rp_ctx::MlpData &d, size_t at, size_t &P) {
+    int r = prepareCycleGrid(c);
+    if (r) return r;
+    SweepParams sp{};
+    sp.g = c->grid;
+    sp.p = c->cur.p;
+    sp.k = c->k;
+    sp.mlpInvRange = 1 / c->totalRange;
+    const auto &sm = c->samplers.front();
+    const double maxT = sm.maxTransImplicit ? minHeight(c->box) / 2 : sm.maxTransStep;
+    const double ts = sm.kind == RP_MOVE_ROTATION ? 0.0 : std::min(maxT, sm.transStep);
+    const double rs = sm.kind == RP_MOVE_TRANSLATION ? 0.0 : std::min(sm.rotStep, M_PI);
+    void *tmp = nullptr;
+    RP_HIP(hipMalloc(&tmp, (sizeof(int) + sizeof(long long)) * (size_t)M));
+    int *dCounts = static_cast<int *>(tmp);
+    long long *dOff = reinterpret_cast<long long *>(dCounts + M + (M & 1));
+    std::vector<int> counts((size_t)M);
+    std::vector<long long> off((size_t)M);
+    const uint32_t stream = c->mlpStream++;
+    hipError_t e = launch_mlp_pairs(c->stream, sp, M, c->seed, stream, ts, rs, 0, dCounts, nullptr, nullptr, nullptr,
+                                    nullptr);
+    if (e == hipSuccess) e = hipMemcpyAsync(counts.data(), dCounts, sizeof(int) * (size_t)M, hipMemcpyDeviceToHost,
+                                            c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    size_t total = 0;
+    for (int q = 0; q < M; q++) {
+        off[(size_t)q] = (long long)(at + total);
+        total += (size_t)counts[(size_t)q];
+    }
+    if (e == hipSuccess && ensureMlpData(c, d, at + total, at) != RP_OK) e = hipErrorOutOfMemory;
+    if (e == hipSuccess) e = hipMemcpy(dOff, off.data(), sizeof(long long) * (size_t)M, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = launch_mlp_pairs(c->stream, sp, M, c->seed, stream, ts, rs, 1, nullptr, dOff, d.x, d.y, d.meta);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    hipFree(tmp);
+    if (e != hipSuccess) return fail(RP_ERR_DEVICE, std::string("candidate pairs: ") + hipGetErrorString(e));
+    P = total;
+    return RP_OK;
+}
+
+// Mean loss of the network over the first n pairs of d, in batches of B (the gradient pass's loss sums).
+int mlpDatasetLoss(rp_ctx *c, const rp_ctx::MlpData &d, size_t n, int B, MlpWork &w, float *grad, const int *dRows,
+                   float *bx, float *by, double &loss) {
+    double sum = 0;
+    size_t done = 0;
+    std::vector<float> parts((size_t)mlp_loss_blocks(B));
+    std::vector<int> rows((size_t)B);
+    for (size_t b0 = 0; b0 + (size_t)B <= n; b0 += (size_t)B) {
+        for (int i = 0; i < B; i++) rows[(size_t)i] = (int)(b0 + (size_t)i);
+        RP_HIP(hipMemcpyAsync(const_cast<int *>(dRows), rows.data(), sizeof(int) * (size_t)B, hipMemcpyHostToDevice,
+                              c->stream));
+        RP_HIP(mlp_gather(c->stream, d.x, d.y, dRows, B, bx, by));
+        RP_HIP(mlp_forward_backward(c->stream, c->mlpTheta, B, bx, by, w, grad));
+        RP_HIP(hipMemcpyAsync(parts.data(), w.lossPart, sizeof(float) * parts.size(), hipMemcpyDeviceToHost,
+                              c->stream));
+        RP_HIP(hipStreamSynchronize(c->stream));
+        for (float v : parts) sum += v;
+        done += (size_t)B;
+    }
+    loss = done ? sum / (double)done : 0;
+    return RP_OK;
+}
+
+}  // namespace
+
+int rp_mlp_train(rp_ctx *c, size_t snapshots, size_t cyclesBetween, size_t movesPerSnapshot, size_t epochs,
+                 size_t batch, double learningRate, rp_mlp_stats *out) {
+    if (!c || !out) return fail(RP_ERR_PRECONDITION, "null argument");
+    if (!c->envSet) return fail(RP_ERR_PRECONDITION, "environment not set");
+    if (c->k.kind != RP_SHAPE_POLYSPHERE)
+        return fail(RP_ERR_UNSUPPORTED, "the learned ordering is built for polysphere sweeps");
+    if (snapshots == 0 || movesPerSnapshot == 0 || movesPerSnapshot > (1u << 24) || batch < 16 ||
+        batch > (1u << 16) || !(learningRate > 0))
+        return fail(RP_ERR_PRECONDITION, "bad training arguments");
+    *out = rp_mlp_stats{};
+    // pairs from the run's own trajectory: cyclesBetween cycles, then movesPerSnapshot sampled trial moves
+    size_t n = 0;
+    for (size_t s = 0; s < snapshots; s++) {
+        for (size_t i = 0; i < cyclesBetween; i++) {
+            int r = runCycle(c);
+            if (r) return r;
+        }
+        size_t P = 0;
+        int r = mlpSample(c, (int)movesPerSnapshot, c->mlpSet, n, P);
+        if (r) return r;
+        n += P;
+    }
+    const int B = (int)batch;
+    if (n < (size_t)B) return fail(RP_ERR_PRECONDITION, "fewer training pairs than one batch");
+    // parameters, Adam moments, gradient
+    if (!c->mlpTheta) RP_HIP(hipMalloc(&c->mlpTheta, sizeof(float) * 4 * MLP_PARAMS));
+    float *m = c->mlpTheta + MLP_PARAMS, *v = m + MLP_PARAMS, *grad = v + MLP_PARAMS;
+    std::vector<float> y(n);
+    RP_HIP(hipMemcpy(y.data(), c->mlpSet.y, sizeof(float) * n, hipMemcpyDeviceToHost));
+    double pos = 0;
+    for (float t : y) pos += t;
+    const double frac = pos / (double)n;
+    // Glorot-uniform weights, zero biases, the output bias at the log-odds of the overlap fraction
+    {
+        std::vector<float> th(MLP_PARAMS, 0.f);
+        Philox rng = philox_init(c->seed ^ 0x6d6c70696e6974ull, 0, 0, 0);
+        auto fill = [&](int offset, int fanOut, int fanIn) {
+            const double a = std::sqrt(6.0 / (fanIn + fanOut));
+            for (int i = 0; i < fanOut * fanIn; i++) th[(size_t)(offset + i)] = (float)philox_uniform(rng, -a, a);
+        };
+        fill(MLP_OFF_W1, MLP_H, MLP_IN);
+        fill(MLP_OFF_W2, MLP_H, MLP_H);
+        fill(MLP_OFF_W3, MLP_H, MLP_H);
+        fill(MLP_OFF_W4, 1, MLP_H);
+        const double f = std::min(std::max(frac, 1e-6), 1 - 1e-6);
+        th[MLP_OFF_B4] = (float)std::log(f / (1 - f));
+        RP_HIP(hipMemcpy(c->mlpTheta, th.data(), sizeof(float) * MLP_PARAMS, hipMemcpyHostToDevice));
+        RP_HIP(hipMemset(m, 0, sizeof(float) * 2 * MLP_PARAMS));
+    }
+    // batch work arrays: rows, gathered batch, activations
+    const size_t lossBlocks = (size_t)mlp_loss_blocks(B);
+    void *work = nullptr;
+    RP_HIP(hipMalloc(&work, sizeof(int) * (size_t)B + sizeof(float) * ((size_t)B * (MLP_IN + 1) +
+                                                                       9 * (size_t)B * MLP_H + B + lossBlocks)));
+    int *dRows = static_cast<int *>(work);
+    float *bx = reinterpret_cast<float *>(dRows + B), *by = bx + (size_t)B * MLP_IN;
+    MlpWork w;
+    float *q = by + B;
+    float **arr[9] = {&w.pre1, &w.a1, &w.pre2, &w.a2, &w.pre3, &w.a3, &w.d1, &w.d2, &w.d3};
+    for (auto p : arr) {
+        *p = q;
+        q += (size_t)B * MLP_H;
+    }
+    w.dz = q;
+    w.lossPart = q + B;
+    struct FreeWork {
+        void *p;
+        ~FreeWork() { hipFree(p); }
+    } freeWork{work};
+    double loss0 = 0, loss1 = 0;
+    int r = mlpDatasetLoss(c, c->mlpSet, n, B, w, grad, dRows, bx, by, loss0);
+    if (r) return r;
+    // epochs over a fresh permutation each, full batches only
+    std::vector<int> perm(n);
+    Philox prng = philox_init(c->seed ^ 0x6d6c7065726dull, 1, 0, 0);
+    int step = 0;
+    for (size_t ep = 0; ep < epochs; ep++) {
+        for (size_t i = 0; i < n; i++) perm[i] = (int)i;
+        for (size_t i = n - 1; i > 0; i--) std::swap(perm[i], perm[philox_below(prng, (uint32_t)(i + 1))]);
+        for (size_t b0 = 0; b0 + (size_t)B <= n; b0 += (size_t)B) {
+            RP_HIP(hipMemcpyAsync(dRows, perm.data() + b0, sizeof(int) * (size_t)B, hipMemcpyHostToDevice,
+                                  c->stream));
+            RP_HIP(mlp_gather(c->stream, c->mlpSet.x, c->mlpSet.y, dRows, B, bx, by));
+            RP_HIP(mlp_forward_backward(c->stream, c->mlpTheta, B, bx, by, w, grad));
+            RP_HIP(mlp_adam(c->stream, c->mlpTheta, grad, m, v, (float)learningRate, ++step));
+            // the host buffer of the next batch is rewritten only after this copy has run
+            RP_HIP(hipStreamSynchronize(c->stream));
+        }
+    }
+    r = mlpDatasetLoss(c, c->mlpSet, n, B, w, grad, dRows, bx, by, loss1);
+    if (r) return r;
+    c->mlpTrained = true;
+    out->pairs = n;
+    out->overlap_fraction = frac;
+    out->loss_initial = loss0;
+    out->loss_final = loss1;
+    out->steps = (uint64_t)step;
+    return RP_OK;
+}
+
+int rp_mlp_order_stats(rp_ctx *c, size_t moves, rp_mlp_stats *out) {
+    if (!c || !out) return fail(RP_ERR_PRECONDITION, "null argument");
+    if (!c->envSet) return fail(RP_ERR_PRECONDITION, "environment not set");
+    if (c->k.kind != RP_SHAPE_POLYSPHERE)
+        return fail(RP_ERR_UNSUPPORTED, "the learned ordering is built for polysphere sweeps");
+    if (!c->mlpTrained) return fail(RP_ERR_PRECONDITION, "the ordering network is not trained (rp_mlp_train)");
+    if (moves == 0 || moves > (1u << 24)) return fail(RP_ERR_PRECONDITION, "moves must be 1 .. 2^24");
+    *out = rp_mlp_stats{};
+    size_t P = 0;
+    int r = mlpSample(c, (int)moves, c->mlpEval, 0, P);
+    if (r) return r;
+    RP_HIP(mlp_infer(c->stream, c->mlpTheta, (int)P, c->mlpEval.x, c->mlpEval.score));
+    std::vector<float> y(P), score(P);
+    std::vector<int> meta(3 * P);
+    RP_HIP(hipMemcpyAsync(y.data(), c->mlpEval.y, sizeof(float) * P, hipMemcpyDeviceToHost, c->stream));
+    RP_HIP(hipMemcpyAsync(score.data(), c->mlpEval.score, sizeof(float) * P, hipMemcpyDeviceToHost, c->stream));
+    RP_HIP(hipMemcpyAsync(meta.data(), c->mlpEval.meta, sizeof(int) * 3 * P, hipMemcpyDeviceToHost, c->stream));
+    RP_HIP(hipStreamSynchronize(c->stream));
+    // per move: the pairs are contiguous and in natural order; exact tests with early exit = 1 + the position of the
+    // first overlapping candidate in the order, or every candidate when none overlaps
+    double natural = 0, shell = 0, learned = 0, cand = 0, pos = 0;
+    uint64_t rejected = 0;
+    std::vector<size_t> idx;
+    for (size_t a = 0; a < P;) {
+        size_t b = a;
+        while (b < P && meta[3 * b] == meta[3 * a]) b++;
+        const size_t cnt = b - a;
+        idx.resize(cnt);
+        auto firstHit = [&](void) {
+            for (size_t i = 0; i < cnt; i++)
+                if (y[idx[i]] > 0.5f) return (double)(i + 1);
+            return (double)cnt;
+        };
+        for (size_t i = 0; i < cnt; i++) idx[i] = a + i;
+        const double nat = firstHit();
+        std::stable_sort(idx.begin(), idx.end(), [&](size_t u, size_t v) { return meta[3 * u + 2] < meta[3 * v + 2]; });
+        const double sh = firstHit();
+        for (size_t i = 0; i < cnt; i++) idx[i] = a + i;
+        std::stable_sort(idx.begin(), idx.end(), [&](size_t u, size_t v) { return score[u] > score[v]; });
+        const double le = firstHit();
+        bool any = false;
+        for (size_t i = a; i < b; i++) {
+            any = any || y[i] > 0.5f;
+            pos += y[i];
+        }
+        rejected += any ? 1 : 0;
+        natural += nat;
+        shell += sh;
+        learned += le;
+        cand += (double)cnt;
+        a = b;
+    }
+    // moves with no close candidate (or leaving their cell) need no exact test in any order
+    out->moves = moves;
+    out->moves_rejected = rejected;
+    out->pairs = P;
+    out->overlap_fraction = P ? pos / (double)P : 0;
+    out->candidates_per_move = cand / (double)moves;
+    out->tests_natural = natural / (double)moves;
+    out->tests_nearest_first = shell / (double)moves;
+    out->tests_learned = learned / (double)moves;
+    return RP_OK;
+}
+
+int rp_mlp_get_weights(rp_ctx *c, float *theta) {
+    if (!c || !theta) return fail(RP_ERR_PRECONDITION, "null argument");
+    if (!c->mlpTrained) return fail(RP_ERR_PRECONDITION, "the ordering network is not trained (rp_mlp_train)");
+    RP_HIP(hipMemcpy(theta, c->mlpTheta, sizeof(float) * MLP_PARAMS, hipMemcpyDeviceToHost));
+    return RP_OK;
+}
+
+int rp_mlp_set_weights(rp_ctx *c, const float *theta) {
+    if (!c || !theta) return fail(RP_ERR_PRECONDITION, "null argument");
+    if (!c->mlpTheta) RP_HIP(hipMalloc(&c->mlpTheta, sizeof(float) * 4 * MLP_PARAMS));
+    RP_HIP(hipMemcpy(c->mlpTheta, theta, sizeof(float) * MLP_PARAMS, hipMemcpyHostToDevice));
+    c->mlpTrained = true;
+    return RP_OK;
+}
+
+int rp_mlp_pairs(rp_ctx *c, size_t moves, size_t cap, float *x, float *y, int *meta, size_t *n) {
+    if (!c || !n) return fail(RP_ERR_PRECONDITION, "null argument");
+    if (c->k.kind != RP_SHAPE_POLYSPHERE)
+        return fail(RP_ERR_UNSUPPORTED, "the learned ordering is built for polysphere sweeps");
+    if (!c->envSet) return fail(RP_ERR_PRECONDITION, "environment not set");
+    if (moves == 0 || moves > (1u << 24)) return fail(RP_ERR_PRECONDITION, "moves must be 1 .. 2^24");
+    size_t P = 0;
+    int r = mlpSample(c, (int)moves, c->mlpEval, 0, P);
+    if (r) return r;
+    *n = P;
+    if (P > cap) return fail(RP_ERR_PRECONDITION, "capacity below the number of pairs");
+    if (P == 0) return RP_OK;
+    if (!x || !y || !meta) return fail(RP_ERR_PRECONDITION, "null pair array");
+    RP_HIP(hipMemcpy(x, c->mlpEval.x, sizeof(float) * MLP_FEATURES * P, hipMemcpyDeviceToHost));
+    RP_HIP(hipMemcpy(y, c->mlpEval.y, sizeof(float) * P, hipMemcpyDeviceToHost));
+    RP_HIP(hipMemcpy(meta, c->mlpEval.meta, sizeof(int) * 3 * P, hipMemcpyDeviceToHost));
+    return RP_OK;
+}
+
+int rp_mlp_gradient(const float *theta, size_t n, const float *x, const float *y, double *loss, float *grad) {
+    if (!theta || !loss || !grad || (n && (!x || !y))) return fail(RP_ERR_PRECONDITION, "null argument");
+    if (n == 0 || n > (1u << 20)) return fail(RP_ERR_PRECONDITION, "batch size must be 1 .. 2^20");
+    const int B = (int)n;
+    const size_t lossBlocks = (size_t)mlp_loss_blocks(B);
+    // theta, grad, x, y, 9 activation arrays of B x 64, dz, loss partials
+    const size_t floats = 2 * (size_t)MLP_PARAMS + (size_t)B * (MLP_IN + 1) + 9 * (size_t)B * MLP_H + B + lossBlocks;
+    float *d = nullptr;
+    RP_HIP(hipMalloc(&d, sizeof(float) * floats));
+    float *dTheta = d, *dGrad = dTheta + MLP_PARAMS, *dX = dGrad + MLP_PARAMS, *dY = dX + (size_t)B * MLP_IN;
+    MlpWork w;
+    float *q = dY + B;
+    float **arr[9] = {&w.pre1, &w.a1, &w.pre2, &w.a2, &w.pre3, &w.a3, &w.d1, &w.d2, &w.d3};
+    for (auto p : arr) {
+        *p = q;
+        q += (size_t)B * MLP_H;
+    }
+    w.dz = q;
+    w.lossPart = q + B;
+    std::vector<float> parts(lossBlocks);
+    hipError_t e = hipMemcpy(dTheta, theta, sizeof(float) * MLP_PARAMS, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dX, x, sizeof(float) * (size_t)B * MLP_IN, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dY, y, sizeof(float) * (size_t)B, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = mlp_forward_backward(nullptr, dTheta, B, dX, dY, w, dGrad);
+    if (e == hipSuccess) e = hipMemcpy(grad, dGrad, sizeof(float) * MLP_PARAMS, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(parts.data(), w.lossPart, sizeof(float) * lossBlocks, hipMemcpyDeviceToHost);
+    hipFree(d);
+    if (e != hipSuccess) return fail(RP_ERR_DEVICE, std::string("mlp gradient: ") + hipGetErrorString(e));
+    double sum = 0;
+    for (float v : parts) sum += v;
+    *loss = sum / (double)B;
+    return RP_OK;
+}
+
+int rp_mlp_infer(const float *theta, size_t n, const float *x, float *score) {
+    if (!theta || (n && (!x || !score))) return fail(RP_ERR_PRECONDITION, "null argument");
+    if (n == 0) return RP_OK;
+    if (n > (size_t)0x7fffffff / MLP_IN) return fail(RP_ERR_PRECONDITION, "too many pairs");
+    float *d = nullptr;
+    RP_HIP(hipMalloc(&d, sizeof(float) * (MLP_PARAMS + n * (MLP_IN + 1))));
+    float *dTheta = d, *dX = d + MLP_PARAMS, *dS = dX + n * MLP_IN;
+    hipError_t e = hipMemcpy(dTheta, theta, sizeof(float) * MLP_PARAMS, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dX, x, sizeof(float) * n * MLP_IN, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = mlp_infer(nullptr, dTheta, (int)n, dX, dS);
+    if (e == hipSuccess) e = hipMemcpy(score, dS, sizeof(float) * n, hipMemcpyDeviceToHost);
+    hipFree(d);
+    if (e != hipSuccess) return fail(RP_ERR_DEVICE, std::string("mlp inference: ") + hipGetErrorString(e));
+    return RP_OK;
 }
 
 int rp_set_precision(rp_ctx *c, int precision) {

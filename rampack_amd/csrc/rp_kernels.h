@@ -86,7 +86,10 @@ struct SweepParams {
     // range2 upper} bounds of the FP32 guard bands, s = 0 for cells whose stencil was staged translated (no wrapped
     // particle), s = 1 otherwise (positions and image translations up to a box length from the centre).
     float g32Lim[2][4];
+    double mlpInvRange;  // candidate pairs of the learned ordering: 1 / total interaction range (feature scale)
 };
+
+constexpr int MLP_FEATURES = 12;
 
 // Phases of a sweep workgroup timed with the device wall clock when SweepParams::phases is set.
 enum SweepPhase : int {
@@ -178,6 +181,13 @@ hipError_t launch_nematic_sums(hipStream_t s, const Particles &p, int n, double 
 
 // Applies fix_rotation to every orientation (no overlap test: soft shapes have no hard part).
 hipError_t launch_fix_all(hipStream_t s, const Particles &p, int n);
+
+// Candidate pairs of the learned ordering for M sampled trial moves on the state of sp (polyspheres; sp.g the sweep
+// grid the records are sorted by): mode 0 writes counts[q], mode 1 the pairs of move q from offsets[q] on
+// (x: 12 features, y: exact decision, meta: move, natural rank, nearest-first key).
+hipError_t launch_mlp_pairs(hipStream_t s, const SweepParams &sp, int M, uint64_t seed, uint32_t stream, double ts,
+                            double rs, int mode, int *counts, const long long *offsets, float *x, float *y,
+                            int *meta);
 
 // Workgroups of the sweep kernel one compute unit can hold at this LDS size (occupancy API).
 int sweep_occupancy(int kind, int cap, int waves);

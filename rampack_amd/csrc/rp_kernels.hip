@@ -2465,6 +2465,120 @@ __global__ __launch_bounds__(WAVE *WAVES, WAVES >= 8 ? 4 : 1) void k_sweep(Sweep
 }
 
 // ---------------------------------------------------------------------------------------------------------------
+// Candidate pairs of the learned ordering (polyspheres).  Move q: a uniform pick among the n sorted records and a
+// rototranslation drawn as k_proposals draws one (translation half-width ts, rotation half-angle rs), from its own
+// Philox stream (seed, stream, q); a trial pose that leaves its checkerboard cell is never tested and has no pairs.
+// The close candidates of a trial pose -- those within the total interaction range under the stencil's image
+// translation (poly_hit's first test), the particle itself skipped -- in natural stencil order (the own cell, then
+// the others in lexicographic order of their offsets; records in storage order within a cell), each with
+//   12 features: the candidate's position in the trial body frame divided by the total range (NO^T (p2 + T - npos)),
+//                and the relative rotation NO^T O_j (row-major),
+//   the exact decision poly_hit (1 = the bead pairs overlap),
+//   meta: move q, its rank in natural order, and its key in the nearest-first order (shell position of its cell
+//         << 20 | rank within the cell).
+// mode 0 counts the close candidates of every move (counts[q]); mode 1 writes them from offsets[q] on.
+constexpr int MLP_PAIR_THREADS = 64;
+
+__global__ __launch_bounds__(MLP_PAIR_THREADS) void k_mlp_pairs(SweepParams P, int M, uint64_t seed, uint32_t stream,
+                                                                double ts, double rs, int mode, int *counts,
+                                                                const long long *offsets, float *x, float *y,
+                                                                int *meta) {
+    __shared__ double sT[81];
+    __shared__ double sBeads[4 * RP_MAX_CENTRES];
+    __shared__ double sTrialAll[MLP_PAIR_THREADS][3 * RP_MAX_CENTRES];
+    __shared__ unsigned char sShellPos[MAX_STENCIL];
+    const Grid &g = P.g;
+    const int tid = threadIdx.x;
+    for (int i = tid; i < 81; i += blockDim.x) sT[i] = g.T[i];
+    for (int i = tid; i < 4 * P.k.nc; i += blockDim.x) sBeads[i] = P.k.beads[i];
+    for (int i = tid; i < MAX_STENCIL; i += blockDim.x) sShellPos[kShellOrder[i]] = (unsigned char)i;
+    __syncthreads();
+    const int q = blockIdx.x * blockDim.x + tid;
+    if (q >= M) return;
+    double *sTrial = sTrialAll[tid];
+    const int n = g.cellStart[g.ncell];
+    Philox rng = philox_init(seed ^ 0x6d6c7061697273ull, stream, 0xfffffffeu, (uint32_t)q);
+    const int gi = (int)philox_below(rng, (uint32_t)n);
+    V3 t;
+    t.x = philox_uniform(rng, -ts, ts);
+    t.y = philox_uniform(rng, -ts, ts);
+    t.z = philox_uniform(rng, -ts, ts);
+    V3 axis;
+    do {
+        axis.x = philox_uniform(rng, -1, 1);
+        axis.y = philox_uniform(rng, -1, 1);
+        axis.z = philox_uniform(rng, -1, 1);
+    } while (dot(axis, axis) > 1 || dot(axis, axis) == 0);
+    double R[9], O[9], NO[9];
+    rodrigues(axis, philox_uniform(rng, -rs, rs), R);
+    for (int c = 0; c < 9; c++) O[c] = P.p.o[c][gi];
+    mulm(R, O, NO);
+    const V3 npos = translate_wrapped(g.dims, g.inv, load3(P.p.x, P.p.y, P.p.z, gi), t);
+    int nc[3];
+    unsigned nw;
+    bin_position(g, npos, nc, nw);
+    const int cell = P.p.cell[gi];
+    int found = 0;
+    if (cell_index(g, nc[0], nc[1], nc[2]) == cell) {
+        for (int a = 0; a < P.k.nc; a++) {
+            const V3 c1 = add(npos, mulv(NO, v3(sBeads[4 * a], sBeads[4 * a + 1], sBeads[4 * a + 2])));
+            sTrial[3 * a] = c1.x;
+            sTrial[3 * a + 1] = c1.y;
+            sTrial[3 * a + 2] = c1.z;
+        }
+        const int cx = cell % g.n[0], cy = (cell / g.n[0]) % g.n[1], cz = cell / (g.n[0] * g.n[1]);
+        const int k = g.reach, side = 2 * k + 1, centre = (side * side * side - 1) / 2;
+        const int ncells = side * side * side;
+        const int wx = nw & 1, wy = (nw >> 1) & 1, wz = (nw >> 2) & 1;
+        const long long out0 = mode == 1 ? offsets[q] : 0;
+        for (int slot = 0; slot < ncells; slot++) {
+            const int tt = slot == 0 ? centre : (slot <= centre ? slot - 1 : slot);
+            int ex, ey, ez;
+            const int mx = wrap_axis(cx, tt / (side * side) - k, g.n[0], ex);
+            const int my = wrap_axis(cy, (tt / side) % side - k, g.n[1], ey);
+            const int mz = wrap_axis(cz, tt % side - k, g.n[2], ez);
+            const int mc = cell_index(g, mx, my, mz);
+            const int ecode = (ex + 1) | ((ey + 1) << 2) | ((ez + 1) << 4);
+            const int b0 = g.cellStart[mc], b1 = g.cellStart[mc + 1];
+            for (int gj = b0; gj < b1; gj++) {
+                if (gj == gi) continue;
+                const V3 p2 = load3(P.p.x, P.p.y, P.p.z, gj);
+                const V3 T = image_translation(sT, image_code(ecode, P.p.wbits[gj]), wx, wy, wz);
+                const V3 d = sub(add(p2, T), npos);
+                if (!(dot(d, d) < P.k.range2)) continue;
+                if (mode == 1) {
+                    const long long o = out0 + found;
+                    double Oj[9];
+                    for (int c = 0; c < 9; c++) Oj[c] = P.p.o[c][gj];
+                    float *f = x + o * MLP_FEATURES;
+                    // NO^T d and NO^T O_j
+                    for (int r = 0; r < 3; r++)
+                        f[r] = (float)((NO[r] * d.x + NO[3 + r] * d.y + NO[6 + r] * d.z) * P.mlpInvRange);
+                    for (int r = 0; r < 3; r++)
+                        for (int c = 0; c < 3; c++)
+                            f[3 + 3 * r + c] = (float)(NO[r] * Oj[c] + NO[3 + r] * Oj[3 + c] + NO[6 + r] * Oj[6 + c]);
+                    y[o] = poly_hit(P.k, sBeads, sTrial, npos, p2, P.p, gj, T) ? 1.f : 0.f;
+                    meta[3 * o] = q;
+                    meta[3 * o + 1] = found;
+                    meta[3 * o + 2] = ((int)sShellPos[tt] << 20) | (gj - b0);
+                }
+                found++;
+            }
+        }
+    }
+    if (mode == 0) counts[q] = found;
+}
+
+hipError_t launch_mlp_pairs(hipStream_t s, const SweepParams &sp, int M, uint64_t seed, uint32_t stream, double ts,
+                            double rs, int mode, int *counts, const long long *offsets, float *x, float *y,
+                            int *meta) {
+    if (M <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_mlp_pairs, dim3((unsigned)((M + MLP_PAIR_THREADS - 1) / MLP_PAIR_THREADS)),
+                       dim3(MLP_PAIR_THREADS), 0, s, sp, M, seed, stream, ts, rs, mode, counts, offsets, x, y, meta);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------------------------------------
 // Nematic Q-tensor sums (NematicOrder::calculate, NematicOrder.cpp:9-17): sum over particles of a_i a_j for the
 // primary axis a = orientation column 2 (SpherocylinderTraits.cpp:52-54).  Each product is rounded once, as in the
 // reference; the sum runs as a fixed tree (block partials in a fixed order, then one workgroup), so a given state

@@ -133,3 +133,17 @@ def rod_replica_start(n: int, packing_fraction: float, radius: float = 0.5, leng
     box, pos = box * scale, np.ascontiguousarray(pos * scale)
     rot = np.tile(np.eye(3).reshape(1, 9), (n, 1))
     return box, pos, rot
+
+
+def kinked_chain_lattice(n_side, packing_fraction: float, n_beads: int = 11, kink_deg: float = 30.0,
+                         z_spacing: float = 11.2):
+    """Kinked chains of touching unit beads (kinked_chain), all with the identity orientation, on an n_side lattice
+    (sc): the in-plane spacing a follows from the packing fraction (n v_chain / (a^2 c n_xyz)), c = z_spacing along
+    the chains.  Returns (box, positions, orientations, centres, radii)."""
+    c, r = kinked_chain(n_beads, 1.0, kink_deg)
+    n = int(np.prod(n_side))
+    v_chain = n_beads * np.pi / 6
+    a = np.sqrt(n * v_chain / (packing_fraction * z_spacing * n))
+    box, pos = sc_positions(n_side, (a, a, z_spacing))
+    rot = np.tile(np.eye(3).reshape(1, 9), (len(pos), 1))
+    return box, pos, np.ascontiguousarray(rot), c, r

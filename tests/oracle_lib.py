@@ -62,6 +62,9 @@ def _setup_common(lib):
     lib.orc_rotation_sc.restype = None
     lib.orc_rotate.argtypes = [ctypes.c_size_t, _dp, _dp, _dp]
     lib.orc_rotate.restype = None
+    _fp = ctypes.POINTER(ctypes.c_float)
+    lib.orc_mlp_loss_grad.argtypes = [_fp, ctypes.c_size_t, _fp, _fp, _dp, _dp]
+    lib.orc_mlp_loss_grad.restype = ctypes.c_double
     lib.orc_sincos.argtypes = [ctypes.c_size_t, _dp, _dp]
     lib.orc_sincos.restype = None
     lib.orc_nematic_p2.argtypes = [ctypes.c_size_t, _dp]
@@ -295,3 +298,16 @@ def replay_sweep(kind: int, radius: float, length: float, rec: dict):
     if rc != 0:
         raise RuntimeError(f"orc_replay_sweep failed ({rc})")
     return dec, post_pos, post_rot
+
+
+def mlp_loss_grad(theta, x, y):
+    """The oracle's network: (mean binary cross-entropy, gradient (9217), logits) in double precision."""
+    fp = ctypes.POINTER(ctypes.c_float)
+    theta = np.ascontiguousarray(theta, dtype=np.float32)
+    x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 12)
+    y = np.ascontiguousarray(y, dtype=np.float32)
+    grad = np.zeros(len(theta))
+    z = np.zeros(len(x))
+    loss = oracle().lib.orc_mlp_loss_grad(theta.ctypes.data_as(fp), len(x), x.ctypes.data_as(fp),
+                                          y.ctypes.data_as(fp), grad.ctypes.data_as(_dp), z.ctypes.data_as(_dp))
+    return loss, grad, z
