@@ -193,12 +193,13 @@ int rp_get_move_samplers(rp_ctx *ctx, rp_move_desc *out, size_t n, size_t *n_mov
  * Simulation.cpp:533-607.  Fails with RP_ERR_VALIDATION if the packing has overlaps (Simulation.cpp:131-132). */
 int rp_run_cycles(rp_ctx *ctx, size_t n_cycles, int adapt_steps, rp_counters *out);
 
-/* rp_run_cycles on n independent packings at once: the replicas of an equation-of-state scan (the reference runs one
- * `rampack casino` per state point; Simulation::integrate, src/core/Simulation.cpp:92-176, per replica), meant to
- * run each context on its own host thread and HIP stream so that small replicas fill the GPU together.  out
- * (nullable) receives n counter records.  A context may appear only once (RP_ERR_PRECONDITION otherwise).
- * SWITCHED OFF: concurrent replicas faulted the GPU on a 32-replica scan and the cause is not found, so after the
- * argument checks every call with n > 0 returns RP_ERR_UNSUPPORTED; run the replicas with rp_run_cycles. */
+/* rp_run_cycles on n independent packings together: the replicas of an equation-of-state scan (the reference runs
+ * one `rampack casino` per state point; Simulation::integrate, src/core/Simulation.cpp:92-176, per replica).  One
+ * host thread and one stream (the first context's) carry every replica's work for the call: each colour pass of all
+ * replicas runs as one launch per sweep kernel (hard spheres and spherocylinders; the other sweeps one launch per
+ * replica), and one synchronisation per cycle decides every replica's box move.  Each replica ends bit for bit where
+ * rp_run_cycles alone takes it.  out (nullable) receives n counter records.  A context may appear only once, all
+ * must have an environment and live on one device (RP_ERR_PRECONDITION otherwise; messages name the replica). */
 int rp_run_replicas(rp_ctx *const *ctxs, size_t n, size_t n_cycles, int adapt_steps, rp_counters *out);
 
 /* Decision record of one cycle's particle moves (no counterpart in the reference: a test hook through which a CPU

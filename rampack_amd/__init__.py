@@ -705,9 +705,9 @@ def _aux_text(aux) -> bytes:
 
 
 def run_replicas(packings, n_cycles: int, adapt_steps: bool = False) -> list:
-    """GpuPacking.run_cycles on every packing at once (rp_run_replicas), one counter dict per packing.  Switched off:
-    concurrent replicas faulted the GPU on a 32-replica scan with the cause not found, so any non-empty list raises
-    RampackError (RP_ERR_UNSUPPORTED); run the packings one by one with run_cycles."""
+    """GpuPacking.run_cycles on every packing together (rp_run_replicas), one counter dict per packing: each colour
+    pass of all packings in one launch per kernel, one host thread and one stream.  Every packing ends bit for bit
+    where its own run_cycles call would take it."""
     packings = list(packings)
     ctxs = (ctypes.c_void_p * len(packings))(*[p._ctx.value for p in packings])
     out = (Counters * len(packings))()

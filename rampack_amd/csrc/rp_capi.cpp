@@ -530,8 +530,8 @@ int countCapLimit(rp_ctx *c) {
     return std::max(64, (int)((double)(c->ldsLimit - fixed) / per) / 64 * 64);
 }
 
-// Early-exit overlap scan of the `alt` arrays sorted by `g`; sets found.
-int scanItems(rp_ctx *c, const Grid &g, const Particles &items, size_t nItems, int earlyExit, uint64_t &count) {
+// Overlap count of `items` sorted by `g`, enqueued with the copy of the counters to the host (no synchronisation).
+int scanLaunch(rp_ctx *c, const Grid &g, const Particles &items, size_t nItems, int earlyExit) {
     RP_HIP(launch_fill_u64(c->stream, c->dCounters + CNT_OVERLAPS, 1, 0));
     RP_HIP(launch_fill_u64(c->stream, c->dCounters + CNT_STAGE_OVERFLOW, 1, 0));
     double lambda = (double)nItems / (double)g.ncell;
@@ -541,12 +541,25 @@ int scanItems(rp_ctx *c, const Grid &g, const Particles &items, size_t nItems, i
     cap = std::min((cap + 63) / 64 * 64, countCapLimit(c));
     RP_HIP(launch_count_overlaps(c->stream, g, (int)nItems, items, c->k, earlyExit, c->dCounters, cap,
                                  timingSlot(c, 2)));
-    int r = readCounters(c);
-    if (r) return r;
+    RP_HIP(hipMemcpyAsync(c->hCounters, c->dCounters, sizeof(unsigned long long) * CNT_NUM, hipMemcpyDeviceToHost,
+                          c->stream));
+    return RP_OK;
+}
+
+// The count of a scanLaunch, once the stream has been synchronised.
+int scanResult(rp_ctx *c, uint64_t &count) {
     if (c->hCounters[CNT_STAGE_OVERFLOW])
         return fail(RP_ERR_UNSUPPORTED, "a grid cell holds more particles than the overlap count can stage");
     count = c->hCounters[CNT_OVERLAPS];
     return RP_OK;
+}
+
+// Early-exit overlap scan of the `alt` arrays sorted by `g`; sets found.
+int scanItems(rp_ctx *c, const Grid &g, const Particles &items, size_t nItems, int earlyExit, uint64_t &count) {
+    int r = scanLaunch(c, g, items, nItems, earlyExit);
+    if (r) return r;
+    RP_HIP(hipStreamSynchronize(c->stream));
+    return scanResult(c, count);
 }
 
 int scanOverlaps(rp_ctx *c, const Grid &g, int earlyExit, uint64_t &count) {
@@ -658,9 +671,9 @@ int totalEnergy(rp_ctx *c, const Grid &g, const Particles &p, double &out) {
     return RP_OK;
 }
 
-int boxMove(rp_ctx *c, bool &accepted) {
-    accepted = false;
-    HostBox nb;
+// A box move's draws from the context's host stream: the new box nb, the Metropolis exponent (without the energy
+// change), the uniform u of the test and the origin shift of the new grid.
+int drawBoxMove(rp_ctx *c, HostBox &nb, double &exponent, double &u, double shiftUnit[3]) {
     double nd[9];
     std::memcpy(nd, c->box.dims, sizeof nd);
     if (c->boxKind == RP_BOX_DELTA_TRICLINIC) {
@@ -717,11 +730,31 @@ int boxMove(rp_ctx *c, bool &accepted) {
     if (newV == 0) return fail(RP_ERR_PRECONDITION, "box move produced a zero-volume box");
     double dV = newV - oldV, factor = newV / oldV;
     double N = (double)c->n;
-    double exponent = N * std::log(factor) - c->pressure * dV / c->temperature;
-    double u = philox_unit(c->hostRng);
-    double shiftUnit[3] = {philox_unit(c->hostRng), philox_unit(c->hostRng), philox_unit(c->hostRng)};
-    if (c->k.soft) return softBoxMove(c, nb, exponent, u, shiftUnit, accepted);
-    if (c->counting) return countingBoxMove(c, nb, exponent, u, shiftUnit, accepted);
+    exponent = N * std::log(factor) - c->pressure * dV / c->temperature;
+    u = philox_unit(c->hostRng);
+    for (int d = 0; d < 3; d++) shiftUnit[d] = philox_unit(c->hostRng);
+    return RP_OK;
+}
+
+// A hard packing's box move whose overlap scan is in flight: decided by boxMoveEnd after a stream synchronisation.
+struct BoxTrial {
+    bool pending = false;
+    HostBox nb;
+    Grid g{};
+};
+
+void acceptBox(rp_ctx *c, const HostBox &nb, const Grid &g, bool &accepted) {
+    c->box = nb;
+    swapBuffers(c);
+    c->grid = g;
+    c->sortedForCycle = true;
+    accepted = true;
+}
+
+// Box move of a hard packing from its draws: the Metropolis test, the rescaled cell list and the overlap scan.
+// With `defer` the scan is only launched (defer->pending); boxMoveEnd decides once the stream is synchronised.
+int hardBoxMove(rp_ctx *c, const HostBox &nb, double exponent, double u, const double shiftUnit[3], bool &accepted,
+                BoxTrial *defer) {
     if (u > std::exp(exponent)) return RP_OK;
 
     Grid g{};
@@ -741,16 +774,42 @@ int boxMove(rp_ctx *c, bool &accepted) {
     if (!cannotOverlap) {
         uint64_t found = 0;
         c->boxScans++;
+        if (defer && !poly) {
+            r = scanLaunch(c, g, c->alt.p, c->n, 1);
+            if (r) return r;
+            defer->pending = true;
+            defer->nb = nb;
+            defer->g = g;
+            return RP_OK;
+        }
         r = poly ? countBeads(c, c->alt.p, nb, 1, found) : scanOverlaps(c, g, 1, found);
         if (r) return r;
         if (found) return RP_OK;
     }
-    c->box = nb;
-    swapBuffers(c);
-    c->grid = g;
-    c->sortedForCycle = true;
-    accepted = true;
+    acceptBox(c, nb, g, accepted);
     return RP_OK;
+}
+
+int boxMoveEnd(rp_ctx *c, BoxTrial &t, bool &accepted) {
+    if (!t.pending) return RP_OK;
+    t.pending = false;
+    uint64_t found = 0;
+    int r = scanResult(c, found);
+    if (r) return r;
+    if (!found) acceptBox(c, t.nb, t.g, accepted);
+    return RP_OK;
+}
+
+// One box move (Simulation::tryScaling, Simulation.cpp:508-517).  defer: see hardBoxMove (hard packings only).
+int boxMove(rp_ctx *c, bool &accepted, BoxTrial *defer = nullptr) {
+    accepted = false;
+    HostBox nb;
+    double exponent = 0, u = 0, shiftUnit[3] = {0, 0, 0};
+    int r = drawBoxMove(c, nb, exponent, u, shiftUnit);
+    if (r) return r;
+    if (c->k.soft) return softBoxMove(c, nb, exponent, u, shiftUnit, accepted);
+    if (c->counting) return countingBoxMove(c, nb, exponent, u, shiftUnit, accepted);
+    return hardBoxMove(c, nb, exponent, u, shiftUnit, accepted, defer);
 }
 
 int prepareCycleGrid(rp_ctx *c) {
@@ -865,13 +924,30 @@ size_t cycleProposals(const rp_ctx *c) {
     return moves > 0 ? (size_t)c->grid.ncell * moves : c->n * (size_t)nSamplers;
 }
 
-// The particle moves of one cycle: proposals, then the colour passes in random order.  logDecisions: every
-// proposal's acceptance flag is written (prop.acc) even with a single move sampler.
-int sweepCycle(rp_ctx *c, bool logDecisions, SweepInfo *info) {
+// One cycle's particle moves, planned: the launch parameters of every colour pass (all but the colour), the launch
+// shape, the colour order and the number of proposals.
+struct SweepPlan {
+    SweepParams sp{};
+    int waves = 1, g32 = 0, colours = 0;
+    int order[27] = {};
+    size_t proposals = 0;
+};
+
+// Whether colour `colour` of grid g has active cells: a colour with odd parity on a one-cell axis has none.
+bool colourHasCells(const Grid &g, int colour) {
+    const int par[3] = {colour % g.period, (colour / g.period) % g.period, colour / (g.period * g.period)};
+    return !((g.n[0] == 1 && par[0]) || (g.n[1] == 1 && par[1]) || (g.n[2] == 1 && par[2]));
+}
+
+// The first part of a cycle's particle moves: the cycle's grid, the proposals (launched), the launch shape and the
+// colour order (drawn from the context's host stream).  logDecisions: every proposal's acceptance flag is written
+// (prop.acc) even with a single move sampler.
+int planSweep(rp_ctx *c, bool logDecisions, SweepPlan &pl) {
     int r = prepareCycleGrid(c);
     if (r) return r;
     const Grid &g = c->grid;
-    SweepParams sp{};
+    SweepParams &sp = pl.sp;
+    sp = SweepParams{};
     sp.g = g;
     sp.p = c->cur.p;
     sp.k = c->k;
@@ -895,11 +971,9 @@ int sweepCycle(rp_ctx *c, bool logDecisions, SweepInfo *info) {
     // (with k samplers, k times as many: each requests N moves)
     sp.moves = lambda >= 3 ? (int)std::lround(lambda) * ms.n : 0;
     sp.mult = ms.n;
-    const size_t proposals = sp.moves > 0 ? (size_t)g.ncell * sp.moves : c->n * (size_t)ms.n;
-    {
-        int r2 = ensureProposals(c, proposals);
-        if (r2) return r2;
-    }
+    pl.proposals = sp.moves > 0 ? (size_t)g.ncell * sp.moves : c->n * (size_t)ms.n;
+    r = ensureProposals(c, pl.proposals);
+    if (r) return r;
     sp.pr = c->prop;
     if (ms.n == 1) {
         sp.pr.type = nullptr;  // one sampler: the totals are its counters
@@ -945,42 +1019,27 @@ int sweepCycle(rp_ctx *c, bool logDecisions, SweepInfo *info) {
     sp.tuning = c->sweepTuning;
     sp.temperature = c->temperature;
     sp.counting = c->counting && !c->k.soft ? 1 : 0;
-    const int colours = g.period * g.period * g.period;
-    int order[27];
-    for (int i = 0; i < colours; i++) order[i] = i;
-    for (int i = colours - 1; i > 0; i--) {
+    pl.waves = waves;
+    pl.g32 = g32;
+    pl.colours = g.period * g.period * g.period;
+    for (int i = 0; i < pl.colours; i++) pl.order[i] = i;
+    for (int i = pl.colours - 1; i > 0; i--) {
         int j = (int)philox_below(c->hostRng, (uint32_t)(i + 1));
-        std::swap(order[i], order[j]);
-    }
-    if (info) {
-        info->colours = colours;
-        std::memcpy(info->order, order, sizeof order);
-        info->moves = sp.moves;
-        info->mult = sp.mult;
-        info->proposals = proposals;
+        std::swap(pl.order[i], pl.order[j]);
     }
     c->tstats.lds_limit = c->ldsLimit;
     c->tstats.sweep_cap = (uint64_t)sp.cap;
     c->tstats.sweep_waves = (uint64_t)waves;
     c->tstats.sweep_period = (uint64_t)g.period;
-    for (int i = 0; i < colours; i++) {
-        sp.colour = order[i];
-        // a colour with odd parity on a one-cell axis has no cells
-        const int par[3] = {sp.colour % g.period, (sp.colour / g.period) % g.period, sp.colour / (g.period * g.period)};
-        if ((g.n[0] == 1 && par[0]) || (g.n[1] == 1 && par[1]) || (g.n[2] == 1 && par[2])) continue;
-        const bool trace = !c->tracePath.empty() && ++c->sweepLaunches == c->traceLaunch;
-        if (trace) {
-            r = traceSweep(c, sp, waves);
-            if (r) return r;
-        } else {
-            // sampled timing: the i-th colour pass of cycle c is timed when i == c mod (colour count)
-            const bool timed = c->timing == 1 || (c->timing == RP_TIMING_SAMPLED && i == (int)(c->cycle % colours));
-            RP_HIP(launch_sweep(c->stream, sp, waves, timed ? timingSlot(c, 0) : Timed{}, g32));
-        }
-    }
+    return RP_OK;
+}
+
+// The last part of a cycle's particle moves, after its colour passes: per-sampler counts and, in overlap-counting
+// mode, the cached overlap count.
+int finishSweep(rp_ctx *c, const SweepPlan &pl) {
     c->sortedForCycle = false;
-    RP_HIP(launch_type_counts(c->stream, sp.pr, proposals, c->dCounters));
-    if (sp.counting) {
+    RP_HIP(launch_type_counts(c->stream, pl.sp.pr, pl.proposals, c->dCounters));
+    if (pl.sp.counting) {
         // Packing::acceptMove adds each accepted move's overlap change to the cached count (Packing.cpp:298-301)
         unsigned long long d = 0;
         RP_HIP(hipMemcpyAsync(&d, c->dCounters + CNT_OVERLAP_DELTA, sizeof d, hipMemcpyDeviceToHost, c->stream));
@@ -991,14 +1050,40 @@ int sweepCycle(rp_ctx *c, bool logDecisions, SweepInfo *info) {
     return RP_OK;
 }
 
-// One cycle: the particle moves (colour passes), then one box move.
-int runCycle(rp_ctx *c) {
-    int r = sweepCycle(c, false, nullptr);
+// The particle moves of one cycle: proposals, then the colour passes in random order.  logDecisions: every
+// proposal's acceptance flag is written (prop.acc) even with a single move sampler.
+int sweepCycle(rp_ctx *c, bool logDecisions, SweepInfo *info) {
+    SweepPlan pl;
+    int r = planSweep(c, logDecisions, pl);
     if (r) return r;
+    if (info) {
+        info->colours = pl.colours;
+        std::memcpy(info->order, pl.order, sizeof pl.order);
+        info->moves = pl.sp.moves;
+        info->mult = pl.sp.mult;
+        info->proposals = pl.proposals;
+    }
+    SweepParams sp = pl.sp;
+    for (int i = 0; i < pl.colours; i++) {
+        sp.colour = pl.order[i];
+        if (!colourHasCells(sp.g, sp.colour)) continue;
+        const bool trace = !c->tracePath.empty() && ++c->sweepLaunches == c->traceLaunch;
+        if (trace) {
+            r = traceSweep(c, sp, pl.waves);
+            if (r) return r;
+        } else {
+            // sampled timing: the i-th colour pass of cycle c is timed when i == c mod (colour count)
+            const bool timed = c->timing == 1 || (c->timing == RP_TIMING_SAMPLED && i == (int)(c->cycle % pl.colours));
+            RP_HIP(launch_sweep(c->stream, sp, pl.waves, timed ? timingSlot(c, 0) : Timed{}, pl.g32));
+        }
+    }
+    return finishSweep(c, pl);
+}
+
+// One cycle: the particle moves (colour passes), then one box move.
+// The end of a cycle: box-move counters (when a box move was tried) and the cycle number.
+void endCycle(rp_ctx *c, bool acc) {
     if (c->boxKind != RP_BOX_NONE) {
-        bool acc = false;
-        r = boxMove(c, acc);
-        if (r) return r;
         c->boxAttempted++;
         c->boxSinceEval++;
         if (acc) {
@@ -1008,6 +1093,17 @@ int runCycle(rp_ctx *c) {
     }
     c->cycle++;
     c->cycles++;
+}
+
+int runCycle(rp_ctx *c) {
+    int r = sweepCycle(c, false, nullptr);
+    if (r) return r;
+    bool acc = false;
+    if (c->boxKind != RP_BOX_NONE) {
+        r = boxMove(c, acc);
+        if (r) return r;
+    }
+    endCycle(c, acc);
     return RP_OK;
 }
 
@@ -1103,6 +1199,260 @@ int countExact(rp_ctx *c, int earlyExit, uint64_t &out) {
     return RP_OK;
 }
 
+
+// Simulation::integrate's check before the first cycle (Simulation.cpp:127-132): no overlap on the reference's grid,
+// or on the checkerboard grid where the reference's grid is off; nothing to check for a purely soft interaction or in
+// overlap-counting mode.
+int validateStart(rp_ctx *c) {
+    uint64_t overl = 0;
+    int r = RP_OK;
+    if ((!c->k.soft || c->k.hardCore) && !c->counting) r = countExact(c, 1, overl);
+    if (r == RP_ERR_UNSUPPORTED) {
+        double zero[3] = {0, 0, 0};
+        c->sortedForCycle = false;
+        Grid g{};
+        r = chooseSweepGrid(c, c->box, zero, g);
+        if (r) return r;
+        r = rebuild(c, g, 0, nullptr);
+        if (r) return r;
+        r = scanOverlaps(c, g, 1, overl);
+    }
+    if (r) return r;
+    if (overl)
+        return fail(RP_ERR_VALIDATION,
+                    "Overlaps are present at the start of integration. Perform overlap reduction beforehand.");
+    return RP_OK;
+}
+
+// After each cycle: the periodic orientation renormalisation (Simulation.cpp:144-145; in counting mode the
+// reference's renormalisation keeps the orientations, Packing.cpp:1179-1190, so none is run) and the step
+// adaptation.
+int afterCycle(rp_ctx *c, bool adapt) {
+    if (c->fixEvery && !c->counting && c->cycles % c->fixEvery == 0) {
+        int r = rp_fix_orientations(c, nullptr);
+        if (r) return r;
+    }
+    return adapt ? adaptSteps(c) : RP_OK;
+}
+
+int reportCounters(rp_ctx *c, rp_counters *out) {
+    int r = readCounters(c);
+    if (r) return r;
+    if (!out) return RP_OK;
+    out->cycles = c->cycles;
+    out->moves_attempted = c->hCounters[CNT_ATTEMPTED];
+    out->moves_accepted = c->hCounters[CNT_ACCEPTED];
+    out->moves_left_cell = c->hCounters[CNT_LEFT_CELL];
+    out->box_attempted = c->boxAttempted;
+    out->box_accepted = c->boxAccepted;
+    out->box_scans = c->boxScans;
+    out->box_too_small = c->boxTooSmall;
+    out->unstaged_blocks = c->hCounters[CNT_UNSTAGED_BLOCKS];
+    out->pair_tests = c->hCounters[CNT_TESTS];
+    // the first sampler that translates and the first that rotates
+    out->trans_step = out->rot_step = 0;
+    for (auto it = c->samplers.rbegin(); it != c->samplers.rend(); ++it) {
+        const double maxT = it->maxTransImplicit ? minHeight(c->box) / 2 : it->maxTransStep;
+        if (it->kind != RP_MOVE_ROTATION) out->trans_step = std::min(maxT, it->transStep);
+        if (it->kind != RP_MOVE_TRANSLATION) out->rot_step = it->rotStep;
+    }
+    out->box_step = c->boxStep;
+    return RP_OK;
+}
+
+// Device and pinned host memory of a replica run: the launch tables of one cycle's colour passes.
+struct BatchMem {
+    SweepParams *dTable = nullptr, *hTable = nullptr;
+    int *dFirst = nullptr, *hFirst = nullptr;
+    size_t tableCap = 0, firstCap = 0;
+    hipEvent_t uploaded = nullptr;
+    ~BatchMem() {
+        if (dTable) hipFree(dTable);
+        if (hTable) hipHostFree(hTable);
+        if (dFirst) hipFree(dFirst);
+        if (hFirst) hipHostFree(hFirst);
+        if (uploaded) hipEventDestroy(uploaded);
+    }
+    int reserve(size_t tables, size_t firsts) {
+        if (tables > tableCap) {
+            if (dTable) hipFree(dTable);
+            if (hTable) hipHostFree(hTable);
+            dTable = hTable = nullptr;
+            tableCap = 0;
+            RP_HIP(hipMalloc(&dTable, sizeof(SweepParams) * tables));
+            RP_HIP(hipHostMalloc(&hTable, sizeof(SweepParams) * tables, hipHostMallocDefault));
+            tableCap = tables;
+        }
+        if (firsts > firstCap) {
+            if (dFirst) hipFree(dFirst);
+            if (hFirst) hipHostFree(hFirst);
+            dFirst = hFirst = nullptr;
+            firstCap = 0;
+            RP_HIP(hipMalloc(&dFirst, sizeof(int) * firsts));
+            RP_HIP(hipHostMalloc(&hFirst, sizeof(int) * firsts, hipHostMallocDefault));
+            firstCap = firsts;
+        }
+        return RP_OK;
+    }
+};
+
+// Every context's work on one stream for the run (the first replica's): one host thread orders all of it, and one
+// synchronisation per cycle decides every replica's box move.  The contexts' own streams are restored at the end.
+struct SharedStream {
+    std::vector<rp_ctx *> ctxs;
+    std::vector<hipStream_t> own;
+    std::vector<int> timing;
+    ~SharedStream() {
+        if (!ctxs.empty()) hipStreamSynchronize(ctxs[0]->stream);
+        for (size_t i = 0; i < ctxs.size(); i++) {
+            ctxs[i]->stream = own[i];
+            ctxs[i]->timing = timing[i];
+        }
+    }
+};
+
+// One colour pass's launches: the packings whose pass i has cells, grouped by kernel (sweep_batch_key), each group in
+// one launch; packings that run one per launch (soft, counting, polysphere, traced) are launched on their own.
+struct PassGroup {
+    int key;
+    size_t entry0, entries, first0;
+    unsigned blocks;
+    size_t lds;
+};
+
+int runReplicas(rp_ctx *const *ctxs, size_t n, size_t nCycles, bool adapt, rp_counters *out) {
+    SharedStream shared;
+    for (size_t i = 0; i < n; i++) RP_HIP(hipStreamSynchronize(ctxs[i]->stream));
+    for (size_t i = 0; i < n; i++) {
+        shared.ctxs.push_back(ctxs[i]);
+        shared.own.push_back(ctxs[i]->stream);
+        shared.timing.push_back(ctxs[i]->timing);
+        ctxs[i]->stream = ctxs[0]->stream;
+        ctxs[i]->timing = 0;  // per-launch timing events belong to single-packing runs
+    }
+    const hipStream_t s = ctxs[0]->stream;
+    auto named = [&](size_t i, int r) {
+        return r ? fail(r, "replica " + std::to_string(i) + ": " + rp_last_error()) : RP_OK;
+    };
+    for (size_t i = 0; i < n; i++) {
+        int r = validateStart(ctxs[i]);
+        if (r) return named(i, r);
+    }
+    BatchMem mem;
+    RP_HIP(hipEventCreateWithFlags(&mem.uploaded, hipEventDisableSystemFence));
+    bool uploadedOnce = false;
+    std::vector<SweepPlan> plans(n);
+    std::vector<BoxTrial> trials(n);
+    std::vector<char> acc(n);
+    std::vector<PassGroup> groups;
+    std::vector<std::pair<int, size_t>> members;  // (key, replica) of one pass
+    for (size_t cyc = 0; cyc < nCycles; cyc++) {
+        int maxColours = 0;
+        for (size_t i = 0; i < n; i++) {
+            int r = planSweep(ctxs[i], false, plans[i]);
+            if (r) return named(i, r);
+            maxColours = std::max(maxColours, plans[i].colours);
+        }
+        // the staging tables are rewritten once the previous cycle's upload has completed
+        if (uploadedOnce) RP_HIP(hipEventSynchronize(mem.uploaded));
+        {
+            int r = mem.reserve((size_t)maxColours * n, (size_t)maxColours * 2 * n);
+            if (r) return r;
+        }
+        groups.clear();
+        size_t entries = 0, firsts = 0;
+        for (int pass = 0; pass < maxColours; pass++) {
+            members.clear();
+            for (size_t i = 0; i < n; i++) {
+                const SweepPlan &pl = plans[i];
+                if (pass >= pl.colours || !colourHasCells(pl.sp.g, pl.order[pass])) continue;
+                members.emplace_back(sweep_batch_key(pl.sp, pl.waves, pl.g32), i);
+            }
+            std::stable_sort(members.begin(), members.end(),
+                             [](const std::pair<int, size_t> &a, const std::pair<int, size_t> &b) {
+                                 return a.first < b.first;
+                             });
+            for (size_t m = 0; m < members.size();) {
+                const int key = members[m].first;
+                size_t e = m;
+                while (e < members.size() && members[e].first == key) e++;
+                if (key < 0) {
+                    // one launch per packing, in pass order (launched below with the batched groups of this pass)
+                    for (size_t q = m; q < e; q++) {
+                        PassGroup pg{-1 - (int)members[q].second, (size_t)pass, 0, 0, 0, 0};
+                        groups.push_back(pg);
+                    }
+                } else {
+                    PassGroup pg{key, entries, e - m, firsts, 0, 0};
+                    mem.hFirst[firsts] = 0;
+                    for (size_t q = m; q < e; q++) {
+                        const size_t i = members[q].second;
+                        SweepParams sp = plans[i].sp;
+                        sp.colour = plans[i].order[pass];
+                        mem.hTable[entries++] = sp;
+                        pg.blocks += (unsigned)(sp.na[0] * sp.na[1] * sp.na[2]);
+                        mem.hFirst[++firsts] = (int)pg.blocks;
+                        pg.lds = std::max(pg.lds, sweep_lds_bytes(sp.k.kind, sp.cap, plans[i].waves, plans[i].g32));
+                    }
+                    firsts++;
+                    groups.push_back(pg);
+                }
+                m = e;
+            }
+            groups.push_back(PassGroup{INT32_MIN, (size_t)pass, 0, 0, 0, 0});  // end of pass
+        }
+        if (entries) {
+            RP_HIP(hipMemcpyAsync(mem.dTable, mem.hTable, sizeof(SweepParams) * entries, hipMemcpyHostToDevice, s));
+            RP_HIP(hipMemcpyAsync(mem.dFirst, mem.hFirst, sizeof(int) * firsts, hipMemcpyHostToDevice, s));
+        }
+        RP_HIP(hipEventRecord(mem.uploaded, s));
+        uploadedOnce = true;
+        int pass = 0;
+        for (const PassGroup &pg : groups) {
+            if (pg.key == INT32_MIN) {
+                pass++;
+                continue;
+            }
+            if (pg.key < 0) {
+                const size_t i = (size_t)(-1 - pg.key);
+                SweepParams sp = plans[i].sp;
+                sp.colour = plans[i].order[pass];
+                RP_HIP(launch_sweep(s, sp, plans[i].waves, Timed{}, plans[i].g32));
+                continue;
+            }
+            const int packings = (int)pg.entries;
+            RP_HIP(launch_sweep_batch(s, pg.key, mem.dTable + pg.entry0, mem.dFirst + pg.first0, packings, pg.blocks,
+                                      pg.lds));
+        }
+        bool pending = false;
+        for (size_t i = 0; i < n; i++) {
+            int r = finishSweep(ctxs[i], plans[i]);
+            if (r) return named(i, r);
+            bool a = false;
+            trials[i] = BoxTrial{};
+            if (ctxs[i]->boxKind != RP_BOX_NONE) {
+                r = boxMove(ctxs[i], a, &trials[i]);
+                if (r) return named(i, r);
+                pending = pending || trials[i].pending;
+            }
+            acc[i] = a;
+        }
+        if (pending) RP_HIP(hipStreamSynchronize(s));
+        for (size_t i = 0; i < n; i++) {
+            bool a = acc[i] != 0;
+            int r = boxMoveEnd(ctxs[i], trials[i], a);
+            if (r) return named(i, r);
+            endCycle(ctxs[i], a);
+            r = afterCycle(ctxs[i], adapt);
+            if (r) return named(i, r);
+        }
+    }
+    for (size_t i = 0; i < n; i++) {
+        int r = reportCounters(ctxs[i], out ? out + i : nullptr);
+        if (r) return named(i, r);
+    }
+    return RP_OK;
+}
 }  // namespace
 
 extern "C" {
@@ -1345,40 +1695,13 @@ int rp_get_move_samplers(rp_ctx *c, rp_move_desc *out, size_t n, size_t *nMoves)
 int rp_run_cycles(rp_ctx *c, size_t nCycles, int adapt, rp_counters *out) {
     if (!c) return fail(RP_ERR_PRECONDITION, "null context");
     if (!c->envSet) return fail(RP_ERR_PRECONDITION, "environment not set");
-    uint64_t overl = 0;
-    int r = RP_OK;
-    // a WCA interaction has no hard part, and counting mode runs on overlapping packings: nothing to validate
-    // (Simulation.cpp:127-132)
-    if ((!c->k.soft || c->k.hardCore) && !c->counting) r = countExact(c, 1, overl);
-    if (r == RP_ERR_UNSUPPORTED) {
-        // the reference's grid is off for this box; check on the checkerboard grid instead
-        double zero[3] = {0, 0, 0};
-        c->sortedForCycle = false;
-        Grid g{};
-        r = chooseSweepGrid(c, c->box, zero, g);
-        if (r) return r;
-        r = rebuild(c, g, 0, nullptr);
-        if (r) return r;
-        r = scanOverlaps(c, g, 1, overl);
-    }
+    int r = validateStart(c);
     if (r) return r;
-    if (overl)
-        return fail(RP_ERR_VALIDATION,
-                    "Overlaps are present at the start of integration. Perform overlap reduction beforehand.");
     for (size_t i = 0; i < nCycles; i++) {
         r = runCycle(c);
         if (r) return r;
-        // Simulation.cpp:144-145; in counting mode the reference's renormalisation keeps the orientations
-        // (Packing::renormalizeOrientations with allowOverlaps resets the packing to its unfixed copy,
-        // Packing.cpp:1179-1190), so none is run there
-        if (c->fixEvery && !c->counting && c->cycles % c->fixEvery == 0) {
-            r = rp_fix_orientations(c, nullptr);
-            if (r) return r;
-        }
-        if (adapt) {
-            r = adaptSteps(c);
-            if (r) return r;
-        }
+        r = afterCycle(c, adapt);
+        if (r) return r;
         if (c->timing && c->evUsed >= 8192) {  // bounded event pool; otherwise read once at the end
             r = timingHarvest(c);
             if (r) return r;
@@ -1386,29 +1709,7 @@ int rp_run_cycles(rp_ctx *c, size_t nCycles, int adapt, rp_counters *out) {
     }
     r = timingHarvest(c);
     if (r) return r;
-    r = readCounters(c);
-    if (r) return r;
-    if (out) {
-        out->cycles = c->cycles;
-        out->moves_attempted = c->hCounters[CNT_ATTEMPTED];
-        out->moves_accepted = c->hCounters[CNT_ACCEPTED];
-        out->moves_left_cell = c->hCounters[CNT_LEFT_CELL];
-        out->box_attempted = c->boxAttempted;
-        out->box_accepted = c->boxAccepted;
-        out->box_scans = c->boxScans;
-        out->box_too_small = c->boxTooSmall;
-        out->unstaged_blocks = c->hCounters[CNT_UNSTAGED_BLOCKS];
-        out->pair_tests = c->hCounters[CNT_TESTS];
-        // the first sampler that translates and the first that rotates
-        out->trans_step = out->rot_step = 0;
-        for (auto it = c->samplers.rbegin(); it != c->samplers.rend(); ++it) {
-            const double maxT = it->maxTransImplicit ? minHeight(c->box) / 2 : it->maxTransStep;
-            if (it->kind != RP_MOVE_ROTATION) out->trans_step = std::min(maxT, it->transStep);
-            if (it->kind != RP_MOVE_TRANSLATION) out->rot_step = it->rotStep;
-        }
-        out->box_step = c->boxStep;
-    }
-    return RP_OK;
+    return reportCounters(c, out);
 }
 
 int rp_run_replicas(rp_ctx *const *ctxs, size_t n, size_t nCycles, int adapt, rp_counters *out) {
@@ -1420,15 +1721,13 @@ int rp_run_replicas(rp_ctx *const *ctxs, size_t n, size_t nCycles, int adapt, rp
                 return fail(RP_ERR_PRECONDITION, "replicas " + std::to_string(j) + " and " + std::to_string(i) +
                                                      " are the same context");
     }
-    (void)nCycles;
-    (void)adapt;
-    (void)out;
+    for (size_t i = 0; i < n; i++) {
+        if (!ctxs[i]->envSet) return fail(RP_ERR_PRECONDITION, "replica " + std::to_string(i) + ": environment not set");
+        if (ctxs[i]->device != ctxs[0]->device)
+            return fail(RP_ERR_PRECONDITION, "replica " + std::to_string(i) + " lives on another device");
+    }
     if (n == 0) return RP_OK;
-    // Running the contexts on concurrent host threads faulted the GPU on a 32-replica scan (N = 1000..20000,
-    // p* = 0.5..4) with the cause not found; until it is, the entry point refuses to run.  Callers run the replicas
-    // one after another with rp_run_cycles.
-    return fail(RP_ERR_UNSUPPORTED, "rp_run_replicas is switched off: concurrent replicas faulted the GPU (cause "
-                                    "not found); run the replicas one by one with rp_run_cycles");
+    return runReplicas(ctxs, n, nCycles, adapt != 0, out);
 }
 
 // Sorted device records (storage order of the current cell list) into host arrays; null arrays are skipped.

@@ -196,5 +196,13 @@ int sweep_occupancy(int kind, int cap, int waves);
 // grouped launch and a hard sphere or spherocylinder.
 hipError_t launch_sweep(hipStream_t s, const SweepParams &sp, int waves, Timed t = {}, int g32 = 0);
 size_t sweep_lds_bytes(int kind, int cap, int waves, int g32 = 0);
+// The kernel launch_sweep(sp, waves, g32) would run, as a key shared by the packings one batched launch may hold, or
+// -1 for the sweeps that run one packing per launch (soft, overlap counting, orientation fix, polyspheres, traced).
+int sweep_batch_key(const SweepParams &sp, int waves, int g32);
+// One colour pass of `packings` packings of one key in one launch: table (device) holds each packing's parameters,
+// first (device, packings + 1 entries) the first workgroup of each packing; blocks = first[packings]; lds the largest
+// sweep_lds_bytes of the packings.
+hipError_t launch_sweep_batch(hipStream_t s, int key, const SweepParams *table, const int *first, int packings,
+                              unsigned blocks, size_t lds);
 
 }  // namespace rp

@@ -1408,6 +1408,9 @@ __device__ __forceinline__ void sweep_cell_pipelined(const SweepParams &P, const
             if (planner) {
                 for (int c = 0; c < 9; c++) oo[c * OWN_CAP + kIdx] = plan[13 + c];
             }
+            // every lane of the planner has read this plan buffer before its lane 0 rewrites it with the plan of
+            // move mi + 2 (a scheduling fence; no instruction)
+            __builtin_amdgcn_wave_barrier();
             if (mi + 1 < cn && pk[mi + 1] == kIdx) {
                 // the next plan used the pose this move just replaced
                 __syncthreads();
@@ -2310,12 +2313,12 @@ __device__ __forceinline__ void stage_stencil_batched(const SweepParams &P, cons
     }
 }
 
-// Two workgroups of eight wavefronts share a compute unit only within 128 registers per lane (4 wavefronts per SIMD)
-// LEAN != 0: a kernel that carries only the grouped sweep of staged cells (and the HBM path for cells that cannot be
+// Active cell b of colour P.colour (one workgroup).
+// LEAN != 0: the kernel carries only the grouped sweep of staged cells (and the HBM path for cells that cannot be
 // staged), with fewer live registers than the general kernel: 1 FP64 scan, 2 guarded FP32 scan, 3 guarded FP32 scan
 // with the FP64 verification of every pair.
-template <int KIND, int WAVES, bool SOFT, int LEAN = 0>
-__global__ __launch_bounds__(WAVE *WAVES, WAVES >= 8 ? 4 : 1) void k_sweep(SweepParams P) {
+template <int KIND, int WAVES, bool SOFT, int LEAN>
+__device__ __forceinline__ void sweep_block(const SweepParams &P, const int b) {
     constexpr int G32 = LEAN >= 2 ? LEAN - 1 : 0;
     constexpr int NT = WAVE * WAVES;
     extern __shared__ double smem[];
@@ -2326,7 +2329,6 @@ __global__ __launch_bounds__(WAVE *WAVES, WAVES >= 8 ? 4 : 1) void k_sweep(Sweep
 
     const Grid &g = P.g;
     const int tid = threadIdx.x;
-    const int b = blockIdx.x;
     const unsigned long long t0 = P.trace ? wall_clock64() : 0;
     const int bx = b % P.na[0], by = (b / P.na[0]) % P.na[1], bz = b / (P.na[0] * P.na[1]);
     const int per = g.period;
@@ -2462,6 +2464,27 @@ __global__ __launch_bounds__(WAVE *WAVES, WAVES >= 8 ? 4 : 1) void k_sweep(Sweep
         t[2] = (unsigned long long)nOwn | ((unsigned long long)total << 16) | ((unsigned long long)cell << 32);
         t[3] = (unsigned long long)__smid();
     }
+}
+
+// Two workgroups of eight wavefronts share a compute unit only within 128 registers per lane (4 wavefronts per SIMD)
+template <int KIND, int WAVES, bool SOFT, int LEAN = 0>
+__global__ __launch_bounds__(WAVE *WAVES, WAVES >= 8 ? 4 : 1) void k_sweep(SweepParams P) {
+    sweep_block<KIND, WAVES, SOFT, LEAN>(P, (int)blockIdx.x);
+}
+
+// One colour pass of several packings in one launch: packing r owns workgroups [first[r], first[r + 1]) and its
+// parameters (grid, particles, proposals, counters, colour) are table[r]; every workgroup runs exactly the sweep of
+// that packing's active cell b - first[r].
+template <int KIND, int WAVES, int LEAN>
+__global__ __launch_bounds__(WAVE *WAVES, WAVES >= 8 ? 4 : 1) void k_sweep_batch(
+    const SweepParams *__restrict__ table, const int *__restrict__ first, int packings) {
+    const int b = (int)blockIdx.x;
+    int lo = 0, hi = packings;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (first[mid] <= b) lo = mid; else hi = mid;
+    }
+    sweep_block<KIND, WAVES, false, LEAN>(table[lo], b - first[lo]);
 }
 
 // ---------------------------------------------------------------------------------------------------------------
@@ -2709,6 +2732,41 @@ hipError_t launch_sweep(hipStream_t s, const SweepParams &sp, int waves, Timed t
     }
 #undef RP_SWEEP_W
 #undef RP_SWEEP
+    return hipGetLastError();
+}
+
+int sweep_batch_key(const SweepParams &sp, int waves, int g32) {
+    if (sp.k.soft || sp.counting || sp.fixMode || sp.k.kind == 2 || sp.trace) return -1;
+    int w = waves >= 8 ? 8 : (waves >= 4 ? 4 : (waves == 2 ? 2 : 1));
+    const bool lean = w == 8 && sp.pipelined == 2 && sp.k.kind == 1;  // as launch_sweep chooses
+    if (g32 && !lean) return -1;
+    return sp.k.kind | (w << 4) | ((lean ? 1 + g32 : 0) << 8);
+}
+
+hipError_t launch_sweep_batch(hipStream_t s, int key, const SweepParams *table, const int *first, int packings,
+                              unsigned blocks, size_t lds) {
+    if (blocks == 0) return hipSuccess;
+    const int kind = key & 15, w = (key >> 4) & 15, lean = key >> 8;
+#define RP_BATCH(KIND, W, L) \
+    hipLaunchKernelGGL((k_sweep_batch<KIND, W, L>), dim3(blocks), dim3(WAVE * W), (uint32_t)lds, s, table, first, packings)
+    if (kind == 1 && lean) {
+        if (lean == 3) RP_BATCH(1, 8, 3);
+        else if (lean == 2) RP_BATCH(1, 8, 2);
+        else RP_BATCH(1, 8, 1);
+    } else if (kind == 0) {
+        if (w == 8) RP_BATCH(0, 8, 0);
+        else if (w == 4) RP_BATCH(0, 4, 0);
+        else if (w == 2) RP_BATCH(0, 2, 0);
+        else RP_BATCH(0, 1, 0);
+    } else if (kind == 1) {
+        if (w == 8) RP_BATCH(1, 8, 0);
+        else if (w == 4) RP_BATCH(1, 4, 0);
+        else if (w == 2) RP_BATCH(1, 2, 0);
+        else RP_BATCH(1, 1, 0);
+    } else {
+        return hipErrorInvalidValue;
+    }
+#undef RP_BATCH
     return hipGetLastError();
 }
 

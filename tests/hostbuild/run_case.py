@@ -1,0 +1,69 @@
+"""One case on the CPU build of the library (run by tests/test_hostbuild.py in a process of its own, with the
+AddressSanitizer runtime preloaded; any access outside an allocation aborts the process with a report).
+
+    python tests/hostbuild/run_case.py LIB replay WAVES PIPE     # decision records of one launch shape vs the oracle
+    python tests/hostbuild/run_case.py LIB replica I CYCLES      # replica I of the 32-replica scan, overlap checks
+"""
+import os
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+sys.path.insert(0, os.path.dirname(HERE))
+
+import numpy as np  # noqa: E402
+
+import rampack_amd as ra  # noqa: E402
+
+ra.load_library(sys.argv[1])
+
+import oracle_lib  # noqa: E402
+from rampack_amd import lattice  # noqa: E402
+
+SC = ra.Shape("spherocylinder", radius=0.5, length=5.0)
+
+
+def replay(waves: str, pipe: str) -> None:
+    from test_gpu_replay import check_record, rod_start
+    os.environ["RAMPACK_SWEEP_WAVES"] = waves
+    os.environ["RAMPACK_SWEEP_PIPELINE"] = pipe
+    box, pos, rot = rod_start()
+    pk = ra.GpuPacking(SC, box, pos, rot, seed=31)
+    pk.set_environment(1.0, 2.0, 0.2, None, box_move=("delta_triclinic", 0.01))
+    pk.run_cycles(5, adapt_steps=True)
+    for _ in range(2):
+        check_record(SC, pk.record_sweep(), flags=not (waves == "8" and pipe != "2"))
+    pk.close()
+
+
+def replica(i: int, cycles: int) -> None:
+    """The scan's replica i: N drawn uniformly from 1000..20000 (seed 7), p* = linspace(0.5, 4, 32)[i], parallel rods
+    on a partly filled sc lattice at packing fraction 0.2; the state is checked for overlaps by the oracle every ten
+    cycles, and the sweep's decisions are replayed through the oracle at the start and at the end."""
+    from test_gpu_replay import check_record
+    n = int(np.random.default_rng(7).integers(1000, 20001, 32)[i])
+    p = float(np.linspace(0.5, 4.0, 32)[i])
+    box, pos, rot = lattice.rod_replica_start(n, 0.2)
+    pk = ra.GpuPacking(SC, box, pos, rot, seed=100 + i)
+    pk.set_environment(1.0, p, 0.5, 0.2, box_move=("log", 0.002))
+    check_record(SC, pk.record_sweep())
+    done = 0
+    while done < cycles:
+        c = pk.run_cycles(min(10, cycles - done), adapt_steps=True)
+        done += min(10, cycles - done)
+        b, q, r = pk.download()
+        overl = oracle_lib.oracle().snapshot_stats(1, 0.5, 5.0, b, q, r)[0]
+        print(f"replica {i} N={n} p*={p:.3f} cycle {c['cycles']}: moves {c['moves_accepted']}/{c['moves_attempted']}, "
+              f"box {c['box_accepted']}/{c['box_attempted']}, unstaged blocks {c['unstaged_blocks']}, "
+              f"phi {pk.packing_fraction():.4f}, overlaps {overl}", flush=True)
+        assert overl == 0
+    check_record(SC, pk.record_sweep())
+    pk.close()
+
+
+if __name__ == "__main__":
+    if sys.argv[2] == "replay":
+        replay(sys.argv[3], sys.argv[4])
+    else:
+        replica(int(sys.argv[3]), int(sys.argv[4]))
+    print("case passed", flush=True)

@@ -69,13 +69,14 @@ def test_structure_layouts_match_header(tmp_path):
     assert seen == sum(len(py._fields_) + 1 for py in mirrors.values())
 
 
-def test_run_replicas_is_switched_off():
-    """The argument checks compare pointers only (nothing is dereferenced); past them every non-empty call is
+def test_run_replicas_argument_checks():
+    """The list checks compare pointers before any context is read: a context listed twice or a null entry is
     refused, and an empty list is a no-op."""
     lib = ra.load_library()
     fake = (ctypes.c_void_p * 3)(0x1000, 0x2000, 0x1000)
     assert lib.rp_run_replicas(fake, 0, 1, 0, None) == ra.RP_OK
     assert lib.rp_run_replicas(fake, 3, 1, 0, None) == ra.RP_ERR_PRECONDITION
-    assert b"same context" in lib.rp_last_error()
-    assert lib.rp_run_replicas(fake, 2, 1, 0, None) == ra.RP_ERR_UNSUPPORTED
-    assert b"switched off" in lib.rp_last_error()
+    assert b"replicas 0 and 2 are the same context" in lib.rp_last_error()
+    nulls = (ctypes.c_void_p * 2)(0x1000, None)
+    assert lib.rp_run_replicas(nulls, 2, 1, 0, None) == ra.RP_ERR_PRECONDITION
+    assert b"replica 1: null context" in lib.rp_last_error()
