@@ -364,6 +364,18 @@ typedef struct rp_guard_stats {
 int rp_pair_decisions_guarded(const rp_shape_desc *shape, size_t n, const double *records, uint8_t *decisions,
                               uint8_t *codes, rp_guard_stats *stats);
 
+/* rp_pair_decisions_guarded on device arrays, allocating nothing: no counterpart in the reference; for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * d_records (27 n doubles), d_decisions (n bytes) and d_codes (n bytes, or NULL) are device pointers on the current
+ * device; the same decisions, codes and statistics as rp_pair_decisions_guarded.  The statistics' per-pair errors
+ * (and the codes when d_codes is NULL) live in a fixed scratch area of the library on each device, which holds
+ * RP_GUARD_NOALLOC_MAX_PAIRS pairs: a larger n is refused (RP_ERR_PRECONDITION).  The area is shared by every call
+ * on that device, so calls that overlap on one device race.  The call is synchronous (the null stream is
+ * synchronised before it returns). */
+#define RP_GUARD_NOALLOC_MAX_PAIRS ((size_t)1 << 20)
+int rp_pair_decisions_guarded_noalloc(const rp_shape_desc *shape, size_t n, const double *d_records,
+                                      uint8_t *d_decisions, uint8_t *d_codes, rp_guard_stats *stats);
+
 /* Precision of a context's sweeps (RP_PREC_*, as rp_create's argument), and the guarded sweep's statistics since the
  * context was created: pairs classified in FP32, fp64_fallback = pairs inside a guard band, and in verify mode
  * mismatches (0) and max_err_centre (largest |FP32 - FP64| squared centre distance); the segment fields stay 0 (the

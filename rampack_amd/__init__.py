@@ -57,6 +57,7 @@ EXPORTED_SYMBOLS = (
     "rp_ramtrj_recorder_open", "rp_ramtrj_record", "rp_ramtrj_record_packing", "rp_ramtrj_recorder_last_cycle",
     "rp_ramtrj_recorder_close", "rp_ramtrj_player_open", "rp_ramtrj_player_header", "rp_ramtrj_next_snapshot",
     "rp_ramtrj_jump_to_snapshot", "rp_ramtrj_player_close", "rp_pair_decisions_guarded",
+    "rp_pair_decisions_guarded_noalloc",
     "rp_try_scaling", "rp_revert_scaling", "rp_set_move_samplers", "rp_get_move_samplers", "rp_run_replicas",
     "rp_record_sweep", "rp_box_info", "rp_apply_moves", "rp_set_precision", "rp_get_guard_stats",
     "rp_mlp_gradient", "rp_mlp_infer", "rp_mlp_train", "rp_mlp_order_stats", "rp_mlp_get_weights",
@@ -231,6 +232,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.rp_pair_decisions_guarded.argtypes = [ctypes.POINTER(ShapeDesc), ctypes.c_size_t, dp,
                                               ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(ctypes.c_uint8),
                                               ctypes.POINTER(GuardStats)]
+    lib.rp_pair_decisions_guarded_noalloc.argtypes = [ctypes.POINTER(ShapeDesc), ctypes.c_size_t, vp, vp, vp,
+                                                      ctypes.POINTER(GuardStats)]
     ip = ctypes.POINTER(ctypes.c_int)
     lib.rp_record_sweep.argtypes = [vp, ctypes.c_size_t, ctypes.POINTER(SweepRecord), dp, dp, ip, ip, ip, dp, dp,
                                     ctypes.POINTER(ctypes.c_uint8), dp, dp]
@@ -348,6 +351,22 @@ def pair_decisions(shape: Shape, records: np.ndarray) -> np.ndarray:
     _check(lib.rp_pair_decisions(ctypes.byref(d), len(rec), _dptr(rec),
                                  out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))))
     return out
+
+
+GUARD_NOALLOC_MAX_PAIRS = 1 << 20
+
+
+def pair_decisions_guarded_noalloc(shape: Shape, n: int, d_records: int, d_decisions: int, d_codes: int = 0,
+                                   stats: bool = True):
+    """rp_pair_decisions_guarded_noalloc on device arrays given by address (records 27 n doubles, decisions and codes
+    n bytes; d_codes 0 for none): allocates nothing; returns the statistics dict (or None with stats=False)."""
+    lib = load_library()
+    d = shape.desc()
+    st = GuardStats()
+    _check(lib.rp_pair_decisions_guarded_noalloc(ctypes.byref(d), int(n), ctypes.c_void_p(d_records),
+                                                 ctypes.c_void_p(d_decisions), ctypes.c_void_p(d_codes or None),
+                                                 ctypes.byref(st) if stats else None))
+    return st.as_dict() if stats else None
 
 
 def pair_decisions_guarded(shape: Shape, records: np.ndarray, verify: bool = True):

@@ -2727,6 +2727,47 @@ int rp_pair_decisions_guarded(const rp_shape_desc *shape, size_t n, const double
     return RP_OK;
 }
 
+int rp_pair_decisions_guarded_noalloc(const rp_shape_desc *shape, size_t n, const double *d_records,
+                                      uint8_t *d_decisions, uint8_t *d_codes, rp_guard_stats *stats) {
+    if (!shape || (n && (!d_records || !d_decisions))) return fail(RP_ERR_PRECONDITION, "null argument");
+    if (shape->kind != RP_SHAPE_SPHERE && shape->kind != RP_SHAPE_SPHEROCYLINDER)
+        return fail(RP_ERR_UNSUPPORTED, "unsupported shape kind");
+    if (!hasHardPart(*shape)) return fail(RP_ERR_UNSUPPORTED, "a soft interaction has no hard part");
+    if (n > RP_GUARD_NOALLOC_MAX_PAIRS)
+        return fail(RP_ERR_PRECONDITION, std::to_string(n) + " pairs exceed the fixed scratch area (" +
+                                             std::to_string((size_t)RP_GUARD_NOALLOC_MAX_PAIRS) + " pairs)");
+    if (stats) *stats = rp_guard_stats{};
+    if (n == 0) return RP_OK;
+    GuardScratch sc{};
+    RP_HIP(guard_scratch(sc));
+    const PairConsts k = makeConsts(*shape);
+    const bool verify = d_codes || stats;
+    uint8_t *codes = d_codes ? d_codes : sc.code;
+    RP_HIP(launch_pair_decisions_guarded(nullptr, k, n, d_records, d_decisions, verify ? codes : nullptr, sc.errC,
+                                         sc.errS));
+    if (stats) {
+        // the area keeps the last call's contents: the counters are cleared (a fill kernel) before every reduction
+        RP_HIP(launch_fill_u64(nullptr, sc.counters, 8, 0));
+        RP_HIP(launch_guard_stats(nullptr, n, codes, sc.errC, sc.errS, sc.counters));
+        unsigned long long h[8] = {};
+        RP_HIP(hipMemcpy(h, sc.counters, sizeof h, hipMemcpyDeviceToHost));
+        float ec = 0, es = 0;
+        const uint32_t bc = (uint32_t)h[4], bs = (uint32_t)h[5];
+        std::memcpy(&ec, &bc, sizeof ec);
+        std::memcpy(&es, &bs, sizeof es);
+        stats->pairs = n;
+        stats->fp64_fallback = h[0];
+        stats->segment_stage = h[1];
+        stats->segment_fallback = h[2];
+        stats->mismatches = h[3];
+        stats->max_err_centre = ec;
+        stats->max_err_segment = es;
+    } else {
+        RP_HIP(hipStreamSynchronize(nullptr));
+    }
+    return RP_OK;
+}
+
 }  // extern "C"
 
 // Error reporting for the library's other host sources (C++ linkage).

@@ -132,6 +132,20 @@ enum Counter : int {
 hipError_t launch_pair_decisions(hipStream_t s, PairConsts k, size_t n, const double *rec, uint8_t *out);
 hipError_t launch_pair_decisions_guarded(hipStream_t s, PairConsts k, size_t n, const double *rec, uint8_t *out,
                                          uint8_t *code, float *errC, float *errS);
+// Guard statistics of n decisions of launch_pair_decisions_guarded (with codes) into counters[0..5]: fallbacks,
+// segment stages, segment fallbacks, mismatches, largest centre and segment errors (float bits); counters start at 0.
+hipError_t launch_guard_stats(hipStream_t s, size_t n, const uint8_t *code, const float *errC, const float *errS,
+                              unsigned long long *counters);
+// The current device's fixed scratch area of the allocation-free guarded decisions (RP_GUARD_NOALLOC_MAX_PAIRS
+// pairs): counters (8), centre errors, segment errors and codes.  The device address is looked up once per device.
+constexpr size_t GUARD_SCRATCH_PAIRS = RP_GUARD_NOALLOC_MAX_PAIRS;
+constexpr size_t GUARD_SCRATCH_BYTES = 256 + GUARD_SCRATCH_PAIRS * (2 * sizeof(float) + 1);
+struct GuardScratch {
+    unsigned long long *counters;
+    float *errC, *errS;
+    uint8_t *code;
+};
+hipError_t guard_scratch(GuardScratch &out);
 // Start / stop events carried by a launch's own dispatch (hipExtLaunchKernel): the kernel's begin and end times
 // are recorded with no marker packets between kernels.  Null events: an untimed launch.
 struct Timed {

@@ -17,7 +17,9 @@
 
 #include <hip/hip_ext.h>
 
+#include <algorithm>
 #include <cfloat>
+#include <mutex>
 
 namespace rp {
 
@@ -235,6 +237,80 @@ hipError_t launch_pair_decisions_guarded(hipStream_t s, PairConsts k, size_t n, 
         hipLaunchKernelGGL(k_pair_decisions_guarded<false>, dim3(blocks), dim3(256), 0, s, k, n, rec, out, code,
                            errC, errS);
     return hipGetLastError();
+}
+
+// Statistics of n guarded decisions (codes and errors of k_pair_decisions_guarded<true>) into counters: [0] guard-band
+// fallbacks, [1] segment stages, [2] segment-stage fallbacks, [3] mismatches, [4] largest centre error and [5] largest
+// segment error as float bits (non-negative floats order as their bits).  The counters start at zero.
+constexpr int GUARD_STATS_THREADS = 256;
+
+__global__ __launch_bounds__(GUARD_STATS_THREADS) void k_guard_stats(size_t n, const uint8_t *__restrict__ code,
+                                                                     const float *__restrict__ errC,
+                                                                     const float *__restrict__ errS,
+                                                                     unsigned long long *counters) {
+    unsigned long long c[4] = {0, 0, 0, 0};
+    float mc = 0.f, ms = 0.f;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const uint8_t v = code[i];
+        c[0] += (v & 3) == GUARD_UNDECIDED ? 1 : 0;
+        c[1] += (v & 4) ? 1 : 0;
+        c[2] += (v & 4) && (v & 3) == GUARD_UNDECIDED ? 1 : 0;
+        c[3] += (v & 8) ? 1 : 0;
+        mc = fmaxf(mc, errC[i]);
+        if (errS[i] >= 0.f) ms = fmaxf(ms, errS[i]);
+    }
+    for (int off = WAVE / 2; off > 0; off >>= 1) {
+        for (int q = 0; q < 4; q++) c[q] += __shfl_xor(c[q], off);
+        mc = fmaxf(mc, __shfl_xor(mc, off));
+        ms = fmaxf(ms, __shfl_xor(ms, off));
+    }
+    if ((threadIdx.x & (WAVE - 1)) == 0) {
+        for (int q = 0; q < 4; q++)
+            if (c[q]) atomicAdd(&counters[q], c[q]);
+        atomicMax(&counters[4], (unsigned long long)__float_as_uint(mc));
+        atomicMax(&counters[5], (unsigned long long)__float_as_uint(ms));
+    }
+}
+
+hipError_t launch_guard_stats(hipStream_t s, size_t n, const uint8_t *code, const float *errC, const float *errS,
+                              unsigned long long *counters) {
+    if (n == 0) return hipSuccess;
+    const unsigned blocks = (unsigned)std::min<size_t>((n + GUARD_STATS_THREADS - 1) / GUARD_STATS_THREADS, 1024);
+    hipLaunchKernelGGL(k_guard_stats, dim3(blocks), dim3(GUARD_STATS_THREADS), 0, s, n, code, errC, errS, counters);
+    return hipGetLastError();
+}
+
+// Fixed device memory of the allocation-free guarded pair decisions: 256 bytes of counters, then the centre and
+// segment errors (GUARD_SCRATCH_PAIRS floats each) and the codes (GUARD_SCRATCH_PAIRS bytes).  Every GPU the library
+// runs on holds its own copy for the life of the process; its device address comes from hipGetSymbolAddress, once
+// per device (the array's name in host code is not a device address).
+static __device__ __align__(256) unsigned char scratch[GUARD_SCRATCH_BYTES];
+
+namespace {
+constexpr int SCRATCH_DEVICES = 64;
+std::mutex scratchMutex;
+void *scratchPtr[SCRATCH_DEVICES] = {};
+}  // namespace
+
+hipError_t guard_scratch(GuardScratch &out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= SCRATCH_DEVICES) return hipErrorInvalidValue;
+    std::lock_guard<std::mutex> lock(scratchMutex);
+    if (!scratchPtr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidValue;
+        scratchPtr[dev] = p;
+    }
+    unsigned char *base = static_cast<unsigned char *>(scratchPtr[dev]);
+    out.counters = reinterpret_cast<unsigned long long *>(base);
+    out.errC = reinterpret_cast<float *>(base + 256);
+    out.errS = out.errC + GUARD_SCRATCH_PAIRS;
+    out.code = reinterpret_cast<uint8_t *>(out.errS + GUARD_SCRATCH_PAIRS);
+    return hipSuccess;
 }
 
 __global__ __launch_bounds__(256) void k_pair_energies(PairConsts k, size_t n, const double *__restrict__ rec,
