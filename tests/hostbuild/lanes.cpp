@@ -154,51 +154,49 @@ void resolve_wave_group(Lane *const *g, int n) {
     }
 }
 
-// Resolves what can be: returns true if some lane became runnable or the block finished.
-bool resolve(bool &finished) {
-    const std::size_t n = nLanes;
-    bool any = false, anyLive = false, allAtBlock = true;
-    for (std::size_t w = 0; w < n; w += 64) {
-        const std::size_t e = std::min(n, w + 64);
-        // group the waiting lanes of this wavefront by (kind, line)
-        Lane *waiting[64];
-        int nw = 0;
-        for (std::size_t i = w; i < e; i++)
-            if (lanes[i].state == AT_WAVE) waiting[nw++] = &lanes[i];
-        bool used[64] = {};
-        for (int a = 0; a < nw; a++) {
-            if (used[a]) continue;
-            Lane *grp[64];
-            int ng = 0;
-            for (int b = a; b < nw; b++)
-                if (!used[b] && waiting[b]->kind == waiting[a]->kind && waiting[b]->line == waiting[a]->line) {
-                    grp[ng++] = waiting[b];
-                    used[b] = true;
-                }
-            resolve_wave_group(grp, ng);
-            any = true;
-        }
+// Resolves the wavefront operations of wave w (lanes [w, w + 64)) once none of its lanes can run: its waiting lanes
+// grouped by (operation, source line).  Returns true if some lane became runnable.
+bool resolve_wave(std::size_t w) {
+    const std::size_t e = std::min(nLanes, w + 64);
+    Lane *waiting[64];
+    int nw = 0;
+    for (std::size_t i = w; i < e; i++) {
+        if (lanes[i].state == RUNNABLE) return false;
+        if (lanes[i].state == AT_WAVE) waiting[nw++] = &lanes[i];
     }
-    if (any) return true;
-    int isOr = -1, line = -1, orAcc = 0;
+    bool used[64] = {};
+    for (int a = 0; a < nw; a++) {
+        if (used[a]) continue;
+        Lane *grp[64];
+        int ng = 0;
+        for (int b = a; b < nw; b++)
+            if (!used[b] && waiting[b]->kind == waiting[a]->kind && waiting[b]->line == waiting[a]->line) {
+                grp[ng++] = waiting[b];
+                used[b] = true;
+            }
+        resolve_wave_group(grp, ng);
+    }
+    return nw > 0;
+}
+
+// Releases a workgroup barrier once every live lane waits at one; finished once no lane is live.
+bool resolve_block(bool &finished) {
+    const std::size_t n = nLanes;
+    bool anyLive = false;
+    int isOr = -1, orAcc = 0;
     for (std::size_t i = 0; i < n; i++) {
         const Lane &l = lanes[i];
         if (l.state == DONE) continue;
         anyLive = true;
-        if (l.state != AT_BLOCK) {
-            allAtBlock = false;
-            continue;
-        }
+        if (l.state != AT_BLOCK) return false;
         if (isOr < 0) isOr = l.isOr;
         if (l.isOr != isOr) deadlock("lanes wait at different kinds of workgroup barrier");
         orAcc |= l.orValue;
-        (void)line;
     }
     if (!anyLive) {
         finished = true;
         return true;
     }
-    if (!allAtBlock) return false;
     for (std::size_t i = 0; i < n; i++) {
         Lane &l = lanes[i];
         if (l.state != AT_BLOCK) continue;
@@ -206,6 +204,64 @@ bool resolve(bool &finished) {
         l.state = RUNNABLE;
     }
     return true;
+}
+
+// Order in which wavefronts advance: all runnable lanes in lane order, then every wavefront's operations (the
+// default); or, with EMU_SCHED_SEED set, one wavefront at a time drawn at random among those that can advance, so
+// that wavefronts run ahead of each other between workgroup barriers by random amounts.
+uint64_t schedState = 0;
+bool schedRandom = false;
+
+uint64_t next_random() {
+    schedState += 0x9E3779B97F4A7C15ull;
+    uint64_t z = schedState;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+void run_block() {
+    const std::size_t nt = nLanes;
+    bool finished = false;
+    while (!finished) {
+        if (!schedRandom) {
+            bool ran = false;
+            for (std::size_t i = 0; i < nt; i++)
+                if (lanes[i].state == RUNNABLE) {
+                    resume(lanes[i]);
+                    ran = true;
+                }
+            if (ran) continue;
+            bool any = false;
+            for (std::size_t w = 0; w < nt; w += 64) any = resolve_wave(w) || any;
+            if (any) continue;
+            if (!resolve_block(finished)) deadlock("deadlock (no lane can proceed)");
+            continue;
+        }
+        std::size_t cand[16];
+        int nc = 0;
+        for (std::size_t w = 0; w < nt; w += 64) {
+            const std::size_t e = std::min(nt, w + 64);
+            bool runnable = false, waiting = false;
+            for (std::size_t i = w; i < e; i++) {
+                runnable = runnable || lanes[i].state == RUNNABLE;
+                waiting = waiting || lanes[i].state == AT_WAVE;
+            }
+            if (runnable || waiting) cand[nc++] = w;
+        }
+        if (nc == 0) {
+            if (!resolve_block(finished)) deadlock("deadlock (no lane can proceed)");
+            continue;
+        }
+        const std::size_t w = cand[next_random() % (uint64_t)nc];
+        bool ran = false;
+        for (std::size_t i = w; i < std::min(nt, w + 64); i++)
+            if (lanes[i].state == RUNNABLE) {
+                resume(lanes[i]);
+                ran = true;
+            }
+        if (!ran) resolve_wave(w);
+    }
 }
 
 }  // namespace
@@ -246,6 +302,14 @@ void run_grid(dim3 grid, dim3 block, std::size_t lds, const std::function<void()
     }
     if (lanes.size() < nt) lanes.resize(nt);
     nLanes = nt;
+    static bool seeded = false;
+    if (!seeded) {
+        seeded = true;
+        if (const char *sd = std::getenv("EMU_SCHED_SEED")) {
+            schedRandom = true;
+            schedState = std::strtoull(sd, nullptr, 10);
+        }
+    }
     body = &b;
     blockDimV = block;
     gridDimV = grid;
@@ -261,17 +325,7 @@ void run_grid(dim3 grid, dim3 block, std::size_t lds, const std::function<void()
                     lanes[i].tid = (unsigned)i;
                     start_lane(lanes[i]);
                 }
-                bool finished = false;
-                while (!finished) {
-                    bool ran = false;
-                    for (std::size_t i = 0; i < nt; i++)
-                        if (lanes[i].state == RUNNABLE) {
-                            resume(lanes[i]);
-                            ran = true;
-                        }
-                    if (ran) continue;
-                    if (!resolve(finished)) deadlock("deadlock (no lane can proceed)");
-                }
+                run_block();
             }
     std::free(dynLds);
     dynLds = nullptr;
