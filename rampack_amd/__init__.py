@@ -895,9 +895,15 @@ class RamtrjPlayer:
             self._p = None
 
 
-def mean_and_error(samples) -> tuple:
-    """Quantity::calculateFromSamples (src/utils/Quantity.cpp:45-62): the mean and the naive standard error of the
-    mean, summed in sample order as the reference does."""
+def mean_and_error(samples, method: str = "naive") -> tuple:
+    """The mean and its standard error.  method "naive": Quantity::calculateFromSamples (src/utils/Quantity.cpp:45-62),
+    the naive standard error of the mean, summed in sample order as the reference does; "blocks": the block-averaged
+    standard error (block_error), which stays valid for correlated samples."""
+    if method == "blocks":
+        s = np.asarray(samples, dtype=np.float64)
+        return (float(s.mean()) if len(s) else 0.0), block_error(s)
+    if method != "naive":
+        raise ValueError(f"unknown error method {method}")
     s = [float(x) for x in samples]
     if not s:
         return 0.0, 0.0
@@ -912,6 +918,23 @@ def mean_and_error(samples) -> tuple:
         err += (value - x) ** 2
     err = err / len(s) / (len(s) - 1)
     return value, math.sqrt(err)
+
+
+def block_error(samples, min_blocks: int = 16) -> float:
+    """Standard error of the mean of a correlated series by block averaging (H. Flyvbjerg and H. G. Petersen,
+    J. Chem. Phys. 91, 461 (1989)).  Level 0 is the series itself; each further level averages neighbouring pairs
+    of the level before (an odd last value is dropped).  At each level with at least min_blocks blocks the naive
+    standard error of its block means is taken; the estimate is the largest of them: once blocks are longer than the
+    correlation time the estimates stop growing (the plateau), and shorter blocks underestimate the error.  Series
+    shorter than min_blocks get the naive error of the samples."""
+    x = np.asarray(samples, dtype=np.float64).ravel()
+    if len(x) < 2:
+        return 0.0
+    best = float(x.std(ddof=1) / np.sqrt(len(x)))
+    while len(x) >= 2 * min_blocks:
+        x = 0.5 * (x[0:len(x) // 2 * 2:2] + x[1:len(x) // 2 * 2:2])
+        best = max(best, float(x.std(ddof=1) / np.sqrt(len(x))))
+    return best
 
 
 def nematic_p2(rot: np.ndarray) -> float:

@@ -25,7 +25,8 @@ Supported subset (anything else stops with an error naming it):
                temperature, pressure, optionally wrapped in scoped(observable, snapshot, averaging, inline)
 
 Output files follow the reference's formats: averages rows "temperature pressure <name> d<name> ..." appended to
-`averages_out` (docs/output-formats.md, "Observable averages"), observable snapshots "cycle <values>" with a
+`averages_out` (docs/output-formats.md, "Observable averages"; the d<name> errors are block-averaged standard errors,
+not the reference's naive ones, which ignore the correlation between successive samples), observable snapshots "cycle <values>" with a
 "cycle <names>" header (ObservablesCollector.cpp:175-203), RAMSNAP and RAMTRJ files.
 
 Usage: python -m rampack_amd.casino -i input.pyon [--verbose]
@@ -436,7 +437,7 @@ class Casino:
                             f.write("temperature pressure " + "".join(f"{k} d{k} " for _, k in avg_names) + "\n")
                         row = [repr(temperature), repr(pressure)]
                         for _, k in avg_names:
-                            m, e = mean_and_error(samples[k])
+                            m, e = mean_and_error(samples[k], method="blocks")
                             row += [repr(m), repr(e)]
                         f.write(" ".join(row) + " \n")
             if obs_file:

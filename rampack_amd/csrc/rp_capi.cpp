@@ -153,6 +153,10 @@ struct rp_ctx {
     int sweepWaves = 0;                  // wavefronts per active cell; 0 = chosen from the stencil size
     int colourPeriod = 0;                // checkerboard colour period (2 or 3); 0 = by shape
     int pipelined = -1;                  // staged sweep: 0 plain, 1 planner/tester split, 2 grouped moves, -1 auto
+    // sweep workgroups launched, and their number when the counters were last read: the stencil cap that lets two
+    // eight-wavefront workgroups share a compute unit is dropped once more than 1 % of them overflow it
+    uint64_t blocksLaunched = 0, blocksAtRead = 0;
+    bool pairCapOff = false;
 
     // learned candidate ordering: the network's parameters with Adam's moments and a gradient (4 x MLP_PARAMS), the
     // training pairs and the pairs of the last order statistics (x: 12 features, y: exact decision, meta, score)
@@ -517,6 +521,7 @@ int readCounters(rp_ctx *c) {
     RP_HIP(hipMemcpyAsync(c->hCounters, c->dCounters, sizeof(unsigned long long) * CNT_NUM, hipMemcpyDeviceToHost,
                           c->stream));
     RP_HIP(hipStreamSynchronize(c->stream));
+    c->blocksAtRead = c->blocksLaunched;
     return RP_OK;
 }
 
@@ -548,6 +553,7 @@ int scanLaunch(rp_ctx *c, const Grid &g, const Particles &items, size_t nItems, 
 
 // The count of a scanLaunch, once the stream has been synchronised.
 int scanResult(rp_ctx *c, uint64_t &count) {
+    c->blocksAtRead = c->blocksLaunched;
     if (c->hCounters[CNT_STAGE_OVERFLOW])
         return fail(RP_ERR_UNSUPPORTED, "a grid cell holds more particles than the overlap count can stage");
     count = c->hCounters[CNT_OVERLAPS];
@@ -1004,11 +1010,15 @@ int planSweep(rp_ctx *c, bool logDecisions, SweepPlan &pl) {
     const size_t fixed = sweep_lds_bytes(c->k.kind, 0, waves, g32);
     int maxCap = (int)((double)(c->ldsLimit - fixed) / perEntry) / 64 * 64;
     sp.cap = std::max(64, std::min(cap, maxCap));
-    if (waves == 8) {
+    // workgroups that read their stencil from HBM so far (hCounters as of the last read)
+    if (!c->pairCapOff && c->blocksAtRead >= 10000 && c->hCounters[CNT_UNSTAGED_BLOCKS] * 100 > c->blocksAtRead)
+        c->pairCapOff = true;
+    if (waves == 8 && !c->pairCapOff) {
         // two eight-wavefront workgroups share a compute unit (128 registers each) only if each takes at most half the
         // unit's LDS (ldsLimit is the unit's LDS less 4 KB; the kernel's static arrays take under 2 KB): a smaller
         // stencil area is worth it while it still holds the expected stencil with a margin of 3 standard deviations
-        // (larger cells read HBM)
+        // (larger cells read HBM); states whose stencils vary more than that (layered smectics, whose cells cut
+        // through one or two layers) overflow it, and once more than 1 % of the workgroups have, the full area is used
         const double half = (double)(c->ldsLimit + 4096) / 2 - 2048;
         const int pairCap = (int)((half - (double)fixed) / perEntry) / 64 * 64;
         if (pairCap < sp.cap && pairCap >= expect + 3 * std::sqrt(expect)) sp.cap = pairCap;
@@ -1076,6 +1086,7 @@ int sweepCycle(rp_ctx *c, bool logDecisions, SweepInfo *info) {
             const bool timed = c->timing == 1 || (c->timing == RP_TIMING_SAMPLED && i == (int)(c->cycle % pl.colours));
             RP_HIP(launch_sweep(c->stream, sp, pl.waves, timed ? timingSlot(c, 0) : Timed{}, pl.g32));
         }
+        c->blocksLaunched += (uint64_t)sp.na[0] * sp.na[1] * sp.na[2];
     }
     return finishSweep(c, pl);
 }
@@ -1367,6 +1378,7 @@ int runReplicas(rp_ctx *const *ctxs, size_t n, size_t nCycles, bool adapt, rp_co
                 const SweepPlan &pl = plans[i];
                 if (pass >= pl.colours || !colourHasCells(pl.sp.g, pl.order[pass])) continue;
                 members.emplace_back(sweep_batch_key(pl.sp, pl.waves, pl.g32), i);
+                ctxs[i]->blocksLaunched += (uint64_t)pl.sp.na[0] * pl.sp.na[1] * pl.sp.na[2];
             }
             std::stable_sort(members.begin(), members.end(),
                              [](const std::pair<int, size_t> &a, const std::pair<int, size_t> &b) {

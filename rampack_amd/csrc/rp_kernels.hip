@@ -2560,7 +2560,11 @@ __global__ __launch_bounds__(WAVE *WAVES, WAVES >= 8 ? 4 : 1) void k_sweep_batch
         const int mid = (lo + hi) >> 1;
         if (first[mid] <= b) lo = mid; else hi = mid;
     }
-    sweep_block<KIND, WAVES, false, LEAN>(table[lo], b - first[lo]);
+    // the table is read-only for the launch: read through the constant address space, its fields load as uniform
+    // scalars, as the single-packing kernel's argument does
+    typedef const __attribute__((address_space(4))) SweepParams ConstParams;
+    ConstParams *entry = (ConstParams *)table + lo;
+    sweep_block<KIND, WAVES, false, LEAN>(*(const SweepParams *)entry, b - first[lo]);
 }
 
 // ---------------------------------------------------------------------------------------------------------------

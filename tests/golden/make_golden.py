@@ -1,7 +1,8 @@
 """Writes tests/golden/golden.json from the reference's own hot path (oracle/_ref/libref.so, compiled from the
 reference sources by oracle/Makefile).  Run in the container that holds the reference tree:
 
-    make -C oracle ref oracle && python tests/golden/make_golden.py [--large-only] [--poly-only] [--sc-npt-only]
+    make -C oracle ref oracle && python tests/golden/make_golden.py [--large-only] [--poly-only] [--sc-npt-only] [--compound-only] [--c1-only]
+        [--wca-only] [--dimer-only]
 
 The fixture stores only data: generator parameters, SHA-256 of the generated inputs, the reference's decision
 bitmaps and the reference's overlap counts / energies for deterministic snapshots.
@@ -99,9 +100,13 @@ WCA_NPT = dict(name="wca_sphere_npt", n_side=8, phi0=0.30, temperature=1.0, pres
 # NpT hard dumbbells (two touching beads of diameter 1) from an aligned orthorhombic start, log isotropic box moves
 # (independent runs: samples within one run are correlated far beyond the sampling interval, so the fixture keeps the
 # mean over runs and the spread between them)
+# The packing fraction still climbs for tens of thousands of cycles after the start (reference means over 16 runs:
+# 0.275 after 4e3 thermalisation cycles, 0.288 after 3e4, 0.290 after 6e4, 0.289 after 1.2e5), so the runs thermalise
+# for 4e4 cycles.
 DIMER_NPT = dict(name="dimer_npt", beads="dimer", n_side=6, spacing=(1.6, 1.6, 2.05), temperature=1.0,
-                 pressure=1.5, trans_step=0.2, box_step=0.02, therm=4000, avg=12000, every=20,
-                 seeds=[2468, 11, 12, 13, 14, 15, 16, 17] + list(range(200, 224)))
+                 pressure=1.5, trans_step=0.2, box_step=0.02, therm=40000, avg=20000, every=20,
+                 seeds=[2468, 11, 12, 13, 14, 15, 16, 17] + list(range(200, 224)),
+                 device_seeds=[97531, 1, 2, 3, 4, 5, 6, 7])
 
 
 # NpT hard spherocylinders L/D = 5 at p* = 1 (an isotropic liquid near packing fraction 0.31) from an aligned
@@ -183,6 +188,37 @@ def reference_compound_statistics(ref):
     return st
 
 
+# The hard-sphere NpT state point of the plain method: N = 1000 spheres of diameter 1 at p* = 5, rototranslation step
+# 0.1 adapted during thermalisation, log (isotropic) box moves every cycle.  The equation of state needs an
+# equilibrated density: from the 10x10x10 sc lattice at packing fraction 0.30 the small box steps random-walk the
+# volume and the packing fraction still climbs after 4e4 cycles, so the runs start from the sc lattice at 0.39 (near
+# the Carnahan-Starling value 0.39325), with box step 0.001; 1e4 thermalisation + 3e4 averaging cycles, the packing
+# fraction sampled every 10 cycles.  Per independent run: the mean packing fraction and its block-averaged error.
+HS_C1 = dict(name="hs_npt_c1", n_side=10, phi0=0.39, temperature=1.0, pressure=5.0, trans_step=0.1, box_step=0.001,
+             therm=10000, avg=30000, every=10, seeds=list(range(500, 516)), device_seeds=list(range(600, 608)))
+
+
+def hs_c1_run(args):
+    st, seed = args
+    from rampack_amd import lattice
+    import rampack_amd as ra
+    ref = oracle_lib.ref()
+    box, pos, rot = lattice.hard_sphere_lattice(st["n_side"], st["phi0"])
+    out = oracle_lib.ref_mc_run(ref, 0, 0.5, 0.0, box, pos, rot, st["temperature"], st["pressure"], st["trans_step"],
+                                0.0, 2, st["box_step"], seed, therm=st["therm"], avg=st["avg"], avg_every=st["every"])
+    phi = out["samples"][:, 0]
+    return float(phi.mean()), ra.block_error(phi)
+
+
+def reference_hs_c1_statistics(ref):
+    st = dict(HS_C1)
+    from concurrent.futures import ProcessPoolExecutor
+    with ProcessPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
+        runs = list(ex.map(hs_c1_run, [(st, seed) for seed in st["seeds"]]))
+    st.update(phi_runs=[r[0] for r in runs], phi_block_err_runs=[r[1] for r in runs])
+    return st
+
+
 def reference_statistics(ref):
     from rampack_amd import lattice
     st = dict(WCA_NPT)
@@ -193,8 +229,9 @@ def reference_statistics(ref):
     s = r["samples"]
     rho = s[:, 0] / (np.pi / 6)   # packing fraction / v0 with the shape radius 0.5
     e = s[:, 2]
-    st.update(rho=float(rho.mean()), rho_err=float(rho.std(ddof=1) / np.sqrt(len(rho))),
-              energy=float(e.mean()), energy_err=float(e.std(ddof=1) / np.sqrt(len(e))), samples=int(len(s)))
+    import rampack_amd as ra
+    st.update(rho=float(rho.mean()), rho_err=ra.block_error(rho), energy=float(e.mean()), energy_err=ra.block_error(e),
+              samples=int(len(s)), errors="block-averaged")
     return st
 
 
@@ -228,6 +265,26 @@ def main():
             out = json.load(f)
         out["statistics"] = [s for s in out["statistics"] if s["name"] != SC_NPT["name"]]
         out["statistics"].append(reference_sc_statistics(ref))
+        print(out["statistics"][-1])
+        with open(path, "w") as f:
+            json.dump(out, f, indent=1)
+        return
+    if "--wca-only" in sys.argv or "--dimer-only" in sys.argv:
+        with open(path) as f:
+            out = json.load(f)
+        name = WCA_NPT["name"] if "--wca-only" in sys.argv else DIMER_NPT["name"]
+        out["statistics"] = [s for s in out["statistics"] if s["name"] != name]
+        out["statistics"].append(reference_statistics(ref) if "--wca-only" in sys.argv else
+                                 reference_dimer_statistics(ref))
+        print(out["statistics"][-1])
+        with open(path, "w") as f:
+            json.dump(out, f, indent=1)
+        return
+    if "--c1-only" in sys.argv:
+        with open(path) as f:
+            out = json.load(f)
+        out["statistics"] = [s for s in out["statistics"] if s["name"] != HS_C1["name"]]
+        out["statistics"].append(reference_hs_c1_statistics(ref))
         print(out["statistics"][-1])
         with open(path, "w") as f:
             json.dump(out, f, indent=1)

@@ -139,14 +139,16 @@ def test_polysphere_try_scaling():
 def test_dimer_npt_packing_fraction_matches_reference(golden):
     """NpT hard dumbbells against the reference's own runs of the same state point (fixture from the reference
     build's Packing with PolysphereTraits, samplers and log scaler, several seeds): the mean packing fraction over
-    independent device runs agrees with the reference's mean over its runs within 4 standard errors of the
-    difference, each side's error taken from the spread between its runs."""
+    independent device runs agrees with the reference's mean over its runs within 2 standard errors of the
+    difference, each side's error taken from the spread between its runs.  Both sides thermalise for 4e4 cycles: the
+    packing fraction climbs for tens of thousands of cycles after the start, and shorter runs compare two
+    unequilibrated states."""
     st = next((s for s in golden.get("statistics", []) if s["name"] == "dimer_npt"), None)
     if st is None or "phi_runs" not in st:
         pytest.skip("no reference statistics in the fixture")
     box, pos, rot = corpus.dimer_start(st)
     means = []
-    for seed in (97531, 1, 2, 3, 4, 5):
+    for seed in st.get("device_seeds", (97531, 1, 2, 3, 4, 5, 6, 7)):
         pk = ra.GpuPacking(poly_shape(st["beads"]), box, pos, rot, seed=seed)
         pk.set_environment(st["temperature"], st["pressure"], st["trans_step"], None,
                            box_move=("log", st["box_step"]))
@@ -156,6 +158,7 @@ def test_dimer_npt_packing_fraction_matches_reference(golden):
             pk.run_cycles(st["every"])
             phi.append(pk.packing_fraction())
         means.append(float(np.mean(phi)))
+        print(f"device run {len(means)} (seed {seed}): phi {means[-1]:.5f}", flush=True)
         b, p, r = pk.download()
         assert poly_overlaps(st["beads"], b, p, r) == 0
         pk.close()
@@ -163,7 +166,7 @@ def test_dimer_npt_packing_fraction_matches_reference(golden):
     err = np.hypot(np.std(means, ddof=1) / np.sqrt(len(means)), ref.std(ddof=1) / np.sqrt(len(ref)))
     print(f"gpu phi {np.mean(means)} (runs {np.round(means, 5)}), reference {ref.mean()} (runs {np.round(ref, 5)}), "
           f"error of the difference {err}", flush=True)
-    assert abs(np.mean(means) - ref.mean()) <= 4 * err
+    assert abs(np.mean(means) - ref.mean()) <= 2 * err
 
 
 def random_chains(n_side=(8, 8, 2), spacing=(3.0, 3.0, 11.5), seed=12):

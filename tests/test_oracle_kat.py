@@ -250,3 +250,44 @@ def test_fix_rotation_matches_reference_and_orthonormalises():
     before = perm.copy()
     oracle_lib.oracle().lib.orc_fix_rotation(1, oracle_lib._d(perm))
     assert np.array_equal(perm, before)
+
+
+def test_block_error_pairs_neighbours_and_takes_the_plateau():
+    """[0, 0, 1, 1] x 8: level 0 (32 values, half 0 half 1) has SE sqrt(8 / (32 * 31)) = sqrt(1 / 124); level 1
+    averages neighbours into 16 alternating 0 / 1 blocks with SE sqrt(4 / (16 * 15)) = sqrt(1 / 60); level 2 would
+    have 8 blocks (< 16).  The estimate is the larger, sqrt(1 / 60).  (Pairing each value with the one half a series
+    away instead would give 16 blocks of 0.5 and SE 0.)"""
+    import math
+    import rampack_amd as ra
+    x = [0.0, 0.0, 1.0, 1.0] * 8
+    assert abs(ra.block_error(x) - math.sqrt(1 / 60)) < 1e-15
+    m, e = ra.mean_and_error(x, method="blocks")
+    assert m == 0.5 and e == ra.block_error(x)
+    # alternating values: neighbours average to 0.5, so blocking adds nothing to the naive error
+    y = [0.0, 1.0] * 16
+    assert abs(ra.block_error(y) - math.sqrt(8 / (32 * 31))) < 1e-15
+    assert ra.mean_and_error(y)[1] == ra.mean_and_error(y, method="naive")[1]
+
+
+def test_block_error_on_correlated_series():
+    """AR(1) series x_t = rho x_{t-1} + e_t with rho = 0.9: the standard error of the mean of n samples is
+    sigma_x / sqrt(n) * sqrt((1 + rho) / (1 - rho)) for n >> correlation time, 4.36 times the naive value; the block
+    estimate lands within 20 % of it (its own scatter at 16+ blocks is about 18 % at the last level), and on
+    independent samples it stays within 20 % of the naive one."""
+    import numpy as np
+    import rampack_amd as ra
+    rng = np.random.Generator(np.random.PCG64(11))
+    rho, n = 0.9, 1 << 17
+    e = rng.normal(size=n)
+    x = np.empty(n)
+    x[0] = e[0] / np.sqrt(1 - rho * rho)
+    for t in range(1, n):
+        x[t] = rho * x[t - 1] + e[t]
+    sigma_x = 1 / np.sqrt(1 - rho * rho)
+    exact = sigma_x / np.sqrt(n) * np.sqrt((1 + rho) / (1 - rho))
+    naive = ra.mean_and_error(x)[1]
+    blocks = ra.block_error(x)
+    assert naive < 0.3 * exact
+    assert abs(blocks / exact - 1) < 0.2
+    iid = rng.normal(size=n)
+    assert abs(ra.block_error(iid) / ra.mean_and_error(iid)[1] - 1) < 0.2
