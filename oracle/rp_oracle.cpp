@@ -635,7 +635,38 @@ static int replaySweep(SweepReplay &R, const int *cellOf, int period, int nColou
                        const double *propT, const double *propR, std::uint8_t *decision) {
     const ShapeSpec &sp = R.spec;
     const double cellSize = ngCellSize(R.box, sp.range(), R.n);
-    if (cellSize == 0) return -2;  // the reference would test all pairs through the real PBC: not replayed here
+    if (cellSize == 0) {
+        // no neighbour grid (Packing.cpp:862-883): one domain holding every particle, the moves in order, each trial
+        // pose tested against every other particle through the nearest image (Packing.cpp:410-420, 582-617; the
+        // sphere test through PeriodicBoundaryConditions::getDistance2, SphereTraits.cpp:56-65)
+        if (R.cells[0] != 1 || R.cells[1] != 1 || R.cells[2] != 1 || sp.soft || !sp.centres.empty()) return -2;
+        const std::size_t moves = movesPerCell > 0 ? (std::size_t)movesPerCell : R.n * nSamplers;
+        if (moves > nProposals) return -4;
+        for (std::size_t i = 0; i < nProposals; i++) decision[i] = 3;
+        for (std::size_t q = 0; q < moves; q++) {
+            if (propK[q] < 0 || (std::size_t)propK[q] >= R.n) return -5;
+            const std::size_t gi = (std::size_t)propK[q];
+            Vec t{{propT[3 * q], propT[3 * q + 1], propT[3 * q + 2]}};
+            Mat Rm;
+            std::memcpy(Rm.m, propR + 9 * q, 72);
+            const Vec np = translateWrapped(R.box, R.pos[gi], t);
+            const Mat no = mulm(Rm, R.rot[gi]);
+            bool overlap = false;
+            for (std::size_t j = 0; j < R.n && !overlap; j++) {
+                if (j == gi) continue;
+                if (sp.kind == SPHERE)
+                    overlap = pbcDistance2(R.box, np, R.pos[j]) < std::pow(2 * sp.radius, 2);
+                else
+                    overlap = pairOverlap(sp, np, no, 0, R.pos[j], R.rot[j], 0, pbcTranslation(R.box, np, R.pos[j]));
+            }
+            decision[q] = overlap ? 0 : 1;
+            if (!overlap) {
+                R.pos[gi] = np;
+                R.rot[gi] = no;
+            }
+        }
+        return 0;
+    }
     Grid g(R.box, cellSize, R.n);
     std::vector<std::size_t> where(R.n);
     for (std::size_t i = 0; i < R.n; i++) {

@@ -432,6 +432,48 @@ int makeSweepGrid(rp_ctx *c, const HostBox &b, const double shiftUnit[3], Grid &
     return RP_OK;
 }
 
+// Whether the reference would run box b without its neighbour grid (Packing::rebuildNeighbourGrid,
+// Packing.cpp:862-883): cells of the interaction range (at least a fifth of the mean spacing) with fewer than four
+// across every direction, or wider than the box in one.  Such packings are run here through every pair and the
+// nearest image, as the reference then does; the all-pairs path covers hard spheres and spherocylinders.
+bool noGridBox(const rp_ctx *c, const HostBox &b) {
+    if (c->k.soft || c->counting || c->k.kind == RP_SHAPE_POLYSPHERE) return false;
+    double h[3];
+    b.heights(h);
+    double cell = c->range;
+    const double minCell = std::cbrt(b.volume() / (double)c->n) / 5;
+    if (c->range < minCell) cell = minCell;
+    const double hmax = std::max(h[0], std::max(h[1], h[2])), hmin = std::min(h[0], std::min(h[1], h[2]));
+    return cell * 4 > hmax || cell * (1 + 1e-12) > hmin;
+}
+
+// The grid of a packing without a neighbour grid: one cell that holds every particle, no origin shift.
+int makeAllPairsGrid(rp_ctx *c, const HostBox &b, Grid &g) {
+    std::memcpy(g.dims, b.dims, sizeof g.dims);
+    std::memcpy(g.inv, b.inv, sizeof g.inv);
+    for (int d = 0; d < 3; d++) {
+        g.n[d] = 1;
+        g.relCell[d] = 1;
+        g.shift[d] = 0;
+    }
+    g.ncell = 1;
+    g.exact = 1;
+    g.reach = 1;
+    g.period = 1;
+    g.allPairs = 1;
+    fillTranslations(g);
+    int r = ensureCells(c, 1);
+    if (r) return r;
+    g.cellStart = c->cellStart;
+    g.cellCount = c->cellCount;
+    return RP_OK;
+}
+
+// Particles the all-pairs sweep stages in LDS: all of them when they fit, else none (it reads them in place).
+int allPairsStaged(const rp_ctx *c) {
+    return allpairs_lds_bytes((int)c->n) <= c->ldsLimit ? (int)c->n : 0;
+}
+
 // Colour period of the checkerboard: RAMPACK_COLOURING (2 or 3) when set; otherwise 3 for polyspheres and 2 for
 // the other shapes.  Period 2 has 8 colours and cells >= range.  Period 3 has 27 colours and cells >= range / 2:
 // at the same density that is 27 launches per cycle of workgroups with a few particles each and 125-cell
@@ -439,6 +481,7 @@ int makeSweepGrid(rp_ctx *c, const HostBox &b, const double shiftUnit[3], Grid &
 // the whole molecule, so period 2 leaves few active cells per colour and stencils of thousands of beads; period 3
 // measured 1.9x faster on 50 653 kinked 11-bead chains.  Period 3 falls back to 2 when the box is too small for it.
 int chooseSweepGrid(rp_ctx *c, const HostBox &b, const double shiftUnit[3], Grid &g) {
+    if (noGridBox(c, b)) return makeAllPairsGrid(c, b, g);
     const int period = c->colourPeriod ? c->colourPeriod : (c->k.kind == RP_SHAPE_POLYSPHERE ? 3 : 2);
     if (period == 3 && makeSweepGrid(c, b, shiftUnit, g, 3) == RP_OK) return RP_OK;
     return makeSweepGrid(c, b, shiftUnit, g, 2);
@@ -544,8 +587,11 @@ int scanLaunch(rp_ctx *c, const Grid &g, const Particles &items, size_t nItems, 
     double expect = ((side * side * side - 1) / 2 + 1) * lambda;
     int cap = (int)std::ceil(expect + 8 * std::sqrt(expect) + 64);
     cap = std::min((cap + 63) / 64 * 64, countCapLimit(c));
-    RP_HIP(launch_count_overlaps(c->stream, g, (int)nItems, items, c->k, earlyExit, c->dCounters, cap,
-                                 timingSlot(c, 2)));
+    if (g.allPairs)
+        RP_HIP(launch_count_allpairs(c->stream, g, (int)nItems, items, c->k, earlyExit, c->dCounters));
+    else
+        RP_HIP(launch_count_overlaps(c->stream, g, (int)nItems, items, c->k, earlyExit, c->dCounters, cap,
+                                     timingSlot(c, 2)));
     RP_HIP(hipMemcpyAsync(c->hCounters, c->dCounters, sizeof(unsigned long long) * CNT_NUM, hipMemcpyDeviceToHost,
                           c->stream));
     return RP_OK;
@@ -976,6 +1022,8 @@ int planSweep(rp_ctx *c, bool logDecisions, SweepPlan &pl) {
     // particles, so a cycle is exactly N moves.
     // (with k samplers, k times as many: each requests N moves)
     sp.moves = lambda >= 3 ? (int)std::lround(lambda) * ms.n : 0;
+    // without a grid: the reference's serial cycle, N moves per sampler, each on any particle (Simulation.cpp:365-445)
+    if (g.allPairs) sp.moves = (int)c->n * ms.n;
     sp.mult = ms.n;
     pl.proposals = sp.moves > 0 ? (size_t)g.ncell * sp.moves : c->n * (size_t)ms.n;
     r = ensureProposals(c, pl.proposals);
@@ -1001,9 +1049,10 @@ int planSweep(rp_ctx *c, bool logDecisions, SweepPlan &pl) {
     // kernel has no registers to spare for them)
     if (waves == 8 && ms.n > 1) waves = 4;
     sp.pipelined = c->pipelined >= 0 ? c->pipelined : (waves >= 8 ? 2 : 1);
-    // guarded FP32 stencil scan: the default launch of hard spherocylinders (eight wavefronts, grouped moves)
+    // guarded FP32 stencil scan: the default launch of hard spherocylinders (eight wavefronts, grouped moves); the
+    // all-pairs path (no grid) tests in FP64
     const int g32 = c->precision != RP_PREC_FP64 && waves == 8 && sp.pipelined == 2 && !c->k.soft && !c->counting &&
-                    c->k.kind == RP_SHAPE_SPHEROCYLINDER ? c->precision : 0;
+                    c->k.kind == RP_SHAPE_SPHEROCYLINDER && !g.allPairs ? c->precision : 0;
     if (g32) g32Limits(c, g, sp.g32Lim);
     const double perEntry =
         (double)(sweep_lds_bytes(c->k.kind, 1024, waves, g32) - sweep_lds_bytes(c->k.kind, 0, waves, g32)) / 1024.0;
@@ -1024,6 +1073,7 @@ int planSweep(rp_ctx *c, bool logDecisions, SweepPlan &pl) {
         if (pairCap < sp.cap && pairCap >= expect + 3 * std::sqrt(expect)) sp.cap = pairCap;
     }
     if (c->k.kind == RP_SHAPE_POLYSPHERE) sp.cap = 64;  // the polysphere sweep reads its stencil from HBM
+    if (g.allPairs) sp.cap = allPairsStaged(c);
     sp.counters = c->dCounters;
     sp.phases = c->dPhases;
     sp.tuning = c->sweepTuning;
@@ -1199,9 +1249,14 @@ int countExact(rp_ctx *c, int earlyExit, uint64_t &out) {
     int disabled = 0;
     int r = makeExactGrid(c, c->box, g, disabled);
     if (r) return r;
-    if (disabled)
+    if (disabled && noGridBox(c, c->box)) {
+        // the reference counts every pair through the nearest image (Packing.cpp:547-556)
+        r = makeAllPairsGrid(c, c->box, g);
+        if (r) return r;
+    } else if (disabled) {
         return fail(RP_ERR_UNSUPPORTED, "the reference would count without a neighbour grid for this box; "
-                                        "the all-pairs path is not implemented on the device");
+                                        "the all-pairs path covers hard spheres and spherocylinders");
+    }
     r = rebuild(c, g, 0, nullptr);
     if (r) return r;
     r = scanOverlaps(c, g, earlyExit, out);
@@ -2040,6 +2095,7 @@ int fixOrientations(rp_ctx *c, uint64_t &rejected) {
     const int maxCap = (int)((double)(c->ldsLimit - sweep_lds_bytes(c->k.kind, 0, waves)) / perEntry) / 64 * 64;
     sp.cap = std::max(64, std::min(cap, maxCap));
     if (c->k.kind == RP_SHAPE_POLYSPHERE) sp.cap = 64;  // read from HBM
+    if (g.allPairs) sp.cap = allPairsStaged(c);
     const int colours = g.period * g.period * g.period;
     for (int colour = 0; colour < colours; colour++) {
         const int par[3] = {colour % g.period, (colour / g.period) % g.period, colour / (g.period * g.period)};

@@ -33,6 +33,8 @@ struct Grid {
     int exact;
     int reach;              // stencil half-width in cells: cells are at least range / reach high
     int period;             // checkerboard colour period per axis (reach + 1); colours = period^3
+    int allPairs;           // 1: no grid (the reference's box too small for its neighbour grid, Packing.cpp:862-873):
+                            // one cell holds every particle and pairs meet through the nearest image only
     int *cellStart;         // ncell + 1
     int *cellCount;         // ncell
 };
@@ -66,7 +68,8 @@ struct SweepParams {
     int mult;               // moves per particle in that case: the number of move samplers
     int colour;
     int na[3];              // active cells per axis (n / period)
-    int cap;                // stencil entries that fit in LDS
+    int cap;                // stencil entries that fit in LDS; without a grid: the particles the all-pairs sweep
+                            // stages in LDS (0: it reads them in place)
     int pipelined;          // 1: wavefront 0 plans the next move while the others test; 2: grouped moves
                             // (both need >= 2 wavefronts)
     double temperature;     // soft interactions: the move is accepted if u <= exp(-dE / temperature)
@@ -166,6 +169,13 @@ hipError_t launch_rebuild(hipStream_t s, const Grid &g, int n, Particles src, Pa
 hipError_t launch_count_overlaps(hipStream_t s, const Grid &g, int n, Particles p, PairConsts k, int earlyExit,
                                  unsigned long long *counters, int cap, Timed t = {});
 size_t count_lds_bytes(int kind, int cap);
+// Every pair i < j of the n particles (a grid with allPairs set) through the nearest image, the particle with the lower
+// caller index the first operand (Packing::countTotalOverlaps without a grid, Packing.cpp:547-556); adds to
+// counters[CNT_OVERLAPS], with earlyExit stopping once any is found.
+hipError_t launch_count_allpairs(hipStream_t s, const Grid &g, int n, Particles p, PairConsts k, int earlyExit,
+                                 unsigned long long *counters);
+// Dynamic LDS of the all-pairs sweep that stages n particles and its proposals.
+size_t allpairs_lds_bytes(int n);
 // Proposals of one cycle: `moves` per cell when moves > 0, else ms.n per particle of its cell (cellOf = cell of every
 // sorted record, n records).
 hipError_t launch_proposals(hipStream_t s, const Grid &g, const Proposals &pr, int moves, const int *cellOf, int n,

@@ -2749,6 +2749,157 @@ hipError_t launch_fix_all(hipStream_t s, const Particles &p, int n) {
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------------------------------------------
+// Packings without a neighbour grid: every pair through the nearest image (PeriodicBoundaryConditions::getTranslation
+// for spherocylinders, getDistance2 for spheres; SpherocylinderTraits.cpp:28-46, SphereTraits.cpp:56-65)
+
+constexpr int ALLPAIRS_THREADS = 256;
+
+template <int KIND>
+__device__ __forceinline__ bool allpairs_hit(const PairConsts &k, const Grid &g, const SCFirst &f, const V3 &p2,
+                                             const V3 &a2) {
+    if (KIND == 0) return pbc_distance2(g.dims, g.inv, f.pos, p2) < k.contact2;
+    return sc_overlap(k, f, p2, a2, pbc_translation(g.dims, g.inv, f.pos, p2));
+}
+
+size_t allpairs_lds_bytes(int n) {
+    return sizeof(double) * 12 * (size_t)n + (sizeof(double) * 12 + sizeof(int)) * ALLPAIRS_THREADS;
+}
+
+// The cycle's moves of a packing without a grid, in order, by one workgroup (Simulation::performMoves without domains,
+// Simulation.cpp:365-445: every move picks any particle; Packing::tryMove / countParticleOverlaps without a grid,
+// Packing.cpp:136-164, 410-420): the trial pose against every other particle, accepted poses written back at once.
+// fixMode: the orientation renormalisation (Packing::renormalizeOrientations), one trial per particle in storage
+// order, fix_rotation(O) in place.
+// LDS: the packing (positions and the nine orientation components, P.cap particles at most) and the proposals,
+// ALLPAIRS_THREADS moves at a time, are staged in LDS and every move reads LDS only; accepted poses are written to
+// both.  Otherwise every read goes to global memory (packings too large for LDS).
+template <int KIND, bool LDS>
+__global__ __launch_bounds__(ALLPAIRS_THREADS) void k_sweep_allpairs(SweepParams P) {
+    extern __shared__ double smem[];
+    constexpr int T = ALLPAIRS_THREADS;
+    const Grid &g = P.g;
+    const int n = g.cellStart[1];
+    const int tid = threadIdx.x;
+    const int moves = P.fixMode ? n : P.moves;
+    if (LDS && n > P.cap) {
+        // the launch's LDS holds P.cap particles
+        if (tid == 0) atomicAdd(&P.counters[CNT_STAGE_OVERFLOW], 1ull);
+        return;
+    }
+    double *sx = nullptr, *sy = nullptr, *sz = nullptr, *so = nullptr, *ct = nullptr, *cR = nullptr;
+    int *ck = nullptr;
+    if (LDS) {
+        sx = smem;
+        sy = sx + n;
+        sz = sy + n;
+        so = sz + n;  // component c of particle j at so[c * n + j]
+        ct = so + 9 * n;
+        cR = ct + 3 * T;
+        ck = (int *)(cR + 9 * T);
+        for (int j = tid; j < n; j += T) {
+            sx[j] = P.p.x[j];
+            sy[j] = P.p.y[j];
+            sz[j] = P.p.z[j];
+            for (int c = 0; c < 9; c++) so[c * n + j] = P.p.o[c][j];
+        }
+    }
+    auto pos_of = [&](int j) { return LDS ? v3(sx[j], sy[j], sz[j]) : load3(P.p.x, P.p.y, P.p.z, j); };
+    auto o_of = [&](int c, int j) { return LDS ? so[c * n + j] : P.p.o[c][j]; };
+    unsigned long long accepted = 0, tests = 0;
+    for (int base = 0; base < moves; base += T) {
+        const int cn = min(T, moves - base);
+        if (LDS) {
+            __syncthreads();  // the packing staged; every thread done with the previous chunk's proposals
+            if (!P.fixMode && tid < cn) {
+                const int i = base + tid;
+                ck[tid] = P.pr.k[i];
+                for (int d = 0; d < 3; d++) ct[d * T + tid] = P.pr.t[d * P.pr.stride + i];
+                for (int c = 0; c < 9; c++) cR[c * T + tid] = P.pr.R[c * P.pr.stride + i];
+            }
+            __syncthreads();
+        }
+        for (int mm = 0; mm < cn; mm++) {
+            const int m = base + mm;
+            const int kIdx = P.fixMode ? m : (LDS ? ck[mm] : P.pr.k[m]);
+            const V3 pos = pos_of(kIdx);
+            V3 npos = pos;
+            double O[9], R[9], NO[9];
+            for (int c = 0; c < 9; c++) O[c] = o_of(c, kIdx);
+            if (P.fixMode) {
+                for (int c = 0; c < 9; c++) NO[c] = O[c];
+                fix_rotation(NO);
+            } else {
+                const V3 t = LDS ? v3(ct[mm], ct[T + mm], ct[2 * T + mm])
+                                 : v3(P.pr.t[m], P.pr.t[P.pr.stride + m], P.pr.t[2 * P.pr.stride + m]);
+                npos = translate_wrapped(g.dims, g.inv, pos, t);
+                for (int c = 0; c < 9; c++) R[c] = LDS ? cR[c * T + mm] : P.pr.R[c * P.pr.stride + m];
+                mulm(R, O, NO);
+            }
+            const SCFirst f = make_first<KIND>(P.k, npos, v3(NO[2], NO[5], NO[8]));
+            bool hit = false;
+            for (int j = tid; j < n && !hit; j += T) {
+                if (j == kIdx) continue;
+                const V3 a2 = KIND == 1 ? v3(o_of(2, j), o_of(5, j), o_of(8, j)) : v3(0, 0, 0);
+                hit = allpairs_hit<KIND>(P.k, g, f, pos_of(j), a2);
+            }
+            tests += (unsigned long long)(n - 1);
+            if (__syncthreads_or(hit)) continue;
+            accepted++;
+            if (tid == 0) {
+                P.p.x[kIdx] = npos.x;
+                P.p.y[kIdx] = npos.y;
+                P.p.z[kIdx] = npos.z;
+                for (int c = 0; c < 9; c++) P.p.o[c][kIdx] = NO[c];
+                P.p.wbits[kIdx] = 0;
+                if (P.pr.acc && !P.fixMode) P.pr.acc[m] = 1;
+                if (LDS) {
+                    sx[kIdx] = npos.x;
+                    sy[kIdx] = npos.y;
+                    sz[kIdx] = npos.z;
+                    for (int c = 0; c < 9; c++) so[c * n + kIdx] = NO[c];
+                }
+            }
+            __syncthreads();
+        }
+    }
+    flush_counters(P, accepted, moves, 0, tests, true);
+}
+
+template <int KIND>
+__global__ __launch_bounds__(ALLPAIRS_THREADS) void k_count_allpairs(Grid g, int n, Particles p, PairConsts k,
+                                                                    int earlyExit, unsigned long long *counters) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n) return;
+    const int ti = p.tag[i];
+    const SCFirst f = make_first<KIND>(k, load3(p.x, p.y, p.z, i), v3(p.o[2][i], p.o[5][i], p.o[8][i]));
+    unsigned long long c = 0;
+    for (int j = 0; j < n; j++) {
+        if (p.tag[j] <= ti) continue;  // each pair once, from the particle with the lower caller index
+        const V3 a2 = KIND == 1 ? v3(p.o[2][j], p.o[5][j], p.o[8][j]) : v3(0, 0, 0);
+        if (allpairs_hit<KIND>(k, g, f, load3(p.x, p.y, p.z, j), a2)) {
+            c++;
+            if (earlyExit) break;
+        }
+    }
+    if (c) atomicAdd(&counters[CNT_OVERLAPS], c);
+}
+
+hipError_t launch_count_allpairs(hipStream_t s, const Grid &g, int n, Particles p, PairConsts k, int earlyExit,
+                                 unsigned long long *counters) {
+    if (n == 0) return hipSuccess;
+    const unsigned blocks = (unsigned)((n + ALLPAIRS_THREADS - 1) / ALLPAIRS_THREADS);
+    if (k.kind == 0)
+        hipLaunchKernelGGL(k_count_allpairs<0>, dim3(blocks), dim3(ALLPAIRS_THREADS), 0, s, g, n, p, k, earlyExit,
+                           counters);
+    else if (k.kind == 1)
+        hipLaunchKernelGGL(k_count_allpairs<1>, dim3(blocks), dim3(ALLPAIRS_THREADS), 0, s, g, n, p, k, earlyExit,
+                           counters);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
 int sweep_occupancy(int kind, int cap, int waves) {
     int w = waves >= 8 ? 8 : (waves >= 4 ? 4 : (waves == 2 ? 2 : 1));
     size_t lds = sweep_lds_bytes(kind, cap, w);
@@ -2770,6 +2921,21 @@ int sweep_occupancy(int kind, int cap, int waves) {
 }
 
 hipError_t launch_sweep(hipStream_t s, const SweepParams &sp, int waves, Timed t, int g32) {
+    if (sp.g.allPairs) {
+        // no grid: one workgroup runs the cycle's moves in order (hard spheres and spherocylinders)
+        if (g32 || sp.k.soft || sp.counting || sp.k.kind == 2 || (!sp.fixMode && sp.moves <= 0))
+            return hipErrorInvalidValue;
+        // sp.cap > 0: the packing (at most sp.cap particles) and the proposals staged in LDS
+        const uint32_t lds = sp.cap > 0 ? (uint32_t)allpairs_lds_bytes(sp.cap) : 0;
+#define RP_ALLPAIRS(KIND, STAGED) \
+    hipExtLaunchKernelGGL((k_sweep_allpairs<KIND, STAGED>), dim3(1), dim3(ALLPAIRS_THREADS), lds, s, t.start, t.stop, 0, sp)
+        if (sp.k.kind == 0 && sp.cap > 0) RP_ALLPAIRS(0, true);
+        else if (sp.k.kind == 0) RP_ALLPAIRS(0, false);
+        else if (sp.cap > 0) RP_ALLPAIRS(1, true);
+        else RP_ALLPAIRS(1, false);
+#undef RP_ALLPAIRS
+        return hipGetLastError();
+    }
     unsigned blocks = (unsigned)(sp.na[0] * sp.na[1] * sp.na[2]);
     int w = waves >= 8 ? 8 : (waves >= 4 ? 4 : (waves == 2 ? 2 : 1));
     if (w == 8 && (sp.k.soft || sp.counting || sp.k.kind == 2)) w = 4;  // eight only for the staged hard sweep
@@ -2816,7 +2982,7 @@ hipError_t launch_sweep(hipStream_t s, const SweepParams &sp, int waves, Timed t
 }
 
 int sweep_batch_key(const SweepParams &sp, int waves, int g32) {
-    if (sp.k.soft || sp.counting || sp.fixMode || sp.k.kind == 2 || sp.trace) return -1;
+    if (sp.k.soft || sp.counting || sp.fixMode || sp.k.kind == 2 || sp.trace || sp.g.allPairs) return -1;
     int w = waves >= 8 ? 8 : (waves >= 4 ? 4 : (waves == 2 ? 2 : 1));
     const bool lean = w == 8 && sp.pipelined == 2 && sp.k.kind == 1;  // as launch_sweep chooses
     if (g32 && !lean) return -1;

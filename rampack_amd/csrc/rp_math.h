@@ -206,6 +206,34 @@ RP_HD bool sphere_overlap(const PairConsts &k, const V3 &p1, const V3 &p2, const
     return dot(d, d) < k.contact2;
 }
 
+// PeriodicBoundaryConditions::getTranslation (PeriodicBoundaryConditions.cpp:24-36): the box translation that brings
+// p2 within half a box of p1 in relative coordinates (the nearest image), box * t with t integral per axis.
+RP_HD V3 pbc_translation(const double *dims, const double *inv, const V3 &p1, const V3 &p2) {
+    const V3 r1 = mulv(inv, p1), r2 = mulv(inv, p2);
+    const double a[3] = {r1.x, r1.y, r1.z}, b[3] = {r2.x, r2.y, r2.z};
+    double t[3] = {0, 0, 0};
+    for (int i = 0; i < 3; i++) {
+        while (b[i] + t[i] - a[i] > 0.5) t[i] -= 1;
+        while (b[i] + t[i] - a[i] < -0.5) t[i] += 1;
+    }
+    return mulv(dims, v3(t[0], t[1], t[2]));
+}
+
+// PeriodicBoundaryConditions::getDistance2 (PeriodicBoundaryConditions.cpp:38-50): per axis |r2 - r1| in relative
+// coordinates folded to at most 0.5 (the fold leaves it negative), back to absolute coordinates, squared norm.
+RP_HD double pbc_distance2(const double *dims, const double *inv, const V3 &p1, const V3 &p2) {
+    const V3 r1 = mulv(inv, p1), r2 = mulv(inv, p2);
+    const double a[3] = {r1.x, r1.y, r1.z}, b[3] = {r2.x, r2.y, r2.z};
+    double d[3];
+    for (int i = 0; i < 3; i++) {
+        double c = fabs(b[i] - a[i]);
+        while (c > 0.5) c -= 1;
+        d[i] = c;
+    }
+    const V3 v = mulv(dims, v3(d[0], d[1], d[2]));
+    return dot(v, v);
+}
+
 // RepulsiveLennardJonesInteraction.cpp:8-16
 RP_HD double wca_energy(double eps, double sigma2, double cut2, double d2) {
     if (d2 >= cut2) return 0;

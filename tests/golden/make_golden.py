@@ -113,7 +113,7 @@ DIMER_NPT = dict(name="dimer_npt", beads="dimer", n_side=6, spacing=(1.6, 1.6, 2
 # simple-cubic start, delta_triclinic box moves; packing fraction and nematic P2 per independent run
 SC_NPT = dict(name="sc_npt_p1", n_side=(10, 10, 3), spacing=(1.6, 1.6, 6.5), temperature=1.0, pressure=1.0,
               trans_step=0.2, rot_step=0.2, box_step=0.01, therm=5000, avg=6000, every=50,
-              seeds=[11, 12, 13, 14, 15, 16, 17, 18])
+              seeds=list(range(11, 35)))
 
 
 def sc_run(args):

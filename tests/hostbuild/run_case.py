@@ -3,6 +3,7 @@ AddressSanitizer runtime preloaded; any access outside an allocation aborts the 
 
     python tests/hostbuild/run_case.py LIB replay WAVES PIPE     # decision records of one launch shape vs the oracle
     python tests/hostbuild/run_case.py LIB replica I CYCLES      # replica I of the 32-replica scan, overlap checks
+    python tests/hostbuild/run_case.py LIB nogrid NAME CYCLES    # a packing without a grid, overlap checks, replays
 """
 import os
 import sys
@@ -10,6 +11,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 sys.path.insert(0, os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(os.path.dirname(HERE), "golden"))
 
 import numpy as np  # noqa: E402
 
@@ -61,9 +63,40 @@ def replica(i: int, cycles: int) -> None:
     pk.close()
 
 
+def nogrid(name: str, cycles: int) -> None:
+    """A packing the reference runs without its neighbour grid: "liquid" (300 rods from the 10 x 10 x 3 lattice, p* = 1,
+    triclinic box moves: the all-pairs sweep with the packing staged in LDS) or "slab" (1500 rods lying in two layers
+    of a 97.5 x 60 x 5.5 box: too many for LDS, read in place).  The decision records replay through the oracle before
+    and after the cycles, and the oracle finds no overlap."""
+    from test_gpu_replay import check_record
+    from test_gpu_nogrid import lying_rod_slab
+    if name == "liquid":
+        box, pos, rot = lattice.spherocylinder_lattice(n_side=(10, 10, 3), spacing=(1.6, 1.6, 6.5))
+    else:
+        box, pos, rot = lying_rod_slab()
+    pk = ra.GpuPacking(SC, box, pos, rot, seed=17)
+    pk.set_environment(1.0, 1.0, 0.2, 0.2, box_move=("delta_triclinic", 0.01))
+    rec = pk.record_sweep()
+    assert rec["cells"] == [1, 1, 1]
+    check_record(SC, rec)
+    c = pk.run_cycles(cycles, adapt_steps=True)
+    t = pk.timing()
+    b, q, r = pk.download()
+    overl = oracle_lib.oracle().snapshot_stats(1, 0.5, 5.0, b, q, r)[0]
+    print(f"{name}: N={len(pos)} moves {c['moves_accepted']}/{c['moves_attempted']}, box {c['box_accepted']}/"
+          f"{c['box_attempted']}, sweep period {t['sweep_period']}, staged {t['sweep_cap']}, overlaps {overl}",
+          flush=True)
+    assert t["sweep_period"] == 1 and t["sweep_cap"] == (len(pos) if name == "liquid" else 0)
+    assert overl == 0
+    check_record(SC, pk.record_sweep())
+    pk.close()
+
+
 if __name__ == "__main__":
     if sys.argv[2] == "replay":
         replay(sys.argv[3], sys.argv[4])
+    elif sys.argv[2] == "nogrid":
+        nogrid(sys.argv[3], int(sys.argv[4]))
     else:
         replica(int(sys.argv[3]), int(sys.argv[4]))
     print("case passed", flush=True)

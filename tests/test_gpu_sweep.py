@@ -45,9 +45,11 @@ def test_sweep_keeps_packing_overlap_free_and_moves_every_particle():
 
 
 def test_two_cell_grid_reads_stencil_from_hbm():
-    """A box two cells wide: the 27-cell stencil repeats cells under different images and exceeds the LDS area,
-    so the blocks read candidates from HBM; the sweep must stay overlap-free there too."""
-    box, pos, rot = lattice.spherocylinder_lattice(n_side=(12, 12, 3), spacing=(1.6, 1.6, 6.4))
+    """A box two cells wide across the rods (21 x 21 x 25.6: tall enough for the reference's grid, which needs four
+    interaction ranges along some direction): the 27-cell stencil repeats cells under different images, and the
+    dense cells hold more rods than the own-cell table, so the blocks read candidates from HBM; the sweep must stay
+    overlap-free there too."""
+    box, pos, rot = lattice.spherocylinder_lattice(n_side=(20, 20, 4), spacing=(1.05, 1.05, 6.4))
     pk = ra.GpuPacking(SC, box, pos, rot, seed=2)
     pk.set_environment(1.0, 1.0, 0.3, None, box_move=("delta_triclinic", 0.02))
     c = pk.run_cycles(20)
@@ -395,16 +397,17 @@ def test_anisotropic_linear_and_one_side_box_moves():
 
 def test_spherocylinder_liquid_npt_matches_reference_runs(golden):
     """NpT hard spherocylinders L/D = 5 at p* = 1 (an isotropic liquid near packing fraction 0.31, 300 rods from an
-    aligned lattice, delta_triclinic box moves) against eight runs of the reference build (its Packing, samplers and
-    scaler under the restated cycle loop): the mean packing fraction and nematic P2 over independent device runs
+    aligned lattice, delta_triclinic box moves) against 24 runs of the reference build (its Packing, samplers and
+    scaler under the restated cycle loop): the mean packing fraction and nematic P2 over 16 independent device runs
     agree with the reference's means within 2 standard errors of the difference, each side's error taken from the
-    spread between its runs."""
+    spread between its runs (the run means scatter by about 0.002 in packing fraction: the box is small enough for
+    the reference to run it without its neighbour grid, and so does the device, through every pair)."""
     st = next((s for s in golden.get("statistics", []) if s["name"] == "sc_npt_p1"), None)
     if st is None:
         pytest.skip("no reference statistics in the fixture")
     box, pos, rot = lattice.spherocylinder_lattice(n_side=tuple(st["n_side"]), spacing=tuple(st["spacing"]))
     phis, p2s = [], []
-    for seed in (1, 2, 3, 4, 5, 6):
+    for seed in range(1, 17):
         pk = ra.GpuPacking(SC, box, pos, rot, seed=seed)
         pk.set_environment(st["temperature"], st["pressure"], st["trans_step"], st["rot_step"],
                            box_move=("delta_triclinic", st["box_step"]))
@@ -417,6 +420,7 @@ def test_spherocylinder_liquid_npt_matches_reference_runs(golden):
             p2.append(o["nematic_p2"])
         phis.append(float(np.mean(phi)))
         p2s.append(float(np.mean(p2)))
+        print(f"seed {seed}: phi {phis[-1]:.5f}, P2 {p2s[-1]:.4f}", flush=True)
         b, p, r = pk.download()
         assert overlaps_by_oracle(SC, b, p, r) == 0
         pk.close()

@@ -6,6 +6,8 @@ exactly the requested size).
 * Every launch shape's decision records replay through the oracle with no mismatch: the CPU build reaches the
   device's decisions, and the kernels' intra-wavefront ordering is explicit (a lane that runs ahead of the others of
   its wavefront, as lanes do here, sees nothing stale).
+* Packings without a neighbour grid (the all-pairs sweep, staged in LDS and read in place) run with box moves:
+  no access outside an allocation, no overlap, and their decision records replay with no mismatch.
 * Replicas of the 32-replica equation-of-state scan (partly filled parallel lattices at packing fraction 0.2,
   p* from 0.5 to 4) run with box moves and step adaptation: no access outside an allocation, no overlap, and their
   decision records replay with no mismatch.
@@ -42,6 +44,11 @@ def run_case(lib, *args, timeout=600):
 @pytest.mark.parametrize("shape", ["1 0", "2 1", "4 1", "4 2", "8 1", "8 2"])
 def test_launch_shapes_replay_on_cpu_build(host_lib, shape):
     run_case(host_lib, "replay", *shape.split())
+
+
+@pytest.mark.parametrize("name,cycles", [("liquid", 5), ("slab", 2)])
+def test_packings_without_grid_on_cpu_build(host_lib, name, cycles):
+    run_case(host_lib, "nogrid", name, cycles)
 
 
 @pytest.mark.parametrize("replica", [13, 8, 1])

@@ -177,3 +177,91 @@ def test_replay_equals_all_image_brute_force(case):
     if case == "rods_one_cell_axis":
         # some moves are rejected only by the particle's own image, which the replay must not test
         assert (dec == 1).sum() > 0
+
+
+def nearest_image(dims, inv, p1, p2):
+    """PeriodicBoundaryConditions::getTranslation (PeriodicBoundaryConditions.cpp:24-36), in the same operations."""
+    r1, r2 = mulv(inv, p1), mulv(inv, p2)
+    t = [0.0, 0.0, 0.0]
+    for i in range(3):
+        while r2[i] + t[i] - r1[i] > 0.5:
+            t[i] -= 1
+        while r2[i] + t[i] - r1[i] < -0.5:
+            t[i] += 1
+    return mulv(dims, t)
+
+
+def nearest_distance2(dims, inv, p1, p2):
+    """PeriodicBoundaryConditions::getDistance2 (PeriodicBoundaryConditions.cpp:38-50), in the same operations."""
+    r1, r2 = mulv(inv, p1), mulv(inv, p2)
+    d = []
+    for i in range(3):
+        c = abs(r2[i] - r1[i])
+        while c > 0.5:
+            c -= 1
+        d.append(c)
+    v = mulv(dims, d)
+    s = 0.0
+    for x in v:
+        s += x * x
+    return s
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_replay_without_grid_equals_nearest_image_brute_force(kind):
+    """A box the reference runs without its neighbour grid (Packing.cpp:862-883: fewer than four interaction-range
+    cells across every direction): the replay tests every pair through the nearest image only, in move order, with no
+    domain rule.  Brute force: the same proposals, each against every other particle under its nearest image."""
+    orc = oracle_lib.oracle()
+    if kind == 0:
+        r, l = 0.5, 0.0
+        box = np.array([[3.2, 0.3, 0.0], [0.0, 3.5, -0.2], [0.0, 0.0, 3.0]])  # every height below 4 ranges
+        pos = np.array([[x, y, z] for x in (0.4, 2.0) for y in (0.5, 2.1) for z in (0.6, 2.1)], dtype=float)
+        trans, angle = 0.6, 0.0
+    else:
+        r, l = 0.5, 5.0
+        box = np.array([[7.5, 0.4, 0.0], [0.0, 7.0, 0.3], [0.0, 0.0, 14.5]])
+        pos = np.array([[x, y, z] for x in (0.8, 2.6, 4.4, 6.2) for y in (0.9, 2.7, 4.5, 6.3) for z in (3.2, 10.4)],
+                       dtype=float)
+        trans, angle = 0.35, 0.25
+    n = len(pos)
+    rot = np.tile(np.eye(3).reshape(9), (n, 1))
+    rng = np.random.Generator(np.random.PCG64(40 + kind))
+    moves = 3 * n
+    prop_rot = np.zeros((moves, 9))
+    for i in range(moves):
+        ax = rng.normal(size=3)
+        ax /= np.linalg.norm(ax)
+        orc.lib.orc_rotation(ax.ctypes.data_as(oracle_lib._dp), float(rng.uniform(-angle, angle)),
+                             prop_rot[i].ctypes.data_as(oracle_lib._dp))
+    rec = {"box": box, "pre_pos": np.ascontiguousarray(pos), "pre_rot": np.ascontiguousarray(rot),
+           "pre_cell": np.zeros(n, dtype=np.int32), "cells": [1, 1, 1], "shift": np.zeros(3), "period": 1,
+           "colour_order": [0], "moves_per_cell": moves, "n_samplers": 1, "n_proposals": moves,
+           "prop_k": rng.integers(0, n, moves).astype(np.int32), "prop_t": rng.uniform(-trans, trans, (moves, 3)),
+           "prop_rot": prop_rot}
+    dec, post_pos, post_rot = oracle_lib.replay_sweep(kind, r, l, rec)
+    # brute force
+    dims = [float(x) for x in box.reshape(9)]
+    inv = np.zeros(9)
+    orc.lib.orc_box_info(np.ascontiguousarray(dims).ctypes.data_as(oracle_lib._dp), inv.ctypes.data_as(oracle_lib._dp),
+                         np.zeros(3).ctypes.data_as(oracle_lib._dp), np.zeros(1).ctypes.data_as(oracle_lib._dp))
+    inv = [float(x) for x in inv]
+    P = [list(map(float, p)) for p in pos]
+    O = [list(map(float, o)) for o in rot]
+    want = np.zeros(moves, dtype=np.uint8)
+    for q in range(moves):
+        gi = int(rec["prop_k"][q])
+        npos = translate_wrapped(dims, inv, P[gi], [float(x) for x in rec["prop_t"][q]])
+        no = mulm([float(x) for x in rec["prop_rot"][q]], O[gi])
+        others = [j for j in range(n) if j != gi]
+        if kind == 0:
+            hit = any(nearest_distance2(dims, inv, npos, P[j]) < 1.0 for j in others)
+        else:
+            recs = np.array([npos + no + P[j] + O[j] + nearest_image(dims, inv, npos, P[j]) for j in others])
+            hit = bool(orc.pair_decisions(kind, r, l, recs).any())
+        want[q] = 0 if hit else 1
+        if not hit:
+            P[gi], O[gi] = npos, no
+    assert np.array_equal(dec, want)
+    assert np.array_equal(post_pos, np.array(P)) and np.array_equal(post_rot, np.array(O))
+    assert (want == 0).sum() > 0 and (want == 1).sum() > 0, np.bincount(want)
