@@ -43,7 +43,7 @@ RECORD_BYTES = 96  # FP64 position + FP64 3x3 orientation, the reference's parti
 # HBM traffic of one k_sweep launch on this workload (profiles/k_sweep_hard_spherocylinders_N1e5_hbm_traffic.txt):
 # rocprofv3 FETCH_SIZE / WRITE_SIZE means in KiB per dispatch, and the same counters on a 256 MiB device-to-device
 # copy, which reads and writes exactly 256 MiB.
-SWEEP_TRAFFIC = dict(fetch_kib=10986.49, write_kib=1401.90, copy_fetch_kib=131084.12, copy_write_kib=262144.0,
+SWEEP_TRAFFIC = dict(fetch_kib=11161.49, write_kib=1402.26, copy_fetch_kib=131084.92, copy_write_kib=262144.0,
                      copy_bytes=268435456)
 
 
