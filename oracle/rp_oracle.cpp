@@ -200,6 +200,14 @@ static inline Vec translateWrapped(const Box &box, const Vec &pos, const Vec &t)
 }
 
 // SegmentDistanceCalculator.h:22-85
+// The method and its steps are Dan Sunday's closest distance between two segments (geomalgorithms.com), which the
+// reference routine is based on.  Its notice, kept with it as it asks:
+//   Copyright 2001 softSurfer, 2012 Dan Sunday
+//   This code may be freely used, distributed and modified for any purpose
+//   providing that this copyright notice is included with it.
+//   SoftSurfer makes no warranty for this code, and cannot be held
+//   liable for any real or imagined damage resulting from its use.
+//   Users of this code must verify correctness for their application.
 static inline double segmentDistance2(const Vec &s11, const Vec &s12, const Vec &s21, const Vec &s22) {
     constexpr double EPS = 0.0000000001;
     Vec u = sub(s12, s11), v = sub(s22, s21), w = sub(s11, s21);

@@ -122,6 +122,14 @@ RP_HD bool bead_overlap(double r1, double r2, const V3 &c1, const V3 &c2, const 
 
 // SegmentDistanceCalculator.h:22-85, with u = s12 - s11 and a = u.u passed in (both depend only on the first
 // segment, so the sweep computes them once per trial move).
+// The method and its steps are Dan Sunday's closest distance between two segments (geomalgorithms.com), which the
+// reference routine is based on.  Its notice, kept with it as it asks:
+//   Copyright 2001 softSurfer, 2012 Dan Sunday
+//   This code may be freely used, distributed and modified for any purpose
+//   providing that this copyright notice is included with it.
+//   SoftSurfer makes no warranty for this code, and cannot be held
+//   liable for any real or imagined damage resulting from its use.
+//   Users of this code must verify correctness for their application.
 RP_HD double segment_distance2(const V3 &s11, const V3 &u, double a, const V3 &s21, const V3 &s22) {
     const double EPS = 0.0000000001;
     V3 v = sub(s22, s21);
