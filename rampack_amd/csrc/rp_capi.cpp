@@ -92,6 +92,8 @@ struct rp_ctx {
     int cellEnergyCap = 0;
     double *sumScratch = nullptr;        // fixed-order sum of the partials
     unsigned long long *dFixCounters = nullptr;  // counters of the orientation renormalisation pass
+    unsigned long long *dSlots = nullptr;        // move-count slots of the sweep, SLOT_WORDS per grid cell
+    size_t slotsCap = 0;                         // cells the slots cover
     uint64_t fixEvery = 10000;           // cycles between orientation renormalisations (0: never)
     uint64_t fixRejected = 0;            // renormalisations skipped because they would create an overlap
     bool counting = false;               // overlap-counting mode (Packing::toggleOverlapCounting)
@@ -371,6 +373,20 @@ int ensureCells(rp_ctx *c, int ncell, bool beads = false) {
     RP_HIP(hipMalloc(&count, sizeof(int) * cap));
     RP_HIP(launch_fill_i32(c->stream, count, (size_t)cap, 0));  // rebuilds keep them zero from here on
     have = cap;
+    return RP_OK;
+}
+
+// Move-count slots for a grid of ncell cells.  Every fold clears the slots it covers, so between cycles they are all
+// zero and may be replaced.
+int ensureSlots(rp_ctx *c, int ncell) {
+    if ((size_t)ncell <= c->slotsCap) return RP_OK;
+    if (c->dSlots) hipFree(c->dSlots);
+    c->dSlots = nullptr;
+    c->slotsCap = 0;
+    const size_t cap = (size_t)ncell + (size_t)ncell / 4 + 64;
+    RP_HIP(hipMalloc(&c->dSlots, sizeof(unsigned long long) * SLOT_WORDS * cap));
+    RP_HIP(launch_fill_u64(c->stream, c->dSlots, SLOT_WORDS * cap, 0));
+    c->slotsCap = cap;
     return RP_OK;
 }
 
@@ -1074,7 +1090,10 @@ int planSweep(rp_ctx *c, bool logDecisions, SweepPlan &pl) {
     }
     if (c->k.kind == RP_SHAPE_POLYSPHERE) sp.cap = 64;  // the polysphere sweep reads its stencil from HBM
     if (g.allPairs) sp.cap = allPairsStaged(c);
+    r = ensureSlots(c, g.ncell);
+    if (r) return r;
     sp.counters = c->dCounters;
+    sp.slots = c->dSlots;
     sp.phases = c->dPhases;
     sp.tuning = c->sweepTuning;
     sp.temperature = c->temperature;
@@ -1098,6 +1117,7 @@ int planSweep(rp_ctx *c, bool logDecisions, SweepPlan &pl) {
 // mode, the cached overlap count.
 int finishSweep(rp_ctx *c, const SweepPlan &pl) {
     c->sortedForCycle = false;
+    RP_HIP(launch_fold_slots(c->stream, c->dSlots, pl.sp.g.ncell, c->dCounters));
     RP_HIP(launch_type_counts(c->stream, pl.sp.pr, pl.proposals, c->dCounters));
     if (pl.sp.counting) {
         // Packing::acceptMove adds each accepted move's overlap change to the cached count (Packing.cpp:298-301)
@@ -2043,6 +2063,7 @@ void rp_destroy(rp_ctx *c) {
     if (c->cellEnergy) hipFree(c->cellEnergy);
     if (c->sumScratch) hipFree(c->sumScratch);
     if (c->dFixCounters) hipFree(c->dFixCounters);
+    if (c->dSlots) hipFree(c->dSlots);
     if (c->mlpTheta) hipFree(c->mlpTheta);
     for (auto *d : {&c->mlpSet, &c->mlpEval})
         if (d->x) hipFree(d->x);
@@ -2084,6 +2105,9 @@ int fixOrientations(rp_ctx *c, uint64_t &rejected) {
     sp.fixMode = 1;
     sp.pipelined = 0;
     sp.counters = c->dFixCounters;
+    r = ensureSlots(c, g.ncell);
+    if (r) return r;
+    sp.slots = c->dSlots;
     const double lambda = (double)c->n / (double)g.ncell;
     const int side = 2 * g.reach + 1;
     const double expect = side * side * side * lambda;
@@ -2103,6 +2127,7 @@ int fixOrientations(rp_ctx *c, uint64_t &rejected) {
         sp.colour = colour;
         RP_HIP(launch_sweep(c->stream, sp, waves));
     }
+    RP_HIP(launch_fold_slots(c->stream, c->dSlots, g.ncell, c->dFixCounters));
     unsigned long long h[CNT_NUM];
     RP_HIP(hipMemcpyAsync(h, c->dFixCounters, sizeof h, hipMemcpyDeviceToHost, c->stream));
     RP_HIP(hipStreamSynchronize(c->stream));

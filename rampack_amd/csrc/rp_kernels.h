@@ -78,6 +78,7 @@ struct SweepParams {
     int fixMode;            // orientation renormalisation pass: one trial per own particle in index order, no
                             // translation, orientation fix_rotation(O) instead of R O (Packing::renormalizeOrientations)
     unsigned long long *counters;
+    unsigned long long *slots;   // per grid cell: its workgroup's move counts (SLOT_WORDS words), see launch_fold_slots
     unsigned long long *phases;  // optional per-phase wall-clock sums (RP_PHASE_*), nullptr when not profiling
     int tuning;                  // grouped sweep wave priorities: bit 0 raises the planner, bit 1 the testers
                                  // that scan one slot more than the others
@@ -130,6 +131,21 @@ enum Counter : int {
     CNT_G32_MISMATCH = 20,    // verify mode: pairs whose guarded class differs from the FP64 class (0)
     CNT_NUM = 24
 };
+
+// Move counts of the sweep's workgroups: the workgroup of cell c stores its counts in slot c (SLOT_WORDS words) with
+// plain stores -- every cell is active once per cycle, so each slot is written at most once between folds -- and
+// launch_fold_slots, once per cycle, adds the slots of the cycle's grid into the counters and clears them.  A plain
+// store completes in the L2 of the workgroup's XCD; a device-scope atomic by every workgroup at the end of each
+// colour pass held the end of the launch back by about 2 us.
+enum SlotWord : int {
+    SLOT_ACCEPTED = 0,
+    SLOT_ATTEMPTED = 1,
+    SLOT_LEFT_CELL = 2,
+    SLOT_TESTS = 3,
+    SLOT_G32_PAIRS = 4,
+    SLOT_WORDS = 5
+};
+hipError_t launch_fold_slots(hipStream_t s, unsigned long long *slots, int ncell, unsigned long long *counters);
 
 // Launch wrappers (rp_kernels.hip).  All enqueue on `s` and return the launch status.
 hipError_t launch_pair_decisions(hipStream_t s, PairConsts k, size_t n, const double *rec, uint8_t *out);

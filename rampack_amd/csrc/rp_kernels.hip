@@ -1154,15 +1154,52 @@ size_t sweep_lds_bytes(int kind, int cap, int waves, int g32) {
            (g32 ? sizeof(float) * (3 * (size_t)cap + 81) : 0);
 }
 
-__device__ __forceinline__ void flush_counters(const SweepParams &P, unsigned long long accepted, int moves,
+__device__ __forceinline__ void flush_counters(const SweepParams &P, int cell, unsigned long long accepted, int moves,
                                                unsigned long long leftCell, unsigned long long tests, bool staged) {
     if (threadIdx.x == 0) {
-        atomicAdd(&P.counters[CNT_ACCEPTED], accepted);
-        atomicAdd(&P.counters[CNT_ATTEMPTED], (unsigned long long)moves);
-        atomicAdd(&P.counters[CNT_LEFT_CELL], leftCell);
-        atomicAdd(&P.counters[CNT_TESTS], tests);
+        unsigned long long *slot = P.slots + (size_t)cell * SLOT_WORDS;
+        slot[SLOT_ACCEPTED] = accepted;
+        slot[SLOT_ATTEMPTED] = (unsigned long long)moves;
+        slot[SLOT_LEFT_CELL] = leftCell;
+        slot[SLOT_TESTS] = tests;
         if (!staged) atomicAdd(&P.counters[CNT_UNSTAGED_BLOCKS], 1ull);
     }
+}
+
+// Adds the move-count slots of ncell cells into the counters and clears them (fixed partition of the cells over the
+// threads, tree sums per block, one atomic per word and block).
+constexpr int FOLD_THREADS = 256;
+
+__global__ __launch_bounds__(FOLD_THREADS) void k_fold_slots(unsigned long long *slots, int ncell,
+                                                             unsigned long long *counters) {
+    __shared__ unsigned long long part[SLOT_WORDS][FOLD_THREADS];
+    unsigned long long v[SLOT_WORDS] = {};
+    for (int c = blockIdx.x * FOLD_THREADS + threadIdx.x; c < ncell; c += gridDim.x * FOLD_THREADS) {
+        unsigned long long *slot = slots + (size_t)c * SLOT_WORDS;
+        for (int w = 0; w < SLOT_WORDS; w++) {
+            v[w] += slot[w];
+            slot[w] = 0;
+        }
+    }
+    for (int w = 0; w < SLOT_WORDS; w++) part[w][threadIdx.x] = v[w];
+    __syncthreads();
+    for (int off = FOLD_THREADS / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off)
+            for (int w = 0; w < SLOT_WORDS; w++) part[w][threadIdx.x] += part[w][threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const int target[SLOT_WORDS] = {CNT_ACCEPTED, CNT_ATTEMPTED, CNT_LEFT_CELL, CNT_TESTS, CNT_G32_PAIRS};
+        for (int w = 0; w < SLOT_WORDS; w++)
+            if (part[w][0]) atomicAdd(&counters[target[w]], part[w][0]);
+    }
+}
+
+hipError_t launch_fold_slots(hipStream_t s, unsigned long long *slots, int ncell, unsigned long long *counters) {
+    if (ncell <= 0) return hipSuccess;
+    const unsigned blocks = (unsigned)std::min(64, (ncell + FOLD_THREADS - 1) / FOLD_THREADS);
+    hipLaunchKernelGGL(k_fold_slots, dim3(blocks), dim3(FOLD_THREADS), 0, s, slots, ncell, counters);
+    return hipGetLastError();
 }
 
 // Copies proposals [first, first + cn) of this cell into LDS.
@@ -1327,7 +1364,7 @@ __device__ __forceinline__ void sweep_cell_staged(const SweepParams &P, const St
         int code = S.code[a];
         P.p.wbits[gi] = (uint8_t)(((code & 3) - 1) | ((((code >> 2) & 3) - 1) << 1) | ((((code >> 4) & 3) - 1) << 2));
     }
-    flush_counters(P, accepted, moves, leftCell, tests, true);
+    flush_counters(P, cell, accepted, moves, leftCell, tests, true);
 }
 
 // Pipelined sweep of one staged cell (two or more wavefronts).  Wavefront 0 is the planner: while the other
@@ -1508,7 +1545,7 @@ __device__ __forceinline__ void sweep_cell_pipelined(const SweepParams &P, const
         int code = S.code[a];
         P.p.wbits[gi] = (uint8_t)(((code & 3) - 1) | ((((code >> 2) & 3) - 1) << 1) | ((((code >> 4) & 3) - 1) << 2));
     }
-    flush_counters(P, accepted, moves, leftCell, tests, true);
+    flush_counters(P, cell, accepted, moves, leftCell, tests, true);
 }
 
 // Sweep of one active cell whose stencil does not fit the LDS area: candidates are read from the cell-sorted
@@ -1678,7 +1715,7 @@ __device__ __forceinline__ void sweep_cell_unstaged(const SweepParams &P, const 
         }
         __syncthreads();
     }
-    flush_counters(P, accepted, moves, leftCell, tests, false);
+    flush_counters(P, cell, accepted, moves, leftCell, tests, false);
 }
 
 // Grouped sweep of one staged cell (two or more wavefronts).  Up to GROUP consecutive trial moves on distinct
@@ -2126,7 +2163,7 @@ __device__ __forceinline__ void sweep_cell_grouped(const SweepParams &P, const S
         int code = S.code[a];
         P.p.wbits[gi] = (uint8_t)(((code & 3) - 1) | ((((code >> 2) & 3) - 1) << 1) | ((((code >> 4) & 3) - 1) << 2));
     }
-    flush_counters(P, accepted, moves, leftCell, tests, true);
+    flush_counters(P, cell, accepted, moves, leftCell, tests, true);
     if (G32 && !planner) {
         // per-wavefront sums of the lanes' statistics, one atomic each
         for (int off = WAVE / 2; off > 0; off >>= 1) {
@@ -2135,7 +2172,7 @@ __device__ __forceinline__ void sweep_cell_grouped(const SweepParams &P, const S
             g32Err = fmaxf(g32Err, __shfl_xor(g32Err, off));
         }
         if (lane == 0) {
-            if (g32Pairs) atomicAdd(&P.counters[CNT_G32_PAIRS], g32Pairs);
+            if (wave == 1) P.slots[(size_t)cell * SLOT_WORDS + SLOT_G32_PAIRS] = g32Pairs;
             if (g32Guard) atomicAdd(&P.counters[CNT_G32_GUARD], (unsigned long long)g32Guard);
             if (G32 == 2) {
                 if (g32Mismatch) atomicAdd(&P.counters[CNT_G32_MISMATCH], (unsigned long long)g32Mismatch);
@@ -2329,7 +2366,7 @@ __device__ __forceinline__ void sweep_cell_soft(const SweepParams &P, const Sten
                 (uint8_t)(((code & 3) - 1) | ((((code >> 2) & 3) - 1) << 1) | ((((code >> 4) & 3) - 1) << 2));
         }
     }
-    flush_counters(P, accepted, moves, leftCell, tests, STAGED);
+    flush_counters(P, cell, accepted, moves, leftCell, tests, STAGED);
     if (P.counting && tid == 0 && overlapDelta)
         atomicAdd(&P.counters[CNT_OVERLAP_DELTA], (unsigned long long)overlapDelta);
 }
@@ -2779,6 +2816,7 @@ __global__ __launch_bounds__(ALLPAIRS_THREADS) void k_sweep_allpairs(SweepParams
     extern __shared__ double smem[];
     constexpr int T = ALLPAIRS_THREADS;
     const Grid &g = P.g;
+    const int cell = 0;  // the one cell of a packing without a grid
     const int n = g.cellStart[1];
     const int tid = threadIdx.x;
     const int moves = P.fixMode ? n : P.moves;
@@ -2863,7 +2901,7 @@ __global__ __launch_bounds__(ALLPAIRS_THREADS) void k_sweep_allpairs(SweepParams
             __syncthreads();
         }
     }
-    flush_counters(P, accepted, moves, 0, tests, true);
+    flush_counters(P, cell, accepted, moves, 0, tests, true);
 }
 
 template <int KIND>
