@@ -93,7 +93,10 @@ struct rp_ctx {
     double *sumScratch = nullptr;        // fixed-order sum of the partials
     unsigned long long *dFixCounters = nullptr;  // counters of the orientation renormalisation pass
     unsigned long long *dSlots = nullptr;        // move-count slots of the sweep, SLOT_WORDS per grid cell
+    int *scanTmp = nullptr;                      // tile sums of the rebuild's scan on large grids
+    size_t scanTmpCap = 0;
     size_t slotsCap = 0;                         // cells the slots cover
+    int slotsDirty = 0;                          // cells of the grid whose slots hold counts not yet folded (0: none)
     uint64_t fixEvery = 10000;           // cycles between orientation renormalisations (0: never)
     uint64_t fixRejected = 0;            // renormalisations skipped because they would create an overlap
     bool counting = false;               // overlap-counting mode (Packing::toggleOverlapCounting)
@@ -373,13 +376,31 @@ int ensureCells(rp_ctx *c, int ncell, bool beads = false) {
     RP_HIP(hipMalloc(&count, sizeof(int) * cap));
     RP_HIP(launch_fill_i32(c->stream, count, (size_t)cap, 0));  // rebuilds keep them zero from here on
     have = cap;
+    const size_t scanInts = rebuild_scan_ints(cap);
+    if (scanInts > c->scanTmpCap) {
+        if (c->scanTmp) hipFree(c->scanTmp);
+        c->scanTmp = nullptr;
+        c->scanTmpCap = 0;
+        RP_HIP(hipMalloc(&c->scanTmp, sizeof(int) * scanInts));
+        c->scanTmpCap = scanInts;
+    }
     return RP_OK;
 }
 
 // Move-count slots for a grid of ncell cells.  Every fold clears the slots it covers, so between cycles they are all
 // zero and may be replaced.
+// Folds move counts the slots still hold into the counters (the next cycle's proposal launch does it otherwise).
+int foldSlots(rp_ctx *c) {
+    if (!c->slotsDirty) return RP_OK;
+    RP_HIP(launch_fold_slots(c->stream, c->dSlots, c->slotsDirty, c->dCounters));
+    c->slotsDirty = 0;
+    return RP_OK;
+}
+
 int ensureSlots(rp_ctx *c, int ncell) {
     if ((size_t)ncell <= c->slotsCap) return RP_OK;
+    int r = foldSlots(c);
+    if (r) return r;
     if (c->dSlots) hipFree(c->dSlots);
     c->dSlots = nullptr;
     c->slotsCap = 0;
@@ -572,11 +593,13 @@ int timingHarvest(rp_ctx *c) {
 
 int rebuild(rp_ctx *c, const Grid &g, int rescale, const double *oldInv) {
     RP_HIP(launch_rebuild(c->stream, g, (int)c->n, c->cur.p, c->alt.p, c->tmpSlot, c->tmpOrder, c->tmpPos, rescale,
-                          oldInv, timingSlot(c, 1)));
+                          oldInv, timingSlot(c, 1), c->scanTmp));
     return RP_OK;
 }
 
 int readCounters(rp_ctx *c) {
+    int r = foldSlots(c);
+    if (r) return r;
     RP_HIP(hipMemcpyAsync(c->hCounters, c->dCounters, sizeof(unsigned long long) * CNT_NUM, hipMemcpyDeviceToHost,
                           c->stream));
     RP_HIP(hipStreamSynchronize(c->stream));
@@ -1049,8 +1072,11 @@ int planSweep(rp_ctx *c, bool logDecisions, SweepPlan &pl) {
         sp.pr.type = nullptr;  // one sampler: the totals are its counters
         if (!logDecisions) sp.pr.acc = nullptr;
     }
+    // the previous cycle's move counts are folded by the same launch
+    const FoldSlots fold = c->slotsDirty ? FoldSlots{c->dSlots, c->slotsDirty, c->dCounters} : FoldSlots{};
     RP_HIP(launch_proposals(c->stream, g, sp.pr, sp.moves, c->cur.p.cell, (int)c->n, ms, c->seed,
-                            (uint32_t)c->cycle, timingSlot(c, 3)));
+                            (uint32_t)c->cycle, timingSlot(c, 3), fold));
+    c->slotsDirty = 0;
     const int side = 2 * g.reach + 1;
     double expect = side * side * side * lambda;
     int cap = (int)std::ceil(expect + 8 * std::sqrt(expect) + 64);
@@ -1117,7 +1143,7 @@ int planSweep(rp_ctx *c, bool logDecisions, SweepPlan &pl) {
 // mode, the cached overlap count.
 int finishSweep(rp_ctx *c, const SweepPlan &pl) {
     c->sortedForCycle = false;
-    RP_HIP(launch_fold_slots(c->stream, c->dSlots, pl.sp.g.ncell, c->dCounters));
+    c->slotsDirty = pl.sp.g.ncell;  // folded by the next cycle's proposal launch or the next counter read
     RP_HIP(launch_type_counts(c->stream, pl.sp.pr, pl.proposals, c->dCounters));
     if (pl.sp.counting) {
         // Packing::acceptMove adds each accepted move's overlap change to the cached count (Packing.cpp:298-301)
@@ -1255,7 +1281,7 @@ int countBeads(rp_ctx *c, const Particles &src, const HostBox &box, int earlyExi
                                         "the all-pairs path is not implemented on the device");
     RP_HIP(launch_centres(c->stream, src, (int)c->n, c->k, g, c->beadA.p));
     RP_HIP(launch_rebuild(c->stream, g, (int)items, c->beadA.p, c->beadB.p, c->tmpSlot, c->tmpOrder, c->tmpPos, 0,
-                          nullptr, timingSlot(c, 1)));
+                          nullptr, timingSlot(c, 1), c->scanTmp));
     r = scanItems(c, g, c->beadB.p, items, earlyExit, out);
     if (r) return r;
     if (earlyExit && out) out = 1;
@@ -2064,6 +2090,7 @@ void rp_destroy(rp_ctx *c) {
     if (c->sumScratch) hipFree(c->sumScratch);
     if (c->dFixCounters) hipFree(c->dFixCounters);
     if (c->dSlots) hipFree(c->dSlots);
+    if (c->scanTmp) hipFree(c->scanTmp);
     if (c->mlpTheta) hipFree(c->mlpTheta);
     for (auto *d : {&c->mlpSet, &c->mlpEval})
         if (d->x) hipFree(d->x);
@@ -2090,6 +2117,8 @@ int fixOrientations(rp_ctx *c, uint64_t &rejected) {
         return RP_OK;
     }
     int r = prepareCycleGrid(c);
+    if (r) return r;
+    r = foldSlots(c);  // the slots are about to count the renormalisation
     if (r) return r;
     if (!c->dFixCounters) RP_HIP(hipMalloc(&c->dFixCounters, sizeof(unsigned long long) * CNT_NUM));
     RP_HIP(launch_fill_u64(c->stream, c->dFixCounters, CNT_NUM, 0));

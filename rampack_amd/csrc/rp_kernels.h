@@ -145,6 +145,11 @@ enum SlotWord : int {
     SLOT_G32_PAIRS = 4,
     SLOT_WORDS = 5
 };
+struct FoldSlots {
+    unsigned long long *slots;     // nullptr: nothing to fold
+    int cells;
+    unsigned long long *counters;
+};
 hipError_t launch_fold_slots(hipStream_t s, unsigned long long *slots, int ncell, unsigned long long *counters);
 
 // Launch wrappers (rp_kernels.hip).  All enqueue on `s` and return the launch status.
@@ -178,8 +183,10 @@ hipError_t launch_fill_u64(hipStream_t s, unsigned long long *p, size_t n, unsig
 hipError_t launch_fill_i32(hipStream_t s, int *p, size_t n, int v);
 // Cell-list rebuild: src -> dst sorted by cell of `g`.  When rescale != 0 the positions are first mapped
 // newDims * (oldInv * pos) (Packing.cpp:183-184) with newDims = g.dims.
+// scanTmp: rebuild_scan_ints(g.ncell) ints (none for small grids).
 hipError_t launch_rebuild(hipStream_t s, const Grid &g, int n, Particles src, Particles dst, int *tmpSlot,
-                          int *tmpOrder, double *tmpPos, int rescale, const double *oldInv, Timed t = {});
+                          int *tmpOrder, double *tmpPos, int rescale, const double *oldInv, Timed t, int *scanTmp);
+size_t rebuild_scan_ints(int ncell);
 // Half-shell overlap count over the grid (Packing.cpp:431-559).  Adds to counters[CNT_OVERLAPS]; with
 // earlyExit the workgroups stop once any overlap is found.  cap: LDS entries per workgroup (>= largest cell).
 hipError_t launch_count_overlaps(hipStream_t s, const Grid &g, int n, Particles p, PairConsts k, int earlyExit,
@@ -195,7 +202,7 @@ size_t allpairs_lds_bytes(int n);
 // Proposals of one cycle: `moves` per cell when moves > 0, else ms.n per particle of its cell (cellOf = cell of every
 // sorted record, n records).
 hipError_t launch_proposals(hipStream_t s, const Grid &g, const Proposals &pr, int moves, const int *cellOf, int n,
-                            const MoveSet &ms, uint64_t seed, uint32_t cycle, Timed t = {});
+                            const MoveSet &ms, uint64_t seed, uint32_t cycle, Timed t, FoldSlots fold);
 // The sweep's move arithmetic on n explicit records (device arrays): pos + t wrapped into the box, the rotation matrix
 // of (axis, angle), R O, and sin / cos of the angle.
 hipError_t launch_apply_moves(hipStream_t s, const double *dims, const double *inv, size_t n, const double *pos,

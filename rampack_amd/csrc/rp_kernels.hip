@@ -403,42 +403,103 @@ __global__ __launch_bounds__(256) void k_bin(Grid g, int n, Particles src, int r
     if (valid) tmpSlot[i] = slot;
 }
 
-// Exclusive scan of the cell counts by one workgroup: contiguous chunk per thread, then a scan of the chunk sums.
-__global__ __launch_bounds__(1024) void k_scan(int ncell, int *__restrict__ count, int *__restrict__ start,
-                                               unsigned long long *counters) {
-    __shared__ int part[1024];
-    __shared__ int pmax[1024];
-    int t = threadIdx.x;
-    int per = (ncell + 1023) / 1024;
-    int b = t * per, e = min(b + per, ncell);
-    int s = 0, mx = 0;
-    for (int i = b; i < e; i++) {
-        int c = count[i];
-        s += c;
-        mx = max(mx, c);
+// Exclusive scan of the cell counts into cellStart (and cellStart[ncell] = n); the counts are cleared for the next
+// rebuild.  Up to SCAN_ONE_BLOCK cells: one workgroup, a contiguous chunk per thread; larger grids (bead grids,
+// million-particle packings): SCAN_TILE cells per workgroup -- tile sums, a scan of the tile sums, then each tile
+// scanned with its offset.  Within a workgroup the thread sums are scanned by wavefront shuffles and one pass over
+// the wavefront totals.
+constexpr int SCAN_THREADS = 1024;
+constexpr int SCAN_ONE_BLOCK = 16384;
+constexpr int SCAN_ITEMS = 8;                            // cells per thread in a tile
+constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;      // cells per workgroup
+constexpr int SCAN_MAX_TILES = SCAN_THREADS;              // tile sums scanned by one workgroup
+
+__device__ __forceinline__ int wave_inclusive_scan(int v) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    for (int off = 1; off < WAVE; off <<= 1) {
+        const int t = __shfl_up(v, off);
+        if (lane >= off) v += t;
     }
-    part[t] = s;
-    pmax[t] = mx;
+    return v;
+}
+
+// Exclusive prefix of v over the workgroup's threads (SCAN_THREADS); *total receives the sum.
+__device__ __forceinline__ int block_exclusive_scan(int v, int *waveTot, int *total) {
+    const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
+    constexpr int NW = SCAN_THREADS / WAVE;
+    const int incl = wave_inclusive_scan(v);
+    if (lane == WAVE - 1) waveTot[wave] = incl;
     __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        int v = t >= off ? part[t - off] : 0;
-        int m = t >= off ? pmax[t - off] : 0;
-        __syncthreads();
-        part[t] += v;
-        pmax[t] = max(pmax[t], m);
-        __syncthreads();
+    if (wave == 0) {
+        int w = lane < NW ? waveTot[lane] : 0;
+        w = wave_inclusive_scan(w);
+        if (lane < NW) waveTot[lane] = w;
     }
-    int run = t == 0 ? 0 : part[t - 1];
+    __syncthreads();
+    *total = waveTot[NW - 1];
+    return incl - v + (wave > 0 ? waveTot[wave - 1] : 0);
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan(int ncell, int *__restrict__ count, int *__restrict__ start) {
+    __shared__ int waveTot[SCAN_THREADS / WAVE];
+    const int t = threadIdx.x;
+    const int per = (ncell + SCAN_THREADS - 1) / SCAN_THREADS;
+    const int b = min(t * per, ncell), e = min(b + per, ncell);
+    int sum = 0;
+    for (int i = b; i < e; i++) sum += count[i];
+    int total = 0;
+    int run = block_exclusive_scan(sum, waveTot, &total);
     for (int i = b; i < e; i++) {
         const int c = count[i];
         start[i] = run;
         run += c;
         count[i] = 0;  // cleared for the next rebuild, so the counts are zero whenever a rebuild starts
     }
-    if (t == 1023) {
-        start[ncell] = part[1023];
-        if (counters) atomicMax(&counters[CNT_MAX_STENCIL], (unsigned long long)pmax[1023]);
+    if (t == 0) start[ncell] = total;
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_tiles(int ncell, const int *__restrict__ count,
+                                                              int *__restrict__ tileSum) {
+    __shared__ int waveTot[SCAN_THREADS / WAVE];
+    const int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+    int sum = 0;
+    for (int k = 0; k < SCAN_ITEMS; k++) sum += base + k < ncell ? count[base + k] : 0;
+    int total = 0;
+    block_exclusive_scan(sum, waveTot, &total);
+    if (threadIdx.x == 0) tileSum[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_tile_sums(int tiles, int *__restrict__ tileSum) {
+    __shared__ int waveTot[SCAN_THREADS / WAVE];
+    const int t = threadIdx.x;
+    const int v = t < tiles ? tileSum[t] : 0;
+    int total = 0;
+    const int excl = block_exclusive_scan(v, waveTot, &total);
+    if (t < tiles) tileSum[t] = excl;
+    if (t == 0) tileSum[tiles] = total;
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(int ncell, int tiles, int *__restrict__ count,
+                                                              const int *__restrict__ tileSum,
+                                                              int *__restrict__ start) {
+    __shared__ int waveTot[SCAN_THREADS / WAVE];
+    const int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+    int c[SCAN_ITEMS];
+    int sum = 0;
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        c[k] = base + k < ncell ? count[base + k] : 0;
+        sum += c[k];
     }
+    int total = 0;
+    int run = block_exclusive_scan(sum, waveTot, &total) + tileSum[blockIdx.x];
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        if (base + k < ncell) {
+            start[base + k] = run;
+            count[base + k] = 0;
+        }
+        run += c[k];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) start[ncell] = tileSum[tiles];
 }
 
 __global__ __launch_bounds__(256) void k_scatter(int n, const int *__restrict__ cellStart,
@@ -449,46 +510,41 @@ __global__ __launch_bounds__(256) void k_scatter(int n, const int *__restrict__ 
     tmpOrder[cellStart[tmpCell[i]] + tmpSlot[i]] = i;
 }
 
-// One wavefront per cell: rank the cell's members by source index (a stable order), then gather their records
-// into the destination arrays.
-__global__ __launch_bounds__(WAVE) void k_gather(Grid g, int n, Particles src, Particles dst,
-                                                 const double *__restrict__ tmpPos,
-                                                 const uint8_t *__restrict__ tmpW,
-                                                 const int *__restrict__ tmpOrder) {
-    int cell = blockIdx.x;
-    int lane = threadIdx.x;
-    int start = g.cellStart[cell];
-    int m = g.cellStart[cell + 1] - start;
-    if (m == 0) return;
-    for (int base = 0; base < m; base += WAVE) {
-        int e = base + lane;
-        int srcIdx = e < m ? tmpOrder[start + e] : 0x7fffffff;
-        int rank = 0;
-        if (m <= WAVE) {
-            for (int q = 0; q < m; q++) {
-                int other = __shfl(srcIdx, q, WAVE);
-                rank += other < srcIdx ? 1 : 0;
-            }
-        } else if (e < m) {
-            for (int q = 0; q < m; q++) rank += tmpOrder[start + q] < srcIdx ? 1 : 0;
-        }
-        if (e < m) {
-            int d = start + rank;
-            dst.x[d] = tmpPos[srcIdx];
-            dst.y[d] = tmpPos[n + srcIdx];
-            dst.z[d] = tmpPos[2 * n + srcIdx];
-            for (int c = 0; c < 9; c++) dst.o[c][d] = src.o[c][srcIdx];
-            dst.tag[d] = src.tag[srcIdx];
-            dst.wbits[d] = tmpW[srcIdx];
-            dst.cell[d] = cell;
-        }
+// One thread per source record: its rank among its cell's members by source index (a stable order), then its record
+// into the destination arrays at the cell's start plus that rank.  A grid of one cell holds every record in source
+// order.
+__global__ __launch_bounds__(256) void k_gather(Grid g, int n, Particles src, Particles dst,
+                                                const double *__restrict__ tmpPos, const uint8_t *__restrict__ tmpW,
+                                                const int *__restrict__ tmpOrder, const int *__restrict__ tmpCell) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int cell = tmpCell[i];
+    const int start = g.cellStart[cell];
+    const int m = g.cellStart[cell + 1] - start;
+    int rank = 0;
+    if (m == n) {
+        rank = i;
+    } else {
+        for (int q = 0; q < m; q++) rank += tmpOrder[start + q] < i ? 1 : 0;
     }
+    const int d = start + rank;
+    dst.x[d] = tmpPos[i];
+    dst.y[d] = tmpPos[n + i];
+    dst.z[d] = tmpPos[2 * n + i];
+    for (int c = 0; c < 9; c++) dst.o[c][d] = src.o[c][i];
+    dst.tag[d] = src.tag[i];
+    dst.wbits[d] = tmpW[i];
+    dst.cell[d] = cell;
+}
+
+size_t rebuild_scan_ints(int ncell) {
+    return ncell > SCAN_ONE_BLOCK ? (size_t)((ncell + SCAN_TILE - 1) / SCAN_TILE + 1) : 0;
 }
 
 hipError_t launch_rebuild(hipStream_t s, const Grid &g, int n, Particles src, Particles dst, int *tmpSlot,
-                          int *tmpOrder, double *tmpPos, int rescale, const double *oldInv, Timed t) {
-    // tmpSlot holds 2n ints (slot, cell); tmpOrder holds n ints followed by n bytes of wrap bits.
-    // the cell counts are zero here: cleared when allocated and by the scan of every rebuild
+                          int *tmpOrder, double *tmpPos, int rescale, const double *oldInv, Timed t, int *scanTmp) {
+    // tmpSlot holds 2n ints (slot, cell); tmpOrder holds n ints followed by n bytes of wrap bits; scanTmp holds
+    // rebuild_scan_ints(ncell) ints.  The cell counts are zero here: cleared when allocated and by every scan.
     int *tmpCell = tmpSlot + n;
     uint8_t *tmpW = reinterpret_cast<uint8_t *>(tmpOrder + n);
     M3 inv{};
@@ -497,11 +553,19 @@ hipError_t launch_rebuild(hipStream_t s, const Grid &g, int n, Particles src, Pa
     unsigned blocks = (unsigned)((n + 255) / 256);
     hipExtLaunchKernelGGL(k_bin, dim3(blocks), dim3(256), 0, s, t.start, nullptr, 0, g, n, src, rescale, inv, tmpPos,
                           tmpCell, tmpSlot, tmpW);
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, g.ncell, g.cellCount, g.cellStart,
-                       (unsigned long long *)nullptr);
+    if (g.ncell <= SCAN_ONE_BLOCK) {
+        hipLaunchKernelGGL(k_scan, dim3(1), dim3(SCAN_THREADS), 0, s, g.ncell, g.cellCount, g.cellStart);
+    } else {
+        const int tiles = (g.ncell + SCAN_TILE - 1) / SCAN_TILE;
+        if (tiles > SCAN_MAX_TILES || !scanTmp) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(k_scan_tiles, dim3(tiles), dim3(SCAN_THREADS), 0, s, g.ncell, g.cellCount, scanTmp);
+        hipLaunchKernelGGL(k_scan_tile_sums, dim3(1), dim3(SCAN_THREADS), 0, s, tiles, scanTmp);
+        hipLaunchKernelGGL(k_scan_apply, dim3(tiles), dim3(SCAN_THREADS), 0, s, g.ncell, tiles, g.cellCount, scanTmp,
+                           g.cellStart);
+    }
     hipLaunchKernelGGL(k_scatter, dim3(blocks), dim3(256), 0, s, n, g.cellStart, tmpCell, tmpSlot, tmpOrder);
-    hipExtLaunchKernelGGL(k_gather, dim3((unsigned)g.ncell), dim3(WAVE), 0, s, nullptr, t.stop, 0, g, n, src, dst,
-                          tmpPos, tmpW, tmpOrder);
+    hipExtLaunchKernelGGL(k_gather, dim3(blocks), dim3(256), 0, s, nullptr, t.stop, 0, g, n, src, dst, tmpPos, tmpW,
+                          tmpOrder, tmpCell);
     return hipGetLastError();
 }
 
@@ -1048,13 +1112,48 @@ __global__ __launch_bounds__(256) void k_apply_moves(M3 dims, M3 inv, size_t n, 
     outSc[2 * i + 1] = cos(angle[i]);
 }
 
+// Folds the move-count slot of cell i (one lane per cell, every lane of the wavefront calling) into the counters
+// and clears it: wavefront sums, one atomic per word and wavefront.
+__device__ __forceinline__ void fold_slots_wave(const FoldSlots &f, int i) {
+    unsigned long long v[SLOT_WORDS];
+    for (int w = 0; w < SLOT_WORDS; w++) v[w] = 0;
+    if (i < f.cells) {
+        unsigned long long *slot = f.slots + (size_t)i * SLOT_WORDS;
+        for (int w = 0; w < SLOT_WORDS; w++) {
+            v[w] = slot[w];
+            slot[w] = 0;
+        }
+    }
+    for (int w = 0; w < SLOT_WORDS; w++)
+        for (int off = WAVE / 2; off > 0; off >>= 1) v[w] += __shfl_xor(v[w], off);
+    if ((threadIdx.x & (WAVE - 1)) == 0) {
+        const int target[SLOT_WORDS] = {CNT_ACCEPTED, CNT_ATTEMPTED, CNT_LEFT_CELL, CNT_TESTS, CNT_G32_PAIRS};
+        for (int w = 0; w < SLOT_WORDS; w++)
+            if (v[w]) atomicAdd(&f.counters[target[w]], v[w]);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_fold_slots(FoldSlots f) {
+    fold_slots_wave(f, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+hipError_t launch_fold_slots(hipStream_t s, unsigned long long *slots, int ncell, unsigned long long *counters) {
+    if (ncell <= 0) return hipSuccess;
+    FoldSlots f{slots, ncell, counters};
+    hipLaunchKernelGGL(k_fold_slots, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, s, f);
+    return hipGetLastError();
+}
+
 // moves > 0: `moves` proposals per cell at [cell * moves, ...); moves == 0: ms.n per particle, cell c's at
 // [ms.n cellStart[c], ms.n cellStart[c + 1]) (the caller passes the cell of every sorted record in `cellOf`).
 // With several samplers the first draw picks the sampler (Simulation.cpp:456-465); a translation sampler's move
 // has the identity rotation and a rotation sampler's a zero translation, which leave the pose bit-identical.
+// fold.slots: the previous cycle's move-count slots are folded into the counters first (fold_slots_wave; the grid
+// covers fold.cells threads as well).
 __global__ __launch_bounds__(256) void k_proposals(Grid g, Proposals pr, int moves, const int *__restrict__ cellOf,
-                                                   int n, MoveSet ms, uint64_t seed, uint32_t cycle) {
+                                                   int n, MoveSet ms, uint64_t seed, uint32_t cycle, FoldSlots fold) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (fold.slots) fold_slots_wave(fold, i);
     int cell, m;
     if (moves > 0) {
         if (i >= g.ncell * moves) return;
@@ -1109,11 +1208,13 @@ hipError_t launch_apply_moves(hipStream_t s, const double *dims, const double *i
 }
 
 hipError_t launch_proposals(hipStream_t s, const Grid &g, const Proposals &pr, int moves, const int *cellOf, int n,
-                            const MoveSet &ms, uint64_t seed, uint32_t cycle, Timed t) {
-    size_t total = moves > 0 ? (size_t)g.ncell * moves : (size_t)n * ms.n;
-    if (total == 0) return hipSuccess;
-    hipExtLaunchKernelGGL(k_proposals, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, t.start, t.stop, 0, g,
-                          pr, moves, cellOf, n, ms, seed, cycle);
+                            const MoveSet &ms, uint64_t seed, uint32_t cycle, Timed t, FoldSlots fold) {
+    const size_t total = moves > 0 ? (size_t)g.ncell * moves : (size_t)n * ms.n;
+    if (!fold.slots) fold.cells = 0;
+    const size_t threads = std::max(total, (size_t)fold.cells);
+    if (threads == 0) return hipSuccess;
+    hipExtLaunchKernelGGL(k_proposals, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, t.start, t.stop, 0,
+                          g, pr, moves, cellOf, n, ms, seed, cycle, fold);
     return hipGetLastError();
 }
 
@@ -1166,41 +1267,6 @@ __device__ __forceinline__ void flush_counters(const SweepParams &P, int cell, u
     }
 }
 
-// Adds the move-count slots of ncell cells into the counters and clears them (fixed partition of the cells over the
-// threads, tree sums per block, one atomic per word and block).
-constexpr int FOLD_THREADS = 256;
-
-__global__ __launch_bounds__(FOLD_THREADS) void k_fold_slots(unsigned long long *slots, int ncell,
-                                                             unsigned long long *counters) {
-    __shared__ unsigned long long part[SLOT_WORDS][FOLD_THREADS];
-    unsigned long long v[SLOT_WORDS] = {};
-    for (int c = blockIdx.x * FOLD_THREADS + threadIdx.x; c < ncell; c += gridDim.x * FOLD_THREADS) {
-        unsigned long long *slot = slots + (size_t)c * SLOT_WORDS;
-        for (int w = 0; w < SLOT_WORDS; w++) {
-            v[w] += slot[w];
-            slot[w] = 0;
-        }
-    }
-    for (int w = 0; w < SLOT_WORDS; w++) part[w][threadIdx.x] = v[w];
-    __syncthreads();
-    for (int off = FOLD_THREADS / 2; off > 0; off >>= 1) {
-        if ((int)threadIdx.x < off)
-            for (int w = 0; w < SLOT_WORDS; w++) part[w][threadIdx.x] += part[w][threadIdx.x + off];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        const int target[SLOT_WORDS] = {CNT_ACCEPTED, CNT_ATTEMPTED, CNT_LEFT_CELL, CNT_TESTS, CNT_G32_PAIRS};
-        for (int w = 0; w < SLOT_WORDS; w++)
-            if (part[w][0]) atomicAdd(&counters[target[w]], part[w][0]);
-    }
-}
-
-hipError_t launch_fold_slots(hipStream_t s, unsigned long long *slots, int ncell, unsigned long long *counters) {
-    if (ncell <= 0) return hipSuccess;
-    const unsigned blocks = (unsigned)std::min(64, (ncell + FOLD_THREADS - 1) / FOLD_THREADS);
-    hipLaunchKernelGGL(k_fold_slots, dim3(blocks), dim3(FOLD_THREADS), 0, s, slots, ncell, counters);
-    return hipGetLastError();
-}
 
 // Copies proposals [first, first + cn) of this cell into LDS.
 __device__ __forceinline__ void load_proposals(const SweepParams &P, int pbase, int first, int cn, double *pt,

@@ -73,7 +73,7 @@ extern dim3 blockIdxV, blockDimV, gridDimV;
 unsigned lane_tid();
 void *dyn_lds();
 
-enum WaveKind { W_BALLOT, W_SHFL, W_SHFL_DOWN, W_SHFL_XOR, W_READLANE, W_READFIRST, W_BARRIER };
+enum WaveKind { W_BALLOT, W_SHFL, W_SHFL_DOWN, W_SHFL_XOR, W_READLANE, W_READFIRST, W_BARRIER, W_SHFL_UP };
 uint64_t wave_op(int kind, uint64_t bits, int arg, int line);
 int block_op(int orValue, int isOr, int line);
 
@@ -131,6 +131,9 @@ EmuThreadIdx emu_thread_idx();
 #define EMU_DOWN2(v, d) emu::wave_value(emu::W_SHFL_DOWN, (v), (int)(d), __LINE__)
 #define EMU_DOWN3(v, d, w) emu::wave_value(emu::W_SHFL_DOWN, (v), (int)(d), __LINE__)
 #define __shfl_down(...) EMU_SHFL_PICK(__VA_ARGS__, EMU_DOWN3, EMU_DOWN2, )(__VA_ARGS__)
+#define EMU_UP2(v, d) emu::wave_value(emu::W_SHFL_UP, (v), (int)(d), __LINE__)
+#define EMU_UP3(v, d, w) emu::wave_value(emu::W_SHFL_UP, (v), (int)(d), __LINE__)
+#define __shfl_up(...) EMU_SHFL_PICK(__VA_ARGS__, EMU_UP3, EMU_UP2, )(__VA_ARGS__)
 #define EMU_XOR2(v, m) emu::wave_value(emu::W_SHFL_XOR, (v), (int)(m), __LINE__)
 #define EMU_XOR3(v, m, w) emu::wave_value(emu::W_SHFL_XOR, (v), (int)(m), __LINE__)
 #define __shfl_xor(...) EMU_SHFL_PICK(__VA_ARGS__, EMU_XOR3, EMU_XOR2, )(__VA_ARGS__)

@@ -141,6 +141,7 @@ void resolve_wave_group(Lane *const *g, int n) {
             case W_BALLOT: l.result = mask; break;
             case W_SHFL: src = l.arg & 63; break;
             case W_SHFL_DOWN: src = (int)me + l.arg < 64 ? (int)me + l.arg : (int)me; break;
+            case W_SHFL_UP: src = (int)me - l.arg >= 0 ? (int)me - l.arg : (int)me; break;
             case W_SHFL_XOR: src = (int)(me ^ (unsigned)l.arg) & 63; break;
             case W_READLANE: src = l.arg & 63; break;
             case W_READFIRST: src = (int)(g[0]->tid & 63); break;

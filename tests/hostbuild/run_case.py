@@ -4,6 +4,7 @@ AddressSanitizer runtime preloaded; any access outside an allocation aborts the 
     python tests/hostbuild/run_case.py LIB replay WAVES PIPE     # decision records of one launch shape vs the oracle
     python tests/hostbuild/run_case.py LIB replica I CYCLES      # replica I of the 32-replica scan, overlap checks
     python tests/hostbuild/run_case.py LIB nogrid NAME CYCLES    # a packing without a grid, overlap checks, replays
+    python tests/hostbuild/run_case.py LIB biggrid CYCLES        # grids larger than one scan workgroup
 """
 import os
 import sys
@@ -92,11 +93,37 @@ def nogrid(name: str, cycles: int) -> None:
     pk.close()
 
 
+def big_grid(cycles: int) -> None:
+    """A dilute packing whose grids have more cells than one workgroup scans (1728 spheres at packing fraction
+    0.048 in a box 26.6 wide: 26^3 cells of the interaction range): the cell list is rebuilt through the tiled
+    scan; the records come back unchanged, the overlap count is 0, and after a few sweep cycles the state is
+    overlap-free and its last sweep replays."""
+    from test_gpu_replay import check_record
+    hs = ra.Shape("sphere", radius=0.5)
+    box, pos, rot = lattice.hard_sphere_lattice(12, 0.048)
+    pk = ra.GpuPacking(hs, box, pos, rot, seed=3)
+    b, q, r = pk.download()
+    assert np.array_equal(q, pos) and np.array_equal(r, rot)
+    assert pk.count_total_overlaps() == 0
+    pk.set_environment(1.0, 1.0, 0.3, 0.2, box_move=("log", 0.001))
+    c = pk.run_cycles(cycles)
+    t = pk.timing()
+    b, q, r = pk.download()
+    overl = oracle_lib.oracle().snapshot_stats(0, 0.5, 0.0, b, q, r)[0]
+    print(f"big grid: N={len(pos)} moves {c['moves_accepted']}/{c['moves_attempted']}, box {c['box_accepted']}/"
+          f"{c['box_attempted']}, overlaps {overl}", flush=True)
+    assert overl == 0 and c["moves_attempted"] == cycles * len(pos)
+    check_record(hs, pk.record_sweep(), rejections=False)
+    pk.close()
+
+
 if __name__ == "__main__":
     if sys.argv[2] == "replay":
         replay(sys.argv[3], sys.argv[4])
     elif sys.argv[2] == "nogrid":
         nogrid(sys.argv[3], int(sys.argv[4]))
+    elif sys.argv[2] == "biggrid":
+        big_grid(int(sys.argv[3]))
     else:
         replica(int(sys.argv[3]), int(sys.argv[4]))
     print("case passed", flush=True)
