@@ -97,6 +97,7 @@ struct rp_ctx {
     size_t scanTmpCap = 0;
     size_t slotsCap = 0;                         // cells the slots cover
     int slotsDirty = 0;                          // cells of the grid whose slots hold counts not yet folded (0: none)
+    int scanHint = 0;                            // a cell where the last overlap count found an overlap
     uint64_t fixEvery = 10000;           // cycles between orientation renormalisations (0: never)
     uint64_t fixRejected = 0;            // renormalisations skipped because they would create an overlap
     bool counting = false;               // overlap-counting mode (Packing::toggleOverlapCounting)
@@ -629,8 +630,10 @@ int scanLaunch(rp_ctx *c, const Grid &g, const Particles &items, size_t nItems, 
     if (g.allPairs)
         RP_HIP(launch_count_allpairs(c->stream, g, (int)nItems, items, c->k, earlyExit, c->dCounters));
     else
+        // an early-exit count starts a few hundred cells before the cell of the last overlap found: a box move that
+        // is rejected for an overlap is most often followed by one that meets the same near pair
         RP_HIP(launch_count_overlaps(c->stream, g, (int)nItems, items, c->k, earlyExit, c->dCounters, cap,
-                                     timingSlot(c, 2)));
+                                     timingSlot(c, 2), earlyExit ? c->scanHint - 300 : 0));
     RP_HIP(hipMemcpyAsync(c->hCounters, c->dCounters, sizeof(unsigned long long) * CNT_NUM, hipMemcpyDeviceToHost,
                           c->stream));
     return RP_OK;
@@ -642,6 +645,7 @@ int scanResult(rp_ctx *c, uint64_t &count) {
     if (c->hCounters[CNT_STAGE_OVERFLOW])
         return fail(RP_ERR_UNSUPPORTED, "a grid cell holds more particles than the overlap count can stage");
     count = c->hCounters[CNT_OVERLAPS];
+    if (count) c->scanHint = (int)c->hCounters[CNT_SCAN_HINT];
     return RP_OK;
 }
 

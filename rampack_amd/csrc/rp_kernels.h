@@ -129,6 +129,7 @@ enum Counter : int {
     CNT_G32_GUARD = 18,       // of those, pairs inside a guard band, decided by the FP64 distance
     CNT_G32_MAXERR = 19,      // verify mode: largest |FP32 - FP64| squared centre distance (float bits)
     CNT_G32_MISMATCH = 20,    // verify mode: pairs whose guarded class differs from the FP64 class (0)
+    CNT_SCAN_HINT = 21,       // a cell in which the last overlap count found an overlap
     CNT_NUM = 24
 };
 
@@ -189,8 +190,9 @@ hipError_t launch_rebuild(hipStream_t s, const Grid &g, int n, Particles src, Pa
 size_t rebuild_scan_ints(int ncell);
 // Half-shell overlap count over the grid (Packing.cpp:431-559).  Adds to counters[CNT_OVERLAPS]; with
 // earlyExit the workgroups stop once any overlap is found.  cap: LDS entries per workgroup (>= largest cell).
+// startCell: the first cell counted (the cells follow in index order, wrapping around).
 hipError_t launch_count_overlaps(hipStream_t s, const Grid &g, int n, Particles p, PairConsts k, int earlyExit,
-                                 unsigned long long *counters, int cap, Timed t = {});
+                                 unsigned long long *counters, int cap, Timed t = {}, int startCell = 0);
 size_t count_lds_bytes(int kind, int cap);
 // Every pair i < j of the n particles (a grid with allPairs set) through the nearest image, the particle with the lower
 // caller index the first operand (Packing::countTotalOverlaps without a grid, Packing.cpp:547-556); adds to

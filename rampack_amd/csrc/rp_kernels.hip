@@ -719,12 +719,18 @@ size_t stencil_lds_bytes(int kind, int cap) {
 
 constexpr int COUNT_THREADS = 256;
 
+// The early-exit flag of an overlap count (counters[CNT_OVERLAPS] != 0) read at device scope: each XCD has its own
+// L2, and a plain load can keep returning a line cached there before another workgroup counted an overlap (the
+// count itself is a device-scope atomic), so workgroups elsewhere would not stop.
+__device__ __forceinline__ bool overlap_found(unsigned long long *counters) {
+    return __hip_atomic_load(counters + CNT_OVERLAPS, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+}
+
 template <int KIND, bool STAGED>
 __device__ void count_cell(const Grid &g, const Particles &p, const PairConsts &k, const Stencil &S, const double *sT,
                            const int *sTag, int nOwn, int total, int earlyExit, unsigned long long *counters,
                            int *sFound) {
     const int tid = threadIdx.x;
-    volatile unsigned long long *flag = counters + CNT_OVERLAPS;
     unsigned long long found = 0;
     for (int a = 0; a < nOwn; a++) {
         V3 pos1, ax1;
@@ -742,7 +748,7 @@ __device__ void count_cell(const Grid &g, const Particles &p, const PairConsts &
             if (pair_overlap<KIND>(k, f, p2, a2, T)) found++;
         }
         if (earlyExit) {
-            int any = __syncthreads_or(found != 0 || *flag != 0);
+            int any = __syncthreads_or(found != 0 || overlap_found(counters));
             if (any) break;
         }
     }
@@ -766,7 +772,6 @@ __device__ void count_cell_flat(const PairConsts &k, const Stencil &S, const dou
                                 int *queue, int *sQueued, const double *sBeadR) {
     const int tid = threadIdx.x;
     const int lane = tid & (WAVE - 1);
-    volatile unsigned long long *flag = counters + CNT_OVERLAPS;
     // first operands: position, cap A, u, a (10 doubles) per own particle
     for (int a = tid; a < nOwn; a += COUNT_THREADS) {
         const V3 pos1 = v3(S.x[a], S.y[a], S.z[a]);
@@ -843,7 +848,7 @@ __device__ void count_cell_flat(const PairConsts &k, const Stencil &S, const dou
                     found++;
             }
         }
-        const int stop = earlyExit ? __syncthreads_or(found != 0 || *flag != 0) : (__syncthreads(), 0);
+        const int stop = earlyExit ? __syncthreads_or(found != 0 || overlap_found(counters)) : (__syncthreads(), 0);
         if (tid == 0) *sQueued = 0;
         __syncthreads();
         if (stop) break;
@@ -853,20 +858,21 @@ __device__ void count_cell_flat(const PairConsts &k, const Stencil &S, const dou
     if (tid == 0 && *sFound) atomicAdd(&counters[CNT_OVERLAPS], (unsigned long long)*sFound);
 }
 
+// rot: workgroup b counts cell (b + rot) mod ncell, so that an early-exit count starts where the previous one found
+// an overlap (counters[CNT_SCAN_HINT], written by a workgroup that finds one); the count does not depend on it.
 template <int KIND>
 __global__ __launch_bounds__(COUNT_THREADS) void k_count_overlaps(Grid g, Particles p, PairConsts k, int earlyExit,
-                                                                  unsigned long long *counters, int cap) {
+                                                                  unsigned long long *counters, int cap, int rot) {
     extern __shared__ double smem[];
     __shared__ int s_start[MAX_HALF + 1], s_base[MAX_HALF], s_ecode[MAX_HALF];
     __shared__ int sFound;
     __shared__ double sT[81];
     const int tid = threadIdx.x;
-    const int cell = blockIdx.x;
+    const int cell = (int)((blockIdx.x + (unsigned)rot) % (unsigned)g.ncell);
     const int own0 = g.cellStart[cell];
     const int nOwn = g.cellStart[cell + 1] - own0;
     if (nOwn == 0) return;
-    volatile unsigned long long *flag = counters + CNT_OVERLAPS;
-    if (earlyExit && *flag != 0) return;
+    if (earlyExit && overlap_found(counters)) return;
     const int cx = cell % g.n[0], cy = (cell / g.n[0]) % g.n[1], cz = cell / (g.n[0] * g.n[1]);
     const int ncells = stencil_cells(g.reach, true);
     for (int i = tid; i < 81; i += COUNT_THREADS) sT[i] = g.T[i];
@@ -907,6 +913,7 @@ __global__ __launch_bounds__(COUNT_THREADS) void k_count_overlaps(Grid g, Partic
         // a cell holding more than `cap` particles: the host sizes cap from the largest cell, so this marks an error
         atomicAdd(&counters[CNT_STAGE_OVERFLOW], 1ull);
     }
+    if (tid == 0 && sFound) counters[CNT_SCAN_HINT] = (unsigned long long)cell;
 }
 
 size_t count_lds_bytes(int kind, int cap) {
@@ -915,18 +922,19 @@ size_t count_lds_bytes(int kind, int cap) {
 }
 
 hipError_t launch_count_overlaps(hipStream_t s, const Grid &g, int n, Particles p, PairConsts k, int earlyExit,
-                                 unsigned long long *counters, int cap, Timed t) {
+                                 unsigned long long *counters, int cap, Timed t, int startCell) {
     if (n == 0) return hipSuccess;
     size_t lds = count_lds_bytes(k.kind, cap);
+    const int rot = g.ncell > 0 ? ((startCell % g.ncell) + g.ncell) % g.ncell : 0;
     if (k.kind == 0)
         hipExtLaunchKernelGGL(k_count_overlaps<0>, dim3((unsigned)g.ncell), dim3(COUNT_THREADS), (uint32_t)lds, s,
-                              t.start, t.stop, 0, g, p, k, earlyExit, counters, cap);
+                              t.start, t.stop, 0, g, p, k, earlyExit, counters, cap, rot);
     else if (k.kind == 1)
         hipExtLaunchKernelGGL(k_count_overlaps<1>, dim3((unsigned)g.ncell), dim3(COUNT_THREADS), (uint32_t)lds, s,
-                              t.start, t.stop, 0, g, p, k, earlyExit, counters, cap);
+                              t.start, t.stop, 0, g, p, k, earlyExit, counters, cap, rot);
     else
         hipExtLaunchKernelGGL(k_count_overlaps<2>, dim3((unsigned)g.ncell), dim3(COUNT_THREADS), (uint32_t)lds, s,
-                              t.start, t.stop, 0, g, p, k, earlyExit, counters, cap);
+                              t.start, t.stop, 0, g, p, k, earlyExit, counters, cap, rot);
     return hipGetLastError();
 }
 
@@ -2856,7 +2864,9 @@ hipError_t launch_fix_all(hipStream_t s, const Particles &p, int n) {
 // Packings without a neighbour grid: every pair through the nearest image (PeriodicBoundaryConditions::getTranslation
 // for spherocylinders, getDistance2 for spheres; SpherocylinderTraits.cpp:28-46, SphereTraits.cpp:56-65)
 
-constexpr int ALLPAIRS_THREADS = 256;
+constexpr int ALLPAIRS_THREADS = 256;         // overlap count: a thread per particle
+constexpr int ALLPAIRS_SWEEP_THREADS = 1024;  // sweep: one workgroup, the candidates of a move spread over its lanes
+constexpr int ALLPAIRS_CHUNK = 256;           // sweep: proposals staged in LDS at a time
 
 template <int KIND>
 __device__ __forceinline__ bool allpairs_hit(const PairConsts &k, const Grid &g, const SCFirst &f, const V3 &p2,
@@ -2866,7 +2876,7 @@ __device__ __forceinline__ bool allpairs_hit(const PairConsts &k, const Grid &g,
 }
 
 size_t allpairs_lds_bytes(int n) {
-    return sizeof(double) * 12 * (size_t)n + (sizeof(double) * 12 + sizeof(int)) * ALLPAIRS_THREADS;
+    return sizeof(double) * 12 * (size_t)n + (sizeof(double) * 12 + sizeof(int)) * ALLPAIRS_CHUNK;
 }
 
 // The cycle's moves of a packing without a grid, in order, by one workgroup (Simulation::performMoves without domains,
@@ -2875,12 +2885,12 @@ size_t allpairs_lds_bytes(int n) {
 // fixMode: the orientation renormalisation (Packing::renormalizeOrientations), one trial per particle in storage
 // order, fix_rotation(O) in place.
 // LDS: the packing (positions and the nine orientation components, P.cap particles at most) and the proposals,
-// ALLPAIRS_THREADS moves at a time, are staged in LDS and every move reads LDS only; accepted poses are written to
+// ALLPAIRS_CHUNK moves at a time, are staged in LDS and every move reads LDS only; accepted poses are written to
 // both.  Otherwise every read goes to global memory (packings too large for LDS).
 template <int KIND, bool LDS>
-__global__ __launch_bounds__(ALLPAIRS_THREADS) void k_sweep_allpairs(SweepParams P) {
+__global__ __launch_bounds__(ALLPAIRS_SWEEP_THREADS) void k_sweep_allpairs(SweepParams P) {
     extern __shared__ double smem[];
-    constexpr int T = ALLPAIRS_THREADS;
+    constexpr int T = ALLPAIRS_SWEEP_THREADS, C = ALLPAIRS_CHUNK;
     const Grid &g = P.g;
     const int cell = 0;  // the one cell of a packing without a grid
     const int n = g.cellStart[1];
@@ -2899,8 +2909,8 @@ __global__ __launch_bounds__(ALLPAIRS_THREADS) void k_sweep_allpairs(SweepParams
         sz = sy + n;
         so = sz + n;  // component c of particle j at so[c * n + j]
         ct = so + 9 * n;
-        cR = ct + 3 * T;
-        ck = (int *)(cR + 9 * T);
+        cR = ct + 3 * C;
+        ck = (int *)(cR + 9 * C);
         for (int j = tid; j < n; j += T) {
             sx[j] = P.p.x[j];
             sy[j] = P.p.y[j];
@@ -2911,15 +2921,15 @@ __global__ __launch_bounds__(ALLPAIRS_THREADS) void k_sweep_allpairs(SweepParams
     auto pos_of = [&](int j) { return LDS ? v3(sx[j], sy[j], sz[j]) : load3(P.p.x, P.p.y, P.p.z, j); };
     auto o_of = [&](int c, int j) { return LDS ? so[c * n + j] : P.p.o[c][j]; };
     unsigned long long accepted = 0, tests = 0;
-    for (int base = 0; base < moves; base += T) {
-        const int cn = min(T, moves - base);
+    for (int base = 0; base < moves; base += C) {
+        const int cn = min(C, moves - base);
         if (LDS) {
             __syncthreads();  // the packing staged; every thread done with the previous chunk's proposals
             if (!P.fixMode && tid < cn) {
                 const int i = base + tid;
                 ck[tid] = P.pr.k[i];
-                for (int d = 0; d < 3; d++) ct[d * T + tid] = P.pr.t[d * P.pr.stride + i];
-                for (int c = 0; c < 9; c++) cR[c * T + tid] = P.pr.R[c * P.pr.stride + i];
+                for (int d = 0; d < 3; d++) ct[d * C + tid] = P.pr.t[d * P.pr.stride + i];
+                for (int c = 0; c < 9; c++) cR[c * C + tid] = P.pr.R[c * P.pr.stride + i];
             }
             __syncthreads();
         }
@@ -2934,10 +2944,10 @@ __global__ __launch_bounds__(ALLPAIRS_THREADS) void k_sweep_allpairs(SweepParams
                 for (int c = 0; c < 9; c++) NO[c] = O[c];
                 fix_rotation(NO);
             } else {
-                const V3 t = LDS ? v3(ct[mm], ct[T + mm], ct[2 * T + mm])
+                const V3 t = LDS ? v3(ct[mm], ct[C + mm], ct[2 * C + mm])
                                  : v3(P.pr.t[m], P.pr.t[P.pr.stride + m], P.pr.t[2 * P.pr.stride + m]);
                 npos = translate_wrapped(g.dims, g.inv, pos, t);
-                for (int c = 0; c < 9; c++) R[c] = LDS ? cR[c * T + mm] : P.pr.R[c * P.pr.stride + m];
+                for (int c = 0; c < 9; c++) R[c] = LDS ? cR[c * C + mm] : P.pr.R[c * P.pr.stride + m];
                 mulm(R, O, NO);
             }
             const SCFirst f = make_first<KIND>(P.k, npos, v3(NO[2], NO[5], NO[8]));
@@ -3032,7 +3042,8 @@ hipError_t launch_sweep(hipStream_t s, const SweepParams &sp, int waves, Timed t
         // sp.cap > 0: the packing (at most sp.cap particles) and the proposals staged in LDS
         const uint32_t lds = sp.cap > 0 ? (uint32_t)allpairs_lds_bytes(sp.cap) : 0;
 #define RP_ALLPAIRS(KIND, STAGED) \
-    hipExtLaunchKernelGGL((k_sweep_allpairs<KIND, STAGED>), dim3(1), dim3(ALLPAIRS_THREADS), lds, s, t.start, t.stop, 0, sp)
+    hipExtLaunchKernelGGL((k_sweep_allpairs<KIND, STAGED>), dim3(1), dim3(ALLPAIRS_SWEEP_THREADS), lds, s, t.start, t.stop, \
+                          0, sp)
         if (sp.k.kind == 0 && sp.cap > 0) RP_ALLPAIRS(0, true);
         else if (sp.k.kind == 0) RP_ALLPAIRS(0, false);
         else if (sp.cap > 0) RP_ALLPAIRS(1, true);

@@ -160,6 +160,13 @@ inline unsigned __float_as_uint(float f) { return emu::from_bits<unsigned>(emu::
 unsigned long long wall_clock64();
 inline unsigned __smid() { return 0; }
 
+#ifndef __HIP_MEMORY_SCOPE_AGENT
+#define __HIP_MEMORY_SCOPE_AGENT 3
+#endif
+template <class T>
+inline T __hip_atomic_load(const T *p, int, int) {
+    return *p;
+}
 template <class T, class U>
 inline T atomicAdd(T *p, U v) {
     T o = *p;
