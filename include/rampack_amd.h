@@ -336,6 +336,23 @@ int rp_get_observables(rp_ctx *ctx, rp_observables *out);
 int rp_set_timing(rp_ctx *ctx, int enable);
 int rp_get_timing(rp_ctx *ctx, rp_timing *out);
 
+/* Launch-shape autotuner (no counterpart in the reference, whose domain decomposition has no launch shape): runs
+ * cycles_per_shape cycles (no step adaptation) with each candidate sweep launch -- wavefronts per active cell and
+ * the move schedule (0 plain, 1 planner/tester pipeline, 2 grouped moves) -- timing every sweep launch, and keeps
+ * the fastest for the cycles that follow.  Every launch shape makes the same decisions, so the call is
+ * rp_run_cycles(ctx, candidates x cycles_per_shape, 0, NULL) bit for bit, whatever shape it keeps.  Candidates: hard
+ * spheres and spherocylinders all six tested shapes (the guarded FP32 precision only its grouped eight-wavefront
+ * launch), polyspheres 1, 2 or 4 wavefronts; soft and overlap-counting sweeps keep their shape.  out (nullable,
+ * RP_AUTOTUNE_MAX entries) receives each candidate and its mean sweep time per cycle, *n_out their number and
+ * *chosen (nullable) the index kept. */
+#define RP_AUTOTUNE_MAX 8
+typedef struct rp_autotune_entry {
+    int waves;
+    int schedule;
+    double sweep_us_per_cycle;
+} rp_autotune_entry;
+int rp_autotune(rp_ctx *ctx, size_t cycles_per_shape, rp_autotune_entry *out, size_t *n_out, size_t *chosen);
+
 void rp_destroy(rp_ctx *ctx);
 
 /* Batched pair decisions on the device: SpherocylinderTraits::overlapBetween (SpherocylinderTraits.cpp:28-46) or

@@ -59,7 +59,7 @@ EXPORTED_SYMBOLS = (
     "rp_ramtrj_jump_to_snapshot", "rp_ramtrj_player_close", "rp_pair_decisions_guarded",
     "rp_pair_decisions_guarded_noalloc",
     "rp_try_scaling", "rp_revert_scaling", "rp_set_move_samplers", "rp_get_move_samplers", "rp_run_replicas",
-    "rp_record_sweep", "rp_box_info", "rp_apply_moves", "rp_set_precision", "rp_get_guard_stats",
+    "rp_record_sweep", "rp_box_info", "rp_apply_moves", "rp_set_precision", "rp_get_guard_stats", "rp_autotune",
     "rp_mlp_gradient", "rp_mlp_infer", "rp_mlp_train", "rp_mlp_order_stats", "rp_mlp_get_weights",
     "rp_mlp_set_weights", "rp_mlp_pairs",
 )
@@ -102,6 +102,14 @@ class Timing(ctypes.Structure):
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class AutotuneEntry(ctypes.Structure):
+    _fields_ = [("waves", ctypes.c_int), ("schedule", ctypes.c_int), ("sweep_us_per_cycle", ctypes.c_double)]
+
+
+AUTOTUNE_MAX = 8
+SCHEDULES = ("plain", "pipelined", "grouped")
 
 
 class Observables(ctypes.Structure):
@@ -195,6 +203,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.rp_download.argtypes = [vp, dp, dp, dp]
     lib.rp_set_timing.argtypes = [vp, ctypes.c_int]
     lib.rp_get_timing.argtypes = [vp, ctypes.POINTER(Timing)]
+    lib.rp_autotune.argtypes = [vp, ctypes.c_size_t, ctypes.POINTER(AutotuneEntry), ctypes.POINTER(ctypes.c_size_t),
+                                ctypes.POINTER(ctypes.c_size_t)]
     lib.rp_destroy.argtypes = [vp]
     lib.rp_destroy.restype = None
     lib.rp_pair_decisions.argtypes = [ctypes.POINTER(ShapeDesc), ctypes.c_size_t, dp,
@@ -616,6 +626,17 @@ class GpuPacking:
         """True: event timing of every sweep, rebuild, scan and proposal launch; "sampled": one sweep launch per cycle
         (rotating through the colours); False: off."""
         _check(_lib.rp_set_timing(self._ctx, 2 if enable == "sampled" else (1 if enable else 0)))
+
+    def autotune(self, cycles_per_shape: int = 10) -> dict:
+        """rp_autotune: runs cycles_per_shape cycles (no step adaptation) with each candidate sweep launch shape and
+        keeps the fastest; the run goes on exactly as it would have without tuning.  Returns {"candidates": [{waves,
+        schedule, sweep_us_per_cycle}], "chosen": index}."""
+        out = (AutotuneEntry * AUTOTUNE_MAX)()
+        n, chosen = ctypes.c_size_t(), ctypes.c_size_t()
+        _check(_lib.rp_autotune(self._ctx, cycles_per_shape, out, ctypes.byref(n), ctypes.byref(chosen)))
+        cands = [{"waves": out[i].waves, "schedule": SCHEDULES[out[i].schedule] if 0 <= out[i].schedule < 3
+                  else "default", "sweep_us_per_cycle": out[i].sweep_us_per_cycle} for i in range(n.value)]
+        return {"candidates": cands, "chosen": chosen.value}
 
     def timing(self) -> dict:
         t = Timing()

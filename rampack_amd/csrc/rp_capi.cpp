@@ -1809,13 +1809,10 @@ int rp_get_move_samplers(rp_ctx *c, rp_move_desc *out, size_t n, size_t *nMoves)
     return RP_OK;
 }
 
-int rp_run_cycles(rp_ctx *c, size_t nCycles, int adapt, rp_counters *out) {
-    if (!c) return fail(RP_ERR_PRECONDITION, "null context");
-    if (!c->envSet) return fail(RP_ERR_PRECONDITION, "environment not set");
-    int r = validateStart(c);
-    if (r) return r;
+// nCycles cycles of a validated packing (the body of rp_run_cycles after its start check).
+static int cycleLoop(rp_ctx *c, size_t nCycles, bool adapt) {
     for (size_t i = 0; i < nCycles; i++) {
-        r = runCycle(c);
+        int r = runCycle(c);
         if (r) return r;
         r = afterCycle(c, adapt);
         if (r) return r;
@@ -1824,7 +1821,15 @@ int rp_run_cycles(rp_ctx *c, size_t nCycles, int adapt, rp_counters *out) {
             if (r) return r;
         }
     }
-    r = timingHarvest(c);
+    return timingHarvest(c);
+}
+
+int rp_run_cycles(rp_ctx *c, size_t nCycles, int adapt, rp_counters *out) {
+    if (!c) return fail(RP_ERR_PRECONDITION, "null context");
+    if (!c->envSet) return fail(RP_ERR_PRECONDITION, "environment not set");
+    int r = validateStart(c);
+    if (r) return r;
+    r = cycleLoop(c, nCycles, adapt != 0);
     if (r) return r;
     return reportCounters(c, out);
 }
@@ -2030,6 +2035,55 @@ int rp_set_timing(rp_ctx *c, int enable) {
     c->tstats.sweep_cap = keep.sweep_cap;
     c->tstats.sweep_waves = keep.sweep_waves;
     c->tstats.sweep_period = keep.sweep_period;
+    return RP_OK;
+}
+
+int rp_autotune(rp_ctx *c, size_t cycles, rp_autotune_entry *out, size_t *nOut, size_t *chosen) {
+    if (!c || !nOut) return fail(RP_ERR_PRECONDITION, "null argument");
+    if (cycles == 0) return fail(RP_ERR_PRECONDITION, "cycles_per_shape must be positive");
+    if (!c->envSet) return fail(RP_ERR_PRECONDITION, "environment not set");
+    struct Shape {
+        int waves, schedule;
+    };
+    std::vector<Shape> cands;
+    const bool hard = !c->k.soft && !c->counting;
+    if (hard && c->k.kind == RP_SHAPE_SPHEROCYLINDER && c->precision != RP_PREC_FP64) cands = {{8, 2}};
+    else if (hard && c->k.kind != RP_SHAPE_POLYSPHERE) cands = {{8, 2}, {4, 2}, {2, 2}, {4, 1}, {2, 1}, {1, 0}};
+    else if (hard) cands = {{4, 1}, {2, 1}, {1, 0}};
+    else cands = {{c->sweepWaves, c->pipelined}};
+    int r = timingHarvest(c);
+    if (r) return r;
+    // one start check, then the candidates' cycles back to back: the same cycles as rp_run_cycles over all of them
+    r = validateStart(c);
+    if (r) return r;
+    const int keepTiming = c->timing;
+    const rp_timing keepStats = c->tstats;
+    double best = 0;
+    size_t bestI = 0;
+    for (size_t i = 0; i < cands.size(); i++) {
+        c->sweepWaves = cands[i].waves;
+        c->pipelined = cands[i].schedule;
+        c->timing = 1;  // every launch timed
+        c->tstats = rp_timing{};
+        r = cycleLoop(c, cycles, false);
+        if (r) {
+            c->timing = keepTiming;
+            c->tstats = keepStats;
+            return r;
+        }
+        const double us = c->tstats.sweep_ms * 1e3 / (double)cycles;
+        if (out && i < RP_AUTOTUNE_MAX) out[i] = rp_autotune_entry{cands[i].waves, cands[i].schedule, us};
+        if (i == 0 || us < best) {
+            best = us;
+            bestI = i;
+        }
+    }
+    c->sweepWaves = cands[bestI].waves;
+    c->pipelined = cands[bestI].schedule;
+    c->timing = keepTiming;
+    c->tstats = keepStats;
+    *nOut = cands.size();
+    if (chosen) *chosen = bestI;
     return RP_OK;
 }
 

@@ -5,6 +5,7 @@ AddressSanitizer runtime preloaded; any access outside an allocation aborts the 
     python tests/hostbuild/run_case.py LIB replica I CYCLES      # replica I of the 32-replica scan, overlap checks
     python tests/hostbuild/run_case.py LIB nogrid NAME CYCLES    # a packing without a grid, overlap checks, replays
     python tests/hostbuild/run_case.py LIB biggrid CYCLES        # grids larger than one scan workgroup
+    python tests/hostbuild/run_case.py LIB autotune CYCLES       # the launch-shape autotuner keeps the trajectory
 """
 import os
 import sys
@@ -117,6 +118,28 @@ def big_grid(cycles: int) -> None:
     pk.close()
 
 
+def autotune(cycles: int) -> None:
+    """The autotuner's six launch shapes on rods: tuning over k cycles per shape, then two cycles, ends bit for bit
+    where 6 k cycles and then two cycles without tuning end."""
+    from test_gpu_replay import rod_start
+    box, pos, rot = rod_start()
+    states = []
+    for tune in (True, False):
+        pk = ra.GpuPacking(SC, box, pos, rot, seed=9)
+        pk.set_environment(1.0, 2.0, 0.2, None, box_move=("delta_triclinic", 0.002))
+        if tune:
+            res = pk.autotune(cycles)
+            assert len(res["candidates"]) == 6
+            pk.run_cycles(2)
+        else:
+            pk.run_cycles(6 * cycles)
+            pk.run_cycles(2)
+        states.append(pk.download())
+        pk.close()
+    assert all(np.array_equal(a, b) for a, b in zip(*states))
+    print("autotune: tuned and untuned runs end in the same state", flush=True)
+
+
 if __name__ == "__main__":
     if sys.argv[2] == "replay":
         replay(sys.argv[3], sys.argv[4])
@@ -124,6 +147,8 @@ if __name__ == "__main__":
         nogrid(sys.argv[3], int(sys.argv[4]))
     elif sys.argv[2] == "biggrid":
         big_grid(int(sys.argv[3]))
+    elif sys.argv[2] == "autotune":
+        autotune(int(sys.argv[3]))
     else:
         replica(int(sys.argv[3]), int(sys.argv[4]))
     print("case passed", flush=True)
