@@ -3053,7 +3053,7 @@ hipError_t launch_sweep(hipStream_t s, const SweepParams &sp, int waves, Timed t
     }
     unsigned blocks = (unsigned)(sp.na[0] * sp.na[1] * sp.na[2]);
     int w = waves >= 8 ? 8 : (waves >= 4 ? 4 : (waves == 2 ? 2 : 1));
-    if (w == 8 && (sp.k.soft || sp.counting || sp.k.kind == 2)) w = 4;  // eight only for the staged hard sweep
+    if (w == 8 && (sp.k.soft || sp.counting)) w = 4;  // eight only for the hard sweeps
     // the lean kernels run spherocylinders only: their first launch on spheres (eight wavefronts forced onto sparse
     // stencils) faulted the GPU and the cause is not found, so spheres keep the general kernel
     if (g32 && (w != 8 || sp.pipelined != 2 || sp.fixMode || sp.k.kind != 1)) return hipErrorInvalidValue;
@@ -3078,9 +3078,10 @@ hipError_t launch_sweep(hipStream_t s, const SweepParams &sp, int waves, Timed t
     else if (w == 2) RP_SWEEP(KIND, 2, SOFT); \
     else RP_SWEEP(KIND, 1, SOFT);
     if (sp.k.kind == 2) {
-        // polyspheres: the hard sweep reading the stencil from HBM, at most four wavefronts; counting mode through
-        // the soft sweep's counting branch
+        // polyspheres: the hard sweep reading the stencil from HBM; counting mode through the soft sweep's counting
+        // branch (at most four wavefronts)
         if (sp.counting) { RP_SWEEP_W(2, true) }
+        else if (w == 8) RP_SWEEP(2, 8, false);
         else { RP_SWEEP_W(2, false) }
     } else if (sp.k.kind == 0) {
         if (sp.k.soft || sp.counting) { RP_SWEEP_W(0, true) }

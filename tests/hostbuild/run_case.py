@@ -6,6 +6,7 @@ AddressSanitizer runtime preloaded; any access outside an allocation aborts the 
     python tests/hostbuild/run_case.py LIB nogrid NAME CYCLES    # a packing without a grid, overlap checks, replays
     python tests/hostbuild/run_case.py LIB biggrid CYCLES        # grids larger than one scan workgroup
     python tests/hostbuild/run_case.py LIB autotune CYCLES       # the launch-shape autotuner keeps the trajectory
+    python tests/hostbuild/run_case.py LIB polywaves CYCLES      # polysphere sweeps on eight and four wavefronts
 """
 import os
 import sys
@@ -118,6 +119,27 @@ def big_grid(cycles: int) -> None:
     pk.close()
 
 
+def poly_waves(cycles: int) -> None:
+    """Kinked 11-bead chains (the polysphere sweep reading its stencil from HBM) with eight wavefronts per active cell
+    and with four: the same trajectory, and no overlap by the oracle."""
+    from test_gpu_poly import chain_start, poly_shape, poly_overlaps
+    box, pos, rot = chain_start()
+    states = []
+    for w in ("8", "4"):
+        os.environ["RAMPACK_SWEEP_WAVES"] = w
+        pk = ra.GpuPacking(poly_shape("kinked11"), box, pos, rot, seed=5)
+        pk.set_environment(1.0, 1.0, 0.3, None, box_move=("delta_triclinic", 0.05))
+        c = pk.run_cycles(cycles)
+        assert pk.timing()["sweep_waves"] == int(w)
+        states.append(pk.download())
+        print(f"{w} wavefronts: moves {c['moves_accepted']}/{c['moves_attempted']}", flush=True)
+        pk.close()
+    del os.environ["RAMPACK_SWEEP_WAVES"]
+    assert all(np.array_equal(a, b) for a, b in zip(*states))
+    b, q, r = states[0]
+    assert poly_overlaps("kinked11", b, q, r) == 0
+
+
 def autotune(cycles: int) -> None:
     """The autotuner's six launch shapes on rods: tuning over k cycles per shape, then two cycles, ends bit for bit
     where 6 k cycles and then two cycles without tuning end."""
@@ -149,6 +171,8 @@ if __name__ == "__main__":
         big_grid(int(sys.argv[3]))
     elif sys.argv[2] == "autotune":
         autotune(int(sys.argv[3]))
+    elif sys.argv[2] == "polywaves":
+        poly_waves(int(sys.argv[3]))
     else:
         replica(int(sys.argv[3]), int(sys.argv[4]))
     print("case passed", flush=True)
