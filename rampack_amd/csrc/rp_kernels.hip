@@ -3052,7 +3052,8 @@ hipError_t launch_sweep(hipStream_t s, const SweepParams &sp, int waves, Timed t
         return hipGetLastError();
     }
     unsigned blocks = (unsigned)(sp.na[0] * sp.na[1] * sp.na[2]);
-    int w = waves >= 8 ? 8 : (waves >= 4 ? 4 : (waves == 2 ? 2 : 1));
+    int w = waves >= 16 ? 16 : (waves >= 8 ? 8 : (waves >= 4 ? 4 : (waves == 2 ? 2 : 1)));
+    if (w == 16 && (sp.k.kind != 2 || sp.counting || sp.fixMode)) w = 8;  // sixteen only for the polysphere sweep
     if (w == 8 && (sp.k.soft || sp.counting)) w = 4;  // eight only for the hard sweeps
     // the lean kernels run spherocylinders only: their first launch on spheres (eight wavefronts forced onto sparse
     // stencils) faulted the GPU and the cause is not found, so spheres keep the general kernel
@@ -3081,6 +3082,7 @@ hipError_t launch_sweep(hipStream_t s, const SweepParams &sp, int waves, Timed t
         // polyspheres: the hard sweep reading the stencil from HBM; counting mode through the soft sweep's counting
         // branch (at most four wavefronts)
         if (sp.counting) { RP_SWEEP_W(2, true) }
+        else if (w == 16) RP_SWEEP(2, 16, false);
         else if (w == 8) RP_SWEEP(2, 8, false);
         else { RP_SWEEP_W(2, false) }
     } else if (sp.k.kind == 0) {

@@ -120,12 +120,12 @@ def big_grid(cycles: int) -> None:
 
 
 def poly_waves(cycles: int) -> None:
-    """Kinked 11-bead chains (the polysphere sweep reading its stencil from HBM) with eight wavefronts per active cell
-    and with four: the same trajectory, and no overlap by the oracle."""
+    """Kinked 11-bead chains (the polysphere sweep reading its stencil from HBM) with sixteen, eight and four
+    wavefronts per active cell: the same trajectory, and no overlap by the oracle."""
     from test_gpu_poly import chain_start, poly_shape, poly_overlaps
     box, pos, rot = chain_start()
     states = []
-    for w in ("8", "4"):
+    for w in ("16", "8", "4"):
         os.environ["RAMPACK_SWEEP_WAVES"] = w
         pk = ra.GpuPacking(poly_shape("kinked11"), box, pos, rot, seed=5)
         pk.set_environment(1.0, 1.0, 0.3, None, box_move=("delta_triclinic", 0.05))
@@ -135,7 +135,8 @@ def poly_waves(cycles: int) -> None:
         print(f"{w} wavefronts: moves {c['moves_accepted']}/{c['moves_attempted']}", flush=True)
         pk.close()
     del os.environ["RAMPACK_SWEEP_WAVES"]
-    assert all(np.array_equal(a, b) for a, b in zip(*states))
+    for other in states[1:]:
+        assert all(np.array_equal(a, b) for a, b in zip(states[0], other))
     b, q, r = states[0]
     assert poly_overlaps("kinked11", b, q, r) == 0
 

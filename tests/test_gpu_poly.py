@@ -89,7 +89,7 @@ def test_polysphere_sweep_stays_overlap_free():
     pk.close()
 
 
-@pytest.mark.parametrize("waves", ["1", "2", "8"])
+@pytest.mark.parametrize("waves", ["1", "2", "8", "16"])
 def test_polysphere_launch_shapes_give_identical_trajectories(waves, monkeypatch):
     box, pos, rot = chain_start()
     runs = []

@@ -10,7 +10,7 @@ exactly the requested size).
   no access outside an allocation, no overlap, and their decision records replay with no mismatch.
 * A grid with more cells than one scan workgroup takes (the tiled scan of the cell counts) rebuilds the records
   unchanged, and sweeps on it replay.
-* Polysphere sweeps on eight wavefronts per cell follow the four-wavefront trajectory.
+* Polysphere sweeps on sixteen and eight wavefronts per cell follow the four-wavefront trajectory.
 * The launch-shape autotuner runs every candidate shape and the run ends where an untuned run ends.
 * Replicas of the 32-replica equation-of-state scan (partly filled parallel lattices at packing fraction 0.2,
   p* from 0.5 to 4) run with box moves and step adaptation: no access outside an allocation, no overlap, and their
@@ -55,7 +55,7 @@ def test_packings_without_grid_on_cpu_build(host_lib, name, cycles):
     run_case(host_lib, "nogrid", name, cycles)
 
 
-def test_polysphere_sweep_on_eight_wavefronts_on_cpu_build(host_lib):
+def test_polysphere_sweep_on_sixteen_and_eight_wavefronts_on_cpu_build(host_lib):
     run_case(host_lib, "polywaves", 3)
 
 
