@@ -342,7 +342,7 @@ int rp_get_timing(rp_ctx *ctx, rp_timing *out);
  * the fastest for the cycles that follow.  Every launch shape makes the same decisions, so the call is
  * rp_run_cycles(ctx, candidates x cycles_per_shape, 0, NULL) bit for bit, whatever shape it keeps.  Candidates: hard
  * spheres and spherocylinders all six tested shapes (the guarded FP32 precision only its grouped eight-wavefront
- * launch), polyspheres 1, 2 or 4 wavefronts; soft and overlap-counting sweeps keep their shape.  out (nullable,
+ * launch), polyspheres 1, 2, 4, 8 or 16 wavefronts; soft and overlap-counting sweeps keep their shape.  out (nullable,
  * RP_AUTOTUNE_MAX entries) receives each candidate and its mean sweep time per cycle, *n_out their number and
  * *chosen (nullable) the index kept. */
 #define RP_AUTOTUNE_MAX 8
