@@ -2875,8 +2875,11 @@ __device__ __forceinline__ bool allpairs_hit(const PairConsts &k, const Grid &g,
     return sc_overlap(k, f, p2, a2, pbc_translation(g.dims, g.inv, f.pos, p2));
 }
 
+constexpr int AP_GROUP = 4;  // moves on distinct particles tested at once by the grouped all-pairs sweep
+
 size_t allpairs_lds_bytes(int n) {
-    return sizeof(double) * 12 * (size_t)n + (sizeof(double) * 12 + sizeof(int)) * ALLPAIRS_CHUNK;
+    return sizeof(double) * 12 * (size_t)n + (sizeof(double) * 12 + sizeof(int)) * ALLPAIRS_CHUNK +
+           sizeof(double) * 12 * AP_GROUP;
 }
 
 // The cycle's moves of a packing without a grid, in order, by one workgroup (Simulation::performMoves without domains,
@@ -2980,6 +2983,134 @@ __global__ __launch_bounds__(ALLPAIRS_SWEEP_THREADS) void k_sweep_allpairs(Sweep
     flush_counters(P, cell, accepted, moves, 0, tests, true);
 }
 
+// Grouped form of the staged all-pairs sweep: up to AP_GROUP consecutive moves on distinct particles are tested at
+// once against every particle and then decided in their order exactly as one at a time -- move k is rejected if it
+// overlaps a particle outside the group, the old pose of a group particle i not accepted before it (i > k, or i < k
+// rejected), or the trial pose of an accepted move i < k -- the rule of the grouped checkerboard sweep.  Same
+// decisions and trajectory as k_sweep_allpairs<KIND, true>; three barriers per group instead of one or two per move.
+template <int KIND>
+__global__ __launch_bounds__(ALLPAIRS_SWEEP_THREADS) void k_sweep_allpairs_grouped(SweepParams P) {
+    extern __shared__ double smem[];
+    __shared__ unsigned sHit[2];  // [0]: 5 bits per move (bit i < 4: old pose of member i, bit 4: outside the group)
+                                  // [1]: trial pairs, bit 4k + i: move k's trial pose overlaps move i's (i < k)
+    constexpr int T = ALLPAIRS_SWEEP_THREADS, C = ALLPAIRS_CHUNK, G = AP_GROUP;
+    const Grid &g = P.g;
+    const int cell = 0;  // the one cell of a packing without a grid
+    const int n = g.cellStart[1];
+    const int tid = threadIdx.x;
+    const int moves = P.moves;
+    if (n > P.cap) {
+        // the launch's LDS holds P.cap particles
+        if (tid == 0) atomicAdd(&P.counters[CNT_STAGE_OVERFLOW], 1ull);
+        return;
+    }
+    double *sx = smem, *sy = sx + n, *sz = sy + n;
+    double *so = sz + n;  // component c of particle j at so[c * n + j]
+    double *ct = so + 9 * n, *cR = ct + 3 * C;
+    int *ck = (int *)(cR + 9 * C);
+    double *plan = (double *)(ck + C);  // per group member: trial position (3) and orientation (9)
+    for (int j = tid; j < n; j += T) {
+        sx[j] = P.p.x[j];
+        sy[j] = P.p.y[j];
+        sz[j] = P.p.z[j];
+        for (int c = 0; c < 9; c++) so[c * n + j] = P.p.o[c][j];
+    }
+    unsigned long long accepted = 0, tests = 0;
+    for (int base = 0; base < moves; base += C) {
+        const int cn = min(C, moves - base);
+        __syncthreads();  // the packing staged; every thread done with the previous chunk's proposals
+        if (tid < cn) {
+            const int i = base + tid;
+            ck[tid] = P.pr.k[i];
+            for (int d = 0; d < 3; d++) ct[d * C + tid] = P.pr.t[d * P.pr.stride + i];
+            for (int c = 0; c < 9; c++) cR[c * C + tid] = P.pr.R[c * P.pr.stride + i];
+        }
+        __syncthreads();
+        for (int m0 = 0; m0 < cn;) {
+            // the group: the longest run of at most G moves on distinct particles (the same in every lane)
+            const int c0 = ck[m0];
+            const int c1 = m0 + 1 < cn ? ck[m0 + 1] : -1, c2 = m0 + 2 < cn ? ck[m0 + 2] : -1;
+            const int c3 = m0 + 3 < cn ? ck[m0 + 3] : -1;
+            const bool d1 = m0 + 1 < cn && c1 != c0;
+            const bool d2 = d1 && m0 + 2 < cn && c2 != c0 && c2 != c1;
+            const bool d3 = d2 && m0 + 3 < cn && c3 != c0 && c3 != c1 && c3 != c2;
+            const int gs = 1 + (int)d1 + (int)d2 + (int)d3;
+            const int k0 = c0, k1 = d1 ? c1 : -1, k2 = d2 ? c2 : -1, k3 = d3 ? c3 : -1;
+            auto kof = [&](int k) { return k == 0 ? k0 : (k == 1 ? k1 : (k == 2 ? k2 : k3)); };
+            if (tid < gs) {
+                // trial pose of move m0 + tid (Shape::translate with the periodic correction, Shape::rotate)
+                const int mm = m0 + tid, ki = kof(tid);
+                const V3 npos = translate_wrapped(g.dims, g.inv, v3(sx[ki], sy[ki], sz[ki]),
+                                                  v3(ct[mm], ct[C + mm], ct[2 * C + mm]));
+                double O[9], R[9], NO[9];
+                for (int c = 0; c < 9; c++) {
+                    O[c] = so[c * n + ki];
+                    R[c] = cR[c * C + mm];
+                }
+                mulm(R, O, NO);
+                double *q = plan + 12 * tid;
+                q[0] = npos.x;
+                q[1] = npos.y;
+                q[2] = npos.z;
+                for (int c = 0; c < 9; c++) q[3 + c] = NO[c];
+            }
+            if (tid < 2) sHit[tid] = 0;
+            __syncthreads();
+            unsigned hw = 0, tw = 0;
+            for (int k = 0; k < gs; k++) {
+                const double *q = plan + 12 * k;
+                const SCFirst f = make_first<KIND>(P.k, v3(q[0], q[1], q[2]), v3(q[5], q[8], q[11]));
+                const int kk = kof(k);
+                for (int j = tid; j < n; j += T) {
+                    if (j == kk) continue;
+                    const int member = j == k0 ? 0 : (j == k1 ? 1 : (j == k2 ? 2 : (j == k3 ? 3 : 4)));
+                    const unsigned bit = 1u << (5 * k + member);
+                    if (hw & bit) continue;
+                    const V3 a2 = KIND == 1 ? v3(so[2 * n + j], so[5 * n + j], so[8 * n + j]) : v3(0, 0, 0);
+                    if (allpairs_hit<KIND>(P.k, g, f, v3(sx[j], sy[j], sz[j]), a2)) hw |= bit;
+                }
+                // lanes 4k + i test move k's trial pose against move i's (i < k)
+                for (int i = 0; i < k; i++) {
+                    if (tid != 4 * k + i) continue;
+                    const double *qi = plan + 12 * i;
+                    const V3 a2 = KIND == 1 ? v3(qi[5], qi[8], qi[11]) : v3(0, 0, 0);
+                    if (allpairs_hit<KIND>(P.k, g, f, v3(qi[0], qi[1], qi[2]), a2)) tw |= 1u << (4 * k + i);
+                }
+            }
+            if (hw) atomicOr(&sHit[0], hw);
+            if (tw) atomicOr(&sHit[1], tw);
+            __syncthreads();
+            // the group's decisions in order (the same in every lane)
+            const unsigned H = sHit[0], TP = sHit[1];
+            unsigned acc = 0;
+            for (int k = 0; k < gs; k++) {
+                const unsigned b = (H >> (5 * k)) & 31u, trial = (TP >> (4 * k)) & 15u;
+                const bool hit = (b & 16u) || ((b & 15u) & ~acc) || (trial & acc);
+                if (!hit) acc |= 1u << k;
+            }
+            accepted += (unsigned long long)__popc(acc);
+            tests += (unsigned long long)(n - 1) * (unsigned long long)gs;
+            if (tid < gs && ((acc >> tid) & 1u)) {
+                const int ki = kof(tid);
+                const double *q = plan + 12 * tid;
+                P.p.x[ki] = q[0];
+                P.p.y[ki] = q[1];
+                P.p.z[ki] = q[2];
+                for (int c = 0; c < 9; c++) P.p.o[c][ki] = q[3 + c];
+                P.p.wbits[ki] = 0;
+                if (P.pr.acc) P.pr.acc[base + m0 + tid] = 1;
+                sx[ki] = q[0];
+                sy[ki] = q[1];
+                sz[ki] = q[2];
+                for (int c = 0; c < 9; c++) so[c * n + ki] = q[3 + c];
+            }
+            __syncthreads();  // accepted poses staged; the plan and hit words free for the next group
+            m0 += gs;
+        }
+    }
+    flush_counters(P, cell, accepted, moves, 0, tests, true);
+}
+
 template <int KIND>
 __global__ __launch_bounds__(ALLPAIRS_THREADS) void k_count_allpairs(Grid g, int n, Particles p, PairConsts k,
                                                                     int earlyExit, unsigned long long *counters) {
@@ -3044,10 +3175,18 @@ hipError_t launch_sweep(hipStream_t s, const SweepParams &sp, int waves, Timed t
 #define RP_ALLPAIRS(KIND, STAGED) \
     hipExtLaunchKernelGGL((k_sweep_allpairs<KIND, STAGED>), dim3(1), dim3(ALLPAIRS_SWEEP_THREADS), lds, s, t.start, t.stop, \
                           0, sp)
-        if (sp.k.kind == 0 && sp.cap > 0) RP_ALLPAIRS(0, true);
+#define RP_ALLPAIRS_GROUPED(KIND) \
+    hipExtLaunchKernelGGL((k_sweep_allpairs_grouped<KIND>), dim3(1), dim3(ALLPAIRS_SWEEP_THREADS), lds, s, t.start, \
+                          t.stop, 0, sp)
+        // staged packings: grouped moves; the orientation fix and packings too large for LDS: one move at a time
+        const bool grouped = sp.cap > 0 && !sp.fixMode;
+        if (sp.k.kind == 0 && grouped) RP_ALLPAIRS_GROUPED(0);
+        else if (sp.k.kind == 0 && sp.cap > 0) RP_ALLPAIRS(0, true);
         else if (sp.k.kind == 0) RP_ALLPAIRS(0, false);
+        else if (grouped) RP_ALLPAIRS_GROUPED(1);
         else if (sp.cap > 0) RP_ALLPAIRS(1, true);
         else RP_ALLPAIRS(1, false);
+#undef RP_ALLPAIRS_GROUPED
 #undef RP_ALLPAIRS
         return hipGetLastError();
     }
