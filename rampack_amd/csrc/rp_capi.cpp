@@ -620,8 +620,7 @@ int countCapLimit(rp_ctx *c) {
 
 // Overlap count of `items` sorted by `g`, enqueued with the copy of the counters to the host (no synchronisation).
 int scanLaunch(rp_ctx *c, const Grid &g, const Particles &items, size_t nItems, int earlyExit) {
-    RP_HIP(launch_fill_u64(c->stream, c->dCounters + CNT_OVERLAPS, 1, 0));
-    RP_HIP(launch_fill_u64(c->stream, c->dCounters + CNT_STAGE_OVERFLOW, 1, 0));
+    RP_HIP(launch_clear_count(c->stream, c->dCounters));
     double lambda = (double)nItems / (double)g.ncell;
     const int side = 2 * g.reach + 1;
     double expect = ((side * side * side - 1) / 2 + 1) * lambda;

@@ -181,6 +181,8 @@ struct Timed {
 hipError_t launch_centres(hipStream_t s, const Particles &src, int n, const PairConsts &k, const Grid &g,
                           const Particles &dst);
 hipError_t launch_fill_u64(hipStream_t s, unsigned long long *p, size_t n, unsigned long long v);
+// counters[CNT_OVERLAPS] = counters[CNT_STAGE_OVERFLOW] = 0 in one launch.
+hipError_t launch_clear_count(hipStream_t s, unsigned long long *counters);
 hipError_t launch_fill_i32(hipStream_t s, int *p, size_t n, int v);
 // Cell-list rebuild: src -> dst sorted by cell of `g`.  When rescale != 0 the positions are first mapped
 // newDims * (oldInv * pos) (Packing.cpp:183-184) with newDims = g.dims.

@@ -355,6 +355,17 @@ hipError_t launch_fill_u64(hipStream_t s, unsigned long long *p, size_t n, unsig
     return hipGetLastError();
 }
 
+// Clears the overlap count and the stage-overflow counter before an overlap count (one launch for both).
+__global__ void k_clear_count(unsigned long long *counters) {
+    if (threadIdx.x == 0) counters[CNT_OVERLAPS] = 0;
+    if (threadIdx.x == 1) counters[CNT_STAGE_OVERFLOW] = 0;
+}
+
+hipError_t launch_clear_count(hipStream_t s, unsigned long long *counters) {
+    hipLaunchKernelGGL(k_clear_count, dim3(1), dim3(64), 0, s, counters);
+    return hipGetLastError();
+}
+
 hipError_t launch_fill_i32(hipStream_t s, int *p, size_t n, int v) {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
